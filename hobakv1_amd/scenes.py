@@ -1,0 +1,310 @@
+"""Synthetic tetrahedral scenes for the backward-Euler Newton step.
+
+The reference builds its meshes from .tobj files (src/Geometry/TET_MESH.cpp:1174-1226) and
+hard-coded scene classes (e.g. src/Scenes/DROP_TEST.h:34-95).  The workloads used here are
+regular cube lattices instead, so this module defines them deterministically:
+
+* Kuhn (Freudenthal) split: every lattice cell is cut into the 6 tets that share the cell's
+  (0,0,0)-(1,1,1) diagonal.  Vertex order inside each tet is chosen so that every rest volume is
+  positive, which the reference asserts (src/Geometry/TET_MESH.cpp:132-137).
+* Vertex numbering is x-fastest: v(i, j, k) = i + (nx+1) * (j + (ny+1) * k).
+* Tet numbering is cell-major (x-fastest cells), 6 tets per cell in a fixed permutation order.
+* Random numbers come from a counter-based SplitMix64 stream, so the same (seed, stream, index)
+  gives the same double everywhere.
+* Density enters as the reference would take it: the lumped mass (one-ring volumes,
+  src/Timestepper/TIMESTEPPER.cpp:176-208) is scaled by rho through scaleMass
+  (src/Timestepper/TIMESTEPPER.h:100) and gravity is passed as the body force rho * g
+  (src/Timestepper/TIMESTEPPER.cpp:334-345).
+* "x = 0 face pinned" is the reference's kinematic CUBE with centre (-5, 0.5, 0.5) and scale 10
+  (src/Geometry/CUBE.cpp:27-55): it contains exactly the vertices with X <= 0.  Pinned
+  displacements are what TIMESTEPPER::applyKinematicConstraints computes
+  (src/Timestepper/TIMESTEPPER.cpp:105-122): world(local(X)) - X.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+from typing import Optional
+
+import numpy as np
+
+MAT_SNH = 0
+MAT_ARAP = 1
+
+PRECOND_JACOBI = 0
+PRECOND_BLOCK_JACOBI = 1
+
+_GOLDEN = np.uint64(0x9E3779B97F4A7C15)
+_M1 = np.uint64(0xBF58476D1CE4E5B9)
+_M2 = np.uint64(0x94D049BB133111EB)
+
+
+def _mix(z: np.ndarray) -> np.ndarray:
+    with np.errstate(over="ignore"):
+        z = (z ^ (z >> np.uint64(30))) * _M1
+        z = (z ^ (z >> np.uint64(27))) * _M2
+    return z ^ (z >> np.uint64(31))
+
+
+def uniform01(seed: int, stream: int, n: int) -> np.ndarray:
+    """n doubles in [0, 1) from the SplitMix64 counter stream (seed, stream)."""
+    with np.errstate(over="ignore"):
+        key = _mix(np.array([(seed * 0x100000001B3 + stream * 0x51_7CC1_B727_220A) & 0xFFFFFFFFFFFFFFFF],
+                            dtype=np.uint64) + _GOLDEN)[0]
+        ctr = key + (np.arange(1, n + 1, dtype=np.uint64) * _GOLDEN)
+    return (_mix(ctr) >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+def uniform(seed: int, stream: int, n: int, lo: float, hi: float) -> np.ndarray:
+    return lo + (hi - lo) * uniform01(seed, stream, n)
+
+
+# The 6 axis orders of the Kuhn split; odd permutations get vertices 1 and 2 swapped so that
+# det[x1-x0, x2-x0, x3-x0] > 0.
+_PERMS = ((0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0))
+_PERM_PARITY = (0, 1, 1, 0, 0, 1)
+
+
+def kuhn_lattice(nx: int, ny: int, nz: int, h: float = None, origin=(0.0, 0.0, 0.0)):
+    """Rest vertices (nv, 3) float64 and tets (nt, 4) int32 of an nx*ny*nz Kuhn lattice."""
+    if h is None:
+        h = 1.0 / max(nx, ny, nz)
+    i, j, k = np.meshgrid(np.arange(nx + 1), np.arange(ny + 1), np.arange(nz + 1), indexing="ij")
+    # x-fastest numbering
+    i = i.transpose(2, 1, 0).ravel()
+    j = j.transpose(2, 1, 0).ravel()
+    k = k.transpose(2, 1, 0).ravel()
+    verts = np.stack([origin[0] + i * h, origin[1] + j * h, origin[2] + k * h], axis=1).astype(np.float64)
+
+    def vid(ci, cj, ck):
+        return ci + (nx + 1) * (cj + (ny + 1) * ck)
+
+    ci, cj, ck = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    ci = ci.transpose(2, 1, 0).ravel()
+    cj = cj.transpose(2, 1, 0).ravel()
+    ck = ck.transpose(2, 1, 0).ravel()
+    ncell = ci.size
+    tets = np.empty((ncell, 6, 4), dtype=np.int64)
+    for p, perm in enumerate(_PERMS):
+        c = [np.zeros(ncell, np.int64)] * 3
+        corner = [ci.copy(), cj.copy(), ck.copy()]
+        chain = [vid(*corner)]
+        for ax in perm:
+            corner[ax] = corner[ax] + 1
+            chain.append(vid(*corner))
+        if _PERM_PARITY[p]:
+            chain[1], chain[2] = chain[2], chain[1]
+        tets[:, p, :] = np.stack(chain, axis=1)
+        del c
+    return verts, tets.reshape(-1, 4).astype(np.int32)
+
+
+def interior_mask(nx, ny, nz):
+    i, j, k = np.meshgrid(np.arange(nx + 1), np.arange(ny + 1), np.arange(nz + 1), indexing="ij")
+    i = i.transpose(2, 1, 0).ravel()
+    j = j.transpose(2, 1, 0).ravel()
+    k = k.transpose(2, 1, 0).ravel()
+    return (i > 0) & (i < nx) & (j > 0) & (j < ny) & (k > 0) & (k < nz)
+
+
+def jitter(verts, nx, ny, nz, h, seed, frac=0.1):
+    """Uniform +-frac*h jitter on every vertex that lies on no face of the box."""
+    out = verts.copy()
+    mask = interior_mask(nx, ny, nz)
+    n = verts.shape[0]
+    d = uniform(seed, 1, 3 * n, -frac * h, frac * h).reshape(n, 3)
+    out[mask] += d[mask]
+    return out
+
+
+def tet_volumes(verts, tets):
+    x0 = verts[tets[:, 0]]
+    d1 = verts[tets[:, 1]] - x0
+    d2 = verts[tets[:, 2]] - x0
+    d3 = verts[tets[:, 3]] - x0
+    return np.einsum("ij,ij->i", d3, np.cross(d1, d2)) / 6.0
+
+
+def lame(E, nu):
+    """HYPERELASTIC::computeMu / computeLambda (src/Hyperelastic/Volume/HYPERELASTIC.cpp:111-122)."""
+    mu = E / (2.0 * (1.0 + nu))
+    lam = (E * nu) / ((1.0 + nu) * (1.0 - 2.0 * nu))
+    return mu, lam
+
+
+class Cube:
+    """Axis-aligned kinematic cube: local = S^-1 R^T (world - t), world = R S local + t with R = I.
+
+    Same transforms as src/Geometry/KINEMATIC_SHAPE.h:55-73 and inside test of
+    src/Geometry/CUBE.cpp:44-55.
+    """
+
+    def __init__(self, center, scale):
+        self.center = np.asarray(center, dtype=np.float64)
+        self.scale = float(scale)
+        self.inv = 1.0 / self.scale
+
+    def to_local(self, w):
+        return self.inv * (w - self.center)
+
+    def to_world(self, l):
+        return self.scale * l + self.center
+
+    def inside(self, w):
+        l = self.to_local(w)
+        return np.all((l <= 0.5) & (l >= -0.5), axis=-1)
+
+
+PIN_X0_CUBE = (-5.0, 0.5, 0.5, 10.0)
+
+
+@dataclasses.dataclass
+class Scene:
+    """Everything one instance of the backward-Euler step needs."""
+
+    name: str
+    rest: np.ndarray          # (nv, 3) float64
+    tets: np.ndarray          # (nt, 4) int32
+    material: int             # MAT_SNH / MAT_ARAP
+    mu: float
+    lam: float
+    dt: float = 1.0 / 60.0
+    rayleigh_alpha: float = 0.01
+    rayleigh_beta: float = 0.01
+    mass_scale: float = 1.0   # density rho, applied like scaleMass(rho)
+    gravity: tuple = (0.0, 0.0, 0.0)   # body force per unit mass-scaled volume (rho * g)
+    pin_cube: Optional[tuple] = None   # (cx, cy, cz, scale) of the pinning CUBE
+    u0: Optional[np.ndarray] = None    # (3 nv) initial displacement
+    v0: Optional[np.ndarray] = None    # (3 nv) initial velocity
+    pcg_tol: float = 1e-8
+    pcg_max_iters: int = 0             # 0 -> 2 * DOFs, the reference default
+    newton_iters: int = 1
+    ground_y: Optional[float] = None   # penalty ground plane (no reference counterpart)
+    ground_k: float = 0.0
+    dims: tuple = ()
+
+    @property
+    def nv(self):
+        return self.rest.shape[0]
+
+    @property
+    def nt(self):
+        return self.tets.shape[0]
+
+    @property
+    def dofs(self):
+        return 3 * self.nv
+
+    def pinned(self):
+        """Vertex ids inside the pin cube, ascending (TIMESTEPPER::attachKinematicConstraints)."""
+        if self.pin_cube is None:
+            return np.zeros(0, np.int32)
+        c = Cube(self.pin_cube[:3], self.pin_cube[3])
+        return np.nonzero(c.inside(self.rest))[0].astype(np.int32)
+
+    def pinned_targets(self):
+        """Displacements that applyKinematicConstraints writes for the pinned vertices."""
+        ids = self.pinned()
+        if ids.size == 0:
+            return np.zeros((0, 3))
+        c = Cube(self.pin_cube[:3], self.pin_cube[3])
+        X = self.rest[ids]
+        return c.to_world(c.to_local(X)) - X
+
+    def initial_u(self):
+        return np.zeros(self.dofs) if self.u0 is None else self.u0.copy()
+
+    def initial_v(self):
+        return np.zeros(self.dofs) if self.v0 is None else self.v0.copy()
+
+
+def lattice_scene(n=(10, 10, 10), seed=1, E=1e5, nu=0.45, material=MAT_SNH, rho=1000.0,
+                  g=-9.8, h=None, stretch_velocity=False, name=None, pin=True, mu=None):
+    """The pinned-cube workload family.
+
+    n=(10,10,10): 6,000 tets / 1,331 vertices; n=(40,40,40): 384,000 tets / 68,921 vertices.
+    stretch_velocity: initial velocity v = (s - 1) * (X - c) / dt with one seeded per-axis
+    stretch s ~ U[0.7, 1.3]^3 and c the box centre; pinned vertices start at rest.
+    """
+    nx, ny, nz = n
+    if h is None:
+        h = 1.0 / max(n)
+    verts, tets = kuhn_lattice(nx, ny, nz, h)
+    verts = jitter(verts, nx, ny, nz, h, seed)
+    m, l = lame(E, nu)
+    if mu is not None:
+        m = mu
+    sc = Scene(
+        name=name or f"lattice{nx}x{ny}x{nz}",
+        rest=verts, tets=tets, material=material, mu=m, lam=l,
+        mass_scale=rho, gravity=(0.0, rho * g, 0.0),
+        pin_cube=PIN_X0_CUBE if pin else None, dims=(nx, ny, nz),
+    )
+    if stretch_velocity:
+        s = uniform(seed, 2, 3, 0.7, 1.3)
+        c = np.array([0.5 * nx * h, 0.5 * ny * h, 0.5 * nz * h])
+        v = ((s - 1.0)[None, :] * (verts - c[None, :]) / sc.dt)
+        ids = sc.pinned()
+        v[ids] = 0.0
+        sc.v0 = v.ravel().copy()
+    return sc
+
+
+def config1(seed=1):
+    """10^3 lattice, SNH E=1e5 nu=0.45 rho=1000, x=0 pinned, dt=1/60, 1 Newton iteration."""
+    return lattice_scene((10, 10, 10), seed=seed, name="cube10_snh")
+
+
+def config2(seed=2):
+    """40^3 lattice with the seeded [0.7, 1.3] stretch applied as a velocity field."""
+    return lattice_scene((40, 40, 40), seed=seed, stretch_velocity=True, name="cube40_snh_stretch")
+
+
+def config3(material=MAT_SNH, seed=2):
+    """The 40^3 stretch scene, SNH (E=1e5, nu=0.45) or ARAP (mu=3.4e4)."""
+    if material == MAT_ARAP:
+        return lattice_scene((40, 40, 40), seed=seed, stretch_velocity=True, material=MAT_ARAP,
+                             mu=3.4e4, name="cube40_arap_stretch")
+    return config2(seed)
+
+
+def batch_instances(count=256, seed=4, lo=4, hi=24):
+    """Independent instances with per-axis lattice dims ~ U{lo..hi}, SNH/ARAP 50/50,
+    E log-uniform in [1e4, 1e6], nu ~ U[0.3, 0.48]."""
+    dims = np.floor(uniform(seed, 10, 3 * count, lo, hi + 1)).astype(int).reshape(count, 3)
+    dims = np.clip(dims, lo, hi)
+    mats = (uniform(seed, 11, count, 0, 1) < 0.5).astype(int)
+    Es = np.exp(uniform(seed, 12, count, math.log(1e4), math.log(1e6)))
+    nus = uniform(seed, 13, count, 0.3, 0.48)
+    out = []
+    for i in range(count):
+        sc = lattice_scene(tuple(int(d) for d in dims[i]), seed=seed * 100003 + i, E=float(Es[i]),
+                           nu=float(nus[i]), material=int(mats[i]), stretch_velocity=True,
+                           name=f"inst{i}")
+        out.append(sc)
+    return out
+
+
+def serving_instance(k, seed=5, ground_y=-0.5, ground_k=1e6):
+    """A request-stream instance: total tets log-uniform in [1e3, 4e5] with aspect ratio in
+    [0.5, 2], mixed SNH/ARAP, dropped from a height in [0, 1] onto a penalty ground plane."""
+    r = uniform(seed * 7919 + k, 20, 8, 0.0, 1.0)
+    tets = math.exp(math.log(1e3) + r[0] * (math.log(4e5) - math.log(1e3)))
+    cells = tets / 6.0
+    ax = 0.5 + 1.5 * r[1]
+    az = 0.5 + 1.5 * r[2]
+    base = (cells / (ax * az)) ** (1.0 / 3.0)
+    nx = max(2, int(round(base * ax)))
+    ny = max(2, int(round(base)))
+    nz = max(2, int(round(base * az)))
+    mat = MAT_SNH if r[3] < 0.5 else MAT_ARAP
+    E = math.exp(math.log(1e4) + r[4] * (math.log(1e6) - math.log(1e4)))
+    nu = 0.3 + 0.18 * r[5]
+    drop = r[6]
+    h = 1.0 / max(nx, ny, nz)
+    sc = lattice_scene((nx, ny, nz), seed=seed * 1000003 + k, E=E, nu=nu, material=mat,
+                       pin=False, name=f"req{k}", h=h)
+    sc.rest = sc.rest + np.array([0.0, drop, 0.0])
+    sc.ground_y = ground_y
+    sc.ground_k = ground_k
+    return sc

@@ -1,0 +1,393 @@
+"""TEST INFRASTRUCTURE ONLY.
+
+ctypes drivers for the two CPU checkers of the backward-Euler Newton step:
+
+* ``Oracle``  -- liboracle.so, this repository's FP64 CPU restatement of the reference path
+  (oracle/hobak_oracle.cpp).  It is the checker on machines without the reference tree.
+* ``RefSim``  -- _ref/libhobak_ref.so, the literal reference compiled from its own sources
+  (oracle/Makefile, oracle/ref_harness.cpp).
+
+Only tests/, __graft_entry__.smoke() and the CPU-baseline leg of bench.py may import this
+package.  The product library (hobakv1_amd) never does.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libhobak_ref.so")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip)
+
+
+def _c64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _c32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+_ref_lib = None
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_SO)
+
+
+def load_ref():
+    global _ref_lib
+    if _ref_lib is None:
+        lib = ctypes.CDLL(REF_SO)
+        lib.ref_create.restype = ctypes.c_void_p
+        lib.ref_create.argtypes = [_dp, ctypes.c_int, _ip, ctypes.c_int, ctypes.c_int,
+                                   ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                   ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, _dp,
+                                   ctypes.c_double, ctypes.c_int]
+        lib.ref_destroy.argtypes = [ctypes.c_void_p]
+        lib.ref_num_pinned.argtypes = [ctypes.c_void_p]
+        lib.ref_pinned_nodes.argtypes = [ctypes.c_void_p, _ip]
+        lib.ref_set_state.argtypes = [ctypes.c_void_p, _dp, _dp]
+        lib.ref_get_state.argtypes = [ctypes.c_void_p, _dp, _dp]
+        lib.ref_step.argtypes = [ctypes.c_void_p, _dp, _dp]
+        lib.ref_step.restype = ctypes.c_int
+        lib.ref_last_rhs_b.argtypes = [ctypes.c_void_p, _dp]
+        lib.ref_mass_diag.argtypes = [ctypes.c_void_p, _dp]
+        lib.ref_mesh_nnz.argtypes = [ctypes.c_void_p]
+        lib.ref_mesh_nnz.restype = ctypes.c_int
+        lib.ref_mesh_forces_hessian.argtypes = [ctypes.c_void_p, _dp, _dp, _ip, _ip, _dp]
+        lib.ref_svd_rv.argtypes = [_dp, _dp, _dp, _dp]
+        lib.ref_psi.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp]
+        lib.ref_psi.restype = ctypes.c_double
+        for f in (lib.ref_pk1, lib.ref_hessian, lib.ref_clamped_hessian):
+            f.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, _dp]
+        _ref_lib = lib
+    return _ref_lib
+
+
+def _colmajor(F):
+    return _c64(np.asarray(F, dtype=np.float64).T.ravel())
+
+
+def _from_colmajor(a, n):
+    return a.reshape(n, n).T.copy()
+
+
+class RefElement:
+    """Element-level calls into the literal reference (row-major numpy in and out)."""
+
+    def __init__(self):
+        self.lib = load_ref()
+
+    def svd_rv(self, F):
+        U = np.zeros(9)
+        V = np.zeros(9)
+        S = np.zeros(3)
+        self.lib.ref_svd_rv(_d(_colmajor(F)), _d(U), _d(S), _d(V))
+        return _from_colmajor(U, 3), S, _from_colmajor(V, 3)
+
+    def psi(self, mat, mu, lam, F):
+        return self.lib.ref_psi(mat, mu, lam, _d(_colmajor(F)))
+
+    def pk1(self, mat, mu, lam, F):
+        P = np.zeros(9)
+        self.lib.ref_pk1(mat, mu, lam, _d(_colmajor(F)), _d(P))
+        return _from_colmajor(P, 3)
+
+    def hessian(self, mat, mu, lam, F):
+        H = np.zeros(81)
+        self.lib.ref_hessian(mat, mu, lam, _d(_colmajor(F)), _d(H))
+        return _from_colmajor(H, 9)
+
+    def clamped_hessian(self, mat, mu, lam, F):
+        H = np.zeros(81)
+        self.lib.ref_clamped_hessian(mat, mu, lam, _d(_colmajor(F)), _d(H))
+        return _from_colmajor(H, 9)
+
+
+class RefSim:
+    """One reference scene: TET_MESH_FASTER + SNH/ARAP + BACKWARD_EULER_POSITION."""
+
+    def __init__(self, scene, pcg_tol=None, pcg_max_iters=None):
+        self.lib = load_ref()
+        self.scene = scene
+        self._rest = _c64(scene.rest.ravel())
+        self._tets = _c32(scene.tets.ravel())
+        g = _c64(np.asarray(scene.gravity))
+        pin = None if scene.pin_cube is None else _c64(np.asarray(scene.pin_cube))
+        tol = scene.pcg_tol if pcg_tol is None else pcg_tol
+        mi = scene.pcg_max_iters if pcg_max_iters is None else pcg_max_iters
+        self.h = self.lib.ref_create(_d(self._rest), scene.nv, _i(self._tets), scene.nt,
+                                     scene.material, scene.mu, scene.lam, scene.dt,
+                                     scene.rayleigh_alpha, scene.rayleigh_beta, scene.mass_scale,
+                                     _d(g), None if pin is None else _d(pin), tol, mi)
+        self.dofs = scene.dofs
+        self.set_state(scene.initial_u(), scene.initial_v())
+
+    def close(self):
+        if self.h:
+            self.lib.ref_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def pinned(self):
+        n = self.lib.ref_num_pinned(self.h)
+        out = np.zeros(n, np.int32)
+        self.lib.ref_pinned_nodes(self.h, _i(out))
+        return out
+
+    def set_state(self, u, v):
+        self.lib.ref_set_state(self.h, _d(_c64(u)), _d(_c64(v)))
+
+    def state(self):
+        u = np.zeros(self.dofs)
+        v = np.zeros(self.dofs)
+        self.lib.ref_get_state(self.h, _d(u), _d(v))
+        return u, v
+
+    def step(self):
+        err = ctypes.c_double(0)
+        sec = ctypes.c_double(0)
+        it = self.lib.ref_step(self.h, ctypes.byref(err), ctypes.byref(sec))
+        return it, err.value, sec.value
+
+    def last_b(self):
+        b = np.zeros(self.dofs)
+        self.lib.ref_last_rhs_b(self.h, _d(b))
+        return b
+
+    def mass_diag(self):
+        m = np.zeros(self.dofs)
+        self.lib.ref_mass_diag(self.h, _d(m))
+        return m
+
+    def forces_hessian(self, u):
+        """R and the clamped K (scipy-free CSC arrays) of the mesh at displacement u."""
+        nnz = self.lib.ref_mesh_nnz(self.h)
+        R = np.zeros(self.dofs)
+        colptr = np.zeros(self.dofs + 1, np.int32)
+        rowidx = np.zeros(nnz, np.int32)
+        vals = np.zeros(nnz)
+        self.lib.ref_mesh_forces_hessian(self.h, _d(_c64(u)), _d(R), _i(colptr), _i(rowidx),
+                                         _d(vals))
+        return R, (colptr, rowidx, vals)
+
+
+_orc_lib = None
+
+
+def oracle_available() -> bool:
+    return os.path.exists(ORACLE_SO)
+
+
+def load_oracle():
+    global _orc_lib
+    if _orc_lib is None:
+        lib = ctypes.CDLL(ORACLE_SO)
+        lib.orc_svd_rv.argtypes = [_dp, _dp, _dp, _dp]
+        lib.orc_psi.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp]
+        lib.orc_psi.restype = ctypes.c_double
+        for f in (lib.orc_pk1, lib.orc_hessian, lib.orc_clamped_hessian):
+            f.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, _dp]
+        lib.orc_clamped_hessian_usv.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                                _dp, _dp, _dp, _dp]
+        lib.orc_tet_force_hessian.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                              _dp, _dp, ctypes.c_double, _dp, _dp]
+        lib.orc_create.restype = ctypes.c_void_p
+        lib.orc_create.argtypes = [_dp, ctypes.c_int, _ip, ctypes.c_int, ctypes.c_int,
+                                   ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                   ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp,
+                                   _ip, ctypes.c_int, _dp, ctypes.c_double, ctypes.c_int,
+                                   ctypes.c_int, ctypes.c_int, ctypes.c_double]
+        lib.orc_destroy.argtypes = [ctypes.c_void_p]
+        lib.orc_set_state.argtypes = [ctypes.c_void_p, _dp, _dp]
+        lib.orc_get_state.argtypes = [ctypes.c_void_p, _dp, _dp]
+        lib.orc_set_ground.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
+        lib.orc_step.argtypes = [ctypes.c_void_p, _dp, _dp, _ip, _ip]
+        lib.orc_step.restype = ctypes.c_int
+        lib.orc_nnzb.argtypes = [ctypes.c_void_p]
+        lib.orc_nnzb.restype = ctypes.c_int
+        lib.orc_bsr_pattern.argtypes = [ctypes.c_void_p, _ip, _ip]
+        lib.orc_forces_hessian.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
+        lib.orc_last_system.argtypes = [ctypes.c_void_p, _dp, _dp]
+        lib.orc_rest_volumes.argtypes = [ctypes.c_void_p, _dp, _dp]
+        lib.orc_set_threads.argtypes = [ctypes.c_int]
+        _orc_lib = lib
+    return _orc_lib
+
+
+class OracleElement:
+    """Element-level calls into the restatement (row-major numpy in and out)."""
+
+    def __init__(self):
+        self.lib = load_oracle()
+
+    def svd_rv(self, F):
+        U = np.zeros(9)
+        V = np.zeros(9)
+        S = np.zeros(3)
+        self.lib.orc_svd_rv(_d(_colmajor(F)), _d(U), _d(S), _d(V))
+        return _from_colmajor(U, 3), S, _from_colmajor(V, 3)
+
+    def psi(self, mat, mu, lam, F):
+        return self.lib.orc_psi(mat, mu, lam, _d(_colmajor(F)))
+
+    def pk1(self, mat, mu, lam, F):
+        P = np.zeros(9)
+        self.lib.orc_pk1(mat, mu, lam, _d(_colmajor(F)), _d(P))
+        return _from_colmajor(P, 3)
+
+    def hessian(self, mat, mu, lam, F):
+        H = np.zeros(81)
+        self.lib.orc_hessian(mat, mu, lam, _d(_colmajor(F)), _d(H))
+        return _from_colmajor(H, 9)
+
+    def clamped_hessian(self, mat, mu, lam, F):
+        H = np.zeros(81)
+        self.lib.orc_clamped_hessian(mat, mu, lam, _d(_colmajor(F)), _d(H))
+        return _from_colmajor(H, 9)
+
+    def clamped_hessian_usv(self, mat, mu, lam, U, S, V):
+        H = np.zeros(81)
+        self.lib.orc_clamped_hessian_usv(mat, mu, lam, _d(_colmajor(U)), _d(_c64(S)),
+                                         _d(_colmajor(V)), _d(H))
+        return _from_colmajor(H, 9)
+
+    def tet_force_hessian(self, mat, mu, lam, x4, DmInv, vol):
+        f = np.zeros(12)
+        H = np.zeros(144)
+        self.lib.orc_tet_force_hessian(mat, mu, lam, _d(_c64(np.asarray(x4).ravel())),
+                                       _d(_colmajor(DmInv)), vol, _d(f), _d(H))
+        return f, _from_colmajor(H, 12)
+
+
+class Oracle:
+    """One scene on the CPU restatement.  precond 0 = Jacobi, 1 = block Jacobi."""
+
+    def __init__(self, scene, precond=1, pcg_tol=None, newton_iters=None, newton_tol=1e-7,
+                 threads=None):
+        self.lib = load_oracle()
+        if threads:
+            self.lib.orc_set_threads(int(threads))
+        self.scene = scene
+        self._rest = _c64(scene.rest.ravel())
+        self._tets = _c32(scene.tets.ravel())
+        g = _c64(np.asarray(scene.gravity))
+        pins = _c32(scene.pinned())
+        targets = _c64(scene.pinned_targets().ravel()) if pins.size else np.zeros(1)
+        tol = scene.pcg_tol if pcg_tol is None else pcg_tol
+        ni = scene.newton_iters if newton_iters is None else newton_iters
+        self.newton_iters = ni
+        self.h = self.lib.orc_create(_d(self._rest), scene.nv, _i(self._tets), scene.nt,
+                                     scene.material, scene.mu, scene.lam, scene.dt,
+                                     scene.rayleigh_alpha, scene.rayleigh_beta, scene.mass_scale,
+                                     _d(g), _i(pins if pins.size else np.zeros(1, np.int32)),
+                                     int(pins.size), _d(targets), tol, scene.pcg_max_iters,
+                                     precond, ni, newton_tol)
+        if scene.ground_y is not None:
+            self.lib.orc_set_ground(self.h, scene.ground_y, scene.ground_k)
+        self.dofs = scene.dofs
+        self.set_state(scene.initial_u(), scene.initial_v())
+
+    def close(self):
+        if self.h:
+            self.lib.orc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_state(self, u, v):
+        self.lib.orc_set_state(self.h, _d(_c64(u)), _d(_c64(v)))
+
+    def state(self):
+        u = np.zeros(self.dofs)
+        v = np.zeros(self.dofs)
+        self.lib.orc_get_state(self.h, _d(u), _d(v))
+        return u, v
+
+    def step(self):
+        err = ctypes.c_double(0)
+        sec = ctypes.c_double(0)
+        per = np.zeros(max(1, self.newton_iters), np.int32)
+        nd = ctypes.c_int(0)
+        it = self.lib.orc_step(self.h, ctypes.byref(err), ctypes.byref(sec), _i(per),
+                               ctypes.byref(nd))
+        self.last_iters_per_newton = per[:nd.value].copy()
+        return it, err.value, sec.value
+
+    def bsr_pattern(self):
+        nnzb = self.lib.orc_nnzb(self.h)
+        rp = np.zeros(self.scene.nv + 1, np.int32)
+        ci = np.zeros(nnzb, np.int32)
+        self.lib.orc_bsr_pattern(self.h, _i(rp), _i(ci))
+        return rp, ci
+
+    def forces_hessian(self, u, with_k=True):
+        nnzb = self.lib.orc_nnzb(self.h)
+        R = np.zeros(self.dofs)
+        K = np.zeros(9 * nnzb) if with_k else None
+        self.lib.orc_forces_hessian(self.h, _d(_c64(u)), _d(R), _d(K) if with_k else None)
+        return R, (K.reshape(nnzb, 3, 3) if with_k else None)
+
+    def last_system(self):
+        nnzb = self.lib.orc_nnzb(self.h)
+        rhs = np.zeros(self.dofs)
+        lhs = np.zeros(9 * nnzb)
+        self.lib.orc_last_system(self.h, _d(rhs), _d(lhs))
+        return rhs, lhs.reshape(nnzb, 3, 3)
+
+    def rest_volumes(self):
+        vol = np.zeros(self.scene.nt)
+        one = np.zeros(self.scene.nv)
+        self.lib.orc_rest_volumes(self.h, _d(vol), _d(one))
+        return vol, one
+
+
+def bsr_to_dense(row_ptr, col_idx, vals, nv):
+    """Dense (3nv x 3nv) matrix from BSR arrays (row-major 3x3 blocks)."""
+    A = np.zeros((3 * nv, 3 * nv))
+    for i in range(nv):
+        for k in range(row_ptr[i], row_ptr[i + 1]):
+            j = col_idx[k]
+            A[3 * i:3 * i + 3, 3 * j:3 * j + 3] = vals[k]
+    return A
+
+
+def csc_to_bsr_vals(csc, row_ptr, col_idx):
+    """Reference CSC (colptr, rowidx, vals) values rearranged into the given BSR pattern."""
+    colptr, rowidx, vals = csc
+    nv = len(row_ptr) - 1
+    out = np.zeros((len(col_idx), 3, 3))
+    lookup = {}
+    for i in range(nv):
+        for k in range(row_ptr[i], row_ptr[i + 1]):
+            lookup[(i, int(col_idx[k]))] = k
+    ncol = len(colptr) - 1
+    for c in range(ncol):
+        for q in range(colptr[c], colptr[c + 1]):
+            r = int(rowidx[q])
+            k = lookup[(r // 3, c // 3)]
+            out[k, r % 3, c % 3] = vals[q]
+    return out

@@ -1,0 +1,249 @@
+// TEST INFRASTRUCTURE ONLY -- never linked into the product.
+//
+// A C-ABI driver over the literal HOBAK reference, compiled from its own sources by
+// oracle/Makefile into oracle/_ref/libhobak_ref.so.  Tests, golden-vector generation and the
+// CPU-baseline leg of bench.py load it through ctypes; nothing in hobakv1_amd/ may.
+//
+// What it drives (reference files, relative to the reference root):
+//   scene step       src/Scenes/SIMULATION_SCENE.h:120-126 (zero f_ext, addGravity, solve)
+//   integrator       src/Timestepper/BACKWARD_EULER_POSITION.cpp:138-327
+//   mesh             src/Geometry/TET_MESH_FASTER.cpp:30-133
+//   pinning          src/Timestepper/TIMESTEPPER.cpp:380-403 with src/Geometry/CUBE.cpp:27-55
+//   materials        src/Hyperelastic/Volume/SNH.cpp, ARAP.cpp, HYPERELASTIC.cpp
+//   SVD              src/util/MATRIX_UTIL.cpp:76-95 (svd_rv)
+//
+// The harness subclasses BACKWARD_EULER_POSITION only to reach the protected Eigen CG object
+// (src/Timestepper/TIMESTEPPER.h:207) so that the PCG tolerance can be set, as the workload
+// definitions ask (the reference itself never calls setTolerance).
+#include <chrono>
+#include <cstring>
+#include <vector>
+
+#include "Geometry/CUBE.h"
+#include "Geometry/TET_MESH_FASTER.h"
+#include "Hyperelastic/Volume/ARAP.h"
+#include "Hyperelastic/Volume/SNH.h"
+#include "Timestepper/BACKWARD_EULER_POSITION.h"
+#include "util/MATRIX_UTIL.h"
+
+using namespace HOBAK;
+
+namespace {
+
+class HarnessBEP : public TIMESTEPPER::BACKWARD_EULER_POSITION {
+ public:
+  HarnessBEP(TET_MESH& mesh, VOLUME::HYPERELASTIC& mat) : BACKWARD_EULER_POSITION(mesh, mat) {}
+  void setTolerance(double t) { _cgSolver.setTolerance(t); }
+  void setMaxIterations(int n) { _cgSolver.setMaxIterations(n); }
+  int iterations() const { return (int)_cgSolver.iterations(); }
+  double error() const { return (double)_cgSolver.error(); }
+  void setRayleigh(double a, double b) { _rayleighAlpha = a; _rayleighBeta = b; }
+  const SPARSE_MATRIX& massMatrix() const { return _M; }
+  const VECTOR& lastB() const { return _b; }
+};
+
+struct RefSim {
+  TET_MESH_FASTER* mesh = nullptr;
+  VOLUME::HYPERELASTIC* material = nullptr;
+  HarnessBEP* solver = nullptr;
+  CUBE* pin = nullptr;
+  VECTOR3 gravity;
+  double lastStepSeconds = 0.0;
+};
+
+VOLUME::HYPERELASTIC* makeMaterial(int kind, double mu, double lambda) {
+  if (kind == 0) return new VOLUME::SNH(mu, lambda);
+  return new VOLUME::ARAP(mu, lambda);
+}
+
+MATRIX3 loadColMajor(const double* a) {
+  MATRIX3 M;
+  for (int j = 0; j < 3; j++)
+    for (int i = 0; i < 3; i++) M(i, j) = a[3 * j + i];
+  return M;
+}
+
+void storeColMajor(const MATRIX3& M, double* a) {
+  for (int j = 0; j < 3; j++)
+    for (int i = 0; i < 3; i++) a[3 * j + i] = M(i, j);
+}
+
+}  // namespace
+
+extern "C" {
+
+// Builds mesh + material + integrator exactly as a scene's buildScene() would.
+//   material: 0 = SNH, 1 = ARAP; (mu, lambda) are the unreparameterised Lame constants.
+//   mass_scale multiplies M once (TIMESTEPPER::scaleMass); gravity is the per-step body force.
+//   pin_cube: {cx, cy, cz, scale} of a CUBE whose inside vertices are pinned, or NULL.
+//   pcg_tol <= 0 keeps Eigen's default; pcg_max_iters <= 0 keeps Eigen's default (2 * DOFs).
+void* ref_create(const double* rest_xyz, int nv, const int* tets, int nt, int material,
+                 double mu, double lambda, double dt, double rayleigh_alpha,
+                 double rayleigh_beta, double mass_scale, const double* gravity,
+                 const double* pin_cube, double pcg_tol, int pcg_max_iters) {
+  std::vector<VECTOR3> verts(nv);
+  for (int i = 0; i < nv; i++)
+    verts[i] = VECTOR3(rest_xyz[3 * i], rest_xyz[3 * i + 1], rest_xyz[3 * i + 2]);
+  std::vector<VECTOR4I> t(nt);
+  for (int i = 0; i < nt; i++)
+    t[i] = VECTOR4I(tets[4 * i], tets[4 * i + 1], tets[4 * i + 2], tets[4 * i + 3]);
+
+  RefSim* sim = new RefSim;
+  sim->mesh = new TET_MESH_FASTER(verts, t);
+  sim->material = makeMaterial(material, mu, lambda);
+  sim->solver = new HarnessBEP(*sim->mesh, *sim->material);
+  sim->solver->vertexFaceSelfCollisionsOn() = false;
+  sim->solver->edgeEdgeSelfCollisionsOn() = false;
+  sim->solver->setDt(dt);
+  sim->solver->setRayleigh(rayleigh_alpha, rayleigh_beta);
+  if (mass_scale != 1.0) sim->solver->scaleMass(mass_scale);
+  sim->gravity = VECTOR3(gravity[0], gravity[1], gravity[2]);
+  if (pin_cube) {
+    sim->pin = new CUBE(VECTOR3(pin_cube[0], pin_cube[1], pin_cube[2]), pin_cube[3]);
+    sim->solver->attachKinematicConstraints(sim->pin);
+  }
+  if (pcg_tol > 0) sim->solver->setTolerance(pcg_tol);
+  if (pcg_max_iters > 0) sim->solver->setMaxIterations(pcg_max_iters);
+  return sim;
+}
+
+void ref_destroy(void* h) {
+  RefSim* sim = (RefSim*)h;
+  delete sim->solver;
+  delete sim->material;
+  delete sim->mesh;
+  delete sim->pin;
+  delete sim;
+}
+
+int ref_num_pinned(void* h) {
+  RefSim* sim = (RefSim*)h;
+  return (int)sim->solver->constrainedNodes().size();
+}
+
+void ref_pinned_nodes(void* h, int* out) {
+  RefSim* sim = (RefSim*)h;
+  std::vector<int> n = sim->solver->constrainedNodes();
+  for (size_t i = 0; i < n.size(); i++) out[i] = n[i];
+}
+
+void ref_set_state(void* h, const double* u, const double* v) {
+  RefSim* sim = (RefSim*)h;
+  const int dofs = (int)sim->solver->position().size();
+  if (u)
+    for (int i = 0; i < dofs; i++) sim->solver->position()[i] = u[i];
+  if (v)
+    for (int i = 0; i < dofs; i++) sim->solver->velocity()[i] = v[i];
+}
+
+void ref_get_state(void* h, double* u, double* v) {
+  RefSim* sim = (RefSim*)h;
+  const int dofs = (int)sim->solver->position().size();
+  if (u)
+    for (int i = 0; i < dofs; i++) u[i] = sim->solver->position()[i];
+  if (v)
+    for (int i = 0; i < dofs; i++) v[i] = sim->solver->velocity()[i];
+}
+
+// One SIMULATION_SCENE::stepSimulation: returns the PCG iteration count.
+int ref_step(void* h, double* pcg_error, double* seconds) {
+  RefSim* sim = (RefSim*)h;
+  auto t0 = std::chrono::steady_clock::now();
+  sim->solver->externalForces().setZero();
+  sim->solver->addGravity(sim->gravity);
+  sim->solver->solve(false);
+  auto t1 = std::chrono::steady_clock::now();
+  if (pcg_error) *pcg_error = sim->solver->error();
+  if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
+  return sim->solver->iterations();
+}
+
+void ref_last_rhs_b(void* h, double* b) {
+  RefSim* sim = (RefSim*)h;
+  const VECTOR& vb = sim->solver->lastB();
+  for (int i = 0; i < vb.size(); i++) b[i] = vb[i];
+}
+
+void ref_mass_diag(void* h, double* m) {
+  RefSim* sim = (RefSim*)h;
+  const SPARSE_MATRIX& M = sim->solver->massMatrix();
+  for (int i = 0; i < M.rows(); i++) m[i] = M.coeff(i, i);
+}
+
+// Internal force R and clamped stiffness K of the mesh at displacement u (TET_MESH.cpp:736-770,
+// TET_MESH_FASTER.cpp:89-133).  K is returned in the reference's own compressed-column layout.
+int ref_mesh_nnz(void* h) {
+  RefSim* sim = (RefSim*)h;
+  const int dofs = (int)sim->solver->position().size();
+  VECTOR zero(dofs);
+  zero.setZero();
+  sim->mesh->setDisplacement(zero);
+  sim->mesh->computeFs();
+  sim->mesh->computeSVDs();
+  SPARSE_MATRIX K = sim->mesh->computeHyperelasticClampedHessian(*sim->material);
+  return (int)K.nonZeros();
+}
+
+void ref_mesh_forces_hessian(void* h, const double* u, double* R, int* colptr, int* rowidx,
+                             double* vals) {
+  RefSim* sim = (RefSim*)h;
+  const int dofs = (int)sim->solver->position().size();
+  VECTOR vu(dofs);
+  for (int i = 0; i < dofs; i++) vu[i] = u[i];
+  sim->mesh->setDisplacement(vu);
+  sim->mesh->computeFs();
+  sim->mesh->computeSVDs();
+  VECTOR f = sim->mesh->computeHyperelasticForces(*sim->material);
+  for (int i = 0; i < dofs; i++) R[i] = f[i];
+  SPARSE_MATRIX K = sim->mesh->computeHyperelasticClampedHessian(*sim->material);
+  for (int j = 0; j <= K.outerSize(); j++) colptr[j] = K.outerIndexPtr()[j];
+  for (int k = 0; k < K.nonZeros(); k++) {
+    rowidx[k] = K.innerIndexPtr()[k];
+    vals[k] = K.valuePtr()[k];
+  }
+}
+
+// ---- element level (column-major 3x3, column-major 9x9) ----
+void ref_svd_rv(const double* F, double* U, double* S, double* V) {
+  MATRIX3 Um, Vm;
+  VECTOR3 Sv;
+  svd_rv(loadColMajor(F), Um, Sv, Vm);
+  storeColMajor(Um, U);
+  storeColMajor(Vm, V);
+  for (int i = 0; i < 3; i++) S[i] = Sv[i];
+}
+
+double ref_psi(int material, double mu, double lambda, const double* F) {
+  VOLUME::HYPERELASTIC* m = makeMaterial(material, mu, lambda);
+  double r = m->psi(loadColMajor(F));
+  delete m;
+  return r;
+}
+
+void ref_pk1(int material, double mu, double lambda, const double* F, double* P) {
+  VOLUME::HYPERELASTIC* m = makeMaterial(material, mu, lambda);
+  storeColMajor(m->PK1(loadColMajor(F)), P);
+  delete m;
+}
+
+void ref_hessian(int material, double mu, double lambda, const double* F, double* H) {
+  VOLUME::HYPERELASTIC* m = makeMaterial(material, mu, lambda);
+  MATRIX9 h = m->hessian(loadColMajor(F));
+  for (int j = 0; j < 9; j++)
+    for (int i = 0; i < 9; i++) H[9 * j + i] = h(i, j);
+  delete m;
+}
+
+// The FASTER path: svd_rv followed by clampedHessian(U, Sigma, V).
+void ref_clamped_hessian(int material, double mu, double lambda, const double* F, double* H) {
+  VOLUME::HYPERELASTIC* m = makeMaterial(material, mu, lambda);
+  MATRIX3 Um, Vm;
+  VECTOR3 Sv;
+  svd_rv(loadColMajor(F), Um, Sv, Vm);
+  MATRIX9 h = m->clampedHessian(Um, Sv, Vm);
+  for (int j = 0; j < 9; j++)
+    for (int i = 0; i < 9; i++) H[9 * j + i] = h(i, j);
+  delete m;
+}
+
+}  // extern "C"
