@@ -1,0 +1,160 @@
+// Fused per-tet element kernel: deformation gradient -> FP64 SVD -> PK1 force -> eigen-clamped
+// stiffness, one lane per tet, outputs staged as coalesced SoA planes.
+//
+// Reference path replaced (relative to the reference root):
+//   TET_MESH::computeF / computeFs        src/Geometry/TET_MESH.cpp:643-655, 706-714
+//   TET_MESH::computeSVDs -> svd_rv       src/Geometry/TET_MESH.cpp:689-701
+//   TET_MESH::computeHyperelasticForces   src/Geometry/TET_MESH.cpp:736-770 (per-tet part)
+//   TET_MESH_FASTER::computeHyperelasticClampedHessian element loop
+//                                         src/Geometry/TET_MESH_FASTER.cpp:93-103
+//
+// The reference forms the 12x12 stiffness as dF/dx^T (-V H9) dF/dx with dense 9x12 products.
+// Here the clamped H9 is kept in the singular-value frame, where it is block diagonal (a 3x3
+// scaling block and three 2x2 twist/flip blocks), and each 3x3 vertex-pair block of the stiffness
+// is formed directly as  V_e * U * M_vw * U^T  with
+//   M_vw[a][a']  = sum_{b,b'} w_v[b] Hhat[(a,b),(a',b')] w_w[b'],   w = (G DmInv) V,
+// G the 4x3 map from corner positions to edge vectors.  That is the same matrix up to round-off,
+// at about a third of the flops, and it needs only ~40 live values per lane.
+#include "device_math.h"
+#include "kernels.h"
+
+namespace hobak {
+
+template <typename T, int MAT>
+__global__ __launch_bounds__(128) void element_kernel(ElementArgs a) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.nt) return;
+  const size_t nt = (size_t)a.nt;
+  const int4 tv = a.tets[t];
+  const int vid[4] = {tv.x, tv.y, tv.z, tv.w};
+
+  // corner positions x = X + u (TET_MESH::setDisplacement) and edge matrix Ds
+  double x[4][3];
+#pragma unroll
+  for (int k = 0; k < 4; k++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) x[k][c] = a.rest[3 * (size_t)vid[k] + c] + a.u[3 * (size_t)vid[k] + c];
+  double Di[3][3];
+#pragma unroll
+  for (int r = 0; r < 3; r++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) Di[r][c] = a.dminv[(3 * r + c) * nt + t];
+  double F[3][3];
+#pragma unroll
+  for (int r = 0; r < 3; r++)
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+      F[r][c] = ((x[1][r] - x[0][r]) * Di[0][c] + (x[2][r] - x[0][r]) * Di[1][c]) +
+                (x[3][r] - x[0][r]) * Di[2][c];
+
+  double Ud[3][3], Sd[3], Vd[3][3];
+  svd_rv(F, Ud, Sd, Vd);
+
+  const T vol = (T)a.vol[t];
+  // Wd: row v = d x_v-coefficient of F's rows (the non-zero pattern of dF/dx, TET_MESH.cpp:197-245)
+  T Wd[4][3];
+#pragma unroll
+  for (int j = 0; j < 3; j++) {
+    Wd[0][j] = (T)((-Di[0][j] - Di[1][j]) - Di[2][j]);
+    Wd[1][j] = (T)Di[0][j];
+    Wd[2][j] = (T)Di[1][j];
+    Wd[3][j] = (T)Di[2][j];
+  }
+  T U[3][3], V[3][3], S[3];
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    S[i] = (T)Sd[i];
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      U[i][j] = (T)Ud[i][j];
+      V[i][j] = (T)Vd[i][j];
+    }
+  }
+
+  // PK1 and the element force f_v = -V_e P Wd_v^T
+  T P[3][3];
+  if (MAT == 0) {
+    T Ft[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+      for (int j = 0; j < 3; j++) Ft[i][j] = (T)F[i][j];
+    snh_pk1<T>((T)a.mu, (T)a.lam, (T)a.alpha, Ft, P);
+  } else {
+    arap_pk1<T>((T)a.mu, U, S, V, P);
+  }
+  T* sf = (T*)a.stage_f;
+#pragma unroll
+  for (int v = 0; v < 4; v++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const T s = (P[c][0] * Wd[v][0] + P[c][1] * Wd[v][1]) + P[c][2] * Wd[v][2];
+      sf[(3 * v + c) * nt + t] = -vol * s;
+    }
+  if (!a.with_hessian) return;
+
+  SigmaHessian<T> h;
+  if (MAT == 0)
+    snh_sigma_hessian<T>((T)a.mu, (T)a.lam, S, h);
+  else
+    arap_sigma_hessian<T>((T)a.mu, S, h);
+
+  T w[4][3];
+#pragma unroll
+  for (int v = 0; v < 4; v++)
+#pragma unroll
+    for (int b = 0; b < 3; b++) w[v][b] = Wd[v][0] * V[0][b] + Wd[v][1] * V[1][b] + Wd[v][2] * V[2][b];
+
+  T* sh = (T*)a.stage_h;
+  int lb = 0;
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+#pragma unroll
+    for (int v2 = v; v2 < 4; v2++, lb++) {
+      T M[3][3];
+#pragma unroll
+      for (int p = 0; p < 3; p++)
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+          if (p == q) {
+            T s = h.Hs[p][p] * w[v][p] * w[v2][p];
+#pragma unroll
+            for (int b = 0; b < 3; b++)
+              if (b != p) s += h.d[3 - p - b] * w[v][b] * w[v2][b];
+            M[p][q] = s;
+          } else {
+            M[p][q] = h.Hs[p][q] * w[v][p] * w[v2][q] + h.o[3 - p - q] * w[v][q] * w[v2][p];
+          }
+        }
+      T UM[3][3];
+#pragma unroll
+      for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int q = 0; q < 3; q++) UM[c][q] = U[c][0] * M[0][q] + U[c][1] * M[1][q] + U[c][2] * M[2][q];
+#pragma unroll
+      for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int c2 = 0; c2 < 3; c2++) {
+          const T s = UM[c][0] * U[c2][0] + UM[c][1] * U[c2][1] + UM[c][2] * U[c2][2];
+          sh[(lb * 9 + 3 * c + c2) * nt + t] = vol * s;
+        }
+    }
+  }
+}
+
+template <typename T>
+hipError_t launch_element(const ElementArgs& a, hipStream_t s) {
+  if (a.nt <= 0) return hipSuccess;
+  const int block = 128;
+  const int grid = (a.nt + block - 1) / block;
+  if (a.material == 0)
+    hipLaunchKernelGGL((element_kernel<T, 0>), dim3(grid), dim3(block), 0, s, a);
+  else
+    hipLaunchKernelGGL((element_kernel<T, 1>), dim3(grid), dim3(block), 0, s, a);
+  return hipGetLastError();
+}
+
+template hipError_t launch_element<double>(const ElementArgs&, hipStream_t);
+template hipError_t launch_element<float>(const ElementArgs&, hipStream_t);
+
+}  // namespace hobak

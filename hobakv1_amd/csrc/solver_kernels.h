@@ -1,0 +1,119 @@
+// Argument blocks and launchers of the assembly / right-hand side / PCG / state kernels.
+#pragma once
+#include "kernels.h"
+
+namespace hobak {
+
+struct AsmArgs {
+  int nrows, nrows_pad, nslots, nt;
+  const int* slice_off;
+  const int* row_len;
+  const int* col;
+  const int* cptr;       // nslots + 1
+  const int* clist;
+  const void* stage_h;   // T
+  const void* D;         // T, 9 planes (mode 1)
+  void* out;             // T, 9 planes
+  int mode;              // 0 plain stiffness sum, 1 system matrix + preconditioner
+  const unsigned char* pinned;
+  void* minv;            // T
+  int precond;
+  int ground_on;
+  double ground_y, ground_k;
+  const double* rest;
+  const double* u;
+};
+
+struct ConstArgs {
+  int nrows, nslots;
+  const int* slice_off;
+  const int* row_len;
+  const int* col;
+  const double* kneg;    // assembled -K_rest, 9 planes
+  const double* mass;    // per DOF
+  double ra, rb, invDt, invDt2;
+  void* D;
+  void* E;
+  void* C;
+};
+
+struct RhsArgs {
+  int nrows, nrows_pad, nslots, nt;
+  const int* slice_off;
+  const int* row_len;
+  const int* col;
+  const int* vptr;
+  const int* vlist;
+  const void* stage_f;   // T
+  const void* Mvec;      // T: E (BEP) or C (BDF_1)
+  const double* vel;
+  const double* u;
+  const double* uOld;
+  const double* vOld;
+  const double* mass;
+  const double* one_ring;
+  const double* rest;
+  double g[3];
+  double invDt, invDt2;
+  int bdf;
+  const unsigned char* pinned;
+  int ground_on;
+  double ground_y, ground_k;
+  double* rhs64;
+  int init_cg;
+  void* r;
+  void* x;
+  void* z;
+  void* p;
+  const void* minv;
+  int precond;
+  double tol;
+  int max_iters;
+  CgScalars* sc;
+  Reduce red;
+};
+
+struct CgArgs {
+  int nrows, nrows_pad, nslots;
+  const int* slice_off;
+  const int* row_len;
+  const int* col;
+  const void* A;
+  void* x;
+  void* r;
+  void* z;
+  void* p;
+  void* t;
+  const void* minv;
+  int precond;
+  CgScalars* sc;
+  Reduce red;
+};
+
+hipError_t launch_begin_step(int nv, double* u, const double* v, double* uOld, double* vOld,
+                             const unsigned char* pinned, const double* target, hipStream_t s);
+hipError_t launch_end_step(int nv, double invDt, const double* u, const double* uOld, double* v,
+                           const unsigned char* pinned, hipStream_t s);
+template <typename T>
+hipError_t launch_apply_dx(int n, double* u, const T* x, hipStream_t s);
+template <typename T>
+hipError_t launch_assemble(const AsmArgs& a, hipStream_t s);
+template <typename T>
+hipError_t launch_constant_parts(const ConstArgs& a, hipStream_t s);
+template <typename T>
+hipError_t launch_rhs_init(const RhsArgs& a, hipStream_t s);
+template <typename T>
+hipError_t launch_cg_iteration(const CgArgs& a, hipStream_t s);
+template <typename T>
+hipError_t launch_cg_spmv(const CgArgs& a, hipStream_t s);
+template <typename T>
+hipError_t launch_cg_rest(const CgArgs& a, hipStream_t s);
+hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
+
+}  // namespace hobak
+
+namespace hobak {
+template <typename T>
+hipError_t launch_force_gather(int nv, int nt, const int* vptr, const int* vlist, const T* stage_f,
+                               double* R, hipStream_t s);
+}  // namespace hobak

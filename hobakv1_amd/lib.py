@@ -1,0 +1,441 @@
+"""ctypes binding of libhobak_hip.so (include/hobak_c.h) and a Python mirror of the reference's
+surface for this path:
+
+    reference (relative to the reference root)                 here
+    TET_MESH(restVertices, tets)  src/Geometry/TET_MESH.h:37  TetMesh(rest, tets)
+    VOLUME::SNH / VOLUME::ARAP    src/Hyperelastic/Volume/    SNH(mu, lam) / ARAP(mu, lam)
+    BACKWARD_EULER_POSITION       src/Timestepper/BACKWARD_EULER_POSITION.h:36
+                                                               BackwardEulerPosition(mesh, material)
+      .solve(verbose) / position() / velocity() / addGravity / attachKinematicConstraints /
+      setDt / setRayleigh / scaleMass                          same names
+
+The GPU library is the only compute path: when libhobak_hip.so is missing or no gfx950 device is
+present, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional
+
+import numpy as np
+
+from . import scenes as _scenes
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libhobak_hip.so")
+
+HOBAK_OK = 0
+ERRORS = {
+    -1: "HOBAK_E_INVALID", -2: "HOBAK_E_HIP", -3: "HOBAK_E_NOMEM", -4: "HOBAK_E_NOT_CONVERGED",
+    -5: "HOBAK_E_NONFINITE", -6: "HOBAK_E_NODEVICE",
+}
+MAX_NEWTON = 8
+
+PRECISION_FP64 = 0
+PRECISION_FP32_REFINE = 1
+
+
+class hobak_material(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("mu", ctypes.c_double), ("lam", ctypes.c_double)]
+
+
+class hobak_step_params(ctypes.Structure):
+    _fields_ = [
+        ("dt", ctypes.c_double), ("rayleigh_alpha", ctypes.c_double),
+        ("rayleigh_beta", ctypes.c_double), ("mass_scale", ctypes.c_double),
+        ("gravity", ctypes.c_double * 3), ("pcg_tol", ctypes.c_double),
+        ("pcg_max_iters", ctypes.c_int32), ("precond", ctypes.c_int32),
+        ("newton_iters", ctypes.c_int32), ("newton_tol", ctypes.c_double),
+        ("precision", ctypes.c_int32), ("refine_inner_tol", ctypes.c_double),
+        ("ground_on", ctypes.c_int32), ("ground_y", ctypes.c_double), ("ground_k", ctypes.c_double),
+    ]
+
+
+class hobak_step_stats(ctypes.Structure):
+    _fields_ = [
+        ("status", ctypes.c_int32), ("newton_iters", ctypes.c_int32),
+        ("pcg_iters", ctypes.c_int32 * MAX_NEWTON), ("refine_iters", ctypes.c_int32 * MAX_NEWTON),
+        ("total_pcg_iters", ctypes.c_int32), ("pcg_error", ctypes.c_double * MAX_NEWTON),
+        ("newton_residual", ctypes.c_double * MAX_NEWTON), ("seconds", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        n = self.newton_iters
+        return {
+            "status": self.status, "newton_iters": n,
+            "pcg_iters": list(self.pcg_iters[:n]), "refine_iters": list(self.refine_iters[:n]),
+            "total_pcg_iters": self.total_pcg_iters, "pcg_error": list(self.pcg_error[:n]),
+            "newton_residual": list(self.newton_residual[:n]), "seconds": self.seconds,
+        }
+
+
+_vp = ctypes.c_void_p
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int32)
+
+# name -> (restype, argtypes); every symbol include/hobak_c.h declares
+SIGNATURES = {
+    "hobak_default_params": (None, [ctypes.POINTER(hobak_step_params)]),
+    "hobak_ctx_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_size_t, ctypes.POINTER(_vp)]),
+    "hobak_ctx_destroy": (None, [_vp]),
+    "hobak_last_error": (ctypes.c_char_p, [_vp]),
+    "hobak_ctx_bytes_in_use": (ctypes.c_size_t, [_vp]),
+    "hobak_inst_create": (ctypes.c_int, [_vp, _dp, ctypes.c_int32, _ip, ctypes.c_int32,
+                                         ctypes.POINTER(hobak_material),
+                                         ctypes.POINTER(hobak_step_params), ctypes.POINTER(_vp)]),
+    "hobak_inst_bytes_estimate": (ctypes.c_size_t, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
+    "hobak_inst_destroy": (None, [_vp]),
+    "hobak_inst_set_pinned": (ctypes.c_int, [_vp, _ip, ctypes.c_int32, _dp]),
+    "hobak_inst_set_state": (ctypes.c_int, [_vp, _dp, _dp]),
+    "hobak_inst_get_state": (ctypes.c_int, [_vp, _dp, _dp]),
+    "hobak_step": (ctypes.c_int, [_vp, ctypes.POINTER(hobak_step_stats)]),
+    "hobak_step_batch": (ctypes.c_int, [_vp, ctypes.POINTER(_vp), ctypes.c_int32,
+                                        ctypes.POINTER(hobak_step_stats)]),
+    "hobak_get_bsr_pattern": (ctypes.c_int, [_vp, _ip, _ip, _ip]),
+    "hobak_eval_forces_hessian": (ctypes.c_int, [_vp, _dp, _dp, _dp]),
+    "hobak_get_last_system": (ctypes.c_int, [_vp, _dp, _dp]),
+    "hobak_last_step_timing": (ctypes.c_int, [_vp, _dp]),
+    "hobak_set_kernel_sampling": (ctypes.c_int, [_vp, ctypes.c_int32]),
+    "hobak_kernel_stats": (ctypes.c_int, [_vp, _dp, ctypes.POINTER(ctypes.c_int64), _dp]),
+    "hobak_copy_probe": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.c_int32, _dp]),
+}
+
+_lib = None
+
+
+class HobakError(RuntimeError):
+    def __init__(self, code, msg=""):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+def load(path: str = LIB_PATH):
+    """Load libhobak_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is missing: build it with `python -m hobakv1_amd.build`")
+        lib = ctypes.CDLL(path)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(lib, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip)
+
+
+def default_params() -> hobak_step_params:
+    p = hobak_step_params()
+    load().hobak_default_params(ctypes.byref(p))
+    return p
+
+
+class Context:
+    """One device + stream + HBM budget (0 = unlimited)."""
+
+    def __init__(self, device: int = 0, budget_bytes: int = 0):
+        self.lib = load()
+        h = _vp()
+        rc = self.lib.hobak_ctx_create(device, budget_bytes, ctypes.byref(h))
+        if rc != HOBAK_OK:
+            raise HobakError(rc, "hobak_ctx_create failed (no gfx950 device?)")
+        self.h = h
+
+    def error(self) -> str:
+        return self.lib.hobak_last_error(self.h).decode()
+
+    def check(self, rc, what=""):
+        if rc != HOBAK_OK:
+            raise HobakError(rc, f"{what}: {self.error()}")
+        return rc
+
+    def copy_probe(self, nbytes, reps=20):
+        ms = ctypes.c_double(0)
+        self.check(self.lib.hobak_copy_probe(self.h, int(nbytes), int(reps), ctypes.byref(ms)),
+                   "copy_probe")
+        return ms.value
+
+    def bytes_in_use(self) -> int:
+        return self.lib.hobak_ctx_bytes_in_use(self.h)
+
+    def close(self):
+        if self.h:
+            self.lib.hobak_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def make_params(scene, precond=_scenes.PRECOND_BLOCK_JACOBI, precision=PRECISION_FP64,
+                newton_iters=None, newton_tol=1e-7, pcg_tol=None, refine_inner_tol=1e-4):
+    p = default_params()
+    p.dt = scene.dt
+    p.rayleigh_alpha = scene.rayleigh_alpha
+    p.rayleigh_beta = scene.rayleigh_beta
+    p.mass_scale = scene.mass_scale
+    for c in range(3):
+        p.gravity[c] = scene.gravity[c]
+    p.pcg_tol = scene.pcg_tol if pcg_tol is None else pcg_tol
+    p.pcg_max_iters = scene.pcg_max_iters
+    p.precond = precond
+    p.newton_iters = scene.newton_iters if newton_iters is None else newton_iters
+    p.newton_tol = newton_tol
+    p.precision = precision
+    p.refine_inner_tol = refine_inner_tol
+    if scene.ground_y is not None:
+        p.ground_on = 1
+        p.ground_y = scene.ground_y
+        p.ground_k = scene.ground_k
+    return p
+
+
+class Instance:
+    """One HBM-resident simulation instance of a Scene."""
+
+    def __init__(self, ctx: Context, scene, precond=_scenes.PRECOND_BLOCK_JACOBI,
+                 precision=PRECISION_FP64, newton_iters=None, newton_tol=1e-7, pcg_tol=None,
+                 refine_inner_tol=1e-4, params: Optional[hobak_step_params] = None):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.scene = scene
+        self.dofs = scene.dofs
+        self.nv = scene.nv
+        self.params = params or make_params(scene, precond, precision, newton_iters, newton_tol,
+                                            pcg_tol, refine_inner_tol)
+        mat = hobak_material(scene.material, scene.mu, scene.lam)
+        rest = np.ascontiguousarray(scene.rest, dtype=np.float64)
+        tets = np.ascontiguousarray(scene.tets, dtype=np.int32)
+        h = _vp()
+        ctx.check(self.lib.hobak_inst_create(ctx.h, _d(rest), scene.nv, _i(tets), scene.nt,
+                                             ctypes.byref(mat), ctypes.byref(self.params),
+                                             ctypes.byref(h)), "hobak_inst_create")
+        self.h = h
+        pins = np.ascontiguousarray(scene.pinned(), dtype=np.int32)
+        if pins.size:
+            tg = np.ascontiguousarray(scene.pinned_targets(), dtype=np.float64)
+            ctx.check(self.lib.hobak_inst_set_pinned(self.h, _i(pins), pins.size, _d(tg)),
+                      "set_pinned")
+        self.set_state(scene.initial_u(), scene.initial_v())
+
+    def close(self):
+        if self.h:
+            self.lib.hobak_inst_destroy(self.h)
+            self.h = None
+
+    def set_state(self, u=None, v=None):
+        uu = None if u is None else np.ascontiguousarray(u, dtype=np.float64)
+        vv = None if v is None else np.ascontiguousarray(v, dtype=np.float64)
+        self.ctx.check(self.lib.hobak_inst_set_state(self.h, _d(uu) if uu is not None else None,
+                                                     _d(vv) if vv is not None else None),
+                       "set_state")
+
+    def state(self):
+        u = np.zeros(self.dofs)
+        v = np.zeros(self.dofs)
+        self.ctx.check(self.lib.hobak_inst_get_state(self.h, _d(u), _d(v)), "get_state")
+        return u, v
+
+    def step(self, allow_unconverged=False):
+        st = hobak_step_stats()
+        rc = self.lib.hobak_step(self.h, ctypes.byref(st))
+        if rc != HOBAK_OK and not (allow_unconverged and rc == -4):
+            raise HobakError(rc, self.ctx.error())
+        return st.as_dict()
+
+    def timing_ms(self):
+        out = np.zeros(5)
+        self.ctx.check(self.lib.hobak_last_step_timing(self.h, _d(out)), "timing")
+        return dict(zip(["element", "assemble", "rhs", "pcg", "update"], out.tolist()))
+
+    def set_kernel_sampling(self, every):
+        self.ctx.check(self.lib.hobak_set_kernel_sampling(self.h, int(every)), "sampling")
+
+    def kernel_stats(self):
+        ms = ctypes.c_double(0)
+        n = ctypes.c_int64(0)
+        b = ctypes.c_double(0)
+        self.ctx.check(self.lib.hobak_kernel_stats(self.h, ctypes.byref(ms), ctypes.byref(n),
+                                                   ctypes.byref(b)), "kernel_stats")
+        return {"spmv_avg_ms": ms.value, "spmv_count": n.value, "spmv_bytes": b.value}
+
+    def bsr_pattern(self):
+        n = ctypes.c_int32(0)
+        self.ctx.check(self.lib.hobak_get_bsr_pattern(self.h, ctypes.byref(n), None, None))
+        rp = np.zeros(self.nv + 1, np.int32)
+        ci = np.zeros(n.value, np.int32)
+        self.ctx.check(self.lib.hobak_get_bsr_pattern(self.h, None, _i(rp), _i(ci)))
+        return rp, ci
+
+    def forces_hessian(self, u, with_k=True):
+        rp, ci = self.bsr_pattern()
+        R = np.zeros(self.dofs)
+        K = np.zeros(9 * ci.size) if with_k else None
+        uu = np.ascontiguousarray(u, dtype=np.float64)
+        self.ctx.check(self.lib.hobak_eval_forces_hessian(self.h, _d(uu), _d(R),
+                                                          _d(K) if with_k else None),
+                       "eval_forces_hessian")
+        return R, (K.reshape(-1, 3, 3) if with_k else None)
+
+    def last_system(self):
+        rp, ci = self.bsr_pattern()
+        rhs = np.zeros(self.dofs)
+        lhs = np.zeros(9 * ci.size)
+        self.ctx.check(self.lib.hobak_get_last_system(self.h, _d(rhs), _d(lhs)), "last_system")
+        return rhs, lhs.reshape(-1, 3, 3)
+
+
+def step_batch(ctx: Context, insts, allow_unconverged=False):
+    n = len(insts)
+    arr = (_vp * n)(*[i.h for i in insts])
+    stats = (hobak_step_stats * n)()
+    rc = ctx.lib.hobak_step_batch(ctx.h, arr, n, stats)
+    if rc != HOBAK_OK and not (allow_unconverged and rc == -4):
+        raise HobakError(rc, ctx.error())
+    return [s.as_dict() for s in stats]
+
+
+# ---------------------------------------------------------------------------------------------
+# Reference-shaped facade
+# ---------------------------------------------------------------------------------------------
+class TetMesh:
+    """TET_MESH(restVertices, tets) -- src/Geometry/TET_MESH.h:37-38."""
+
+    def __init__(self, rest_vertices, tets):
+        self._rest = np.ascontiguousarray(rest_vertices, dtype=np.float64).reshape(-1, 3)
+        self._tets = np.ascontiguousarray(tets, dtype=np.int32).reshape(-1, 4)
+
+    def restVertices(self):
+        return self._rest
+
+    def tets(self):
+        return self._tets
+
+    def DOFs(self):
+        return 3 * self._rest.shape[0]
+
+
+class HYPERELASTIC:
+    kind = -1
+
+    def __init__(self, mu, lam):
+        self.mu = float(mu)
+        self.lam = float(lam)
+
+    @staticmethod
+    def computeMu(E, nu):
+        return _scenes.lame(E, nu)[0]
+
+    @staticmethod
+    def computeLambda(E, nu):
+        return _scenes.lame(E, nu)[1]
+
+
+class SNH(HYPERELASTIC):
+    kind = _scenes.MAT_SNH
+
+    def name(self):
+        return "SNH"
+
+
+class ARAP(HYPERELASTIC):
+    kind = _scenes.MAT_ARAP
+
+    def name(self):
+        return "ARAP"
+
+
+class BackwardEulerPosition:
+    """BACKWARD_EULER_POSITION(tetMesh, hyperelastic) with the TIMESTEPPER setters.
+
+    The device instance is created at the first solve(), after all setters have run, the way the
+    reference reads its members inside solveRayleighDamped.
+    """
+
+    def __init__(self, mesh: TetMesh, material: HYPERELASTIC, ctx: Optional[Context] = None,
+                 precond=_scenes.PRECOND_BLOCK_JACOBI, precision=PRECISION_FP64):
+        self.mesh = mesh
+        self.material = material
+        self.ctx = ctx or Context()
+        self._owns_ctx = ctx is None
+        n = mesh.DOFs()
+        self._position = np.zeros(n)
+        self._velocity = np.zeros(n)
+        self._gravity = np.zeros(3)
+        self._dt = 1.0 / 60.0
+        self._alpha = 0.01
+        self._beta = 0.01
+        self._mass_scale = 1.0
+        self._pins = np.zeros(0, np.int32)
+        self._targets = np.zeros((0, 3))
+        self._precond = precond
+        self._precision = precision
+        self._tol = 0.0
+        self._inst = None
+        self.last_stats = None
+
+    # TIMESTEPPER.h:44-58
+    def position(self):
+        return self._position
+
+    def velocity(self):
+        return self._velocity
+
+    def setDt(self, dt):
+        self._dt = float(dt)
+
+    def setRayleigh(self, alpha, beta):
+        self._alpha, self._beta = float(alpha), float(beta)
+
+    def scaleMass(self, s):
+        self._mass_scale *= float(s)
+
+    def addGravity(self, g):
+        self._gravity = np.asarray(g, dtype=np.float64)
+
+    def setTolerance(self, tol):
+        self._tol = float(tol)
+
+    def attachKinematicConstraints(self, cube: _scenes.Cube):
+        X = self.mesh.restVertices()
+        ids = np.nonzero(cube.inside(X))[0].astype(np.int32)
+        self._pins = np.union1d(self._pins, ids).astype(np.int32)
+        self._targets = cube.to_world(cube.to_local(X[self._pins])) - X[self._pins]
+
+    def _ensure(self):
+        if self._inst is not None:
+            return
+        sc = _scenes.Scene(name="facade", rest=self.mesh.restVertices(), tets=self.mesh.tets(),
+                           material=self.material.kind, mu=self.material.mu,
+                           lam=self.material.lam, dt=self._dt, rayleigh_alpha=self._alpha,
+                           rayleigh_beta=self._beta, mass_scale=self._mass_scale,
+                           gravity=tuple(self._gravity), pcg_tol=self._tol)
+        self._inst = Instance(self.ctx, sc, precond=self._precond, precision=self._precision)
+        if self._pins.size:
+            self.ctx.check(self.ctx.lib.hobak_inst_set_pinned(
+                self._inst.h, _i(np.ascontiguousarray(self._pins)), self._pins.size,
+                _d(np.ascontiguousarray(self._targets, dtype=np.float64))))
+
+    def solve(self, verbose=False):
+        self._ensure()
+        self._inst.set_state(self._position, self._velocity)
+        self.last_stats = self._inst.step(allow_unconverged=True)
+        if verbose:
+            print(f"  PCG iters: {self.last_stats['pcg_iters'][0]:3d} "
+                  f"err: {self.last_stats['pcg_error'][0]:6.4e}")
+        u, v = self._inst.state()
+        self._position[:] = u
+        self._velocity[:] = v
+        return True

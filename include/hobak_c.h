@@ -1,0 +1,161 @@
+/*
+ * hobak_c.h -- C ABI of the MI355X backward-Euler Newton step (libhobak_hip.so).
+ *
+ * The reference exposes this path as C++ virtual interfaces, not an FFI.  Every entry point below
+ * names the reference interface it replaces (paths relative to the reference root):
+ *
+ *   TET_MESH(restVertices, tets)                     src/Geometry/TET_MESH.h:37-38
+ *   TET_MESH_FASTER(restVertices, tets)              src/Geometry/TET_MESH_FASTER.h:28-32
+ *   VOLUME::SNH(mu, lambda), VOLUME::ARAP(mu, lambda) src/Hyperelastic/Volume/SNH.h:28, ARAP.h:28
+ *   BACKWARD_EULER_POSITION(mesh, material)          src/Timestepper/BACKWARD_EULER_POSITION.h:36
+ *   TIMESTEPPER::solve(verbose)                      src/Timestepper/TIMESTEPPER.h:77
+ *   position() / velocity() / externalForces()       src/Timestepper/TIMESTEPPER.h:44-58
+ *   addGravity, attachKinematicConstraints, setDt,   src/Timestepper/TIMESTEPPER.h:70-100
+ *   setRayeligh, scaleMass
+ *   computeHyperelasticForces                        src/Geometry/TET_MESH.h:72-75
+ *   computeHyperelasticClampedHessian                src/Geometry/TET_MESH.h:107-110
+ *
+ * Conventions
+ *   - Every function returns HOBAK_OK (0) or a negative HOBAK_E_* code; no C++ exception crosses
+ *     the ABI.  hobak_last_error() gives the message of the last failure on a context.
+ *   - Host buffers are caller-owned and copied in or out; device memory is library-owned and
+ *     stays resident in HBM between steps.
+ *   - Vectors are DOF-ordered like the reference's VECTOR: entry 3*v + c is coordinate c of vertex
+ *     v.  position/displacement is u = x - X (the reference's TIMESTEPPER::position()).
+ *   - Block matrices are exported as BSR with 3x3 row-major blocks.
+ *   - A context is not thread-safe without external synchronisation.  Each instance runs on its
+ *     context's HIP stream.
+ */
+#ifndef HOBAK_C_H
+#define HOBAK_C_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HOBAK_OK 0
+#define HOBAK_E_INVALID (-1)     /* bad argument or shape */
+#define HOBAK_E_HIP (-2)         /* a HIP runtime call failed */
+#define HOBAK_E_NOMEM (-3)       /* device allocation failed or the HBM budget would be exceeded */
+#define HOBAK_E_NOT_CONVERGED (-4) /* PCG hit its iteration limit (state is still advanced) */
+#define HOBAK_E_NONFINITE (-5)   /* NaN or Inf appeared in the solve (state is not advanced) */
+#define HOBAK_E_NODEVICE (-6)    /* no usable gfx950 device */
+
+#define HOBAK_MAT_SNH 0          /* Stable Neo-Hookean, src/Hyperelastic/Volume/SNH.cpp */
+#define HOBAK_MAT_ARAP 1         /* As-Rigid-As-Possible, src/Hyperelastic/Volume/ARAP.cpp */
+
+#define HOBAK_PRECOND_JACOBI 0       /* the reference's DiagonalPreconditioner */
+#define HOBAK_PRECOND_BLOCK_JACOBI 1 /* 3x3 block Jacobi */
+
+#define HOBAK_PRECISION_FP64 0       /* everything in FP64 */
+#define HOBAK_PRECISION_FP32_REFINE 1 /* FP32 element/assembly/PCG + FP64 iterative refinement */
+
+#define HOBAK_MAX_NEWTON 8
+
+typedef struct hobak_ctx hobak_ctx;
+typedef struct hobak_inst hobak_inst;
+
+/* VOLUME::SNH(mu, lambda) / VOLUME::ARAP(mu, lambda): unreparameterised Lame constants. */
+typedef struct {
+  int32_t kind;
+  double mu;
+  double lambda;
+} hobak_material;
+
+/* Integrator settings.  Defaults that match BACKWARD_EULER_POSITION::initialize
+ * (src/Timestepper/BACKWARD_EULER_POSITION.cpp:50-79) come from hobak_default_params(). */
+typedef struct {
+  double dt;              /* setDt */
+  double rayleigh_alpha;  /* setRayeligh(alpha, beta) */
+  double rayleigh_beta;
+  double mass_scale;      /* scaleMass(rho) applied once */
+  double gravity[3];      /* body force passed to addGravity every step */
+  double pcg_tol;         /* Eigen CG tolerance (<= 0: machine epsilon, the Eigen default) */
+  int32_t pcg_max_iters;  /* <= 0: 2 * DOFs, the Eigen default */
+  int32_t precond;        /* HOBAK_PRECOND_* */
+  int32_t newton_iters;   /* 1: BACKWARD_EULER_POSITION; >1: BDF_1 Newton loop (src/Timestepper/BDF_1.cpp:78-239) */
+  double newton_tol;      /* BDF_1 residual test ||S(b - A z)||^2 > tol (NEWMARK.cpp:43 default 1e-7) */
+  int32_t precision;      /* HOBAK_PRECISION_* */
+  double refine_inner_tol; /* FP32 inner PCG relative tolerance for the refinement form */
+  int32_t ground_on;      /* penalty ground plane y = ground_y, stiffness ground_k (no reference counterpart) */
+  double ground_y;
+  double ground_k;
+} hobak_step_params;
+
+typedef struct {
+  int32_t status;                          /* HOBAK_OK or an error code for this step */
+  int32_t newton_iters;                    /* Newton iterations taken */
+  int32_t pcg_iters[HOBAK_MAX_NEWTON];     /* PCG iterations per Newton iteration (Eigen count) */
+  int32_t refine_iters[HOBAK_MAX_NEWTON];  /* FP64 refinement sweeps per Newton iteration */
+  int32_t total_pcg_iters;
+  double pcg_error[HOBAK_MAX_NEWTON];      /* sqrt(|r|^2 / |b|^2) as Eigen's error() */
+  double newton_residual[HOBAK_MAX_NEWTON];/* |S b|^2 before each solve */
+  double seconds;                          /* host wall time of the call */
+} hobak_step_stats;
+
+void hobak_default_params(hobak_step_params* p);
+
+/* Context: one device, one stream, an HBM budget for admission (0 = no limit). */
+int hobak_ctx_create(int device, size_t hbm_budget_bytes, hobak_ctx** out);
+void hobak_ctx_destroy(hobak_ctx* ctx);
+const char* hobak_last_error(hobak_ctx* ctx);
+/* Bytes of device memory currently held by this context's instances. */
+size_t hobak_ctx_bytes_in_use(hobak_ctx* ctx);
+
+/* new TET_MESH_FASTER(rest, tets) + new SNH/ARAP + new BACKWARD_EULER_POSITION(mesh, material).
+ * rest_xyz: nv*3 doubles; tets: nt*4 int32 (positive rest volume required, as TET_MESH.cpp:132-137). */
+int hobak_inst_create(hobak_ctx* ctx, const double* rest_xyz, int32_t nv, const int32_t* tets,
+                      int32_t nt, const hobak_material* material, const hobak_step_params* params,
+                      hobak_inst** out);
+/* Device bytes hobak_inst_create would allocate for this mesh (admission control). */
+size_t hobak_inst_bytes_estimate(int32_t nv, int32_t nt, int32_t precision);
+void hobak_inst_destroy(hobak_inst* inst);
+
+/* attachKinematicConstraints: pinned vertex ids (ascending) and the displacement each is held at
+ * (world(local(X)) - X in the reference; NULL means zero). */
+int hobak_inst_set_pinned(hobak_inst* inst, const int32_t* vids, int32_t n, const double* targets);
+/* position() = u and velocity() = v, DOF-length host arrays; NULL leaves that vector unchanged. */
+int hobak_inst_set_state(hobak_inst* inst, const double* u, const double* v);
+int hobak_inst_get_state(hobak_inst* inst, double* u, double* v);
+
+/* One SIMULATION_SCENE::stepSimulation (SIMULATION_SCENE.h:120-126): f_ext = one-ring volume *
+ * gravity, then BACKWARD_EULER_POSITION::solve (or the BDF_1 loop when newton_iters > 1). */
+int hobak_step(hobak_inst* inst, hobak_step_stats* stats);
+
+/* Advance n independent instances by one step each, batched on the device.  stats: n entries. */
+int hobak_step_batch(hobak_ctx* ctx, hobak_inst* const* insts, int32_t n, hobak_step_stats* stats);
+
+/* Parity hooks.  BSR pattern of the stiffness (block rows by vertex, columns ascending). */
+int hobak_get_bsr_pattern(hobak_inst* inst, int32_t* nnzb, int32_t* row_ptr, int32_t* col_idx);
+/* computeHyperelasticForces and computeHyperelasticClampedHessian at displacement u:
+ * forces (DOFs) and K (nnzb*9, row-major blocks, the reference's sign: K = -stiffness).
+ * Either output may be NULL. */
+int hobak_eval_forces_hessian(hobak_inst* inst, const double* u, double* forces, double* kvals);
+/* The last Newton iteration's projected system: rhs = S b (DOFs) and LHS = S A S + (I - S)
+ * (nnzb*9).  Either output may be NULL. */
+int hobak_get_last_system(hobak_inst* inst, double* rhs, double* lhs_vals);
+
+/* Device timing of the last hobak_step: milliseconds spent in the element kernel, assembly,
+ * right-hand side, PCG and state update (HIP events).  out: 5 doubles. */
+int hobak_last_step_timing(hobak_inst* inst, double* out_ms);
+
+/* Measurement hooks (no counterpart in the reference).
+ * hobak_set_kernel_sampling: every `every`-th PCG iteration of the following steps brackets its
+ * SpMV launch with HIP events on the instance's stream (0 turns sampling off and clears the
+ * accumulators).  hobak_kernel_stats: average duration (ms) and count of the sampled SpMV
+ * launches that did work, and the SpMV's algorithmic bytes per launch
+ * nnzb * (9 s + 4) + 4 (nv + 1) + 2 * 3 s * nv  (s = bytes per real). */
+int hobak_set_kernel_sampling(hobak_inst* inst, int32_t every);
+int hobak_kernel_stats(hobak_inst* inst, double* spmv_avg_ms, int64_t* spmv_count,
+                       double* spmv_bytes);
+/* Streaming copy of `bytes` device bytes (a calibration kernel for hardware byte counters);
+ * returns the kernel time in ms. */
+int hobak_copy_probe(hobak_ctx* ctx, size_t bytes, int32_t reps, double* ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HOBAK_C_H */

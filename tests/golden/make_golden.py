@@ -1,0 +1,192 @@
+"""Write the golden vectors of the backward-Euler Newton step from the literal reference.
+
+Runs only where the reference tree is present: it loads oracle/_ref/libhobak_ref.so, which
+oracle/Makefile compiles from the reference's own sources (make -C oracle ref).  The outputs are
+small .npz fixtures (no pickles) committed beside this script:
+
+  element_fixtures.npz  per-F svd_rv, PK1, psi and clamped dP/dF for SNH and ARAP
+                        (src/util/MATRIX_UTIL.cpp:76-95, src/Hyperelastic/Volume/SNH.cpp, ARAP.cpp)
+  mesh_small.npz        forces R and clamped stiffness K of a 3x3x3 lattice at a seeded displacement
+                        (src/Geometry/TET_MESH.cpp:736-770, TET_MESH_FASTER.cpp:89-133)
+  c1_traj.npz           the 10^3 pinned SNH cube, 20 BACKWARD_EULER_POSITION steps with the
+                        reference's (Jacobi) PCG at tol 1e-8: iterations, errors, states
+  c1_arap.npz           the same scene with ARAP (mu = 3.4e4), 10 steps
+  c2_traj.npz           the 40^3 stretch scene, 20 steps: iterations, norms, strided samples of u, v
+
+Usage:  python tests/golden/make_golden.py [--skip-c2]
+"""
+from __future__ import annotations
+
+import argparse
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+import oracle  # noqa: E402
+from hobakv1_amd import scenes  # noqa: E402
+
+C2_STRIDE = 17
+
+
+def element_fs(seed=7):
+    """A fixed set of deformation gradients: random, near-rest, stretched, near-inverted,
+    inverted, with repeated singular values, rotations and reflections."""
+    rng = np.random.default_rng(seed)
+    Fs = []
+
+    def rot(v):
+        q, _ = np.linalg.qr(v)
+        if np.linalg.det(q) < 0:
+            q[:, 0] *= -1
+        return q
+
+    for _ in range(12):
+        Fs.append(rng.uniform(-1, 1, (3, 3)))
+    for _ in range(6):
+        Fs.append(np.eye(3) + 1e-3 * rng.uniform(-1, 1, (3, 3)))
+    for s in ([2, 3, 5], [-2, 3, 5], [-2, -3, -5], [1, 1, 1], [1.2, 1.2, 0.8], [0.5, 0.5, 0.5],
+              [1, 1, -1], [3, 1e-3, 2], [1.0, 1.0, 1e-6], [0.7, 1.3, 1.0]):
+        U = rot(rng.normal(size=(3, 3)))
+        V = rot(rng.normal(size=(3, 3)))
+        Fs.append(U @ np.diag(s) @ V.T)
+    for _ in range(6):
+        Fs.append(rot(rng.normal(size=(3, 3))))
+    for _ in range(6):
+        R = rot(rng.normal(size=(3, 3)))
+        Fs.append(R @ np.diag([1.0, 1.0, -1.0]) @ rot(rng.normal(size=(3, 3))))
+    for _ in range(8):
+        F = rng.uniform(-1, 1, (3, 3))
+        F[:, 2] = 0.5 * F[:, 0] - 0.25 * F[:, 1] + 1e-4 * rng.uniform(-1, 1, 3)
+        Fs.append(F)
+    return np.array(Fs)
+
+
+def make_element(out):
+    ref = oracle.RefElement()
+    Fs = element_fs()
+    params = {"snh": (scenes.MAT_SNH, 1.0, 10.0), "snh_stiff": (scenes.MAT_SNH, 34482.75862068966,
+                                                                310344.8275862069),
+              "arap": (scenes.MAT_ARAP, 1.0, 10.0)}
+    data = {"F": Fs}
+    for key, (mat, mu, lam) in params.items():
+        data[f"{key}_params"] = np.array([mat, mu, lam])
+        data[f"{key}_psi"] = np.array([ref.psi(mat, mu, lam, F) for F in Fs])
+        data[f"{key}_P"] = np.array([ref.pk1(mat, mu, lam, F) for F in Fs])
+        data[f"{key}_H"] = np.array([ref.clamped_hessian(mat, mu, lam, F) for F in Fs])
+        data[f"{key}_Hraw"] = np.array([ref.hessian(mat, mu, lam, F) for F in Fs])
+    svd = [ref.svd_rv(F) for F in Fs]
+    data["U"] = np.array([s[0] for s in svd])
+    data["S"] = np.array([s[1] for s in svd])
+    data["V"] = np.array([s[2] for s in svd])
+    np.savez_compressed(out, **data)
+
+
+def make_mesh_small(out):
+    data = {}
+    for mat, tag in ((scenes.MAT_SNH, "snh"), (scenes.MAT_ARAP, "arap")):
+        sc = scenes.lattice_scene((3, 3, 3), seed=11, E=1e5, nu=0.45, material=mat, pin=False)
+        rng = np.random.default_rng(3)
+        u = 0.05 * rng.uniform(-1, 1, sc.dofs)
+        r = oracle.RefSim(sc)
+        R, csc = r.forces_hessian(u)
+        o = oracle.Oracle(sc)
+        rp, ci = o.bsr_pattern()
+        K = oracle.csc_to_bsr_vals(csc, rp, ci)
+        data[f"{tag}_rest"] = sc.rest
+        data[f"{tag}_tets"] = sc.tets
+        data[f"{tag}_mat"] = np.array([mat, sc.mu, sc.lam])
+        data[f"{tag}_u"] = u
+        data[f"{tag}_R"] = R
+        data[f"{tag}_row_ptr"] = rp
+        data[f"{tag}_col_idx"] = ci
+        data[f"{tag}_K"] = K
+        r.close()
+        o.close()
+    np.savez_compressed(out, **data)
+
+
+def run_traj(sc, steps, keep_steps, stride=1):
+    r = oracle.RefSim(sc)
+    iters, errs, secs, unorm, vnorm = [], [], [], [], []
+    us, vs = {}, {}
+    b0 = None
+    for k in range(1, steps + 1):
+        it, err, sec = r.step()
+        if k == 1:
+            b0 = r.last_b()
+        u, v = r.state()
+        iters.append(it)
+        errs.append(err)
+        secs.append(sec)
+        unorm.append(np.linalg.norm(u))
+        vnorm.append(np.linalg.norm(v))
+        if k in keep_steps:
+            us[k] = u[::stride].copy()
+            vs[k] = v[::stride].copy()
+        print(f"  step {k}: iters {it} err {err:.3e} {sec:.3f}s", flush=True)
+    m = r.mass_diag()
+    r.close()
+    return dict(iters=np.array(iters), errs=np.array(errs), secs=np.array(secs),
+                unorm=np.array(unorm), vnorm=np.array(vnorm), us=us, vs=vs, b0=b0, mass=m)
+
+
+def make_c1(out):
+    sc = scenes.config1()
+    t = run_traj(sc, 20, keep_steps=(1, 2, 5, 10, 20))
+    data = dict(iters=t["iters"], errs=t["errs"], secs=t["secs"], unorm=t["unorm"],
+                vnorm=t["vnorm"], b0=t["b0"], mass=t["mass"],
+                keep=np.array(sorted(t["us"])))
+    for k, u in t["us"].items():
+        data[f"u{k}"] = u
+        data[f"v{k}"] = t["vs"][k]
+    np.savez_compressed(out, **data)
+
+
+def make_c1_arap(out):
+    sc = scenes.lattice_scene((10, 10, 10), seed=1, material=scenes.MAT_ARAP, mu=3.4e4,
+                              name="cube10_arap")
+    t = run_traj(sc, 10, keep_steps=(1, 10))
+    data = dict(iters=t["iters"], errs=t["errs"], unorm=t["unorm"], vnorm=t["vnorm"],
+                u1=t["us"][1], u10=t["us"][10], v10=t["vs"][10])
+    np.savez_compressed(out, **data)
+
+
+def make_c2(out):
+    sc = scenes.config2()
+    t = run_traj(sc, 20, keep_steps=(1, 10, 20), stride=C2_STRIDE)
+    data = dict(iters=t["iters"], errs=t["errs"], secs=t["secs"], unorm=t["unorm"],
+                vnorm=t["vnorm"], stride=np.array(C2_STRIDE))
+    for k, u in t["us"].items():
+        data[f"u{k}"] = u
+        data[f"v{k}"] = t["vs"][k]
+    np.savez_compressed(out, **data)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--skip-c2", action="store_true")
+    ap.add_argument("--only", default=None)
+    args = ap.parse_args()
+    if not oracle.ref_available():
+        sys.exit("oracle/_ref/libhobak_ref.so is not built (make -C oracle ref)")
+    jobs = [("element_fixtures.npz", make_element), ("mesh_small.npz", make_mesh_small),
+            ("c1_traj.npz", make_c1), ("c1_arap.npz", make_c1_arap)]
+    if not args.skip_c2:
+        jobs.append(("c2_traj.npz", make_c2))
+    for name, fn in jobs:
+        if args.only and args.only != name:
+            continue
+        t0 = time.time()
+        print(name, flush=True)
+        fn(os.path.join(HERE, name))
+        print(f"  wrote {name} in {time.time() - t0:.1f}s", flush=True)
+
+
+if __name__ == "__main__":
+    main()
