@@ -43,7 +43,7 @@ namespace {
 struct Pattern {
   int nv = 0, nt = 0, nslices = 0, nslots = 0, nrows_pad = 0, nnzb = 0;
   std::vector<int> row_ptr, col_idx;  // BSR
-  std::vector<int> slice_off, row_len, col, bsr_slot, diag_slot;
+  std::vector<int> slice_off, row_len, col, slot_row, bsr_slot, diag_slot;
   std::vector<int> cptr, clist;       // per slot contributions
   std::vector<int> vptr, vlist;       // per vertex force gather
 };
@@ -69,6 +69,8 @@ void build_pattern(int nv, int nt, const int32_t* tets, Pattern& P) {
     std::vector<int>().swap(n);
   }
   P.nnzb = P.row_ptr[nv];
+  // SELL-8, k-interleaved: slice width rounded up to a multiple of kLanesPerRow so that each
+  // slice's slots form whole 64-slot wave loads
   P.nslices = (nv + kSlice - 1) / kSlice;
   P.nrows_pad = P.nslices * kSlice;
   P.slice_off.assign(P.nslices + 1, 0);
@@ -79,10 +81,12 @@ void build_pattern(int nv, int nt, const int32_t* tets, Pattern& P) {
       P.row_len[r] = P.row_ptr[r + 1] - P.row_ptr[r];
       w = std::max(w, P.row_len[r]);
     }
+    w = (w + kLanesPerRow - 1) / kLanesPerRow * kLanesPerRow;
     P.slice_off[s + 1] = P.slice_off[s] + w * kSlice;
   }
   P.nslots = P.slice_off[P.nslices];
   P.col.assign(P.nslots, 0);
+  P.slot_row.assign(P.nslots, -1);
   P.bsr_slot.assign(P.nnzb, 0);
   P.diag_slot.assign(nv, -1);
   for (int s = 0; s < P.nslices; s++) {
@@ -94,6 +98,7 @@ void build_pattern(int nv, int nt, const int32_t* tets, Pattern& P) {
         if (r < nv && k < P.row_len[r]) {
           const int j = P.col_idx[P.row_ptr[r] + k];
           P.col[slot] = j;
+          P.slot_row[slot] = r;
           P.bsr_slot[P.row_ptr[r] + k] = slot;
           if (j == r) P.diag_slot[r] = slot;
         } else {
@@ -160,15 +165,18 @@ struct hobak_inst {
   double *pin_target = nullptr, *rhs64 = nullptr, *tmp64 = nullptr, *x64 = nullptr, *res64 = nullptr;
   double* kneg64 = nullptr;
   int4* tets = nullptr;
-  int *slice_off = nullptr, *row_len = nullptr, *col = nullptr, *cptr = nullptr, *clist = nullptr;
+  int *slice_off = nullptr, *row_len = nullptr, *col = nullptr, *slot_row = nullptr;
+  int *cptr = nullptr, *clist = nullptr;
   int *vptr = nullptr, *vlist = nullptr;
   unsigned char* pinned = nullptr;
   // device, real type T
   void *stage_h = nullptr, *stage_f = nullptr, *A = nullptr, *D = nullptr, *E = nullptr, *C = nullptr;
-  void *minv = nullptr, *r = nullptr, *x = nullptr, *z = nullptr, *p = nullptr, *t = nullptr;
+  void *minv = nullptr, *r = nullptr, *x = nullptr, *z = nullptr, *t = nullptr;
+  void *pbuf0 = nullptr, *pbuf1 = nullptr;
   CgScalars* sc = nullptr;
   CgScalars* sc_host = nullptr;
   double* partial = nullptr;
+  size_t partial_capacity = 0;  // doubles
   unsigned* counters = nullptr;
   hipEvent_t ev[6] = {};
   double last_ms[5] = {0, 0, 0, 0, 0};
@@ -261,11 +269,9 @@ ElementArgs element_args(hobak_inst* I, const double* u, int with_h) {
 AsmArgs asm_args(hobak_inst* I, int mode, void* out) {
   AsmArgs a{};
   a.nrows = I->nv;
-  a.nrows_pad = I->P.nrows_pad;
   a.nslots = I->P.nslots;
   a.nt = I->nt;
-  a.slice_off = I->slice_off;
-  a.row_len = I->row_len;
+  a.slot_row = I->slot_row;
   a.col = I->col;
   a.cptr = I->cptr;
   a.clist = I->clist;
@@ -287,7 +293,6 @@ AsmArgs asm_args(hobak_inst* I, int mode, void* out) {
 CgArgs cg_args(hobak_inst* I) {
   CgArgs a{};
   a.nrows = I->nv;
-  a.nrows_pad = I->P.nrows_pad;
   a.nslots = I->P.nslots;
   a.slice_off = I->slice_off;
   a.row_len = I->row_len;
@@ -296,7 +301,8 @@ CgArgs cg_args(hobak_inst* I) {
   a.x = I->x;
   a.r = I->r;
   a.z = I->z;
-  a.p = I->p;
+  a.pbuf0 = I->pbuf0;
+  a.pbuf1 = I->pbuf1;
   a.t = I->t;
   a.minv = I->minv;
   a.precond = I->prm.precond;
@@ -308,7 +314,6 @@ CgArgs cg_args(hobak_inst* I) {
 RhsArgs rhs_args(hobak_inst* I, int bdf) {
   RhsArgs a{};
   a.nrows = I->nv;
-  a.nrows_pad = I->P.nrows_pad;
   a.nslots = I->P.nslots;
   a.nt = I->nt;
   a.slice_off = I->slice_off;
@@ -338,7 +343,8 @@ RhsArgs rhs_args(hobak_inst* I, int bdf) {
   a.r = I->r;
   a.x = I->x;
   a.z = I->z;
-  a.p = I->p;
+  a.pbuf0 = I->pbuf0;
+  a.pbuf1 = I->pbuf1;
   a.minv = I->minv;
   a.precond = I->prm.precond;
   a.tol = I->prm.pcg_tol > 0 ? I->prm.pcg_tol : 2.220446049250313e-16;
@@ -356,10 +362,8 @@ int setup_constant_parts(hobak_inst* I) {
   HIPCHK(I->ctx, launch_element<double>(element_args(I, I->tmp64, 1), s));
   HIPCHK(I->ctx, launch_assemble<double>(asm_args(I, 0, I->kneg64), s));
   ConstArgs c{};
-  c.nrows = I->nv;
   c.nslots = I->P.nslots;
-  c.slice_off = I->slice_off;
-  c.row_len = I->row_len;
+  c.slot_row = I->slot_row;
   c.col = I->col;
   c.kneg = I->kneg64;
   c.mass = I->mass;
@@ -371,6 +375,14 @@ int setup_constant_parts(hobak_inst* I) {
   c.E = I->E;
   c.C = I->C;
   HIPCHK(I->ctx, launch_constant_parts<T>(c, s));
+  return HOBAK_OK;
+}
+
+// The reductions write kMaxReduceVals partials per workgroup; refuse to launch if the buffer
+// sized at creation cannot hold them.
+int check_reduce_capacity(hobak_inst* I) {
+  if ((size_t)kMaxReduceVals * reduction_grid_max(I->nv) > I->partial_capacity)
+    return fail(I->ctx, HOBAK_E_INVALID, "reduction buffer smaller than the launch grid");
   return HOBAK_OK;
 }
 
@@ -427,6 +439,7 @@ int step_impl(hobak_inst* I, hobak_step_stats* st) {
   const int bdf = N > 1;
   const double invDt = 1.0 / I->prm.dt;
   double ms[5] = {0, 0, 0, 0, 0};
+  TRY(check_reduce_capacity(I));
   HIPCHK(ctx, launch_begin_step(I->nv, I->u, I->v, I->u_old, I->v_old, I->pinned, I->pin_target, s));
   int done = 0;
   double maxR = 10 * I->prm.newton_tol;
@@ -676,6 +689,7 @@ int hobak_inst_create(hobak_ctx* ctx, const double* rest_xyz, int32_t nv, const 
   ALLOC(slice_off, P.slice_off.size());
   ALLOC(row_len, (size_t)nv);
   ALLOC(col, (size_t)P.nslots);
+  ALLOC(slot_row, (size_t)P.nslots);
   ALLOC(cptr, P.cptr.size());
   ALLOC(clist, P.clist.size());
   ALLOC(vptr, P.vptr.size());
@@ -691,11 +705,12 @@ int hobak_inst_create(hobak_ctx* ctx, const double* rest_xyz, int32_t nv, const 
   ALLOCT(r, 3 * np);
   ALLOCT(x, 3 * np);
   ALLOCT(z, 3 * np);
-  ALLOCT(p, 3 * np);
+  ALLOCT(pbuf0, 3 * np);
+  ALLOCT(pbuf1, 3 * np);
   ALLOCT(t, 3 * np);
   ALLOC(sc, 1);
-  const int maxGrid = (nv + kBlock - 1) / kBlock + 1;
-  ALLOC(partial, 4 * (size_t)maxGrid);
+  I->partial_capacity = (size_t)kMaxReduceVals * reduction_grid_max(nv);
+  ALLOC(partial, I->partial_capacity);
   ALLOC(counters, 16);
 #undef ALLOC
 #undef ALLOCT
@@ -717,6 +732,7 @@ int hobak_inst_create(hobak_ctx* ctx, const double* rest_xyz, int32_t nv, const 
       (rc = upload(I, I->slice_off, P.slice_off.data(), P.slice_off.size())) ||
       (rc = upload(I, I->row_len, P.row_len.data(), P.row_len.size())) ||
       (rc = upload(I, I->col, P.col.data(), P.col.size())) ||
+      (rc = upload(I, I->slot_row, P.slot_row.data(), P.slot_row.size())) ||
       (rc = upload(I, I->cptr, P.cptr.data(), P.cptr.size())) ||
       (rc = upload(I, I->clist, P.clist.data(), P.clist.size())) ||
       (rc = upload(I, I->vptr, P.vptr.data(), P.vptr.size())) ||

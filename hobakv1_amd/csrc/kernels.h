@@ -6,11 +6,16 @@
 //   DmInv          9 planes of nt doubles (plane 3*r + c holds DmInv(r, c))       -> coalesced
 //   element stage  90 planes of nt (10 upper 3x3 blocks of the 12x12 stiffness) + 12 planes of
 //                  forces; plane index (block * 9 + 3 * a + b)
-//   block matrix   SELL-64: rows (vertices) in slices of 64; slice s owns slots
-//                  [slice_off[s], slice_off[s+1]), slot = slice_off[s] + k * 64 + (row % 64);
-//                  9 value planes of nslots (plane 3a+b = entry (a, b) of the 3x3 block) and one
-//                  int32 column (vertex) per slot.  Padding slots point at their own row and hold
-//                  zeros.  Columns of a row are ascending in k.
+//   block matrix   SELL-8, k-interleaved: rows (vertices) in slices of 8; slice s owns slots
+//                  [slice_off[s], slice_off[s+1]) (a multiple of 64), and the k-th block of row r
+//                  sits at slot = slice_off[r / 8] + 8 * k + (r % 8).  A wavefront that gives
+//                  each of a slice's 8 rows 8 lanes (lane = 8 * (k % 8) + r % 8) therefore reads
+//                  slot_off + 64 * (k / 8) + lane: contiguous 512-byte FP64 wave loads.
+//                  9 value planes of nslots (plane 3a+b = entry (a, b) of the 3x3 block), one
+//                  int32 column (vertex) and one int32 owning row (-1 for padding) per slot.
+//                  Columns of a row are ascending in k.
+//   preconditioner per row: 9 contiguous values (block Jacobi inverse, row-major) or per DOF
+//                  (Jacobi 1/a_ii)
 //   scatter map    per slot, the ascending-tet list of (tet, staged block, transpose) codes that
 //                  sum into it: code = tet * 32 + transpose * 16 + block
 #pragma once
@@ -20,7 +25,8 @@
 
 namespace hobak {
 
-constexpr int kSlice = 64;
+constexpr int kSlice = 8;     // rows per SELL slice
+constexpr int kLanesPerRow = 8;
 constexpr int kBlock = 256;
 constexpr int kStageBlocks = 10;
 constexpr int kStagePlanes = 90;
@@ -34,7 +40,7 @@ struct CgScalars {
   double beta;
   double resNorm2;
   double maxR;        // |S b|^2, the BDF_1 Newton test value
-  int iters;
+  int iters;         // also selects the p double buffer: iteration i reads p[i & 1]
   int maxIters;
   int done;           // 1 once converged / exhausted / failed
   int status;         // 0 ok, 1 not converged, 2 non-finite

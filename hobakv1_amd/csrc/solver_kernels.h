@@ -4,10 +4,12 @@
 
 namespace hobak {
 
+// Row-major 3x3 block-Jacobi inverse per row (9 values) or Jacobi 1/a_ii per DOF.
+constexpr int kMinvStride = 9;
+
 struct AsmArgs {
-  int nrows, nrows_pad, nslots, nt;
-  const int* slice_off;
-  const int* row_len;
+  int nrows, nslots, nt;
+  const int* slot_row;   // owning row of each slot, -1 for padding
   const int* col;
   const int* cptr;       // nslots + 1
   const int* clist;
@@ -25,9 +27,8 @@ struct AsmArgs {
 };
 
 struct ConstArgs {
-  int nrows, nslots;
-  const int* slice_off;
-  const int* row_len;
+  int nslots;
+  const int* slot_row;
   const int* col;
   const double* kneg;    // assembled -K_rest, 9 planes
   const double* mass;    // per DOF
@@ -38,7 +39,7 @@ struct ConstArgs {
 };
 
 struct RhsArgs {
-  int nrows, nrows_pad, nslots, nt;
+  int nrows, nslots, nt;
   const int* slice_off;
   const int* row_len;
   const int* col;
@@ -64,7 +65,8 @@ struct RhsArgs {
   void* r;
   void* x;
   void* z;
-  void* p;
+  void* pbuf0;
+  void* pbuf1;
   const void* minv;
   int precond;
   double tol;
@@ -74,7 +76,7 @@ struct RhsArgs {
 };
 
 struct CgArgs {
-  int nrows, nrows_pad, nslots;
+  int nrows, nslots;
   const int* slice_off;
   const int* row_len;
   const int* col;
@@ -82,7 +84,8 @@ struct CgArgs {
   void* x;
   void* r;
   void* z;
-  void* p;
+  void* pbuf0;   // p double buffer: iteration i reads pbuf[i & 1] and writes pbuf[(i + 1) & 1]
+  void* pbuf1;
   void* t;
   const void* minv;
   int precond;
@@ -108,12 +111,13 @@ template <typename T>
 hipError_t launch_cg_spmv(const CgArgs& a, hipStream_t s);
 template <typename T>
 hipError_t launch_cg_rest(const CgArgs& a, hipStream_t s);
-hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
-
-}  // namespace hobak
-
-namespace hobak {
 template <typename T>
 hipError_t launch_force_gather(int nv, int nt, const int* vptr, const int* vlist, const T* stage_f,
                                double* R, hipStream_t s);
+hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
+// Largest grid (in workgroups) of any kernel that runs a grid_reduce over `nrows` rows; the
+// partial-sum buffer of a reduction needs kMaxReduceVals * reduction_grid_max(nrows) doubles.
+int reduction_grid_max(int nrows);
+constexpr int kMaxReduceVals = 2;
+
 }  // namespace hobak

@@ -28,7 +28,8 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 // Block-reduce NV values, publish the partials, and return true in the last block to arrive,
-// with the grid totals in `tot` (valid in every lane of that block).
+// with the grid totals in `tot` (valid in every lane of that block).  Every thread of every block
+// must call it.
 template <int NV>
 __device__ bool grid_reduce(double (&vals)[NV], const Reduce& R, double (&tot)[NV]) {
   __shared__ double sh[NV][kBlock / 64];
@@ -80,9 +81,9 @@ __device__ bool grid_reduce(double (&vals)[NV], const Reduce& R, double (&tot)[N
   return true;
 }
 
-// First slot of row r; its k-th block sits at slot_base + k * kSlice.
-__device__ __forceinline__ int slot_base(const int* slice_off, int r) {
-  return slice_off[r >> 6] + (r & 63);
+// Slot of the k-th block of row r.
+__device__ __forceinline__ int slot_of(const int* slice_off, int r, int k) {
+  return slice_off[r >> 3] + (k << 3) + (r & 7);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -126,13 +127,12 @@ __global__ void end_step_kernel(int nv, double invDt, const double* u, const dou
 // assembly
 // ---------------------------------------------------------------------------------------------
 template <typename T>
-__device__ __forceinline__ void write_precond(const T (&a)[9], T* minv, int nrows_pad, int r,
-                                              int precond) {
+__device__ __forceinline__ void write_precond(const T (&a)[9], T* minv, int r, int precond) {
   if (precond == 0) {  // DiagonalPreconditioner: 1/a_ii, 1 where a_ii == 0
 #pragma unroll
     for (int c = 0; c < 3; c++) {
       const T d = a[4 * c];
-      minv[c * (size_t)nrows_pad + r] = (d != T(0)) ? T(1) / d : T(1);
+      minv[3 * (size_t)r + c] = (d != T(0)) ? T(1) / d : T(1);
     }
     return;
   }
@@ -144,72 +144,64 @@ __device__ __forceinline__ void write_precond(const T (&a)[9], T* minv, int nrow
   const T c0 = cof(0, 0), c1 = cof(1, 0), c2 = cof(2, 0);
   const T det = (c0 * a[0] + c1 * a[3]) + c2 * a[6];
   const T invdet = T(1) / det;
-  minv[0 * (size_t)nrows_pad + r] = c0 * invdet;
-  minv[1 * (size_t)nrows_pad + r] = c1 * invdet;
-  minv[2 * (size_t)nrows_pad + r] = c2 * invdet;
+  T* m = minv + (size_t)kMinvStride * r;
+  m[0] = c0 * invdet;
+  m[1] = c1 * invdet;
+  m[2] = c2 * invdet;
 #pragma unroll
   for (int row = 1; row < 3; row++)
 #pragma unroll
-    for (int k = 0; k < 3; k++) minv[(3 * row + k) * (size_t)nrows_pad + r] = cof(k, row) * invdet;
+    for (int k = 0; k < 3; k++) m[3 * row + k] = cof(k, row) * invdet;
 }
 
-// One lane per block row.  mode 0: out = sum of staged contributions (the stiffness, -K).
-// mode 1: out = LHS = S (D - K) S + (I - S) with the preconditioner of its diagonal block.
+// One lane per block slot.  mode 0: out = sum of staged contributions (the stiffness, -K).
+// mode 1: out = LHS = S (D - K) S + (I - S); the diagonal slot also writes its preconditioner.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void assemble_kernel(AsmArgs a) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= a.nrows) return;
-  const int width = a.row_len[r];
-  const int base = slot_base(a.slice_off, r);
+  const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+  if (slot >= a.nslots) return;
+  const int r = a.slot_row[slot];
+  if (r < 0) return;
+  const int c = a.col[slot];
   const T* stage = (const T*)a.stage_h;
-  const T* D = (const T*)a.D;
-  T* out = (T*)a.out;
   const size_t nt = (size_t)a.nt;
   const size_t ns = (size_t)a.nslots;
-  const bool pr = a.mode == 1 && a.pinned[r];
-  for (int k = 0; k < width; k++) {
-    const int slot = base + k * kSlice;
-    const int c = a.col[slot];
-    T acc[9];
+  T acc[9];
 #pragma unroll
-    for (int e = 0; e < 9; e++) acc[e] = T(0);
-    const int q1 = a.cptr[slot + 1];
-    for (int q = a.cptr[slot]; q < q1; q++) {
-      const int code = a.clist[q];
-      const size_t tet = (size_t)(code >> 5);
-      const int lb = code & 15;
-      const T* src = stage + (size_t)lb * 9 * nt + tet;
-      if (code & 16) {
+  for (int e = 0; e < 9; e++) acc[e] = T(0);
+  const int q1 = a.cptr[slot + 1];
+  for (int q = a.cptr[slot]; q < q1; q++) {
+    const int code = a.clist[q];
+    const T* src = stage + (size_t)(code & 15) * 9 * nt + (size_t)(code >> 5);
+    if (code & 16) {
 #pragma unroll
-        for (int i = 0; i < 3; i++)
+      for (int i = 0; i < 3; i++)
 #pragma unroll
-          for (int j = 0; j < 3; j++) acc[3 * i + j] += src[(3 * j + i) * nt];
-      } else {
+        for (int j = 0; j < 3; j++) acc[3 * i + j] += src[(3 * j + i) * nt];
+    } else {
 #pragma unroll
-        for (int e = 0; e < 9; e++) acc[e] += src[e * nt];
-      }
+      for (int e = 0; e < 9; e++) acc[e] += src[e * nt];
     }
-    if (a.mode == 0) {
-#pragma unroll
-      for (int e = 0; e < 9; e++) out[e * ns + slot] = acc[e];
-      continue;
-    }
-    const bool pin = pr || a.pinned[c];
-    if (a.ground_on && c == r && !pin) {
-      const double y = a.rest[3 * (size_t)r + 1] + a.u[3 * (size_t)r + 1];
-      if (y < a.ground_y) acc[4] += (T)a.ground_k;
-    }
-    T val[9];
-#pragma unroll
-    for (int e = 0; e < 9; e++) {
-      if (pin)
-        val[e] = (c == r && (e % 4) == 0) ? T(1) : T(0);
-      else
-        val[e] = D[e * ns + slot] + acc[e];
-      out[e * ns + slot] = val[e];
-    }
-    if (c == r) write_precond<T>(val, (T*)a.minv, a.nrows_pad, r, a.precond);
   }
+  T* out = (T*)a.out;
+  if (a.mode == 0) {
+#pragma unroll
+    for (int e = 0; e < 9; e++) out[e * ns + slot] = acc[e];
+    return;
+  }
+  const bool pin = a.pinned[r] || a.pinned[c];
+  if (a.ground_on && c == r && !pin) {
+    const double y = a.rest[3 * (size_t)r + 1] + a.u[3 * (size_t)r + 1];
+    if (y < a.ground_y) acc[4] += (T)a.ground_k;
+  }
+  const T* D = (const T*)a.D;
+  T val[9];
+#pragma unroll
+  for (int e = 0; e < 9; e++) {
+    val[e] = pin ? ((c == r && (e % 4) == 0) ? T(1) : T(0)) : D[e * ns + slot] + acc[e];
+    out[e * ns + slot] = val[e];
+  }
+  if (c == r) write_precond<T>(val, (T*)a.minv, r, a.precond);
 }
 
 // D = M/dt^2 - C/dt, E = M/dt - C, C = alpha M + beta K_rest, from the assembled rest stiffness
@@ -217,40 +209,46 @@ __global__ __launch_bounds__(kBlock) void assemble_kernel(AsmArgs a) {
 // TIMESTEPPER.cpp:232-233.
 template <typename T>
 __global__ void constant_parts_kernel(ConstArgs a) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= a.nrows) return;
-  const int width = a.row_len[r];
-  const int base = slot_base(a.slice_off, r);
+  const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+  if (slot >= a.nslots) return;
+  const int r = a.slot_row[slot];
+  if (r < 0) return;
+  const int c = a.col[slot];
   const size_t ns = (size_t)a.nslots;
-  for (int k = 0; k < width; k++) {
-    const int slot = base + k * kSlice;
-    const int c = a.col[slot];
-    const bool real = true;
 #pragma unroll
-    for (int e = 0; e < 9; e++) {
-      const double krest = -a.kneg[e * ns + slot];
-      const bool dg = real && (c == r) && (e % 4 == 0);
-      const double m = dg ? a.mass[3 * (size_t)r + e / 4] : 0.0;
-      const double cc = dg ? a.ra * m + a.rb * krest : a.rb * krest;
-      double d = dg ? m * a.invDt2 - cc * a.invDt : -(cc * a.invDt);
-      double ee = dg ? a.invDt * m - cc : -cc;
-      if (!real) {
-        d = 0.0;
-        ee = 0.0;
-      }
-      ((T*)a.D)[e * ns + slot] = (T)d;
-      ((T*)a.E)[e * ns + slot] = (T)ee;
-      ((T*)a.C)[e * ns + slot] = (T)(real ? cc : 0.0);
-    }
+  for (int e = 0; e < 9; e++) {
+    const double krest = -a.kneg[e * ns + slot];
+    const bool dg = (c == r) && (e % 4 == 0);
+    const double m = dg ? a.mass[3 * (size_t)r + e / 4] : 0.0;
+    const double cc = dg ? a.ra * m + a.rb * krest : a.rb * krest;
+    const double d = dg ? m * a.invDt2 - cc * a.invDt : -(cc * a.invDt);
+    const double ee = dg ? a.invDt * m - cc : -cc;
+    ((T*)a.D)[e * ns + slot] = (T)d;
+    ((T*)a.E)[e * ns + slot] = (T)ee;
+    ((T*)a.C)[e * ns + slot] = (T)cc;
   }
 }
 
 // ---------------------------------------------------------------------------------------------
 // right-hand side + PCG initialisation
 // ---------------------------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ void apply_precond(const T* minv, int precond, int r, const T (&rv)[3],
+                                              T (&zv)[3]) {
+  if (precond == 0) {
+#pragma unroll
+    for (int c = 0; c < 3; c++) zv[c] = minv[3 * (size_t)r + c] * rv[c];
+  } else {
+    const T* m = minv + (size_t)kMinvStride * r;
+#pragma unroll
+    for (int i = 0; i < 3; i++) zv[i] = m[3 * i] * rv[0] + m[3 * i + 1] * rv[1] + m[3 * i + 2] * rv[2];
+  }
+}
+
 // Per block row: R = ascending-tet gather of staged forces; b = (E v + R) + f_ext (BEP) or
-// b = C vb + ((M a + R) + f_ext) (BDF_1); rhs = S b.  Then x = 0, r = rhs, z = M^-1 r, p = z and
-// the grid sums |rhs|^2 and r.z seed Eigen's CG (ConjugateGradient.h:43-66).
+// b = C vb + ((M a + R) + f_ext) (BDF_1); rhs = S b.  Then x = 0, r = rhs, z = M^-1 r, p = 0 with
+// beta = 0 (so the first SpMV forms p = z), and the grid sums |rhs|^2 and r.z seed Eigen's CG
+// (ConjugateGradient.h:43-66).
 template <typename T>
 __global__ __launch_bounds__(kBlock) void rhs_init_kernel(RhsArgs a) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
@@ -270,13 +268,12 @@ __global__ __launch_bounds__(kBlock) void rhs_init_kernel(RhsArgs a) {
       const double y = a.rest[3 * (size_t)r + 1] + a.u[3 * (size_t)r + 1];
       if (y < a.ground_y) R[1] += a.ground_k * (a.ground_y - y);
     }
-    const int width = a.row_len[r];
-    const int base = slot_base(a.slice_off, r);
     const size_t ns = (size_t)a.nslots;
     const T* Mx = (const T*)a.Mvec;  // E (BEP) or C (BDF_1)
     double s[3] = {0.0, 0.0, 0.0};
-    for (int k = 0; k < width; k++) {
-      const int slot = base + k * kSlice;
+    const int len = a.row_len[r];
+    for (int k = 0; k < len; k++) {
+      const int slot = slot_of(a.slice_off, r, k);
       const int c = a.col[slot];
       double xv[3];
 #pragma unroll
@@ -308,32 +305,17 @@ __global__ __launch_bounds__(kBlock) void rhs_init_kernel(RhsArgs a) {
       part[0] += b[c] * b[c];
     }
     if (a.init_cg) {
-      T* rr = (T*)a.r;
-      T* xx = (T*)a.x;
-      T* zz = (T*)a.z;
-      T* pp = (T*)a.p;
-      const T* mi = (const T*)a.minv;
-      const size_t np = (size_t)a.nrows_pad;
-      T rv[3];
+      T rv[3], zv[3];
 #pragma unroll
       for (int c = 0; c < 3; c++) rv[c] = (T)b[c];
-      T zv[3];
-      if (a.precond == 0) {
-#pragma unroll
-        for (int c = 0; c < 3; c++) zv[c] = mi[c * np + r] * rv[c];
-      } else {
-#pragma unroll
-        for (int i = 0; i < 3; i++)
-          zv[i] = mi[(3 * i) * np + r] * rv[0] + mi[(3 * i + 1) * np + r] * rv[1] +
-                  mi[(3 * i + 2) * np + r] * rv[2];
-      }
+      apply_precond<T>((const T*)a.minv, a.precond, r, rv, zv);
 #pragma unroll
       for (int c = 0; c < 3; c++) {
         const size_t i = 3 * (size_t)r + c;
-        rr[i] = rv[c];
-        xx[i] = T(0);
-        zz[i] = zv[c];
-        pp[i] = zv[c];
+        ((T*)a.r)[i] = rv[c];
+        ((T*)a.x)[i] = T(0);
+        ((T*)a.z)[i] = zv[c];
+        ((T*)a.pbuf0)[i] = T(0);
         part[1] += (double)rv[c] * (double)zv[c];
       }
     }
@@ -363,90 +345,116 @@ __global__ __launch_bounds__(kBlock) void rhs_init_kernel(RhsArgs a) {
           sc->status = 2;
         }
       }
+      if (a.max_iters <= 0) sc->done = 1;
       sc->absNew = tot[1];
     }
   }
 }
 
 // ---------------------------------------------------------------------------------------------
-// PCG iteration: three kernels
-//   spmv_dot:  t = A p,  pt = p.t,  alpha = absNew / pt
-//   update:    x += alpha p, r -= alpha t, |r|^2 test, z = M^-1 r, absNew = r.z, beta
-//   p_update:  p = z + beta p
+// PCG iteration: two kernels
+//   spmv:    p_new = z + beta p_old (formed on the fly at every column, written for own rows),
+//            t = A p_new, alpha = absNew / (p_new . t)
+//   update:  x += alpha p_new, r -= alpha t, |r|^2 test, z = M^-1 r, absNew = r.z, beta
 // Each kernel returns at once when the solve is finished, so a fixed chunk of iterations can be
-// queued (or replayed from a graph) and polled from the host between chunks.
+// queued and polled from the host between chunks.
 // ---------------------------------------------------------------------------------------------
+// One wavefront per 8-row slice, 8 lanes per row: lane = 8 * kk + rr handles the row's blocks
+// k = kk, kk + 8, ... and the 8 partial sums of a row are combined with cross-lane adds.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void cg_spmv_kernel(CgArgs a) {
   if (a.sc->done) return;
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  double part[1] = {0.0};
-  const T* p = (const T*)a.p;
-  if (r < a.nrows) {
-    const int width = a.row_len[r];
-    const int base = slot_base(a.slice_off, r);
-    const size_t ns = (size_t)a.nslots;
-    const T* A = (const T*)a.A;
-    T s0 = T(0), s1 = T(0), s2 = T(0);
-    for (int k = 0; k < width; k++) {
-      const int slot = base + k * kSlice;
+  const int slice = (int)((blockIdx.x * (unsigned)blockDim.x + threadIdx.x) >> 6);
+  const int lane = threadIdx.x & 63;
+  const int kk = lane >> 3;
+  const int r = slice * kSlice + (lane & 7);
+  const int parity = a.sc->iters & 1;
+  const T* pold = (const T*)(parity ? a.pbuf1 : a.pbuf0);
+  T* pnew = (T*)(parity ? a.pbuf0 : a.pbuf1);
+  const T* z = (const T*)a.z;
+  const T beta = (T)a.sc->beta;
+  const T* A = (const T*)a.A;
+  const size_t ns = (size_t)a.nslots;
+  T s0 = T(0), s1 = T(0), s2 = T(0);
+  const bool live = r < a.nrows;
+  if (live) {
+    const int len = a.row_len[r];
+    const int base = a.slice_off[slice] + (lane & 7);
+    for (int k = kk; k < len; k += kLanesPerRow) {
+      const int slot = base + (k << 3);
       const size_t c = 3 * (size_t)a.col[slot];
-      const T p0 = p[c], p1 = p[c + 1], p2 = p[c + 2];
+      const T p0 = z[c] + beta * pold[c];
+      const T p1 = z[c + 1] + beta * pold[c + 1];
+      const T p2 = z[c + 2] + beta * pold[c + 2];
       s0 += A[0 * ns + slot] * p0 + A[1 * ns + slot] * p1 + A[2 * ns + slot] * p2;
       s1 += A[3 * ns + slot] * p0 + A[4 * ns + slot] * p1 + A[5 * ns + slot] * p2;
       s2 += A[6 * ns + slot] * p0 + A[7 * ns + slot] * p1 + A[8 * ns + slot] * p2;
     }
-    T* t = (T*)a.t;
+  }
+#pragma unroll
+  for (int off = 8; off < 64; off <<= 1) {
+    s0 += __shfl_xor(s0, off, 64);
+    s1 += __shfl_xor(s1, off, 64);
+    s2 += __shfl_xor(s2, off, 64);
+  }
+  double part[1] = {0.0};
+  if (live && kk == 0) {
     const size_t i = 3 * (size_t)r;
+    T* t = (T*)a.t;
     t[i] = s0;
     t[i + 1] = s1;
     t[i + 2] = s2;
-    part[0] = (double)p[i] * (double)s0 + (double)p[i + 1] * (double)s1 + (double)p[i + 2] * (double)s2;
+    const T q0 = z[i] + beta * pold[i];
+    const T q1 = z[i + 1] + beta * pold[i + 1];
+    const T q2 = z[i + 2] + beta * pold[i + 2];
+    pnew[i] = q0;
+    pnew[i + 1] = q1;
+    pnew[i + 2] = q2;
+    part[0] = (double)q0 * (double)s0 + (double)q1 * (double)s1 + (double)q2 * (double)s2;
   }
   double tot[1];
-  if (grid_reduce<1>(part, a.red, tot) && threadIdx.x == 0) {
-    a.sc->alpha = a.sc->absNew / tot[0];
-  }
+  if (grid_reduce<1>(part, a.red, tot) && threadIdx.x == 0) a.sc->alpha = a.sc->absNew / tot[0];
 }
 
+// One lane per DOF, 21 rows (63 lanes) per wavefront so that a row's three DOFs share a wave and
+// the 3x3 preconditioner apply can read its siblings' residuals with cross-lane moves.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void cg_update_kernel(CgArgs a) {
   if (a.sc->done) return;
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  double part[2] = {0.0, 0.0};
-  if (r < a.nrows) {
-    const T alpha = (T)a.sc->alpha;
+  const int wave = (int)((blockIdx.x * (unsigned)blockDim.x + threadIdx.x) >> 6);
+  const int lane = threadIdx.x & 63;
+  const int row = wave * 21 + lane / 3;
+  const int c = lane - 3 * (lane / 3);
+  const bool live = lane < 63 && row < a.nrows;
+  const int parity = a.sc->iters & 1;
+  const T* p = (const T*)(parity ? a.pbuf0 : a.pbuf1);  // the p_new this iteration's SpMV wrote
+  const T alpha = (T)a.sc->alpha;
+  const size_t i = 3 * (size_t)row + c;
+  T rv = T(0);
+  if (live) {
     T* x = (T*)a.x;
     T* rr = (T*)a.r;
-    T* z = (T*)a.z;
-    const T* p = (const T*)a.p;
-    const T* t = (const T*)a.t;
+    x[i] += alpha * p[i];
+    rv = rr[i] - alpha * ((const T*)a.t)[i];
+    rr[i] = rv;
+  }
+  const int b = lane - c;
+  const T r0 = __shfl(rv, b, 64);
+  const T r1 = __shfl(rv, b + 1, 64);
+  const T r2 = __shfl(rv, b + 2, 64);
+  double part[2] = {0.0, 0.0};
+  if (live) {
     const T* mi = (const T*)a.minv;
-    const size_t np = (size_t)a.nrows_pad;
-    T rv[3];
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-      const size_t i = 3 * (size_t)r + c;
-      x[i] += alpha * p[i];
-      rv[c] = rr[i] - alpha * t[i];
-      rr[i] = rv[c];
-      part[0] += (double)rv[c] * (double)rv[c];
-    }
-    T zv[3];
+    T zv;
     if (a.precond == 0) {
-#pragma unroll
-      for (int c = 0; c < 3; c++) zv[c] = mi[c * np + r] * rv[c];
+      zv = mi[i] * rv;
     } else {
-#pragma unroll
-      for (int i = 0; i < 3; i++)
-        zv[i] = mi[(3 * i) * np + r] * rv[0] + mi[(3 * i + 1) * np + r] * rv[1] +
-                mi[(3 * i + 2) * np + r] * rv[2];
+      const T* m = mi + (size_t)kMinvStride * row + 3 * c;
+      zv = m[0] * r0 + m[1] * r1 + m[2] * r2;
     }
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-      z[3 * (size_t)r + c] = zv[c];
-      part[1] += (double)rv[c] * (double)zv[c];
-    }
+    ((T*)a.z)[i] = zv;
+    part[0] = (double)rv * (double)rv;
+    part[1] = (double)rv * (double)zv;
   }
   double tot[2];
   if (grid_reduce<2>(part, a.red, tot) && threadIdx.x == 0) {
@@ -470,20 +478,35 @@ __global__ __launch_bounds__(kBlock) void cg_update_kernel(CgArgs a) {
   }
 }
 
+// computeHyperelasticForces' scatter as a per-vertex gather in ascending tet order
+// (TET_MESH.cpp:756-767); output in FP64.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void cg_p_kernel(CgArgs a) {
-  if (a.sc->done) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= 3 * a.nrows) return;
-  const T beta = (T)a.sc->beta;
-  T* p = (T*)a.p;
-  p[i] = ((const T*)a.z)[i] + beta * p[i];
+__global__ void force_gather_kernel(int nv, int nt, const int* vptr, const int* vlist,
+                                    const T* stage_f, double* R) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nv) return;
+  double s[3] = {0.0, 0.0, 0.0};
+  for (int q = vptr[r]; q < vptr[r + 1]; q++) {
+    const int code = vlist[q];
+    const size_t tet = (size_t)(code >> 2);
+    const int loc = code & 3;
+#pragma unroll
+    for (int c = 0; c < 3; c++) s[c] += (double)stage_f[(3 * loc + c) * (size_t)nt + tet];
+  }
+#pragma unroll
+  for (int c = 0; c < 3; c++) R[3 * (size_t)r + c] = s[c];
+}
+
+// Plain streaming copy, used only to calibrate hardware byte counters against a known size.
+__global__ void copy_kernel(const double4* __restrict__ src, double4* __restrict__ dst, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
 }
 
 // ---------------------------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------------------------
-static inline int grid_for(int n, int block = kBlock) { return (n + block - 1) / block; }
+static inline int grid_for(long n, int block = kBlock) { return (int)((n + block - 1) / block); }
 
 hipError_t launch_begin_step(int nv, double* u, const double* v, double* uOld, double* vOld,
                              const unsigned char* pinned, const double* target, hipStream_t s) {
@@ -507,13 +530,13 @@ hipError_t launch_end_step(int nv, double invDt, const double* u, const double* 
 
 template <typename T>
 hipError_t launch_assemble(const AsmArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL((assemble_kernel<T>), dim3(grid_for(a.nrows)), dim3(kBlock), 0, s, a);
+  hipLaunchKernelGGL((assemble_kernel<T>), dim3(grid_for(a.nslots)), dim3(kBlock), 0, s, a);
   return hipGetLastError();
 }
 
 template <typename T>
 hipError_t launch_constant_parts(const ConstArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL((constant_parts_kernel<T>), dim3(grid_for(a.nrows)), dim3(kBlock), 0, s, a);
+  hipLaunchKernelGGL((constant_parts_kernel<T>), dim3(grid_for(a.nslots)), dim3(kBlock), 0, s, a);
   return hipGetLastError();
 }
 
@@ -523,16 +546,26 @@ hipError_t launch_rhs_init(const RhsArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
+// grid sizes: SpMV one wave per 8-row slice; update one wave per 21 rows; rhs one lane per row
+static inline int spmv_grid(int nrows) { return grid_for((long)((nrows + kSlice - 1) / kSlice) * 64); }
+static inline int update_grid(int nrows) { return grid_for((long)((nrows + 20) / 21) * 64); }
+
+int reduction_grid_max(int nrows) {
+  int g = grid_for(nrows);
+  if (spmv_grid(nrows) > g) g = spmv_grid(nrows);
+  if (update_grid(nrows) > g) g = update_grid(nrows);
+  return g;
+}
+
 template <typename T>
 hipError_t launch_cg_spmv(const CgArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL((cg_spmv_kernel<T>), dim3(grid_for(a.nrows)), dim3(kBlock), 0, s, a);
+  hipLaunchKernelGGL((cg_spmv_kernel<T>), dim3(spmv_grid(a.nrows)), dim3(kBlock), 0, s, a);
   return hipGetLastError();
 }
 
 template <typename T>
 hipError_t launch_cg_rest(const CgArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL((cg_update_kernel<T>), dim3(grid_for(a.nrows)), dim3(kBlock), 0, s, a);
-  hipLaunchKernelGGL((cg_p_kernel<T>), dim3(grid_for(3 * a.nrows)), dim3(kBlock), 0, s, a);
+  hipLaunchKernelGGL((cg_update_kernel<T>), dim3(update_grid(a.nrows)), dim3(kBlock), 0, s, a);
   return hipGetLastError();
 }
 
@@ -543,10 +576,12 @@ hipError_t launch_cg_iteration(const CgArgs& a, hipStream_t s) {
   return launch_cg_rest<T>(a, s);
 }
 
-// Plain streaming copy, used only to calibrate hardware byte counters against a known size.
-__global__ void copy_kernel(const double4* __restrict__ src, double4* __restrict__ dst, size_t n) {
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-    dst[i] = src[i];
+template <typename T>
+hipError_t launch_force_gather(int nv, int nt, const int* vptr, const int* vlist, const T* stage_f,
+                               double* R, hipStream_t s) {
+  hipLaunchKernelGGL((force_gather_kernel<T>), dim3(grid_for(nv)), dim3(kBlock), 0, s, nv, nt, vptr,
+                     vlist, stage_f, R);
+  return hipGetLastError();
 }
 
 hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s) {
@@ -555,54 +590,18 @@ hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s) 
   return hipGetLastError();
 }
 
-template hipError_t launch_apply_dx<double>(int, double*, const double*, hipStream_t);
-template hipError_t launch_apply_dx<float>(int, double*, const float*, hipStream_t);
-template hipError_t launch_assemble<double>(const AsmArgs&, hipStream_t);
-template hipError_t launch_assemble<float>(const AsmArgs&, hipStream_t);
-template hipError_t launch_constant_parts<double>(const ConstArgs&, hipStream_t);
-template hipError_t launch_constant_parts<float>(const ConstArgs&, hipStream_t);
-template hipError_t launch_rhs_init<double>(const RhsArgs&, hipStream_t);
-template hipError_t launch_rhs_init<float>(const RhsArgs&, hipStream_t);
-template hipError_t launch_cg_iteration<double>(const CgArgs&, hipStream_t);
-template hipError_t launch_cg_iteration<float>(const CgArgs&, hipStream_t);
-template hipError_t launch_cg_spmv<double>(const CgArgs&, hipStream_t);
-template hipError_t launch_cg_spmv<float>(const CgArgs&, hipStream_t);
-template hipError_t launch_cg_rest<double>(const CgArgs&, hipStream_t);
-template hipError_t launch_cg_rest<float>(const CgArgs&, hipStream_t);
-
-}  // namespace hobak
-
-namespace hobak {
-
-// computeHyperelasticForces' scatter as a per-vertex gather in ascending tet order
-// (TET_MESH.cpp:756-767); output in FP64.
-template <typename T>
-__global__ void force_gather_kernel(int nv, int nt, const int* vptr, const int* vlist,
-                                    const T* stage_f, double* R) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= nv) return;
-  double s[3] = {0.0, 0.0, 0.0};
-  for (int q = vptr[r]; q < vptr[r + 1]; q++) {
-    const int code = vlist[q];
-    const size_t tet = (size_t)(code >> 2);
-    const int loc = code & 3;
-#pragma unroll
-    for (int c = 0; c < 3; c++) s[c] += (double)stage_f[(3 * loc + c) * (size_t)nt + tet];
-  }
-#pragma unroll
-  for (int c = 0; c < 3; c++) R[3 * (size_t)r + c] = s[c];
-}
-
-template <typename T>
-hipError_t launch_force_gather(int nv, int nt, const int* vptr, const int* vlist, const T* stage_f,
-                               double* R, hipStream_t s) {
-  hipLaunchKernelGGL((force_gather_kernel<T>), dim3((nv + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
-                     nv, nt, vptr, vlist, stage_f, R);
-  return hipGetLastError();
-}
-template hipError_t launch_force_gather<double>(int, int, const int*, const int*, const double*,
-                                                double*, hipStream_t);
-template hipError_t launch_force_gather<float>(int, int, const int*, const int*, const float*,
-                                               double*, hipStream_t);
+#define INST(T)                                                                          \
+  template hipError_t launch_apply_dx<T>(int, double*, const T*, hipStream_t);           \
+  template hipError_t launch_assemble<T>(const AsmArgs&, hipStream_t);                   \
+  template hipError_t launch_constant_parts<T>(const ConstArgs&, hipStream_t);           \
+  template hipError_t launch_rhs_init<T>(const RhsArgs&, hipStream_t);                   \
+  template hipError_t launch_cg_iteration<T>(const CgArgs&, hipStream_t);                \
+  template hipError_t launch_cg_spmv<T>(const CgArgs&, hipStream_t);                     \
+  template hipError_t launch_cg_rest<T>(const CgArgs&, hipStream_t);                     \
+  template hipError_t launch_force_gather<T>(int, int, const int*, const int*, const T*, \
+                                             double*, hipStream_t);
+INST(double)
+INST(float)
+#undef INST
 
 }  // namespace hobak
