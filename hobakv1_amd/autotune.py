@@ -1,0 +1,89 @@
+"""Offline autotuner for the PCG kernels' launch shapes.
+
+Sweeps the block-matrix slice height (rows per SELL slice, which sets lanes per row in the SpMV)
+and how many PCG iterations are queued between host polls, on a given workload, and records per configuration the average
+step time and the sampled SpMV launch time.  The best configuration per precision is written to
+tuning.json next to this file; lib.Context applies it to every instance it creates.
+
+    python -m hobakv1_amd.autotune --n 40 --steps 4 [--precision fp64|fp32] [--write]
+"""
+from __future__ import annotations
+
+import argparse
+import itertools
+import json
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+TABLE = os.path.join(HERE, "tuning.json")
+
+
+def load_table():
+    if not os.path.exists(TABLE):
+        return {}
+    with open(TABLE) as f:
+        return json.load(f)
+
+
+def measure(ctx, scene, precond, precision, steps, warmup):
+    from . import lib
+
+    inst = lib.Instance(ctx, scene, precond=precond, precision=precision)
+    for _ in range(warmup):
+        inst.step()
+    inst.set_state(scene.initial_u(), scene.initial_v())
+    inst.set_kernel_sampling(2)
+    t0 = time.perf_counter()
+    iters = 0
+    for _ in range(steps):
+        st = inst.step()
+        iters += st["total_pcg_iters"]
+    dt = time.perf_counter() - t0
+    ks = inst.kernel_stats()
+    inst.close()
+    return {"ms_per_step": 1e3 * dt / steps, "pcg_iters": iters / steps,
+            "ms_per_pcg_iter": 1e3 * dt / max(1, iters),
+            "spmv_ms": ks["spmv_avg_ms"],
+            "spmv_GBps": ks["spmv_bytes"] / max(1e-12, ks["spmv_avg_ms"] * 1e-3) / 1e9}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--n", type=int, default=40)
+    ap.add_argument("--steps", type=int, default=4)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
+    ap.add_argument("--slice-rows", default="8,16,32,64")
+    ap.add_argument("--poll", default="8,32")
+    ap.add_argument("--write", action="store_true")
+    args = ap.parse_args()
+    sys.path.insert(0, os.path.dirname(HERE))
+    from hobakv1_amd import lib, scenes
+
+    precision = lib.PRECISION_FP64 if args.precision == "fp64" else lib.PRECISION_FP32_REFINE
+    sc = scenes.lattice_scene((args.n,) * 3, seed=2, stretch_velocity=True)
+    results = []
+    for H, poll in itertools.product([int(x) for x in args.slice_rows.split(",")],
+                                     [int(x) for x in args.poll.split(",")]):
+        ctx = lib.Context(0, apply_table=False)
+        ctx.set_tuning(slice_rows=H, poll_chunk=poll)
+        r = measure(ctx, sc, scenes.PRECOND_BLOCK_JACOBI, precision, args.steps, args.warmup)
+        ctx.close()
+        r.update({"slice_rows": H, "poll_chunk": poll})
+        results.append(r)
+        print(json.dumps(r), flush=True)
+    best = min(results, key=lambda r: r["ms_per_step"])
+    print("best", json.dumps(best), flush=True)
+    if args.write:
+        table = load_table()
+        table[args.precision] = {"slice_rows": best["slice_rows"], "poll_chunk": best["poll_chunk"],
+                                 "measured": {"workload": f"{args.n}^3 lattice", **best}}
+        with open(TABLE, "w") as f:
+            json.dump(table, f, indent=1, sort_keys=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

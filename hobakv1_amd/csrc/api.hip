@@ -31,6 +31,8 @@ using namespace hobak;
 
 struct hobak_ctx {
   int device = 0;
+  int slice_rows = hobak::kDefaultSliceRows;  // tuning, applied to instances created afterwards
+  int poll_chunk = 8;
   hipStream_t stream = nullptr;
   size_t budget = 0;
   size_t in_use = 0;
@@ -42,6 +44,7 @@ namespace {
 
 struct Pattern {
   int nv = 0, nt = 0, nslices = 0, nslots = 0, nrows_pad = 0, nnzb = 0;
+  int H = 16, shift = 4;  // rows per slice and its log2
   std::vector<int> row_ptr, col_idx;  // BSR
   std::vector<int> slice_off, row_len, col, slot_row, bsr_slot, diag_slot;
   std::vector<int> cptr, clist;       // per slot contributions
@@ -51,9 +54,14 @@ struct Pattern {
 // Staged block index of the vertex pair (x, y), x <= y: the 10 upper blocks in row order.
 inline int stage_block(int x, int y) { return x * 4 - x * (x - 1) / 2 + (y - x); }
 
-void build_pattern(int nv, int nt, const int32_t* tets, Pattern& P) {
+void build_pattern(int nv, int nt, const int32_t* tets, int H, Pattern& P) {
   P.nv = nv;
   P.nt = nt;
+  P.H = H;
+  P.shift = 0;
+  while ((1 << P.shift) < H) P.shift++;
+  const int kSlice = H;
+  const int kLanesPerRow = 64 / H;
   std::vector<std::vector<int>> nbr(nv);
   for (int t = 0; t < nt; t++)
     for (int a = 0; a < 4; a++)
@@ -69,8 +77,8 @@ void build_pattern(int nv, int nt, const int32_t* tets, Pattern& P) {
     std::vector<int>().swap(n);
   }
   P.nnzb = P.row_ptr[nv];
-  // SELL-8, k-interleaved: slice width rounded up to a multiple of kLanesPerRow so that each
-  // slice's slots form whole 64-slot wave loads
+  // SELL-H, k-interleaved: slice width rounded up to a multiple of the lanes per row so that
+  // each slice's slots form whole 64-slot wave loads
   P.nslices = (nv + kSlice - 1) / kSlice;
   P.nrows_pad = P.nslices * kSlice;
   P.slice_off.assign(P.nslices + 1, 0);
@@ -171,17 +179,19 @@ struct hobak_inst {
   unsigned char* pinned = nullptr;
   // device, real type T
   void *stage_h = nullptr, *stage_f = nullptr, *A = nullptr, *D = nullptr, *E = nullptr, *C = nullptr;
-  void *minv = nullptr, *r = nullptr, *x = nullptr, *z = nullptr, *t = nullptr;
-  void *pbuf0 = nullptr, *pbuf1 = nullptr;
-  CgScalars* sc = nullptr;
-  CgScalars* sc_host = nullptr;
-  double* partial = nullptr;
-  size_t partial_capacity = 0;  // doubles
-  unsigned* counters = nullptr;
+  void *minv = nullptr, *r = nullptr, *x = nullptr, *z = nullptr, *t = nullptr, *p = nullptr;
+  CgScalars* sc = nullptr;       // two ping-pong sets
+  CgScalars* sc_host = nullptr;  // pinned host copy of one set
+  // per-workgroup partial sums: rhs -> init, SpMV -> update, update -> p
+  double *part_rhs_a = nullptr, *part_rhs_b = nullptr, *part_pt = nullptr;
+  double *part_rr = nullptr, *part_rz = nullptr;
+  int n_rhs = 0, n_spmv = 0, n_update = 0;
+  int last_it = 0;  // iterations queued by the last solve
   hipEvent_t ev[6] = {};
   double last_ms[5] = {0, 0, 0, 0, 0};
   int prev_iters = -1;
   bool has_system = false;
+  int poll_chunk = 8;
   // sampled SpMV timing (measurement hook)
   int sample_every = 0;
   std::vector<hipEvent_t> sample_ev;   // pairs
@@ -294,6 +304,7 @@ CgArgs cg_args(hobak_inst* I) {
   CgArgs a{};
   a.nrows = I->nv;
   a.nslots = I->P.nslots;
+  a.slice_shift = I->P.shift;
   a.slice_off = I->slice_off;
   a.row_len = I->row_len;
   a.col = I->col;
@@ -301,13 +312,23 @@ CgArgs cg_args(hobak_inst* I) {
   a.x = I->x;
   a.r = I->r;
   a.z = I->z;
-  a.pbuf0 = I->pbuf0;
-  a.pbuf1 = I->pbuf1;
+  a.p = I->p;
   a.t = I->t;
   a.minv = I->minv;
   a.precond = I->prm.precond;
   a.sc = I->sc;
-  a.red = Reduce{I->partial, I->counters};
+  a.it = 0;
+  a.pt = Partials{I->part_pt, nullptr, I->n_spmv};
+  a.rz = Partials{I->part_rr, I->part_rz, I->n_update};
+  return a;
+}
+
+InitArgs init_args(hobak_inst* I) {
+  InitArgs a{};
+  a.in = Partials{I->part_rhs_a, I->part_rhs_b, I->n_rhs};
+  a.sc = I->sc;
+  a.tol = I->prm.pcg_tol > 0 ? I->prm.pcg_tol : 2.220446049250313e-16;
+  a.max_iters = I->prm.pcg_max_iters > 0 ? I->prm.pcg_max_iters : 2 * I->dofs;
   return a;
 }
 
@@ -316,6 +337,7 @@ RhsArgs rhs_args(hobak_inst* I, int bdf) {
   a.nrows = I->nv;
   a.nslots = I->P.nslots;
   a.nt = I->nt;
+  a.slice_shift = I->P.shift;
   a.slice_off = I->slice_off;
   a.row_len = I->row_len;
   a.col = I->col;
@@ -343,14 +365,10 @@ RhsArgs rhs_args(hobak_inst* I, int bdf) {
   a.r = I->r;
   a.x = I->x;
   a.z = I->z;
-  a.pbuf0 = I->pbuf0;
-  a.pbuf1 = I->pbuf1;
+  a.p = I->p;
   a.minv = I->minv;
   a.precond = I->prm.precond;
-  a.tol = I->prm.pcg_tol > 0 ? I->prm.pcg_tol : 2.220446049250313e-16;
-  a.max_iters = I->prm.pcg_max_iters > 0 ? I->prm.pcg_max_iters : 2 * I->dofs;
-  a.sc = I->sc;
-  a.red = Reduce{I->partial, I->counters + 1};
+  a.out = Partials{I->part_rhs_a, I->part_rhs_b, I->n_rhs};
   return a;
 }
 
@@ -378,11 +396,12 @@ int setup_constant_parts(hobak_inst* I) {
   return HOBAK_OK;
 }
 
-// The reductions write kMaxReduceVals partials per workgroup; refuse to launch if the buffer
-// sized at creation cannot hold them.
-int check_reduce_capacity(hobak_inst* I) {
-  if ((size_t)kMaxReduceVals * reduction_grid_max(I->nv) > I->partial_capacity)
-    return fail(I->ctx, HOBAK_E_INVALID, "reduction buffer smaller than the launch grid");
+// Each partial array holds one entry per workgroup of its producing kernel; refuse to launch if
+// the grids (functions of nv and the slice height) no longer match the arrays sized at creation.
+int check_partials(hobak_inst* I) {
+  if (rhs_grid(I->nv) != I->n_rhs || spmv_grid(I->nv, I->P.shift) != I->n_spmv ||
+      update_grid(I->nv) != I->n_update)
+    return fail(I->ctx, HOBAK_E_INVALID, "partial-sum arrays do not match the launch grids");
   return HOBAK_OK;
 }
 
@@ -390,36 +409,39 @@ int check_reduce_capacity(hobak_inst* I) {
 template <typename T>
 int run_pcg(hobak_inst* I) {
   hipStream_t s = I->ctx->stream;
-  const CgArgs a = cg_args(I);
+  CgArgs a = cg_args(I);
   const int maxIt = I->prm.pcg_max_iters > 0 ? I->prm.pcg_max_iters : 2 * I->dofs;
   int launched = 0;
-  int chunk = I->prev_iters > 8 ? I->prev_iters - 4 : 8;
+  int chunk = I->prev_iters > I->poll_chunk ? I->prev_iters - I->poll_chunk / 2 : I->poll_chunk;
   // sampled launches: (iteration index, event pair slot)
   std::vector<std::pair<int, int>> samples;
   const int max_samples = (int)I->sample_ev.size() / 2;
   while (true) {
     for (int k = 0; k < chunk; k++) {
-      const int it = launched + k;
-      const bool sample = I->sample_every > 0 && (it % I->sample_every) == 0 &&
+      a.it = launched + k;
+      const bool sample = I->sample_every > 0 && (a.it % I->sample_every) == 0 &&
                           (int)samples.size() < max_samples;
       if (sample) {
         const int slot = (int)samples.size();
         HIPCHK(I->ctx, hipEventRecord(I->sample_ev[2 * slot], s));
         HIPCHK(I->ctx, launch_cg_spmv<T>(a, s));
         HIPCHK(I->ctx, hipEventRecord(I->sample_ev[2 * slot + 1], s));
-        samples.emplace_back(it, slot);
+        samples.emplace_back(a.it, slot);
         HIPCHK(I->ctx, launch_cg_rest<T>(a, s));
       } else {
         HIPCHK(I->ctx, launch_cg_iteration<T>(a, s));
       }
     }
     launched += chunk;
-    HIPCHK(I->ctx, hipMemcpyAsync(I->sc_host, I->sc, sizeof(CgScalars), hipMemcpyDeviceToHost, s));
+    // after `launched` iterations the current scalars are in set launched & 1
+    HIPCHK(I->ctx, hipMemcpyAsync(I->sc_host, I->sc + (launched & 1), sizeof(CgScalars),
+                                  hipMemcpyDeviceToHost, s));
     HIPCHK(I->ctx, hipStreamSynchronize(s));
     if (I->sc_host->done) break;
     if (launched > maxIt + 64) return fail(I->ctx, HOBAK_E_HIP, "PCG loop did not terminate");
-    chunk = 8;
+    chunk = I->poll_chunk;
   }
+  I->last_it = launched;
   // SpMV launches 0..iters did work (the converging pass included); later ones returned at once.
   for (auto& sm : samples) {
     if (sm.first > I->sc_host->iters) continue;
@@ -439,7 +461,7 @@ int step_impl(hobak_inst* I, hobak_step_stats* st) {
   const int bdf = N > 1;
   const double invDt = 1.0 / I->prm.dt;
   double ms[5] = {0, 0, 0, 0, 0};
-  TRY(check_reduce_capacity(I));
+  TRY(check_partials(I));
   HIPCHK(ctx, launch_begin_step(I->nv, I->u, I->v, I->u_old, I->v_old, I->pinned, I->pin_target, s));
   int done = 0;
   double maxR = 10 * I->prm.newton_tol;
@@ -452,6 +474,7 @@ int step_impl(hobak_inst* I, hobak_step_stats* st) {
     HIPCHK(ctx, launch_assemble<T>(asm_args(I, 1, I->A), s));
     HIPCHK(ctx, hipEventRecord(I->ev[2], s));
     HIPCHK(ctx, launch_rhs_init<T>(rhs_args(I, bdf), s));
+    HIPCHK(ctx, launch_cg_init(init_args(I), s));
     HIPCHK(ctx, hipEventRecord(I->ev[3], s));
     TRY(run_pcg<T>(I));
     HIPCHK(ctx, hipEventRecord(I->ev[4], s));
@@ -653,7 +676,8 @@ int hobak_inst_create(hobak_ctx* ctx, const double* rest_xyz, int32_t nv, const 
     I->alpha = 0.0;
   }
   I->tsize = 8;
-  build_pattern(nv, nt, tets, I->P);
+  build_pattern(nv, nt, tets, ctx->slice_rows, I->P);
+  I->poll_chunk = ctx->poll_chunk;
   const Pattern& P = I->P;
   const size_t need = estimate_bytes(nv, nt, P.nnzb, P.nslots, params->precision);
   if (ctx->budget && ctx->in_use + need > ctx->budget) {
@@ -705,13 +729,17 @@ int hobak_inst_create(hobak_ctx* ctx, const double* rest_xyz, int32_t nv, const 
   ALLOCT(r, 3 * np);
   ALLOCT(x, 3 * np);
   ALLOCT(z, 3 * np);
-  ALLOCT(pbuf0, 3 * np);
-  ALLOCT(pbuf1, 3 * np);
+  ALLOCT(p, 3 * np);
   ALLOCT(t, 3 * np);
-  ALLOC(sc, 1);
-  I->partial_capacity = (size_t)kMaxReduceVals * reduction_grid_max(nv);
-  ALLOC(partial, I->partial_capacity);
-  ALLOC(counters, 16);
+  ALLOC(sc, 2);
+  I->n_rhs = rhs_grid(nv);
+  I->n_spmv = spmv_grid(nv, P.shift);
+  I->n_update = update_grid(nv);
+  ALLOC(part_rhs_a, (size_t)I->n_rhs);
+  ALLOC(part_rhs_b, (size_t)I->n_rhs);
+  ALLOC(part_pt, (size_t)I->n_spmv);
+  ALLOC(part_rr, (size_t)I->n_update);
+  ALLOC(part_rz, (size_t)I->n_update);
 #undef ALLOC
 #undef ALLOCT
   if (hipHostMalloc((void**)&I->sc_host, sizeof(CgScalars), hipHostMallocDefault) != hipSuccess)
@@ -916,6 +944,23 @@ int hobak_copy_probe(hobak_ctx* ctx, size_t bytes, int32_t reps, double* ms) {
   (void)hipFree(a);
   (void)hipFree(b);
   return err == hipSuccess ? HOBAK_OK : fail(ctx, HOBAK_E_HIP, "copy probe failed");
+}
+
+int hobak_ctx_set_tuning(hobak_ctx* ctx, int32_t key, int32_t value) {
+  if (!ctx) return HOBAK_E_INVALID;
+  switch (key) {
+    case HOBAK_TUNE_SLICE_ROWS:
+      if (value != 8 && value != 16 && value != 32 && value != 64)
+        return fail(ctx, HOBAK_E_INVALID, "slice rows must be 8, 16, 32 or 64");
+      ctx->slice_rows = value;
+      return HOBAK_OK;
+    case HOBAK_TUNE_POLL_CHUNK:
+      if (value < 1 || value > 4096) return fail(ctx, HOBAK_E_INVALID, "poll chunk out of range");
+      ctx->poll_chunk = value;
+      return HOBAK_OK;
+    default:
+      return fail(ctx, HOBAK_E_INVALID, "unknown tuning key");
+  }
 }
 
 }  // extern "C"

@@ -6,11 +6,13 @@
 //   DmInv          9 planes of nt doubles (plane 3*r + c holds DmInv(r, c))       -> coalesced
 //   element stage  90 planes of nt (10 upper 3x3 blocks of the 12x12 stiffness) + 12 planes of
 //                  forces; plane index (block * 9 + 3 * a + b)
-//   block matrix   SELL-8, k-interleaved: rows (vertices) in slices of 8; slice s owns slots
-//                  [slice_off[s], slice_off[s+1]) (a multiple of 64), and the k-th block of row r
-//                  sits at slot = slice_off[r / 8] + 8 * k + (r % 8).  A wavefront that gives
-//                  each of a slice's 8 rows 8 lanes (lane = 8 * (k % 8) + r % 8) therefore reads
-//                  slot_off + 64 * (k / 8) + lane: contiguous 512-byte FP64 wave loads.
+//   block matrix   SELL-H, k-interleaved (H = 8, 16, 32 or 64 rows per slice, L = 64 / H lanes
+//                  per row): slice s owns slots [slice_off[s], slice_off[s+1]) (a multiple of 64),
+//                  and the k-th block of row r sits at slot = slice_off[r / H] + H * k + (r % H).
+//                  A wavefront that gives each of a slice's H rows L lanes (lane = H * (k % L) +
+//                  r % H) therefore reads slice_off + 64 * (k / L) + lane: contiguous 512-byte
+//                  FP64 wave loads, while the H lanes that share k gather the vector at H
+//                  consecutive rows' k-th neighbours (consecutive on a lattice).
 //                  9 value planes of nslots (plane 3a+b = entry (a, b) of the 3x3 block), one
 //                  int32 column (vertex) and one int32 owning row (-1 for padding) per slot.
 //                  Columns of a row are ascending in k.
@@ -25,13 +27,15 @@
 
 namespace hobak {
 
-constexpr int kSlice = 8;     // rows per SELL slice
-constexpr int kLanesPerRow = 8;
+constexpr int kDefaultSliceRows = 64;
 constexpr int kBlock = 256;
 constexpr int kStageBlocks = 10;
 constexpr int kStagePlanes = 90;
 
 // Scalars of one PCG solve, resident on the device so the loop runs without host round trips.
+// Two copies ping-pong: iteration `it` (counted by the host that queues the launches) reads
+// set it & 1 and its last kernel writes set (it + 1) & 1 from one workgroup only, so no kernel
+// ever reads a scalar that another of its own workgroups writes.
 struct CgScalars {
   double rhsNorm2;
   double threshold;
@@ -40,16 +44,19 @@ struct CgScalars {
   double beta;
   double resNorm2;
   double maxR;        // |S b|^2, the BDF_1 Newton test value
-  int iters;         // also selects the p double buffer: iteration i reads p[i & 1]
+  int iters;
   int maxIters;
   int done;           // 1 once converged / exhausted / failed
   int status;         // 0 ok, 1 not converged, 2 non-finite
   int pad[4];
 };
 
-struct Reduce {
-  double* partial;    // [nvals * grid]
-  unsigned* counter;  // one per reduction site
+// Per-workgroup partial sums written by one kernel and reduced, in a fixed order, by every
+// workgroup of the next kernel (no atomics, no in-kernel fences).
+struct Partials {
+  double* a;  // [grid] first value per producing workgroup
+  double* b;  // [grid] second value
+  int n;      // producing grid size
 };
 
 template <typename T>

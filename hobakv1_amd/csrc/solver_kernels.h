@@ -40,6 +40,7 @@ struct ConstArgs {
 
 struct RhsArgs {
   int nrows, nslots, nt;
+  int slice_shift;       // log2 of rows per slice
   const int* slice_off;
   const int* row_len;
   const int* col;
@@ -61,22 +62,28 @@ struct RhsArgs {
   int ground_on;
   double ground_y, ground_k;
   double* rhs64;
-  int init_cg;
+  int init_cg;           // also seed the PCG vectors (x = 0, r = rhs, z = p = M^-1 r)
   void* r;
   void* x;
   void* z;
-  void* pbuf0;
-  void* pbuf1;
+  void* p;
   const void* minv;
   int precond;
+  Partials out;          // per workgroup: a = |S b|^2, b = r.z
+};
+
+// Single-workgroup kernel that turns the right-hand side's partials into the PCG's first
+// scalar set (ConjugateGradient.h:45-66).
+struct InitArgs {
+  Partials in;
+  CgScalars* sc;         // sc[0] is written
   double tol;
   int max_iters;
-  CgScalars* sc;
-  Reduce red;
 };
 
 struct CgArgs {
   int nrows, nslots;
+  int slice_shift;       // log2 of rows per slice
   const int* slice_off;
   const int* row_len;
   const int* col;
@@ -84,13 +91,14 @@ struct CgArgs {
   void* x;
   void* r;
   void* z;
-  void* pbuf0;   // p double buffer: iteration i reads pbuf[i & 1] and writes pbuf[(i + 1) & 1]
-  void* pbuf1;
+  void* p;
   void* t;
   const void* minv;
   int precond;
-  CgScalars* sc;
-  Reduce red;
+  CgScalars* sc;         // two sets, see CgScalars
+  int it;                // host iteration counter: this launch reads sc[it & 1]
+  Partials pt;           // SpMV -> update: p.t
+  Partials rz;           // update -> p: a = r.r, b = r.z
 };
 
 hipError_t launch_begin_step(int nv, double* u, const double* v, double* uOld, double* vOld,
@@ -105,6 +113,7 @@ template <typename T>
 hipError_t launch_constant_parts(const ConstArgs& a, hipStream_t s);
 template <typename T>
 hipError_t launch_rhs_init(const RhsArgs& a, hipStream_t s);
+hipError_t launch_cg_init(const InitArgs& a, hipStream_t s);
 template <typename T>
 hipError_t launch_cg_iteration(const CgArgs& a, hipStream_t s);
 template <typename T>
@@ -115,9 +124,9 @@ template <typename T>
 hipError_t launch_force_gather(int nv, int nt, const int* vptr, const int* vlist, const T* stage_f,
                                double* R, hipStream_t s);
 hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
-// Largest grid (in workgroups) of any kernel that runs a grid_reduce over `nrows` rows; the
-// partial-sum buffer of a reduction needs kMaxReduceVals * reduction_grid_max(nrows) doubles.
-int reduction_grid_max(int nrows);
-constexpr int kMaxReduceVals = 2;
+// Workgroup counts of the kernels that publish partial sums (size of each Partials array).
+int rhs_grid(int nrows);
+int spmv_grid(int nrows, int slice_shift);
+int update_grid(int nrows);
 
 }  // namespace hobak

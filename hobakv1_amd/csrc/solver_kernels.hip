@@ -27,63 +27,54 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// Block-reduce NV values, publish the partials, and return true in the last block to arrive,
-// with the grid totals in `tot` (valid in every lane of that block).  Every thread of every block
-// must call it.
+// Sum of one value over the workgroup, returned in every lane.
 template <int NV>
-__device__ bool grid_reduce(double (&vals)[NV], const Reduce& R, double (&tot)[NV]) {
+__device__ __forceinline__ void block_sum(double (&v)[NV]) {
   __shared__ double sh[NV][kBlock / 64];
-  __shared__ int is_last;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
 #pragma unroll
   for (int k = 0; k < NV; k++) {
-    const double s = wave_sum(vals[k]);
+    const double s = wave_sum(v[k]);
     if (lane == 0) sh[k][wave] = s;
   }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int k = 0; k < NV; k++) {
-      double s = 0.0;
-#pragma unroll
-      for (int w = 0; w < kBlock / 64; w++) s += sh[k][w];
-      R.partial[k * gridDim.x + blockIdx.x] = s;
-    }
-    __threadfence();
-    const unsigned prev = atomicAdd(R.counter, 1u);
-    is_last = (prev == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!is_last) return false;
-  __threadfence();
-  double acc[NV];
-#pragma unroll
-  for (int k = 0; k < NV; k++) {
-    double s = 0.0;
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += kBlock)
-      s += __builtin_nontemporal_load(&R.partial[k * gridDim.x + b]);
-    acc[k] = wave_sum(s);
-  }
-  __syncthreads();
-  if (lane == 0)
-#pragma unroll
-    for (int k = 0; k < NV; k++) sh[k][wave] = acc[k];
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < NV; k++) {
     double s = 0.0;
 #pragma unroll
     for (int w = 0; w < kBlock / 64; w++) s += sh[k][w];
-    tot[k] = s;
+    v[k] = s;
   }
-  if (threadIdx.x == 0) *R.counter = 0u;
-  return true;
+  __syncthreads();
 }
 
-// Slot of the k-th block of row r.
-__device__ __forceinline__ int slot_of(const int* slice_off, int r, int k) {
-  return slice_off[r >> 3] + (k << 3) + (r & 7);
+// Publish this workgroup's sums of (a, b) to the Partials arrays.
+__device__ __forceinline__ void publish(double a, double b, const Partials& P, bool two) {
+  double v[2] = {a, b};
+  block_sum<2>(v);
+  if (threadIdx.x == 0) {
+    P.a[blockIdx.x] = v[0];
+    if (two) P.b[blockIdx.x] = v[1];
+  }
+}
+
+// Every workgroup reduces the previous kernel's partials in the same fixed order, so all of
+// them (and every run) obtain bit-identical totals.
+__device__ __forceinline__ void gather_partials(const Partials& P, bool two, double& ta, double& tb) {
+  double v[2] = {0.0, 0.0};
+  for (int i = threadIdx.x; i < P.n; i += kBlock) {
+    v[0] += P.a[i];
+    if (two) v[1] += P.b[i];
+  }
+  block_sum<2>(v);
+  ta = v[0];
+  tb = v[1];
+}
+
+// Slot of the k-th block of row r (slice height 1 << sh).
+__device__ __forceinline__ int slot_of(const int* slice_off, int sh, int r, int k) {
+  return slice_off[r >> sh] + (k << sh) + (r & ((1 << sh) - 1));
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -273,7 +264,7 @@ __global__ __launch_bounds__(kBlock) void rhs_init_kernel(RhsArgs a) {
     double s[3] = {0.0, 0.0, 0.0};
     const int len = a.row_len[r];
     for (int k = 0; k < len; k++) {
-      const int slot = slot_of(a.slice_off, r, k);
+      const int slot = slot_of(a.slice_off, a.slice_shift, r, k);
       const int c = a.col[slot];
       double xv[3];
 #pragma unroll
@@ -315,167 +306,213 @@ __global__ __launch_bounds__(kBlock) void rhs_init_kernel(RhsArgs a) {
         ((T*)a.r)[i] = rv[c];
         ((T*)a.x)[i] = T(0);
         ((T*)a.z)[i] = zv[c];
-        ((T*)a.pbuf0)[i] = T(0);
+        ((T*)a.p)[i] = zv[c];
         part[1] += (double)rv[c] * (double)zv[c];
       }
     }
   }
-  double tot[2];
-  if (grid_reduce<2>(part, a.red, tot) && threadIdx.x == 0) {
-    CgScalars* sc = a.sc;
-    sc->maxR = tot[0];
-    if (a.init_cg) {
-      const double rhsNorm2 = tot[0];
-      sc->rhsNorm2 = rhsNorm2;
-      sc->iters = 0;
-      sc->status = 0;
-      sc->done = 0;
-      sc->maxIters = a.max_iters;
-      sc->beta = 0.0;
-      sc->alpha = 0.0;
-      sc->resNorm2 = rhsNorm2;
-      if (rhsNorm2 == 0.0) {
-        sc->done = 1;
-      } else {
-        const double thr = a.tol * a.tol * rhsNorm2;
-        sc->threshold = thr > DBL_MIN ? thr : DBL_MIN;
-        if (rhsNorm2 < sc->threshold) sc->done = 1;
-        if (!(rhsNorm2 == rhsNorm2) || rhsNorm2 > DBL_MAX) {
-          sc->done = 1;
-          sc->status = 2;
-        }
-      }
-      if (a.max_iters <= 0) sc->done = 1;
-      sc->absNew = tot[1];
-    }
+  publish(part[0], part[1], a.out, true);
+}
+
+// sc[0] from the right-hand side's partials: Eigen's early exits on a zero or already small
+// right-hand side (ConjugateGradient.h:45-61) and the first r.z.
+__global__ __launch_bounds__(kBlock) void cg_init_kernel(InitArgs a) {
+  double bb, rz;
+  gather_partials(a.in, true, bb, rz);
+  if (threadIdx.x != 0) return;
+  CgScalars* s = &a.sc[0];
+  const double thr0 = a.tol * a.tol * bb;
+  const double thr = thr0 > DBL_MIN ? thr0 : DBL_MIN;
+  int done = (bb == 0.0 || bb < thr || a.max_iters <= 0) ? 1 : 0;
+  int status = 0;
+  if (!(bb == bb) || bb > DBL_MAX) {
+    done = 1;
+    status = 2;
   }
+  s->maxR = bb;
+  s->rhsNorm2 = bb;
+  s->resNorm2 = bb;
+  s->threshold = thr;
+  s->absNew = rz;
+  s->alpha = 0.0;
+  s->beta = 0.0;
+  s->iters = 0;
+  s->maxIters = a.max_iters;
+  s->done = done;
+  s->status = status;
 }
 
 // ---------------------------------------------------------------------------------------------
-// PCG iteration: two kernels
-//   spmv:    p_new = z + beta p_old (formed on the fly at every column, written for own rows),
-//            t = A p_new, alpha = absNew / (p_new . t)
-//   update:  x += alpha p_new, r -= alpha t, |r|^2 test, z = M^-1 r, absNew = r.z, beta
-// Each kernel returns at once when the solve is finished, so a fixed chunk of iterations can be
-// queued and polled from the host between chunks.
+// PCG iteration `it`: three kernels, all reading the scalar set S = sc[it & 1]
+//   spmv:    t = A p; partials of p.t
+//   update:  alpha = S.absNew / (p.t); x += alpha p; r -= alpha t; z = M^-1 r; partials of
+//            r.r and r.z
+//   p:       |r|^2 < threshold -> done (the converging pass is not counted); otherwise
+//            beta = r.z / S.absNew, p = z + beta p, iters + 1; workgroup 0 writes the next set
+//            sc[(it + 1) & 1] (a finished set is carried forward unchanged)
+// Kernels of a finished solve return at once, so chunks of iterations can be queued ahead and
+// the host polls the flag between chunks.
 // ---------------------------------------------------------------------------------------------
-// One wavefront per 8-row slice, 8 lanes per row: lane = 8 * kk + rr handles the row's blocks
-// k = kk, kk + 8, ... and the 8 partial sums of a row are combined with cross-lane adds.
-template <typename T>
+// One wavefront per H-row slice, L = 64 / H lanes per row: lane = H * kk + rr handles the row's
+// blocks k = kk, kk + L, ...; a row's L partial sums are combined with cross-lane adds.
+template <typename T, int H>
 __global__ __launch_bounds__(kBlock) void cg_spmv_kernel(CgArgs a) {
-  if (a.sc->done) return;
+  if (a.sc[a.it & 1].done) return;  // a single field load
+  constexpr int L = 64 / H;
   const int slice = (int)((blockIdx.x * (unsigned)blockDim.x + threadIdx.x) >> 6);
   const int lane = threadIdx.x & 63;
-  const int kk = lane >> 3;
-  const int r = slice * kSlice + (lane & 7);
-  const int parity = a.sc->iters & 1;
-  const T* pold = (const T*)(parity ? a.pbuf1 : a.pbuf0);
-  T* pnew = (T*)(parity ? a.pbuf0 : a.pbuf1);
-  const T* z = (const T*)a.z;
-  const T beta = (T)a.sc->beta;
+  const int kk = lane / H;
+  const int rr = lane % H;
+  const int r = slice * H + rr;
+  const T* p = (const T*)a.p;
   const T* A = (const T*)a.A;
   const size_t ns = (size_t)a.nslots;
   T s0 = T(0), s1 = T(0), s2 = T(0);
   const bool live = r < a.nrows;
   if (live) {
     const int len = a.row_len[r];
-    const int base = a.slice_off[slice] + (lane & 7);
-    for (int k = kk; k < len; k += kLanesPerRow) {
-      const int slot = base + (k << 3);
+    const int base = a.slice_off[slice] + rr;
+    for (int k = kk; k < len; k += L) {
+      const int slot = base + k * H;
       const size_t c = 3 * (size_t)a.col[slot];
-      const T p0 = z[c] + beta * pold[c];
-      const T p1 = z[c + 1] + beta * pold[c + 1];
-      const T p2 = z[c + 2] + beta * pold[c + 2];
+      const T p0 = p[c], p1 = p[c + 1], p2 = p[c + 2];
       s0 += A[0 * ns + slot] * p0 + A[1 * ns + slot] * p1 + A[2 * ns + slot] * p2;
       s1 += A[3 * ns + slot] * p0 + A[4 * ns + slot] * p1 + A[5 * ns + slot] * p2;
       s2 += A[6 * ns + slot] * p0 + A[7 * ns + slot] * p1 + A[8 * ns + slot] * p2;
     }
   }
 #pragma unroll
-  for (int off = 8; off < 64; off <<= 1) {
+  for (int off = H; off < 64; off <<= 1) {
     s0 += __shfl_xor(s0, off, 64);
     s1 += __shfl_xor(s1, off, 64);
     s2 += __shfl_xor(s2, off, 64);
   }
-  double part[1] = {0.0};
+  double pt = 0.0;
   if (live && kk == 0) {
     const size_t i = 3 * (size_t)r;
     T* t = (T*)a.t;
     t[i] = s0;
     t[i + 1] = s1;
     t[i + 2] = s2;
-    const T q0 = z[i] + beta * pold[i];
-    const T q1 = z[i + 1] + beta * pold[i + 1];
-    const T q2 = z[i + 2] + beta * pold[i + 2];
-    pnew[i] = q0;
-    pnew[i + 1] = q1;
-    pnew[i + 2] = q2;
-    part[0] = (double)q0 * (double)s0 + (double)q1 * (double)s1 + (double)q2 * (double)s2;
+    pt = (double)p[i] * (double)s0 + (double)p[i + 1] * (double)s1 + (double)p[i + 2] * (double)s2;
   }
-  double tot[1];
-  if (grid_reduce<1>(part, a.red, tot) && threadIdx.x == 0) a.sc->alpha = a.sc->absNew / tot[0];
+  publish(pt, 0.0, a.pt, false);
 }
 
 // One lane per DOF, 21 rows (63 lanes) per wavefront so that a row's three DOFs share a wave and
-// the 3x3 preconditioner apply can read its siblings' residuals with cross-lane moves.
+// the 3x3 preconditioner apply can read its siblings' residuals with cross-lane moves.  A fixed
+// grid strides over the waves, which keeps the partial-sum count (and the work of every
+// workgroup that later re-reduces them) small.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void cg_update_kernel(CgArgs a) {
-  if (a.sc->done) return;
-  const int wave = (int)((blockIdx.x * (unsigned)blockDim.x + threadIdx.x) >> 6);
+  const CgScalars* S = &a.sc[a.it & 1];
+  if (S->done) return;
+  const double absNew = S->absNew;
+  double pt, unused;
+  gather_partials(a.pt, false, pt, unused);
+  const T alpha = (T)(absNew / pt);
   const int lane = threadIdx.x & 63;
-  const int row = wave * 21 + lane / 3;
   const int c = lane - 3 * (lane / 3);
-  const bool live = lane < 63 && row < a.nrows;
-  const int parity = a.sc->iters & 1;
-  const T* p = (const T*)(parity ? a.pbuf0 : a.pbuf1);  // the p_new this iteration's SpMV wrote
-  const T alpha = (T)a.sc->alpha;
-  const size_t i = 3 * (size_t)row + c;
-  T rv = T(0);
-  if (live) {
-    T* x = (T*)a.x;
-    T* rr = (T*)a.r;
-    x[i] += alpha * p[i];
-    rv = rr[i] - alpha * ((const T*)a.t)[i];
-    rr[i] = rv;
-  }
   const int b = lane - c;
-  const T r0 = __shfl(rv, b, 64);
-  const T r1 = __shfl(rv, b + 1, 64);
-  const T r2 = __shfl(rv, b + 2, 64);
-  double part[2] = {0.0, 0.0};
-  if (live) {
-    const T* mi = (const T*)a.minv;
-    T zv;
-    if (a.precond == 0) {
-      zv = mi[i] * rv;
-    } else {
-      const T* m = mi + (size_t)kMinvStride * row + 3 * c;
-      zv = m[0] * r0 + m[1] * r1 + m[2] * r2;
+  const int nwaves = (a.nrows + 20) / 21;
+  const int stride = (int)(gridDim.x * (kBlock / 64));
+  const T* p = (const T*)a.p;
+  const T* mi = (const T*)a.minv;
+  T* x = (T*)a.x;
+  T* rr = (T*)a.r;
+  T* z = (T*)a.z;
+  const T* t = (const T*)a.t;
+  double rr2 = 0.0, rz = 0.0;
+  for (int wave = (int)((blockIdx.x * (unsigned)blockDim.x + threadIdx.x) >> 6); wave < nwaves;
+       wave += stride) {
+    const int row = wave * 21 + lane / 3;
+    const bool live = lane < 63 && row < a.nrows;
+    const size_t i = 3 * (size_t)row + c;
+    T rv = T(0);
+    if (live) {
+      x[i] += alpha * p[i];
+      rv = rr[i] - alpha * t[i];
+      rr[i] = rv;
     }
-    ((T*)a.z)[i] = zv;
-    part[0] = (double)rv * (double)rv;
-    part[1] = (double)rv * (double)zv;
-  }
-  double tot[2];
-  if (grid_reduce<2>(part, a.red, tot) && threadIdx.x == 0) {
-    CgScalars* sc = a.sc;
-    sc->resNorm2 = tot[0];
-    if (!(tot[0] == tot[0]) || tot[0] > DBL_MAX || !(tot[1] == tot[1])) {
-      sc->done = 1;
-      sc->status = 2;
-    } else if (tot[0] < sc->threshold) {
-      sc->done = 1;
-    } else {
-      const double absOld = sc->absNew;
-      sc->absNew = tot[1];
-      sc->beta = tot[1] / absOld;
-      sc->iters += 1;
-      if (sc->iters >= sc->maxIters) {
-        sc->done = 1;
-        sc->status = 1;
+    const T r0 = __shfl(rv, b, 64);
+    const T r1 = __shfl(rv, b + 1, 64);
+    const T r2 = __shfl(rv, b + 2, 64);
+    if (live) {
+      T zv;
+      if (a.precond == 0) {
+        zv = mi[i] * rv;
+      } else {
+        const T* m = mi + (size_t)kMinvStride * row + 3 * c;
+        zv = m[0] * r0 + m[1] * r1 + m[2] * r2;
       }
+      z[i] = zv;
+      rr2 += (double)rv * (double)rv;
+      rz += (double)rv * (double)zv;
     }
   }
+  publish(rr2, rz, a.rz, true);
+}
+
+// Scalars are read and written field by field: a by-value copy of the struct is turned by the
+// compiler into a per-lane LDS buffer that costs microseconds per launch.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void cg_p_kernel(CgArgs a) {
+  const CgScalars* S = &a.sc[a.it & 1];
+  CgScalars* N = &a.sc[(a.it + 1) & 1];
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  const int done = S->done;
+  const double absOld = S->absNew;
+  const int iters = S->iters;
+  const int maxIters = S->maxIters;
+  const double threshold = S->threshold;
+  if (lead) {  // carry the set forward; the fields that change are overwritten below
+    N->rhsNorm2 = S->rhsNorm2;
+    N->threshold = threshold;
+    N->absNew = absOld;
+    N->alpha = S->alpha;
+    N->beta = S->beta;
+    N->resNorm2 = S->resNorm2;
+    N->maxR = S->maxR;
+    N->iters = iters;
+    N->maxIters = maxIters;
+    N->done = done;
+    N->status = S->status;
+  }
+  if (done) return;
+  double rr, rz;
+  gather_partials(a.rz, true, rr, rz);
+  if (!(rr == rr) || rr > DBL_MAX || !(rz == rz)) {
+    if (lead) {
+      N->resNorm2 = rr;
+      N->done = 1;
+      N->status = 2;
+    }
+    return;
+  }
+  if (rr < threshold) {  // converged: Eigen breaks before z, beta and the count (CG.h:77-79)
+    if (lead) {
+      N->resNorm2 = rr;
+      N->done = 1;
+    }
+    return;
+  }
+  const double beta = rz / absOld;
+  if (lead) {
+    N->resNorm2 = rr;
+    N->beta = beta;
+    N->absNew = rz;
+    N->iters = iters + 1;
+    if (iters + 1 >= maxIters) {
+      N->done = 1;
+      N->status = 1;
+    }
+  }
+  T* p = (T*)a.p;
+  const T* z = (const T*)a.z;
+  const T tb = (T)beta;
+  const int ndof = 3 * a.nrows;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ndof; i += gridDim.x * blockDim.x)
+    p[i] = z[i] + tb * p[i];
 }
 
 // computeHyperelasticForces' scatter as a per-vertex gather in ascending tet order
@@ -540,32 +577,49 @@ hipError_t launch_constant_parts(const ConstArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
+int rhs_grid(int nrows) { return grid_for(nrows); }
+int spmv_grid(int nrows, int sh) { return grid_for((long)((nrows + (1 << sh) - 1) >> sh) * 64); }
+// The update and p kernels stride over their work with at most kVectorBlocks workgroups.
+constexpr int kVectorBlocks = 512;
+int update_grid(int nrows) {
+  const int g = grid_for((long)((nrows + 20) / 21) * 64);
+  return g < kVectorBlocks ? g : kVectorBlocks;
+}
+static inline int p_grid(int nrows) {
+  const int g = grid_for(3L * nrows);
+  return g < kVectorBlocks ? g : kVectorBlocks;
+}
+
 template <typename T>
 hipError_t launch_rhs_init(const RhsArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL((rhs_init_kernel<T>), dim3(grid_for(a.nrows)), dim3(kBlock), 0, s, a);
+  hipLaunchKernelGGL((rhs_init_kernel<T>), dim3(rhs_grid(a.nrows)), dim3(kBlock), 0, s, a);
   return hipGetLastError();
 }
 
-// grid sizes: SpMV one wave per 8-row slice; update one wave per 21 rows; rhs one lane per row
-static inline int spmv_grid(int nrows) { return grid_for((long)((nrows + kSlice - 1) / kSlice) * 64); }
-static inline int update_grid(int nrows) { return grid_for((long)((nrows + 20) / 21) * 64); }
-
-int reduction_grid_max(int nrows) {
-  int g = grid_for(nrows);
-  if (spmv_grid(nrows) > g) g = spmv_grid(nrows);
-  if (update_grid(nrows) > g) g = update_grid(nrows);
-  return g;
+hipError_t launch_cg_init(const InitArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(cg_init_kernel, dim3(1), dim3(kBlock), 0, s, a);
+  return hipGetLastError();
 }
 
 template <typename T>
 hipError_t launch_cg_spmv(const CgArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL((cg_spmv_kernel<T>), dim3(spmv_grid(a.nrows)), dim3(kBlock), 0, s, a);
+  const dim3 g(spmv_grid(a.nrows, a.slice_shift)), b(kBlock);
+  switch (a.slice_shift) {
+    case 3: hipLaunchKernelGGL((cg_spmv_kernel<T, 8>), g, b, 0, s, a); break;
+    case 4: hipLaunchKernelGGL((cg_spmv_kernel<T, 16>), g, b, 0, s, a); break;
+    case 5: hipLaunchKernelGGL((cg_spmv_kernel<T, 32>), g, b, 0, s, a); break;
+    case 6: hipLaunchKernelGGL((cg_spmv_kernel<T, 64>), g, b, 0, s, a); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 
+// The p kernel's grid must cover every DOF and the update kernel's grid every row; the pt / rz
+// partial arrays hold spmv_grid / update_grid entries.
 template <typename T>
 hipError_t launch_cg_rest(const CgArgs& a, hipStream_t s) {
   hipLaunchKernelGGL((cg_update_kernel<T>), dim3(update_grid(a.nrows)), dim3(kBlock), 0, s, a);
+  hipLaunchKernelGGL((cg_p_kernel<T>), dim3(p_grid(a.nrows)), dim3(kBlock), 0, s, a);
   return hipGetLastError();
 }
 

@@ -99,7 +99,11 @@ SIGNATURES = {
     "hobak_set_kernel_sampling": (ctypes.c_int, [_vp, ctypes.c_int32]),
     "hobak_kernel_stats": (ctypes.c_int, [_vp, _dp, ctypes.POINTER(ctypes.c_int64), _dp]),
     "hobak_copy_probe": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.c_int32, _dp]),
+    "hobak_ctx_set_tuning": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32]),
 }
+
+TUNE_SLICE_ROWS = 0
+TUNE_POLL_CHUNK = 2
 
 _lib = None
 
@@ -140,15 +144,33 @@ def default_params() -> hobak_step_params:
 
 
 class Context:
-    """One device + stream + HBM budget (0 = unlimited)."""
+    """One device + stream + HBM budget (0 = unlimited).
 
-    def __init__(self, device: int = 0, budget_bytes: int = 0):
+    apply_table: apply the autotuned launch shapes of tuning.json (hobakv1_amd/autotune.py) for
+    `precision` to the instances this context creates.
+    """
+
+    def __init__(self, device: int = 0, budget_bytes: int = 0, apply_table: bool = True,
+                 precision: str = "fp64"):
         self.lib = load()
         h = _vp()
         rc = self.lib.hobak_ctx_create(device, budget_bytes, ctypes.byref(h))
         if rc != HOBAK_OK:
             raise HobakError(rc, "hobak_ctx_create failed (no gfx950 device?)")
         self.h = h
+        if apply_table:
+            self.apply_tuning_table(precision)
+
+    def apply_tuning_table(self, precision="fp64"):
+        path = os.path.join(HERE, "tuning.json")
+        if not os.path.exists(path):
+            return
+        import json
+
+        with open(path) as f:
+            entry = json.load(f).get(precision)
+        if entry:
+            self.set_tuning(slice_rows=entry.get("slice_rows"), poll_chunk=entry.get("poll_chunk"))
 
     def error(self) -> str:
         return self.lib.hobak_last_error(self.h).decode()
@@ -157,6 +179,12 @@ class Context:
         if rc != HOBAK_OK:
             raise HobakError(rc, f"{what}: {self.error()}")
         return rc
+
+    def set_tuning(self, slice_rows=None, poll_chunk=None):
+        """Kernel tuning for instances created afterwards (see hobak_ctx_set_tuning)."""
+        for key, val in ((TUNE_SLICE_ROWS, slice_rows), (TUNE_POLL_CHUNK, poll_chunk)):
+            if val is not None:
+                self.check(self.lib.hobak_ctx_set_tuning(self.h, key, int(val)), "set_tuning")
 
     def copy_probe(self, nbytes, reps=20):
         ms = ctypes.c_double(0)

@@ -151,6 +151,14 @@ int hobak_last_step_timing(hobak_inst* inst, double* out_ms);
 int hobak_set_kernel_sampling(hobak_inst* inst, int32_t every);
 int hobak_kernel_stats(hobak_inst* inst, double* spmv_avg_ms, int64_t* spmv_count,
                        double* spmv_bytes);
+/* Kernel tuning (no counterpart in the reference); applies to instances created afterwards.
+ *   HOBAK_TUNE_SLICE_ROWS  rows per block-matrix slice: 8, 16, 32 or 64 (64 / rows lanes per row
+ *                          in the SpMV)
+ *   HOBAK_TUNE_POLL_CHUNK  PCG iterations queued between host polls of the convergence flag */
+#define HOBAK_TUNE_SLICE_ROWS 0
+#define HOBAK_TUNE_POLL_CHUNK 2
+int hobak_ctx_set_tuning(hobak_ctx* ctx, int32_t key, int32_t value);
+
 /* Streaming copy of `bytes` device bytes (a calibration kernel for hardware byte counters);
  * returns the kernel time in ms. */
 int hobak_copy_probe(hobak_ctx* ctx, size_t bytes, int32_t reps, double* ms);
