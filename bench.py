@@ -12,7 +12,9 @@ Output: one JSON line (see the field list in main()).
   value      tet-Newton-steps/s = nt * Newton iterations * K / timed seconds
   roofline   the PCG's BSR SpMV (the dominant kernel): algorithmic bytes per launch
              nnzb*(9*8+4) + 4*(nv+1) + 2*3*8*nv divided by its average launch time, sampled live
-             with HIP events on the library's stream during the timed steps
+             during the timed steps with HIP start/stop events that the SpMV's own dispatch
+             carries (hipExtLaunchKernelGGL) on the library's stream; `traffic` is the HBM bytes
+             per launch from the committed rocprofv3 --pmc summary of this workload
   cpu_baseline  the literal reference (oracle/_ref, compiled from its sources) timed on a bounded
              sample of the same workload on this host's cores
 """
@@ -46,6 +48,24 @@ def parse():
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run the calibration copy kernel (for counter scaling)")
     return ap.parse_args()
+
+
+def measured_traffic(n, precision="fp64"):
+    """HBM bytes per SpMV launch from the committed rocprofv3 --pmc summary, when it was taken on
+    this workload (FETCH_SIZE and WRITE_SIZE scaled by a calibration copy kernel)."""
+    path = os.path.join(ROOT, "profiles", f"cg_spmv_{n}cube_{precision}_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    with open(path) as f:
+        for line in f:
+            if ":" in line and not line.startswith((" ", "#", "|")):
+                k, v = line.split(":", 1)
+                vals[k.strip()] = v.strip()
+    if vals.get("workload_n") != str(n) or vals.get("precision") != precision:
+        return None
+    return {"bytes": float(vals["traffic_bytes_per_launch"]), "kernel": vals.get("kernel"),
+            "source": os.path.relpath(path, ROOT)}
 
 
 def cpu_baseline(scene, steps, kind):
@@ -116,6 +136,7 @@ def main():
     finite = bool(np.all(np.isfinite(u)))
 
     spmv_gbs = ks["spmv_bytes"] / (ks["spmv_avg_ms"] * 1e-3) / 1e9 if ks["spmv_avg_ms"] > 0 else 0.0
+    tr = measured_traffic(args.n)
     roofline = {
         "bound": "hbm",
         "kernel": "cg_spmv_kernel<double>",
@@ -123,7 +144,8 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(spmv_gbs / HBM_PEAK_GBS, 4),
-        "traffic": None,
+        "traffic": tr["bytes"] if tr else None,
+        "traffic_source": tr["source"] if tr else None,
         "bytes_per_launch": ks["spmv_bytes"],
         "avg_launch_ms": ks["spmv_avg_ms"],
         "launches_sampled": ks["spmv_count"],

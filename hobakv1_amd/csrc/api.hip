@@ -47,12 +47,9 @@ struct Pattern {
   int H = 16, shift = 4;  // rows per slice and its log2
   std::vector<int> row_ptr, col_idx;  // BSR
   std::vector<int> slice_off, row_len, col, slot_row, bsr_slot, diag_slot;
-  std::vector<int> cptr, clist;       // per slot contributions
-  std::vector<int> vptr, vlist;       // per vertex force gather
+  std::vector<int> cptr, hdest;       // per slot contribution ranges; per (tet, x, y) position
+  std::vector<int> vptr, fdest;       // per vertex force ranges; per (tet, x) position
 };
-
-// Staged block index of the vertex pair (x, y), x <= y: the 10 upper blocks in row order.
-inline int stage_block(int x, int y) { return x * 4 - x * (x - 1) / 2 + (y - x); }
 
 void build_pattern(int nv, int nt, const int32_t* tets, int H, Pattern& P) {
   P.nv = nv;
@@ -132,25 +129,25 @@ void build_pattern(int nv, int nt, const int32_t* tets, int H, Pattern& P) {
       }
   P.cptr.assign(P.nslots + 1, 0);
   for (int s = 0; s < P.nslots; s++) P.cptr[s + 1] = P.cptr[s] + cnt[s + 1];
-  P.clist.assign(P.cptr[P.nslots], 0);
+  // Position of every element block in its slot's contribution list: looping tets in ascending
+  // order fills each list in ascending tet order (TET_MESH_FASTER.cpp:213-236 pushes the same way)
+  P.hdest.assign(16 * (size_t)nt, 0);
   std::vector<int> fill(P.cptr.begin(), P.cptr.end() - 1);
-  for (int t = 0; t < nt; t++)  // ascending tet order within every slot
+  for (int t = 0; t < nt; t++)
     for (int x = 0; x < 4; x++)
       for (int y = 0; y < 4; y++) {
         const int s = slots[16 * (size_t)t + 4 * x + y];
-        const int lb = x <= y ? stage_block(x, y) : stage_block(y, x);
-        const int tr = x <= y ? 0 : 1;
-        P.clist[fill[s]++] = t * 32 + tr * 16 + lb;
+        P.hdest[16 * (size_t)t + 4 * x + y] = fill[s]++;
       }
   std::vector<int> vc(nv + 1, 0);
   for (int t = 0; t < nt; t++)
     for (int a = 0; a < 4; a++) vc[tets[4 * t + a] + 1]++;
   P.vptr.assign(nv + 1, 0);
   for (int i = 0; i < nv; i++) P.vptr[i + 1] = P.vptr[i] + vc[i + 1];
-  P.vlist.assign(P.vptr[nv], 0);
+  P.fdest.assign(4 * (size_t)nt, 0);
   std::vector<int> vf(P.vptr.begin(), P.vptr.end() - 1);
   for (int t = 0; t < nt; t++)
-    for (int a = 0; a < 4; a++) P.vlist[vf[tets[4 * t + a]]++] = 4 * t + a;
+    for (int a = 0; a < 4; a++) P.fdest[4 * (size_t)t + a] = vf[tets[4 * t + a]]++;
 }
 
 }  // namespace
@@ -174,8 +171,8 @@ struct hobak_inst {
   double* kneg64 = nullptr;
   int4* tets = nullptr;
   int *slice_off = nullptr, *row_len = nullptr, *col = nullptr, *slot_row = nullptr;
-  int *cptr = nullptr, *clist = nullptr;
-  int *vptr = nullptr, *vlist = nullptr;
+  int *cptr = nullptr, *hdest = nullptr;
+  int *vptr = nullptr, *fdest = nullptr;
   unsigned char* pinned = nullptr;
   // device, real type T
   void *stage_h = nullptr, *stage_f = nullptr, *A = nullptr, *D = nullptr, *E = nullptr, *C = nullptr;
@@ -249,8 +246,8 @@ size_t estimate_bytes(int nv, int nt, int nnzb_est, int nslots_est, int precisio
   b += (size_t)nt * (16 + 9 * 8 + 8);           // tets, dminv, vol
   b += (size_t)nslots_est * (4 + 9 * 8);        // col + kneg64
   b += (size_t)nslots_est * 9 * ts * 4;         // A, D, E, C
-  b += (size_t)nslots_est * 4 + (size_t)nt * 16 * 4;  // cptr + clist
-  b += (size_t)nt * (kStagePlanes + 12) * ts;   // staging
+  b += (size_t)nslots_est * 4 + (size_t)nt * 20 * 4;  // cptr + hdest + fdest
+  b += (size_t)nt * (16 * 9 + 12) * ts;         // staging
   b += (size_t)nv * (9 + 15) * ts;              // minv + CG vectors
   b += (size_t)nt * 4 * 4 + (size_t)nv * 16;    // vertex lists, misc
   (void)nnzb_est;
@@ -272,6 +269,8 @@ ElementArgs element_args(hobak_inst* I, const double* u, int with_h) {
   a.vol = I->vol;
   a.stage_h = I->stage_h;
   a.stage_f = I->stage_f;
+  a.hdest = I->hdest;
+  a.fdest = I->fdest;
   a.with_hessian = with_h;
   return a;
 }
@@ -284,7 +283,6 @@ AsmArgs asm_args(hobak_inst* I, int mode, void* out) {
   a.slot_row = I->slot_row;
   a.col = I->col;
   a.cptr = I->cptr;
-  a.clist = I->clist;
   a.stage_h = I->stage_h;
   a.D = I->D;
   a.out = out;
@@ -342,7 +340,6 @@ RhsArgs rhs_args(hobak_inst* I, int bdf) {
   a.row_len = I->row_len;
   a.col = I->col;
   a.vptr = I->vptr;
-  a.vlist = I->vlist;
   a.stage_f = I->stage_f;
   a.Mvec = bdf ? I->C : I->E;
   a.vel = I->v;
@@ -423,9 +420,7 @@ int run_pcg(hobak_inst* I) {
                           (int)samples.size() < max_samples;
       if (sample) {
         const int slot = (int)samples.size();
-        HIPCHK(I->ctx, hipEventRecord(I->sample_ev[2 * slot], s));
-        HIPCHK(I->ctx, launch_cg_spmv<T>(a, s));
-        HIPCHK(I->ctx, hipEventRecord(I->sample_ev[2 * slot + 1], s));
+        HIPCHK(I->ctx, launch_cg_spmv<T>(a, s, I->sample_ev[2 * slot], I->sample_ev[2 * slot + 1]));
         samples.emplace_back(a.it, slot);
         HIPCHK(I->ctx, launch_cg_rest<T>(a, s));
       } else {
@@ -715,12 +710,12 @@ int hobak_inst_create(hobak_ctx* ctx, const double* rest_xyz, int32_t nv, const 
   ALLOC(col, (size_t)P.nslots);
   ALLOC(slot_row, (size_t)P.nslots);
   ALLOC(cptr, P.cptr.size());
-  ALLOC(clist, P.clist.size());
+  ALLOC(hdest, P.hdest.size());
   ALLOC(vptr, P.vptr.size());
-  ALLOC(vlist, P.vlist.size());
+  ALLOC(fdest, P.fdest.size());
   ALLOC(pinned, (size_t)nv);
-  ALLOCT(stage_h, (size_t)kStagePlanes * nt);
-  ALLOCT(stage_f, (size_t)12 * nt);
+  ALLOCT(stage_h, (size_t)9 * P.cptr[P.nslots]);
+  ALLOCT(stage_f, (size_t)3 * P.vptr[nv]);
   ALLOCT(A, 9 * (size_t)P.nslots);
   ALLOCT(D, 9 * (size_t)P.nslots);
   ALLOCT(E, 9 * (size_t)P.nslots);
@@ -762,9 +757,9 @@ int hobak_inst_create(hobak_ctx* ctx, const double* rest_xyz, int32_t nv, const 
       (rc = upload(I, I->col, P.col.data(), P.col.size())) ||
       (rc = upload(I, I->slot_row, P.slot_row.data(), P.slot_row.size())) ||
       (rc = upload(I, I->cptr, P.cptr.data(), P.cptr.size())) ||
-      (rc = upload(I, I->clist, P.clist.data(), P.clist.size())) ||
+      (rc = upload(I, I->hdest, P.hdest.data(), P.hdest.size())) ||
       (rc = upload(I, I->vptr, P.vptr.data(), P.vptr.size())) ||
-      (rc = upload(I, I->vlist, P.vlist.data(), P.vlist.size())))
+      (rc = upload(I, I->fdest, P.fdest.data(), P.fdest.size())))
     return bail(rc);
   if ((rc = setup_constant_parts<double>(I)) != HOBAK_OK) return bail(rc);
   if (hipStreamSynchronize(ctx->stream) != hipSuccess)
@@ -857,7 +852,7 @@ int hobak_eval_forces_hessian(hobak_inst* I, const double* u, double* forces, do
   TRY(upload(I, I->tmp64, u, (size_t)I->dofs));
   HIPCHK(I->ctx, launch_element<double>(element_args(I, I->tmp64, kvals != nullptr), s));
   if (forces) {
-    HIPCHK(I->ctx, launch_force_gather<double>(I->nv, I->nt, I->vptr, I->vlist, (const double*)I->stage_f,
+    HIPCHK(I->ctx, launch_force_gather<double>(I->nv, I->vptr, (const double*)I->stage_f,
                                                I->rhs64, s));
     HIPCHK(I->ctx, hipMemcpyAsync(forces, I->rhs64, sizeof(double) * I->dofs, hipMemcpyDeviceToHost, s));
   }

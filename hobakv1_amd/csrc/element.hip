@@ -14,7 +14,8 @@
 // is formed directly as  V_e * U * M_vw * U^T  with
 //   M_vw[a][a']  = sum_{b,b'} w_v[b] Hhat[(a,b),(a',b')] w_w[b'],   w = (G DmInv) V,
 // G the 4x3 map from corner positions to edge vectors.  That is the same matrix up to round-off,
-// at about a third of the flops, and it needs only ~40 live values per lane.
+// at about a third of the flops, and it needs only ~40 live values per lane.  Each block is
+// written straight to its place in the assembly's per-slot contribution list (hdest).
 #include "device_math.h"
 #include "kernels.h"
 
@@ -26,6 +27,7 @@ __global__ __launch_bounds__(128) void element_kernel(ElementArgs a) {
   if (t >= a.nt) return;
   const size_t nt = (size_t)a.nt;
   const int4 tv = a.tets[t];
+  (void)kStageBlocks;
   const int vid[4] = {tv.x, tv.y, tv.z, tv.w};
 
   // corner positions x = X + u (TET_MESH::setDisplacement) and edge matrix Ds
@@ -85,12 +87,14 @@ __global__ __launch_bounds__(128) void element_kernel(ElementArgs a) {
   }
   T* sf = (T*)a.stage_f;
 #pragma unroll
-  for (int v = 0; v < 4; v++)
+  for (int v = 0; v < 4; v++) {
+    T* dst = sf + 3 * (size_t)a.fdest[4 * (size_t)t + v];
 #pragma unroll
     for (int c = 0; c < 3; c++) {
       const T s = (P[c][0] * Wd[v][0] + P[c][1] * Wd[v][1]) + P[c][2] * Wd[v][2];
-      sf[(3 * v + c) * nt + t] = -vol * s;
+      dst[c] = -vol * s;
     }
+  }
   if (!a.with_hessian) return;
 
   SigmaHessian<T> h;
@@ -106,11 +110,11 @@ __global__ __launch_bounds__(128) void element_kernel(ElementArgs a) {
     for (int b = 0; b < 3; b++) w[v][b] = Wd[v][0] * V[0][b] + Wd[v][1] * V[1][b] + Wd[v][2] * V[2][b];
 
   T* sh = (T*)a.stage_h;
-  int lb = 0;
+  const int* hd = a.hdest + 16 * (size_t)t;
 #pragma unroll
   for (int v = 0; v < 4; v++) {
 #pragma unroll
-    for (int v2 = v; v2 < 4; v2++, lb++) {
+    for (int v2 = v; v2 < 4; v2++) {
       T M[3][3];
 #pragma unroll
       for (int p = 0; p < 3; p++)
@@ -131,13 +135,25 @@ __global__ __launch_bounds__(128) void element_kernel(ElementArgs a) {
       for (int c = 0; c < 3; c++)
 #pragma unroll
         for (int q = 0; q < 3; q++) UM[c][q] = U[c][0] * M[0][q] + U[c][1] * M[1][q] + U[c][2] * M[2][q];
+      T B[3][3];
 #pragma unroll
       for (int c = 0; c < 3; c++)
 #pragma unroll
-        for (int c2 = 0; c2 < 3; c2++) {
-          const T s = UM[c][0] * U[c2][0] + UM[c][1] * U[c2][1] + UM[c][2] * U[c2][2];
-          sh[(lb * 9 + 3 * c + c2) * nt + t] = vol * s;
-        }
+        for (int c2 = 0; c2 < 3; c2++)
+          B[c][c2] = vol * (UM[c][0] * U[c2][0] + UM[c][1] * U[c2][1] + UM[c][2] * U[c2][2]);
+      // block (v, v2) and, off the diagonal, its transpose as block (v2, v)
+      T* d0 = sh + 9 * (size_t)hd[4 * v + v2];
+#pragma unroll
+      for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int c2 = 0; c2 < 3; c2++) d0[3 * c + c2] = B[c][c2];
+      if (v2 != v) {
+        T* d1 = sh + 9 * (size_t)hd[4 * v2 + v];
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+#pragma unroll
+          for (int c2 = 0; c2 < 3; c2++) d1[3 * c + c2] = B[c2][c];
+      }
     }
   }
 }

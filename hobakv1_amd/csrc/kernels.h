@@ -4,8 +4,11 @@
 //   vectors        DOF-ordered AoS: entry 3*v + c (the reference's flat VECTOR order)
 //   tets           int4 per tet
 //   DmInv          9 planes of nt doubles (plane 3*r + c holds DmInv(r, c))       -> coalesced
-//   element stage  90 planes of nt (10 upper 3x3 blocks of the 12x12 stiffness) + 12 planes of
-//                  forces; plane index (block * 9 + 3 * a + b)
+//   element stage  written directly in the order the assembly reads it: the 3x3 block of vertex
+//                  pair (x, y) of tet t goes to stage_h[9 * hdest[16 t + 4 x + y] + 3 a + b], where
+//                  hdest is the position of that contribution in its slot's ascending-tet list,
+//                  so every slot sums one contiguous run; likewise force 3-vectors go to
+//                  stage_f[3 * fdest[4 t + x] + c] in each vertex's ascending-tet list
 //   block matrix   SELL-H, k-interleaved (H = 8, 16, 32 or 64 rows per slice, L = 64 / H lanes
 //                  per row): slice s owns slots [slice_off[s], slice_off[s+1]) (a multiple of 64),
 //                  and the k-th block of row r sits at slot = slice_off[r / H] + H * k + (r % H).
@@ -29,8 +32,7 @@ namespace hobak {
 
 constexpr int kDefaultSliceRows = 64;
 constexpr int kBlock = 256;
-constexpr int kStageBlocks = 10;
-constexpr int kStagePlanes = 90;
+constexpr int kStageBlocks = 16;
 
 // Scalars of one PCG solve, resident on the device so the loop runs without host round trips.
 // Two copies ping-pong: iteration `it` (counted by the host that queues the launches) reads
@@ -77,8 +79,10 @@ struct ElementArgs {
   const double* u;
   const double* dminv;  // 9 planes of nt
   const double* vol;
-  void* stage_h;        // 90 planes of nt, T
-  void* stage_f;        // 12 planes of nt, T
+  void* stage_h;        // 16 * 9 * nt values, T, in assembly order
+  void* stage_f;        // 4 * 3 * nt values, T, in force-gather order
+  const int* hdest;     // 16 per tet
+  const int* fdest;     // 4 per tet
   int with_hessian;
 };
 

@@ -11,8 +11,7 @@ struct AsmArgs {
   int nrows, nslots, nt;
   const int* slot_row;   // owning row of each slot, -1 for padding
   const int* col;
-  const int* cptr;       // nslots + 1
-  const int* clist;
+  const int* cptr;       // nslots + 1: slot s sums stage_h blocks [cptr[s], cptr[s + 1])
   const void* stage_h;   // T
   const void* D;         // T, 9 planes (mode 1)
   void* out;             // T, 9 planes
@@ -44,8 +43,7 @@ struct RhsArgs {
   const int* slice_off;
   const int* row_len;
   const int* col;
-  const int* vptr;
-  const int* vlist;
+  const int* vptr;       // vertex r sums stage_f 3-vectors [vptr[r], vptr[r + 1])
   const void* stage_f;   // T
   const void* Mvec;      // T: E (BEP) or C (BDF_1)
   const double* vel;
@@ -117,12 +115,12 @@ hipError_t launch_cg_init(const InitArgs& a, hipStream_t s);
 template <typename T>
 hipError_t launch_cg_iteration(const CgArgs& a, hipStream_t s);
 template <typename T>
-hipError_t launch_cg_spmv(const CgArgs& a, hipStream_t s);
+hipError_t launch_cg_spmv(const CgArgs& a, hipStream_t s, hipEvent_t start = nullptr,
+                          hipEvent_t stop = nullptr);
 template <typename T>
 hipError_t launch_cg_rest(const CgArgs& a, hipStream_t s);
 template <typename T>
-hipError_t launch_force_gather(int nv, int nt, const int* vptr, const int* vlist, const T* stage_f,
-                               double* R, hipStream_t s);
+hipError_t launch_force_gather(int nv, const int* vptr, const T* stage_f, double* R, hipStream_t s);
 hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
 // Workgroup counts of the kernels that publish partial sums (size of each Partials array).
 int rhs_grid(int nrows);

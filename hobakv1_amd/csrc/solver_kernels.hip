@@ -14,6 +14,8 @@
 // fixed order.  There are no floating-point atomics anywhere.
 #include "solver_kernels.h"
 
+#include <hip/hip_ext.h>
+
 #include <cfloat>
 
 namespace hobak {
@@ -154,25 +156,16 @@ __global__ __launch_bounds__(kBlock) void assemble_kernel(AsmArgs a) {
   const int r = a.slot_row[slot];
   if (r < 0) return;
   const int c = a.col[slot];
-  const T* stage = (const T*)a.stage_h;
-  const size_t nt = (size_t)a.nt;
   const size_t ns = (size_t)a.nslots;
   T acc[9];
 #pragma unroll
   for (int e = 0; e < 9; e++) acc[e] = T(0);
-  const int q1 = a.cptr[slot + 1];
-  for (int q = a.cptr[slot]; q < q1; q++) {
-    const int code = a.clist[q];
-    const T* src = stage + (size_t)(code & 15) * 9 * nt + (size_t)(code >> 5);
-    if (code & 16) {
+  // the slot's contributions are contiguous and in ascending tet order
+  const T* src = (const T*)a.stage_h + 9 * (size_t)a.cptr[slot];
+  const T* end = (const T*)a.stage_h + 9 * (size_t)a.cptr[slot + 1];
+  for (; src < end; src += 9) {
 #pragma unroll
-      for (int i = 0; i < 3; i++)
-#pragma unroll
-        for (int j = 0; j < 3; j++) acc[3 * i + j] += src[(3 * j + i) * nt];
-    } else {
-#pragma unroll
-      for (int e = 0; e < 9; e++) acc[e] += src[e * nt];
-    }
+    for (int e = 0; e < 9; e++) acc[e] += src[e];
   }
   T* out = (T*)a.out;
   if (a.mode == 0) {
@@ -246,14 +239,10 @@ __global__ __launch_bounds__(kBlock) void rhs_init_kernel(RhsArgs a) {
   double part[2] = {0.0, 0.0};
   if (r < a.nrows) {
     const T* sf = (const T*)a.stage_f;
-    const size_t nt = (size_t)a.nt;
     double R[3] = {0.0, 0.0, 0.0};
     for (int q = a.vptr[r]; q < a.vptr[r + 1]; q++) {
-      const int code = a.vlist[q];
-      const size_t tet = (size_t)(code >> 2);
-      const int loc = code & 3;
 #pragma unroll
-      for (int c = 0; c < 3; c++) R[c] += (double)sf[(3 * loc + c) * nt + tet];
+      for (int c = 0; c < 3; c++) R[c] += (double)sf[3 * (size_t)q + c];
     }
     if (a.ground_on) {
       const double y = a.rest[3 * (size_t)r + 1] + a.u[3 * (size_t)r + 1];
@@ -518,17 +507,13 @@ __global__ __launch_bounds__(kBlock) void cg_p_kernel(CgArgs a) {
 // computeHyperelasticForces' scatter as a per-vertex gather in ascending tet order
 // (TET_MESH.cpp:756-767); output in FP64.
 template <typename T>
-__global__ void force_gather_kernel(int nv, int nt, const int* vptr, const int* vlist,
-                                    const T* stage_f, double* R) {
+__global__ void force_gather_kernel(int nv, const int* vptr, const T* stage_f, double* R) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nv) return;
   double s[3] = {0.0, 0.0, 0.0};
   for (int q = vptr[r]; q < vptr[r + 1]; q++) {
-    const int code = vlist[q];
-    const size_t tet = (size_t)(code >> 2);
-    const int loc = code & 3;
 #pragma unroll
-    for (int c = 0; c < 3; c++) s[c] += (double)stage_f[(3 * loc + c) * (size_t)nt + tet];
+    for (int c = 0; c < 3; c++) s[c] += (double)stage_f[3 * (size_t)q + c];
   }
 #pragma unroll
   for (int c = 0; c < 3; c++) R[3 * (size_t)r + c] = s[c];
@@ -601,16 +586,24 @@ hipError_t launch_cg_init(const InitArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
+// With events, the launch carries them in its own dispatch (kernel start and end), so the
+// measured interval is the kernel's execution, not the queue's.
 template <typename T>
-hipError_t launch_cg_spmv(const CgArgs& a, hipStream_t s) {
+hipError_t launch_cg_spmv(const CgArgs& a, hipStream_t s, hipEvent_t start, hipEvent_t stop) {
   const dim3 g(spmv_grid(a.nrows, a.slice_shift)), b(kBlock);
+#define SPMV_LAUNCH(H)                                                                \
+  if (start)                                                                          \
+    hipExtLaunchKernelGGL((cg_spmv_kernel<T, H>), g, b, 0, s, start, stop, 0, a);     \
+  else                                                                                \
+    hipLaunchKernelGGL((cg_spmv_kernel<T, H>), g, b, 0, s, a);
   switch (a.slice_shift) {
-    case 3: hipLaunchKernelGGL((cg_spmv_kernel<T, 8>), g, b, 0, s, a); break;
-    case 4: hipLaunchKernelGGL((cg_spmv_kernel<T, 16>), g, b, 0, s, a); break;
-    case 5: hipLaunchKernelGGL((cg_spmv_kernel<T, 32>), g, b, 0, s, a); break;
-    case 6: hipLaunchKernelGGL((cg_spmv_kernel<T, 64>), g, b, 0, s, a); break;
+    case 3: SPMV_LAUNCH(8) break;
+    case 4: SPMV_LAUNCH(16) break;
+    case 5: SPMV_LAUNCH(32) break;
+    case 6: SPMV_LAUNCH(64) break;
     default: return hipErrorInvalidValue;
   }
+#undef SPMV_LAUNCH
   return hipGetLastError();
 }
 
@@ -631,10 +624,9 @@ hipError_t launch_cg_iteration(const CgArgs& a, hipStream_t s) {
 }
 
 template <typename T>
-hipError_t launch_force_gather(int nv, int nt, const int* vptr, const int* vlist, const T* stage_f,
-                               double* R, hipStream_t s) {
-  hipLaunchKernelGGL((force_gather_kernel<T>), dim3(grid_for(nv)), dim3(kBlock), 0, s, nv, nt, vptr,
-                     vlist, stage_f, R);
+hipError_t launch_force_gather(int nv, const int* vptr, const T* stage_f, double* R, hipStream_t s) {
+  hipLaunchKernelGGL((force_gather_kernel<T>), dim3(grid_for(nv)), dim3(kBlock), 0, s, nv, vptr,
+                     stage_f, R);
   return hipGetLastError();
 }
 
@@ -650,10 +642,9 @@ hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s) 
   template hipError_t launch_constant_parts<T>(const ConstArgs&, hipStream_t);           \
   template hipError_t launch_rhs_init<T>(const RhsArgs&, hipStream_t);                   \
   template hipError_t launch_cg_iteration<T>(const CgArgs&, hipStream_t);                \
-  template hipError_t launch_cg_spmv<T>(const CgArgs&, hipStream_t);                     \
+  template hipError_t launch_cg_spmv<T>(const CgArgs&, hipStream_t, hipEvent_t, hipEvent_t); \
   template hipError_t launch_cg_rest<T>(const CgArgs&, hipStream_t);                     \
-  template hipError_t launch_force_gather<T>(int, int, const int*, const int*, const T*, \
-                                             double*, hipStream_t);
+  template hipError_t launch_force_gather<T>(int, const int*, const T*, double*, hipStream_t);
 INST(double)
 INST(float)
 #undef INST
