@@ -176,7 +176,8 @@ struct hobak_inst {
   unsigned char* pinned = nullptr;
   // device, real type T
   void *stage_h = nullptr, *stage_f = nullptr, *A = nullptr, *D = nullptr, *E = nullptr, *C = nullptr;
-  void *minv = nullptr, *r = nullptr, *x = nullptr, *z = nullptr, *t = nullptr, *p = nullptr;
+  void *minv = nullptr, *r = nullptr, *x = nullptr, *z = nullptr, *t = nullptr;
+  void *p0 = nullptr, *p1 = nullptr;
   CgScalars* sc = nullptr;       // two ping-pong sets
   CgScalars* sc_host = nullptr;  // pinned host copy of one set
   // per-workgroup partial sums: rhs -> init, SpMV -> update, update -> p
@@ -310,7 +311,8 @@ CgArgs cg_args(hobak_inst* I) {
   a.x = I->x;
   a.r = I->r;
   a.z = I->z;
-  a.p = I->p;
+  a.p0 = I->p0;
+  a.p1 = I->p1;
   a.t = I->t;
   a.minv = I->minv;
   a.precond = I->prm.precond;
@@ -362,7 +364,7 @@ RhsArgs rhs_args(hobak_inst* I, int bdf) {
   a.r = I->r;
   a.x = I->x;
   a.z = I->z;
-  a.p = I->p;
+  a.p0 = I->p0;
   a.minv = I->minv;
   a.precond = I->prm.precond;
   a.out = Partials{I->part_rhs_a, I->part_rhs_b, I->n_rhs};
@@ -724,7 +726,8 @@ int hobak_inst_create(hobak_ctx* ctx, const double* rest_xyz, int32_t nv, const 
   ALLOCT(r, 3 * np);
   ALLOCT(x, 3 * np);
   ALLOCT(z, 3 * np);
-  ALLOCT(p, 3 * np);
+  ALLOCT(p0, 3 * np);
+  ALLOCT(p1, 3 * np);
   ALLOCT(t, 3 * np);
   ALLOC(sc, 2);
   I->n_rhs = rhs_grid(nv);

@@ -64,7 +64,7 @@ struct RhsArgs {
   void* r;
   void* x;
   void* z;
-  void* p;
+  void* p0;              // zeroed: iteration 0 forms p = z + 0 * p0
   const void* minv;
   int precond;
   Partials out;          // per workgroup: a = |S b|^2, b = r.z
@@ -89,14 +89,15 @@ struct CgArgs {
   void* x;
   void* r;
   void* z;
-  void* p;
+  void* p0;              // p double buffer: iteration it reads p[it & 1], writes p[(it + 1) & 1]
+  void* p1;
   void* t;
   const void* minv;
   int precond;
   CgScalars* sc;         // two sets, see CgScalars
-  int it;                // host iteration counter: this launch reads sc[it & 1]
+  int it;                // host iteration counter
   Partials pt;           // SpMV -> update: p.t
-  Partials rz;           // update -> p: a = r.r, b = r.z
+  Partials rz;           // update -> next SpMV: a = r.r, b = r.z
 };
 
 hipError_t launch_begin_step(int nv, double* u, const double* v, double* uOld, double* vOld,

@@ -230,8 +230,8 @@ __device__ __forceinline__ void apply_precond(const T* minv, int precond, int r,
 }
 
 // Per block row: R = ascending-tet gather of staged forces; b = (E v + R) + f_ext (BEP) or
-// b = C vb + ((M a + R) + f_ext) (BDF_1); rhs = S b.  Then x = 0, r = rhs, z = M^-1 r, p = 0 with
-// beta = 0 (so the first SpMV forms p = z), and the grid sums |rhs|^2 and r.z seed Eigen's CG
+// b = C vb + ((M a + R) + f_ext) (BDF_1); rhs = S b.  Then x = 0, r = rhs, z = M^-1 r and p = 0
+// (the first SpMV forms p = z + 0 p), and the partial sums of |rhs|^2 and r.z seed Eigen's CG
 // (ConjugateGradient.h:43-66).
 template <typename T>
 __global__ __launch_bounds__(kBlock) void rhs_init_kernel(RhsArgs a) {
@@ -295,7 +295,7 @@ __global__ __launch_bounds__(kBlock) void rhs_init_kernel(RhsArgs a) {
         ((T*)a.r)[i] = rv[c];
         ((T*)a.x)[i] = T(0);
         ((T*)a.z)[i] = zv[c];
-        ((T*)a.p)[i] = zv[c];
+        ((T*)a.p0)[i] = T(0);
         part[1] += (double)rv[c] * (double)zv[c];
       }
     }
@@ -332,28 +332,86 @@ __global__ __launch_bounds__(kBlock) void cg_init_kernel(InitArgs a) {
 }
 
 // ---------------------------------------------------------------------------------------------
-// PCG iteration `it`: three kernels, all reading the scalar set S = sc[it & 1]
-//   spmv:    t = A p; partials of p.t
-//   update:  alpha = S.absNew / (p.t); x += alpha p; r -= alpha t; z = M^-1 r; partials of
-//            r.r and r.z
-//   p:       |r|^2 < threshold -> done (the converging pass is not counted); otherwise
-//            beta = r.z / S.absNew, p = z + beta p, iters + 1; workgroup 0 writes the next set
-//            sc[(it + 1) & 1] (a finished set is carried forward unchanged)
+// PCG iteration `it`: two kernels (Eigen's loop body, ConjugateGradient.h:69-88, regrouped)
+//   spmv(it):   S = sc[it & 1].  For it > 0 it first finishes iteration it - 1: from the update
+//               kernel's partials, |r|^2 < threshold -> done (the converging pass is not
+//               counted); else beta = r.z / S.absNew, absNew = r.z, iters + 1 (done at
+//               maxIters).  Workgroup 0 writes that as the next set sc[(it + 1) & 1] (a
+//               finished set is carried forward).  Then p_new = z + beta p_old, formed on the fly
+//               at every column (p double-buffered by parity), t = A p_new, partials of p.t.
+//   update(it): S' = sc[(it + 1) & 1]; alpha = S'.absNew / (p.t); x += alpha p_new;
+//               r -= alpha t; z = M^-1 r; partials of r.r and r.z.
 // Kernels of a finished solve return at once, so chunks of iterations can be queued ahead and
-// the host polls the flag between chunks.
+// the host polls the flag between chunks.  Scalars are read and written field by field: a
+// by-value copy of the struct becomes a per-lane LDS buffer that costs microseconds per launch.
 // ---------------------------------------------------------------------------------------------
 // One wavefront per H-row slice, L = 64 / H lanes per row: lane = H * kk + rr handles the row's
 // blocks k = kk, kk + L, ...; a row's L partial sums are combined with cross-lane adds.
 template <typename T, int H>
 __global__ __launch_bounds__(kBlock) void cg_spmv_kernel(CgArgs a) {
-  if (a.sc[a.it & 1].done) return;  // a single field load
+  const CgScalars* S = &a.sc[a.it & 1];
+  CgScalars* N = &a.sc[(a.it + 1) & 1];
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  const int done = S->done;
+  const double absOld = S->absNew;
+  const int iters = S->iters;
+  if (lead) {
+    N->rhsNorm2 = S->rhsNorm2;
+    N->threshold = S->threshold;
+    N->absNew = absOld;
+    N->alpha = S->alpha;
+    N->beta = S->beta;
+    N->resNorm2 = S->resNorm2;
+    N->maxR = S->maxR;
+    N->iters = iters;
+    N->maxIters = S->maxIters;
+    N->done = done;
+    N->status = S->status;
+  }
+  if (done) return;
+  T beta = T(0);
+  if (a.it > 0) {
+    double rr, rz;
+    gather_partials(a.rz, true, rr, rz);
+    if (!(rr == rr) || rr > DBL_MAX || !(rz == rz)) {
+      if (lead) {
+        N->resNorm2 = rr;
+        N->done = 1;
+        N->status = 2;
+      }
+      return;
+    }
+    if (rr < S->threshold) {
+      if (lead) {
+        N->resNorm2 = rr;
+        N->done = 1;
+      }
+      return;
+    }
+    const double b = rz / absOld;
+    const int maxIters = S->maxIters;
+    if (lead) {
+      N->resNorm2 = rr;
+      N->beta = b;
+      N->absNew = rz;
+      N->iters = iters + 1;
+      if (iters + 1 >= maxIters) {
+        N->done = 1;
+        N->status = 1;
+      }
+    }
+    if (iters + 1 >= maxIters) return;
+    beta = (T)b;
+  }
   constexpr int L = 64 / H;
   const int slice = (int)((blockIdx.x * (unsigned)blockDim.x + threadIdx.x) >> 6);
   const int lane = threadIdx.x & 63;
   const int kk = lane / H;
   const int rr = lane % H;
   const int r = slice * H + rr;
-  const T* p = (const T*)a.p;
+  const T* pold = (const T*)((a.it & 1) ? a.p1 : a.p0);
+  T* pnew = (T*)((a.it & 1) ? a.p0 : a.p1);
+  const T* z = (const T*)a.z;
   const T* A = (const T*)a.A;
   const size_t ns = (size_t)a.nslots;
   T s0 = T(0), s1 = T(0), s2 = T(0);
@@ -364,7 +422,9 @@ __global__ __launch_bounds__(kBlock) void cg_spmv_kernel(CgArgs a) {
     for (int k = kk; k < len; k += L) {
       const int slot = base + k * H;
       const size_t c = 3 * (size_t)a.col[slot];
-      const T p0 = p[c], p1 = p[c + 1], p2 = p[c + 2];
+      const T p0 = z[c] + beta * pold[c];
+      const T p1 = z[c + 1] + beta * pold[c + 1];
+      const T p2 = z[c + 2] + beta * pold[c + 2];
       s0 += A[0 * ns + slot] * p0 + A[1 * ns + slot] * p1 + A[2 * ns + slot] * p2;
       s1 += A[3 * ns + slot] * p0 + A[4 * ns + slot] * p1 + A[5 * ns + slot] * p2;
       s2 += A[6 * ns + slot] * p0 + A[7 * ns + slot] * p1 + A[8 * ns + slot] * p2;
@@ -383,7 +443,13 @@ __global__ __launch_bounds__(kBlock) void cg_spmv_kernel(CgArgs a) {
     t[i] = s0;
     t[i + 1] = s1;
     t[i + 2] = s2;
-    pt = (double)p[i] * (double)s0 + (double)p[i + 1] * (double)s1 + (double)p[i + 2] * (double)s2;
+    const T q0 = z[i] + beta * pold[i];
+    const T q1 = z[i + 1] + beta * pold[i + 1];
+    const T q2 = z[i + 2] + beta * pold[i + 2];
+    pnew[i] = q0;
+    pnew[i + 1] = q1;
+    pnew[i + 2] = q2;
+    pt = (double)q0 * (double)s0 + (double)q1 * (double)s1 + (double)q2 * (double)s2;
   }
   publish(pt, 0.0, a.pt, false);
 }
@@ -394,7 +460,7 @@ __global__ __launch_bounds__(kBlock) void cg_spmv_kernel(CgArgs a) {
 // workgroup that later re-reduces them) small.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void cg_update_kernel(CgArgs a) {
-  const CgScalars* S = &a.sc[a.it & 1];
+  const CgScalars* S = &a.sc[(a.it + 1) & 1];
   if (S->done) return;
   const double absNew = S->absNew;
   double pt, unused;
@@ -405,7 +471,7 @@ __global__ __launch_bounds__(kBlock) void cg_update_kernel(CgArgs a) {
   const int b = lane - c;
   const int nwaves = (a.nrows + 20) / 21;
   const int stride = (int)(gridDim.x * (kBlock / 64));
-  const T* p = (const T*)a.p;
+  const T* p = (const T*)((a.it & 1) ? a.p0 : a.p1);  // the p_new this iteration's SpMV wrote
   const T* mi = (const T*)a.minv;
   T* x = (T*)a.x;
   T* rr = (T*)a.r;
@@ -440,68 +506,6 @@ __global__ __launch_bounds__(kBlock) void cg_update_kernel(CgArgs a) {
     }
   }
   publish(rr2, rz, a.rz, true);
-}
-
-// Scalars are read and written field by field: a by-value copy of the struct is turned by the
-// compiler into a per-lane LDS buffer that costs microseconds per launch.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void cg_p_kernel(CgArgs a) {
-  const CgScalars* S = &a.sc[a.it & 1];
-  CgScalars* N = &a.sc[(a.it + 1) & 1];
-  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
-  const int done = S->done;
-  const double absOld = S->absNew;
-  const int iters = S->iters;
-  const int maxIters = S->maxIters;
-  const double threshold = S->threshold;
-  if (lead) {  // carry the set forward; the fields that change are overwritten below
-    N->rhsNorm2 = S->rhsNorm2;
-    N->threshold = threshold;
-    N->absNew = absOld;
-    N->alpha = S->alpha;
-    N->beta = S->beta;
-    N->resNorm2 = S->resNorm2;
-    N->maxR = S->maxR;
-    N->iters = iters;
-    N->maxIters = maxIters;
-    N->done = done;
-    N->status = S->status;
-  }
-  if (done) return;
-  double rr, rz;
-  gather_partials(a.rz, true, rr, rz);
-  if (!(rr == rr) || rr > DBL_MAX || !(rz == rz)) {
-    if (lead) {
-      N->resNorm2 = rr;
-      N->done = 1;
-      N->status = 2;
-    }
-    return;
-  }
-  if (rr < threshold) {  // converged: Eigen breaks before z, beta and the count (CG.h:77-79)
-    if (lead) {
-      N->resNorm2 = rr;
-      N->done = 1;
-    }
-    return;
-  }
-  const double beta = rz / absOld;
-  if (lead) {
-    N->resNorm2 = rr;
-    N->beta = beta;
-    N->absNew = rz;
-    N->iters = iters + 1;
-    if (iters + 1 >= maxIters) {
-      N->done = 1;
-      N->status = 1;
-    }
-  }
-  T* p = (T*)a.p;
-  const T* z = (const T*)a.z;
-  const T tb = (T)beta;
-  const int ndof = 3 * a.nrows;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ndof; i += gridDim.x * blockDim.x)
-    p[i] = z[i] + tb * p[i];
 }
 
 // computeHyperelasticForces' scatter as a per-vertex gather in ascending tet order
@@ -564,16 +568,13 @@ hipError_t launch_constant_parts(const ConstArgs& a, hipStream_t s) {
 
 int rhs_grid(int nrows) { return grid_for(nrows); }
 int spmv_grid(int nrows, int sh) { return grid_for((long)((nrows + (1 << sh) - 1) >> sh) * 64); }
-// The update and p kernels stride over their work with at most kVectorBlocks workgroups.
+// The update kernel strides over its work with at most kVectorBlocks workgroups.
 constexpr int kVectorBlocks = 512;
 int update_grid(int nrows) {
   const int g = grid_for((long)((nrows + 20) / 21) * 64);
   return g < kVectorBlocks ? g : kVectorBlocks;
 }
-static inline int p_grid(int nrows) {
-  const int g = grid_for(3L * nrows);
-  return g < kVectorBlocks ? g : kVectorBlocks;
-}
+
 
 template <typename T>
 hipError_t launch_rhs_init(const RhsArgs& a, hipStream_t s) {
@@ -607,12 +608,10 @@ hipError_t launch_cg_spmv(const CgArgs& a, hipStream_t s, hipEvent_t start, hipE
   return hipGetLastError();
 }
 
-// The p kernel's grid must cover every DOF and the update kernel's grid every row; the pt / rz
-// partial arrays hold spmv_grid / update_grid entries.
+// The pt / rz partial arrays hold spmv_grid / update_grid entries.
 template <typename T>
 hipError_t launch_cg_rest(const CgArgs& a, hipStream_t s) {
   hipLaunchKernelGGL((cg_update_kernel<T>), dim3(update_grid(a.nrows)), dim3(kBlock), 0, s, a);
-  hipLaunchKernelGGL((cg_p_kernel<T>), dim3(p_grid(a.nrows)), dim3(kBlock), 0, s, a);
   return hipGetLastError();
 }
 
