@@ -168,7 +168,7 @@ struct hobak_inst {
   double *rest = nullptr, *u = nullptr, *v = nullptr, *u_old = nullptr, *v_old = nullptr;
   double *dminv = nullptr, *vol = nullptr, *one_ring = nullptr, *mass = nullptr;
   double *pin_target = nullptr, *rhs64 = nullptr, *tmp64 = nullptr, *x64 = nullptr, *res64 = nullptr;
-  double* kneg64 = nullptr;
+  void* kneg = nullptr;  // assembled -K_rest, T
   int4* tets = nullptr;
   int *slice_off = nullptr, *row_len = nullptr, *col = nullptr, *slot_row = nullptr;
   int *cptr = nullptr, *hdest = nullptr;
@@ -178,6 +178,8 @@ struct hobak_inst {
   void *stage_h = nullptr, *stage_f = nullptr, *A = nullptr, *D = nullptr, *E = nullptr, *C = nullptr;
   void *minv = nullptr, *r = nullptr, *x = nullptr, *z = nullptr, *t = nullptr;
   void *p0 = nullptr, *p1 = nullptr;
+  RefineState* rstate = nullptr;       // FP64 refinement state (precision form)
+  RefineState* rstate_host = nullptr;
   CgScalars* sc = nullptr;       // two ping-pong sets
   CgScalars* sc_host = nullptr;  // pinned host copy of one set
   // per-workgroup partial sums: rhs -> init, SpMV -> update, update -> p
@@ -245,7 +247,7 @@ size_t estimate_bytes(int nv, int nt, int nnzb_est, int nslots_est, int precisio
   size_t b = 0;
   b += (size_t)nv * 3 * 8 * 10;                 // rest, u, v, olds, target, rhs, tmp, x64, res64, mass
   b += (size_t)nt * (16 + 9 * 8 + 8);           // tets, dminv, vol
-  b += (size_t)nslots_est * (4 + 9 * 8);        // col + kneg64
+  b += (size_t)nslots_est * (4 + 4 + 9 * ts);   // col + slot_row + kneg
   b += (size_t)nslots_est * 9 * ts * 4;         // A, D, E, C
   b += (size_t)nslots_est * 4 + (size_t)nt * 20 * 4;  // cptr + hdest + fdest
   b += (size_t)nt * (16 * 9 + 12) * ts;         // staging
@@ -332,6 +334,33 @@ InitArgs init_args(hobak_inst* I) {
   return a;
 }
 
+RefineArgs refine_args(hobak_inst* I, bool residual_parts) {
+  RefineArgs a{};
+  a.nrows = I->nv;
+  a.nslots = I->P.nslots;
+  a.slice_shift = I->P.shift;
+  a.slice_off = I->slice_off;
+  a.row_len = I->row_len;
+  a.col = I->col;
+  a.A = I->A;
+  a.rhs64 = I->rhs64;
+  a.x64 = I->x64;
+  a.res64 = I->res64;
+  a.r = I->r;
+  a.x = I->x;
+  a.z = I->z;
+  a.p0 = I->p0;
+  a.minv = I->minv;
+  a.precond = I->prm.precond;
+  // the inner-solve seed publishes one partial per rhs-shaped workgroup, the residual one per
+  // SpMV-shaped workgroup; both reuse arrays that are consumed before they are overwritten
+  a.out = residual_parts ? Partials{I->part_pt, nullptr, I->n_spmv}
+                         : Partials{I->part_rhs_a, I->part_rhs_b, I->n_rhs};
+  a.state = I->rstate;
+  a.tol = I->prm.pcg_tol > 0 ? I->prm.pcg_tol : 2.220446049250313e-16;
+  return a;
+}
+
 RhsArgs rhs_args(hobak_inst* I, int bdf) {
   RhsArgs a{};
   a.nrows = I->nv;
@@ -376,13 +405,13 @@ int setup_constant_parts(hobak_inst* I) {
   hipStream_t s = I->ctx->stream;
   // rest-state stiffness (u = 0): TIMESTEPPER::buildRayleighDampingMatrix (TIMESTEPPER.cpp:213-236)
   HIPCHK(I->ctx, hipMemsetAsync(I->tmp64, 0, sizeof(double) * I->dofs, s));
-  HIPCHK(I->ctx, launch_element<double>(element_args(I, I->tmp64, 1), s));
-  HIPCHK(I->ctx, launch_assemble<double>(asm_args(I, 0, I->kneg64), s));
+  HIPCHK(I->ctx, launch_element<T>(element_args(I, I->tmp64, 1), s));
+  HIPCHK(I->ctx, launch_assemble<T>(asm_args(I, 0, I->kneg), s));
   ConstArgs c{};
   c.nslots = I->P.nslots;
   c.slot_row = I->slot_row;
   c.col = I->col;
-  c.kneg = I->kneg64;
+  c.kneg = I->kneg;
   c.mass = I->mass;
   c.ra = I->prm.rayleigh_alpha;
   c.rb = I->prm.rayleigh_beta;
@@ -450,12 +479,49 @@ int run_pcg(hobak_inst* I) {
   return HOBAK_OK;
 }
 
+// FP32 PCG inside FP64 iterative refinement: returns inner iterations, sweeps, status.
+template <typename T>
+int solve_refined(hobak_inst* I, int& iters, int& sweeps, double& err, int& status) {
+  hipStream_t s = I->ctx->stream;
+  hobak_ctx* ctx = I->ctx;
+  const int max_sweeps = 32;
+  iters = 0;
+  sweeps = 0;
+  status = HOBAK_OK;
+  const RefineArgs ri = refine_args(I, false);
+  const RefineArgs rr = refine_args(I, true);
+  HIPCHK(ctx, launch_refine_begin(ri, Partials{I->part_rhs_a, nullptr, I->n_rhs}, s));
+  HIPCHK(ctx, hipMemcpyAsync(I->rstate_host, I->rstate, sizeof(RefineState), hipMemcpyDeviceToHost, s));
+  HIPCHK(ctx, hipStreamSynchronize(s));
+  InitArgs ia = init_args(I);
+  ia.tol = I->prm.refine_inner_tol > 0 ? I->prm.refine_inner_tol : 1e-4;
+  while (!I->rstate_host->done && sweeps < max_sweeps) {
+    HIPCHK(ctx, launch_refine_init<T>(ri, s));
+    HIPCHK(ctx, launch_cg_init(ia, s));
+    TRY(run_pcg<T>(I));
+    iters += I->sc_host->iters;
+    if (I->sc_host->status == 2) status = HOBAK_E_NONFINITE;
+    HIPCHK(ctx, launch_refine_residual<T>(rr, s));
+    HIPCHK(ctx, launch_refine_check(rr, s));
+    HIPCHK(ctx, hipMemcpyAsync(I->rstate_host, I->rstate, sizeof(RefineState), hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipStreamSynchronize(s));
+    sweeps++;
+    if (status != HOBAK_OK) break;
+  }
+  const RefineState& h = *I->rstate_host;
+  err = h.b2 > 0 ? std::sqrt(h.res2 / h.b2) : 0.0;
+  if (h.status == 2) status = HOBAK_E_NONFINITE;
+  else if (!h.done && status == HOBAK_OK) status = HOBAK_E_NOT_CONVERGED;
+  return HOBAK_OK;
+}
+
 template <typename T>
 int step_impl(hobak_inst* I, hobak_step_stats* st) {
   hipStream_t s = I->ctx->stream;
   hobak_ctx* ctx = I->ctx;
   const int N = std::max(1, std::min(I->prm.newton_iters, HOBAK_MAX_NEWTON));
   const int bdf = N > 1;
+  const bool refine = I->prm.precision == HOBAK_PRECISION_FP32_REFINE;
   const double invDt = 1.0 / I->prm.dt;
   double ms[5] = {0, 0, 0, 0, 0};
   TRY(check_partials(I));
@@ -470,12 +536,30 @@ int step_impl(hobak_inst* I, hobak_step_stats* st) {
     HIPCHK(ctx, hipEventRecord(I->ev[1], s));
     HIPCHK(ctx, launch_assemble<T>(asm_args(I, 1, I->A), s));
     HIPCHK(ctx, hipEventRecord(I->ev[2], s));
-    HIPCHK(ctx, launch_rhs_init<T>(rhs_args(I, bdf), s));
-    HIPCHK(ctx, launch_cg_init(init_args(I), s));
-    HIPCHK(ctx, hipEventRecord(I->ev[3], s));
-    TRY(run_pcg<T>(I));
-    HIPCHK(ctx, hipEventRecord(I->ev[4], s));
-    HIPCHK(ctx, launch_apply_dx<T>(I->dofs, I->u, (const T*)I->x, s));
+    RhsArgs ra = rhs_args(I, bdf);
+    ra.init_cg = refine ? 0 : 1;
+    HIPCHK(ctx, launch_rhs_init<T>(ra, s));
+    int iters = 0, sweeps = 0, st_solve = HOBAK_OK;
+    double err = 0.0;
+    if (refine) {
+      HIPCHK(ctx, hipEventRecord(I->ev[3], s));
+      TRY(solve_refined<T>(I, iters, sweeps, err, st_solve));
+      maxR = I->rstate_host->b2;
+      HIPCHK(ctx, hipEventRecord(I->ev[4], s));
+      HIPCHK(ctx, launch_apply_dx<double>(I->dofs, I->u, I->x64, s));
+    } else {
+      HIPCHK(ctx, launch_cg_init(init_args(I), s));
+      HIPCHK(ctx, hipEventRecord(I->ev[3], s));
+      TRY(run_pcg<T>(I));
+      HIPCHK(ctx, hipEventRecord(I->ev[4], s));
+      HIPCHK(ctx, launch_apply_dx<T>(I->dofs, I->u, (const T*)I->x, s));
+      const CgScalars& h = *I->sc_host;
+      maxR = h.maxR;
+      iters = h.iters;
+      err = h.rhsNorm2 > 0 ? std::sqrt(h.resNorm2 / h.rhsNorm2) : 0.0;
+      if (h.status == 2) st_solve = HOBAK_E_NONFINITE;
+      else if (h.status == 1) st_solve = HOBAK_E_NOT_CONVERGED;
+    }
     float e01, e12, e23, e34;
     HIPCHK(ctx, hipEventSynchronize(I->ev[4]));
     (void)hipEventElapsedTime(&e01, I->ev[0], I->ev[1]);
@@ -486,18 +570,16 @@ int step_impl(hobak_inst* I, hobak_step_stats* st) {
     ms[1] += e12;
     ms[2] += e23;
     ms[3] += e34;
-    const CgScalars& h = *I->sc_host;
-    maxR = h.maxR;
     if (st) {
-      st->pcg_iters[done] = h.iters;
-      st->refine_iters[done] = 0;
-      st->pcg_error[done] = h.rhsNorm2 > 0 ? std::sqrt(h.resNorm2 / h.rhsNorm2) : 0.0;
-      st->newton_residual[done] = h.maxR;
+      st->pcg_iters[done] = iters;
+      st->refine_iters[done] = sweeps;
+      st->pcg_error[done] = err;
+      st->newton_residual[done] = maxR;
     }
-    I->prev_iters = h.iters;
-    total += h.iters;
-    if (h.status == 2) status = HOBAK_E_NONFINITE;
-    else if (h.status == 1 && status == HOBAK_OK) status = HOBAK_E_NOT_CONVERGED;
+    I->prev_iters = refine ? -1 : iters;
+    total += iters;
+    if (st_solve == HOBAK_E_NONFINITE) status = HOBAK_E_NONFINITE;
+    else if (st_solve == HOBAK_E_NOT_CONVERGED && status == HOBAK_OK) status = HOBAK_E_NOT_CONVERGED;
     done++;
   }
   HIPCHK(ctx, hipEventRecord(I->ev[0], s));
@@ -524,6 +606,7 @@ void free_inst(hobak_inst* I) {
   }
   I->allocs.clear();
   if (I->sc_host) (void)hipHostFree(I->sc_host);
+  if (I->rstate_host) (void)hipHostFree(I->rstate_host);
   for (auto& e : I->sample_ev)
     if (e) (void)hipEventDestroy(e);
   I->sample_ev.clear();
@@ -615,8 +698,8 @@ int hobak_inst_create(hobak_ctx* ctx, const double* rest_xyz, int32_t nv, const 
   if (!(params->dt > 0)) return fail(ctx, HOBAK_E_INVALID, "dt must be positive");
   if (params->precond != HOBAK_PRECOND_JACOBI && params->precond != HOBAK_PRECOND_BLOCK_JACOBI)
     return fail(ctx, HOBAK_E_INVALID, "unknown preconditioner");
-  if (params->precision != HOBAK_PRECISION_FP64)
-    return fail(ctx, HOBAK_E_INVALID, "precision form not available on this entry point");
+  if (params->precision != HOBAK_PRECISION_FP64 && params->precision != HOBAK_PRECISION_FP32_REFINE)
+    return fail(ctx, HOBAK_E_INVALID, "unknown precision form");
   for (size_t i = 0; i < 4 * (size_t)nt; i++)
     if (tets[i] < 0 || tets[i] >= nv) return fail(ctx, HOBAK_E_INVALID, "tet index out of range");
   // rest volumes must be positive (TET_MESH.cpp:132-137)
@@ -672,7 +755,7 @@ int hobak_inst_create(hobak_ctx* ctx, const double* rest_xyz, int32_t nv, const 
     I->lam_eff = material->lambda;
     I->alpha = 0.0;
   }
-  I->tsize = 8;
+  I->tsize = params->precision == HOBAK_PRECISION_FP64 ? 8 : 4;
   build_pattern(nv, nt, tets, ctx->slice_rows, I->P);
   I->poll_chunk = ctx->poll_chunk;
   const Pattern& P = I->P;
@@ -705,7 +788,10 @@ int hobak_inst_create(hobak_ctx* ctx, const double* rest_xyz, int32_t nv, const 
   ALLOC(tmp64, 3 * (size_t)nv);
   ALLOC(dminv, 9 * (size_t)nt);
   ALLOC(vol, (size_t)nt);
-  ALLOC(kneg64, 9 * (size_t)P.nslots);
+  ALLOCT(kneg, 9 * (size_t)P.nslots);
+  ALLOC(x64, 3 * (size_t)nv);
+  ALLOC(res64, 3 * (size_t)nv);
+  ALLOC(rstate, 1);
   ALLOC(tets, (size_t)nt);
   ALLOC(slice_off, P.slice_off.size());
   ALLOC(row_len, (size_t)nv);
@@ -740,7 +826,8 @@ int hobak_inst_create(hobak_ctx* ctx, const double* rest_xyz, int32_t nv, const 
   ALLOC(part_rz, (size_t)I->n_update);
 #undef ALLOC
 #undef ALLOCT
-  if (hipHostMalloc((void**)&I->sc_host, sizeof(CgScalars), hipHostMallocDefault) != hipSuccess)
+  if (hipHostMalloc((void**)&I->sc_host, sizeof(CgScalars), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&I->rstate_host, sizeof(RefineState), hipHostMallocDefault) != hipSuccess)
     return bail(fail(ctx, HOBAK_E_HIP, "hipHostMalloc failed"));
   for (auto& e : I->ev)
     if (hipEventCreate(&e) != hipSuccess) return bail(fail(ctx, HOBAK_E_HIP, "hipEventCreate failed"));
@@ -764,7 +851,8 @@ int hobak_inst_create(hobak_ctx* ctx, const double* rest_xyz, int32_t nv, const 
       (rc = upload(I, I->vptr, P.vptr.data(), P.vptr.size())) ||
       (rc = upload(I, I->fdest, P.fdest.data(), P.fdest.size())))
     return bail(rc);
-  if ((rc = setup_constant_parts<double>(I)) != HOBAK_OK) return bail(rc);
+  if ((rc = (I->tsize == 8 ? setup_constant_parts<double>(I) : setup_constant_parts<float>(I))) != HOBAK_OK)
+    return bail(rc);
   if (hipStreamSynchronize(ctx->stream) != hipSuccess)
     return bail(fail(ctx, HOBAK_E_HIP, "set-up kernels failed"));
   ctx->insts.insert(I);
@@ -821,7 +909,7 @@ int hobak_step(hobak_inst* I, hobak_step_stats* st) {
   auto t0 = std::chrono::steady_clock::now();
   hobak_step_stats local;
   std::memset(&local, 0, sizeof(local));
-  int rc = step_impl<double>(I, &local);
+  int rc = I->tsize == 8 ? step_impl<double>(I, &local) : step_impl<float>(I, &local);
   auto t1 = std::chrono::steady_clock::now();
   local.seconds = std::chrono::duration<double>(t1 - t0).count();
   if (st) *st = local;
@@ -848,29 +936,40 @@ int hobak_get_bsr_pattern(hobak_inst* I, int32_t* nnzb, int32_t* row_ptr, int32_
   return HOBAK_OK;
 }
 
-int hobak_eval_forces_hessian(hobak_inst* I, const double* u, double* forces, double* kvals) {
-  if (!I || !u) return HOBAK_E_INVALID;
-  (void)hipSetDevice(I->ctx->device);
+}  // extern "C"
+
+namespace {
+template <typename T>
+int eval_forces_hessian(hobak_inst* I, const double* u, double* forces, double* kvals) {
   hipStream_t s = I->ctx->stream;
   TRY(upload(I, I->tmp64, u, (size_t)I->dofs));
-  HIPCHK(I->ctx, launch_element<double>(element_args(I, I->tmp64, kvals != nullptr), s));
+  HIPCHK(I->ctx, launch_element<T>(element_args(I, I->tmp64, kvals != nullptr), s));
   if (forces) {
-    HIPCHK(I->ctx, launch_force_gather<double>(I->nv, I->vptr, (const double*)I->stage_f,
-                                               I->rhs64, s));
+    HIPCHK(I->ctx, launch_force_gather<T>(I->nv, I->vptr, (const T*)I->stage_f, I->rhs64, s));
     HIPCHK(I->ctx, hipMemcpyAsync(forces, I->rhs64, sizeof(double) * I->dofs, hipMemcpyDeviceToHost, s));
   }
   if (kvals) {
-    double* tmpK = nullptr;
-    HIPCHK(I->ctx, hipMalloc(&tmpK, sizeof(double) * 9 * (size_t)I->P.nslots));
-    HIPCHK(I->ctx, hipMemsetAsync(tmpK, 0, sizeof(double) * 9 * (size_t)I->P.nslots, s));
-    HIPCHK(I->ctx, launch_assemble<double>(asm_args(I, 0, tmpK), s));
-    int rc = download_sell_as_bsr<double>(I, tmpK, kvals, -1.0);
+    void* tmpK = nullptr;
+    HIPCHK(I->ctx, hipMalloc(&tmpK, sizeof(T) * 9 * (size_t)I->P.nslots));
+    HIPCHK(I->ctx, hipMemsetAsync(tmpK, 0, sizeof(T) * 9 * (size_t)I->P.nslots, s));
+    HIPCHK(I->ctx, launch_assemble<T>(asm_args(I, 0, tmpK), s));
+    int rc = download_sell_as_bsr<T>(I, tmpK, kvals, -1.0);
     (void)hipFree(tmpK);
     if (rc != HOBAK_OK) return rc;
   }
   HIPCHK(I->ctx, hipStreamSynchronize(s));
   I->has_system = false;  // rhs64 was reused
   return HOBAK_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int hobak_eval_forces_hessian(hobak_inst* I, const double* u, double* forces, double* kvals) {
+  if (!I || !u) return HOBAK_E_INVALID;
+  (void)hipSetDevice(I->ctx->device);
+  return I->tsize == 8 ? eval_forces_hessian<double>(I, u, forces, kvals)
+                       : eval_forces_hessian<float>(I, u, forces, kvals);
 }
 
 int hobak_get_last_system(hobak_inst* I, double* rhs, double* lhs_vals) {
@@ -881,7 +980,9 @@ int hobak_get_last_system(hobak_inst* I, double* rhs, double* lhs_vals) {
     HIPCHK(I->ctx, hipMemcpyAsync(rhs, I->rhs64, sizeof(double) * I->dofs, hipMemcpyDeviceToHost, s));
     HIPCHK(I->ctx, hipStreamSynchronize(s));
   }
-  if (lhs_vals) TRY(download_sell_as_bsr<double>(I, I->A, lhs_vals, 1.0));
+  if (lhs_vals)
+    TRY(I->tsize == 8 ? download_sell_as_bsr<double>(I, I->A, lhs_vals, 1.0)
+                      : download_sell_as_bsr<float>(I, I->A, lhs_vals, 1.0));
   return HOBAK_OK;
 }
 

@@ -29,7 +29,7 @@ struct ConstArgs {
   int nslots;
   const int* slot_row;
   const int* col;
-  const double* kneg;    // assembled -K_rest, 9 planes
+  const void* kneg;      // assembled -K_rest, 9 planes, T
   const double* mass;    // per DOF
   double ra, rb, invDt, invDt2;
   void* D;
@@ -100,6 +100,36 @@ struct CgArgs {
   Partials rz;           // update -> next SpMV: a = r.r, b = r.z
 };
 
+// FP64 iterative refinement around the FP32 PCG (the precision form).
+struct RefineState {
+  double b2;      // |S b|^2
+  double res2;    // |rhs - A x64|^2 of the current iterate
+  int done;       // res2 < tol^2 b2 (or b2 == 0), or a non-finite residual
+  int status;     // 0 ok, 2 non-finite
+  int sweeps;
+  int pad;
+};
+
+struct RefineArgs {
+  int nrows, nslots, slice_shift;
+  const int* slice_off;
+  const int* row_len;
+  const int* col;
+  const void* A;         // T
+  const double* rhs64;
+  double* x64;
+  double* res64;
+  void* r;               // T inner PCG vectors
+  void* x;
+  void* z;
+  void* p0;
+  const void* minv;
+  int precond;
+  Partials out;          // a = |.|^2, b = r.z
+  RefineState* state;
+  double tol;            // outer tolerance on |res| / |b|
+};
+
 hipError_t launch_begin_step(int nv, double* u, const double* v, double* uOld, double* vOld,
                              const unsigned char* pinned, const double* target, hipStream_t s);
 hipError_t launch_end_step(int nv, double invDt, const double* u, const double* uOld, double* v,
@@ -120,6 +150,12 @@ hipError_t launch_cg_spmv(const CgArgs& a, hipStream_t s, hipEvent_t start = nul
                           hipEvent_t stop = nullptr);
 template <typename T>
 hipError_t launch_cg_rest(const CgArgs& a, hipStream_t s);
+hipError_t launch_refine_begin(const RefineArgs& a, const Partials& rhs_parts, hipStream_t s);
+template <typename T>
+hipError_t launch_refine_init(const RefineArgs& a, hipStream_t s);
+template <typename T>
+hipError_t launch_refine_residual(const RefineArgs& a, hipStream_t s);
+hipError_t launch_refine_check(const RefineArgs& a, hipStream_t s);
 template <typename T>
 hipError_t launch_force_gather(int nv, const int* vptr, const T* stage_f, double* R, hipStream_t s);
 hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s);

@@ -201,7 +201,7 @@ __global__ void constant_parts_kernel(ConstArgs a) {
   const size_t ns = (size_t)a.nslots;
 #pragma unroll
   for (int e = 0; e < 9; e++) {
-    const double krest = -a.kneg[e * ns + slot];
+    const double krest = -(double)((const T*)a.kneg)[e * ns + slot];
     const bool dg = (c == r) && (e % 4 == 0);
     const double m = dg ? a.mass[3 * (size_t)r + e / 4] : 0.0;
     const double cc = dg ? a.ra * m + a.rb * krest : a.rb * krest;
@@ -508,6 +508,125 @@ __global__ __launch_bounds__(kBlock) void cg_update_kernel(CgArgs a) {
   publish(rr2, rz, a.rz, true);
 }
 
+// ---------------------------------------------------------------------------------------------
+// FP64 iterative refinement of the FP32 solve
+//   begin:     b2 = |S b|^2; x64 = 0; res64 = S b
+//   init:      r = (T) res64, x = 0, z = M^-1 r, p = 0 -> the inner PCG (same kernels as above)
+//   residual:  x64 += x; res64 = rhs64 - A x64 with A's T values promoted to FP64
+//   check:     res2 = |res64|^2; done when res2 < tol^2 b2
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void refine_begin_kernel(RefineArgs a, Partials rhs_parts) {
+  double b2, unused;
+  gather_partials(rhs_parts, false, b2, unused);
+  const int ndof = 3 * a.nrows;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ndof; i += gridDim.x * blockDim.x) {
+    a.x64[i] = 0.0;
+    a.res64[i] = a.rhs64[i];
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    a.state->b2 = b2;
+    a.state->res2 = b2;
+    a.state->sweeps = 0;
+    a.state->status = (!(b2 == b2) || b2 > DBL_MAX) ? 2 : 0;
+    a.state->done = (b2 == 0.0 || a.state->status) ? 1 : 0;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void refine_init_kernel(RefineArgs a) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  double rr = 0.0, rz = 0.0;
+  if (r < a.nrows) {
+    T rv[3], zv[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) rv[c] = (T)a.res64[3 * (size_t)r + c];
+    apply_precond<T>((const T*)a.minv, a.precond, r, rv, zv);
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const size_t i = 3 * (size_t)r + c;
+      ((T*)a.r)[i] = rv[c];
+      ((T*)a.x)[i] = T(0);
+      ((T*)a.z)[i] = zv[c];
+      ((T*)a.p0)[i] = T(0);
+      rr += (double)rv[c] * (double)rv[c];
+      rz += (double)rv[c] * (double)zv[c];
+    }
+  }
+  publish(rr, rz, a.out, true);
+}
+
+// One wavefront per H-row slice as in the SpMV; x64 is advanced by the inner solution on the fly
+// for every column, and the kernel's own rows write it back.
+template <typename T, int H>
+__global__ __launch_bounds__(kBlock) void refine_residual_kernel(RefineArgs a) {
+  constexpr int L = 64 / H;
+  const int slice = (int)((blockIdx.x * (unsigned)blockDim.x + threadIdx.x) >> 6);
+  const int lane = threadIdx.x & 63;
+  const int kk = lane / H;
+  const int rr = lane % H;
+  const int r = slice * H + rr;
+  const T* A = (const T*)a.A;
+  const T* xi = (const T*)a.x;
+  const size_t ns = (size_t)a.nslots;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  const bool live = r < a.nrows;
+  if (live) {
+    const int len = a.row_len[r];
+    const int base = a.slice_off[slice] + rr;
+    for (int k = kk; k < len; k += L) {
+      const int slot = base + k * H;
+      const size_t c = 3 * (size_t)a.col[slot];
+      const double x0 = a.x64[c] + (double)xi[c];
+      const double x1 = a.x64[c + 1] + (double)xi[c + 1];
+      const double x2 = a.x64[c + 2] + (double)xi[c + 2];
+      s0 += (double)A[0 * ns + slot] * x0 + (double)A[1 * ns + slot] * x1 + (double)A[2 * ns + slot] * x2;
+      s1 += (double)A[3 * ns + slot] * x0 + (double)A[4 * ns + slot] * x1 + (double)A[5 * ns + slot] * x2;
+      s2 += (double)A[6 * ns + slot] * x0 + (double)A[7 * ns + slot] * x1 + (double)A[8 * ns + slot] * x2;
+    }
+  }
+#pragma unroll
+  for (int off = H; off < 64; off <<= 1) {
+    s0 += __shfl_xor(s0, off, 64);
+    s1 += __shfl_xor(s1, off, 64);
+    s2 += __shfl_xor(s2, off, 64);
+  }
+  double res2 = 0.0;
+  if (live && kk == 0) {
+    const double s[3] = {s0, s1, s2};
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const size_t i = 3 * (size_t)r + c;
+      const double res = a.rhs64[i] - s[c];
+      a.res64[i] = res;
+      res2 += res * res;
+    }
+  }
+  publish(res2, 0.0, a.out, false);
+}
+
+// x64 += x for the rows themselves happens after every residual kernel has read the old x64:
+// a separate pass keeps the on-the-fly sums above race-free.
+template <typename T>
+__global__ void refine_accumulate_kernel(int n, double* x64, const T* x) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x64[i] += (double)x[i];
+}
+
+__global__ __launch_bounds__(kBlock) void refine_check_kernel(RefineArgs a) {
+  double res2, unused;
+  gather_partials(a.out, false, res2, unused);
+  if (threadIdx.x != 0) return;
+  RefineState* st = a.state;
+  st->res2 = res2;
+  st->sweeps += 1;
+  if (!(res2 == res2) || res2 > DBL_MAX) {
+    st->done = 1;
+    st->status = 2;
+  } else if (res2 < a.tol * a.tol * st->b2) {
+    st->done = 1;
+  }
+}
+
 // computeHyperelasticForces' scatter as a per-vertex gather in ascending tet order
 // (TET_MESH.cpp:756-767); output in FP64.
 template <typename T>
@@ -622,6 +741,40 @@ hipError_t launch_cg_iteration(const CgArgs& a, hipStream_t s) {
   return launch_cg_rest<T>(a, s);
 }
 
+hipError_t launch_refine_begin(const RefineArgs& a, const Partials& rhs_parts, hipStream_t s) {
+  const long g = grid_for(3L * a.nrows);
+  hipLaunchKernelGGL(refine_begin_kernel, dim3(g < kVectorBlocks ? g : kVectorBlocks), dim3(kBlock), 0,
+                     s, a, rhs_parts);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_refine_init(const RefineArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL((refine_init_kernel<T>), dim3(rhs_grid(a.nrows)), dim3(kBlock), 0, s, a);
+  return hipGetLastError();
+}
+
+// residual: accumulate-free SpMV on x64 + x, then x64 += x; partials go to a.out (spmv grid)
+template <typename T>
+hipError_t launch_refine_residual(const RefineArgs& a, hipStream_t s) {
+  const dim3 g(spmv_grid(a.nrows, a.slice_shift)), b(kBlock);
+  switch (a.slice_shift) {
+    case 3: hipLaunchKernelGGL((refine_residual_kernel<T, 8>), g, b, 0, s, a); break;
+    case 4: hipLaunchKernelGGL((refine_residual_kernel<T, 16>), g, b, 0, s, a); break;
+    case 5: hipLaunchKernelGGL((refine_residual_kernel<T, 32>), g, b, 0, s, a); break;
+    case 6: hipLaunchKernelGGL((refine_residual_kernel<T, 64>), g, b, 0, s, a); break;
+    default: return hipErrorInvalidValue;
+  }
+  hipLaunchKernelGGL((refine_accumulate_kernel<T>), dim3(grid_for(3L * a.nrows)), dim3(kBlock), 0, s,
+                     3 * a.nrows, a.x64, (const T*)a.x);
+  return hipGetLastError();
+}
+
+hipError_t launch_refine_check(const RefineArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(refine_check_kernel, dim3(1), dim3(kBlock), 0, s, a);
+  return hipGetLastError();
+}
+
 template <typename T>
 hipError_t launch_force_gather(int nv, const int* vptr, const T* stage_f, double* R, hipStream_t s) {
   hipLaunchKernelGGL((force_gather_kernel<T>), dim3(grid_for(nv)), dim3(kBlock), 0, s, nv, vptr,
@@ -643,7 +796,9 @@ hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s) 
   template hipError_t launch_cg_iteration<T>(const CgArgs&, hipStream_t);                \
   template hipError_t launch_cg_spmv<T>(const CgArgs&, hipStream_t, hipEvent_t, hipEvent_t); \
   template hipError_t launch_cg_rest<T>(const CgArgs&, hipStream_t);                     \
-  template hipError_t launch_force_gather<T>(int, const int*, const T*, double*, hipStream_t);
+  template hipError_t launch_force_gather<T>(int, const int*, const T*, double*, hipStream_t); \
+  template hipError_t launch_refine_init<T>(const RefineArgs&, hipStream_t);                 \
+  template hipError_t launch_refine_residual<T>(const RefineArgs&, hipStream_t);
 INST(double)
 INST(float)
 #undef INST
