@@ -47,6 +47,8 @@ def parse():
     ap.add_argument("--sample-every", type=int, default=4)
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run the calibration copy kernel (for counter scaling)")
+    ap.add_argument("--precision-form", default="on", choices=["on", "off"],
+                    help="also time the FP32 + FP64-refinement path on the same workload")
     return ap.parse_args()
 
 
@@ -97,6 +99,41 @@ def cpu_baseline(scene, steps, kind):
         "sample": f"first {steps} steps of the same {scene.dims[0]}^3 scene, Jacobi PCG tol 1e-8 "
                   f"(the reference's preconditioner); {total:.2f} s; PCG iters {iters}",
         "ms_per_step": 1e3 * total / steps,
+    }
+
+
+def precision_form(lib, sc, precond, args, fp64_total, u64):
+    """The same K steps with FP32 element/assembly/PCG and FP64 refinement; compared with the
+    FP64 GPU path's positions after the same steps."""
+    ctx = lib.Context(0, precision="fp32")
+    inst = lib.Instance(ctx, sc, precond=precond, precision=lib.PRECISION_FP32_REFINE)
+    for _ in range(args.warmup):
+        inst.step()
+    inst.set_state(sc.initial_u(), sc.initial_v())
+    step_s, iters, sweeps = [], [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        t1 = time.perf_counter()
+        st = inst.step()
+        step_s.append(time.perf_counter() - t1)
+        iters.append(st["pcg_iters"][0])
+        sweeps.append(st["refine_iters"][0])
+    total = time.perf_counter() - t0
+    u, _ = inst.state()
+    X = sc.rest.ravel()
+    err = float(np.linalg.norm(u - u64) / np.linalg.norm(X + u64))
+    inst.close()
+    ctx.close()
+    return {
+        "value": sc.nt * args.steps / total,
+        "unit": "tet-Newton-steps/s",
+        "ms_per_step": 1e3 * total / args.steps,
+        "p50_step_ms": 1e3 * float(np.percentile(step_s, 50)),
+        "p99_step_ms": 1e3 * float(np.percentile(step_s, 99)),
+        "speedup_vs_fp64": fp64_total / total,
+        "rel_position_error_vs_fp64_after_steps": err,
+        "inner_pcg_iters_per_step": iters,
+        "refine_sweeps_per_step": sweeps,
     }
 
 
@@ -180,6 +217,8 @@ def main():
         "finite": finite,
         "roofline": roofline,
     }
+    if args.precision_form == "on":
+        out["fp32_refine"] = precision_form(lib, sc, precond, args, total, u)
     if args.copy_probe:
         nb = 512 << 20
         ms = ctx.copy_probe(nb, 20)

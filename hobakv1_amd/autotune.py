@@ -27,10 +27,11 @@ def load_table():
         return json.load(f)
 
 
-def measure(ctx, scene, precond, precision, steps, warmup):
+def measure(ctx, scene, precond, precision, steps, warmup, inner_tol=1e-4):
     from . import lib
 
-    inst = lib.Instance(ctx, scene, precond=precond, precision=precision)
+    inst = lib.Instance(ctx, scene, precond=precond, precision=precision,
+                        refine_inner_tol=inner_tol)
     for _ in range(warmup):
         inst.step()
     inst.set_state(scene.initial_u(), scene.initial_v())
@@ -57,6 +58,7 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--slice-rows", default="8,16,32,64")
     ap.add_argument("--poll", default="8,32")
+    ap.add_argument("--inner-tol", default="1e-4", help="FP32 inner PCG tolerances to sweep")
     ap.add_argument("--write", action="store_true")
     args = ap.parse_args()
     sys.path.insert(0, os.path.dirname(HERE))
@@ -65,13 +67,17 @@ def main():
     precision = lib.PRECISION_FP64 if args.precision == "fp64" else lib.PRECISION_FP32_REFINE
     sc = scenes.lattice_scene((args.n,) * 3, seed=2, stretch_velocity=True)
     results = []
-    for H, poll in itertools.product([int(x) for x in args.slice_rows.split(",")],
-                                     [int(x) for x in args.poll.split(",")]):
+    tols = [float(x) for x in args.inner_tol.split(",")] if args.precision == "fp32" else [0.0]
+    for H, poll, tol in itertools.product([int(x) for x in args.slice_rows.split(",")],
+                                          [int(x) for x in args.poll.split(",")], tols):
         ctx = lib.Context(0, apply_table=False)
         ctx.set_tuning(slice_rows=H, poll_chunk=poll)
-        r = measure(ctx, sc, scenes.PRECOND_BLOCK_JACOBI, precision, args.steps, args.warmup)
+        r = measure(ctx, sc, scenes.PRECOND_BLOCK_JACOBI, precision, args.steps, args.warmup,
+                    inner_tol=tol or 1e-4)
         ctx.close()
         r.update({"slice_rows": H, "poll_chunk": poll})
+        if args.precision == "fp32":
+            r["refine_inner_tol"] = tol
         results.append(r)
         print(json.dumps(r), flush=True)
     best = min(results, key=lambda r: r["ms_per_step"])
@@ -80,6 +86,8 @@ def main():
         table = load_table()
         table[args.precision] = {"slice_rows": best["slice_rows"], "poll_chunk": best["poll_chunk"],
                                  "measured": {"workload": f"{args.n}^3 lattice", **best}}
+        if "refine_inner_tol" in best:
+            table[args.precision]["refine_inner_tol"] = best["refine_inner_tol"]
         with open(TABLE, "w") as f:
             json.dump(table, f, indent=1, sort_keys=True)
     return 0

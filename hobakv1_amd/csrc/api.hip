@@ -494,8 +494,17 @@ int solve_refined(hobak_inst* I, int& iters, int& sweeps, double& err, int& stat
   HIPCHK(ctx, hipMemcpyAsync(I->rstate_host, I->rstate, sizeof(RefineState), hipMemcpyDeviceToHost, s));
   HIPCHK(ctx, hipStreamSynchronize(s));
   InitArgs ia = init_args(I);
-  ia.tol = I->prm.refine_inner_tol > 0 ? I->prm.refine_inner_tol : 1e-4;
+  const double tau0 = I->prm.refine_inner_tol > 0 ? I->prm.refine_inner_tol : 1e-4;
+  const double outer = I->prm.pcg_tol > 0 ? I->prm.pcg_tol : 2.220446049250313e-16;
+  ia.tol = tau0;
   while (!I->rstate_host->done && sweeps < max_sweeps) {
+    // After the first sweep, aim each correction solve at the outer target with a 5x margin
+    // (0.2 tol |b| / |res|), floored at FP32's reach, so the last sweep overshoots the target
+    // instead of stopping just short of it.
+    if (sweeps > 0 && I->rstate_host->res2 > 0) {
+      const double want = 0.2 * outer * std::sqrt(I->rstate_host->b2 / I->rstate_host->res2);
+      ia.tol = std::min(tau0, std::max(want, 1e-6));
+    }
     HIPCHK(ctx, launch_refine_init<T>(ri, s));
     HIPCHK(ctx, launch_cg_init(ia, s));
     TRY(run_pcg<T>(I));

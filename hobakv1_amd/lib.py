@@ -158,6 +158,7 @@ class Context:
         if rc != HOBAK_OK:
             raise HobakError(rc, "hobak_ctx_create failed (no gfx950 device?)")
         self.h = h
+        self.refine_inner_tol = None
         if apply_table:
             self.apply_tuning_table(precision)
 
@@ -171,6 +172,7 @@ class Context:
             entry = json.load(f).get(precision)
         if entry:
             self.set_tuning(slice_rows=entry.get("slice_rows"), poll_chunk=entry.get("poll_chunk"))
+            self.refine_inner_tol = entry.get("refine_inner_tol")
 
     def error(self) -> str:
         return self.lib.hobak_last_error(self.h).decode()
@@ -235,7 +237,9 @@ class Instance:
 
     def __init__(self, ctx: Context, scene, precond=_scenes.PRECOND_BLOCK_JACOBI,
                  precision=PRECISION_FP64, newton_iters=None, newton_tol=1e-7, pcg_tol=None,
-                 refine_inner_tol=1e-4, params: Optional[hobak_step_params] = None):
+                 refine_inner_tol=None, params: Optional[hobak_step_params] = None):
+        if refine_inner_tol is None:
+            refine_inner_tol = getattr(ctx, "refine_inner_tol", None) or 1e-4
         self.ctx = ctx
         self.lib = ctx.lib
         self.scene = scene
