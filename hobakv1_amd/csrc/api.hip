@@ -1,5 +1,6 @@
-// Host side of libhobak_hip.so: mesh set-up (sparsity, scatter map), device residency, step
-// orchestration and the C ABI declared in include/hobak_c.h.
+// Host side of libhobak_hip.so: mesh set-up (sparsity, scatter map), device residency, the
+// batched step engine (plans, ticks, captured graphs, continuous batching) and the C ABI declared
+// in include/hobak_c.h.
 //
 // Reference interfaces replaced (relative to the reference root):
 //   TET_MESH::TET_MESH                    src/Geometry/TET_MESH.cpp:43-107 (volumes, one-rings, DmInv)
@@ -10,6 +11,17 @@
 //   BACKWARD_EULER_POSITION::solveRayleighDamped
 //                                         src/Timestepper/BACKWARD_EULER_POSITION.cpp:138-327
 //   BDF_1::solveRayleighDamped            src/Timestepper/BDF_1.cpp:78-239
+//   SIMULATION_SCENE::stepSimulation      src/Scenes/SIMULATION_SCENE.h:120-126
+//
+// Engine.  Every step, batched or not, runs through one tick loop:
+//   1. instances whose solve finished at the last poll get x applied, then either end the step
+//      (velocity update, request complete) or re-enter for the next BDF_1 Newton iteration;
+//   2. newly started and continuing instances run the prologue (element, assembly, right-hand
+//      side, PCG / refinement seed) as one segmented launch per kernel over just those instances;
+//   3. groups of PCG iterations run over the scheduler's whole member set (a captured graph per
+//      member set); kernels of instances that are not solving return at once;
+//   4. one status kernel and one small copy, then the host waits and reads the statuses.
+// Requests submitted while others are on the GPU join at the next tick: continuous batching.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -17,27 +29,47 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
-#include <mutex>
+#include <deque>
 #include <new>
 #include <set>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/hobak_c.h"
 #include "kernels.h"
-#include "solver_kernels.h"
 
 using namespace hobak;
 
+namespace {
+struct Plan;
+constexpr double kEps = 2.220446049250313e-16;  // Eigen's default CG tolerance (machine epsilon)
+constexpr int kMaxPlans = 32;
+constexpr int kMaxSamples = 512;
+}  // namespace
+
 struct hobak_ctx {
   int device = 0;
-  int slice_rows = hobak::kDefaultSliceRows;  // tuning, applied to instances created afterwards
-  int poll_chunk = 8;
+  int slice_rows = kDefaultSliceRows;  // tuning, fixed once instances exist
+  int chunk_iters = 16;                // PCG iterations per queued group (even)
+  int max_chunks = 64;                 // groups queued between host polls
+  int use_graphs = 1;
+  int grid_cap = 0;                    // max workgroups per launch (0: one per virtual workgroup)
   hipStream_t stream = nullptr;
   size_t budget = 0;
   size_t in_use = 0;
   std::string err;
   std::set<hobak_inst*> insts;
+  std::set<hobak_sched*> scheds;
+  std::vector<Plan*> plans;  // cached member plans, least recently used evicted
+  uint64_t stamp = 0;
+  // tables of tick-local batches; reused whenever the stream is idle
+  char* arena_dev = nullptr;
+  char* arena_host = nullptr;
+  size_t arena_cap = 0;
+  size_t arena_off = 0;
+  hipEvent_t ev[8] = {};
+  std::vector<hipEvent_t> sample_ev;  // pairs
 };
 
 namespace {
@@ -150,6 +182,7 @@ void build_pattern(int nv, int nt, const int32_t* tets, int H, Pattern& P) {
     for (int a = 0; a < 4; a++) P.fdest[4 * (size_t)t + a] = vf[tets[4 * t + a]]++;
 }
 
+
 }  // namespace
 
 struct hobak_inst {
@@ -163,40 +196,46 @@ struct hobak_inst {
   std::vector<int> pinned_ids;
   std::vector<std::pair<void*, size_t>> allocs;
   size_t bytes = 0;
-
-  // device, FP64
-  double *rest = nullptr, *u = nullptr, *v = nullptr, *u_old = nullptr, *v_old = nullptr;
-  double *dminv = nullptr, *vol = nullptr, *one_ring = nullptr, *mass = nullptr;
-  double *pin_target = nullptr, *rhs64 = nullptr, *tmp64 = nullptr, *x64 = nullptr, *res64 = nullptr;
-  void* kneg = nullptr;  // assembled -K_rest, T
-  int4* tets = nullptr;
-  int *slice_off = nullptr, *row_len = nullptr, *col = nullptr, *slot_row = nullptr;
-  int *cptr = nullptr, *hdest = nullptr;
-  int *vptr = nullptr, *fdest = nullptr;
-  unsigned char* pinned = nullptr;
-  // device, real type T
-  void *stage_h = nullptr, *stage_f = nullptr, *A = nullptr, *D = nullptr, *E = nullptr, *C = nullptr;
-  void *minv = nullptr, *r = nullptr, *x = nullptr, *z = nullptr, *t = nullptr;
-  void *p0 = nullptr, *p1 = nullptr;
-  RefineState* rstate = nullptr;       // FP64 refinement state (precision form)
-  RefineState* rstate_host = nullptr;
-  CgScalars* sc = nullptr;       // two ping-pong sets
-  CgScalars* sc_host = nullptr;  // pinned host copy of one set
-  // per-workgroup partial sums: rhs -> init, SpMV -> update, update -> p
-  double *part_rhs_a = nullptr, *part_rhs_b = nullptr, *part_pt = nullptr;
-  double *part_rr = nullptr, *part_rz = nullptr;
-  int n_rhs = 0, n_spmv = 0, n_update = 0;
-  int last_it = 0;  // iterations queued by the last solve
-  hipEvent_t ev[6] = {};
-  double last_ms[5] = {0, 0, 0, 0, 0};
-  int prev_iters = -1;
+  InstDev h{};              // host mirror of the device record
+  InstDev* dev = nullptr;   // the device record
+  Ctl* ctl = nullptr;
   bool has_system = false;
-  int poll_chunk = 8;
+  double last_ms[5] = {0, 0, 0, 0, 0};
+  int requests = 0;                 // queued or running requests
+  hobak_sched* sched = nullptr;     // scheduler it is attached to, if any
+  int pred[HOBAK_MAX_NEWTON];       // PCG iterations of the last step, per Newton iteration
   // sampled SpMV timing (measurement hook)
   int sample_every = 0;
-  std::vector<hipEvent_t> sample_ev;   // pairs
-  double spmv_ms_sum = 0.0;
+  double spmv_ms_sum = 0.0, spmv_bytes_sum = 0.0;
   long long spmv_count = 0;
+};
+
+struct hobak_sched {
+  hobak_ctx* ctx = nullptr;
+  bool ephemeral = false;       // a private scheduler of one hobak_step / hobak_step_batch call
+  std::vector<hobak_inst*> members;
+  std::unordered_map<hobak_inst*, int> index;
+  bool dirty = true;
+  Plan* mplan = nullptr;
+  struct Run {
+    hobak_inst* I = nullptr;
+    uint64_t tag = 0;
+    double t_submit = 0, t_start = -1;
+    int newton_k = 0;
+    int stage = 0;      // 0 needs prologue, 1 solving, 2 solve finished (epilogue next)
+    int begin = 1;      // prologue starts a new step (u_old, v_old, pinned targets)
+    int done_iters = 0; // PCG iterations of this Newton iteration at the last poll
+    hobak_step_stats st{};
+  };
+  std::deque<Run> queue;   // submitted, not started
+  std::vector<Run> active;
+  std::vector<hobak_completion> done;
+  std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+  double ms[5] = {0, 0, 0, 0, 0};
+  // sampled SpMV launches of the current batch: (iteration index since the batch's prologue, slot)
+  std::vector<std::pair<int, int>> samples;
+  int sample_it = 0;
+  bool sampling = false;
 };
 
 namespace {
@@ -211,6 +250,12 @@ int fail(hobak_ctx* ctx, int code, const std::string& msg) {
     hipError_t e_ = (expr);                                                                 \
     if (e_ != hipSuccess)                                                                   \
       return fail((ctx), HOBAK_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define TRY(expr)                    \
+  do {                               \
+    int rc_ = (expr);                \
+    if (rc_ != HOBAK_OK) return rc_; \
   } while (0)
 
 template <typename P>
@@ -236,376 +281,619 @@ int upload(hobak_inst* I, P* dst, const P* src, size_t count) {
   return HOBAK_OK;
 }
 
-#define TRY(expr)              \
-  do {                         \
-    int rc_ = (expr);          \
-    if (rc_ != HOBAK_OK) return rc_; \
-  } while (0)
-
-size_t estimate_bytes(int nv, int nt, int nnzb_est, int nslots_est, int precision) {
+size_t estimate_bytes(int nv, int nt, int nslots_est, int precision) {
   const size_t ts = precision == HOBAK_PRECISION_FP64 ? 8 : 4;
   size_t b = 0;
-  b += (size_t)nv * 3 * 8 * 10;                 // rest, u, v, olds, target, rhs, tmp, x64, res64, mass
+  b += (size_t)nv * 3 * 8 * 12;                 // rest, u, v, olds, target, rhs, tmp, x64, res64, mass
   b += (size_t)nt * (16 + 9 * 8 + 8);           // tets, dminv, vol
   b += (size_t)nslots_est * (4 + 4 + 9 * ts);   // col + slot_row + kneg
   b += (size_t)nslots_est * 9 * ts * 4;         // A, D, E, C
   b += (size_t)nslots_est * 4 + (size_t)nt * 20 * 4;  // cptr + hdest + fdest
   b += (size_t)nt * (16 * 9 + 12) * ts;         // staging
-  b += (size_t)nv * (9 + 15) * ts;              // minv + CG vectors
+  b += (size_t)nv * (9 + 18) * ts;              // minv + PCG vectors
   b += (size_t)nt * 4 * 4 + (size_t)nv * 16;    // vertex lists, misc
-  (void)nnzb_est;
+  b += 64 * 1024;                               // records, control block, partial sums
   return b;
 }
 
-// ---- kernel argument builders ----
-ElementArgs element_args(hobak_inst* I, const double* u, int with_h) {
-  ElementArgs a{};
-  a.nt = I->nt;
-  a.material = I->mat.kind;
-  a.mu = I->mat.mu;
-  a.lam = I->lam_eff;
-  a.alpha = I->alpha;
-  a.tets = I->tets;
-  a.rest = I->rest;
-  a.u = u;
-  a.dminv = I->dminv;
-  a.vol = I->vol;
-  a.stage_h = I->stage_h;
-  a.stage_f = I->stage_f;
-  a.hdest = I->hdest;
-  a.fdest = I->fdest;
-  a.with_hessian = with_h;
-  return a;
+// ---------------------------------------------------------------------------------------------
+// plans: the device tables of one segmented batch
+// ---------------------------------------------------------------------------------------------
+constexpr size_t kAlign = 256;
+size_t al(size_t x) { return (x + kAlign - 1) & ~(kAlign - 1); }
+
+struct Plan {
+  std::vector<hobak_inst*> members;
+  int n = 0, tsize = 8, shift = 5, refine = 0;
+  bool owned = false;
+  char* dev = nullptr;
+  char* host = nullptr;
+  size_t bytes = 0;
+  Batch b[K_COUNT];
+  InstStatus* status_dev = nullptr;
+  InstStatus* status_host = nullptr;
+  double spmv_bytes = 0.0;
+  hipGraphExec_t chunk_exec = nullptr;
+  int chunk_k = 0;
+  hipGraphExec_t pro_exec[2] = {nullptr, nullptr};  // prologue without / with the step start
+  hipGraphExec_t epi_exec = nullptr;
+  uint64_t stamp = 0;
+};
+
+size_t plan_bytes(size_t n) {
+  return al(sizeof(void*) * n) + K_COUNT * al(sizeof(int) * (n + 1)) + al(sizeof(InstStatus) * n);
 }
 
-AsmArgs asm_args(hobak_inst* I, int mode, void* out) {
-  AsmArgs a{};
-  a.nrows = I->nv;
-  a.nslots = I->P.nslots;
-  a.nt = I->nt;
-  a.slot_row = I->slot_row;
-  a.col = I->col;
-  a.cptr = I->cptr;
-  a.stage_h = I->stage_h;
-  a.D = I->D;
-  a.out = out;
-  a.mode = mode;
-  a.pinned = I->pinned;
-  a.minv = I->minv;
-  a.precond = I->prm.precond;
-  a.ground_on = I->prm.ground_on;
-  a.ground_y = I->prm.ground_y;
-  a.ground_k = I->prm.ground_k;
-  a.rest = I->rest;
-  a.u = I->u;
-  return a;
+double spmv_bytes_of(const hobak_inst* I) {
+  const double sz = (double)I->tsize;
+  return (double)I->P.nnzb * (9.0 * sz + 4.0) + 4.0 * (I->nv + 1) + 2.0 * 3.0 * sz * I->nv;
 }
 
-CgArgs cg_args(hobak_inst* I) {
-  CgArgs a{};
-  a.nrows = I->nv;
-  a.nslots = I->P.nslots;
-  a.slice_shift = I->P.shift;
-  a.slice_off = I->slice_off;
-  a.row_len = I->row_len;
-  a.col = I->col;
-  a.A = I->A;
-  a.x = I->x;
-  a.r = I->r;
-  a.z = I->z;
-  a.p0 = I->p0;
-  a.p1 = I->p1;
-  a.t = I->t;
-  a.minv = I->minv;
-  a.precond = I->prm.precond;
-  a.sc = I->sc;
-  a.it = 0;
-  a.pt = Partials{I->part_pt, nullptr, I->n_spmv};
-  a.rz = Partials{I->part_rr, I->part_rz, I->n_update};
-  return a;
+// Lays the tables out in the plan's host staging memory, uploads them and fills the Batch
+// descriptors.  All members share the real type and the slice height.
+int plan_fill(hobak_ctx* ctx, Plan& P) {
+  const int n = P.n;
+  size_t off = 0;
+  const InstDev** hp = (const InstDev**)(P.host + off);
+  const InstDev* const* dp = (const InstDev* const*)(P.dev + off);
+  P.spmv_bytes = 0.0;
+  for (int i = 0; i < n; i++) {
+    hp[i] = P.members[i]->dev;
+    P.spmv_bytes += spmv_bytes_of(P.members[i]);
+  }
+  off += al(sizeof(void*) * n);
+  for (int k = 0; k < K_COUNT; k++) {
+    int* bo = (int*)(P.host + off);
+    bo[0] = 0;
+    for (int i = 0; i < n; i++) {
+      const long next = (long)bo[i] + P.members[i]->h.nb[k];
+      if (next > INT32_MAX) return fail(ctx, HOBAK_E_INVALID, "batch too large for one launch");
+      bo[i + 1] = (int)next;
+    }
+    P.b[k] = Batch{dp, (const int*)(P.dev + off), n == 1 ? P.members[0]->dev : nullptr, n, bo[n]};
+    off += al(sizeof(int) * (n + 1));
+  }
+  P.status_dev = (InstStatus*)(P.dev + off);
+  P.status_host = (InstStatus*)(P.host + off);
+  HIPCHK(ctx, hipMemcpyAsync(P.dev, P.host, off, hipMemcpyHostToDevice, ctx->stream));
+  return HOBAK_OK;
 }
 
-InitArgs init_args(hobak_inst* I) {
-  InitArgs a{};
-  a.in = Partials{I->part_rhs_a, I->part_rhs_b, I->n_rhs};
-  a.sc = I->sc;
-  a.tol = I->prm.pcg_tol > 0 ? I->prm.pcg_tol : 2.220446049250313e-16;
-  a.max_iters = I->prm.pcg_max_iters > 0 ? I->prm.pcg_max_iters : 2 * I->dofs;
-  return a;
+int plan_check_members(hobak_ctx* ctx, const std::vector<hobak_inst*>& m) {
+  if (m.empty()) return fail(ctx, HOBAK_E_INVALID, "empty batch");
+  for (auto* I : m)
+    if (I->tsize != m[0]->tsize || I->P.shift != m[0]->P.shift)
+      return fail(ctx, HOBAK_E_INVALID, "instances of one batch must share precision and slice height");
+  return HOBAK_OK;
 }
 
-RefineArgs refine_args(hobak_inst* I, bool residual_parts) {
-  RefineArgs a{};
-  a.nrows = I->nv;
-  a.nslots = I->P.nslots;
-  a.slice_shift = I->P.shift;
-  a.slice_off = I->slice_off;
-  a.row_len = I->row_len;
-  a.col = I->col;
-  a.A = I->A;
-  a.rhs64 = I->rhs64;
-  a.x64 = I->x64;
-  a.res64 = I->res64;
-  a.r = I->r;
-  a.x = I->x;
-  a.z = I->z;
-  a.p0 = I->p0;
-  a.minv = I->minv;
-  a.precond = I->prm.precond;
-  // the inner-solve seed publishes one partial per rhs-shaped workgroup, the residual one per
-  // SpMV-shaped workgroup; both reuse arrays that are consumed before they are overwritten
-  a.out = residual_parts ? Partials{I->part_pt, nullptr, I->n_spmv}
-                         : Partials{I->part_rhs_a, I->part_rhs_b, I->n_rhs};
-  a.state = I->rstate;
-  a.tol = I->prm.pcg_tol > 0 ? I->prm.pcg_tol : 2.220446049250313e-16;
-  return a;
+void plan_free(Plan* P) {
+  if (!P) return;
+  if (P->chunk_exec) (void)hipGraphExecDestroy(P->chunk_exec);
+  for (auto& g : P->pro_exec)
+    if (g) (void)hipGraphExecDestroy(g);
+  if (P->epi_exec) (void)hipGraphExecDestroy(P->epi_exec);
+  if (P->owned) {
+    if (P->dev) (void)hipFree(P->dev);
+    if (P->host) (void)hipHostFree(P->host);
+  }
+  delete P;
 }
 
-RhsArgs rhs_args(hobak_inst* I, int bdf) {
-  RhsArgs a{};
-  a.nrows = I->nv;
-  a.nslots = I->P.nslots;
-  a.nt = I->nt;
-  a.slice_shift = I->P.shift;
-  a.slice_off = I->slice_off;
-  a.row_len = I->row_len;
-  a.col = I->col;
-  a.vptr = I->vptr;
-  a.stage_f = I->stage_f;
-  a.Mvec = bdf ? I->C : I->E;
-  a.vel = I->v;
-  a.u = I->u;
-  a.uOld = I->u_old;
-  a.vOld = I->v_old;
-  a.mass = I->mass;
-  a.one_ring = I->one_ring;
-  a.rest = I->rest;
-  for (int c = 0; c < 3; c++) a.g[c] = I->prm.gravity[c];
-  a.invDt = 1.0 / I->prm.dt;
-  a.invDt2 = a.invDt * a.invDt;
-  a.bdf = bdf;
-  a.pinned = I->pinned;
-  a.ground_on = I->prm.ground_on;
-  a.ground_y = I->prm.ground_y;
-  a.ground_k = I->prm.ground_k;
-  a.rhs64 = I->rhs64;
-  a.init_cg = 1;
-  a.r = I->r;
-  a.x = I->x;
-  a.z = I->z;
-  a.p0 = I->p0;
-  a.minv = I->minv;
-  a.precond = I->prm.precond;
-  a.out = Partials{I->part_rhs_a, I->part_rhs_b, I->n_rhs};
-  return a;
+// Transient plan in the context's arena (valid until the stream is next idle and the arena reset).
+int plan_transient(hobak_ctx* ctx, const std::vector<hobak_inst*>& m, Plan& P) {
+  TRY(plan_check_members(ctx, m));
+  const size_t need = plan_bytes(m.size());
+  if (ctx->arena_off + need > ctx->arena_cap) return fail(ctx, HOBAK_E_INVALID, "batch table arena exhausted");
+  P = Plan{};
+  P.members = m;
+  P.n = (int)m.size();
+  P.tsize = m[0]->tsize;
+  P.shift = m[0]->P.shift;
+  P.refine = m[0]->h.refine;
+  P.dev = ctx->arena_dev + ctx->arena_off;
+  P.host = ctx->arena_host + ctx->arena_off;
+  P.bytes = need;
+  ctx->arena_off += need;
+  return plan_fill(ctx, P);
+}
+
+// Grows the arena to hold `need` bytes; only called while the stream is idle.
+int arena_ensure(hobak_ctx* ctx, size_t need) {
+  if (need <= ctx->arena_cap) return HOBAK_OK;
+  size_t cap = std::max<size_t>(need, 2 * ctx->arena_cap);
+  cap = std::max<size_t>(cap, 1 << 20);
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  if (ctx->arena_dev) (void)hipFree(ctx->arena_dev);
+  if (ctx->arena_host) (void)hipHostFree(ctx->arena_host);
+  ctx->arena_dev = nullptr;
+  ctx->arena_host = nullptr;
+  ctx->arena_cap = 0;
+  HIPCHK(ctx, hipMalloc((void**)&ctx->arena_dev, cap));
+  HIPCHK(ctx, hipHostMalloc((void**)&ctx->arena_host, cap, hipHostMallocDefault));
+  ctx->arena_cap = cap;
+  ctx->arena_off = 0;
+  return HOBAK_OK;
+}
+
+// Cached member plan (own memory, graphs), keyed by the exact member list.
+int plan_get(hobak_ctx* ctx, const std::vector<hobak_inst*>& m, Plan** out) {
+  for (auto* P : ctx->plans)
+    if (P->members == m) {
+      P->stamp = ++ctx->stamp;
+      *out = P;
+      return HOBAK_OK;
+    }
+  TRY(plan_check_members(ctx, m));
+  if ((int)ctx->plans.size() >= kMaxPlans) {
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    auto it = std::min_element(ctx->plans.begin(), ctx->plans.end(),
+                               [](Plan* a, Plan* b) { return a->stamp < b->stamp; });
+    plan_free(*it);
+    ctx->plans.erase(it);
+  }
+  Plan* P = new (std::nothrow) Plan;
+  if (!P) return fail(ctx, HOBAK_E_NOMEM, "host allocation failed");
+  P->members = m;
+  P->n = (int)m.size();
+  P->tsize = m[0]->tsize;
+  P->shift = m[0]->P.shift;
+  P->refine = m[0]->h.refine;
+  P->owned = true;
+  P->bytes = plan_bytes(m.size());
+  if (hipMalloc((void**)&P->dev, P->bytes) != hipSuccess ||
+      hipHostMalloc((void**)&P->host, P->bytes, hipHostMallocDefault) != hipSuccess) {
+    plan_free(P);
+    return fail(ctx, HOBAK_E_NOMEM, "plan allocation failed");
+  }
+  int rc = plan_fill(ctx, *P);
+  if (rc != HOBAK_OK) {
+    plan_free(P);
+    return rc;
+  }
+  P->stamp = ++ctx->stamp;
+  ctx->plans.push_back(P);
+  *out = P;
+  return HOBAK_OK;
+}
+
+// Drops every cached plan that names I (its records are about to go).
+void plans_forget(hobak_ctx* ctx, hobak_inst* I) {
+  for (size_t k = 0; k < ctx->plans.size();) {
+    Plan* P = ctx->plans[k];
+    if (std::find(P->members.begin(), P->members.end(), I) != P->members.end()) {
+      plan_free(P);
+      ctx->plans.erase(ctx->plans.begin() + k);
+    } else {
+      k++;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// the step engine
+// ---------------------------------------------------------------------------------------------
+// Records (hipStreamBeginCapture .. hipStreamEndCapture) whatever `body` enqueues into an
+// executable graph.
+template <typename F>
+int capture(hobak_ctx* ctx, hipGraphExec_t* out, F&& body) {
+  hipStream_t s = ctx->stream;
+  hipGraph_t g = nullptr;
+  HIPCHK(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  const int rc = body();
+  const hipError_t e = hipStreamEndCapture(s, &g);
+  if (rc != HOBAK_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(ctx, HOBAK_E_HIP, std::string("graph capture failed: ") + hipGetErrorString(e));
+  const hipError_t ei = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (ei != hipSuccess)
+    return fail(ctx, HOBAK_E_HIP, std::string("graph instantiation failed: ") + hipGetErrorString(ei));
+  return HOBAK_OK;
+}
+
+// Element kernel, assembly, right-hand side and the PCG (or refinement) seed for the instances of
+// J; S (may be null) are those that start a new step.  With `timed`, events 1 and 2 mark the ends
+// of the element kernel and of the assembly.
+template <typename T>
+int launch_prologue(hobak_ctx* ctx, const Plan& J, const Plan* S, bool timed) {
+  hipStream_t s = ctx->stream;
+  const int cap = ctx->grid_cap;
+  if (S) HIPCHK(ctx, launch_begin_step(S->b[K_ROW], cap, s));
+  HIPCHK(ctx, launch_element<T>(J.b[K_ELEM], ELEM_U, 1, cap, s));
+  if (timed) HIPCHK(ctx, hipEventRecord(ctx->ev[1], s));
+  HIPCHK(ctx, launch_assemble<T>(J.b[K_SLOT], ASM_SYSTEM, cap, s));
+  if (timed) HIPCHK(ctx, hipEventRecord(ctx->ev[2], s));
+  HIPCHK(ctx, launch_rhs<T>(J.b[K_ROW], J.refine ? 0 : 1, cap, s));
+  if (J.refine) {
+    HIPCHK(ctx, launch_refine_begin(J.b[K_ROW], cap, s));
+    HIPCHK(ctx, launch_refine_init<T>(J.b[K_ROW], 0, cap, s));
+    HIPCHK(ctx, launch_cg_init(J.b[K_ONE], 1, 0, s));
+  } else {
+    HIPCHK(ctx, launch_cg_init(J.b[K_ONE], 0, 0, s));
+  }
+  return HOBAK_OK;
+}
+
+// k PCG iterations over the batch (k even, so the current scalar set is set 0 again at the end),
+// then the gated end-of-solve kernels: the refinement residual, check and re-seed, or the plain
+// form's finish flag.  With a sampling scheduler, some SpMV launches carry timing events.
+template <typename T>
+int launch_chunk(hobak_ctx* ctx, const Plan& P, int k, hobak_sched* S, int every) {
+  hipStream_t s = ctx->stream;
+  const int cap = ctx->grid_cap;
+  for (int j = 0; j < k; j++) {
+    hipEvent_t a = nullptr, b = nullptr;
+    if (S) {
+      const int g = S->sample_it++;
+      if (every > 0 && g % every == 0 && (int)S->samples.size() < kMaxSamples) {
+        const int slot = (int)S->samples.size();
+        a = ctx->sample_ev[2 * slot];
+        b = ctx->sample_ev[2 * slot + 1];
+        S->samples.emplace_back(g, slot);
+      }
+    }
+    HIPCHK(ctx, launch_cg_spmv<T>(P.b[K_SPMV], P.shift, j & 1, cap, s, a, b));
+    HIPCHK(ctx, launch_cg_update<T>(P.b[K_UPD], j & 1, cap, s));
+  }
+  if (P.refine) {
+    HIPCHK(ctx, launch_refine_residual<T>(P.b[K_SPMV], P.b[K_DOF], P.shift, cap, s));
+    HIPCHK(ctx, launch_finish_check(P.b[K_ONE], 1, s));
+    HIPCHK(ctx, launch_refine_init<T>(P.b[K_ROW], 1, cap, s));
+    HIPCHK(ctx, launch_cg_init(P.b[K_ONE], 1, 1, s));
+  } else {
+    HIPCHK(ctx, launch_finish_check(P.b[K_ONE], 0, s));
+  }
+  return HOBAK_OK;
+}
+
+template <typename T>
+int launch_epilogue(hobak_ctx* ctx, const Plan& F, const Plan* C) {
+  hipStream_t s = ctx->stream;
+  HIPCHK(ctx, launch_apply_dx<T>(F.b[K_DOF], F.refine, ctx->grid_cap, s));
+  if (C) HIPCHK(ctx, launch_end_step(C->b[K_ROW], ctx->grid_cap, s));
+  return HOBAK_OK;
+}
+
+double clock_s(const hobak_sched* S) {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now() - S->t0).count();
+}
+
+float elapsed(hobak_ctx* ctx, int a, int b) {
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, ctx->ev[a], ctx->ev[b]) != hipSuccess) return 0.f;
+  return ms;
+}
+
+int worse(int a, int b) {
+  if (a == HOBAK_E_NONFINITE || b == HOBAK_E_NONFINITE) return HOBAK_E_NONFINITE;
+  if (a != HOBAK_OK) return a;
+  return b;
+}
+
+// One tick of the engine (see the file comment).  Requires an idle stream on entry; leaves it idle.
+template <typename T>
+int tick_impl(hobak_sched* S) {
+  hobak_ctx* ctx = S->ctx;
+  hipStream_t s = ctx->stream;
+  TRY(arena_ensure(ctx, 4 * plan_bytes(S->members.size() + 1)));
+  ctx->arena_off = 0;
+  TRY(plan_get(ctx, S->members, &S->mplan));
+  if (S->dirty) {
+    S->index.clear();
+    for (size_t i = 0; i < S->members.size(); i++) S->index[S->members[i]] = (int)i;
+    S->dirty = false;
+  }
+  Plan& M = *S->mplan;
+  const bool graphs = ctx->use_graphs && !S->sampling;
+
+  // 1. solves that finished at the last poll: x is applied, then the step ends or the next BDF_1
+  //    Newton iteration re-enters (the test uses |S b|^2 of the system just solved, BDF_1.cpp:111, 172)
+  std::vector<hobak_inst*> fin, comp;
+  for (auto& r : S->active) {
+    if (r.stage != 2) continue;
+    fin.push_back(r.I);
+    const hobak_step_params& p = r.I->prm;
+    const bool more = p.newton_iters > 1 && r.newton_k + 1 < p.newton_iters &&
+                      r.st.newton_residual[r.newton_k] > p.newton_tol;
+    if (more) {
+      r.newton_k++;
+      r.stage = 0;
+      r.begin = 0;
+      r.done_iters = 0;
+    } else {
+      comp.push_back(r.I);
+      r.stage = 3;  // complete once this tick's work has run
+    }
+  }
+  const bool epi = !fin.empty();
+  if (epi) {
+    HIPCHK(ctx, hipEventRecord(ctx->ev[6], s));
+    if (graphs && (int)fin.size() == M.n && comp.size() == fin.size()) {
+      if (!M.epi_exec) TRY(capture(ctx, &M.epi_exec, [&]() { return launch_epilogue<T>(ctx, M, &M); }));
+      HIPCHK(ctx, hipGraphLaunch(M.epi_exec, s));
+    } else {
+      Plan PF, PC;
+      TRY(plan_transient(ctx, fin, PF));
+      const Plan* pc = nullptr;
+      if (comp.size() == fin.size()) {
+        pc = &PF;
+      } else if (!comp.empty()) {
+        TRY(plan_transient(ctx, comp, PC));
+        pc = &PC;
+      }
+      TRY(launch_epilogue<T>(ctx, PF, pc));
+    }
+    HIPCHK(ctx, hipEventRecord(ctx->ev[7], s));
+  }
+
+  // 2. queued requests whose instance is free start
+  for (auto it = S->queue.begin(); it != S->queue.end();) {
+    bool busy = false;
+    for (auto& r : S->active)
+      if (r.I == it->I) busy = true;
+    if (busy) {
+      ++it;
+      continue;
+    }
+    S->active.push_back(*it);
+    S->active.back().stage = 0;
+    S->active.back().begin = 1;
+    it = S->queue.erase(it);
+  }
+
+  // 3. prologue of every run that needs one
+  std::vector<hobak_inst*> join, starters;
+  for (auto& r : S->active)
+    if (r.stage == 0) {
+      join.push_back(r.I);
+      if (r.begin) starters.push_back(r.I);
+    }
+  const bool pro = !join.empty();
+  bool pro_split = false;
+  if (pro) {
+    HIPCHK(ctx, hipEventRecord(ctx->ev[0], s));
+    const bool all = (int)join.size() == M.n && (starters.empty() || starters.size() == join.size());
+    if (graphs && all) {
+      const int wb = starters.empty() ? 0 : 1;
+      if (!M.pro_exec[wb])
+        TRY(capture(ctx, &M.pro_exec[wb], [&]() { return launch_prologue<T>(ctx, M, wb ? &M : nullptr, false); }));
+      HIPCHK(ctx, hipGraphLaunch(M.pro_exec[wb], s));
+    } else {
+      Plan PJ, PS;
+      TRY(plan_transient(ctx, join, PJ));
+      const Plan* ps = nullptr;
+      if (starters.size() == join.size()) {
+        ps = &PJ;
+      } else if (!starters.empty()) {
+        TRY(plan_transient(ctx, starters, PS));
+        ps = &PS;
+      }
+      TRY(launch_prologue<T>(ctx, PJ, ps, true));
+      pro_split = true;
+    }
+    HIPCHK(ctx, hipEventRecord(ctx->ev[3], s));
+    if (S->sampling && all && !starters.empty()) {
+      S->sample_it = 0;
+      S->samples.clear();
+    }
+    const double now = clock_s(S);
+    for (auto& r : S->active)
+      if (r.stage == 0) {
+        r.stage = 1;
+        r.done_iters = 0;
+        if (r.t_start < 0) r.t_start = now;
+      }
+  }
+
+  // 4. groups of PCG iterations over the member set: as many as the solving instance closest to
+  //    finishing is predicted to need (from its last step), at least one
+  bool ran = false;
+  int m = ctx->max_chunks;
+  int nsolving = 0;
+  const int k = ctx->chunk_iters;
+  for (auto& r : S->active) {
+    if (r.stage != 1) continue;
+    nsolving++;
+    const int p = r.I->pred[r.newton_k];
+    const int q = p > 0 ? (p - r.done_iters) / k : 0;
+    m = std::min(m, std::max(1, q));
+  }
+  if (nsolving) {
+    HIPCHK(ctx, hipEventRecord(ctx->ev[4], s));
+    if (graphs) {
+      if (!M.chunk_exec || M.chunk_k != k) {
+        if (M.chunk_exec) (void)hipGraphExecDestroy(M.chunk_exec);
+        M.chunk_exec = nullptr;
+        TRY(capture(ctx, &M.chunk_exec, [&]() { return launch_chunk<T>(ctx, M, k, nullptr, 0); }));
+        M.chunk_k = k;
+      }
+      for (int j = 0; j < m; j++) HIPCHK(ctx, hipGraphLaunch(M.chunk_exec, s));
+    } else {
+      int every = 0;
+      for (auto* I : S->members)
+        if (I->sample_every > 0) every = every ? std::min(every, I->sample_every) : I->sample_every;
+      for (int j = 0; j < m; j++) TRY(launch_chunk<T>(ctx, M, k, S->sampling ? S : nullptr, every));
+    }
+    HIPCHK(ctx, hipEventRecord(ctx->ev[5], s));
+    ran = true;
+  }
+
+  // 5. statuses
+  HIPCHK(ctx, launch_status(M.b[K_ONE], M.status_dev, s));
+  HIPCHK(ctx, hipMemcpyAsync(M.status_host, M.status_dev, sizeof(InstStatus) * M.n, hipMemcpyDeviceToHost, s));
+  HIPCHK(ctx, hipStreamSynchronize(s));
+  const double now = clock_s(S);
+  if (pro) {
+    if (pro_split) {
+      S->ms[0] += elapsed(ctx, 0, 1);
+      S->ms[1] += elapsed(ctx, 1, 2);
+      S->ms[2] += elapsed(ctx, 2, 3);
+    } else {
+      S->ms[0] += elapsed(ctx, 0, 3);
+    }
+  }
+  if (ran) S->ms[3] += elapsed(ctx, 4, 5);
+  if (epi) S->ms[4] += elapsed(ctx, 6, 7);
+
+  // completions: their epilogue ran before the wait above
+  for (size_t i = 0; i < S->active.size();) {
+    auto& r = S->active[i];
+    if (r.stage != 3) {
+      i++;
+      continue;
+    }
+    hobak_completion c;
+    std::memset(&c, 0, sizeof(c));
+    c.inst = r.I;
+    c.tag = r.tag;
+    c.stats = r.st;
+    c.stats.newton_iters = r.newton_k + 1;
+    c.stats.seconds = now - r.t_submit;
+    c.submit_s = r.t_submit;
+    c.start_s = r.t_start;
+    c.end_s = now;
+    r.I->requests--;
+    r.I->has_system = true;
+    S->done.push_back(c);
+    S->active.erase(S->active.begin() + i);
+  }
+  // solving instances: finished solves are recorded; the epilogue follows next tick
+  for (auto& r : S->active) {
+    if (r.stage != 1) continue;
+    const InstStatus& q = M.status_host[S->index[r.I]];
+    const bool refine = r.I->h.refine != 0;
+    if (q.phase == PHASE_FINISHED) {
+      const int nk = r.newton_k;
+      const int iters = refine ? q.inner_total : q.iters;
+      r.st.pcg_iters[nk] = iters;
+      r.st.refine_iters[nk] = refine ? q.sweeps : 0;
+      r.st.pcg_error[nk] = q.err;
+      r.st.newton_residual[nk] = q.maxR;
+      r.st.total_pcg_iters += iters;
+      int code = HOBAK_OK;
+      if (refine) {
+        if (q.ref_status == 2) code = HOBAK_E_NONFINITE;
+        else if (q.ref_status == 1 || !q.ref_done) code = HOBAK_E_NOT_CONVERGED;
+      } else {
+        if (q.cg_status == 2) code = HOBAK_E_NONFINITE;
+        else if (q.cg_status == 1) code = HOBAK_E_NOT_CONVERGED;
+      }
+      r.st.status = worse(r.st.status, code);
+      r.I->pred[nk] = iters;
+      r.stage = 2;
+    } else {
+      r.done_iters = refine ? q.inner_total + q.iters : q.iters;
+    }
+  }
+  return HOBAK_OK;
+}
+
+int tick(hobak_sched* S) {
+  if (S->members.empty()) return HOBAK_OK;
+  return S->members[0]->tsize == 8 ? tick_impl<double>(S) : tick_impl<float>(S);
+}
+
+// Drops a scheduler's queued and running requests of I (after the stream is idle).
+void sched_drop(hobak_sched* S, hobak_inst* I) {
+  for (auto it = S->queue.begin(); it != S->queue.end();)
+    if (it->I == I) {
+      I->requests--;
+      it = S->queue.erase(it);
+    } else {
+      ++it;
+    }
+  for (auto it = S->active.begin(); it != S->active.end();)
+    if (it->I == I) {
+      I->requests--;
+      it = S->active.erase(it);
+    } else {
+      ++it;
+    }
+}
+
+void sched_detach(hobak_sched* S, hobak_inst* I) {
+  sched_drop(S, I);
+  auto it = std::find(S->members.begin(), S->members.end(), I);
+  if (it != S->members.end()) {
+    S->members.erase(it);
+    S->dirty = true;
+  }
+  if (I->sched == S) I->sched = nullptr;
+}
+
+// Advances every instance of `m` by one step through a private scheduler; stats[i] is m[i]'s.
+int run_batch(hobak_ctx* ctx, const std::vector<hobak_inst*>& m, hobak_step_stats* stats) {
+  hobak_sched S;
+  S.ctx = ctx;
+  S.ephemeral = true;
+  S.members = m;
+  S.sampling = m[0]->tsize == 8;
+  for (auto* I : m) S.sampling = S.sampling && I->sample_every > 0;
+  if (S.sampling && ctx->sample_ev.empty()) {
+    ctx->sample_ev.assign(2 * kMaxSamples, nullptr);
+    for (auto& e : ctx->sample_ev) HIPCHK(ctx, hipEventCreate(&e));
+  }
+  for (size_t i = 0; i < m.size(); i++) {
+    hobak_sched::Run r;
+    r.I = m[i];
+    r.tag = i;
+    r.t_submit = clock_s(&S);
+    S.queue.push_back(r);
+    m[i]->requests++;
+  }
+  int rc = HOBAK_OK;
+  long guard = 0;
+  while ((!S.queue.empty() || !S.active.empty()) && rc == HOBAK_OK) {
+    rc = tick(&S);
+    if (++guard > 100000000L) rc = fail(ctx, HOBAK_E_HIP, "step engine did not terminate");
+  }
+  if (rc != HOBAK_OK) {
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto* I : m) sched_drop(&S, I);
+    return rc;
+  }
+  int worst = HOBAK_OK;
+  for (auto& c : S.done) {
+    if (stats) stats[c.tag] = c.stats;
+    worst = worse(worst, c.stats.status);
+  }
+  // sampled SpMV launches did work for every member while their index was <= every member's
+  // first-solve iteration count (launch 0 forms p = z; launch j finishes iteration j - 1)
+  if (S.sampling && !S.samples.empty()) {
+    int lim = INT32_MAX;
+    for (auto& c : S.done) lim = std::min(lim, c.stats.pcg_iters[0]);
+    for (auto& sm : S.samples) {
+      if (sm.first > lim) continue;
+      float ms = 0.f;
+      HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->sample_ev[2 * sm.second], ctx->sample_ev[2 * sm.second + 1]));
+      for (auto* I : m) {
+        I->spmv_ms_sum += ms;
+        I->spmv_count += 1;
+        I->spmv_bytes_sum += S.mplan->spmv_bytes;
+      }
+    }
+  }
+  for (auto* I : m) std::memcpy(I->last_ms, S.ms, sizeof(S.ms));
+  return worst;
 }
 
 template <typename T>
 int setup_constant_parts(hobak_inst* I) {
-  hipStream_t s = I->ctx->stream;
+  hobak_ctx* ctx = I->ctx;
+  hipStream_t s = ctx->stream;
+  ctx->arena_off = 0;
+  Plan P;
+  TRY(plan_transient(ctx, {I}, P));
   // rest-state stiffness (u = 0): TIMESTEPPER::buildRayleighDampingMatrix (TIMESTEPPER.cpp:213-236)
-  HIPCHK(I->ctx, hipMemsetAsync(I->tmp64, 0, sizeof(double) * I->dofs, s));
-  HIPCHK(I->ctx, launch_element<T>(element_args(I, I->tmp64, 1), s));
-  HIPCHK(I->ctx, launch_assemble<T>(asm_args(I, 0, I->kneg), s));
-  ConstArgs c{};
-  c.nslots = I->P.nslots;
-  c.slot_row = I->slot_row;
-  c.col = I->col;
-  c.kneg = I->kneg;
-  c.mass = I->mass;
-  c.ra = I->prm.rayleigh_alpha;
-  c.rb = I->prm.rayleigh_beta;
-  c.invDt = 1.0 / I->prm.dt;
-  c.invDt2 = c.invDt * c.invDt;
-  c.D = I->D;
-  c.E = I->E;
-  c.C = I->C;
-  HIPCHK(I->ctx, launch_constant_parts<T>(c, s));
-  return HOBAK_OK;
-}
-
-// Each partial array holds one entry per workgroup of its producing kernel; refuse to launch if
-// the grids (functions of nv and the slice height) no longer match the arrays sized at creation.
-int check_partials(hobak_inst* I) {
-  if (rhs_grid(I->nv) != I->n_rhs || spmv_grid(I->nv, I->P.shift) != I->n_spmv ||
-      update_grid(I->nv) != I->n_update)
-    return fail(I->ctx, HOBAK_E_INVALID, "partial-sum arrays do not match the launch grids");
-  return HOBAK_OK;
-}
-
-// Run PCG chunks until the device says the solve is done.  Reads back the scalars.
-template <typename T>
-int run_pcg(hobak_inst* I) {
-  hipStream_t s = I->ctx->stream;
-  CgArgs a = cg_args(I);
-  const int maxIt = I->prm.pcg_max_iters > 0 ? I->prm.pcg_max_iters : 2 * I->dofs;
-  int launched = 0;
-  int chunk = I->prev_iters > I->poll_chunk ? I->prev_iters - I->poll_chunk / 2 : I->poll_chunk;
-  // sampled launches: (iteration index, event pair slot)
-  std::vector<std::pair<int, int>> samples;
-  const int max_samples = (int)I->sample_ev.size() / 2;
-  while (true) {
-    for (int k = 0; k < chunk; k++) {
-      a.it = launched + k;
-      const bool sample = I->sample_every > 0 && (a.it % I->sample_every) == 0 &&
-                          (int)samples.size() < max_samples;
-      if (sample) {
-        const int slot = (int)samples.size();
-        HIPCHK(I->ctx, launch_cg_spmv<T>(a, s, I->sample_ev[2 * slot], I->sample_ev[2 * slot + 1]));
-        samples.emplace_back(a.it, slot);
-        HIPCHK(I->ctx, launch_cg_rest<T>(a, s));
-      } else {
-        HIPCHK(I->ctx, launch_cg_iteration<T>(a, s));
-      }
-    }
-    launched += chunk;
-    // after `launched` iterations the current scalars are in set launched & 1
-    HIPCHK(I->ctx, hipMemcpyAsync(I->sc_host, I->sc + (launched & 1), sizeof(CgScalars),
-                                  hipMemcpyDeviceToHost, s));
-    HIPCHK(I->ctx, hipStreamSynchronize(s));
-    if (I->sc_host->done) break;
-    if (launched > maxIt + 64) return fail(I->ctx, HOBAK_E_HIP, "PCG loop did not terminate");
-    chunk = I->poll_chunk;
-  }
-  I->last_it = launched;
-  // SpMV launches 0..iters did work (the converging pass included); later ones returned at once.
-  for (auto& sm : samples) {
-    if (sm.first > I->sc_host->iters) continue;
-    float ms = 0.f;
-    HIPCHK(I->ctx, hipEventElapsedTime(&ms, I->sample_ev[2 * sm.second], I->sample_ev[2 * sm.second + 1]));
-    I->spmv_ms_sum += ms;
-    I->spmv_count += 1;
-  }
-  return HOBAK_OK;
-}
-
-// FP32 PCG inside FP64 iterative refinement: returns inner iterations, sweeps, status.
-template <typename T>
-int solve_refined(hobak_inst* I, int& iters, int& sweeps, double& err, int& status) {
-  hipStream_t s = I->ctx->stream;
-  hobak_ctx* ctx = I->ctx;
-  const int max_sweeps = 32;
-  iters = 0;
-  sweeps = 0;
-  status = HOBAK_OK;
-  const RefineArgs ri = refine_args(I, false);
-  const RefineArgs rr = refine_args(I, true);
-  HIPCHK(ctx, launch_refine_begin(ri, Partials{I->part_rhs_a, nullptr, I->n_rhs}, s));
-  HIPCHK(ctx, hipMemcpyAsync(I->rstate_host, I->rstate, sizeof(RefineState), hipMemcpyDeviceToHost, s));
+  HIPCHK(ctx, hipMemsetAsync(I->h.tmp64, 0, sizeof(double) * I->dofs, s));
+  HIPCHK(ctx, launch_element<T>(P.b[K_ELEM], ELEM_TMP, 1, ctx->grid_cap, s));
+  HIPCHK(ctx, launch_assemble<T>(P.b[K_SLOT], ASM_REST, ctx->grid_cap, s));
+  HIPCHK(ctx, launch_constant_parts<T>(P.b[K_SLOT], ctx->grid_cap, s));
   HIPCHK(ctx, hipStreamSynchronize(s));
-  InitArgs ia = init_args(I);
-  const double tau0 = I->prm.refine_inner_tol > 0 ? I->prm.refine_inner_tol : 1e-4;
-  const double outer = I->prm.pcg_tol > 0 ? I->prm.pcg_tol : 2.220446049250313e-16;
-  ia.tol = tau0;
-  while (!I->rstate_host->done && sweeps < max_sweeps) {
-    // After the first sweep, aim each correction solve at the outer target with a 5x margin
-    // (0.2 tol |b| / |res|), floored at FP32's reach, so the last sweep overshoots the target
-    // instead of stopping just short of it.
-    if (sweeps > 0 && I->rstate_host->res2 > 0) {
-      const double want = 0.2 * outer * std::sqrt(I->rstate_host->b2 / I->rstate_host->res2);
-      ia.tol = std::min(tau0, std::max(want, 1e-6));
-    }
-    HIPCHK(ctx, launch_refine_init<T>(ri, s));
-    HIPCHK(ctx, launch_cg_init(ia, s));
-    TRY(run_pcg<T>(I));
-    iters += I->sc_host->iters;
-    if (I->sc_host->status == 2) status = HOBAK_E_NONFINITE;
-    HIPCHK(ctx, launch_refine_residual<T>(rr, s));
-    HIPCHK(ctx, launch_refine_check(rr, s));
-    HIPCHK(ctx, hipMemcpyAsync(I->rstate_host, I->rstate, sizeof(RefineState), hipMemcpyDeviceToHost, s));
-    HIPCHK(ctx, hipStreamSynchronize(s));
-    sweeps++;
-    if (status != HOBAK_OK) break;
-  }
-  const RefineState& h = *I->rstate_host;
-  err = h.b2 > 0 ? std::sqrt(h.res2 / h.b2) : 0.0;
-  if (h.status == 2) status = HOBAK_E_NONFINITE;
-  else if (!h.done && status == HOBAK_OK) status = HOBAK_E_NOT_CONVERGED;
   return HOBAK_OK;
-}
-
-template <typename T>
-int step_impl(hobak_inst* I, hobak_step_stats* st) {
-  hipStream_t s = I->ctx->stream;
-  hobak_ctx* ctx = I->ctx;
-  const int N = std::max(1, std::min(I->prm.newton_iters, HOBAK_MAX_NEWTON));
-  const int bdf = N > 1;
-  const bool refine = I->prm.precision == HOBAK_PRECISION_FP32_REFINE;
-  const double invDt = 1.0 / I->prm.dt;
-  double ms[5] = {0, 0, 0, 0, 0};
-  TRY(check_partials(I));
-  HIPCHK(ctx, launch_begin_step(I->nv, I->u, I->v, I->u_old, I->v_old, I->pinned, I->pin_target, s));
-  int done = 0;
-  double maxR = 10 * I->prm.newton_tol;
-  int total = 0;
-  int status = HOBAK_OK;
-  while (done < N && (N == 1 || maxR > I->prm.newton_tol)) {
-    HIPCHK(ctx, hipEventRecord(I->ev[0], s));
-    HIPCHK(ctx, launch_element<T>(element_args(I, I->u, 1), s));
-    HIPCHK(ctx, hipEventRecord(I->ev[1], s));
-    HIPCHK(ctx, launch_assemble<T>(asm_args(I, 1, I->A), s));
-    HIPCHK(ctx, hipEventRecord(I->ev[2], s));
-    RhsArgs ra = rhs_args(I, bdf);
-    ra.init_cg = refine ? 0 : 1;
-    HIPCHK(ctx, launch_rhs_init<T>(ra, s));
-    int iters = 0, sweeps = 0, st_solve = HOBAK_OK;
-    double err = 0.0;
-    if (refine) {
-      HIPCHK(ctx, hipEventRecord(I->ev[3], s));
-      TRY(solve_refined<T>(I, iters, sweeps, err, st_solve));
-      maxR = I->rstate_host->b2;
-      HIPCHK(ctx, hipEventRecord(I->ev[4], s));
-      HIPCHK(ctx, launch_apply_dx<double>(I->dofs, I->u, I->x64, s));
-    } else {
-      HIPCHK(ctx, launch_cg_init(init_args(I), s));
-      HIPCHK(ctx, hipEventRecord(I->ev[3], s));
-      TRY(run_pcg<T>(I));
-      HIPCHK(ctx, hipEventRecord(I->ev[4], s));
-      HIPCHK(ctx, launch_apply_dx<T>(I->dofs, I->u, (const T*)I->x, s));
-      const CgScalars& h = *I->sc_host;
-      maxR = h.maxR;
-      iters = h.iters;
-      err = h.rhsNorm2 > 0 ? std::sqrt(h.resNorm2 / h.rhsNorm2) : 0.0;
-      if (h.status == 2) st_solve = HOBAK_E_NONFINITE;
-      else if (h.status == 1) st_solve = HOBAK_E_NOT_CONVERGED;
-    }
-    float e01, e12, e23, e34;
-    HIPCHK(ctx, hipEventSynchronize(I->ev[4]));
-    (void)hipEventElapsedTime(&e01, I->ev[0], I->ev[1]);
-    (void)hipEventElapsedTime(&e12, I->ev[1], I->ev[2]);
-    (void)hipEventElapsedTime(&e23, I->ev[2], I->ev[3]);
-    (void)hipEventElapsedTime(&e34, I->ev[3], I->ev[4]);
-    ms[0] += e01;
-    ms[1] += e12;
-    ms[2] += e23;
-    ms[3] += e34;
-    if (st) {
-      st->pcg_iters[done] = iters;
-      st->refine_iters[done] = sweeps;
-      st->pcg_error[done] = err;
-      st->newton_residual[done] = maxR;
-    }
-    I->prev_iters = refine ? -1 : iters;
-    total += iters;
-    if (st_solve == HOBAK_E_NONFINITE) status = HOBAK_E_NONFINITE;
-    else if (st_solve == HOBAK_E_NOT_CONVERGED && status == HOBAK_OK) status = HOBAK_E_NOT_CONVERGED;
-    done++;
-  }
-  HIPCHK(ctx, hipEventRecord(I->ev[0], s));
-  HIPCHK(ctx, launch_end_step(I->nv, invDt, I->u, I->u_old, I->v, I->pinned, s));
-  HIPCHK(ctx, hipEventRecord(I->ev[5], s));
-  HIPCHK(ctx, hipEventSynchronize(I->ev[5]));
-  float e;
-  (void)hipEventElapsedTime(&e, I->ev[0], I->ev[5]);
-  ms[4] = e;
-  std::memcpy(I->last_ms, ms, sizeof(ms));
-  I->has_system = true;
-  if (st) {
-    st->newton_iters = done;
-    st->total_pcg_iters = total;
-    st->status = status;
-  }
-  return status;
 }
 
 void free_inst(hobak_inst* I) {
@@ -614,13 +902,6 @@ void free_inst(hobak_inst* I) {
     if (I->ctx) I->ctx->in_use -= a.second;
   }
   I->allocs.clear();
-  if (I->sc_host) (void)hipHostFree(I->sc_host);
-  if (I->rstate_host) (void)hipHostFree(I->rstate_host);
-  for (auto& e : I->sample_ev)
-    if (e) (void)hipEventDestroy(e);
-  I->sample_ev.clear();
-  for (auto& e : I->ev)
-    if (e) (void)hipEventDestroy(e);
 }
 
 template <typename T>
@@ -633,6 +914,28 @@ int download_sell_as_bsr(hobak_inst* I, const void* dev, double* out, double sig
     const int slot = I->P.bsr_slot[k];
     for (int e = 0; e < 9; e++) out[9 * (size_t)k + e] = sign * (double)h[e * ns + slot];
   }
+  return HOBAK_OK;
+}
+
+template <typename T>
+int eval_forces_hessian(hobak_inst* I, const double* u, double* forces, double* kvals) {
+  hobak_ctx* ctx = I->ctx;
+  hipStream_t s = ctx->stream;
+  ctx->arena_off = 0;
+  Plan P;
+  TRY(plan_transient(ctx, {I}, P));
+  TRY(upload(I, I->h.tmp64, u, (size_t)I->dofs));
+  HIPCHK(ctx, launch_element<T>(P.b[K_ELEM], ELEM_TMP, kvals != nullptr, ctx->grid_cap, s));
+  if (forces) {
+    HIPCHK(ctx, launch_force_gather<T>(P.b[K_ROW], ctx->grid_cap, s));
+    HIPCHK(ctx, hipMemcpyAsync(forces, I->h.rhs64, sizeof(double) * I->dofs, hipMemcpyDeviceToHost, s));
+  }
+  I->has_system = false;  // rhs64 and A are reused
+  if (kvals) {
+    HIPCHK(ctx, launch_assemble<T>(P.b[K_SLOT], ASM_PLAIN_A, ctx->grid_cap, s));
+    TRY(download_sell_as_bsr<T>(I, I->h.A, kvals, -1.0));
+  }
+  HIPCHK(ctx, hipStreamSynchronize(s));
   return HOBAK_OK;
 }
 
@@ -672,6 +975,12 @@ int hobak_ctx_create(int device, size_t hbm_budget_bytes, hobak_ctx** out) {
     delete ctx;
     return HOBAK_E_HIP;
   }
+  bool ok = true;
+  for (auto& e : ctx->ev) ok = ok && hipEventCreate(&e) == hipSuccess;
+  if (!ok || arena_ensure(ctx, 1 << 20) != HOBAK_OK) {
+    hobak_ctx_destroy(ctx);
+    return HOBAK_E_HIP;
+  }
   *out = ctx;
   return HOBAK_OK;
 }
@@ -679,10 +988,20 @@ int hobak_ctx_create(int device, size_t hbm_budget_bytes, hobak_ctx** out) {
 void hobak_ctx_destroy(hobak_ctx* ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
-  (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  std::vector<hobak_sched*> ss(ctx->scheds.begin(), ctx->scheds.end());
+  for (auto* S : ss) hobak_sched_destroy(S);
   std::vector<hobak_inst*> all(ctx->insts.begin(), ctx->insts.end());
   for (auto* I : all) hobak_inst_destroy(I);
-  (void)hipStreamDestroy(ctx->stream);
+  for (auto* P : ctx->plans) plan_free(P);
+  ctx->plans.clear();
+  if (ctx->arena_dev) (void)hipFree(ctx->arena_dev);
+  if (ctx->arena_host) (void)hipHostFree(ctx->arena_host);
+  for (auto& e : ctx->ev)
+    if (e) (void)hipEventDestroy(e);
+  for (auto& e : ctx->sample_ev)
+    if (e) (void)hipEventDestroy(e);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
 
@@ -691,8 +1010,9 @@ const char* hobak_last_error(hobak_ctx* ctx) { return ctx ? ctx->err.c_str() : "
 size_t hobak_ctx_bytes_in_use(hobak_ctx* ctx) { return ctx ? ctx->in_use : 0; }
 
 size_t hobak_inst_bytes_estimate(int32_t nv, int32_t nt, int32_t precision) {
+  // a Kuhn lattice has ~15 blocks per row; padding of the slices adds a few percent
   const size_t nnzb = (size_t)nv * 16;
-  return estimate_bytes(nv, nt, (int)nnzb, (int)(nnzb * 1.15), precision);
+  return estimate_bytes(nv, nt, (int)(nnzb * 1.15), precision);
 }
 
 int hobak_inst_create(hobak_ctx* ctx, const double* rest_xyz, int32_t nv, const int32_t* tets,
@@ -712,7 +1032,7 @@ int hobak_inst_create(hobak_ctx* ctx, const double* rest_xyz, int32_t nv, const 
   for (size_t i = 0; i < 4 * (size_t)nt; i++)
     if (tets[i] < 0 || tets[i] >= nv) return fail(ctx, HOBAK_E_INVALID, "tet index out of range");
   // rest volumes must be positive (TET_MESH.cpp:132-137)
-  std::vector<double> vol(nt), dminv(9 * (size_t)nt), one_ring(nv, 0.0);
+  std::vector<double> vol_h(nt), dminv_h(9 * (size_t)nt), one_ring_h(nv, 0.0);
   for (int t = 0; t < nt; t++) {
     const int32_t* T = tets + 4 * (size_t)t;
     double d[3][3];
@@ -721,8 +1041,8 @@ int hobak_inst_create(hobak_ctx* ctx, const double* rest_xyz, int32_t nv, const 
     const double cx = d[0][1] * d[1][2] - d[0][2] * d[1][1];
     const double cy = d[0][2] * d[1][0] - d[0][0] * d[1][2];
     const double cz = d[0][0] * d[1][1] - d[0][1] * d[1][0];
-    vol[t] = (d[2][0] * cx + d[2][1] * cy + d[2][2] * cz) / 6.0;
-    if (!(vol[t] > 0.0)) return fail(ctx, HOBAK_E_INVALID, "tet with non-positive rest volume");
+    vol_h[t] = (d[2][0] * cx + d[2][1] * cy + d[2][2] * cz) / 6.0;
+    if (!(vol_h[t] > 0.0)) return fail(ctx, HOBAK_E_INVALID, "tet with non-positive rest volume");
     // Dm = [d0 d1 d2] as columns; inverse by cofactors (Eigen InverseImpl.h:126-169 order)
     double m[3][3];
     for (int r = 0; r < 3; r++)
@@ -741,11 +1061,11 @@ int hobak_inst_create(hobak_ctx* ctx, const double* rest_xyz, int32_t nv, const 
     for (int row = 1; row < 3; row++)
       for (int k = 0; k < 3; k++) inv[row][k] = cof(k, row) * invdet;
     for (int r = 0; r < 3; r++)
-      for (int c = 0; c < 3; c++) dminv[(3 * r + c) * (size_t)nt + t] = inv[r][c];
+      for (int c = 0; c < 3; c++) dminv_h[(3 * r + c) * (size_t)nt + t] = inv[r][c];
   }
   for (int t = 0; t < nt; t++) {  // one-ring volumes, ascending tet (TET_MESH.cpp:156-173)
-    const double q = 0.25 * vol[t];
-    for (int k = 0; k < 4; k++) one_ring[tets[4 * (size_t)t + k]] += q;
+    const double q = 0.25 * vol_h[t];
+    for (int k = 0; k < 4; k++) one_ring_h[tets[4 * (size_t)t + k]] += q;
   }
   (void)hipSetDevice(ctx->device);
   hobak_inst* I = new (std::nothrow) hobak_inst;
@@ -757,6 +1077,7 @@ int hobak_inst_create(hobak_ctx* ctx, const double* rest_xyz, int32_t nv, const 
   I->mat = *material;
   I->prm = *params;
   I->prm.newton_iters = std::max(1, std::min(params->newton_iters, HOBAK_MAX_NEWTON));
+  for (auto& p : I->pred) p = -1;
   if (material->kind == HOBAK_MAT_SNH) {  // SNH.cpp:23-32 reparameterisation
     I->lam_eff = material->lambda + material->mu;
     I->alpha = 1.0 + material->mu / I->lam_eff;
@@ -766,9 +1087,8 @@ int hobak_inst_create(hobak_ctx* ctx, const double* rest_xyz, int32_t nv, const 
   }
   I->tsize = params->precision == HOBAK_PRECISION_FP64 ? 8 : 4;
   build_pattern(nv, nt, tets, ctx->slice_rows, I->P);
-  I->poll_chunk = ctx->poll_chunk;
   const Pattern& P = I->P;
-  const size_t need = estimate_bytes(nv, nt, P.nnzb, P.nslots, params->precision);
+  const size_t need = estimate_bytes(nv, nt, P.nslots, params->precision);
   if (ctx->budget && ctx->in_use + need > ctx->budget) {
     delete I;
     return fail(ctx, HOBAK_E_NOMEM, "HBM budget would be exceeded");
@@ -778,30 +1098,56 @@ int hobak_inst_create(hobak_ctx* ctx, const double* rest_xyz, int32_t nv, const 
     delete I;
     return rc;
   };
+  InstDev& h = I->h;
+  h.nv = nv;
+  h.nt = nt;
+  h.nslots = P.nslots;
+  h.shift = P.shift;
+  for (int k = 0; k < K_COUNT; k++) h.nb[k] = blocks_of(k, nv, nt, P.nslots, P.shift);
+  h.material = material->kind;
+  h.precond = params->precond;
+  h.bdf = I->prm.newton_iters > 1;
+  h.ground_on = params->ground_on;
+  h.refine = params->precision == HOBAK_PRECISION_FP32_REFINE;
+  h.max_iters = params->pcg_max_iters > 0 ? params->pcg_max_iters : 2 * I->dofs;
+  h.mu = material->mu;
+  h.lam = I->lam_eff;
+  h.alpha = I->alpha;
+  h.invDt = 1.0 / params->dt;
+  h.invDt2 = h.invDt * h.invDt;
+  h.ra = params->rayleigh_alpha;
+  h.rb = params->rayleigh_beta;
+  for (int c = 0; c < 3; c++) h.g[c] = params->gravity[c];
+  h.ground_y = params->ground_y;
+  h.ground_k = params->ground_k;
+  h.tol = params->pcg_tol > 0 ? params->pcg_tol : kEps;
+  h.tau0 = params->refine_inner_tol > 0 ? params->refine_inner_tol : 1e-4;
   const size_t ts = I->tsize;
   const size_t np = P.nrows_pad;
   int rc = HOBAK_OK;
 #define ALLOC(ptr, n) \
-  if ((rc = dalloc(I, &I->ptr, (n))) != HOBAK_OK) return bail(rc)
+  if ((rc = dalloc(I, &ptr, (n))) != HOBAK_OK) return bail(rc)
 #define ALLOCT(ptr, n) \
-  if ((rc = dalloc(I, (char**)&I->ptr, (n), ts)) != HOBAK_OK) return bail(rc)
+  if ((rc = dalloc(I, (char**)&ptr, (n), ts)) != HOBAK_OK) return bail(rc)
+  double *rest, *mass, *one_ring, *pin_target, *dminv, *vol;
+  int4* t4d;
+  int *slice_off, *row_len, *col, *slot_row, *cptr, *hdest, *vptr, *fdest;
+  unsigned char* pinned;
   ALLOC(rest, 3 * (size_t)nv);
-  ALLOC(u, 3 * (size_t)nv);
-  ALLOC(v, 3 * (size_t)nv);
-  ALLOC(u_old, 3 * (size_t)nv);
-  ALLOC(v_old, 3 * (size_t)nv);
+  ALLOC(h.u, 3 * (size_t)nv);
+  ALLOC(h.v, 3 * (size_t)nv);
+  ALLOC(h.u_old, 3 * (size_t)nv);
+  ALLOC(h.v_old, 3 * (size_t)nv);
   ALLOC(mass, 3 * (size_t)nv);
   ALLOC(one_ring, (size_t)nv);
   ALLOC(pin_target, 3 * (size_t)nv);
-  ALLOC(rhs64, 3 * (size_t)nv);
-  ALLOC(tmp64, 3 * (size_t)nv);
+  ALLOC(h.rhs64, 3 * (size_t)nv);
+  ALLOC(h.tmp64, 3 * (size_t)nv);
+  ALLOC(h.x64, 3 * (size_t)nv);
+  ALLOC(h.res64, 3 * (size_t)nv);
   ALLOC(dminv, 9 * (size_t)nt);
   ALLOC(vol, (size_t)nt);
-  ALLOCT(kneg, 9 * (size_t)P.nslots);
-  ALLOC(x64, 3 * (size_t)nv);
-  ALLOC(res64, 3 * (size_t)nv);
-  ALLOC(rstate, 1);
-  ALLOC(tets, (size_t)nt);
+  ALLOC(t4d, (size_t)nt);
   ALLOC(slice_off, P.slice_off.size());
   ALLOC(row_len, (size_t)nv);
   ALLOC(col, (size_t)P.nslots);
@@ -811,59 +1157,78 @@ int hobak_inst_create(hobak_ctx* ctx, const double* rest_xyz, int32_t nv, const 
   ALLOC(vptr, P.vptr.size());
   ALLOC(fdest, P.fdest.size());
   ALLOC(pinned, (size_t)nv);
-  ALLOCT(stage_h, (size_t)9 * P.cptr[P.nslots]);
-  ALLOCT(stage_f, (size_t)3 * P.vptr[nv]);
-  ALLOCT(A, 9 * (size_t)P.nslots);
-  ALLOCT(D, 9 * (size_t)P.nslots);
-  ALLOCT(E, 9 * (size_t)P.nslots);
-  ALLOCT(C, 9 * (size_t)P.nslots);
-  ALLOCT(minv, 9 * np);
-  ALLOCT(r, 3 * np);
-  ALLOCT(x, 3 * np);
-  ALLOCT(z, 3 * np);
-  ALLOCT(p0, 3 * np);
-  ALLOCT(p1, 3 * np);
-  ALLOCT(t, 3 * np);
-  ALLOC(sc, 2);
-  I->n_rhs = rhs_grid(nv);
-  I->n_spmv = spmv_grid(nv, P.shift);
-  I->n_update = update_grid(nv);
-  ALLOC(part_rhs_a, (size_t)I->n_rhs);
-  ALLOC(part_rhs_b, (size_t)I->n_rhs);
-  ALLOC(part_pt, (size_t)I->n_spmv);
-  ALLOC(part_rr, (size_t)I->n_update);
-  ALLOC(part_rz, (size_t)I->n_update);
+  ALLOCT(h.kneg, 9 * (size_t)P.nslots);
+  ALLOCT(h.stage_h, (size_t)9 * P.cptr[P.nslots]);
+  ALLOCT(h.stage_f, (size_t)3 * P.vptr[nv]);
+  ALLOCT(h.A, 9 * (size_t)P.nslots);
+  ALLOCT(h.D, 9 * (size_t)P.nslots);
+  ALLOCT(h.E, 9 * (size_t)P.nslots);
+  ALLOCT(h.C, 9 * (size_t)P.nslots);
+  ALLOCT(h.minv, 9 * np);
+  ALLOCT(h.r, 3 * np);
+  ALLOCT(h.x, 3 * np);
+  ALLOCT(h.z, 3 * np);
+  ALLOCT(h.p0, 3 * np);
+  ALLOCT(h.p1, 3 * np);
+  ALLOCT(h.t, 3 * np);
+  ALLOC(h.ctl, 1);
+  ALLOC(h.part_row_a, (size_t)h.nb[K_ROW]);
+  ALLOC(h.part_row_b, (size_t)h.nb[K_ROW]);
+  ALLOC(h.part_pt, (size_t)h.nb[K_SPMV]);
+  ALLOC(h.part_res, (size_t)h.nb[K_SPMV]);
+  ALLOC(h.part_rr, (size_t)h.nb[K_UPD]);
+  ALLOC(h.part_rz, (size_t)h.nb[K_UPD]);
+  ALLOC(I->dev, 1);
 #undef ALLOC
 #undef ALLOCT
-  if (hipHostMalloc((void**)&I->sc_host, sizeof(CgScalars), hipHostMallocDefault) != hipSuccess ||
-      hipHostMalloc((void**)&I->rstate_host, sizeof(RefineState), hipHostMallocDefault) != hipSuccess)
-    return bail(fail(ctx, HOBAK_E_HIP, "hipHostMalloc failed"));
-  for (auto& e : I->ev)
-    if (hipEventCreate(&e) != hipSuccess) return bail(fail(ctx, HOBAK_E_HIP, "hipEventCreate failed"));
-  std::vector<double> mass(3 * (size_t)nv);
+  h.rest = rest;
+  h.mass = mass;
+  h.one_ring = one_ring;
+  h.pin_target = pin_target;
+  h.dminv = dminv;
+  h.vol = vol;
+  h.tets = t4d;
+  h.slice_off = slice_off;
+  h.row_len = row_len;
+  h.col = col;
+  h.slot_row = slot_row;
+  h.cptr = cptr;
+  h.hdest = hdest;
+  h.vptr = vptr;
+  h.fdest = fdest;
+  h.pinned = pinned;
+  I->ctl = h.ctl;
+  // no solve active: both scalar sets done, so every PCG kernel passes this instance over
+  Ctl ctl0;
+  std::memset(&ctl0, 0, sizeof(ctl0));
+  ctl0.sc[0].done = ctl0.sc[1].done = 1;
+  ctl0.phase = PHASE_IDLE;
+  std::vector<double> mass_h(3 * (size_t)nv);
   for (int i = 0; i < nv; i++)  // buildMassMatrix + scaleMass
-    for (int c = 0; c < 3; c++) mass[3 * (size_t)i + c] = one_ring[i] * params->mass_scale;
+    for (int c = 0; c < 3; c++) mass_h[3 * (size_t)i + c] = one_ring_h[i] * params->mass_scale;
   std::vector<int4> t4(nt);
   for (int t = 0; t < nt; t++) t4[t] = make_int4(tets[4 * t], tets[4 * t + 1], tets[4 * t + 2], tets[4 * t + 3]);
-  if ((rc = upload(I, I->rest, rest_xyz, 3 * (size_t)nv)) ||
-      (rc = upload(I, I->mass, mass.data(), mass.size())) ||
-      (rc = upload(I, I->one_ring, one_ring.data(), one_ring.size())) ||
-      (rc = upload(I, I->dminv, dminv.data(), dminv.size())) ||
-      (rc = upload(I, I->vol, vol.data(), vol.size())) ||
-      (rc = upload(I, I->tets, t4.data(), t4.size())) ||
-      (rc = upload(I, I->slice_off, P.slice_off.data(), P.slice_off.size())) ||
-      (rc = upload(I, I->row_len, P.row_len.data(), P.row_len.size())) ||
-      (rc = upload(I, I->col, P.col.data(), P.col.size())) ||
-      (rc = upload(I, I->slot_row, P.slot_row.data(), P.slot_row.size())) ||
-      (rc = upload(I, I->cptr, P.cptr.data(), P.cptr.size())) ||
-      (rc = upload(I, I->hdest, P.hdest.data(), P.hdest.size())) ||
-      (rc = upload(I, I->vptr, P.vptr.data(), P.vptr.size())) ||
-      (rc = upload(I, I->fdest, P.fdest.data(), P.fdest.size())))
-    return bail(rc);
-  if ((rc = (I->tsize == 8 ? setup_constant_parts<double>(I) : setup_constant_parts<float>(I))) != HOBAK_OK)
+  if ((rc = upload(I, rest, rest_xyz, 3 * (size_t)nv)) ||
+      (rc = upload(I, mass, mass_h.data(), mass_h.size())) ||
+      (rc = upload(I, one_ring, one_ring_h.data(), one_ring_h.size())) ||
+      (rc = upload(I, dminv, dminv_h.data(), dminv_h.size())) ||
+      (rc = upload(I, vol, vol_h.data(), vol_h.size())) ||
+      (rc = upload(I, t4d, t4.data(), t4.size())) ||
+      (rc = upload(I, slice_off, P.slice_off.data(), P.slice_off.size())) ||
+      (rc = upload(I, row_len, P.row_len.data(), P.row_len.size())) ||
+      (rc = upload(I, col, P.col.data(), P.col.size())) ||
+      (rc = upload(I, slot_row, P.slot_row.data(), P.slot_row.size())) ||
+      (rc = upload(I, cptr, P.cptr.data(), P.cptr.size())) ||
+      (rc = upload(I, hdest, P.hdest.data(), P.hdest.size())) ||
+      (rc = upload(I, vptr, P.vptr.data(), P.vptr.size())) ||
+      (rc = upload(I, fdest, P.fdest.data(), P.fdest.size())) ||
+      (rc = upload(I, I->ctl, &ctl0, 1)) ||
+      (rc = upload(I, I->dev, &h, 1)))
     return bail(rc);
   if (hipStreamSynchronize(ctx->stream) != hipSuccess)
-    return bail(fail(ctx, HOBAK_E_HIP, "set-up kernels failed"));
+    return bail(fail(ctx, HOBAK_E_HIP, "upload failed"));
+  if ((rc = (I->tsize == 8 ? setup_constant_parts<double>(I) : setup_constant_parts<float>(I))) != HOBAK_OK)
+    return bail(rc);
   ctx->insts.insert(I);
   *out = I;
   return HOBAK_OK;
@@ -871,15 +1236,19 @@ int hobak_inst_create(hobak_ctx* ctx, const double* rest_xyz, int32_t nv, const 
 
 void hobak_inst_destroy(hobak_inst* I) {
   if (!I) return;
-  (void)hipSetDevice(I->ctx->device);
-  (void)hipStreamSynchronize(I->ctx->stream);
-  I->ctx->insts.erase(I);
+  hobak_ctx* ctx = I->ctx;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  if (I->sched) sched_detach(I->sched, I);
+  plans_forget(ctx, I);
+  ctx->insts.erase(I);
   free_inst(I);
   delete I;
 }
 
 int hobak_inst_set_pinned(hobak_inst* I, const int32_t* vids, int32_t n, const double* targets) {
   if (!I || n < 0 || (n > 0 && !vids)) return HOBAK_E_INVALID;
+  if (I->requests) return fail(I->ctx, HOBAK_E_INVALID, "instance has a step in flight");
   std::vector<unsigned char> flag(I->nv, 0);
   std::vector<double> tgt(3 * (size_t)I->nv, 0.0);
   for (int k = 0; k < n; k++) {
@@ -888,53 +1257,69 @@ int hobak_inst_set_pinned(hobak_inst* I, const int32_t* vids, int32_t n, const d
     for (int c = 0; c < 3; c++) tgt[3 * (size_t)vids[k] + c] = targets ? targets[3 * (size_t)k + c] : 0.0;
   }
   I->pinned_ids.assign(vids, vids + n);
-  TRY(upload(I, I->pinned, flag.data(), flag.size()));
-  TRY(upload(I, I->pin_target, tgt.data(), tgt.size()));
+  TRY(upload(I, (unsigned char*)I->h.pinned, flag.data(), flag.size()));
+  TRY(upload(I, (double*)I->h.pin_target, tgt.data(), tgt.size()));
   HIPCHK(I->ctx, hipStreamSynchronize(I->ctx->stream));
   return HOBAK_OK;
 }
 
 int hobak_inst_set_state(hobak_inst* I, const double* u, const double* v) {
   if (!I) return HOBAK_E_INVALID;
-  if (u) TRY(upload(I, I->u, u, (size_t)I->dofs));
-  if (v) TRY(upload(I, I->v, v, (size_t)I->dofs));
+  if (I->requests) return fail(I->ctx, HOBAK_E_INVALID, "instance has a step in flight");
+  if (u) TRY(upload(I, I->h.u, u, (size_t)I->dofs));
+  if (v) TRY(upload(I, I->h.v, v, (size_t)I->dofs));
   HIPCHK(I->ctx, hipStreamSynchronize(I->ctx->stream));
-  I->prev_iters = -1;
+  for (auto& p : I->pred) p = -1;
   return HOBAK_OK;
 }
 
 int hobak_inst_get_state(hobak_inst* I, double* u, double* v) {
   if (!I) return HOBAK_E_INVALID;
   hipStream_t s = I->ctx->stream;
-  if (u) HIPCHK(I->ctx, hipMemcpyAsync(u, I->u, sizeof(double) * I->dofs, hipMemcpyDeviceToHost, s));
-  if (v) HIPCHK(I->ctx, hipMemcpyAsync(v, I->v, sizeof(double) * I->dofs, hipMemcpyDeviceToHost, s));
+  if (u) HIPCHK(I->ctx, hipMemcpyAsync(u, I->h.u, sizeof(double) * I->dofs, hipMemcpyDeviceToHost, s));
+  if (v) HIPCHK(I->ctx, hipMemcpyAsync(v, I->h.v, sizeof(double) * I->dofs, hipMemcpyDeviceToHost, s));
   HIPCHK(I->ctx, hipStreamSynchronize(s));
   return HOBAK_OK;
 }
 
-int hobak_step(hobak_inst* I, hobak_step_stats* st) {
-  if (!I) return HOBAK_E_INVALID;
-  (void)hipSetDevice(I->ctx->device);
+int hobak_step_batch(hobak_ctx* ctx, hobak_inst* const* insts, int32_t n, hobak_step_stats* stats) {
+  if (!ctx || n < 0 || (n > 0 && !insts)) return HOBAK_E_INVALID;
+  if (n == 0) return HOBAK_OK;
+  (void)hipSetDevice(ctx->device);
+  std::vector<hobak_inst*> g64, g32;
+  std::vector<int> i64, i32;
+  std::set<hobak_inst*> seen;
+  for (int i = 0; i < n; i++) {
+    hobak_inst* I = insts[i];
+    if (!I || I->ctx != ctx) return fail(ctx, HOBAK_E_INVALID, "instance of another context");
+    if (!seen.insert(I).second) return fail(ctx, HOBAK_E_INVALID, "instance listed twice in one batch");
+    if (I->requests) return fail(ctx, HOBAK_E_INVALID, "instance has a step in flight");
+    (I->tsize == 8 ? g64 : g32).push_back(I);
+    (I->tsize == 8 ? i64 : i32).push_back(i);
+  }
   auto t0 = std::chrono::steady_clock::now();
-  hobak_step_stats local;
-  std::memset(&local, 0, sizeof(local));
-  int rc = I->tsize == 8 ? step_impl<double>(I, &local) : step_impl<float>(I, &local);
-  auto t1 = std::chrono::steady_clock::now();
-  local.seconds = std::chrono::duration<double>(t1 - t0).count();
-  if (st) *st = local;
-  return rc;
+  int worst = HOBAK_OK;
+  for (int pass = 0; pass < 2; pass++) {
+    const auto& g = pass ? g32 : g64;
+    const auto& ix = pass ? i32 : i64;
+    if (g.empty()) continue;
+    std::vector<hobak_step_stats> st(g.size());
+    std::memset(st.data(), 0, sizeof(hobak_step_stats) * st.size());
+    const int rc = run_batch(ctx, g, st.data());
+    if (rc == HOBAK_E_HIP || rc == HOBAK_E_INVALID || rc == HOBAK_E_NOMEM) return rc;
+    worst = worse(worst, rc);
+    if (stats)
+      for (size_t j = 0; j < g.size(); j++) stats[ix[j]] = st[j];
+  }
+  const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  if (stats)
+    for (int i = 0; i < n; i++) stats[i].seconds = sec;
+  return worst;
 }
 
-int hobak_step_batch(hobak_ctx* ctx, hobak_inst* const* insts, int32_t n, hobak_step_stats* stats) {
-  if (!ctx || (n > 0 && !insts)) return HOBAK_E_INVALID;
-  int worst = HOBAK_OK;
-  for (int i = 0; i < n; i++) {
-    if (!insts[i] || insts[i]->ctx != ctx) return fail(ctx, HOBAK_E_INVALID, "instance of another context");
-    int rc = hobak_step(insts[i], stats ? &stats[i] : nullptr);
-    if (rc == HOBAK_E_HIP) return rc;
-    if (rc != HOBAK_OK && worst == HOBAK_OK) worst = rc;
-  }
-  return worst;
+int hobak_step(hobak_inst* I, hobak_step_stats* st) {
+  if (!I) return HOBAK_E_INVALID;
+  return hobak_step_batch(I->ctx, &I, 1, st);
 }
 
 int hobak_get_bsr_pattern(hobak_inst* I, int32_t* nnzb, int32_t* row_ptr, int32_t* col_idx) {
@@ -945,37 +1330,9 @@ int hobak_get_bsr_pattern(hobak_inst* I, int32_t* nnzb, int32_t* row_ptr, int32_
   return HOBAK_OK;
 }
 
-}  // extern "C"
-
-namespace {
-template <typename T>
-int eval_forces_hessian(hobak_inst* I, const double* u, double* forces, double* kvals) {
-  hipStream_t s = I->ctx->stream;
-  TRY(upload(I, I->tmp64, u, (size_t)I->dofs));
-  HIPCHK(I->ctx, launch_element<T>(element_args(I, I->tmp64, kvals != nullptr), s));
-  if (forces) {
-    HIPCHK(I->ctx, launch_force_gather<T>(I->nv, I->vptr, (const T*)I->stage_f, I->rhs64, s));
-    HIPCHK(I->ctx, hipMemcpyAsync(forces, I->rhs64, sizeof(double) * I->dofs, hipMemcpyDeviceToHost, s));
-  }
-  if (kvals) {
-    void* tmpK = nullptr;
-    HIPCHK(I->ctx, hipMalloc(&tmpK, sizeof(T) * 9 * (size_t)I->P.nslots));
-    HIPCHK(I->ctx, hipMemsetAsync(tmpK, 0, sizeof(T) * 9 * (size_t)I->P.nslots, s));
-    HIPCHK(I->ctx, launch_assemble<T>(asm_args(I, 0, tmpK), s));
-    int rc = download_sell_as_bsr<T>(I, tmpK, kvals, -1.0);
-    (void)hipFree(tmpK);
-    if (rc != HOBAK_OK) return rc;
-  }
-  HIPCHK(I->ctx, hipStreamSynchronize(s));
-  I->has_system = false;  // rhs64 was reused
-  return HOBAK_OK;
-}
-}  // namespace
-
-extern "C" {
-
 int hobak_eval_forces_hessian(hobak_inst* I, const double* u, double* forces, double* kvals) {
   if (!I || !u) return HOBAK_E_INVALID;
+  if (I->requests) return fail(I->ctx, HOBAK_E_INVALID, "instance has a step in flight");
   (void)hipSetDevice(I->ctx->device);
   return I->tsize == 8 ? eval_forces_hessian<double>(I, u, forces, kvals)
                        : eval_forces_hessian<float>(I, u, forces, kvals);
@@ -986,12 +1343,12 @@ int hobak_get_last_system(hobak_inst* I, double* rhs, double* lhs_vals) {
   if (!I->has_system) return fail(I->ctx, HOBAK_E_INVALID, "no system formed since the last step");
   hipStream_t s = I->ctx->stream;
   if (rhs) {
-    HIPCHK(I->ctx, hipMemcpyAsync(rhs, I->rhs64, sizeof(double) * I->dofs, hipMemcpyDeviceToHost, s));
+    HIPCHK(I->ctx, hipMemcpyAsync(rhs, I->h.rhs64, sizeof(double) * I->dofs, hipMemcpyDeviceToHost, s));
     HIPCHK(I->ctx, hipStreamSynchronize(s));
   }
   if (lhs_vals)
-    TRY(I->tsize == 8 ? download_sell_as_bsr<double>(I, I->A, lhs_vals, 1.0)
-                      : download_sell_as_bsr<float>(I, I->A, lhs_vals, 1.0));
+    TRY(I->tsize == 8 ? download_sell_as_bsr<double>(I, I->h.A, lhs_vals, 1.0)
+                      : download_sell_as_bsr<float>(I, I->h.A, lhs_vals, 1.0));
   return HOBAK_OK;
 }
 
@@ -1005,23 +1362,17 @@ int hobak_set_kernel_sampling(hobak_inst* I, int32_t every) {
   if (!I || every < 0) return HOBAK_E_INVALID;
   I->sample_every = every;
   I->spmv_ms_sum = 0.0;
+  I->spmv_bytes_sum = 0.0;
   I->spmv_count = 0;
-  if (every > 0 && I->sample_ev.empty()) {
-    I->sample_ev.assign(2 * 512, nullptr);
-    for (auto& e : I->sample_ev)
-      HIPCHK(I->ctx, hipEventCreate(&e));
-  }
   return HOBAK_OK;
 }
 
 int hobak_kernel_stats(hobak_inst* I, double* spmv_avg_ms, int64_t* spmv_count, double* spmv_bytes) {
   if (!I) return HOBAK_E_INVALID;
-  if (spmv_avg_ms) *spmv_avg_ms = I->spmv_count ? I->spmv_ms_sum / (double)I->spmv_count : 0.0;
+  const double cnt = (double)I->spmv_count;
+  if (spmv_avg_ms) *spmv_avg_ms = I->spmv_count ? I->spmv_ms_sum / cnt : 0.0;
   if (spmv_count) *spmv_count = I->spmv_count;
-  if (spmv_bytes) {
-    const double sz = (double)I->tsize;
-    *spmv_bytes = (double)I->P.nnzb * (9.0 * sz + 4.0) + 4.0 * (I->nv + 1) + 2.0 * 3.0 * sz * I->nv;
-  }
+  if (spmv_bytes) *spmv_bytes = I->spmv_count ? I->spmv_bytes_sum / cnt : spmv_bytes_of(I);
   return HOBAK_OK;
 }
 
@@ -1060,15 +1411,98 @@ int hobak_ctx_set_tuning(hobak_ctx* ctx, int32_t key, int32_t value) {
     case HOBAK_TUNE_SLICE_ROWS:
       if (value != 8 && value != 16 && value != 32 && value != 64)
         return fail(ctx, HOBAK_E_INVALID, "slice rows must be 8, 16, 32 or 64");
+      if (!ctx->insts.empty() && value != ctx->slice_rows)
+        return fail(ctx, HOBAK_E_INVALID, "the slice height is fixed once instances exist");
       ctx->slice_rows = value;
       return HOBAK_OK;
     case HOBAK_TUNE_POLL_CHUNK:
       if (value < 1 || value > 4096) return fail(ctx, HOBAK_E_INVALID, "poll chunk out of range");
-      ctx->poll_chunk = value;
+      ctx->chunk_iters = (value + 1) & ~1;
+      return HOBAK_OK;
+    case HOBAK_TUNE_MAX_CHUNKS:
+      if (value < 1 || value > 65536) return fail(ctx, HOBAK_E_INVALID, "max chunks out of range");
+      ctx->max_chunks = value;
+      return HOBAK_OK;
+    case HOBAK_TUNE_GRAPHS:
+      if (value != 0 && value != 1) return fail(ctx, HOBAK_E_INVALID, "graphs is 0 or 1");
+      ctx->use_graphs = value;
+      return HOBAK_OK;
+    case HOBAK_TUNE_GRID_CAP:
+      if (value < 0) return fail(ctx, HOBAK_E_INVALID, "grid cap must be >= 0");
+      ctx->grid_cap = value;
       return HOBAK_OK;
     default:
       return fail(ctx, HOBAK_E_INVALID, "unknown tuning key");
   }
 }
+
+// ---- continuous-batching scheduler ----
+int hobak_sched_create(hobak_ctx* ctx, hobak_sched** out) {
+  if (!ctx || !out) return HOBAK_E_INVALID;
+  hobak_sched* S = new (std::nothrow) hobak_sched;
+  if (!S) return fail(ctx, HOBAK_E_NOMEM, "host allocation failed");
+  S->ctx = ctx;
+  ctx->scheds.insert(S);
+  *out = S;
+  return HOBAK_OK;
+}
+
+void hobak_sched_destroy(hobak_sched* S) {
+  if (!S) return;
+  hobak_ctx* ctx = S->ctx;
+  (void)hipStreamSynchronize(ctx->stream);
+  std::vector<hobak_inst*> m = S->members;
+  for (auto* I : m) sched_detach(S, I);
+  ctx->scheds.erase(S);
+  delete S;
+}
+
+int hobak_sched_submit(hobak_sched* S, hobak_inst* I, uint64_t tag) {
+  if (!S || !I) return HOBAK_E_INVALID;
+  hobak_ctx* ctx = S->ctx;
+  if (I->ctx != ctx) return fail(ctx, HOBAK_E_INVALID, "instance of another context");
+  if (I->sched && I->sched != S) return fail(ctx, HOBAK_E_INVALID, "instance is attached to another scheduler");
+  if (!I->sched) {
+    if (!S->members.empty() && (S->members[0]->tsize != I->tsize || S->members[0]->P.shift != I->P.shift))
+      return fail(ctx, HOBAK_E_INVALID, "a scheduler's instances must share precision and slice height");
+    if (I->requests) return fail(ctx, HOBAK_E_INVALID, "instance has a step in flight");
+    S->members.push_back(I);
+    S->dirty = true;
+    I->sched = S;
+  }
+  hobak_sched::Run r;
+  r.I = I;
+  r.tag = tag;
+  r.t_submit = clock_s(S);
+  S->queue.push_back(r);
+  I->requests++;
+  return HOBAK_OK;
+}
+
+int hobak_sched_detach(hobak_sched* S, hobak_inst* I) {
+  if (!S || !I) return HOBAK_E_INVALID;
+  if (I->sched != S) return fail(S->ctx, HOBAK_E_INVALID, "instance is not attached to this scheduler");
+  if (I->requests) return fail(S->ctx, HOBAK_E_INVALID, "instance has a step in flight");
+  sched_detach(S, I);
+  return HOBAK_OK;
+}
+
+int hobak_sched_tick(hobak_sched* S, hobak_completion* out, int32_t max_out, int32_t* n_out,
+                     int32_t* in_flight) {
+  if (!S || max_out < 0 || (max_out > 0 && !out)) return HOBAK_E_INVALID;
+  (void)hipSetDevice(S->ctx->device);
+  if (!S->queue.empty() || !S->active.empty()) {
+    const int rc = tick(S);
+    if (rc != HOBAK_OK) return rc;
+  }
+  const int n = std::min<int>(max_out, (int)S->done.size());
+  for (int i = 0; i < n; i++) out[i] = S->done[i];
+  S->done.erase(S->done.begin(), S->done.begin() + n);
+  if (n_out) *n_out = n;
+  if (in_flight) *in_flight = (int32_t)(S->queue.size() + S->active.size() + S->done.size());
+  return HOBAK_OK;
+}
+
+double hobak_sched_clock(hobak_sched* S) { return S ? clock_s(S) : 0.0; }
 
 }  // extern "C"

@@ -21,13 +21,26 @@
 
 namespace hobak {
 
+// Inputs of one tet's evaluation, read from the instance record.
+struct ElementIn {
+  int nt;
+  double mu, lam, alpha;
+  const int4* tets;
+  const double* rest;
+  const double* u;
+  const double* dminv;
+  const double* vol;
+  void* stage_h;
+  void* stage_f;
+  const int* hdest;
+  const int* fdest;
+  int with_hessian;
+};
+
 template <typename T, int MAT>
-__global__ __launch_bounds__(128) void element_kernel(ElementArgs a) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= a.nt) return;
+__device__ __forceinline__ void element_tet(const ElementIn& a, int t) {
   const size_t nt = (size_t)a.nt;
   const int4 tv = a.tets[t];
-  (void)kStageBlocks;
   const int vid[4] = {tv.x, tv.y, tv.z, tv.w};
 
   // corner positions x = X + u (TET_MESH::setDisplacement) and edge matrix Ds
@@ -158,19 +171,48 @@ __global__ __launch_bounds__(128) void element_kernel(ElementArgs a) {
   }
 }
 
+// Segmented over instances (one lane per tet, kElementBlock tets per virtual workgroup); the
+// material is a per-instance, workgroup-uniform branch.
 template <typename T>
-hipError_t launch_element(const ElementArgs& a, hipStream_t s) {
-  if (a.nt <= 0) return hipSuccess;
-  const int block = 128;
-  const int grid = (a.nt + block - 1) / block;
-  if (a.material == 0)
-    hipLaunchKernelGGL((element_kernel<T, 0>), dim3(grid), dim3(block), 0, s, a);
-  else
-    hipLaunchKernelGGL((element_kernel<T, 1>), dim3(grid), dim3(block), 0, s, a);
+__global__ __launch_bounds__(kElementBlock) void element_kernel(Batch b, int src, int with_hessian) {
+  for (int vb = blockIdx.x; vb < b.total; vb += gridDim.x) {
+    int lb, lg;
+    const InstDev* __restrict__ I = seg_find(b, vb, lb, lg);
+    ElementIn a;
+    a.nt = I->nt;
+    a.mu = I->mu;
+    a.lam = I->lam;
+    a.alpha = I->alpha;
+    a.tets = I->tets;
+    a.rest = I->rest;
+    a.u = src == ELEM_TMP ? I->tmp64 : I->u;
+    a.dminv = I->dminv;
+    a.vol = I->vol;
+    a.stage_h = I->stage_h;
+    a.stage_f = I->stage_f;
+    a.hdest = I->hdest;
+    a.fdest = I->fdest;
+    a.with_hessian = with_hessian;
+    const int material = I->material;
+    const int t = lb * kElementBlock + threadIdx.x;
+    if (t < a.nt) {
+      if (material == 0)
+        element_tet<T, 0>(a, t);
+      else
+        element_tet<T, 1>(a, t);
+    }
+  }
+}
+
+template <typename T>
+hipError_t launch_element(const Batch& b, int src, int with_hessian, int cap, hipStream_t s) {
+  if (b.total <= 0) return hipSuccess;
+  const int grid = cap > 0 && cap < b.total ? cap : b.total;
+  hipLaunchKernelGGL((element_kernel<T>), dim3(grid), dim3(kElementBlock), 0, s, b, src, with_hessian);
   return hipGetLastError();
 }
 
-template hipError_t launch_element<double>(const ElementArgs&, hipStream_t);
-template hipError_t launch_element<float>(const ElementArgs&, hipStream_t);
+template hipError_t launch_element<double>(const Batch&, int, int, int, hipStream_t);
+template hipError_t launch_element<float>(const Batch&, int, int, int, hipStream_t);
 
 }  // namespace hobak

@@ -1,6 +1,7 @@
-// Kernel argument blocks and host-side launchers shared by the translation units of
-// libhobak_hip.so.  Device layouts (all resident in HBM):
+// Device records, segmented launch descriptors and launchers shared by the translation units of
+// libhobak_hip.so.
 //
+// Device layouts (all resident in HBM, one set per simulation instance):
 //   vectors        DOF-ordered AoS: entry 3*v + c (the reference's flat VECTOR order)
 //   tets           int4 per tet
 //   DmInv          9 planes of nt doubles (plane 3*r + c holds DmInv(r, c))       -> coalesced
@@ -21,8 +22,13 @@
 //                  Columns of a row are ascending in k.
 //   preconditioner per row: 9 contiguous values (block Jacobi inverse, row-major) or per DOF
 //                  (Jacobi 1/a_ii)
-//   scatter map    per slot, the ascending-tet list of (tet, staged block, transpose) codes that
-//                  sum into it: code = tet * 32 + transpose * 16 + block
+//
+// Batching.  Every kernel is *segmented*: one launch serves any set of instances.  Each instance
+// owns a read-only InstDev record in device memory; a Batch names the records of the launch and
+// a prefix table of virtual workgroups (boff[i] = first virtual workgroup of instance i).  A
+// workgroup finds its instance by binary search, so no workgroup spans two instances, and
+// workgroups stride over the virtual workgroups when the launch grid is capped (persistent
+// form).  A single instance is a batch of one.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -30,14 +36,22 @@
 
 namespace hobak {
 
-constexpr int kDefaultSliceRows = 64;
+constexpr int kDefaultSliceRows = 32;
 constexpr int kBlock = 256;
-constexpr int kStageBlocks = 16;
+constexpr int kElementBlock = 128;
+constexpr int kMinvStride = 9;   // block-Jacobi inverse per row; Jacobi uses 1/a_ii per DOF
+constexpr int kUpdateCap = 512;  // per-instance workgroup cap of the PCG update kernel
+constexpr int kMaxSweeps = 32;   // FP64 refinement sweeps per linear solve
+
+// Virtual-workgroup kinds: each instance contributes nb[kind] workgroups to a launch of a kernel
+// of that kind.
+enum Kind { K_ELEM = 0, K_SLOT, K_ROW, K_DOF, K_SPMV, K_UPD, K_ONE, K_COUNT };
 
 // Scalars of one PCG solve, resident on the device so the loop runs without host round trips.
-// Two copies ping-pong: iteration `it` (counted by the host that queues the launches) reads
-// set it & 1 and its last kernel writes set (it + 1) & 1 from one workgroup only, so no kernel
-// ever reads a scalar that another of its own workgroups writes.
+// Two copies ping-pong: an iteration of parity p reads set p and its SpMV kernel writes set
+// p ^ 1 from one workgroup only, so no kernel reads a scalar that another of its own workgroups
+// writes.  `first` marks the set written by the initialisation, so the iteration kernels need
+// no iteration index and can be captured once and replayed.
 struct CgScalars {
   double rhsNorm2;
   double threshold;
@@ -48,48 +62,136 @@ struct CgScalars {
   double maxR;        // |S b|^2, the BDF_1 Newton test value
   int iters;
   int maxIters;
-  int done;           // 1 once converged / exhausted / failed
+  int done;           // 1 once converged / exhausted / failed, or when no solve is active
   int status;         // 0 ok, 1 not converged, 2 non-finite
-  int pad[4];
+  int first;          // 1 until the first SpMV of the solve has run
+  int pad[3];
 };
 
-// Per-workgroup partial sums written by one kernel and reduced, in a fixed order, by every
-// workgroup of the next kernel (no atomics, no in-kernel fences).
-struct Partials {
-  double* a;  // [grid] first value per producing workgroup
-  double* b;  // [grid] second value
-  int n;      // producing grid size
+// FP64 iterative refinement around the reduced-precision PCG (the precision form).
+struct RefineState {
+  double b2;        // |S b|^2
+  double res2;      // |rhs - A x64|^2 of the current iterate
+  double tau;       // relative tolerance of the next inner solve
+  int done;         // res2 < tol^2 b2 (or b2 == 0), sweep limit, or a non-finite value
+  int status;       // 0 ok, 1 sweep limit, 2 non-finite
+  int sweeps;
+  int inner_iters;  // PCG iterations summed over the sweeps
 };
 
-template <typename T>
-struct SellView {
-  int nrows;
-  int nslots;
-  const int* slice_off;
-  const int* col;
-  T* val;  // 9 planes of nslots
+enum Phase { PHASE_IDLE = 0, PHASE_SOLVING = 1, PHASE_FINISHED = 2 };
+
+// Mutable per-instance control block (written by kernels with ordinary vector stores).
+struct Ctl {
+  CgScalars sc[2];
+  RefineState rs;
+  int phase;
+  int pad[3];
 };
 
-struct ElementArgs {
-  int nt;
-  int material;        // 0 SNH, 1 ARAP
-  double mu, lam, alpha;  // SNH: lam = lambda + mu, alpha = 1 + mu / lam
+// What the host reads back after a group of PCG iterations, one per instance of a batch.
+struct InstStatus {
+  int phase;
+  int iters;        // iterations of the current (or last) inner solve
+  int inner_total;  // refinement: iterations summed over finished sweeps
+  int sweeps;
+  int cg_status;
+  int ref_status;
+  int ref_done;
+  int pad;
+  double err;       // sqrt(res^2 / b^2) of the finished solve
+  double maxR;      // |S b|^2
+};
+
+// Read-only per-instance record.  Pointers typed void* hold the solve's real type T.
+struct InstDev {
+  int nv, nt, nslots, shift;
+  int nb[K_COUNT];
+  int material, precond, bdf, ground_on, refine, max_iters;
+  double mu, lam, alpha;           // SNH: lam = lambda + mu, alpha = 1 + mu / lam
+  double invDt, invDt2, ra, rb;
+  double g[3];
+  double ground_y, ground_k;
+  double tol, tau0;                // outer PCG / refinement tolerance; first inner tolerance
   const int4* tets;
-  const double* rest;
-  const double* u;
-  const double* dminv;  // 9 planes of nt
-  const double* vol;
-  void* stage_h;        // 16 * 9 * nt values, T, in assembly order
-  void* stage_f;        // 4 * 3 * nt values, T, in force-gather order
-  const int* hdest;     // 16 per tet
-  const int* fdest;     // 4 per tet
-  int with_hessian;
+  const double *rest, *dminv, *vol, *one_ring, *mass, *pin_target;
+  double *u, *v, *u_old, *v_old, *tmp64, *rhs64, *x64, *res64;
+  const unsigned char* pinned;
+  const int *slice_off, *row_len, *col, *slot_row, *cptr, *hdest, *vptr, *fdest;
+  void *stage_h, *stage_f, *A, *D, *E, *C, *kneg, *minv, *r, *x, *z, *t, *p0, *p1;
+  Ctl* ctl;
+  // per-workgroup partial sums, sized by the producing kernel's nb[kind]
+  double *part_row_a, *part_row_b;  // K_ROW: right-hand side / inner-solve seed
+  double *part_pt;                  // K_SPMV: p.t
+  double *part_rr, *part_rz;        // K_UPD: r.r, r.z
+  double *part_res;                 // K_SPMV: refinement residual
 };
 
+// One segmented launch: the instances and the virtual-workgroup prefix table of the launch's kind.
+struct Batch {
+  const InstDev* const* inst;  // device, n entries
+  const int* boff;             // device, n + 1 entries
+  const InstDev* one;          // the record itself when n == 1 (saves a dependent load)
+  int n;
+  int total;                   // boff[n]
+};
 
-// Launchers (defined in the .hip files).  T is double or float.
+// The instance of virtual workgroup vb, its workgroup index inside the instance (lb) and the
+// instance's workgroup count of this kind (lgrid).  vb is uniform, so are the loads.
+__device__ __forceinline__ const InstDev* seg_find(const Batch& b, int vb, int& lb, int& lgrid) {
+  if (b.n == 1) {
+    lb = vb;
+    lgrid = b.total;
+    return b.one;
+  }
+  int lo = 0, hi = b.n;  // boff[lo] <= vb < boff[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (b.boff[mid] <= vb)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  const int b0 = b.boff[lo];
+  lb = vb - b0;
+  lgrid = b.boff[lo + 1] - b0;
+  return b.inst[lo];
+}
+
+// Workgroups per kind for an instance (host side; the same formulas size the partial arrays).
+int blocks_of(int kind, int nv, int nt, int nslots, int shift);
+
+// Launchers.  `cap` limits the launch grid (0: one workgroup per virtual workgroup).
+enum ElemSrc { ELEM_U = 0, ELEM_TMP = 1 };
+enum AsmTarget { ASM_REST = 0, ASM_PLAIN_A = 1, ASM_SYSTEM = 2 };
+
 template <typename T>
-hipError_t launch_element(const ElementArgs& a, hipStream_t s);
-
+hipError_t launch_element(const Batch& b, int src, int with_hessian, int cap, hipStream_t s);
+hipError_t launch_begin_step(const Batch& b, int cap, hipStream_t s);
+hipError_t launch_end_step(const Batch& b, int cap, hipStream_t s);
+template <typename T>
+hipError_t launch_apply_dx(const Batch& b, int use64, int cap, hipStream_t s);
+template <typename T>
+hipError_t launch_assemble(const Batch& b, int target, int cap, hipStream_t s);
+template <typename T>
+hipError_t launch_constant_parts(const Batch& b, int cap, hipStream_t s);
+template <typename T>
+hipError_t launch_rhs(const Batch& b, int init_cg, int cap, hipStream_t s);
+hipError_t launch_cg_init(const Batch& b, int refine, int gate, hipStream_t s);
+template <typename T>
+hipError_t launch_cg_spmv(const Batch& b, int shift, int parity, int cap, hipStream_t s,
+                          hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
+template <typename T>
+hipError_t launch_cg_update(const Batch& b, int parity, int cap, hipStream_t s);
+hipError_t launch_refine_begin(const Batch& b, int cap, hipStream_t s);
+template <typename T>
+hipError_t launch_refine_init(const Batch& b, int gate, int cap, hipStream_t s);
+template <typename T>
+hipError_t launch_refine_residual(const Batch& spmv, const Batch& dof, int shift, int cap, hipStream_t s);
+hipError_t launch_finish_check(const Batch& b, int refine, hipStream_t s);
+hipError_t launch_status(const Batch& b, InstStatus* out, hipStream_t s);
+template <typename T>
+hipError_t launch_force_gather(const Batch& b, int cap, hipStream_t s);
+hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
 
 }  // namespace hobak

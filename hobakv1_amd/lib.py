@@ -70,6 +70,13 @@ class hobak_step_stats(ctypes.Structure):
         }
 
 
+class hobak_completion(ctypes.Structure):
+    _fields_ = [
+        ("inst", ctypes.c_void_p), ("tag", ctypes.c_uint64), ("stats", hobak_step_stats),
+        ("submit_s", ctypes.c_double), ("start_s", ctypes.c_double), ("end_s", ctypes.c_double),
+    ]
+
+
 _vp = ctypes.c_void_p
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int32)
@@ -100,10 +107,20 @@ SIGNATURES = {
     "hobak_kernel_stats": (ctypes.c_int, [_vp, _dp, ctypes.POINTER(ctypes.c_int64), _dp]),
     "hobak_copy_probe": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.c_int32, _dp]),
     "hobak_ctx_set_tuning": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32]),
+    "hobak_sched_create": (ctypes.c_int, [_vp, ctypes.POINTER(_vp)]),
+    "hobak_sched_destroy": (None, [_vp]),
+    "hobak_sched_submit": (ctypes.c_int, [_vp, _vp, ctypes.c_uint64]),
+    "hobak_sched_detach": (ctypes.c_int, [_vp, _vp]),
+    "hobak_sched_tick": (ctypes.c_int, [_vp, ctypes.POINTER(hobak_completion), ctypes.c_int32,
+                                        _ip, _ip]),
+    "hobak_sched_clock": (ctypes.c_double, [_vp]),
 }
 
 TUNE_SLICE_ROWS = 0
 TUNE_POLL_CHUNK = 2
+TUNE_MAX_CHUNKS = 3
+TUNE_GRAPHS = 4
+TUNE_GRID_CAP = 5
 
 _lib = None
 
@@ -171,7 +188,9 @@ class Context:
         with open(path) as f:
             entry = json.load(f).get(precision)
         if entry:
-            self.set_tuning(slice_rows=entry.get("slice_rows"), poll_chunk=entry.get("poll_chunk"))
+            self.set_tuning(slice_rows=entry.get("slice_rows"), poll_chunk=entry.get("poll_chunk"),
+                            max_chunks=entry.get("max_chunks"), graphs=entry.get("graphs"),
+                            grid_cap=entry.get("grid_cap"))
             self.refine_inner_tol = entry.get("refine_inner_tol")
 
     def error(self) -> str:
@@ -182,9 +201,12 @@ class Context:
             raise HobakError(rc, f"{what}: {self.error()}")
         return rc
 
-    def set_tuning(self, slice_rows=None, poll_chunk=None):
-        """Kernel tuning for instances created afterwards (see hobak_ctx_set_tuning)."""
-        for key, val in ((TUNE_SLICE_ROWS, slice_rows), (TUNE_POLL_CHUNK, poll_chunk)):
+    def set_tuning(self, slice_rows=None, poll_chunk=None, max_chunks=None, graphs=None,
+                   grid_cap=None):
+        """Kernel and engine tuning (see hobak_ctx_set_tuning)."""
+        for key, val in ((TUNE_SLICE_ROWS, slice_rows), (TUNE_POLL_CHUNK, poll_chunk),
+                         (TUNE_MAX_CHUNKS, max_chunks), (TUNE_GRAPHS, graphs),
+                         (TUNE_GRID_CAP, grid_cap)):
             if val is not None:
                 self.check(self.lib.hobak_ctx_set_tuning(self.h, key, int(val)), "set_tuning")
 
@@ -337,6 +359,49 @@ def step_batch(ctx: Context, insts, allow_unconverged=False):
     if rc != HOBAK_OK and not (allow_unconverged and rc == -4):
         raise HobakError(rc, ctx.error())
     return [s.as_dict() for s in stats]
+
+
+class Scheduler:
+    """Continuous-batching scheduler (hobak_sched_*): submit step requests for instances, call
+    tick() repeatedly; each tick returns the completed requests."""
+
+    def __init__(self, ctx: Context, max_out: int = 4096):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        h = _vp()
+        ctx.check(self.lib.hobak_sched_create(ctx.h, ctypes.byref(h)), "sched_create")
+        self.h = h
+        self._buf = (hobak_completion * max_out)()
+        self._by_handle = {}
+
+    def submit(self, inst: "Instance", tag: int):
+        self._by_handle[inst.h.value] = inst
+        self.ctx.check(self.lib.hobak_sched_submit(self.h, inst.h, int(tag)), "sched_submit")
+
+    def detach(self, inst: "Instance"):
+        self.ctx.check(self.lib.hobak_sched_detach(self.h, inst.h), "sched_detach")
+        self._by_handle.pop(inst.h.value, None)
+
+    def clock(self) -> float:
+        return self.lib.hobak_sched_clock(self.h)
+
+    def tick(self):
+        """Runs one tick; returns (completions, requests still in flight)."""
+        n = ctypes.c_int32(0)
+        inflight = ctypes.c_int32(0)
+        self.ctx.check(self.lib.hobak_sched_tick(self.h, self._buf, len(self._buf), ctypes.byref(n),
+                                                 ctypes.byref(inflight)), "sched_tick")
+        out = []
+        for k in range(n.value):
+            c = self._buf[k]
+            out.append({"inst": self._by_handle.get(c.inst), "tag": c.tag, "stats": c.stats.as_dict(),
+                        "submit_s": c.submit_s, "start_s": c.start_s, "end_s": c.end_s})
+        return out, inflight.value
+
+    def close(self):
+        if self.h:
+            self.lib.hobak_sched_destroy(self.h)
+            self.h = None
 
 
 # ---------------------------------------------------------------------------------------------

@@ -41,7 +41,7 @@ extern "C" {
 #define HOBAK_E_HIP (-2)         /* a HIP runtime call failed */
 #define HOBAK_E_NOMEM (-3)       /* device allocation failed or the HBM budget would be exceeded */
 #define HOBAK_E_NOT_CONVERGED (-4) /* PCG hit its iteration limit (state is still advanced) */
-#define HOBAK_E_NONFINITE (-5)   /* NaN or Inf appeared in the solve (state is not advanced) */
+#define HOBAK_E_NONFINITE (-5)   /* NaN or Inf appeared in the solve */
 #define HOBAK_E_NODEVICE (-6)    /* no usable gfx950 device */
 
 #define HOBAK_MAT_SNH 0          /* Stable Neo-Hookean, src/Hyperelastic/Volume/SNH.cpp */
@@ -57,6 +57,7 @@ extern "C" {
 
 typedef struct hobak_ctx hobak_ctx;
 typedef struct hobak_inst hobak_inst;
+typedef struct hobak_sched hobak_sched;
 
 /* VOLUME::SNH(mu, lambda) / VOLUME::ARAP(mu, lambda): unreparameterised Lame constants. */
 typedef struct {
@@ -125,7 +126,11 @@ int hobak_inst_get_state(hobak_inst* inst, double* u, double* v);
  * gravity, then BACKWARD_EULER_POSITION::solve (or the BDF_1 loop when newton_iters > 1). */
 int hobak_step(hobak_inst* inst, hobak_step_stats* stats);
 
-/* Advance n independent instances by one step each, batched on the device.  stats: n entries. */
+/* Advance n independent instances by one step each, batched on the device: every kernel is one
+ * segmented launch over all of them, and the block-diagonal PCG masks out converged instances.
+ * Instances of one precision form run together (FP64 ones first, then FP32 ones); all share the
+ * context's slice height.  stats: n entries.  hobak_step is the batch of one.  There is no
+ * reference counterpart: the reference steps one mesh per solve() call. */
 int hobak_step_batch(hobak_ctx* ctx, hobak_inst* const* insts, int32_t n, hobak_step_stats* stats);
 
 /* Parity hooks.  BSR pattern of the stiffness (block rows by vertex, columns ascending). */
@@ -151,13 +156,51 @@ int hobak_last_step_timing(hobak_inst* inst, double* out_ms);
 int hobak_set_kernel_sampling(hobak_inst* inst, int32_t every);
 int hobak_kernel_stats(hobak_inst* inst, double* spmv_avg_ms, int64_t* spmv_count,
                        double* spmv_bytes);
-/* Kernel tuning (no counterpart in the reference); applies to instances created afterwards.
+/* Kernel and engine tuning (no counterpart in the reference).
  *   HOBAK_TUNE_SLICE_ROWS  rows per block-matrix slice: 8, 16, 32 or 64 (64 / rows lanes per row
- *                          in the SpMV)
- *   HOBAK_TUNE_POLL_CHUNK  PCG iterations queued between host polls of the convergence flag */
+ *                          in the SpMV); fixed once the context holds instances
+ *   HOBAK_TUNE_POLL_CHUNK  PCG iterations per queued group (rounded up to even); groups are
+ *                          captured once per member set as a HIP graph
+ *   HOBAK_TUNE_MAX_CHUNKS  most groups queued between two host polls
+ *   HOBAK_TUNE_GRAPHS      1: replay captured graphs; 0: launch every kernel directly
+ *   HOBAK_TUNE_GRID_CAP    most workgroups per launch (0: one per virtual workgroup); capped
+ *                          launches stride over their virtual workgroups (persistent form) */
 #define HOBAK_TUNE_SLICE_ROWS 0
 #define HOBAK_TUNE_POLL_CHUNK 2
+#define HOBAK_TUNE_MAX_CHUNKS 3
+#define HOBAK_TUNE_GRAPHS 4
+#define HOBAK_TUNE_GRID_CAP 5
 int hobak_ctx_set_tuning(hobak_ctx* ctx, int32_t key, int32_t value);
+
+/* Continuous-batching scheduler (no counterpart in the reference, whose callers step one scene
+ * at a time through SIMULATION_SCENE::stepSimulation, SIMULATION_SCENE.h:120-126).
+ * A request advances one instance by one step.  Submitting attaches the instance to the
+ * scheduler (its state stays resident in HBM); every tick finishes the solves that ended, starts
+ * the requests that have arrived since (their prologue kernels run as one segmented launch), runs
+ * a group of PCG iterations over every attached instance with one captured graph (instances not
+ * solving return at once) and polls.  Requests that arrive while others are on the GPU thus join
+ * the running batch at the next tick.  Requests of one instance run in submission order. */
+typedef struct {
+  hobak_inst* inst;
+  uint64_t tag;
+  hobak_step_stats stats;   /* stats.seconds = end_s - submit_s */
+  double submit_s;          /* scheduler clock: seconds since hobak_sched_create */
+  double start_s;           /* the request's first kernels were queued */
+  double end_s;             /* the tick that completed it returned */
+} hobak_completion;
+
+int hobak_sched_create(hobak_ctx* ctx, hobak_sched** out);
+/* Drops queued and running requests and detaches every instance. */
+void hobak_sched_destroy(hobak_sched* sched);
+int hobak_sched_submit(hobak_sched* sched, hobak_inst* inst, uint64_t tag);
+/* Detaches an idle instance (hobak_inst_destroy detaches by itself). */
+int hobak_sched_detach(hobak_sched* sched, hobak_inst* inst);
+/* Runs one tick (when any request is queued or running) and returns up to max_out completions;
+ * *in_flight counts the requests not yet returned. */
+int hobak_sched_tick(hobak_sched* sched, hobak_completion* out, int32_t max_out, int32_t* n_out,
+                     int32_t* in_flight);
+/* The scheduler clock (seconds since hobak_sched_create). */
+double hobak_sched_clock(hobak_sched* sched);
 
 /* Streaming copy of `bytes` device bytes (a calibration kernel for hardware byte counters);
  * returns the kernel time in ms. */

@@ -1,0 +1,163 @@
+"""Batched and continuously batched stepping (hobak_step_batch, hobak_sched_*) against stepping
+each instance alone, and against the oracle.
+
+Every kernel of a batch is one segmented launch over all instances; each instance's arithmetic
+(its reductions' partial counts and order included) does not depend on what else is in the
+batch.  So the bar for "same result alone and in the mix" is bitwise equality of positions,
+velocities and PCG iteration counts, stronger than the 1e-6 position tolerance stated for the
+serving form.  Against the oracle, batched FP64 instances meet the plain form's tolerances
+(iterations +-1, positions 1e-9 relative).
+"""
+import numpy as np
+import pytest
+
+import oracle
+from helpers import rel
+from hobakv1_amd import lib, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _mixed_scenes(k=5, seed=21):
+    """Small instances of different sizes, materials and stiffness (C4-style, scaled down)."""
+    return scenes.batch_instances(count=k, seed=seed, lo=3, hi=9)
+
+
+def _run_alone(ctx, scs, steps, precision, **kw):
+    out = []
+    for sc in scs:
+        inst = lib.Instance(ctx, sc, precision=precision, **kw)
+        its = [inst.step()["pcg_iters"] for _ in range(steps)]
+        out.append((inst.state(), its))
+        inst.close()
+    return out
+
+
+def _run_batched(ctx, scs, steps, precision, **kw):
+    insts = [lib.Instance(ctx, sc, precision=precision, **kw) for sc in scs]
+    its = [[] for _ in scs]
+    for _ in range(steps):
+        st = lib.step_batch(ctx, insts)
+        for i, s in enumerate(st):
+            its[i].append(s["pcg_iters"])
+    out = [(inst.state(), its[i]) for i, inst in enumerate(insts)]
+    for inst in insts:
+        inst.close()
+    return out
+
+
+@pytest.mark.parametrize("precision", [lib.PRECISION_FP64, lib.PRECISION_FP32_REFINE])
+def test_batch_bitwise_equals_alone(gpu_ctx, precision):
+    scs = _mixed_scenes()
+    _say(f"batch of {len(scs)} (nt {[s.nt for s in scs]}) precision {precision}")
+    alone = _run_alone(gpu_ctx, scs, 4, precision)
+    batched = _run_batched(gpu_ctx, scs, 4, precision)
+    for i, ((ua, va), ia) in enumerate(alone):
+        (ub, vb), ib = batched[i]
+        assert ia == ib, (i, ia, ib)
+        assert np.array_equal(ua, ub) and np.array_equal(va, vb), i
+
+
+def test_batch_graphs_match_direct_launches():
+    scs = _mixed_scenes(k=4, seed=22)
+    res = []
+    for graphs in (1, 0):
+        ctx = lib.Context(0)
+        ctx.set_tuning(graphs=graphs)
+        res.append(_run_batched(ctx, scs, 3, lib.PRECISION_FP64))
+        ctx.close()
+    for (sa, ia), (sb, ib) in zip(res[0], res[1]):
+        assert ia == ib
+        assert np.array_equal(sa[0], sb[0]) and np.array_equal(sa[1], sb[1])
+
+
+def test_batch_persistent_grid_matches():
+    """A capped launch grid (workgroups striding over virtual workgroups) changes nothing."""
+    scs = _mixed_scenes(k=4, seed=23)
+    res = []
+    for cap in (0, 7):
+        ctx = lib.Context(0)
+        ctx.set_tuning(grid_cap=cap)
+        res.append(_run_batched(ctx, scs, 2, lib.PRECISION_FP64))
+        ctx.close()
+    for (sa, ia), (sb, ib) in zip(res[0], res[1]):
+        assert ia == ib
+        assert np.array_equal(sa[0], sb[0]) and np.array_equal(sa[1], sb[1])
+
+
+def test_batch_fp64_matches_oracle(gpu_ctx):
+    scs = _mixed_scenes(k=4, seed=24)
+    insts = [lib.Instance(gpu_ctx, sc) for sc in scs]
+    orcs = [oracle.Oracle(sc, precond=scenes.PRECOND_BLOCK_JACOBI) for sc in scs]
+    for k in range(5):
+        st = lib.step_batch(gpu_ctx, insts)
+        for i, o in enumerate(orcs):
+            it, _, _ = o.step()
+            assert abs(st[i]["pcg_iters"][0] - it) <= 1, (k, i, st[i]["pcg_iters"], it)
+    for i, (inst, o) in enumerate(zip(insts, orcs)):
+        u, _ = inst.state()
+        uo, _ = o.state()
+        X = scs[i].rest.ravel()
+        assert rel(X + u, X + uo) <= 1e-9, i
+        inst.close()
+
+
+def test_batch_bdf_newton_equals_alone(gpu_ctx):
+    scs = _mixed_scenes(k=3, seed=25)
+    alone = _run_alone(gpu_ctx, scs, 2, lib.PRECISION_FP64, newton_iters=3)
+    batched = _run_batched(gpu_ctx, scs, 2, lib.PRECISION_FP64, newton_iters=3)
+    for i, ((ua, va), ia) in enumerate(alone):
+        (ub, vb), ib = batched[i]
+        _say(f"  bdf inst {i}: newton iters alone {ia} batched {ib}")
+        assert ia == ib
+        assert np.array_equal(ua, ub) and np.array_equal(va, vb)
+
+
+def test_scheduler_continuous_batching_equals_alone(gpu_ctx):
+    """Requests arrive while others run; each result equals the instance stepped alone."""
+    scs = [scenes.serving_instance(k, seed=7) for k in range(4)]
+    scs = [s for s in scs if s.nt <= 60000][:3] or scs[:2]
+    steps = 3
+    alone = _run_alone(gpu_ctx, scs, steps, lib.PRECISION_FP32_REFINE)
+    insts = [lib.Instance(gpu_ctx, sc, precision=lib.PRECISION_FP32_REFINE) for sc in scs]
+    sch = lib.Scheduler(gpu_ctx)
+    remaining = [steps] * len(insts)
+    # staggered arrivals: instance i's first request arrives after i ticks
+    submitted_first = [False] * len(insts)
+    overlapped = False
+    running = set()
+    tick = 0
+    done_at = {}
+    while any(remaining) or running:
+        for i in range(len(insts)):
+            if not submitted_first[i] and tick >= i:
+                sch.submit(insts[i], i)
+                submitted_first[i] = True
+                running.add(i)
+        if len(running) > 1:
+            overlapped = True
+        comps, _ = sch.tick()
+        for c in comps:
+            i = int(c["tag"])
+            assert c["stats"]["status"] == 0
+            remaining[i] -= 1
+            running.discard(i)
+            if remaining[i] > 0:
+                sch.submit(insts[i], i)
+                running.add(i)
+            else:
+                done_at[i] = tick
+        tick += 1
+        assert tick < 100000
+    assert overlapped
+    for i, inst in enumerate(insts):
+        u, v = inst.state()
+        (ua, va), _ = alone[i]
+        assert np.array_equal(u, ua) and np.array_equal(v, va), i
+    sch.close()
+    for inst in insts:
+        inst.close()
