@@ -83,43 +83,66 @@ struct Pattern {
   std::vector<int> vptr, fdest;       // per vertex force ranges; per (tet, x) position
 };
 
+// Block sparsity (BSR rows by vertex, columns ascending), its SELL-H slot layout, and the
+// contribution lists of the assembly and force gathers, built from the vertex -> tet incidence
+// lists (ascending tet) with marker arrays: O(16 nt) with no searches.
 void build_pattern(int nv, int nt, const int32_t* tets, int H, Pattern& P) {
   P.nv = nv;
   P.nt = nt;
   P.H = H;
   P.shift = 0;
   while ((1 << P.shift) < H) P.shift++;
-  const int kSlice = H;
-  const int kLanesPerRow = 64 / H;
-  std::vector<std::vector<int>> nbr(nv);
-  for (int t = 0; t < nt; t++)
-    for (int a = 0; a < 4; a++)
-      for (int b = 0; b < 4; b++) nbr[tets[4 * t + a]].push_back(tets[4 * t + b]);
+  const int L = 64 / H;
+  // vertex -> incident tets, ascending (also the force-gather lists: fdest = position in them)
+  P.vptr.assign(nv + 1, 0);
+  for (size_t i = 0; i < 4 * (size_t)nt; i++) P.vptr[tets[i] + 1]++;
+  for (int i = 0; i < nv; i++) P.vptr[i + 1] += P.vptr[i];
+  std::vector<int> inc(P.vptr[nv]);
+  P.fdest.assign(4 * (size_t)nt, 0);
+  {
+    std::vector<int> fill(P.vptr.begin(), P.vptr.end() - 1);
+    for (int t = 0; t < nt; t++)
+      for (int a = 0; a < 4; a++) {
+        const int v = tets[4 * (size_t)t + a];
+        const int q = fill[v]++;
+        inc[q] = t;
+        P.fdest[4 * (size_t)t + a] = q;
+      }
+  }
+  // rows: distinct vertices of each vertex's incident tets, ascending
+  std::vector<int> mark(nv, -1), tmp;
+  tmp.reserve(128);
   P.row_ptr.assign(nv + 1, 0);
   P.col_idx.clear();
+  P.col_idx.reserve((size_t)nv * 16);
   for (int i = 0; i < nv; i++) {
-    auto& n = nbr[i];
-    std::sort(n.begin(), n.end());
-    n.erase(std::unique(n.begin(), n.end()), n.end());
-    P.row_ptr[i + 1] = P.row_ptr[i] + (int)n.size();
-    P.col_idx.insert(P.col_idx.end(), n.begin(), n.end());
-    std::vector<int>().swap(n);
+    tmp.clear();
+    for (int q = P.vptr[i]; q < P.vptr[i + 1]; q++) {
+      const int32_t* T = tets + 4 * (size_t)inc[q];
+      for (int b = 0; b < 4; b++)
+        if (mark[T[b]] != i) {
+          mark[T[b]] = i;
+          tmp.push_back(T[b]);
+        }
+    }
+    std::sort(tmp.begin(), tmp.end());
+    P.col_idx.insert(P.col_idx.end(), tmp.begin(), tmp.end());
+    P.row_ptr[i + 1] = (int)P.col_idx.size();
   }
   P.nnzb = P.row_ptr[nv];
-  // SELL-H, k-interleaved: slice width rounded up to a multiple of the lanes per row so that
-  // each slice's slots form whole 64-slot wave loads
-  P.nslices = (nv + kSlice - 1) / kSlice;
-  P.nrows_pad = P.nslices * kSlice;
+  // SELL-H slices, width rounded up to a multiple of the lanes per row
+  P.nslices = (nv + H - 1) / H;
+  P.nrows_pad = P.nslices * H;
   P.slice_off.assign(P.nslices + 1, 0);
   P.row_len.assign(nv, 0);
   for (int s = 0; s < P.nslices; s++) {
     int w = 0;
-    for (int r = s * kSlice; r < std::min(nv, (s + 1) * kSlice); r++) {
+    for (int r = s * H; r < std::min(nv, (s + 1) * H); r++) {
       P.row_len[r] = P.row_ptr[r + 1] - P.row_ptr[r];
       w = std::max(w, P.row_len[r]);
     }
-    w = (w + kLanesPerRow - 1) / kLanesPerRow * kLanesPerRow;
-    P.slice_off[s + 1] = P.slice_off[s] + w * kSlice;
+    w = (w + L - 1) / L * L;
+    P.slice_off[s + 1] = P.slice_off[s] + w * H;
   }
   P.nslots = P.slice_off[P.nslices];
   P.col.assign(P.nslots, 0);
@@ -127,11 +150,11 @@ void build_pattern(int nv, int nt, const int32_t* tets, int H, Pattern& P) {
   P.bsr_slot.assign(P.nnzb, 0);
   P.diag_slot.assign(nv, -1);
   for (int s = 0; s < P.nslices; s++) {
-    const int w = (P.slice_off[s + 1] - P.slice_off[s]) / kSlice;
-    for (int lane = 0; lane < kSlice; lane++) {
-      const int r = s * kSlice + lane;
+    const int w = (P.slice_off[s + 1] - P.slice_off[s]) / H;
+    for (int lane = 0; lane < H; lane++) {
+      const int r = s * H + lane;
       for (int k = 0; k < w; k++) {
-        const int slot = P.slice_off[s] + k * kSlice + lane;
+        const int slot = P.slice_off[s] + k * H + lane;
         if (r < nv && k < P.row_len[r]) {
           const int j = P.col_idx[P.row_ptr[r] + k];
           P.col[slot] = j;
@@ -144,42 +167,35 @@ void build_pattern(int nv, int nt, const int32_t* tets, int H, Pattern& P) {
       }
     }
   }
-  auto slot_of = [&](int i, int j) {
-    const int* b = P.col_idx.data() + P.row_ptr[i];
-    const int* e = P.col_idx.data() + P.row_ptr[i + 1];
-    const int k = (int)(std::lower_bound(b, e, j) - b);
-    return P.slice_off[i / kSlice] + k * kSlice + (i % kSlice);
-  };
+  // contributions: block (x, y) of tet t sums into slot (tets[t][x], tets[t][y]).  Walking row i's
+  // incident tets in ascending order visits every contribution of row i's slots in ascending tet
+  // order, which is the order of the reference's gather lists (TET_MESH_FASTER.cpp:213-236).
+  std::vector<int> pos(nv, 0);
   std::vector<int> cnt(P.nslots + 1, 0);
-  std::vector<int> slots(16 * (size_t)nt);
-  for (int t = 0; t < nt; t++)
-    for (int x = 0; x < 4; x++)
-      for (int y = 0; y < 4; y++) {
-        const int s = slot_of(tets[4 * t + x], tets[4 * t + y]);
-        slots[16 * (size_t)t + 4 * x + y] = s;
-        cnt[s + 1]++;
-      }
+  auto row_slots = [&](int i) {
+    for (int k = P.row_ptr[i]; k < P.row_ptr[i + 1]; k++) pos[P.col_idx[k]] = P.bsr_slot[k];
+  };
+  for (int i = 0; i < nv; i++) {
+    row_slots(i);
+    for (int q = P.vptr[i]; q < P.vptr[i + 1]; q++) {
+      const int32_t* T = tets + 4 * (size_t)inc[q];
+      for (int b = 0; b < 4; b++) cnt[pos[T[b]] + 1]++;
+    }
+  }
   P.cptr.assign(P.nslots + 1, 0);
   for (int s = 0; s < P.nslots; s++) P.cptr[s + 1] = P.cptr[s] + cnt[s + 1];
-  // Position of every element block in its slot's contribution list: looping tets in ascending
-  // order fills each list in ascending tet order (TET_MESH_FASTER.cpp:213-236 pushes the same way)
   P.hdest.assign(16 * (size_t)nt, 0);
   std::vector<int> fill(P.cptr.begin(), P.cptr.end() - 1);
-  for (int t = 0; t < nt; t++)
-    for (int x = 0; x < 4; x++)
-      for (int y = 0; y < 4; y++) {
-        const int s = slots[16 * (size_t)t + 4 * x + y];
-        P.hdest[16 * (size_t)t + 4 * x + y] = fill[s]++;
-      }
-  std::vector<int> vc(nv + 1, 0);
-  for (int t = 0; t < nt; t++)
-    for (int a = 0; a < 4; a++) vc[tets[4 * t + a] + 1]++;
-  P.vptr.assign(nv + 1, 0);
-  for (int i = 0; i < nv; i++) P.vptr[i + 1] = P.vptr[i] + vc[i + 1];
-  P.fdest.assign(4 * (size_t)nt, 0);
-  std::vector<int> vf(P.vptr.begin(), P.vptr.end() - 1);
-  for (int t = 0; t < nt; t++)
-    for (int a = 0; a < 4; a++) P.fdest[4 * (size_t)t + a] = vf[tets[4 * t + a]]++;
+  for (int i = 0; i < nv; i++) {
+    row_slots(i);
+    for (int q = P.vptr[i]; q < P.vptr[i + 1]; q++) {
+      const int t = inc[q];
+      const int32_t* T = tets + 4 * (size_t)t;
+      const int x = P.fdest[4 * (size_t)t + 0] == q ? 0 : P.fdest[4 * (size_t)t + 1] == q ? 1
+                  : P.fdest[4 * (size_t)t + 2] == q ? 2 : 3;
+      for (int y = 0; y < 4; y++) P.hdest[16 * (size_t)t + 4 * x + y] = fill[pos[T[y]]]++;
+    }
+  }
 }
 
 
@@ -1172,6 +1188,7 @@ int hobak_inst_create(hobak_ctx* ctx, const double* rest_xyz, int32_t nv, const 
   ALLOCT(h.p1, 3 * np);
   ALLOCT(h.t, 3 * np);
   ALLOC(h.ctl, 1);
+  if (h.refine) ALLOC(h.dg, 3 * np);
   ALLOC(h.part_row_a, (size_t)h.nb[K_ROW]);
   ALLOC(h.part_row_b, (size_t)h.nb[K_ROW]);
   ALLOC(h.part_pt, (size_t)h.nb[K_SPMV]);

@@ -356,4 +356,43 @@ HD void arap_pk1(T mu, const T U[3][3], const T Sg[3], const T V[3][3], T P[3][3
     }
 }
 
+// Reduced-precision PK1s with the near-rest cancellation taken out in FP64.  At F = I both forms
+// above are differences of O(mu) terms that cancel to O(mu * strain), which in FP32 leaves a
+// relative force error of ~1e-7 / strain.  Rewritten with strain-like factors that vanish at rest,
+// formed in FP64 from the FP64 F and SVD, the T arithmetic that follows carries only T's relative
+// error of the force itself:
+//   SNH   mu F + lam (J - alpha) cof F = mu (F - cof F) + lam (J - 1) cof F   (alpha = 1 + mu / lam)
+//   ARAP  R 2 mu (S - I) = 2 mu U (Sigma - I) V^T
+template <typename T>
+HD void snh_pk1_split(T mu, T lam, const double F[3][3], T P[3][3]) {
+  double G[3][3];
+  G[0][0] = F[1][1] * F[2][2] - F[2][1] * F[1][2];
+  G[1][0] = F[2][1] * F[0][2] - F[0][1] * F[2][2];
+  G[2][0] = F[0][1] * F[1][2] - F[1][1] * F[0][2];
+  G[0][1] = F[1][2] * F[2][0] - F[2][2] * F[1][0];
+  G[1][1] = F[2][2] * F[0][0] - F[0][2] * F[2][0];
+  G[2][1] = F[0][2] * F[1][0] - F[1][2] * F[0][0];
+  G[0][2] = F[1][0] * F[2][1] - F[2][0] * F[1][1];
+  G[1][2] = F[2][0] * F[0][1] - F[0][0] * F[2][1];
+  G[2][2] = F[0][0] * F[1][1] - F[1][0] * F[0][1];
+  const T j1 = (T)(det3t(F) - 1.0);
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) P[i][j] = mu * (T)(F[i][j] - G[i][j]) + lam * j1 * (T)G[i][j];
+}
+
+template <typename T>
+HD void arap_pk1_split(T mu, const T U[3][3], const double Sd[3], const T V[3][3], T P[3][3]) {
+  T d[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) d[k] = (T)(Sd[k] - 1.0);
+  const T tm = T(2) * mu;
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+      P[i][j] = tm * ((U[i][0] * d[0] * V[j][0] + U[i][1] * d[1] * V[j][1]) + U[i][2] * d[2] * V[j][2]);
+}
+
 }  // namespace hobak

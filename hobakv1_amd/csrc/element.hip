@@ -88,7 +88,13 @@ __device__ __forceinline__ void element_tet(const ElementIn& a, int t) {
 
   // PK1 and the element force f_v = -V_e P Wd_v^T
   T P[3][3];
-  if (MAT == 0) {
+  if constexpr (sizeof(T) < sizeof(double)) {
+    // reduced precision: strain-like factors in FP64 (see snh_pk1_split)
+    if (MAT == 0)
+      snh_pk1_split<T>((T)a.mu, (T)a.lam, F, P);
+    else
+      arap_pk1_split<T>((T)a.mu, U, Sd, V, P);
+  } else if (MAT == 0) {
     T Ft[3][3];
 #pragma unroll
     for (int i = 0; i < 3; i++)

@@ -23,6 +23,17 @@
 //   preconditioner per row: 9 contiguous values (block Jacobi inverse, row-major) or per DOF
 //                  (Jacobi 1/a_ii)
 //
+// Difference form (the reduced-precision form's operators).  Every element stiffness has block
+// rows that sum to zero (a translation does not deform), so with m the lumped mass,
+//   (A x)_i = dg_i x_i + sum_j A_ij (x_j - x_i),   dg_i = m_i (1/dt^2 - alpha/dt) (+ ground),
+// exactly, where the sum runs over the stored blocks (the diagonal one multiplies zero).  Rounding
+// A_ij to FP32 then perturbs only the response to deformation: a rigid translation is annihilated
+// whatever the rounding, while in the standard form FP32 entries give it a spurious stiffness
+// force of relative size ~1e-7 * stiffness / mass that grows over a trajectory.  Rows of pinned
+// vertices store zero off-diagonal blocks and dg = 1; couplings to pinned columns are kept (every
+// solver vector is zero at pinned vertices).  E v and C v_b in the right-hand side use the same
+// form.  The plain FP64 form keeps the standard form, which is the reference's arithmetic.
+//
 // Batching.  Every kernel is *segmented*: one launch serves any set of instances.  Each instance
 // owns a read-only InstDev record in device memory; a Batch names the records of the launch and
 // a prefix table of virtual workgroups (boff[i] = first virtual workgroup of instance i).  A
@@ -120,6 +131,7 @@ struct InstDev {
   const int *slice_off, *row_len, *col, *slot_row, *cptr, *hdest, *vptr, *fdest;
   void *stage_h, *stage_f, *A, *D, *E, *C, *kneg, *minv, *r, *x, *z, *t, *p0, *p1;
   Ctl* ctl;
+  double* dg;  // reduced-precision form: per-DOF diagonal term of the difference-form operator
   // per-workgroup partial sums, sized by the producing kernel's nb[kind]
   double *part_row_a, *part_row_b;  // K_ROW: right-hand side / inner-solve seed
   double *part_pt;                  // K_SPMV: p.t

@@ -204,10 +204,14 @@ __global__ __launch_bounds__(kBlock) void assemble_kernel(Batch b, int target) {
       for (int e = 0; e < 9; e++) out[e * ns + slot] = acc[e];
       continue;
     }
-    const bool pin = I->pinned[r] || I->pinned[c];
+    constexpr bool kDiff = sizeof(T) < sizeof(double);
+    // standard form: pinned rows and columns are zeroed; difference form: only pinned rows
+    const bool pin = kDiff ? (bool)I->pinned[r] : (I->pinned[r] || I->pinned[c]);
+    bool ground = false;
     if (I->ground_on && c == r && !pin) {
       const double y = I->rest[3 * (size_t)r + 1] + I->u[3 * (size_t)r + 1];
-      if (y < I->ground_y) acc[4] += (T)I->ground_k;
+      ground = y < I->ground_y;
+      if (ground) acc[4] += (T)I->ground_k;
     }
     const T* D = (const T*)I->D;
     T* out = (T*)I->A;
@@ -217,7 +221,16 @@ __global__ __launch_bounds__(kBlock) void assemble_kernel(Batch b, int target) {
       val[e] = pin ? ((c == r && (e % 4) == 0) ? T(1) : T(0)) : D[e * ns + slot] + acc[e];
       out[e * ns + slot] = val[e];
     }
-    if (c == r) write_precond<T>(val, (T*)I->minv, r, I->precond);
+    if (c == r) {
+      write_precond<T>(val, (T*)I->minv, r, I->precond);
+      if (kDiff) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+          const double mk = I->mass[3 * (size_t)r + k] * (I->invDt2 - I->ra * I->invDt);
+          I->dg[3 * (size_t)r + k] = pin ? 1.0 : mk + ((k == 1 && ground) ? I->ground_k : 0.0);
+        }
+      }
+    }
   }
 }
 
@@ -300,7 +313,20 @@ __global__ __launch_bounds__(kBlock) void rhs_kernel(Batch b, int init_cg) {
       const T* Mx = (const T*)(bdf ? I->C : I->E);
       const int sh = I->shift;
       const int* col = I->col;
+      constexpr bool kDiff = sizeof(T) < sizeof(double);
+      // the row's own velocity (difference form: E v = e_m v_i + sum_j E_ij (v_j - v_i), with
+      // e_m = m (1/dt - alpha) for E and m alpha for C)
+      double own[3] = {0.0, 0.0, 0.0};
       double s[3] = {0.0, 0.0, 0.0};
+      if (kDiff) {
+        const double em = bdf ? I->ra : invDt - I->ra;
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+          const size_t i = 3 * (size_t)r + j;
+          own[j] = bdf ? invDt * (u[i] - uOld[i]) - vOld[i] : vel[i];
+          s[j] = I->mass[i] * em * own[j];
+        }
+      }
       const int len = I->row_len[r];
       const int base = I->slice_off[r >> sh] + (r & ((1 << sh) - 1));
       for (int k = 0; k < len; k++) {
@@ -310,7 +336,7 @@ __global__ __launch_bounds__(kBlock) void rhs_kernel(Batch b, int init_cg) {
 #pragma unroll
         for (int j = 0; j < 3; j++) {
           const size_t i = 3 * (size_t)c + j;
-          xv[j] = bdf ? invDt * (u[i] - uOld[i]) - vOld[i] : vel[i];
+          xv[j] = (bdf ? invDt * (u[i] - uOld[i]) - vOld[i] : vel[i]) - own[j];
         }
 #pragma unroll
         for (int i = 0; i < 3; i++) {
@@ -398,13 +424,15 @@ __global__ __launch_bounds__(kBlock) void cg_init_kernel(Batch b, int refine, in
 
 // ---------------------------------------------------------------------------------------------
 // PCG iteration of parity p: two kernels (Eigen's loop body, ConjugateGradient.h:69-88, regrouped)
-//   spmv(p):    S = sc[p].  Unless S is the first set of the solve, it first finishes the
+//   spmv(p):    S = sc[p].  Unless S is the first set of the solve, it also finishes the
 //               previous iteration: from the update kernel's partials, |r|^2 < threshold -> done
 //               (the converging pass is not counted); else beta = r.z / S.absNew, absNew = r.z,
 //               iters + 1 (done at maxIters).  The instance's workgroup 0 writes that as the next
-//               set sc[p ^ 1] (a finished set is carried forward).  Then p_new = z + beta p_old,
-//               formed on the fly at every column (p double-buffered by parity), t = A p_new,
-//               partials of p.t.
+//               set sc[p ^ 1] (a finished set is carried forward).  The product t = A p_new with
+//               p_new = z + beta p_old is formed as A z + beta (A p_old): the matrix streams
+//               while the partials are still being reduced, and beta is applied at the end.
+//               p_new is written for the kernel's own rows (p double-buffered by parity), and the
+//               partials of p.t are published.
 //   update(p):  S' = sc[p ^ 1]; alpha = S'.absNew / (p.t); x += alpha p_new; r -= alpha t;
 //               z = M^-1 r; partials of r.r and r.z.
 // Kernels of a finished or inactive solve return at once, so groups of iterations can be queued
@@ -426,55 +454,23 @@ __global__ __launch_bounds__(kBlock) void cg_spmv_kernel(Batch b, int parity) {
     const int first = S->first;
     const double absOld = S->absNew;
     const int iters = S->iters;
+    const double threshold = S->threshold;
+    const int maxIters = S->maxIters;
     if (lead) {
       N->rhsNorm2 = S->rhsNorm2;
-      N->threshold = S->threshold;
+      N->threshold = threshold;
       N->absNew = absOld;
       N->alpha = S->alpha;
       N->beta = S->beta;
       N->resNorm2 = S->resNorm2;
       N->maxR = S->maxR;
       N->iters = iters;
-      N->maxIters = S->maxIters;
+      N->maxIters = maxIters;
       N->done = done;
       N->status = S->status;
       N->first = 0;
     }
     if (done) continue;
-    T beta = T(0);
-    if (!first) {
-      double rr, rz;
-      gather(I->part_rr, I->part_rz, I->nb[K_UPD], rr, rz);
-      if (nonfinite(rr) || !(rz == rz)) {
-        if (lead) {
-          N->resNorm2 = rr;
-          N->done = 1;
-          N->status = 2;
-        }
-        continue;
-      }
-      if (rr < S->threshold) {
-        if (lead) {
-          N->resNorm2 = rr;
-          N->done = 1;
-        }
-        continue;
-      }
-      const double bt = rz / absOld;
-      const int maxIters = S->maxIters;
-      if (lead) {
-        N->resNorm2 = rr;
-        N->beta = bt;
-        N->absNew = rz;
-        N->iters = iters + 1;
-        if (iters + 1 >= maxIters) {
-          N->done = 1;
-          N->status = 1;
-        }
-      }
-      if (iters + 1 >= maxIters) continue;
-      beta = (T)bt;
-    }
     constexpr int L = 64 / H;
     const int slice = (int)((lb * (unsigned)kBlock + threadIdx.x) >> 6);
     const int lane = threadIdx.x & 63;
@@ -488,42 +484,110 @@ __global__ __launch_bounds__(kBlock) void cg_spmv_kernel(Batch b, int parity) {
     const int* col = I->col;
     const size_t ns = (size_t)I->nslots;
     const int nrows = I->nv;
-    T s0 = T(0), s1 = T(0), s2 = T(0);
+    // a0..a2 = (A z)_r, c0..c2 = (A p_old)_r; in the difference form (T = float) the column
+    // values are taken relative to the row's own (zo, qo) and dg adds the diagonal term
+    constexpr bool kDiff = sizeof(T) < sizeof(double);
+    T a0 = T(0), a1 = T(0), a2 = T(0), c0 = T(0), c1 = T(0), c2 = T(0);
+    T zo0 = T(0), zo1 = T(0), zo2 = T(0), qo0 = T(0), qo1 = T(0), qo2 = T(0);
     const bool live = r < nrows;
     if (live) {
       const int len = I->row_len[r];
       const int base = I->slice_off[slice] + rr;
+      if (kDiff) {
+        const size_t i = 3 * (size_t)r;
+        zo0 = z[i];
+        zo1 = z[i + 1];
+        zo2 = z[i + 2];
+        if (!first) {
+          qo0 = pold[i];
+          qo1 = pold[i + 1];
+          qo2 = pold[i + 2];
+        }
+      }
       if (first) {
         for (int k = kk; k < len; k += L) {
           const int slot = base + k * H;
           const size_t c = 3 * (size_t)col[slot];
-          const T p0 = z[c], p1 = z[c + 1], p2 = z[c + 2];
-          s0 += A[0 * ns + slot] * p0 + A[1 * ns + slot] * p1 + A[2 * ns + slot] * p2;
-          s1 += A[3 * ns + slot] * p0 + A[4 * ns + slot] * p1 + A[5 * ns + slot] * p2;
-          s2 += A[6 * ns + slot] * p0 + A[7 * ns + slot] * p1 + A[8 * ns + slot] * p2;
+          const T z0 = z[c] - zo0, z1 = z[c + 1] - zo1, z2 = z[c + 2] - zo2;
+          a0 += A[0 * ns + slot] * z0 + A[1 * ns + slot] * z1 + A[2 * ns + slot] * z2;
+          a1 += A[3 * ns + slot] * z0 + A[4 * ns + slot] * z1 + A[5 * ns + slot] * z2;
+          a2 += A[6 * ns + slot] * z0 + A[7 * ns + slot] * z1 + A[8 * ns + slot] * z2;
         }
       } else {
         for (int k = kk; k < len; k += L) {
           const int slot = base + k * H;
           const size_t c = 3 * (size_t)col[slot];
-          const T p0 = z[c] + beta * pold[c];
-          const T p1 = z[c + 1] + beta * pold[c + 1];
-          const T p2 = z[c + 2] + beta * pold[c + 2];
-          s0 += A[0 * ns + slot] * p0 + A[1 * ns + slot] * p1 + A[2 * ns + slot] * p2;
-          s1 += A[3 * ns + slot] * p0 + A[4 * ns + slot] * p1 + A[5 * ns + slot] * p2;
-          s2 += A[6 * ns + slot] * p0 + A[7 * ns + slot] * p1 + A[8 * ns + slot] * p2;
+          const T z0 = z[c] - zo0, z1 = z[c + 1] - zo1, z2 = z[c + 2] - zo2;
+          const T q0 = pold[c] - qo0, q1 = pold[c + 1] - qo1, q2 = pold[c + 2] - qo2;
+          const T m0 = A[0 * ns + slot], m1 = A[1 * ns + slot], m2 = A[2 * ns + slot];
+          const T m3 = A[3 * ns + slot], m4 = A[4 * ns + slot], m5 = A[5 * ns + slot];
+          const T m6 = A[6 * ns + slot], m7 = A[7 * ns + slot], m8 = A[8 * ns + slot];
+          a0 += m0 * z0 + m1 * z1 + m2 * z2;
+          a1 += m3 * z0 + m4 * z1 + m5 * z2;
+          a2 += m6 * z0 + m7 * z1 + m8 * z2;
+          c0 += m0 * q0 + m1 * q1 + m2 * q2;
+          c1 += m3 * q0 + m4 * q1 + m5 * q2;
+          c2 += m6 * q0 + m7 * q1 + m8 * q2;
         }
       }
     }
 #pragma unroll
     for (int off = H; off < 64; off <<= 1) {
-      s0 += __shfl_xor(s0, off, 64);
-      s1 += __shfl_xor(s1, off, 64);
-      s2 += __shfl_xor(s2, off, 64);
+      a0 += __shfl_xor(a0, off, 64);
+      a1 += __shfl_xor(a1, off, 64);
+      a2 += __shfl_xor(a2, off, 64);
+      if (!first) {
+        c0 += __shfl_xor(c0, off, 64);
+        c1 += __shfl_xor(c1, off, 64);
+        c2 += __shfl_xor(c2, off, 64);
+      }
+    }
+    T beta = T(0);
+    if (!first) {
+      double rrv, rz;
+      gather(I->part_rr, I->part_rz, I->nb[K_UPD], rrv, rz);
+      if (nonfinite(rrv) || !(rz == rz)) {
+        if (lead) {
+          N->resNorm2 = rrv;
+          N->done = 1;
+          N->status = 2;
+        }
+        continue;
+      }
+      if (rrv < threshold) {
+        if (lead) {
+          N->resNorm2 = rrv;
+          N->done = 1;
+        }
+        continue;
+      }
+      const double bt = rz / absOld;
+      if (lead) {
+        N->resNorm2 = rrv;
+        N->beta = bt;
+        N->absNew = rz;
+        N->iters = iters + 1;
+        if (iters + 1 >= maxIters) {
+          N->done = 1;
+          N->status = 1;
+        }
+      }
+      if (iters + 1 >= maxIters) continue;
+      beta = (T)bt;
     }
     double pt = 0.0;
     if (live && kk == 0) {
       const size_t i = 3 * (size_t)r;
+      if (kDiff) {
+        const T d0 = (T)I->dg[i], d1 = (T)I->dg[i + 1], d2 = (T)I->dg[i + 2];
+        a0 += d0 * zo0;
+        a1 += d1 * zo1;
+        a2 += d2 * zo2;
+        c0 += d0 * qo0;
+        c1 += d1 * qo1;
+        c2 += d2 * qo2;
+      }
+      const T s0 = a0 + beta * c0, s1 = a1 + beta * c1, s2 = a2 + beta * c2;
       T* t = (T*)I->t;
       t[i] = s0;
       t[i + 1] = s1;
@@ -546,7 +610,8 @@ __global__ __launch_bounds__(kBlock) void cg_spmv_kernel(Batch b, int parity) {
 // One lane per DOF, 21 rows (63 lanes) per wavefront so that a row's three DOFs share a wave and
 // the 3x3 preconditioner apply can read its siblings' residuals with cross-lane moves.  Each
 // instance's workgroups stride over its waves (at most kUpdateCap workgroups per instance), which
-// keeps the partial-sum count (and the work of every workgroup that re-reduces them) small.
+// keeps the partial-sum count (and the work of every workgroup that re-reduces them) small.  The
+// first wave's vectors are loaded before the p.t partials are reduced, so the two overlap.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void cg_update_kernel(Batch b, int parity) {
   for (int vb = blockIdx.x; vb < b.total; vb += gridDim.x) {
@@ -555,9 +620,6 @@ __global__ __launch_bounds__(kBlock) void cg_update_kernel(Batch b, int parity) 
     const CgScalars* S = &I->ctl->sc[parity ^ 1];
     if (S->done) continue;
     const double absNew = S->absNew;
-    double pt, unused;
-    gather(I->part_pt, nullptr, I->nb[K_SPMV], pt, unused);
-    const T alpha = (T)(absNew / pt);
     const int lane = threadIdx.x & 63;
     const int c = lane - 3 * (lane / 3);
     const int bse = lane - c;
@@ -571,15 +633,36 @@ __global__ __launch_bounds__(kBlock) void cg_update_kernel(Batch b, int parity) 
     T* z = (T*)I->z;
     const T* t = (const T*)I->t;
     const int precond = I->precond;
+    const int w0 = (int)((lb * (unsigned)kBlock + threadIdx.x) >> 6);
+    T pv = T(0), tv = T(0), rv0 = T(0), xv = T(0);
+    {
+      const int row = w0 * 21 + lane / 3;
+      if (w0 < nwaves && lane < 63 && row < nrows) {
+        const size_t i = 3 * (size_t)row + c;
+        pv = p[i];
+        tv = t[i];
+        rv0 = rv_[i];
+        xv = x[i];
+      }
+    }
+    double pt, unused;
+    gather(I->part_pt, nullptr, I->nb[K_SPMV], pt, unused);
+    const T alpha = (T)(absNew / pt);
     double rr2 = 0.0, rz = 0.0;
-    for (int wave = (int)((lb * (unsigned)kBlock + threadIdx.x) >> 6); wave < nwaves; wave += stride) {
+    for (int wave = w0; wave < nwaves; wave += stride) {
       const int row = wave * 21 + lane / 3;
       const bool live = lane < 63 && row < nrows;
       const size_t i = 3 * (size_t)row + c;
+      if (wave != w0 && live) {
+        pv = p[i];
+        tv = t[i];
+        rv0 = rv_[i];
+        xv = x[i];
+      }
       T rv = T(0);
       if (live) {
-        x[i] += alpha * p[i];
-        rv = rv_[i] - alpha * t[i];
+        x[i] = xv + alpha * pv;
+        rv = rv0 - alpha * tv;
         rv_[i] = rv;
       }
       const T r0 = __shfl(rv, bse, 64);
@@ -687,17 +770,25 @@ __global__ __launch_bounds__(kBlock) void refine_residual_kernel(Batch b) {
     const double* x64 = I->x64;
     const int* col = I->col;
     const size_t ns = (size_t)I->nslots;
+    constexpr bool kDiff = sizeof(T) < sizeof(double);
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    double o0 = 0.0, o1 = 0.0, o2 = 0.0;  // the row's own x64 + x (difference form)
     const bool live = r < I->nv;
     if (live) {
       const int len = I->row_len[r];
       const int base = I->slice_off[slice] + rr;
+      if (kDiff) {
+        const size_t i = 3 * (size_t)r;
+        o0 = x64[i] + (double)xi[i];
+        o1 = x64[i + 1] + (double)xi[i + 1];
+        o2 = x64[i + 2] + (double)xi[i + 2];
+      }
       for (int k = kk; k < len; k += L) {
         const int slot = base + k * H;
         const size_t c = 3 * (size_t)col[slot];
-        const double x0 = x64[c] + (double)xi[c];
-        const double x1 = x64[c + 1] + (double)xi[c + 1];
-        const double x2 = x64[c + 2] + (double)xi[c + 2];
+        const double x0 = (x64[c] + (double)xi[c]) - o0;
+        const double x1 = (x64[c + 1] + (double)xi[c + 1]) - o1;
+        const double x2 = (x64[c + 2] + (double)xi[c + 2]) - o2;
         s0 += (double)A[0 * ns + slot] * x0 + (double)A[1 * ns + slot] * x1 + (double)A[2 * ns + slot] * x2;
         s1 += (double)A[3 * ns + slot] * x0 + (double)A[4 * ns + slot] * x1 + (double)A[5 * ns + slot] * x2;
         s2 += (double)A[6 * ns + slot] * x0 + (double)A[7 * ns + slot] * x1 + (double)A[8 * ns + slot] * x2;
@@ -711,6 +802,12 @@ __global__ __launch_bounds__(kBlock) void refine_residual_kernel(Batch b) {
     }
     double res2 = 0.0;
     if (live && kk == 0) {
+      if (kDiff) {
+        const size_t i = 3 * (size_t)r;
+        s0 += I->dg[i] * o0;
+        s1 += I->dg[i + 1] * o1;
+        s2 += I->dg[i + 2] * o2;
+      }
       const double s[3] = {s0, s1, s2};
 #pragma unroll
       for (int c = 0; c < 3; c++) {

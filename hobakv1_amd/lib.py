@@ -154,6 +154,11 @@ def _i(a):
     return a.ctypes.data_as(_ip)
 
 
+def inst_bytes_estimate(nv: int, nt: int, precision: int) -> int:
+    """Device bytes an instance of this size takes (hobak_inst_bytes_estimate)."""
+    return int(load().hobak_inst_bytes_estimate(int(nv), int(nt), int(precision)))
+
+
 def default_params() -> hobak_step_params:
     p = hobak_step_params()
     load().hobak_default_params(ctypes.byref(p))
