@@ -1,8 +1,10 @@
 """Offline autotuner for the PCG kernels' launch shapes.
 
-Sweeps the block-matrix slice height (rows per SELL slice, which sets lanes per row in the SpMV)
-and how many PCG iterations are queued between host polls, on a given workload, and records per configuration the average
-step time and the sampled SpMV launch time.  The best configuration per precision is written to
+Sweeps the block-matrix slice height (rows per SELL slice, which sets lanes per row in the SpMV),
+the PCG update kernel's workgroup cap (the partial sums the next SpMV re-reduces), how many PCG
+iterations are queued per group, and (reduced-precision form) the first inner tolerance, on a
+given workload, and records per configuration the average step time and the sampled SpMV launch
+time.  The best configuration per precision is written to
 tuning.json next to this file; lib.Context applies it to every instance it creates.
 
     python -m hobakv1_amd.autotune --n 40 --steps 4 [--precision fp64|fp32] [--write]
@@ -47,7 +49,8 @@ def measure(ctx, scene, precond, precision, steps, warmup, inner_tol=1e-4):
     return {"ms_per_step": 1e3 * dt / steps, "pcg_iters": iters / steps,
             "ms_per_pcg_iter": 1e3 * dt / max(1, iters),
             "spmv_ms": ks["spmv_avg_ms"],
-            "spmv_GBps": ks["spmv_bytes"] / max(1e-12, ks["spmv_avg_ms"] * 1e-3) / 1e9}
+            "spmv_GBps": (ks["spmv_bytes"] / (ks["spmv_avg_ms"] * 1e-3) / 1e9
+                          if ks["spmv_avg_ms"] > 0 else None)}
 
 
 def main():
@@ -58,6 +61,7 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--slice-rows", default="8,16,32,64")
     ap.add_argument("--poll", default="8,32")
+    ap.add_argument("--update-wgs", default="512")
     ap.add_argument("--inner-tol", default="1e-4", help="FP32 inner PCG tolerances to sweep")
     ap.add_argument("--write", action="store_true")
     args = ap.parse_args()
@@ -68,14 +72,15 @@ def main():
     sc = scenes.lattice_scene((args.n,) * 3, seed=2, stretch_velocity=True)
     results = []
     tols = [float(x) for x in args.inner_tol.split(",")] if args.precision == "fp32" else [0.0]
-    for H, poll, tol in itertools.product([int(x) for x in args.slice_rows.split(",")],
-                                          [int(x) for x in args.poll.split(",")], tols):
+    for H, poll, uw, tol in itertools.product([int(x) for x in args.slice_rows.split(",")],
+                                              [int(x) for x in args.poll.split(",")],
+                                              [int(x) for x in args.update_wgs.split(",")], tols):
         ctx = lib.Context(0, apply_table=False)
-        ctx.set_tuning(slice_rows=H, poll_chunk=poll)
+        ctx.set_tuning(slice_rows=H, poll_chunk=poll, update_wgs=uw)
         r = measure(ctx, sc, scenes.PRECOND_BLOCK_JACOBI, precision, args.steps, args.warmup,
                     inner_tol=tol or 1e-4)
         ctx.close()
-        r.update({"slice_rows": H, "poll_chunk": poll})
+        r.update({"slice_rows": H, "poll_chunk": poll, "update_wgs": uw})
         if args.precision == "fp32":
             r["refine_inner_tol"] = tol
         results.append(r)
@@ -85,6 +90,7 @@ def main():
     if args.write:
         table = load_table()
         table[args.precision] = {"slice_rows": best["slice_rows"], "poll_chunk": best["poll_chunk"],
+                                 "update_wgs": best["update_wgs"],
                                  "measured": {"workload": f"{args.n}^3 lattice", **best}}
         if "refine_inner_tol" in best:
             table[args.precision]["refine_inner_tol"] = best["refine_inner_tol"]

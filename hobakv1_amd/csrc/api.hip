@@ -55,6 +55,7 @@ struct hobak_ctx {
   int max_chunks = 64;                 // groups queued between host polls
   int use_graphs = 1;
   int grid_cap = 0;                    // max workgroups per launch (0: one per virtual workgroup)
+  int update_cap = kUpdateCap;         // per-instance workgroups of the PCG update kernel
   hipStream_t stream = nullptr;
   size_t budget = 0;
   size_t in_use = 0;
@@ -248,6 +249,9 @@ struct hobak_sched {
   std::vector<hobak_completion> done;
   std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
   double ms[5] = {0, 0, 0, 0, 0};
+  // counters since creation: ticks, PCG groups queued, GPU ms of prologues / PCG / epilogues,
+  // host wall ms inside ticks, solves finished
+  double stat[7] = {0, 0, 0, 0, 0, 0, 0};
   // sampled SpMV launches of the current batch: (iteration index since the batch's prologue, slot)
   std::vector<std::pair<int, int>> samples;
   int sample_it = 0;
@@ -737,6 +741,7 @@ int tick_impl(hobak_sched* S) {
     ran = true;
   }
 
+  S->stat[1] += nsolving ? m : 0;
   // 5. statuses
   HIPCHK(ctx, launch_status(M.b[K_ONE], M.status_dev, s));
   HIPCHK(ctx, hipMemcpyAsync(M.status_host, M.status_dev, sizeof(InstStatus) * M.n, hipMemcpyDeviceToHost, s));
@@ -753,6 +758,10 @@ int tick_impl(hobak_sched* S) {
   }
   if (ran) S->ms[3] += elapsed(ctx, 4, 5);
   if (epi) S->ms[4] += elapsed(ctx, 6, 7);
+  S->stat[0] += 1;
+  if (pro) S->stat[2] += elapsed(ctx, 0, 3);
+  if (ran) S->stat[3] += elapsed(ctx, 4, 5);
+  if (epi) S->stat[4] += elapsed(ctx, 6, 7);
 
   // completions: their epilogue ran before the wait above
   for (size_t i = 0; i < S->active.size();) {
@@ -800,6 +809,7 @@ int tick_impl(hobak_sched* S) {
       r.st.status = worse(r.st.status, code);
       r.I->pred[nk] = iters;
       r.stage = 2;
+      S->stat[6] += 1;
     } else {
       r.done_iters = refine ? q.inner_total + q.iters : q.iters;
     }
@@ -1119,7 +1129,7 @@ int hobak_inst_create(hobak_ctx* ctx, const double* rest_xyz, int32_t nv, const 
   h.nt = nt;
   h.nslots = P.nslots;
   h.shift = P.shift;
-  for (int k = 0; k < K_COUNT; k++) h.nb[k] = blocks_of(k, nv, nt, P.nslots, P.shift);
+  for (int k = 0; k < K_COUNT; k++) h.nb[k] = blocks_of(k, nv, nt, P.nslots, P.shift, ctx->update_cap);
   h.material = material->kind;
   h.precond = params->precond;
   h.bdf = I->prm.newton_iters > 1;
@@ -1187,7 +1197,6 @@ int hobak_inst_create(hobak_ctx* ctx, const double* rest_xyz, int32_t nv, const 
   ALLOCT(h.p0, 3 * np);
   ALLOCT(h.p1, 3 * np);
   ALLOCT(h.t, 3 * np);
-  ALLOC(h.ctl, 1);
   if (h.refine) ALLOC(h.dg, 3 * np);
   ALLOC(h.part_row_a, (size_t)h.nb[K_ROW]);
   ALLOC(h.part_row_b, (size_t)h.nb[K_ROW]);
@@ -1195,7 +1204,12 @@ int hobak_inst_create(hobak_ctx* ctx, const double* rest_xyz, int32_t nv, const 
   ALLOC(h.part_res, (size_t)h.nb[K_SPMV]);
   ALLOC(h.part_rr, (size_t)h.nb[K_UPD]);
   ALLOC(h.part_rz, (size_t)h.nb[K_UPD]);
-  ALLOC(I->dev, 1);
+  {
+    char* rec = nullptr;  // the record and, kCtlOffset bytes after it, the control block
+    if ((rc = dalloc(I, &rec, kCtlOffset + sizeof(Ctl), 1)) != HOBAK_OK) return bail(rc);
+    I->dev = (InstDev*)rec;
+    h.ctl = ctl_of(I->dev);
+  }
 #undef ALLOC
 #undef ALLOCT
   h.rest = rest;
@@ -1448,6 +1462,10 @@ int hobak_ctx_set_tuning(hobak_ctx* ctx, int32_t key, int32_t value) {
       if (value < 0) return fail(ctx, HOBAK_E_INVALID, "grid cap must be >= 0");
       ctx->grid_cap = value;
       return HOBAK_OK;
+    case HOBAK_TUNE_UPDATE_WGS:
+      if (value < 1 || value > 65536) return fail(ctx, HOBAK_E_INVALID, "update workgroups out of range");
+      ctx->update_cap = value;
+      return HOBAK_OK;
     default:
       return fail(ctx, HOBAK_E_INVALID, "unknown tuning key");
   }
@@ -1509,7 +1527,9 @@ int hobak_sched_tick(hobak_sched* S, hobak_completion* out, int32_t max_out, int
   if (!S || max_out < 0 || (max_out > 0 && !out)) return HOBAK_E_INVALID;
   (void)hipSetDevice(S->ctx->device);
   if (!S->queue.empty() || !S->active.empty()) {
+    const auto t0 = std::chrono::steady_clock::now();
     const int rc = tick(S);
+    S->stat[5] += 1e3 * std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if (rc != HOBAK_OK) return rc;
   }
   const int n = std::min<int>(max_out, (int)S->done.size());
@@ -1521,5 +1541,11 @@ int hobak_sched_tick(hobak_sched* S, hobak_completion* out, int32_t max_out, int
 }
 
 double hobak_sched_clock(hobak_sched* S) { return S ? clock_s(S) : 0.0; }
+
+int hobak_sched_stats(hobak_sched* S, double* out) {
+  if (!S || !out) return HOBAK_E_INVALID;
+  std::memcpy(out, S->stat, sizeof(S->stat));
+  return HOBAK_OK;
+}
 
 }  // extern "C"

@@ -25,23 +25,23 @@ namespace hobak {
 struct ElementIn {
   int nt;
   double mu, lam, alpha;
-  const int4* tets;
-  const double* rest;
-  const double* u;
-  const double* dminv;
-  const double* vol;
-  void* stage_h;
-  void* stage_f;
-  const int* hdest;
-  const int* fdest;
+  const HOBAK_GLOBAL int* tets;  // 4 per tet
+  const HOBAK_GLOBAL double* rest;
+  const HOBAK_GLOBAL double* u;
+  const HOBAK_GLOBAL double* dminv;
+  const HOBAK_GLOBAL double* vol;
+  HOBAK_GLOBAL void* stage_h;
+  HOBAK_GLOBAL void* stage_f;
+  const HOBAK_GLOBAL int* hdest;
+  const HOBAK_GLOBAL int* fdest;
   int with_hessian;
 };
 
 template <typename T, int MAT>
 __device__ __forceinline__ void element_tet(const ElementIn& a, int t) {
   const size_t nt = (size_t)a.nt;
-  const int4 tv = a.tets[t];
-  const int vid[4] = {tv.x, tv.y, tv.z, tv.w};
+  const int vid[4] = {a.tets[4 * (size_t)t], a.tets[4 * (size_t)t + 1], a.tets[4 * (size_t)t + 2],
+                      a.tets[4 * (size_t)t + 3]};
 
   // corner positions x = X + u (TET_MESH::setDisplacement) and edge matrix Ds
   double x[4][3];
@@ -104,10 +104,10 @@ __device__ __forceinline__ void element_tet(const ElementIn& a, int t) {
   } else {
     arap_pk1<T>((T)a.mu, U, S, V, P);
   }
-  T* sf = (T*)a.stage_f;
+  auto sf = (HOBAK_GLOBAL T*)a.stage_f;
 #pragma unroll
   for (int v = 0; v < 4; v++) {
-    T* dst = sf + 3 * (size_t)a.fdest[4 * (size_t)t + v];
+    auto dst = sf + 3 * (size_t)a.fdest[4 * (size_t)t + v];
 #pragma unroll
     for (int c = 0; c < 3; c++) {
       const T s = (P[c][0] * Wd[v][0] + P[c][1] * Wd[v][1]) + P[c][2] * Wd[v][2];
@@ -128,8 +128,8 @@ __device__ __forceinline__ void element_tet(const ElementIn& a, int t) {
 #pragma unroll
     for (int b = 0; b < 3; b++) w[v][b] = Wd[v][0] * V[0][b] + Wd[v][1] * V[1][b] + Wd[v][2] * V[2][b];
 
-  T* sh = (T*)a.stage_h;
-  const int* hd = a.hdest + 16 * (size_t)t;
+  auto sh = (HOBAK_GLOBAL T*)a.stage_h;
+  auto hd = a.hdest + 16 * (size_t)t;
 #pragma unroll
   for (int v = 0; v < 4; v++) {
 #pragma unroll
@@ -161,13 +161,13 @@ __device__ __forceinline__ void element_tet(const ElementIn& a, int t) {
         for (int c2 = 0; c2 < 3; c2++)
           B[c][c2] = vol * (UM[c][0] * U[c2][0] + UM[c][1] * U[c2][1] + UM[c][2] * U[c2][2]);
       // block (v, v2) and, off the diagonal, its transpose as block (v2, v)
-      T* d0 = sh + 9 * (size_t)hd[4 * v + v2];
+      auto d0 = sh + 9 * (size_t)hd[4 * v + v2];
 #pragma unroll
       for (int c = 0; c < 3; c++)
 #pragma unroll
         for (int c2 = 0; c2 < 3; c2++) d0[3 * c + c2] = B[c][c2];
       if (v2 != v) {
-        T* d1 = sh + 9 * (size_t)hd[4 * v2 + v];
+        auto d1 = sh + 9 * (size_t)hd[4 * v2 + v];
 #pragma unroll
         for (int c = 0; c < 3; c++)
 #pragma unroll
@@ -183,21 +183,21 @@ template <typename T>
 __global__ __launch_bounds__(kElementBlock) void element_kernel(Batch b, int src, int with_hessian) {
   for (int vb = blockIdx.x; vb < b.total; vb += gridDim.x) {
     int lb, lg;
-    const InstDev* __restrict__ I = seg_find(b, vb, lb, lg);
+    const HOBAK_GLOBAL InstDev* __restrict__ I = seg_find(b, vb, lb, lg);
     ElementIn a;
     a.nt = I->nt;
     a.mu = I->mu;
     a.lam = I->lam;
     a.alpha = I->alpha;
-    a.tets = I->tets;
-    a.rest = I->rest;
-    a.u = src == ELEM_TMP ? I->tmp64 : I->u;
-    a.dminv = I->dminv;
-    a.vol = I->vol;
-    a.stage_h = I->stage_h;
-    a.stage_f = I->stage_f;
-    a.hdest = I->hdest;
-    a.fdest = I->fdest;
+    a.tets = gbl((const int*)I->tets);
+    a.rest = gbl(I->rest);
+    a.u = gbl(src == ELEM_TMP ? I->tmp64 : I->u);
+    a.dminv = gbl(I->dminv);
+    a.vol = gbl(I->vol);
+    a.stage_h = gbl(I->stage_h);
+    a.stage_f = gbl(I->stage_f);
+    a.hdest = gbl(I->hdest);
+    a.fdest = gbl(I->fdest);
     a.with_hessian = with_hessian;
     const int material = I->material;
     const int t = lb * kElementBlock + threadIdx.x;

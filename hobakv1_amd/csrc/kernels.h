@@ -51,7 +51,7 @@ constexpr int kDefaultSliceRows = 32;
 constexpr int kBlock = 256;
 constexpr int kElementBlock = 128;
 constexpr int kMinvStride = 9;   // block-Jacobi inverse per row; Jacobi uses 1/a_ii per DOF
-constexpr int kUpdateCap = 512;  // per-instance workgroup cap of the PCG update kernel
+constexpr int kUpdateCap = 512;  // default per-instance workgroup cap of the PCG update kernel
 constexpr int kMaxSweeps = 32;   // FP64 refinement sweeps per linear solve
 
 // Virtual-workgroup kinds: each instance contributes nb[kind] workgroups to a launch of a kernel
@@ -130,7 +130,7 @@ struct InstDev {
   const unsigned char* pinned;
   const int *slice_off, *row_len, *col, *slot_row, *cptr, *hdest, *vptr, *fdest;
   void *stage_h, *stage_f, *A, *D, *E, *C, *kneg, *minv, *r, *x, *z, *t, *p0, *p1;
-  Ctl* ctl;
+  Ctl* ctl;    // == ctl_of(this) on the device
   double* dg;  // reduced-precision form: per-DOF diagonal term of the difference-form operator
   // per-workgroup partial sums, sized by the producing kernel's nb[kind]
   double *part_row_a, *part_row_b;  // K_ROW: right-hand side / inner-solve seed
@@ -138,6 +138,17 @@ struct InstDev {
   double *part_rr, *part_rz;        // K_UPD: r.r, r.z
   double *part_res;                 // K_SPMV: refinement residual
 };
+
+// The control block lives in the same allocation as the record, kCtlOffset bytes after it, so a
+// kernel reaches it without loading a pointer first.
+constexpr size_t kCtlOffset = (sizeof(InstDev) + 255) / 256 * 256;
+__host__ __device__ __forceinline__ Ctl* ctl_of(const InstDev* I) {
+  return (Ctl*)((char*)const_cast<InstDev*>(I) + kCtlOffset);
+}
+__device__ __forceinline__ __attribute__((address_space(1))) Ctl* ctl_of(
+    const __attribute__((address_space(1))) InstDev* I) {
+  return (__attribute__((address_space(1))) Ctl*)((const __attribute__((address_space(1))) char*)I + kCtlOffset);
+}
 
 // One segmented launch: the instances and the virtual-workgroup prefix table of the launch's kind.
 struct Batch {
@@ -150,11 +161,13 @@ struct Batch {
 
 // The instance of virtual workgroup vb, its workgroup index inside the instance (lb) and the
 // instance's workgroup count of this kind (lgrid).  vb is uniform, so are the loads.
-__device__ __forceinline__ const InstDev* seg_find(const Batch& b, int vb, int& lb, int& lgrid) {
+__device__ __forceinline__ const __attribute__((address_space(1))) InstDev* seg_find(const Batch& b, int vb,
+                                                                                int& lb, int& lgrid) {
+  typedef const __attribute__((address_space(1))) InstDev* GI;
   if (b.n == 1) {
     lb = vb;
     lgrid = b.total;
-    return b.one;
+    return (GI)b.one;
   }
   int lo = 0, hi = b.n;  // boff[lo] <= vb < boff[hi]
   while (hi - lo > 1) {
@@ -167,11 +180,20 @@ __device__ __forceinline__ const InstDev* seg_find(const Batch& b, int vb, int& 
   const int b0 = b.boff[lo];
   lb = vb - b0;
   lgrid = b.boff[lo + 1] - b0;
-  return b.inst[lo];
+  return (GI)b.inst[lo];
+}
+
+// Pointers read out of an InstDev are generic to the compiler, which then emits flat memory
+// instructions that it must wait on one by one.  Every kernel converts them with gbl() first,
+// so loads are global_load instructions that can be in flight together.
+#define HOBAK_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ HOBAK_GLOBAL T* gbl(T* p) {
+  return (HOBAK_GLOBAL T*)p;
 }
 
 // Workgroups per kind for an instance (host side; the same formulas size the partial arrays).
-int blocks_of(int kind, int nv, int nt, int nslots, int shift);
+int blocks_of(int kind, int nv, int nt, int nslots, int shift, int update_cap);
 
 // Launchers.  `cap` limits the launch grid (0: one workgroup per virtual workgroup).
 enum ElemSrc { ELEM_U = 0, ELEM_TMP = 1 };

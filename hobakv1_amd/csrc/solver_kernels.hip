@@ -55,7 +55,8 @@ __device__ __forceinline__ void block_sum(double (&v)[NV]) {
 }
 
 // Publish this workgroup's sums of (a, b) as entry lb of the partial arrays (pb may be null).
-__device__ __forceinline__ void publish(double a, double b, double* pa, double* pb, int lb) {
+__device__ __forceinline__ void publish(double a, double b, HOBAK_GLOBAL double* pa, HOBAK_GLOBAL double* pb,
+                                        int lb) {
   double v[2] = {a, b};
   block_sum<2>(v);
   if (threadIdx.x == 0) {
@@ -66,9 +67,43 @@ __device__ __forceinline__ void publish(double a, double b, double* pa, double* 
 
 // Every consuming workgroup reduces the producer's n partials in the same fixed order, so all of
 // them (and every run) obtain bit-identical totals.
-__device__ __forceinline__ void gather(const double* pa, const double* pb, int n, double& ta, double& tb) {
+__device__ __forceinline__ void gather(const HOBAK_GLOBAL double* pa, const HOBAK_GLOBAL double* pb, int n,
+                                       double& ta, double& tb) {
   double v[2] = {0.0, 0.0};
   for (int i = threadIdx.x; i < n; i += kBlock) {
+    v[0] += pa[i];
+    if (pb) v[1] += pb[i];
+  }
+  block_sum<2>(v);
+  ta = v[0];
+  tb = v[1];
+}
+
+// The same reduction in two halves: pre_load issues the loads of up to kPre partials per thread
+// (all in flight at once, before the caller's other work), pre_sum adds them (and any beyond
+// kPre * kBlock) and reduces over the workgroup.  Summation order is fixed, as in gather().
+constexpr int kPre = 4;
+struct PreParts {
+  double a[kPre], b[kPre];
+};
+__device__ __forceinline__ void pre_load(const HOBAK_GLOBAL double* pa, const HOBAK_GLOBAL double* pb, int n,
+                                         PreParts& q) {
+#pragma unroll
+  for (int k = 0; k < kPre; k++) {
+    const int i = threadIdx.x + k * kBlock;
+    q.a[k] = i < n ? pa[i] : 0.0;
+    q.b[k] = (pb && i < n) ? pb[i] : 0.0;
+  }
+}
+__device__ __forceinline__ void pre_sum(const HOBAK_GLOBAL double* pa, const HOBAK_GLOBAL double* pb, int n,
+                                        const PreParts& q, double& ta, double& tb) {
+  double v[2] = {0.0, 0.0};
+#pragma unroll
+  for (int k = 0; k < kPre; k++) {
+    v[0] += q.a[k];
+    v[1] += q.b[k];
+  }
+  for (int i = threadIdx.x + kPre * kBlock; i < n; i += kBlock) {
     v[0] += pa[i];
     if (pb) v[1] += pb[i];
   }
@@ -80,7 +115,7 @@ __device__ __forceinline__ void gather(const double* pa, const double* pb, int n
 __device__ __forceinline__ bool nonfinite(double v) { return !(v == v) || v > DBL_MAX || v < -DBL_MAX; }
 
 // An instance whose current inner solve has ended and has not yet been handled.
-__device__ __forceinline__ bool solve_ended(const Ctl* c) {
+__device__ __forceinline__ bool solve_ended(const HOBAK_GLOBAL Ctl* c) {
   return c->phase == PHASE_SOLVING && c->sc[0].done;
 }
 
@@ -94,20 +129,20 @@ static inline int grid_of(const Batch& b, int cap) { return cap > 0 && cap < b.t
 __global__ __launch_bounds__(kBlock) void begin_step_kernel(Batch b) {
   for (int vb = blockIdx.x; vb < b.total; vb += gridDim.x) {
     int lb, lg;
-    const InstDev* __restrict__ I = seg_find(b, vb, lb, lg);
+    const HOBAK_GLOBAL InstDev* __restrict__ I = seg_find(b, vb, lb, lg);
     const int r = lb * kBlock + threadIdx.x;
     if (r < I->nv) {
-      double* u = I->u;
-      const double* v = I->v;
-      double* uo = I->u_old;
-      double* vo = I->v_old;
-      const bool pin = I->pinned[r];
+      auto u = gbl(I->u);
+      auto v = gbl(I->v);
+      auto uo = gbl(I->u_old);
+      auto vo = gbl(I->v_old);
+      const bool pin = gbl(I->pinned)[r];
 #pragma unroll
       for (int c = 0; c < 3; c++) {
         const size_t i = 3 * (size_t)r + c;
         uo[i] = u[i];
         vo[i] = v[i];
-        if (pin) u[i] = I->pin_target[i];
+        if (pin) u[i] = gbl(I->pin_target)[i];
       }
     }
   }
@@ -118,9 +153,9 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void apply_dx_kernel(Batch b, int use64) {
   for (int vb = blockIdx.x; vb < b.total; vb += gridDim.x) {
     int lb, lg;
-    const InstDev* __restrict__ I = seg_find(b, vb, lb, lg);
+    const HOBAK_GLOBAL InstDev* __restrict__ I = seg_find(b, vb, lb, lg);
     const int i = lb * kBlock + threadIdx.x;
-    if (i < 3 * I->nv) I->u[i] += use64 ? I->x64[i] : (double)((const T*)I->x)[i];
+    if (i < 3 * I->nv) gbl(I->u)[i] += use64 ? gbl(I->x64)[i] : (double)(gbl((const T*)I->x))[i];
   }
 }
 
@@ -128,15 +163,15 @@ __global__ __launch_bounds__(kBlock) void apply_dx_kernel(Batch b, int use64) {
 __global__ __launch_bounds__(kBlock) void end_step_kernel(Batch b) {
   for (int vb = blockIdx.x; vb < b.total; vb += gridDim.x) {
     int lb, lg;
-    const InstDev* __restrict__ I = seg_find(b, vb, lb, lg);
+    const HOBAK_GLOBAL InstDev* __restrict__ I = seg_find(b, vb, lb, lg);
     const int r = lb * kBlock + threadIdx.x;
     if (r < I->nv) {
       const double invDt = I->invDt;
-      const bool pin = I->pinned[r];
+      const bool pin = gbl(I->pinned)[r];
 #pragma unroll
       for (int c = 0; c < 3; c++) {
         const size_t i = 3 * (size_t)r + c;
-        I->v[i] = pin ? 0.0 : invDt * (I->u[i] - I->u_old[i]);
+        gbl(I->v)[i] = pin ? 0.0 : invDt * (gbl(I->u)[i] - gbl(I->u_old)[i]);
       }
     }
   }
@@ -146,7 +181,7 @@ __global__ __launch_bounds__(kBlock) void end_step_kernel(Batch b) {
 // assembly
 // ---------------------------------------------------------------------------------------------
 template <typename T>
-__device__ __forceinline__ void write_precond(const T (&a)[9], T* minv, int r, int precond) {
+__device__ __forceinline__ void write_precond(const T (&a)[9], HOBAK_GLOBAL T* minv, int r, int precond) {
   if (precond == 0) {  // DiagonalPreconditioner: 1/a_ii, 1 where a_ii == 0
 #pragma unroll
     for (int c = 0; c < 3; c++) {
@@ -163,7 +198,7 @@ __device__ __forceinline__ void write_precond(const T (&a)[9], T* minv, int r, i
   const T c0 = cof(0, 0), c1 = cof(1, 0), c2 = cof(2, 0);
   const T det = (c0 * a[0] + c1 * a[3]) + c2 * a[6];
   const T invdet = T(1) / det;
-  T* m = minv + (size_t)kMinvStride * r;
+  auto m = minv + (size_t)kMinvStride * r;
   m[0] = c0 * invdet;
   m[1] = c1 * invdet;
   m[2] = c2 * invdet;
@@ -180,41 +215,41 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void assemble_kernel(Batch b, int target) {
   for (int vb = blockIdx.x; vb < b.total; vb += gridDim.x) {
     int lb, lg;
-    const InstDev* __restrict__ I = seg_find(b, vb, lb, lg);
+    const HOBAK_GLOBAL InstDev* __restrict__ I = seg_find(b, vb, lb, lg);
     const int slot = lb * kBlock + threadIdx.x;
     const int nslots = I->nslots;
     if (slot >= nslots) continue;
-    const int r = I->slot_row[slot];
+    const int r = gbl(I->slot_row)[slot];
     if (r < 0) continue;
-    const int c = I->col[slot];
+    const int c = gbl(I->col)[slot];
     const size_t ns = (size_t)nslots;
     T acc[9];
 #pragma unroll
     for (int e = 0; e < 9; e++) acc[e] = T(0);
     // the slot's contributions are contiguous and in ascending tet order
-    const T* src = (const T*)I->stage_h + 9 * (size_t)I->cptr[slot];
-    const T* end = (const T*)I->stage_h + 9 * (size_t)I->cptr[slot + 1];
+    auto src = gbl((const T*)I->stage_h) + 9 * (size_t)gbl(I->cptr)[slot];
+    auto end = gbl((const T*)I->stage_h) + 9 * (size_t)gbl(I->cptr)[slot + 1];
     for (; src < end; src += 9) {
 #pragma unroll
       for (int e = 0; e < 9; e++) acc[e] += src[e];
     }
     if (target != ASM_SYSTEM) {
-      T* out = (T*)(target == ASM_REST ? I->kneg : I->A);
+      auto out = gbl((T*)(target == ASM_REST ? I->kneg : I->A));
 #pragma unroll
       for (int e = 0; e < 9; e++) out[e * ns + slot] = acc[e];
       continue;
     }
     constexpr bool kDiff = sizeof(T) < sizeof(double);
     // standard form: pinned rows and columns are zeroed; difference form: only pinned rows
-    const bool pin = kDiff ? (bool)I->pinned[r] : (I->pinned[r] || I->pinned[c]);
+    const bool pin = kDiff ? (bool)gbl(I->pinned)[r] : (gbl(I->pinned)[r] || gbl(I->pinned)[c]);
     bool ground = false;
     if (I->ground_on && c == r && !pin) {
-      const double y = I->rest[3 * (size_t)r + 1] + I->u[3 * (size_t)r + 1];
+      const double y = gbl(I->rest)[3 * (size_t)r + 1] + gbl(I->u)[3 * (size_t)r + 1];
       ground = y < I->ground_y;
       if (ground) acc[4] += (T)I->ground_k;
     }
-    const T* D = (const T*)I->D;
-    T* out = (T*)I->A;
+    auto D = gbl((const T*)I->D);
+    auto out = gbl((T*)I->A);
     T val[9];
 #pragma unroll
     for (int e = 0; e < 9; e++) {
@@ -222,12 +257,12 @@ __global__ __launch_bounds__(kBlock) void assemble_kernel(Batch b, int target) {
       out[e * ns + slot] = val[e];
     }
     if (c == r) {
-      write_precond<T>(val, (T*)I->minv, r, I->precond);
+      write_precond<T>(val, gbl((T*)I->minv), r, I->precond);
       if (kDiff) {
 #pragma unroll
         for (int k = 0; k < 3; k++) {
-          const double mk = I->mass[3 * (size_t)r + k] * (I->invDt2 - I->ra * I->invDt);
-          I->dg[3 * (size_t)r + k] = pin ? 1.0 : mk + ((k == 1 && ground) ? I->ground_k : 0.0);
+          const double mk = gbl(I->mass)[3 * (size_t)r + k] * (I->invDt2 - I->ra * I->invDt);
+          gbl(I->dg)[3 * (size_t)r + k] = pin ? 1.0 : mk + ((k == 1 && ground) ? I->ground_k : 0.0);
         }
       }
     }
@@ -241,26 +276,26 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void constant_parts_kernel(Batch b) {
   for (int vb = blockIdx.x; vb < b.total; vb += gridDim.x) {
     int lb, lg;
-    const InstDev* __restrict__ I = seg_find(b, vb, lb, lg);
+    const HOBAK_GLOBAL InstDev* __restrict__ I = seg_find(b, vb, lb, lg);
     const int slot = lb * kBlock + threadIdx.x;
     const int nslots = I->nslots;
     if (slot >= nslots) continue;
-    const int r = I->slot_row[slot];
+    const int r = gbl(I->slot_row)[slot];
     if (r < 0) continue;
-    const int c = I->col[slot];
+    const int c = gbl(I->col)[slot];
     const size_t ns = (size_t)nslots;
     const double ra = I->ra, rb = I->rb, invDt = I->invDt, invDt2 = I->invDt2;
 #pragma unroll
     for (int e = 0; e < 9; e++) {
-      const double krest = -(double)((const T*)I->kneg)[e * ns + slot];
+      const double krest = -(double)(gbl((const T*)I->kneg))[e * ns + slot];
       const bool dg = (c == r) && (e % 4 == 0);
-      const double m = dg ? I->mass[3 * (size_t)r + e / 4] : 0.0;
+      const double m = dg ? gbl(I->mass)[3 * (size_t)r + e / 4] : 0.0;
       const double cc = dg ? ra * m + rb * krest : rb * krest;
       const double d = dg ? m * invDt2 - cc * invDt : -(cc * invDt);
       const double ee = dg ? invDt * m - cc : -cc;
-      ((T*)I->D)[e * ns + slot] = (T)d;
-      ((T*)I->E)[e * ns + slot] = (T)ee;
-      ((T*)I->C)[e * ns + slot] = (T)cc;
+      (gbl((T*)I->D))[e * ns + slot] = (T)d;
+      (gbl((T*)I->E))[e * ns + slot] = (T)ee;
+      (gbl((T*)I->C))[e * ns + slot] = (T)cc;
     }
   }
 }
@@ -269,13 +304,13 @@ __global__ __launch_bounds__(kBlock) void constant_parts_kernel(Batch b) {
 // right-hand side + PCG initialisation
 // ---------------------------------------------------------------------------------------------
 template <typename T>
-__device__ __forceinline__ void apply_precond(const T* minv, int precond, int r, const T (&rv)[3],
+__device__ __forceinline__ void apply_precond(const HOBAK_GLOBAL T* minv, int precond, int r, const T (&rv)[3],
                                               T (&zv)[3]) {
   if (precond == 0) {
 #pragma unroll
     for (int c = 0; c < 3; c++) zv[c] = minv[3 * (size_t)r + c] * rv[c];
   } else {
-    const T* m = minv + (size_t)kMinvStride * r;
+    auto m = minv + (size_t)kMinvStride * r;
 #pragma unroll
     for (int i = 0; i < 3; i++) zv[i] = m[3 * i] * rv[0] + m[3 * i + 1] * rv[1] + m[3 * i + 2] * rv[2];
   }
@@ -288,31 +323,31 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void rhs_kernel(Batch b, int init_cg) {
   for (int vb = blockIdx.x; vb < b.total; vb += gridDim.x) {
     int lb, lg;
-    const InstDev* __restrict__ I = seg_find(b, vb, lb, lg);
+    const HOBAK_GLOBAL InstDev* __restrict__ I = seg_find(b, vb, lb, lg);
     const int r = lb * kBlock + threadIdx.x;
     double part[2] = {0.0, 0.0};
     if (r < I->nv) {
-      const T* sf = (const T*)I->stage_f;
-      const int* vptr = I->vptr;
+      auto sf = gbl((const T*)I->stage_f);
+      auto vptr = gbl(I->vptr);
       double R[3] = {0.0, 0.0, 0.0};
       for (int q = vptr[r]; q < vptr[r + 1]; q++) {
 #pragma unroll
         for (int c = 0; c < 3; c++) R[c] += (double)sf[3 * (size_t)q + c];
       }
-      const double* u = I->u;
+      auto u = gbl(I->u);
       if (I->ground_on) {
-        const double y = I->rest[3 * (size_t)r + 1] + u[3 * (size_t)r + 1];
+        const double y = gbl(I->rest)[3 * (size_t)r + 1] + u[3 * (size_t)r + 1];
         if (y < I->ground_y) R[1] += I->ground_k * (I->ground_y - y);
       }
       const int bdf = I->bdf;
       const double invDt = I->invDt, invDt2 = I->invDt2;
-      const double* uOld = I->u_old;
-      const double* vOld = I->v_old;
-      const double* vel = I->v;
+      auto uOld = gbl(I->u_old);
+      auto vOld = gbl(I->v_old);
+      auto vel = gbl(I->v);
       const size_t ns = (size_t)I->nslots;
-      const T* Mx = (const T*)(bdf ? I->C : I->E);
+      auto Mx = gbl((const T*)(bdf ? I->C : I->E));
       const int sh = I->shift;
-      const int* col = I->col;
+      auto col = gbl(I->col);
       constexpr bool kDiff = sizeof(T) < sizeof(double);
       // the row's own velocity (difference form: E v = e_m v_i + sum_j E_ij (v_j - v_i), with
       // e_m = m (1/dt - alpha) for E and m alpha for C)
@@ -324,11 +359,11 @@ __global__ __launch_bounds__(kBlock) void rhs_kernel(Batch b, int init_cg) {
         for (int j = 0; j < 3; j++) {
           const size_t i = 3 * (size_t)r + j;
           own[j] = bdf ? invDt * (u[i] - uOld[i]) - vOld[i] : vel[i];
-          s[j] = I->mass[i] * em * own[j];
+          s[j] = gbl(I->mass)[i] * em * own[j];
         }
       }
-      const int len = I->row_len[r];
-      const int base = I->slice_off[r >> sh] + (r & ((1 << sh) - 1));
+      const int len = gbl(I->row_len)[r];
+      const int base = gbl(I->slice_off)[r >> sh] + (r & ((1 << sh) - 1));
       for (int k = 0; k < len; k++) {
         const int slot = base + (k << sh);
         const int c = col[slot];
@@ -345,8 +380,8 @@ __global__ __launch_bounds__(kBlock) void rhs_kernel(Batch b, int init_cg) {
           s[i] += (double)Mx[(3 * i + 2) * ns + slot] * xv[2];
         }
       }
-      const bool pin = I->pinned[r];
-      const double orv = I->one_ring[r];
+      const bool pin = gbl(I->pinned)[r];
+      const double orv = gbl(I->one_ring)[r];
       double bv[3];
 #pragma unroll
       for (int c = 0; c < 3; c++) {
@@ -354,30 +389,30 @@ __global__ __launch_bounds__(kBlock) void rhs_kernel(Batch b, int init_cg) {
         const double fext = orv * I->g[c];
         if (bdf) {
           const double acc = -invDt2 * (u[i] - uOld[i]) + invDt * vOld[i];
-          bv[c] = s[c] + (((I->mass[i] * acc + R[c]) + fext));
+          bv[c] = s[c] + (((gbl(I->mass)[i] * acc + R[c]) + fext));
         } else {
           bv[c] = (s[c] + R[c]) + fext;
         }
         if (pin) bv[c] = 0.0;
-        I->rhs64[i] = bv[c];
+        gbl(I->rhs64)[i] = bv[c];
         part[0] += bv[c] * bv[c];
       }
       if (init_cg) {
         T rv[3], zv[3];
 #pragma unroll
         for (int c = 0; c < 3; c++) rv[c] = (T)bv[c];
-        apply_precond<T>((const T*)I->minv, I->precond, r, rv, zv);
+        apply_precond<T>(gbl((const T*)I->minv), I->precond, r, rv, zv);
 #pragma unroll
         for (int c = 0; c < 3; c++) {
           const size_t i = 3 * (size_t)r + c;
-          ((T*)I->r)[i] = rv[c];
-          ((T*)I->x)[i] = T(0);
-          ((T*)I->z)[i] = zv[c];
+          (gbl((T*)I->r))[i] = rv[c];
+          (gbl((T*)I->x))[i] = T(0);
+          (gbl((T*)I->z))[i] = zv[c];
           part[1] += (double)rv[c] * (double)zv[c];
         }
       }
     }
-    publish(part[0], part[1], I->part_row_a, I->part_row_b, lb);
+    publish(part[0], part[1], gbl(I->part_row_a), gbl(I->part_row_b), lb);
   }
 }
 
@@ -388,11 +423,11 @@ __global__ __launch_bounds__(kBlock) void rhs_kernel(Batch b, int init_cg) {
 __global__ __launch_bounds__(kBlock) void cg_init_kernel(Batch b, int refine, int gate) {
   for (int vb = blockIdx.x; vb < b.total; vb += gridDim.x) {
     int lb, lg;
-    const InstDev* __restrict__ I = seg_find(b, vb, lb, lg);
-    Ctl* c = I->ctl;
+    const HOBAK_GLOBAL InstDev* __restrict__ I = seg_find(b, vb, lb, lg);
+    auto c = ctl_of(I);
     if (gate && !solve_ended(c)) continue;
     double bb, rz;
-    gather(I->part_row_a, I->part_row_b, I->nb[K_ROW], bb, rz);
+    gather(gbl(I->part_row_a), gbl(I->part_row_b), I->nb[K_ROW], bb, rz);
     if (threadIdx.x != 0) continue;
     const double tol = refine ? c->rs.tau : I->tol;
     const int rdone = refine ? c->rs.done : 0;
@@ -405,7 +440,7 @@ __global__ __launch_bounds__(kBlock) void cg_init_kernel(Batch b, int refine, in
       done = 1;
       status = 2;
     }
-    CgScalars* s = &c->sc[0];
+    auto s = &c->sc[0];
     s->maxR = refine ? c->rs.b2 : bb;
     s->rhsNorm2 = bb;
     s->resNorm2 = bb;
@@ -445,12 +480,30 @@ template <typename T, int H>
 __global__ __launch_bounds__(kBlock) void cg_spmv_kernel(Batch b, int parity) {
   for (int vb = blockIdx.x; vb < b.total; vb += gridDim.x) {
     int lb, lg;
-    const InstDev* __restrict__ I = seg_find(b, vb, lb, lg);
-    Ctl* ctl = I->ctl;
-    const CgScalars* S = &ctl->sc[parity];
-    CgScalars* N = &ctl->sc[parity ^ 1];
+    const HOBAK_GLOBAL InstDev* __restrict__ I = seg_find(b, vb, lb, lg);
+    const int lane = threadIdx.x & 63;
+    const int slice = (int)((lb * (unsigned)kBlock + threadIdx.x) >> 6);
+    const int rr = lane % H;
+    const int r = slice * H + rr;
+    const int nrows = I->nv;
+    const int nslices = (nrows + H - 1) / H;
+    const bool wave_live = slice < nslices;
+    // loads that depend only on the record go out together: the slice extent, the update
+    // kernel's partials, and the scalar set
+    auto ctl = ctl_of(I);
+    const auto S = &ctl->sc[parity];
+    auto N = &ctl->sc[parity ^ 1];
     const bool lead = lb == 0 && threadIdx.x == 0;
     const int done = S->done;
+    int sbeg = 0, send = 0;
+    PreParts q;
+    if (!done) {
+      if (wave_live) {
+        sbeg = gbl(I->slice_off)[slice];
+        send = gbl(I->slice_off)[slice + 1];
+      }
+      pre_load(gbl(I->part_rr), gbl(I->part_rz), I->nb[K_UPD], q);
+    }
     const int first = S->first;
     const double absOld = S->absNew;
     const int iters = S->iters;
@@ -471,81 +524,10 @@ __global__ __launch_bounds__(kBlock) void cg_spmv_kernel(Batch b, int parity) {
       N->first = 0;
     }
     if (done) continue;
-    constexpr int L = 64 / H;
-    const int slice = (int)((lb * (unsigned)kBlock + threadIdx.x) >> 6);
-    const int lane = threadIdx.x & 63;
-    const int kk = lane / H;
-    const int rr = lane % H;
-    const int r = slice * H + rr;
-    const T* pold = (const T*)(parity ? I->p1 : I->p0);
-    T* pnew = (T*)(parity ? I->p0 : I->p1);
-    const T* z = (const T*)I->z;
-    const T* A = (const T*)I->A;
-    const int* col = I->col;
-    const size_t ns = (size_t)I->nslots;
-    const int nrows = I->nv;
-    // a0..a2 = (A z)_r, c0..c2 = (A p_old)_r; in the difference form (T = float) the column
-    // values are taken relative to the row's own (zo, qo) and dg adds the diagonal term
-    constexpr bool kDiff = sizeof(T) < sizeof(double);
-    T a0 = T(0), a1 = T(0), a2 = T(0), c0 = T(0), c1 = T(0), c2 = T(0);
-    T zo0 = T(0), zo1 = T(0), zo2 = T(0), qo0 = T(0), qo1 = T(0), qo2 = T(0);
-    const bool live = r < nrows;
-    if (live) {
-      const int len = I->row_len[r];
-      const int base = I->slice_off[slice] + rr;
-      if (kDiff) {
-        const size_t i = 3 * (size_t)r;
-        zo0 = z[i];
-        zo1 = z[i + 1];
-        zo2 = z[i + 2];
-        if (!first) {
-          qo0 = pold[i];
-          qo1 = pold[i + 1];
-          qo2 = pold[i + 2];
-        }
-      }
-      if (first) {
-        for (int k = kk; k < len; k += L) {
-          const int slot = base + k * H;
-          const size_t c = 3 * (size_t)col[slot];
-          const T z0 = z[c] - zo0, z1 = z[c + 1] - zo1, z2 = z[c + 2] - zo2;
-          a0 += A[0 * ns + slot] * z0 + A[1 * ns + slot] * z1 + A[2 * ns + slot] * z2;
-          a1 += A[3 * ns + slot] * z0 + A[4 * ns + slot] * z1 + A[5 * ns + slot] * z2;
-          a2 += A[6 * ns + slot] * z0 + A[7 * ns + slot] * z1 + A[8 * ns + slot] * z2;
-        }
-      } else {
-        for (int k = kk; k < len; k += L) {
-          const int slot = base + k * H;
-          const size_t c = 3 * (size_t)col[slot];
-          const T z0 = z[c] - zo0, z1 = z[c + 1] - zo1, z2 = z[c + 2] - zo2;
-          const T q0 = pold[c] - qo0, q1 = pold[c + 1] - qo1, q2 = pold[c + 2] - qo2;
-          const T m0 = A[0 * ns + slot], m1 = A[1 * ns + slot], m2 = A[2 * ns + slot];
-          const T m3 = A[3 * ns + slot], m4 = A[4 * ns + slot], m5 = A[5 * ns + slot];
-          const T m6 = A[6 * ns + slot], m7 = A[7 * ns + slot], m8 = A[8 * ns + slot];
-          a0 += m0 * z0 + m1 * z1 + m2 * z2;
-          a1 += m3 * z0 + m4 * z1 + m5 * z2;
-          a2 += m6 * z0 + m7 * z1 + m8 * z2;
-          c0 += m0 * q0 + m1 * q1 + m2 * q2;
-          c1 += m3 * q0 + m4 * q1 + m5 * q2;
-          c2 += m6 * q0 + m7 * q1 + m8 * q2;
-        }
-      }
-    }
-#pragma unroll
-    for (int off = H; off < 64; off <<= 1) {
-      a0 += __shfl_xor(a0, off, 64);
-      a1 += __shfl_xor(a1, off, 64);
-      a2 += __shfl_xor(a2, off, 64);
-      if (!first) {
-        c0 += __shfl_xor(c0, off, 64);
-        c1 += __shfl_xor(c1, off, 64);
-        c2 += __shfl_xor(c2, off, 64);
-      }
-    }
     T beta = T(0);
     if (!first) {
       double rrv, rz;
-      gather(I->part_rr, I->part_rz, I->nb[K_UPD], rrv, rz);
+      pre_sum(gbl(I->part_rr), gbl(I->part_rz), I->nb[K_UPD], q, rrv, rz);
       if (nonfinite(rrv) || !(rz == rz)) {
         if (lead) {
           N->resNorm2 = rrv;
@@ -575,49 +557,116 @@ __global__ __launch_bounds__(kBlock) void cg_spmv_kernel(Batch b, int parity) {
       if (iters + 1 >= maxIters) continue;
       beta = (T)bt;
     }
+    auto pold = gbl((const T*)(parity ? I->p1 : I->p0));
+    auto pnew = gbl((T*)(parity ? I->p0 : I->p1));
+    auto z = gbl((const T*)I->z);
+    auto A = gbl((const T*)I->A);
+    auto col = gbl(I->col);
+    const size_t ns = (size_t)I->nslots;
+    const bool live = r < nrows;
+    // p_new = z + beta p_old at every column (p_old is not read for the first iteration).  In
+    // the difference form (T = float) columns are taken relative to the row's own p (po) and dg
+    // adds the diagonal term.  The wave walks its slice's padded width: iteration j reads the 64
+    // consecutive slots slice_off + 64 j + lane (padding slots hold zero blocks and point at the
+    // row itself), so the trip count is uniform and several iterations' loads are in flight.
+    constexpr bool kDiff = sizeof(T) < sizeof(double);
+    T po0 = T(0), po1 = T(0), po2 = T(0);
+    if (live) {
+      const size_t i = 3 * (size_t)r;
+      po0 = z[i];
+      po1 = z[i + 1];
+      po2 = z[i + 2];
+      if (!first) {
+        po0 += beta * pold[i];
+        po1 += beta * pold[i + 1];
+        po2 += beta * pold[i + 2];
+      }
+    }
+    T s0 = T(0), s1 = T(0), s2 = T(0);
+    if (wave_live) {
+      const int nit = (send - sbeg) >> 6;
+      auto cp = col + sbeg + lane;
+      auto Ap = A + sbeg + lane;
+      const T d0 = kDiff ? po0 : T(0), d1 = kDiff ? po1 : T(0), d2 = kDiff ? po2 : T(0);
+      const T bt = first ? T(0) : beta;
+      auto pq = first ? z : pold;  // p_old is not read for the first iteration (bt = 0)
+      // two slice steps per pass: all of their loads are issued before any arithmetic
+      int j = 0;
+      for (; j + 1 < nit; j += 2) {
+        const int o0 = 64 * j, o1 = o0 + 64;
+        const size_t ca = 3 * (size_t)cp[o0];
+        const size_t cb = 3 * (size_t)cp[o1];
+        T ma[9], mb[9];
+#pragma unroll
+        for (int e = 0; e < 9; e++) {
+          ma[e] = Ap[e * ns + o0];
+          mb[e] = Ap[e * ns + o1];
+        }
+        const T za0 = z[ca], za1 = z[ca + 1], za2 = z[ca + 2];
+        const T zb0 = z[cb], zb1 = z[cb + 1], zb2 = z[cb + 2];
+        const T qa0 = pq[ca], qa1 = pq[ca + 1], qa2 = pq[ca + 2];
+        const T qb0 = pq[cb], qb1 = pq[cb + 1], qb2 = pq[cb + 2];
+        const T pa0 = (za0 + bt * qa0) - d0, pa1 = (za1 + bt * qa1) - d1, pa2 = (za2 + bt * qa2) - d2;
+        const T pb0 = (zb0 + bt * qb0) - d0, pb1 = (zb1 + bt * qb1) - d1, pb2 = (zb2 + bt * qb2) - d2;
+        s0 += ma[0] * pa0 + ma[1] * pa1 + ma[2] * pa2;
+        s1 += ma[3] * pa0 + ma[4] * pa1 + ma[5] * pa2;
+        s2 += ma[6] * pa0 + ma[7] * pa1 + ma[8] * pa2;
+        s0 += mb[0] * pb0 + mb[1] * pb1 + mb[2] * pb2;
+        s1 += mb[3] * pb0 + mb[4] * pb1 + mb[5] * pb2;
+        s2 += mb[6] * pb0 + mb[7] * pb1 + mb[8] * pb2;
+      }
+      if (j < nit) {
+        const int o0 = 64 * j;
+        const size_t ca = 3 * (size_t)cp[o0];
+        T ma[9];
+#pragma unroll
+        for (int e = 0; e < 9; e++) ma[e] = Ap[e * ns + o0];
+        const T pa0 = (z[ca] + bt * pq[ca]) - d0;
+        const T pa1 = (z[ca + 1] + bt * pq[ca + 1]) - d1;
+        const T pa2 = (z[ca + 2] + bt * pq[ca + 2]) - d2;
+        s0 += ma[0] * pa0 + ma[1] * pa1 + ma[2] * pa2;
+        s1 += ma[3] * pa0 + ma[4] * pa1 + ma[5] * pa2;
+        s2 += ma[6] * pa0 + ma[7] * pa1 + ma[8] * pa2;
+      }
+    }
+#pragma unroll
+    for (int off = H; off < 64; off <<= 1) {
+      s0 += __shfl_xor(s0, off, 64);
+      s1 += __shfl_xor(s1, off, 64);
+      s2 += __shfl_xor(s2, off, 64);
+    }
     double pt = 0.0;
-    if (live && kk == 0) {
+    if (live && lane < H) {
       const size_t i = 3 * (size_t)r;
       if (kDiff) {
-        const T d0 = (T)I->dg[i], d1 = (T)I->dg[i + 1], d2 = (T)I->dg[i + 2];
-        a0 += d0 * zo0;
-        a1 += d1 * zo1;
-        a2 += d2 * zo2;
-        c0 += d0 * qo0;
-        c1 += d1 * qo1;
-        c2 += d2 * qo2;
+        s0 += (T)gbl(I->dg)[i] * po0;
+        s1 += (T)gbl(I->dg)[i + 1] * po1;
+        s2 += (T)gbl(I->dg)[i + 2] * po2;
       }
-      const T s0 = a0 + beta * c0, s1 = a1 + beta * c1, s2 = a2 + beta * c2;
-      T* t = (T*)I->t;
+      auto t = gbl((T*)I->t);
       t[i] = s0;
       t[i + 1] = s1;
       t[i + 2] = s2;
-      T q0 = z[i], q1 = z[i + 1], q2 = z[i + 2];
-      if (!first) {
-        q0 += beta * pold[i];
-        q1 += beta * pold[i + 1];
-        q2 += beta * pold[i + 2];
-      }
-      pnew[i] = q0;
-      pnew[i + 1] = q1;
-      pnew[i + 2] = q2;
-      pt = (double)q0 * (double)s0 + (double)q1 * (double)s1 + (double)q2 * (double)s2;
+      pnew[i] = po0;
+      pnew[i + 1] = po1;
+      pnew[i + 2] = po2;
+      pt = (double)po0 * (double)s0 + (double)po1 * (double)s1 + (double)po2 * (double)s2;
     }
-    publish(pt, 0.0, I->part_pt, nullptr, lb);
+    publish(pt, 0.0, gbl(I->part_pt), nullptr, lb);
   }
 }
 
 // One lane per DOF, 21 rows (63 lanes) per wavefront so that a row's three DOFs share a wave and
 // the 3x3 preconditioner apply can read its siblings' residuals with cross-lane moves.  Each
-// instance's workgroups stride over its waves (at most kUpdateCap workgroups per instance), which
+// instance's workgroups stride over its waves (at most a tuned cap per instance), which
 // keeps the partial-sum count (and the work of every workgroup that re-reduces them) small.  The
 // first wave's vectors are loaded before the p.t partials are reduced, so the two overlap.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void cg_update_kernel(Batch b, int parity) {
   for (int vb = blockIdx.x; vb < b.total; vb += gridDim.x) {
     int lb, lg;
-    const InstDev* __restrict__ I = seg_find(b, vb, lb, lg);
-    const CgScalars* S = &I->ctl->sc[parity ^ 1];
+    const HOBAK_GLOBAL InstDev* __restrict__ I = seg_find(b, vb, lb, lg);
+    const auto S = &ctl_of(I)->sc[parity ^ 1];
     if (S->done) continue;
     const double absNew = S->absNew;
     const int lane = threadIdx.x & 63;
@@ -626,12 +675,12 @@ __global__ __launch_bounds__(kBlock) void cg_update_kernel(Batch b, int parity) 
     const int nrows = I->nv;
     const int nwaves = (nrows + 20) / 21;
     const int stride = lg * (kBlock / 64);
-    const T* p = (const T*)(parity ? I->p0 : I->p1);  // the p_new this iteration's SpMV wrote
-    const T* mi = (const T*)I->minv;
-    T* x = (T*)I->x;
-    T* rv_ = (T*)I->r;
-    T* z = (T*)I->z;
-    const T* t = (const T*)I->t;
+    auto p = gbl((const T*)(parity ? I->p0 : I->p1));  // the p_new this iteration's SpMV wrote
+    auto mi = gbl((const T*)I->minv);
+    auto x = gbl((T*)I->x);
+    auto rv_ = gbl((T*)I->r);
+    auto z = gbl((T*)I->z);
+    auto t = gbl((const T*)I->t);
     const int precond = I->precond;
     const int w0 = (int)((lb * (unsigned)kBlock + threadIdx.x) >> 6);
     T pv = T(0), tv = T(0), rv0 = T(0), xv = T(0);
@@ -645,8 +694,10 @@ __global__ __launch_bounds__(kBlock) void cg_update_kernel(Batch b, int parity) 
         xv = x[i];
       }
     }
+    PreParts qp;
+    pre_load(gbl(I->part_pt), nullptr, I->nb[K_SPMV], qp);
     double pt, unused;
-    gather(I->part_pt, nullptr, I->nb[K_SPMV], pt, unused);
+    pre_sum(gbl(I->part_pt), nullptr, I->nb[K_SPMV], qp, pt, unused);
     const T alpha = (T)(absNew / pt);
     double rr2 = 0.0, rz = 0.0;
     for (int wave = w0; wave < nwaves; wave += stride) {
@@ -673,7 +724,7 @@ __global__ __launch_bounds__(kBlock) void cg_update_kernel(Batch b, int parity) 
         if (precond == 0) {
           zv = mi[i] * rv;
         } else {
-          const T* m = mi + (size_t)kMinvStride * row + 3 * c;
+          auto m = mi + (size_t)kMinvStride * row + 3 * c;
           zv = m[0] * r0 + m[1] * r1 + m[2] * r2;
         }
         z[i] = zv;
@@ -681,7 +732,7 @@ __global__ __launch_bounds__(kBlock) void cg_update_kernel(Batch b, int parity) 
         rz += (double)rv * (double)zv;
       }
     }
-    publish(rr2, rz, I->part_rr, I->part_rz, lb);
+    publish(rr2, rz, gbl(I->part_rr), gbl(I->part_rz), lb);
   }
 }
 
@@ -697,21 +748,21 @@ __global__ __launch_bounds__(kBlock) void cg_update_kernel(Batch b, int parity) 
 __global__ __launch_bounds__(kBlock) void refine_begin_kernel(Batch b) {
   for (int vb = blockIdx.x; vb < b.total; vb += gridDim.x) {
     int lb, lg;
-    const InstDev* __restrict__ I = seg_find(b, vb, lb, lg);
+    const HOBAK_GLOBAL InstDev* __restrict__ I = seg_find(b, vb, lb, lg);
     const int r = lb * kBlock + threadIdx.x;
     if (r < I->nv) {
 #pragma unroll
       for (int c = 0; c < 3; c++) {
         const size_t i = 3 * (size_t)r + c;
-        I->x64[i] = 0.0;
-        I->res64[i] = I->rhs64[i];
+        gbl(I->x64)[i] = 0.0;
+        gbl(I->res64)[i] = gbl(I->rhs64)[i];
       }
     }
     if (lb != 0) continue;
     double b2, unused;
-    gather(I->part_row_a, nullptr, I->nb[K_ROW], b2, unused);
+    gather(gbl(I->part_row_a), nullptr, I->nb[K_ROW], b2, unused);
     if (threadIdx.x == 0) {
-      RefineState* st = &I->ctl->rs;
+      auto st = &ctl_of(I)->rs;
       const int bad = nonfinite(b2) ? 1 : 0;
       st->b2 = b2;
       st->res2 = b2;
@@ -728,26 +779,26 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void refine_init_kernel(Batch b, int gate) {
   for (int vb = blockIdx.x; vb < b.total; vb += gridDim.x) {
     int lb, lg;
-    const InstDev* __restrict__ I = seg_find(b, vb, lb, lg);
-    if (gate && !solve_ended(I->ctl)) continue;
+    const HOBAK_GLOBAL InstDev* __restrict__ I = seg_find(b, vb, lb, lg);
+    if (gate && !solve_ended(ctl_of(I))) continue;
     const int r = lb * kBlock + threadIdx.x;
     double rr = 0.0, rz = 0.0;
     if (r < I->nv) {
       T rv[3], zv[3];
 #pragma unroll
-      for (int c = 0; c < 3; c++) rv[c] = (T)I->res64[3 * (size_t)r + c];
-      apply_precond<T>((const T*)I->minv, I->precond, r, rv, zv);
+      for (int c = 0; c < 3; c++) rv[c] = (T)gbl(I->res64)[3 * (size_t)r + c];
+      apply_precond<T>(gbl((const T*)I->minv), I->precond, r, rv, zv);
 #pragma unroll
       for (int c = 0; c < 3; c++) {
         const size_t i = 3 * (size_t)r + c;
-        ((T*)I->r)[i] = rv[c];
-        ((T*)I->x)[i] = T(0);
-        ((T*)I->z)[i] = zv[c];
+        (gbl((T*)I->r))[i] = rv[c];
+        (gbl((T*)I->x))[i] = T(0);
+        (gbl((T*)I->z))[i] = zv[c];
         rr += (double)rv[c] * (double)rv[c];
         rz += (double)rv[c] * (double)zv[c];
       }
     }
-    publish(rr, rz, I->part_row_a, I->part_row_b, lb);
+    publish(rr, rz, gbl(I->part_row_a), gbl(I->part_row_b), lb);
   }
 }
 
@@ -756,8 +807,8 @@ template <typename T, int H>
 __global__ __launch_bounds__(kBlock) void refine_residual_kernel(Batch b) {
   for (int vb = blockIdx.x; vb < b.total; vb += gridDim.x) {
     int lb, lg;
-    const InstDev* __restrict__ I = seg_find(b, vb, lb, lg);
-    const Ctl* ctl = I->ctl;
+    const HOBAK_GLOBAL InstDev* __restrict__ I = seg_find(b, vb, lb, lg);
+    const auto ctl = ctl_of(I);
     if (!solve_ended(ctl) || ctl->rs.done) continue;
     constexpr int L = 64 / H;
     const int slice = (int)((lb * (unsigned)kBlock + threadIdx.x) >> 6);
@@ -765,33 +816,38 @@ __global__ __launch_bounds__(kBlock) void refine_residual_kernel(Batch b) {
     const int kk = lane / H;
     const int rr = lane % H;
     const int r = slice * H + rr;
-    const T* A = (const T*)I->A;
-    const T* xi = (const T*)I->x;
-    const double* x64 = I->x64;
-    const int* col = I->col;
+    auto A = gbl((const T*)I->A);
+    auto xi = gbl((const T*)I->x);
+    auto x64 = gbl(I->x64);
+    auto col = gbl(I->col);
     const size_t ns = (size_t)I->nslots;
     constexpr bool kDiff = sizeof(T) < sizeof(double);
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
     double o0 = 0.0, o1 = 0.0, o2 = 0.0;  // the row's own x64 + x (difference form)
     const bool live = r < I->nv;
-    if (live) {
-      const int len = I->row_len[r];
-      const int base = I->slice_off[slice] + rr;
-      if (kDiff) {
+    const int nslices = (I->nv + H - 1) / H;
+    if (slice < nslices) {
+      // the slice's padded width, as in the SpMV
+      const int sbeg = gbl(I->slice_off)[slice];
+      const int nit = (gbl(I->slice_off)[slice + 1] - sbeg) >> 6;
+      auto cp = col + sbeg + lane;
+      auto Ap = A + sbeg + lane;
+      if (kDiff && live) {
         const size_t i = 3 * (size_t)r;
         o0 = x64[i] + (double)xi[i];
         o1 = x64[i + 1] + (double)xi[i + 1];
         o2 = x64[i + 2] + (double)xi[i + 2];
       }
-      for (int k = kk; k < len; k += L) {
-        const int slot = base + k * H;
-        const size_t c = 3 * (size_t)col[slot];
+#pragma unroll 4
+      for (int j = 0; j < nit; j++) {
+        const int o = 64 * j;
+        const size_t c = 3 * (size_t)cp[o];
         const double x0 = (x64[c] + (double)xi[c]) - o0;
         const double x1 = (x64[c + 1] + (double)xi[c + 1]) - o1;
         const double x2 = (x64[c + 2] + (double)xi[c + 2]) - o2;
-        s0 += (double)A[0 * ns + slot] * x0 + (double)A[1 * ns + slot] * x1 + (double)A[2 * ns + slot] * x2;
-        s1 += (double)A[3 * ns + slot] * x0 + (double)A[4 * ns + slot] * x1 + (double)A[5 * ns + slot] * x2;
-        s2 += (double)A[6 * ns + slot] * x0 + (double)A[7 * ns + slot] * x1 + (double)A[8 * ns + slot] * x2;
+        s0 += (double)Ap[0 * ns + o] * x0 + (double)Ap[1 * ns + o] * x1 + (double)Ap[2 * ns + o] * x2;
+        s1 += (double)Ap[3 * ns + o] * x0 + (double)Ap[4 * ns + o] * x1 + (double)Ap[5 * ns + o] * x2;
+        s2 += (double)Ap[6 * ns + o] * x0 + (double)Ap[7 * ns + o] * x1 + (double)Ap[8 * ns + o] * x2;
       }
     }
 #pragma unroll
@@ -804,20 +860,20 @@ __global__ __launch_bounds__(kBlock) void refine_residual_kernel(Batch b) {
     if (live && kk == 0) {
       if (kDiff) {
         const size_t i = 3 * (size_t)r;
-        s0 += I->dg[i] * o0;
-        s1 += I->dg[i + 1] * o1;
-        s2 += I->dg[i + 2] * o2;
+        s0 += gbl(I->dg)[i] * o0;
+        s1 += gbl(I->dg)[i + 1] * o1;
+        s2 += gbl(I->dg)[i + 2] * o2;
       }
       const double s[3] = {s0, s1, s2};
 #pragma unroll
       for (int c = 0; c < 3; c++) {
         const size_t i = 3 * (size_t)r + c;
-        const double res = I->rhs64[i] - s[c];
-        I->res64[i] = res;
+        const double res = gbl(I->rhs64)[i] - s[c];
+        gbl(I->res64)[i] = res;
         res2 += res * res;
       }
     }
-    publish(res2, 0.0, I->part_res, nullptr, lb);
+    publish(res2, 0.0, gbl(I->part_res), nullptr, lb);
   }
 }
 
@@ -827,11 +883,11 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void refine_accumulate_kernel(Batch b) {
   for (int vb = blockIdx.x; vb < b.total; vb += gridDim.x) {
     int lb, lg;
-    const InstDev* __restrict__ I = seg_find(b, vb, lb, lg);
-    const Ctl* ctl = I->ctl;
+    const HOBAK_GLOBAL InstDev* __restrict__ I = seg_find(b, vb, lb, lg);
+    const auto ctl = ctl_of(I);
     if (!solve_ended(ctl) || ctl->rs.done) continue;
     const int i = lb * kBlock + threadIdx.x;
-    if (i < 3 * I->nv) I->x64[i] += (double)((const T*)I->x)[i];
+    if (i < 3 * I->nv) gbl(I->x64)[i] += (double)(gbl((const T*)I->x))[i];
   }
 }
 
@@ -842,8 +898,8 @@ __global__ __launch_bounds__(kBlock) void refine_accumulate_kernel(Batch b) {
 __global__ __launch_bounds__(kBlock) void finish_check_kernel(Batch b, int refine) {
   for (int vb = blockIdx.x; vb < b.total; vb += gridDim.x) {
     int lb, lg;
-    const InstDev* __restrict__ I = seg_find(b, vb, lb, lg);
-    Ctl* c = I->ctl;
+    const HOBAK_GLOBAL InstDev* __restrict__ I = seg_find(b, vb, lb, lg);
+    auto c = ctl_of(I);
     if (!solve_ended(c)) continue;
     if (!refine) {
       if (threadIdx.x == 0) c->phase = PHASE_FINISHED;
@@ -854,9 +910,9 @@ __global__ __launch_bounds__(kBlock) void finish_check_kernel(Batch b, int refin
       continue;
     }
     double res2, unused;
-    gather(I->part_res, nullptr, I->nb[K_SPMV], res2, unused);
+    gather(gbl(I->part_res), nullptr, I->nb[K_SPMV], res2, unused);
     if (threadIdx.x != 0) continue;
-    RefineState* st = &c->rs;
+    auto st = &c->rs;
     const double b2 = st->b2;
     const double tol = I->tol;
     const int sweeps = st->sweeps + 1;
@@ -886,10 +942,10 @@ __global__ __launch_bounds__(kBlock) void finish_check_kernel(Batch b, int refin
 __global__ __launch_bounds__(kBlock) void status_kernel(Batch b, InstStatus* out) {
   for (int vb = blockIdx.x; vb < b.total; vb += gridDim.x) {
     int lb, lg;
-    const InstDev* __restrict__ I = seg_find(b, vb, lb, lg);
+    const HOBAK_GLOBAL InstDev* __restrict__ I = seg_find(b, vb, lb, lg);
     if (threadIdx.x != 0) continue;
-    const Ctl* c = I->ctl;
-    const CgScalars* s = &c->sc[0];
+    const auto c = ctl_of(I);
+    const auto s = &c->sc[0];
     InstStatus o;
     o.phase = c->phase;
     o.iters = s->iters;
@@ -916,17 +972,17 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void force_gather_kernel(Batch b) {
   for (int vb = blockIdx.x; vb < b.total; vb += gridDim.x) {
     int lb, lg;
-    const InstDev* __restrict__ I = seg_find(b, vb, lb, lg);
+    const HOBAK_GLOBAL InstDev* __restrict__ I = seg_find(b, vb, lb, lg);
     const int r = lb * kBlock + threadIdx.x;
     if (r >= I->nv) continue;
-    const T* sf = (const T*)I->stage_f;
+    auto sf = gbl((const T*)I->stage_f);
     double s[3] = {0.0, 0.0, 0.0};
-    for (int q = I->vptr[r]; q < I->vptr[r + 1]; q++) {
+    for (int q = gbl(I->vptr)[r]; q < gbl(I->vptr)[r + 1]; q++) {
 #pragma unroll
       for (int c = 0; c < 3; c++) s[c] += (double)sf[3 * (size_t)q + c];
     }
 #pragma unroll
-    for (int c = 0; c < 3; c++) I->rhs64[3 * (size_t)r + c] = s[c];
+    for (int c = 0; c < 3; c++) gbl(I->rhs64)[3 * (size_t)r + c] = s[c];
   }
 }
 
@@ -941,7 +997,7 @@ __global__ void copy_kernel(const double4* __restrict__ src, double4* __restrict
 // ---------------------------------------------------------------------------------------------
 static inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 
-int blocks_of(int kind, int nv, int nt, int nslots, int shift) {
+int blocks_of(int kind, int nv, int nt, int nslots, int shift, int update_cap) {
   switch (kind) {
     case K_ELEM: return cdiv(nt, kElementBlock);
     case K_SLOT: return cdiv(nslots, kBlock);
@@ -950,7 +1006,7 @@ int blocks_of(int kind, int nv, int nt, int nslots, int shift) {
     case K_SPMV: return cdiv((long)cdiv(nv, 1 << shift) * 64, kBlock);
     case K_UPD: {
       const int g = cdiv((long)cdiv(nv, 21) * 64, kBlock);
-      return g < kUpdateCap ? g : kUpdateCap;
+      return g < update_cap ? g : update_cap;
     }
     case K_ONE: return 1;
     default: return 0;

@@ -114,6 +114,7 @@ SIGNATURES = {
     "hobak_sched_tick": (ctypes.c_int, [_vp, ctypes.POINTER(hobak_completion), ctypes.c_int32,
                                         _ip, _ip]),
     "hobak_sched_clock": (ctypes.c_double, [_vp]),
+    "hobak_sched_stats": (ctypes.c_int, [_vp, _dp]),
 }
 
 TUNE_SLICE_ROWS = 0
@@ -121,6 +122,7 @@ TUNE_POLL_CHUNK = 2
 TUNE_MAX_CHUNKS = 3
 TUNE_GRAPHS = 4
 TUNE_GRID_CAP = 5
+TUNE_UPDATE_WGS = 6
 
 _lib = None
 
@@ -195,7 +197,7 @@ class Context:
         if entry:
             self.set_tuning(slice_rows=entry.get("slice_rows"), poll_chunk=entry.get("poll_chunk"),
                             max_chunks=entry.get("max_chunks"), graphs=entry.get("graphs"),
-                            grid_cap=entry.get("grid_cap"))
+                            grid_cap=entry.get("grid_cap"), update_wgs=entry.get("update_wgs"))
             self.refine_inner_tol = entry.get("refine_inner_tol")
 
     def error(self) -> str:
@@ -207,11 +209,11 @@ class Context:
         return rc
 
     def set_tuning(self, slice_rows=None, poll_chunk=None, max_chunks=None, graphs=None,
-                   grid_cap=None):
+                   grid_cap=None, update_wgs=None):
         """Kernel and engine tuning (see hobak_ctx_set_tuning)."""
         for key, val in ((TUNE_SLICE_ROWS, slice_rows), (TUNE_POLL_CHUNK, poll_chunk),
                          (TUNE_MAX_CHUNKS, max_chunks), (TUNE_GRAPHS, graphs),
-                         (TUNE_GRID_CAP, grid_cap)):
+                         (TUNE_GRID_CAP, grid_cap), (TUNE_UPDATE_WGS, update_wgs)):
             if val is not None:
                 self.check(self.lib.hobak_ctx_set_tuning(self.h, key, int(val)), "set_tuning")
 
@@ -389,6 +391,13 @@ class Scheduler:
 
     def clock(self) -> float:
         return self.lib.hobak_sched_clock(self.h)
+
+    def stats(self):
+        out = np.zeros(7)
+        self.ctx.check(self.lib.hobak_sched_stats(self.h, _d(out)), "sched_stats")
+        keys = ["ticks", "pcg_groups", "prologue_gpu_ms", "pcg_gpu_ms", "epilogue_gpu_ms",
+                "tick_host_ms", "solves"]
+        return dict(zip(keys, out.tolist()))
 
     def tick(self):
         """Runs one tick; returns (completions, requests still in flight)."""

@@ -173,7 +173,9 @@ def run(ctx, stream, scene_of, budget=0, precision=lib.PRECISION_FP32_REFINE,
             else:
                 finish(k)
     makespan = clock.now()
+    engine = None
     if sch is not None:
+        engine = sch.stats()
         sch.close()
     return {
         "mode": mode,
@@ -190,6 +192,7 @@ def run(ctx, stream, scene_of, budget=0, precision=lib.PRECISION_FP32_REFINE,
         "nonzero_status": int(sum(1 for s in statuses if s != 0)),
         "max_concurrent_sessions": max_live,
         "admission_waits": admission_waits,
+        "engine": engine,
         "final": final,
     }
 

@@ -164,12 +164,16 @@ int hobak_kernel_stats(hobak_inst* inst, double* spmv_avg_ms, int64_t* spmv_coun
  *   HOBAK_TUNE_MAX_CHUNKS  most groups queued between two host polls
  *   HOBAK_TUNE_GRAPHS      1: replay captured graphs; 0: launch every kernel directly
  *   HOBAK_TUNE_GRID_CAP    most workgroups per launch (0: one per virtual workgroup); capped
- *                          launches stride over their virtual workgroups (persistent form) */
+ *                          launches stride over their virtual workgroups (persistent form)
+ *   HOBAK_TUNE_UPDATE_WGS  most workgroups of the PCG update kernel per instance (each is one
+ *                          partial sum the next SpMV re-reduces); applies to instances created
+ *                          afterwards */
 #define HOBAK_TUNE_SLICE_ROWS 0
 #define HOBAK_TUNE_POLL_CHUNK 2
 #define HOBAK_TUNE_MAX_CHUNKS 3
 #define HOBAK_TUNE_GRAPHS 4
 #define HOBAK_TUNE_GRID_CAP 5
+#define HOBAK_TUNE_UPDATE_WGS 6
 int hobak_ctx_set_tuning(hobak_ctx* ctx, int32_t key, int32_t value);
 
 /* Continuous-batching scheduler (no counterpart in the reference, whose callers step one scene
@@ -201,6 +205,9 @@ int hobak_sched_tick(hobak_sched* sched, hobak_completion* out, int32_t max_out,
                      int32_t* in_flight);
 /* The scheduler clock (seconds since hobak_sched_create). */
 double hobak_sched_clock(hobak_sched* sched);
+/* Counters since creation, 7 doubles: ticks, PCG groups queued, GPU milliseconds of prologues,
+ * of PCG groups and of epilogues, host milliseconds inside hobak_sched_tick, solves finished. */
+int hobak_sched_stats(hobak_sched* sched, double* out);
 
 /* Streaming copy of `bytes` device bytes (a calibration kernel for hardware byte counters);
  * returns the kernel time in ms. */
