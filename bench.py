@@ -2,7 +2,7 @@
 """Benchmark of the backward-Euler Newton step on one MI355X.
 
 A "step" is one timestep of the workload: one Newton iteration over the whole mesh (element
-kernel, assembly, right-hand side, PCG to tol 1e-8, state update).  The default workload is the
+kernel, assembly, right-hand side, PCG to tol 1e-8, state update).  The headline workload is the
 40^3 Kuhn-lattice cube (384,000 tets, 68,921 vertices), Stable Neo-Hookean E=1e5 nu=0.45 rho=1000,
 x=0 face pinned, seeded [0.7, 1.3] stretch applied as a velocity field, dt = 1/60, FP64,
 block-Jacobi PCG.  Inputs are resident in HBM before timing starts; the warmup steps are followed
@@ -17,6 +17,13 @@ Output: one JSON line (see the field list in main()).
              per launch from the committed rocprofv3 --pmc summary of this workload
   cpu_baseline  the literal reference (oracle/_ref, compiled from its sources) timed on a bounded
              sample of the same workload on this host's cores
+  fp32_refine   the same K steps in the precision form (FP32 element/assembly/PCG, FP64
+             refinement), its speed-up and its position error against the FP64 run
+  batch      256 independent instances (dims U{4..24}^3, SNH/ARAP, seeded E and nu) advanced
+             together by hobak_step_batch, FP64 and FP32+refinement, against stepping each alone
+  serving    a Poisson stream of sessions (tets log-uniform in [1e3, 4e5], penalty ground,
+             20 step requests each) served by the continuous-batching scheduler in the precision
+             form, against the plain FP64 path serving the same stream one step at a time
 """
 from __future__ import annotations
 
@@ -33,6 +40,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "tet-Newton-steps/s + p50/p99 step latency vs FP64 GPU path at 1e-6 error"
 HBM_PEAK_GBS = 8000.0
+GRAPHS = [1]  # --graphs, applied to every context the benchmark creates
 
 
 def parse():
@@ -49,6 +57,19 @@ def parse():
                     help="also run the calibration copy kernel (for counter scaling)")
     ap.add_argument("--precision-form", default="on", choices=["on", "off"],
                     help="also time the FP32 + FP64-refinement path on the same workload")
+    ap.add_argument("--batch", default="on", choices=["on", "off"],
+                    help="also time 256 instances stepped together (hobak_step_batch)")
+    ap.add_argument("--batch-instances", type=int, default=256)
+    ap.add_argument("--batch-steps", type=int, default=5)
+    ap.add_argument("--serving", default="on", choices=["on", "off"],
+                    help="also run the serving stream against the plain FP64 path")
+    ap.add_argument("--sessions", type=int, default=48)
+    ap.add_argument("--rate", type=float, default=400.0, help="session arrivals per second")
+    ap.add_argument("--session-steps", type=int, default=20)
+    ap.add_argument("--budget-gib", type=float, default=8.0)
+    ap.add_argument("--graphs", default="on", choices=["on", "off"],
+                    help="off: launch every kernel directly (for profilers that do not trace "
+                         "graph launches)")
     return ap.parse_args()
 
 
@@ -75,6 +96,19 @@ def cpu_baseline(scene, steps, kind):
     import oracle  # test/baseline infrastructure only
 
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    # the reference prints progress to stdout; keep stdout for the one JSON line
+    sys.stdout.flush()
+    saved = os.dup(1)
+    os.dup2(2, 1)
+    try:
+        return _cpu_baseline(oracle, scene, steps, kind, cores)
+    finally:
+        sys.stdout.flush()
+        os.dup2(saved, 1)
+        os.close(saved)
+
+
+def _cpu_baseline(oracle, scene, steps, kind, cores):
     if kind in ("auto", "reference") and oracle.ref_available():
         sim = oracle.RefSim(scene)
         k = "reference"
@@ -106,6 +140,7 @@ def precision_form(lib, sc, precond, args, fp64_total, u64):
     """The same K steps with FP32 element/assembly/PCG and FP64 refinement; compared with the
     FP64 GPU path's positions after the same steps."""
     ctx = lib.Context(0, precision="fp32")
+    ctx.set_tuning(graphs=GRAPHS[0])
     inst = lib.Instance(ctx, sc, precond=precond, precision=lib.PRECISION_FP32_REFINE)
     for _ in range(args.warmup):
         inst.step()
@@ -137,18 +172,110 @@ def precision_form(lib, sc, precond, args, fp64_total, u64):
     }
 
 
+def batch_form(lib, scenes, args):
+    """hobak_step_batch over many independent instances against stepping each one alone."""
+    scs = scenes.batch_instances(args.batch_instances)
+    nt = sum(sc.nt for sc in scs)
+    out = {"instances": len(scs), "tets": nt, "steps": args.batch_steps,
+           "workload": f"{len(scs)} instances, dims U{{4..24}}^3 Kuhn lattices, SNH/ARAP 50/50, "
+                       f"E log-U[1e4,1e6], nu U[0.3,0.48], seeded stretch velocity, block-Jacobi"}
+    for prec, name in ((lib.PRECISION_FP64, "fp64"), (lib.PRECISION_FP32_REFINE, "fp32_refine")):
+        ctx = lib.Context(0, precision="fp64" if prec == lib.PRECISION_FP64 else "fp32")
+        ctx.set_tuning(graphs=GRAPHS[0])
+        insts = [lib.Instance(ctx, sc, precision=prec) for sc in scs]
+
+        def reset():
+            for inst, sc in zip(insts, scs):
+                inst.set_state(sc.initial_u(), sc.initial_v())
+
+        lib.step_batch(ctx, insts, allow_unconverged=True)
+        reset()
+        step_s = []
+        t0 = time.perf_counter()
+        for _ in range(args.batch_steps):
+            t1 = time.perf_counter()
+            lib.step_batch(ctx, insts, allow_unconverged=True)
+            step_s.append(time.perf_counter() - t1)
+        tb = time.perf_counter() - t0
+        ub = [inst.state()[0] for inst in insts]
+        reset()
+        t0 = time.perf_counter()
+        for _ in range(args.batch_steps):
+            for inst in insts:
+                inst.step(allow_unconverged=True)
+        ta = time.perf_counter() - t0
+        same = all(np.array_equal(inst.state()[0], u) for inst, u in zip(insts, ub))
+        res = {"value": nt * args.batch_steps / tb, "unit": "tet-Newton-steps/s",
+               "ms_per_batch_step": 1e3 * tb / args.batch_steps,
+               "p50_batch_step_ms": 1e3 * float(np.percentile(step_s, 50)),
+               "p99_batch_step_ms": 1e3 * float(np.percentile(step_s, 99)),
+               "alone_value": nt * args.batch_steps / ta,
+               "speedup_vs_alone": ta / tb, "bitwise_equal_to_alone": same}
+        if prec == lib.PRECISION_FP64:
+            for inst in insts:
+                inst.set_kernel_sampling(2)
+            reset()
+            lib.step_batch(ctx, insts, allow_unconverged=True)
+            ks = insts[0].kernel_stats()
+            if ks["spmv_avg_ms"] > 0:
+                gbs = ks["spmv_bytes"] / (ks["spmv_avg_ms"] * 1e-3) / 1e9
+                res["roofline"] = {"bound": "hbm", "kernel": "cg_spmv_kernel<double>",
+                                   "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                   "frac": round(gbs / HBM_PEAK_GBS, 4),
+                                   "bytes_per_launch": ks["spmv_bytes"],
+                                   "avg_launch_ms": ks["spmv_avg_ms"],
+                                   "launches_sampled": ks["spmv_count"],
+                                   "note": "launches during which every instance was solving"}
+        out[name] = res
+        for inst in insts:
+            inst.close()
+        ctx.close()
+    return out
+
+
+def serving_form(lib, scenes, args):
+    from hobakv1_amd import serve
+
+    stream = serve.request_stream(args.sessions, args.rate, args.session_steps, seed=5)
+    cache = {}
+
+    def scene_of(k):
+        if k not in cache:
+            cache[k] = scenes.serving_instance(k, seed=5)
+        return cache[k]
+
+    for s_ in stream:
+        scene_of(s_.k)
+    ca = lib.Context(0, precision="fp32")
+    cb = lib.Context(0)
+    ca.set_tuning(graphs=GRAPHS[0])
+    cb.set_tuning(graphs=GRAPHS[0])
+    out = serve.compare(ca, cb, stream, scene_of, int(args.budget_gib * (1 << 30)))
+    ca.close()
+    cb.close()
+    out["rate_sessions_per_s"] = args.rate
+    out["mean_tets"] = float(np.mean([scene_of(s_.k).nt for s_ in stream]))
+    out["workload"] = (f"{args.sessions} sessions, Poisson arrivals {args.rate:g}/s, "
+                       f"{args.session_steps} step requests each, tets log-U[1e3,4e5], aspect "
+                       f"[0.5,2], SNH/ARAP, penalty ground y=-0.5 k=1e6, HBM budget "
+                       f"{args.budget_gib:g} GiB; instance set-up excluded from both clocks")
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              "(the path shards by instance, see DESIGN.md)", file=sys.stderr)
+              "(the path shards by instance: run one process per GPU, each with its own instances)", file=sys.stderr)
         return 2
     from hobakv1_amd import lib, scenes
 
     sc = scenes.lattice_scene((args.n, args.n, args.n), seed=2, stretch_velocity=True,
                               name=f"cube{args.n}_snh_stretch")
     precond = scenes.PRECOND_BLOCK_JACOBI if args.precond == "block" else scenes.PRECOND_JACOBI
+    GRAPHS[0] = 1 if args.graphs == "on" else 0
     ctx = lib.Context(0)
+    ctx.set_tuning(graphs=GRAPHS[0])
     t0 = time.time()
     inst = lib.Instance(ctx, sc, precond=precond)
     setup_s = time.time() - t0
@@ -219,6 +346,10 @@ def main():
     }
     if args.precision_form == "on":
         out["fp32_refine"] = precision_form(lib, sc, precond, args, total, u)
+    if args.batch == "on":
+        out["batch"] = batch_form(lib, scenes, args)
+    if args.serving == "on":
+        out["serving"] = serving_form(lib, scenes, args)
     if args.copy_probe:
         nb = 512 << 20
         ms = ctx.copy_probe(nb, 20)
