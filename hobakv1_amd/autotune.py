@@ -62,6 +62,7 @@ def main():
     ap.add_argument("--slice-rows", default="8,16,32,64")
     ap.add_argument("--poll", default="8,32")
     ap.add_argument("--update-wgs", default="512")
+    ap.add_argument("--spmv-pad", default="0,1")
     ap.add_argument("--inner-tol", default="1e-4", help="FP32 inner PCG tolerances to sweep")
     ap.add_argument("--write", action="store_true")
     args = ap.parse_args()
@@ -72,15 +73,16 @@ def main():
     sc = scenes.lattice_scene((args.n,) * 3, seed=2, stretch_velocity=True)
     results = []
     tols = [float(x) for x in args.inner_tol.split(",")] if args.precision == "fp32" else [0.0]
-    for H, poll, uw, tol in itertools.product([int(x) for x in args.slice_rows.split(",")],
-                                              [int(x) for x in args.poll.split(",")],
-                                              [int(x) for x in args.update_wgs.split(",")], tols):
+    for H, poll, uw, pad, tol in itertools.product([int(x) for x in args.slice_rows.split(",")],
+                                                   [int(x) for x in args.poll.split(",")],
+                                                   [int(x) for x in args.update_wgs.split(",")],
+                                                   [int(x) for x in args.spmv_pad.split(",")], tols):
         ctx = lib.Context(0, apply_table=False)
-        ctx.set_tuning(slice_rows=H, poll_chunk=poll, update_wgs=uw)
+        ctx.set_tuning(slice_rows=H, poll_chunk=poll, update_wgs=uw, spmv_pad=pad)
         r = measure(ctx, sc, scenes.PRECOND_BLOCK_JACOBI, precision, args.steps, args.warmup,
                     inner_tol=tol or 1e-4)
         ctx.close()
-        r.update({"slice_rows": H, "poll_chunk": poll, "update_wgs": uw})
+        r.update({"slice_rows": H, "poll_chunk": poll, "update_wgs": uw, "spmv_pad": pad})
         if args.precision == "fp32":
             r["refine_inner_tol"] = tol
         results.append(r)
@@ -90,7 +92,7 @@ def main():
     if args.write:
         table = load_table()
         table[args.precision] = {"slice_rows": best["slice_rows"], "poll_chunk": best["poll_chunk"],
-                                 "update_wgs": best["update_wgs"],
+                                 "update_wgs": best["update_wgs"], "spmv_pad": best["spmv_pad"],
                                  "measured": {"workload": f"{args.n}^3 lattice", **best}}
         if "refine_inner_tol" in best:
             table[args.precision]["refine_inner_tol"] = best["refine_inner_tol"]

@@ -56,6 +56,7 @@ struct hobak_ctx {
   int use_graphs = 1;
   int grid_cap = 0;                    // max workgroups per launch (0: one per virtual workgroup)
   int update_cap = kUpdateCap;         // per-instance workgroups of the PCG update kernel
+  int spmv_pad = 0;                    // SpMV loop over padded slice widths (1) or row lengths (0)
   hipStream_t stream = nullptr;
   size_t budget = 0;
   size_t in_use = 0;
@@ -555,7 +556,7 @@ int launch_chunk(hobak_ctx* ctx, const Plan& P, int k, hobak_sched* S, int every
         S->samples.emplace_back(g, slot);
       }
     }
-    HIPCHK(ctx, launch_cg_spmv<T>(P.b[K_SPMV], P.shift, j & 1, cap, s, a, b));
+    HIPCHK(ctx, launch_cg_spmv<T>(P.b[K_SPMV], P.shift, ctx->spmv_pad, j & 1, cap, s, a, b));
     HIPCHK(ctx, launch_cg_update<T>(P.b[K_UPD], j & 1, cap, s));
   }
   if (P.refine) {
@@ -1461,6 +1462,10 @@ int hobak_ctx_set_tuning(hobak_ctx* ctx, int32_t key, int32_t value) {
     case HOBAK_TUNE_GRID_CAP:
       if (value < 0) return fail(ctx, HOBAK_E_INVALID, "grid cap must be >= 0");
       ctx->grid_cap = value;
+      return HOBAK_OK;
+    case HOBAK_TUNE_SPMV_PAD:
+      if (value != 0 && value != 1) return fail(ctx, HOBAK_E_INVALID, "spmv pad is 0 or 1");
+      ctx->spmv_pad = value;
       return HOBAK_OK;
     case HOBAK_TUNE_UPDATE_WGS:
       if (value < 1 || value > 65536) return fail(ctx, HOBAK_E_INVALID, "update workgroups out of range");

@@ -212,8 +212,9 @@ hipError_t launch_constant_parts(const Batch& b, int cap, hipStream_t s);
 template <typename T>
 hipError_t launch_rhs(const Batch& b, int init_cg, int cap, hipStream_t s);
 hipError_t launch_cg_init(const Batch& b, int refine, int gate, hipStream_t s);
+// pad: 1 walks every slice's padded width (uniform trip count), 0 only each row's own blocks
 template <typename T>
-hipError_t launch_cg_spmv(const Batch& b, int shift, int parity, int cap, hipStream_t s,
+hipError_t launch_cg_spmv(const Batch& b, int shift, int pad, int parity, int cap, hipStream_t s,
                           hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
 template <typename T>
 hipError_t launch_cg_update(const Batch& b, int parity, int cap, hipStream_t s);

@@ -476,7 +476,7 @@ __global__ __launch_bounds__(kBlock) void cg_init_kernel(Batch b, int refine, in
 // ---------------------------------------------------------------------------------------------
 // One wavefront per H-row slice, L = 64 / H lanes per row: lane = H * kk + rr handles the row's
 // blocks k = kk, kk + L, ...; a row's L partial sums are combined with cross-lane adds.
-template <typename T, int H>
+template <typename T, int H, bool PAD>
 __global__ __launch_bounds__(kBlock) void cg_spmv_kernel(Batch b, int parity) {
   for (int vb = blockIdx.x; vb < b.total; vb += gridDim.x) {
     int lb, lg;
@@ -495,12 +495,13 @@ __global__ __launch_bounds__(kBlock) void cg_spmv_kernel(Batch b, int parity) {
     auto N = &ctl->sc[parity ^ 1];
     const bool lead = lb == 0 && threadIdx.x == 0;
     const int done = S->done;
-    int sbeg = 0, send = 0;
+    int sbeg = 0, send = 0, len = 0;
     PreParts q;
     if (!done) {
       if (wave_live) {
         sbeg = gbl(I->slice_off)[slice];
         send = gbl(I->slice_off)[slice + 1];
+        if (!PAD && r < nrows) len = gbl(I->row_len)[r];
       }
       pre_load(gbl(I->part_rr), gbl(I->part_rz), I->nb[K_UPD], q);
     }
@@ -585,48 +586,43 @@ __global__ __launch_bounds__(kBlock) void cg_spmv_kernel(Batch b, int parity) {
     T s0 = T(0), s1 = T(0), s2 = T(0);
     if (wave_live) {
       const int nit = (send - sbeg) >> 6;
+      // per-lane step count: the padded width (PAD), or just the lane's own blocks of its row
+      const int nj = PAD ? nit : (live ? (len - lane / H + (64 / H) - 1) / (64 / H) : 0);
       auto cp = col + sbeg + lane;
       auto Ap = A + sbeg + lane;
       const T d0 = kDiff ? po0 : T(0), d1 = kDiff ? po1 : T(0), d2 = kDiff ? po2 : T(0);
       const T bt = first ? T(0) : beta;
       auto pq = first ? z : pold;  // p_old is not read for the first iteration (bt = 0)
       // two slice steps per pass: all of their loads are issued before any arithmetic
-      int j = 0;
-      for (; j + 1 < nit; j += 2) {
-        const int o0 = 64 * j, o1 = o0 + 64;
-        const size_t ca = 3 * (size_t)cp[o0];
-        const size_t cb = 3 * (size_t)cp[o1];
+      for (int j = 0; j < nit; j += 2) {
         T ma[9], mb[9];
+        T pa0 = T(0), pa1 = T(0), pa2 = T(0), pb0 = T(0), pb1 = T(0), pb2 = T(0);
 #pragma unroll
-        for (int e = 0; e < 9; e++) {
-          ma[e] = Ap[e * ns + o0];
-          mb[e] = Ap[e * ns + o1];
+        for (int e = 0; e < 9; e++) ma[e] = mb[e] = T(0);
+        if (j < nj) {
+          const int o0 = 64 * j;
+          const size_t ca = 3 * (size_t)cp[o0];
+#pragma unroll
+          for (int e = 0; e < 9; e++) ma[e] = Ap[e * ns + o0];
+          pa0 = (z[ca] + bt * pq[ca]) - d0;
+          pa1 = (z[ca + 1] + bt * pq[ca + 1]) - d1;
+          pa2 = (z[ca + 2] + bt * pq[ca + 2]) - d2;
         }
-        const T za0 = z[ca], za1 = z[ca + 1], za2 = z[ca + 2];
-        const T zb0 = z[cb], zb1 = z[cb + 1], zb2 = z[cb + 2];
-        const T qa0 = pq[ca], qa1 = pq[ca + 1], qa2 = pq[ca + 2];
-        const T qb0 = pq[cb], qb1 = pq[cb + 1], qb2 = pq[cb + 2];
-        const T pa0 = (za0 + bt * qa0) - d0, pa1 = (za1 + bt * qa1) - d1, pa2 = (za2 + bt * qa2) - d2;
-        const T pb0 = (zb0 + bt * qb0) - d0, pb1 = (zb1 + bt * qb1) - d1, pb2 = (zb2 + bt * qb2) - d2;
+        if (j + 1 < nj) {
+          const int o1 = 64 * (j + 1);
+          const size_t cb = 3 * (size_t)cp[o1];
+#pragma unroll
+          for (int e = 0; e < 9; e++) mb[e] = Ap[e * ns + o1];
+          pb0 = (z[cb] + bt * pq[cb]) - d0;
+          pb1 = (z[cb + 1] + bt * pq[cb + 1]) - d1;
+          pb2 = (z[cb + 2] + bt * pq[cb + 2]) - d2;
+        }
         s0 += ma[0] * pa0 + ma[1] * pa1 + ma[2] * pa2;
         s1 += ma[3] * pa0 + ma[4] * pa1 + ma[5] * pa2;
         s2 += ma[6] * pa0 + ma[7] * pa1 + ma[8] * pa2;
         s0 += mb[0] * pb0 + mb[1] * pb1 + mb[2] * pb2;
         s1 += mb[3] * pb0 + mb[4] * pb1 + mb[5] * pb2;
         s2 += mb[6] * pb0 + mb[7] * pb1 + mb[8] * pb2;
-      }
-      if (j < nit) {
-        const int o0 = 64 * j;
-        const size_t ca = 3 * (size_t)cp[o0];
-        T ma[9];
-#pragma unroll
-        for (int e = 0; e < 9; e++) ma[e] = Ap[e * ns + o0];
-        const T pa0 = (z[ca] + bt * pq[ca]) - d0;
-        const T pa1 = (z[ca + 1] + bt * pq[ca + 1]) - d1;
-        const T pa2 = (z[ca + 2] + bt * pq[ca + 2]) - d2;
-        s0 += ma[0] * pa0 + ma[1] * pa1 + ma[2] * pa2;
-        s1 += ma[3] * pa0 + ma[4] * pa1 + ma[5] * pa2;
-        s2 += ma[6] * pa0 + ma[7] * pa1 + ma[8] * pa2;
       }
     }
 #pragma unroll
@@ -1046,25 +1042,31 @@ hipError_t launch_cg_init(const Batch& b, int refine, int gate, hipStream_t s) {
 // With events, the launch carries them in its own dispatch (kernel start and end), so the
 // measured interval is the kernel's execution, not the queue's.  All instances of a batch share
 // one slice height (1 << shift rows).
-template <typename T, int H>
-static hipError_t spmv_h(const Batch& b, int parity, int cap, hipStream_t s, hipEvent_t start, hipEvent_t stop) {
+template <typename T, int H, bool PAD>
+static hipError_t spmv_hp(const Batch& b, int parity, int cap, hipStream_t s, hipEvent_t start, hipEvent_t stop) {
   const dim3 g(grid_of(b, cap)), blk(kBlock);
   if (start)
-    hipExtLaunchKernelGGL((cg_spmv_kernel<T, H>), g, blk, 0, s, start, stop, 0, b, parity);
+    hipExtLaunchKernelGGL((cg_spmv_kernel<T, H, PAD>), g, blk, 0, s, start, stop, 0, b, parity);
   else
-    hipLaunchKernelGGL((cg_spmv_kernel<T, H>), g, blk, 0, s, b, parity);
+    hipLaunchKernelGGL((cg_spmv_kernel<T, H, PAD>), g, blk, 0, s, b, parity);
   return hipGetLastError();
+}
+template <typename T, int H>
+static hipError_t spmv_h(const Batch& b, int parity, int cap, int pad, hipStream_t s, hipEvent_t start,
+                         hipEvent_t stop) {
+  return pad ? spmv_hp<T, H, true>(b, parity, cap, s, start, stop)
+             : spmv_hp<T, H, false>(b, parity, cap, s, start, stop);
 }
 
 template <typename T>
-hipError_t launch_cg_spmv(const Batch& b, int shift, int parity, int cap, hipStream_t s, hipEvent_t start,
-                          hipEvent_t stop) {
+hipError_t launch_cg_spmv(const Batch& b, int shift, int pad, int parity, int cap, hipStream_t s,
+                          hipEvent_t start, hipEvent_t stop) {
   if (b.total <= 0) return hipSuccess;
   switch (shift) {
-    case 3: return spmv_h<T, 8>(b, parity, cap, s, start, stop);
-    case 4: return spmv_h<T, 16>(b, parity, cap, s, start, stop);
-    case 5: return spmv_h<T, 32>(b, parity, cap, s, start, stop);
-    case 6: return spmv_h<T, 64>(b, parity, cap, s, start, stop);
+    case 3: return spmv_h<T, 8>(b, parity, cap, pad, s, start, stop);
+    case 4: return spmv_h<T, 16>(b, parity, cap, pad, s, start, stop);
+    case 5: return spmv_h<T, 32>(b, parity, cap, pad, s, start, stop);
+    case 6: return spmv_h<T, 64>(b, parity, cap, pad, s, start, stop);
     default: return hipErrorInvalidValue;
   }
 }
@@ -1113,7 +1115,7 @@ hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s) 
   template hipError_t launch_assemble<T>(const Batch&, int, int, hipStream_t);               \
   template hipError_t launch_constant_parts<T>(const Batch&, int, hipStream_t);              \
   template hipError_t launch_rhs<T>(const Batch&, int, int, hipStream_t);                    \
-  template hipError_t launch_cg_spmv<T>(const Batch&, int, int, int, hipStream_t, hipEvent_t, hipEvent_t); \
+  template hipError_t launch_cg_spmv<T>(const Batch&, int, int, int, int, hipStream_t, hipEvent_t, hipEvent_t); \
   template hipError_t launch_cg_update<T>(const Batch&, int, int, hipStream_t);              \
   template hipError_t launch_refine_init<T>(const Batch&, int, int, hipStream_t);            \
   template hipError_t launch_refine_residual<T>(const Batch&, const Batch&, int, int, hipStream_t); \

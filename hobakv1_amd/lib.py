@@ -123,6 +123,7 @@ TUNE_MAX_CHUNKS = 3
 TUNE_GRAPHS = 4
 TUNE_GRID_CAP = 5
 TUNE_UPDATE_WGS = 6
+TUNE_SPMV_PAD = 7
 
 _lib = None
 
@@ -197,7 +198,8 @@ class Context:
         if entry:
             self.set_tuning(slice_rows=entry.get("slice_rows"), poll_chunk=entry.get("poll_chunk"),
                             max_chunks=entry.get("max_chunks"), graphs=entry.get("graphs"),
-                            grid_cap=entry.get("grid_cap"), update_wgs=entry.get("update_wgs"))
+                            grid_cap=entry.get("grid_cap"), update_wgs=entry.get("update_wgs"),
+                            spmv_pad=entry.get("spmv_pad"))
             self.refine_inner_tol = entry.get("refine_inner_tol")
 
     def error(self) -> str:
@@ -209,11 +211,12 @@ class Context:
         return rc
 
     def set_tuning(self, slice_rows=None, poll_chunk=None, max_chunks=None, graphs=None,
-                   grid_cap=None, update_wgs=None):
+                   grid_cap=None, update_wgs=None, spmv_pad=None):
         """Kernel and engine tuning (see hobak_ctx_set_tuning)."""
         for key, val in ((TUNE_SLICE_ROWS, slice_rows), (TUNE_POLL_CHUNK, poll_chunk),
                          (TUNE_MAX_CHUNKS, max_chunks), (TUNE_GRAPHS, graphs),
-                         (TUNE_GRID_CAP, grid_cap), (TUNE_UPDATE_WGS, update_wgs)):
+                         (TUNE_GRID_CAP, grid_cap), (TUNE_UPDATE_WGS, update_wgs),
+                         (TUNE_SPMV_PAD, spmv_pad)):
             if val is not None:
                 self.check(self.lib.hobak_ctx_set_tuning(self.h, key, int(val)), "set_tuning")
 

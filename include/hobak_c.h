@@ -167,13 +167,16 @@ int hobak_kernel_stats(hobak_inst* inst, double* spmv_avg_ms, int64_t* spmv_coun
  *                          launches stride over their virtual workgroups (persistent form)
  *   HOBAK_TUNE_UPDATE_WGS  most workgroups of the PCG update kernel per instance (each is one
  *                          partial sum the next SpMV re-reduces); applies to instances created
- *                          afterwards */
+ *                          afterwards
+ *   HOBAK_TUNE_SPMV_PAD    1: the SpMV walks every slice's padded width with a uniform trip
+ *                          count; 0: each lane walks only its own row's blocks */
 #define HOBAK_TUNE_SLICE_ROWS 0
 #define HOBAK_TUNE_POLL_CHUNK 2
 #define HOBAK_TUNE_MAX_CHUNKS 3
 #define HOBAK_TUNE_GRAPHS 4
 #define HOBAK_TUNE_GRID_CAP 5
 #define HOBAK_TUNE_UPDATE_WGS 6
+#define HOBAK_TUNE_SPMV_PAD 7
 int hobak_ctx_set_tuning(hobak_ctx* ctx, int32_t key, int32_t value);
 
 /* Continuous-batching scheduler (no counterpart in the reference, whose callers step one scene
