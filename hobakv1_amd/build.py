@@ -46,9 +46,23 @@ def build_library(verbose: bool = False) -> str:
     return LIB
 
 
+DRIVER = os.path.join(HERE, "host", "facade_driver")
+
+
+def build_facade_driver() -> str:
+    """The C++ façade's test driver (host-only code linked against libhobak_hip.so)."""
+    cxx = os.environ.get("CXX", "g++")
+    _run([cxx, "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"),
+          os.path.join(HERE, "host", "facade_driver.cpp"), "-o", DRIVER + ".tmp",
+          "-L", HERE, "-lhobak_hip", "-Wl,-rpath,$ORIGIN/.."])
+    os.replace(DRIVER + ".tmp", DRIVER)
+    return DRIVER
+
+
 def main():
     build_library()
-    print("built", LIB)
+    build_facade_driver()
+    print("built", LIB, DRIVER)
 
 
 if __name__ == "__main__":

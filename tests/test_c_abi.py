@@ -4,6 +4,8 @@ import ctypes
 import os
 import re
 
+import pytest
+
 from hobakv1_amd import lib, scenes
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -48,3 +50,16 @@ def test_struct_layout_matches_header():
     # sizes are pinned so a field added on one side only is caught
     assert ctypes.sizeof(lib.hobak_material) == 24
     assert ctypes.sizeof(lib.hobak_step_stats) == 4 + 4 + 32 + 32 + 4 + 4 + 64 + 64 + 8
+
+
+def test_cpp_facade_compiles_against_the_header(tmp_path):
+    """The C++ façade (hobakv1_amd/host/hobak.hpp) and its driver compile as host C++17."""
+    import shutil
+    import subprocess
+
+    cxx = shutil.which("g++")
+    if cxx is None:
+        pytest.skip("no host C++ compiler")
+    src = os.path.join(ROOT, "hobakv1_amd", "host", "facade_driver.cpp")
+    subprocess.run([cxx, "-std=c++17", "-Wall", "-Werror", "-fsyntax-only", "-I",
+                    os.path.join(ROOT, "include"), src], check=True)
