@@ -57,6 +57,8 @@ def load_ref():
                                    ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                    ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, _dp,
                                    ctypes.c_double, ctypes.c_int]
+        lib.ref_create_bdf1.restype = ctypes.c_void_p
+        lib.ref_create_bdf1.argtypes = lib.ref_create.argtypes + [ctypes.c_int, ctypes.c_double]
         lib.ref_destroy.argtypes = [ctypes.c_void_p]
         lib.ref_num_pinned.argtypes = [ctypes.c_void_p]
         lib.ref_pinned_nodes.argtypes = [ctypes.c_void_p, _ip]
@@ -119,9 +121,11 @@ class RefElement:
 
 
 class RefSim:
-    """One reference scene: TET_MESH_FASTER + SNH/ARAP + BACKWARD_EULER_POSITION."""
+    """One reference scene: TET_MESH_FASTER + SNH/ARAP + BACKWARD_EULER_POSITION, or BDF_1
+    (src/Timestepper/BDF_1.cpp:78-239) when newton_iters > 1."""
 
-    def __init__(self, scene, pcg_tol=None, pcg_max_iters=None):
+    def __init__(self, scene, pcg_tol=None, pcg_max_iters=None, newton_iters=None,
+                 newton_tol=1e-7):
         self.lib = load_ref()
         self.scene = scene
         self._rest = _c64(scene.rest.ravel())
@@ -130,10 +134,14 @@ class RefSim:
         pin = None if scene.pin_cube is None else _c64(np.asarray(scene.pin_cube))
         tol = scene.pcg_tol if pcg_tol is None else pcg_tol
         mi = scene.pcg_max_iters if pcg_max_iters is None else pcg_max_iters
-        self.h = self.lib.ref_create(_d(self._rest), scene.nv, _i(self._tets), scene.nt,
-                                     scene.material, scene.mu, scene.lam, scene.dt,
-                                     scene.rayleigh_alpha, scene.rayleigh_beta, scene.mass_scale,
-                                     _d(g), None if pin is None else _d(pin), tol, mi)
+        ni = scene.newton_iters if newton_iters is None else newton_iters
+        args = (_d(self._rest), scene.nv, _i(self._tets), scene.nt, scene.material, scene.mu,
+                scene.lam, scene.dt, scene.rayleigh_alpha, scene.rayleigh_beta, scene.mass_scale,
+                _d(g), None if pin is None else _d(pin), tol, mi)
+        if ni > 1:
+            self.h = self.lib.ref_create_bdf1(*args, ni, newton_tol)
+        else:
+            self.h = self.lib.ref_create(*args)
         self.dofs = scene.dofs
         self.set_state(scene.initial_u(), scene.initial_v())
 

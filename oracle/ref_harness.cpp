@@ -6,15 +6,17 @@
 //
 // What it drives (reference files, relative to the reference root):
 //   scene step       src/Scenes/SIMULATION_SCENE.h:120-126 (zero f_ext, addGravity, solve)
-//   integrator       src/Timestepper/BACKWARD_EULER_POSITION.cpp:138-327
+//   integrator       src/Timestepper/BACKWARD_EULER_POSITION.cpp:138-327, or the BDF_1 Newton
+//                    loop src/Timestepper/BDF_1.cpp:78-239 (ref_create_bdf1)
 //   mesh             src/Geometry/TET_MESH_FASTER.cpp:30-133
 //   pinning          src/Timestepper/TIMESTEPPER.cpp:380-403 with src/Geometry/CUBE.cpp:27-55
 //   materials        src/Hyperelastic/Volume/SNH.cpp, ARAP.cpp, HYPERELASTIC.cpp
 //   SVD              src/util/MATRIX_UTIL.cpp:76-95 (svd_rv)
 //
-// The harness subclasses BACKWARD_EULER_POSITION only to reach the protected Eigen CG object
-// (src/Timestepper/TIMESTEPPER.h:207) so that the PCG tolerance can be set, as the workload
-// definitions ask (the reference itself never calls setTolerance).
+// The harness subclasses BACKWARD_EULER_POSITION and BDF_1 only to reach protected members: the
+// Eigen CG object (src/Timestepper/TIMESTEPPER.h:207), so that the PCG tolerance can be set as the
+// workload definitions ask (the reference itself never calls setTolerance), and BDF_1's Newton
+// limits (src/Timestepper/NEWMARK.h:61-63).
 #include <chrono>
 #include <cstring>
 #include <vector>
@@ -24,37 +26,95 @@
 #include "Hyperelastic/Volume/ARAP.h"
 #include "Hyperelastic/Volume/SNH.h"
 #include "Timestepper/BACKWARD_EULER_POSITION.h"
+#include "Timestepper/BDF_1.h"
 #include "util/MATRIX_UTIL.h"
 
 using namespace HOBAK;
 
 namespace {
 
-class HarnessBEP : public TIMESTEPPER::BACKWARD_EULER_POSITION {
- public:
-  HarnessBEP(TET_MESH& mesh, VOLUME::HYPERELASTIC& mat) : BACKWARD_EULER_POSITION(mesh, mat) {}
-  void setTolerance(double t) { _cgSolver.setTolerance(t); }
-  void setMaxIterations(int n) { _cgSolver.setMaxIterations(n); }
-  int iterations() const { return (int)_cgSolver.iterations(); }
-  double error() const { return (double)_cgSolver.error(); }
-  void setRayleigh(double a, double b) { _rayleighAlpha = a; _rayleighBeta = b; }
-  const SPARSE_MATRIX& massMatrix() const { return _M; }
-  const VECTOR& lastB() const { return _b; }
-};
-
-struct RefSim {
-  TET_MESH_FASTER* mesh = nullptr;
-  VOLUME::HYPERELASTIC* material = nullptr;
-  HarnessBEP* solver = nullptr;
-  CUBE* pin = nullptr;
-  VECTOR3 gravity;
-  double lastStepSeconds = 0.0;
-};
-
 VOLUME::HYPERELASTIC* makeMaterial(int kind, double mu, double lambda) {
   if (kind == 0) return new VOLUME::SNH(mu, lambda);
   return new VOLUME::ARAP(mu, lambda);
 }
+
+// Exposes the protected members the harness needs, for either integrator.
+template <typename Base>
+class Harness : public Base {
+ public:
+  Harness(TET_MESH& mesh, VOLUME::HYPERELASTIC& mat) : Base(mesh, mat) {}
+  void setTolerance(double t) { this->_cgSolver.setTolerance(t); }
+  void setMaxIterations(int n) { this->_cgSolver.setMaxIterations(n); }
+  int iterations() const { return (int)this->_cgSolver.iterations(); }
+  double error() const { return (double)this->_cgSolver.error(); }
+  void setRayleigh(double a, double b) {
+    this->_rayleighAlpha = a;
+    this->_rayleighBeta = b;
+  }
+  const SPARSE_MATRIX& massMatrix() const { return this->_M; }
+  const VECTOR& lastB() const { return this->_b; }
+};
+typedef Harness<TIMESTEPPER::BACKWARD_EULER_POSITION> HarnessBEP;
+
+typedef Harness<HOBAK::TIMESTEPPER::BDF_1> HarnessBDF1Base;
+class HarnessBDF1 : public HarnessBDF1Base {
+ public:
+  HarnessBDF1(TET_MESH& mesh, VOLUME::HYPERELASTIC& mat) : HarnessBDF1Base(mesh, mat) {}
+  void setNewton(int max_iters, double residual_tol) {
+    _maxNewtonIterations = max_iters;
+    _residualTolerance = residual_tol;
+  }
+};
+
+// The integrator behind one handle: exactly one of bep / bdf is set.
+struct RefSim {
+  TET_MESH_FASTER* mesh = nullptr;
+  VOLUME::HYPERELASTIC* material = nullptr;
+  HarnessBEP* bep = nullptr;
+  HarnessBDF1* bdf = nullptr;
+  CUBE* pin = nullptr;
+  VECTOR3 gravity;
+  double lastStepSeconds = 0.0;
+  TIMESTEPPER::TIMESTEPPER* base() const {
+    return bep ? (TIMESTEPPER::TIMESTEPPER*)bep : (TIMESTEPPER::TIMESTEPPER*)bdf;
+  }
+  int iterations() const { return bep ? bep->iterations() : bdf->iterations(); }
+  double error() const { return bep ? bep->error() : bdf->error(); }
+  const SPARSE_MATRIX& massMatrix() const { return bep ? bep->massMatrix() : bdf->massMatrix(); }
+  const VECTOR& lastB() const { return bep ? bep->lastB() : bdf->lastB(); }
+};
+
+template <typename H>
+void configure(RefSim* sim, H* h, double dt, double ra, double rb, double mass_scale,
+               const double* pin_cube, double pcg_tol, int pcg_max_iters) {
+  h->vertexFaceSelfCollisionsOn() = false;
+  h->edgeEdgeSelfCollisionsOn() = false;
+  h->setDt(dt);
+  h->setRayleigh(ra, rb);
+  if (mass_scale != 1.0) h->scaleMass(mass_scale);
+  if (pin_cube) {
+    sim->pin = new CUBE(VECTOR3(pin_cube[0], pin_cube[1], pin_cube[2]), pin_cube[3]);
+    h->attachKinematicConstraints(sim->pin);
+  }
+  if (pcg_tol > 0) h->setTolerance(pcg_tol);
+  if (pcg_max_iters > 0) h->setMaxIterations(pcg_max_iters);
+}
+
+RefSim* make_sim(const double* rest_xyz, int nv, const int* tets, int nt, int material, double mu,
+                 double lambda, const double* gravity) {
+  std::vector<VECTOR3> verts(nv);
+  for (int i = 0; i < nv; i++)
+    verts[i] = VECTOR3(rest_xyz[3 * i], rest_xyz[3 * i + 1], rest_xyz[3 * i + 2]);
+  std::vector<VECTOR4I> t(nt);
+  for (int i = 0; i < nt; i++)
+    t[i] = VECTOR4I(tets[4 * i], tets[4 * i + 1], tets[4 * i + 2], tets[4 * i + 3]);
+  RefSim* sim = new RefSim;
+  sim->mesh = new TET_MESH_FASTER(verts, t);
+  sim->material = makeMaterial(material, mu, lambda);
+  sim->gravity = VECTOR3(gravity[0], gravity[1], gravity[2]);
+  return sim;
+}
+
 
 MATRIX3 loadColMajor(const double* a) {
   MATRIX3 M;
@@ -81,35 +141,32 @@ void* ref_create(const double* rest_xyz, int nv, const int* tets, int nt, int ma
                  double mu, double lambda, double dt, double rayleigh_alpha,
                  double rayleigh_beta, double mass_scale, const double* gravity,
                  const double* pin_cube, double pcg_tol, int pcg_max_iters) {
-  std::vector<VECTOR3> verts(nv);
-  for (int i = 0; i < nv; i++)
-    verts[i] = VECTOR3(rest_xyz[3 * i], rest_xyz[3 * i + 1], rest_xyz[3 * i + 2]);
-  std::vector<VECTOR4I> t(nt);
-  for (int i = 0; i < nt; i++)
-    t[i] = VECTOR4I(tets[4 * i], tets[4 * i + 1], tets[4 * i + 2], tets[4 * i + 3]);
+  RefSim* sim = make_sim(rest_xyz, nv, tets, nt, material, mu, lambda, gravity);
+  sim->bep = new HarnessBEP(*sim->mesh, *sim->material);
+  configure(sim, sim->bep, dt, rayleigh_alpha, rayleigh_beta, mass_scale, pin_cube, pcg_tol,
+            pcg_max_iters);
+  return sim;
+}
 
-  RefSim* sim = new RefSim;
-  sim->mesh = new TET_MESH_FASTER(verts, t);
-  sim->material = makeMaterial(material, mu, lambda);
-  sim->solver = new HarnessBEP(*sim->mesh, *sim->material);
-  sim->solver->vertexFaceSelfCollisionsOn() = false;
-  sim->solver->edgeEdgeSelfCollisionsOn() = false;
-  sim->solver->setDt(dt);
-  sim->solver->setRayleigh(rayleigh_alpha, rayleigh_beta);
-  if (mass_scale != 1.0) sim->solver->scaleMass(mass_scale);
-  sim->gravity = VECTOR3(gravity[0], gravity[1], gravity[2]);
-  if (pin_cube) {
-    sim->pin = new CUBE(VECTOR3(pin_cube[0], pin_cube[1], pin_cube[2]), pin_cube[3]);
-    sim->solver->attachKinematicConstraints(sim->pin);
-  }
-  if (pcg_tol > 0) sim->solver->setTolerance(pcg_tol);
-  if (pcg_max_iters > 0) sim->solver->setMaxIterations(pcg_max_iters);
+// The same scene with the BDF_1 Newton loop: at most max_newton Newton iterations per step while
+// |S(b - A z)|^2 > residual_tol (BDF_1.cpp:111; defaults 3 and 1e-7, BDF_1.cpp:36, NEWMARK.cpp:43).
+void* ref_create_bdf1(const double* rest_xyz, int nv, const int* tets, int nt, int material,
+                      double mu, double lambda, double dt, double rayleigh_alpha,
+                      double rayleigh_beta, double mass_scale, const double* gravity,
+                      const double* pin_cube, double pcg_tol, int pcg_max_iters, int max_newton,
+                      double residual_tol) {
+  RefSim* sim = make_sim(rest_xyz, nv, tets, nt, material, mu, lambda, gravity);
+  sim->bdf = new HarnessBDF1(*sim->mesh, *sim->material);
+  configure(sim, sim->bdf, dt, rayleigh_alpha, rayleigh_beta, mass_scale, pin_cube, pcg_tol,
+            pcg_max_iters);
+  sim->bdf->setNewton(max_newton, residual_tol);
   return sim;
 }
 
 void ref_destroy(void* h) {
   RefSim* sim = (RefSim*)h;
-  delete sim->solver;
+  delete sim->bep;
+  delete sim->bdf;
   delete sim->material;
   delete sim->mesh;
   delete sim->pin;
@@ -118,55 +175,55 @@ void ref_destroy(void* h) {
 
 int ref_num_pinned(void* h) {
   RefSim* sim = (RefSim*)h;
-  return (int)sim->solver->constrainedNodes().size();
+  return (int)sim->base()->constrainedNodes().size();
 }
 
 void ref_pinned_nodes(void* h, int* out) {
   RefSim* sim = (RefSim*)h;
-  std::vector<int> n = sim->solver->constrainedNodes();
+  std::vector<int> n = sim->base()->constrainedNodes();
   for (size_t i = 0; i < n.size(); i++) out[i] = n[i];
 }
 
 void ref_set_state(void* h, const double* u, const double* v) {
   RefSim* sim = (RefSim*)h;
-  const int dofs = (int)sim->solver->position().size();
+  const int dofs = (int)sim->base()->position().size();
   if (u)
-    for (int i = 0; i < dofs; i++) sim->solver->position()[i] = u[i];
+    for (int i = 0; i < dofs; i++) sim->base()->position()[i] = u[i];
   if (v)
-    for (int i = 0; i < dofs; i++) sim->solver->velocity()[i] = v[i];
+    for (int i = 0; i < dofs; i++) sim->base()->velocity()[i] = v[i];
 }
 
 void ref_get_state(void* h, double* u, double* v) {
   RefSim* sim = (RefSim*)h;
-  const int dofs = (int)sim->solver->position().size();
+  const int dofs = (int)sim->base()->position().size();
   if (u)
-    for (int i = 0; i < dofs; i++) u[i] = sim->solver->position()[i];
+    for (int i = 0; i < dofs; i++) u[i] = sim->base()->position()[i];
   if (v)
-    for (int i = 0; i < dofs; i++) v[i] = sim->solver->velocity()[i];
+    for (int i = 0; i < dofs; i++) v[i] = sim->base()->velocity()[i];
 }
 
 // One SIMULATION_SCENE::stepSimulation: returns the PCG iteration count.
 int ref_step(void* h, double* pcg_error, double* seconds) {
   RefSim* sim = (RefSim*)h;
   auto t0 = std::chrono::steady_clock::now();
-  sim->solver->externalForces().setZero();
-  sim->solver->addGravity(sim->gravity);
-  sim->solver->solve(false);
+  sim->base()->externalForces().setZero();
+  sim->base()->addGravity(sim->gravity);
+  sim->base()->solve(false);
   auto t1 = std::chrono::steady_clock::now();
-  if (pcg_error) *pcg_error = sim->solver->error();
+  if (pcg_error) *pcg_error = sim->error();
   if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
-  return sim->solver->iterations();
+  return sim->iterations();
 }
 
 void ref_last_rhs_b(void* h, double* b) {
   RefSim* sim = (RefSim*)h;
-  const VECTOR& vb = sim->solver->lastB();
+  const VECTOR& vb = sim->lastB();
   for (int i = 0; i < vb.size(); i++) b[i] = vb[i];
 }
 
 void ref_mass_diag(void* h, double* m) {
   RefSim* sim = (RefSim*)h;
-  const SPARSE_MATRIX& M = sim->solver->massMatrix();
+  const SPARSE_MATRIX& M = sim->massMatrix();
   for (int i = 0; i < M.rows(); i++) m[i] = M.coeff(i, i);
 }
 
@@ -174,7 +231,7 @@ void ref_mass_diag(void* h, double* m) {
 // TET_MESH_FASTER.cpp:89-133).  K is returned in the reference's own compressed-column layout.
 int ref_mesh_nnz(void* h) {
   RefSim* sim = (RefSim*)h;
-  const int dofs = (int)sim->solver->position().size();
+  const int dofs = (int)sim->base()->position().size();
   VECTOR zero(dofs);
   zero.setZero();
   sim->mesh->setDisplacement(zero);
@@ -187,7 +244,7 @@ int ref_mesh_nnz(void* h) {
 void ref_mesh_forces_hessian(void* h, const double* u, double* R, int* colptr, int* rowidx,
                              double* vals) {
   RefSim* sim = (RefSim*)h;
-  const int dofs = (int)sim->solver->position().size();
+  const int dofs = (int)sim->base()->position().size();
   VECTOR vu(dofs);
   for (int i = 0; i < dofs; i++) vu[i] = u[i];
   sim->mesh->setDisplacement(vu);

@@ -12,6 +12,9 @@ small .npz fixtures (no pickles) committed beside this script:
                         reference's (Jacobi) PCG at tol 1e-8: iterations, errors, states
   c1_arap.npz           the same scene with ARAP (mu = 3.4e4), 10 steps
   c2_traj.npz           the 40^3 stretch scene, 20 steps: iterations, norms, strided samples of u, v
+  c1_bdf1.npz           the C1 scene with the BDF_1 Newton loop (src/Timestepper/BDF_1.cpp:78-239,
+                        3 Newton iterations, residual tolerance 1e-7), 10 steps: the last Newton
+                        iteration's PCG count per step, states after steps 1, 5 and 10
 
 Usage:  python tests/golden/make_golden.py [--skip-c2]
 """
@@ -111,8 +114,8 @@ def make_mesh_small(out):
     np.savez_compressed(out, **data)
 
 
-def run_traj(sc, steps, keep_steps, stride=1):
-    r = oracle.RefSim(sc)
+def run_traj(sc, steps, keep_steps, stride=1, newton_iters=None):
+    r = oracle.RefSim(sc, newton_iters=newton_iters)
     iters, errs, secs, unorm, vnorm = [], [], [], [], []
     us, vs = {}, {}
     b0 = None
@@ -168,6 +171,17 @@ def make_c2(out):
     np.savez_compressed(out, **data)
 
 
+def make_c1_bdf1(out):
+    sc = scenes.config1()
+    t = run_traj(sc, 10, keep_steps=(1, 5, 10), newton_iters=3)
+    data = dict(iters_last_newton=t["iters"], errs=t["errs"], unorm=t["unorm"], vnorm=t["vnorm"],
+                newton_iters=np.array(3), newton_tol=np.array(1e-7))
+    for k, u in t["us"].items():
+        data[f"u{k}"] = u
+        data[f"v{k}"] = t["vs"][k]
+    np.savez_compressed(out, **data)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--skip-c2", action="store_true")
@@ -176,7 +190,7 @@ def main():
     if not oracle.ref_available():
         sys.exit("oracle/_ref/libhobak_ref.so is not built (make -C oracle ref)")
     jobs = [("element_fixtures.npz", make_element), ("mesh_small.npz", make_mesh_small),
-            ("c1_traj.npz", make_c1), ("c1_arap.npz", make_c1_arap)]
+            ("c1_traj.npz", make_c1), ("c1_arap.npz", make_c1_arap), ("c1_bdf1.npz", make_c1_bdf1)]
     if not args.skip_c2:
         jobs.append(("c2_traj.npz", make_c2))
     for name, fn in jobs:

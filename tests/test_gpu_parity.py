@@ -237,3 +237,22 @@ def test_c2_block_jacobi_matches_oracle(gpu_ctx):
         assert abs(st["pcg_iters"][0] - it) <= 1
         assert rel(X + u, X + uo) <= 1e-9
     inst.close()
+
+
+def test_c1_bdf1_matches_reference(gpu_ctx):
+    """The BDF_1 Newton loop (3 Newton iterations, residual test |S b|^2 > 1e-7, Jacobi PCG) on C1
+    against the reference's BDF_1: the last Newton iteration's PCG count within +-1 every step,
+    positions within 1e-9 after 1, 5 and 10 steps."""
+    g = golden("c1_bdf1.npz")
+    sc = scenes.config1()
+    _say("C1 BDF_1, 10 steps")
+    inst = lib.Instance(gpu_ctx, sc, precond=scenes.PRECOND_JACOBI,
+                        newton_iters=int(g["newton_iters"]), newton_tol=float(g["newton_tol"]))
+    X = sc.rest.ravel()
+    for k in range(1, 11):
+        st = inst.step()
+        assert abs(st["pcg_iters"][-1] - int(g["iters_last_newton"][k - 1])) <= 1, (k, st)
+        if k in (1, 5, 10):
+            u, _ = inst.state()
+            assert rel(X + u, X + g[f"u{k}"]) <= 1e-9, k
+    inst.close()

@@ -119,3 +119,21 @@ def test_c2_first_steps_match_reference():
         assert abs(out[k - 1][0] - g["iters"][k - 1]) <= 1
         assert rel(out[k - 1][1][::s], g[f"u{k}"]) <= 1e-9
     assert abs(out[1][0] - g["iters"][1]) <= 1
+
+
+def test_c1_bdf1_trajectory_matches_reference():
+    """The BDF_1 Newton loop (3 Newton iterations, residual tolerance 1e-7) on C1, 10 steps,
+    against the reference's own BDF_1 (src/Timestepper/BDF_1.cpp:78-239)."""
+    g = golden("c1_bdf1.npz")
+    sc = scenes.config1()
+    o = oracle.Oracle(sc, precond=scenes.PRECOND_JACOBI, newton_iters=int(g["newton_iters"]),
+                      newton_tol=float(g["newton_tol"]))
+    X = sc.rest.ravel()
+    for k in range(1, 11):
+        o.step()
+        assert abs(int(o.last_iters_per_newton[-1]) - int(g["iters_last_newton"][k - 1])) <= 1, k
+        if k in (1, 5, 10):
+            u, v = o.state()
+            assert rel(X + u, X + g[f"u{k}"]) <= 1e-12, k
+            assert rel(u, g[f"u{k}"]) <= 1e-10, k
+            assert rel(v, g[f"v{k}"]) <= 1e-10, k
