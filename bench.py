@@ -24,6 +24,8 @@ Output: one JSON line (see the field list in main()).
   serving    a Poisson stream of sessions (tets log-uniform in [1e3, 4e5], penalty ground,
              20 step requests each) served by the continuous-batching scheduler in the precision
              form, against the plain FP64 path serving the same stream one step at a time
+  beyond_mall   the SpMV's roofline on one 88^3 lattice instance (4.09 M tets, 0.83 GB FP64 /
+             0.44 GB FP32 per launch), FP64 and FP32, where no cache holds the matrix
 """
 from __future__ import annotations
 
@@ -52,7 +54,9 @@ def parse():
     ap.add_argument("--precond", default="block", choices=["block", "jacobi"])
     ap.add_argument("--cpu-baseline", default="auto", choices=["auto", "reference", "port", "none"])
     ap.add_argument("--cpu-steps", type=int, default=3)
-    ap.add_argument("--sample-every", type=int, default=4)
+    ap.add_argument("--sample-every", type=int, default=256,
+                    help="time every n-th SpMV launch of the timed steps with HIP events; a PCG "
+                         "group holding one runs uncaptured, the others replay their graph")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run the calibration copy kernel (for counter scaling)")
     ap.add_argument("--precision-form", default="on", choices=["on", "off"],
@@ -67,6 +71,9 @@ def parse():
     ap.add_argument("--rate", type=float, default=400.0, help="session arrivals per second")
     ap.add_argument("--session-steps", type=int, default=20)
     ap.add_argument("--budget-gib", type=float, default=8.0)
+    ap.add_argument("--beyond-mall", default="on", choices=["on", "off"],
+                    help="the SpMV on one large instance whose matrix exceeds the Infinity Cache")
+    ap.add_argument("--large-n", type=int, default=88, help="lattice cells per axis of that instance")
     ap.add_argument("--graphs", default="on", choices=["on", "off"],
                     help="off: launch every kernel directly (for profilers that do not trace "
                          "graph launches)")
@@ -180,7 +187,8 @@ def batch_form(lib, scenes, args):
            "workload": f"{len(scs)} instances, dims U{{4..24}}^3 Kuhn lattices, SNH/ARAP 50/50, "
                        f"E log-U[1e4,1e6], nu U[0.3,0.48], seeded stretch velocity, block-Jacobi"}
     for prec, name in ((lib.PRECISION_FP64, "fp64"), (lib.PRECISION_FP32_REFINE, "fp32_refine")):
-        ctx = lib.Context(0, precision="fp64" if prec == lib.PRECISION_FP64 else "fp32")
+        ctx = lib.Context(0, precision="fp64" if prec == lib.PRECISION_FP64 else "fp32",
+                          workload="batch")
         ctx.set_tuning(graphs=GRAPHS[0])
         insts = [lib.Instance(ctx, sc, precision=prec) for sc in scs]
 
@@ -211,24 +219,63 @@ def batch_form(lib, scenes, args):
                "p99_batch_step_ms": 1e3 * float(np.percentile(step_s, 99)),
                "alone_value": nt * args.batch_steps / ta,
                "speedup_vs_alone": ta / tb, "bitwise_equal_to_alone": same}
-        if prec == lib.PRECISION_FP64:
-            for inst in insts:
-                inst.set_kernel_sampling(2)
-            reset()
-            lib.step_batch(ctx, insts, allow_unconverged=True)
-            ks = insts[0].kernel_stats()
-            if ks["spmv_avg_ms"] > 0:
-                gbs = ks["spmv_bytes"] / (ks["spmv_avg_ms"] * 1e-3) / 1e9
-                res["roofline"] = {"bound": "hbm", "kernel": "cg_spmv_kernel<double>",
-                                   "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                                   "frac": round(gbs / HBM_PEAK_GBS, 4),
-                                   "bytes_per_launch": ks["spmv_bytes"],
-                                   "avg_launch_ms": ks["spmv_avg_ms"],
-                                   "launches_sampled": ks["spmv_count"],
-                                   "note": "launches during which every instance was solving"}
+        for inst in insts:
+            inst.set_kernel_sampling(2)
+        reset()
+        lib.step_batch(ctx, insts, allow_unconverged=True)
+        ks = insts[0].kernel_stats()
+        nv = sum(sc.nv for sc in scs)
+        rf = spmv_roofline(ks, prec == lib.PRECISION_FP64, nv)
+        if rf:
+            rf["note"] = "launches during which every instance was solving (its first inner solve)"
+            res["roofline"] = rf
         out[name] = res
         for inst in insts:
             inst.close()
+        ctx.close()
+    return out
+
+
+def spmv_roofline(ks, fp64, nv):
+    """Roofline entry of the sampled SpMV launches: the algorithmic bytes per launch
+    nnzb*(9s+4) + 4*(nv+1) + 2*3s*nv over the average launch time, and the same with the
+    kernel's two vector writes (t and p_new, 2*3s*nv) added."""
+    if ks["spmv_avg_ms"] <= 0:
+        return None
+    s = 8 if fp64 else 4
+    sec = ks["spmv_avg_ms"] * 1e-3
+    gbs = ks["spmv_bytes"] / sec / 1e9
+    gbs_w = (ks["spmv_bytes"] + 2 * 3 * s * nv) / sec / 1e9
+    return {"bound": "hbm", "kernel": f"cg_spmv_kernel<{'double' if fp64 else 'float'}>",
+            "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(gbs / HBM_PEAK_GBS, 4),
+            "achieved_incl_vector_writes": round(gbs_w, 1),
+            "frac_incl_vector_writes": round(gbs_w / HBM_PEAK_GBS, 4),
+            "bytes_per_launch": ks["spmv_bytes"], "avg_launch_ms": ks["spmv_avg_ms"],
+            "launches_sampled": ks["spmv_count"]}
+
+
+def beyond_mall_form(lib, scenes, args):
+    """The SpMV on a working set well beyond the 256 MB Infinity Cache: one n^3 lattice instance
+    of about the 256-instance batch's total size (and with the batch's tuned launch shapes), one
+    step per precision with every SpMV launch of the first (inner) solve sampled."""
+    n = args.large_n
+    sc = scenes.lattice_scene((n, n, n), seed=2, stretch_velocity=True)
+    out = {"workload": f"{n}^3 Kuhn lattice cube ({sc.nt} tets, {sc.nv} vertices), stretch scene, "
+                       f"block-Jacobi, one step", "tets": sc.nt, "vertices": sc.nv}
+    for prec, name in ((lib.PRECISION_FP64, "fp64"), (lib.PRECISION_FP32_REFINE, "fp32_refine")):
+        ctx = lib.Context(0, precision="fp64" if prec == lib.PRECISION_FP64 else "fp32",
+                          workload="batch")
+        ctx.set_tuning(graphs=GRAPHS[0])
+        inst = lib.Instance(ctx, sc, precision=prec)
+        inst.set_kernel_sampling(1)
+        t0 = time.perf_counter()
+        st = inst.step(allow_unconverged=True)
+        dt = time.perf_counter() - t0
+        rf = spmv_roofline(inst.kernel_stats(), prec == lib.PRECISION_FP64, sc.nv)
+        out[name] = {"ms_step_sampled": 1e3 * dt, "pcg_iters": st["total_pcg_iters"],
+                     "roofline": rf}
+        inst.close()
         ctx.close()
     return out
 
@@ -246,7 +293,7 @@ def serving_form(lib, scenes, args):
 
     for s_ in stream:
         scene_of(s_.k)
-    ca = lib.Context(0, precision="fp32")
+    ca = lib.Context(0, precision="fp32", workload="batch")
     cb = lib.Context(0)
     ca.set_tuning(graphs=GRAPHS[0])
     cb.set_tuning(graphs=GRAPHS[0])
@@ -350,6 +397,8 @@ def main():
         out["batch"] = batch_form(lib, scenes, args)
     if args.serving == "on":
         out["serving"] = serving_form(lib, scenes, args)
+    if args.beyond_mall == "on":
+        out["beyond_mall"] = beyond_mall_form(lib, scenes, args)
     if args.copy_probe:
         nb = 512 << 20
         ms = ctx.copy_probe(nb, 20)

@@ -8,6 +8,10 @@ time.  The best configuration per precision is written to
 tuning.json next to this file; lib.Context applies it to every instance it creates.
 
     python -m hobakv1_amd.autotune --n 40 --steps 4 [--precision fp64|fp32] [--write]
+    python -m hobakv1_amd.autotune --workload batch --instances 256 [--precision ...] [--write]
+
+The batch workload (the 256-instance C4 set stepped together by hobak_step_batch) is written
+as the "<precision>_batch" entry, which contexts created for batched or served instances apply.
 """
 from __future__ import annotations
 
@@ -37,17 +41,51 @@ def measure(ctx, scene, precond, precision, steps, warmup, inner_tol=1e-4):
     for _ in range(warmup):
         inst.step()
     inst.set_state(scene.initial_u(), scene.initial_v())
-    inst.set_kernel_sampling(2)
     t0 = time.perf_counter()
     iters = 0
     for _ in range(steps):
         st = inst.step()
         iters += st["total_pcg_iters"]
     dt = time.perf_counter() - t0
+    # the SpMV is timed in a separate step: sampled launches carry events and run uncaptured
+    inst.set_state(scene.initial_u(), scene.initial_v())
+    inst.set_kernel_sampling(2)
+    inst.step()
     ks = inst.kernel_stats()
     inst.close()
     return {"ms_per_step": 1e3 * dt / steps, "pcg_iters": iters / steps,
             "ms_per_pcg_iter": 1e3 * dt / max(1, iters),
+            "spmv_ms": ks["spmv_avg_ms"],
+            "spmv_GBps": (ks["spmv_bytes"] / (ks["spmv_avg_ms"] * 1e-3) / 1e9
+                          if ks["spmv_avg_ms"] > 0 else None)}
+
+
+def measure_batch(ctx, scs, precision, steps, warmup):
+    from . import lib
+
+    insts = [lib.Instance(ctx, sc, precision=precision) for sc in scs]
+
+    def reset():
+        for inst, sc in zip(insts, scs):
+            inst.set_state(sc.initial_u(), sc.initial_v())
+
+    for _ in range(warmup):
+        lib.step_batch(ctx, insts, allow_unconverged=True)
+    reset()
+    t0 = time.perf_counter()
+    iters = 0
+    for _ in range(steps):
+        st = lib.step_batch(ctx, insts, allow_unconverged=True)
+        iters += sum(s["total_pcg_iters"] for s in st)
+    dt = time.perf_counter() - t0
+    for inst in insts:
+        inst.set_kernel_sampling(2)
+    reset()
+    lib.step_batch(ctx, insts, allow_unconverged=True)
+    ks = insts[0].kernel_stats()
+    for inst in insts:
+        inst.close()
+    return {"ms_per_step": 1e3 * dt / steps, "pcg_iters": iters / steps,
             "spmv_ms": ks["spmv_avg_ms"],
             "spmv_GBps": (ks["spmv_bytes"] / (ks["spmv_avg_ms"] * 1e-3) / 1e9
                           if ks["spmv_avg_ms"] > 0 else None)}
@@ -64,6 +102,8 @@ def main():
     ap.add_argument("--update-wgs", default="512")
     ap.add_argument("--spmv-pad", default="0,1")
     ap.add_argument("--inner-tol", default="1e-4", help="FP32 inner PCG tolerances to sweep")
+    ap.add_argument("--workload", default="single", choices=["single", "batch"])
+    ap.add_argument("--instances", type=int, default=256, help="batch workload size")
     ap.add_argument("--write", action="store_true")
     args = ap.parse_args()
     sys.path.insert(0, os.path.dirname(HERE))
@@ -71,6 +111,7 @@ def main():
 
     precision = lib.PRECISION_FP64 if args.precision == "fp64" else lib.PRECISION_FP32_REFINE
     sc = scenes.lattice_scene((args.n,) * 3, seed=2, stretch_velocity=True)
+    scs = scenes.batch_instances(args.instances) if args.workload == "batch" else None
     results = []
     tols = [float(x) for x in args.inner_tol.split(",")] if args.precision == "fp32" else [0.0]
     for H, poll, uw, pad, tol in itertools.product([int(x) for x in args.slice_rows.split(",")],
@@ -79,8 +120,12 @@ def main():
                                                    [int(x) for x in args.spmv_pad.split(",")], tols):
         ctx = lib.Context(0, apply_table=False)
         ctx.set_tuning(slice_rows=H, poll_chunk=poll, update_wgs=uw, spmv_pad=pad)
-        r = measure(ctx, sc, scenes.PRECOND_BLOCK_JACOBI, precision, args.steps, args.warmup,
-                    inner_tol=tol or 1e-4)
+        if scs is not None:
+            ctx.refine_inner_tol = tol or 1e-4
+            r = measure_batch(ctx, scs, precision, args.steps, args.warmup)
+        else:
+            r = measure(ctx, sc, scenes.PRECOND_BLOCK_JACOBI, precision, args.steps, args.warmup,
+                        inner_tol=tol or 1e-4)
         ctx.close()
         r.update({"slice_rows": H, "poll_chunk": poll, "update_wgs": uw, "spmv_pad": pad})
         if args.precision == "fp32":
@@ -91,11 +136,13 @@ def main():
     print("best", json.dumps(best), flush=True)
     if args.write:
         table = load_table()
-        table[args.precision] = {"slice_rows": best["slice_rows"], "poll_chunk": best["poll_chunk"],
-                                 "update_wgs": best["update_wgs"], "spmv_pad": best["spmv_pad"],
-                                 "measured": {"workload": f"{args.n}^3 lattice", **best}}
+        key = args.precision + ("_batch" if scs is not None else "")
+        what = (f"{args.instances}-instance batch" if scs is not None else f"{args.n}^3 lattice")
+        table[key] = {"slice_rows": best["slice_rows"], "poll_chunk": best["poll_chunk"],
+                      "update_wgs": best["update_wgs"], "spmv_pad": best["spmv_pad"],
+                      "measured": {"workload": what, **best}}
         if "refine_inner_tol" in best:
-            table[args.precision]["refine_inner_tol"] = best["refine_inner_tol"]
+            table[key]["refine_inner_tol"] = best["refine_inner_tol"]
         with open(TABLE, "w") as f:
             json.dump(table, f, indent=1, sort_keys=True)
     return 0

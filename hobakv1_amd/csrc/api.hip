@@ -57,6 +57,7 @@ struct hobak_ctx {
   int grid_cap = 0;                    // max workgroups per launch (0: one per virtual workgroup)
   int update_cap = kUpdateCap;         // per-instance workgroups of the PCG update kernel
   int spmv_pad = 0;                    // SpMV loop over padded slice widths (1) or row lengths (0)
+  int spmv_stream_mb = 192;            // non-temporal matrix loads above this SpMV size (MiB)
   hipStream_t stream = nullptr;
   size_t budget = 0;
   size_t in_use = 0;
@@ -224,6 +225,7 @@ struct hobak_inst {
   int pred[HOBAK_MAX_NEWTON];       // PCG iterations of the last step, per Newton iteration
   // sampled SpMV timing (measurement hook)
   int sample_every = 0;
+  int first_iters = 0;              // iterations of the last step's first (inner) solve
   double spmv_ms_sum = 0.0, spmv_bytes_sum = 0.0;
   long long spmv_count = 0;
 };
@@ -333,7 +335,8 @@ struct Plan {
   Batch b[K_COUNT];
   InstStatus* status_dev = nullptr;
   InstStatus* status_host = nullptr;
-  double spmv_bytes = 0.0;
+  double spmv_bytes = 0.0;      // summed over the members
+  double spmv_bytes_max = 0.0;  // of the largest member
   hipGraphExec_t chunk_exec = nullptr;
   int chunk_k = 0;
   hipGraphExec_t pro_exec[2] = {nullptr, nullptr};  // prologue without / with the step start
@@ -358,9 +361,12 @@ int plan_fill(hobak_ctx* ctx, Plan& P) {
   const InstDev** hp = (const InstDev**)(P.host + off);
   const InstDev* const* dp = (const InstDev* const*)(P.dev + off);
   P.spmv_bytes = 0.0;
+  P.spmv_bytes_max = 0.0;
   for (int i = 0; i < n; i++) {
     hp[i] = P.members[i]->dev;
-    P.spmv_bytes += spmv_bytes_of(P.members[i]);
+    const double sb = spmv_bytes_of(P.members[i]);
+    P.spmv_bytes += sb;
+    P.spmv_bytes_max = std::max(P.spmv_bytes_max, sb);
   }
   off += al(sizeof(void*) * n);
   for (int k = 0; k < K_COUNT; k++) {
@@ -545,18 +551,22 @@ template <typename T>
 int launch_chunk(hobak_ctx* ctx, const Plan& P, int k, hobak_sched* S, int every) {
   hipStream_t s = ctx->stream;
   const int cap = ctx->grid_cap;
+  // streamed matrix loads pay off when one instance's matrix is too large to stay cached (its
+  // gathered vectors are then re-fetched if the matrix displaces them); a batch of small
+  // instances gathers from small windows either way
+  const int stream = P.spmv_bytes_max > (double)ctx->spmv_stream_mb * (1 << 20) ? 1 : 0;
   for (int j = 0; j < k; j++) {
     hipEvent_t a = nullptr, b = nullptr;
     if (S) {
       const int g = S->sample_it++;
-      if (every > 0 && g % every == 0 && (int)S->samples.size() < kMaxSamples) {
+      if (every > 0 && g % every == every - 1 && (int)S->samples.size() < kMaxSamples) {
         const int slot = (int)S->samples.size();
         a = ctx->sample_ev[2 * slot];
         b = ctx->sample_ev[2 * slot + 1];
         S->samples.emplace_back(g, slot);
       }
     }
-    HIPCHK(ctx, launch_cg_spmv<T>(P.b[K_SPMV], P.shift, ctx->spmv_pad, j & 1, cap, s, a, b));
+    HIPCHK(ctx, launch_cg_spmv<T>(P.b[K_SPMV], P.shift, ctx->spmv_pad, stream, j & 1, cap, s, a, b));
     HIPCHK(ctx, launch_cg_update<T>(P.b[K_UPD], j & 1, cap, s));
   }
   if (P.refine) {
@@ -608,7 +618,7 @@ int tick_impl(hobak_sched* S) {
     S->dirty = false;
   }
   Plan& M = *S->mplan;
-  const bool graphs = ctx->use_graphs && !S->sampling;
+  const bool graphs = ctx->use_graphs != 0;
 
   // 1. solves that finished at the last poll: x is applied, then the step ends or the next BDF_1
   //    Newton iteration re-enters (the test uses |S b|^2 of the system just solved, BDF_1.cpp:111, 172)
@@ -724,19 +734,28 @@ int tick_impl(hobak_sched* S) {
   }
   if (nsolving) {
     HIPCHK(ctx, hipEventRecord(ctx->ev[4], s));
-    if (graphs) {
-      if (!M.chunk_exec || M.chunk_k != k) {
-        if (M.chunk_exec) (void)hipGraphExecDestroy(M.chunk_exec);
-        M.chunk_exec = nullptr;
-        TRY(capture(ctx, &M.chunk_exec, [&]() { return launch_chunk<T>(ctx, M, k, nullptr, 0); }));
-        M.chunk_k = k;
-      }
-      for (int j = 0; j < m; j++) HIPCHK(ctx, hipGraphLaunch(M.chunk_exec, s));
-    } else {
-      int every = 0;
+    int every = 0;
+    if (S->sampling)
       for (auto* I : S->members)
         if (I->sample_every > 0) every = every ? std::min(every, I->sample_every) : I->sample_every;
-      for (int j = 0; j < m; j++) TRY(launch_chunk<T>(ctx, M, k, S->sampling ? S : nullptr, every));
+    if (graphs && (!M.chunk_exec || M.chunk_k != k)) {
+      if (M.chunk_exec) (void)hipGraphExecDestroy(M.chunk_exec);
+      M.chunk_exec = nullptr;
+      TRY(capture(ctx, &M.chunk_exec, [&]() { return launch_chunk<T>(ctx, M, k, nullptr, 0); }));
+      M.chunk_k = k;
+    }
+    for (int j = 0; j < m; j++) {
+      // a group that holds a sampled SpMV launch (index g with g % every == every - 1) runs
+      // uncaptured, with the events on that launch; every other group replays the graph
+      const int g0 = S->sample_it;
+      const bool sampled = every > 0 && (int)S->samples.size() < kMaxSamples &&
+                           (g0 / every) * every + every - 1 < g0 + k;
+      if (graphs && !sampled) {
+        HIPCHK(ctx, hipGraphLaunch(M.chunk_exec, s));
+        if (S->sampling) S->sample_it += k;
+      } else {
+        TRY(launch_chunk<T>(ctx, M, k, S->sampling ? S : nullptr, every));
+      }
     }
     HIPCHK(ctx, hipEventRecord(ctx->ev[5], s));
     ran = true;
@@ -809,6 +828,7 @@ int tick_impl(hobak_sched* S) {
       }
       r.st.status = worse(r.st.status, code);
       r.I->pred[nk] = iters;
+      if (nk == 0) r.I->first_iters = q.first_iters;
       r.stage = 2;
       S->stat[6] += 1;
     } else {
@@ -857,7 +877,7 @@ int run_batch(hobak_ctx* ctx, const std::vector<hobak_inst*>& m, hobak_step_stat
   S.ctx = ctx;
   S.ephemeral = true;
   S.members = m;
-  S.sampling = m[0]->tsize == 8;
+  S.sampling = true;
   for (auto* I : m) S.sampling = S.sampling && I->sample_every > 0;
   if (S.sampling && ctx->sample_ev.empty()) {
     ctx->sample_ev.assign(2 * kMaxSamples, nullptr);
@@ -888,10 +908,11 @@ int run_batch(hobak_ctx* ctx, const std::vector<hobak_inst*>& m, hobak_step_stat
     worst = worse(worst, c.stats.status);
   }
   // sampled SpMV launches did work for every member while their index was <= every member's
-  // first-solve iteration count (launch 0 forms p = z; launch j finishes iteration j - 1)
+  // first-solve iteration count (launch 0 forms p = z; launch j finishes iteration j - 1); with
+  // refinement, that of the first inner solve
   if (S.sampling && !S.samples.empty()) {
     int lim = INT32_MAX;
-    for (auto& c : S.done) lim = std::min(lim, c.stats.pcg_iters[0]);
+    for (auto* I : m) lim = std::min(lim, I->first_iters);
     for (auto& sm : S.samples) {
       if (sm.first > lim) continue;
       float ms = 0.f;
@@ -1466,6 +1487,10 @@ int hobak_ctx_set_tuning(hobak_ctx* ctx, int32_t key, int32_t value) {
     case HOBAK_TUNE_SPMV_PAD:
       if (value != 0 && value != 1) return fail(ctx, HOBAK_E_INVALID, "spmv pad is 0 or 1");
       ctx->spmv_pad = value;
+      return HOBAK_OK;
+    case HOBAK_TUNE_SPMV_STREAM_MB:
+      if (value < 0) return fail(ctx, HOBAK_E_INVALID, "stream threshold must be >= 0");
+      ctx->spmv_stream_mb = value;
       return HOBAK_OK;
     case HOBAK_TUNE_UPDATE_WGS:
       if (value < 1 || value > 65536) return fail(ctx, HOBAK_E_INVALID, "update workgroups out of range");

@@ -181,7 +181,8 @@ __device__ __forceinline__ void element_tet(const ElementIn& a, int t) {
 // material is a per-instance, workgroup-uniform branch.
 template <typename T>
 __global__ __launch_bounds__(kElementBlock) void element_kernel(Batch b, int src, int with_hessian) {
-  for (int vb = blockIdx.x; vb < b.total; vb += gridDim.x) {
+  for (int ws = blockIdx.x; ws < b.total; ws += gridDim.x) {
+    const int vb = xcd_vb(ws, b.total);
     int lb, lg;
     const HOBAK_GLOBAL InstDev* __restrict__ I = seg_find(b, vb, lb, lg);
     ElementIn a;

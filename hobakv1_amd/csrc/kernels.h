@@ -37,7 +37,8 @@
 // Batching.  Every kernel is *segmented*: one launch serves any set of instances.  Each instance
 // owns a read-only InstDev record in device memory; a Batch names the records of the launch and
 // a prefix table of virtual workgroups (boff[i] = first virtual workgroup of instance i).  A
-// workgroup finds its instance by binary search, so no workgroup spans two instances, and
+// workgroup finds its instance by counting the prefix entries at or below its index, so no
+// workgroup spans two instances, and
 // workgroups stride over the virtual workgroups when the launch grid is capped (persistent
 // form).  A single instance is a batch of one.
 #pragma once
@@ -88,6 +89,8 @@ struct RefineState {
   int status;       // 0 ok, 1 sweep limit, 2 non-finite
   int sweeps;
   int inner_iters;  // PCG iterations summed over the sweeps
+  int first_iters;  // PCG iterations of the first sweep
+  int pad;
 };
 
 enum Phase { PHASE_IDLE = 0, PHASE_SOLVING = 1, PHASE_FINISHED = 2 };
@@ -109,7 +112,7 @@ struct InstStatus {
   int cg_status;
   int ref_status;
   int ref_done;
-  int pad;
+  int first_iters;  // iterations of the first (refinement: first sweep's) inner solve
   double err;       // sqrt(res^2 / b^2) of the finished solve
   double maxR;      // |S b|^2
 };
@@ -159,28 +162,51 @@ struct Batch {
   int total;                   // boff[n]
 };
 
+// The virtual workgroup that launch slot ws runs.  The dispatcher hands consecutive workgroups
+// to the 8 XCDs in turn and each XCD has its own L2, so the map gives XCD x (the slots with
+// ws % 8 == x) one contiguous range of virtual workgroups: neighbouring rows, whose gathers touch
+// the same vector entries, then share one L2.  A bijection of [0, total), so every virtual
+// workgroup still runs exactly once, and results do not depend on it.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_vb(int ws, int total) {
+  const int q = total / kXcds, r = total % kXcds;
+  const int x = ws % kXcds, i = ws / kXcds;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + i;
+}
+
 // The instance of virtual workgroup vb, its workgroup index inside the instance (lb) and the
-// instance's workgroup count of this kind (lgrid).  vb is uniform, so are the loads.
+// instance's workgroup count of this kind (lgrid).  Instance k owns virtual workgroups
+// boff[k] .. boff[k+1]-1, so k is the number of entries of boff[1..n-1] that are <= vb.  The wave
+// counts them with independent loads (one round for up to 256 instances) and a ballot, instead of
+// a binary search whose every step waits on the previous load.  vb is uniform, so is k.
 __device__ __forceinline__ const __attribute__((address_space(1))) InstDev* seg_find(const Batch& b, int vb,
                                                                                 int& lb, int& lgrid) {
   typedef const __attribute__((address_space(1))) InstDev* GI;
+  typedef const __attribute__((address_space(1))) int* GInt;
   if (b.n == 1) {
     lb = vb;
     lgrid = b.total;
     return (GI)b.one;
   }
-  int lo = 0, hi = b.n;  // boff[lo] <= vb < boff[hi]
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (b.boff[mid] <= vb)
-      lo = mid;
-    else
-      hi = mid;
+  const GInt boff = (GInt)b.boff;
+  const int lane = threadIdx.x & 63;
+  int k = 0;
+  for (int base = 1; base < b.n; base += 256) {
+    bool le[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {  // unconditional loads (boff has n + 1 entries), one round
+      const int i = base + 64 * q + lane;
+      const int v = boff[i < b.n ? i : b.n];
+      le[q] = (i < b.n) & (v <= vb);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) k += __popcll(__ballot(le[q]));
   }
-  const int b0 = b.boff[lo];
+  const int b0 = boff[k];
   lb = vb - b0;
-  lgrid = b.boff[lo + 1] - b0;
-  return (GI)b.inst[lo];
+  lgrid = boff[k + 1] - b0;
+  typedef const InstDev* const __attribute__((address_space(1)))* GTab;
+  return (GI)((GTab)b.inst)[k];
 }
 
 // Pointers read out of an InstDev are generic to the compiler, which then emits flat memory
@@ -212,9 +238,12 @@ hipError_t launch_constant_parts(const Batch& b, int cap, hipStream_t s);
 template <typename T>
 hipError_t launch_rhs(const Batch& b, int init_cg, int cap, hipStream_t s);
 hipError_t launch_cg_init(const Batch& b, int refine, int gate, hipStream_t s);
-// pad: 1 walks every slice's padded width (uniform trip count), 0 only each row's own blocks
+// pad: 1 walks every slice's padded width (uniform trip count), 0 only each row's own blocks.
+// stream: 1 reads the matrix with non-temporal loads (for matrices larger than the Infinity
+// Cache, whose lines would only displace the gathered vectors), 0 with ordinary loads (a matrix
+// that fits stays cached from one iteration to the next).
 template <typename T>
-hipError_t launch_cg_spmv(const Batch& b, int shift, int pad, int parity, int cap, hipStream_t s,
+hipError_t launch_cg_spmv(const Batch& b, int shift, int pad, int stream, int parity, int cap, hipStream_t s,
                           hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
 template <typename T>
 hipError_t launch_cg_update(const Batch& b, int parity, int cap, hipStream_t s);

@@ -81,20 +81,26 @@ __device__ __forceinline__ void gather(const HOBAK_GLOBAL double* pa, const HOBA
 
 // The same reduction in two halves: pre_load issues the loads of up to kPre partials per thread
 // (all in flight at once, before the caller's other work), pre_sum adds them (and any beyond
-// kPre * kBlock) and reduces over the workgroup.  Summation order is fixed, as in gather().
+// kPre * kBlock) and reduces over the workgroup.  Summation order is fixed, as in gather().  TWO
+// selects one array (pa) or two (pa, pb); n >= 1, so the loads need no branch.
 constexpr int kPre = 4;
 struct PreParts {
   double a[kPre], b[kPre];
 };
+template <bool TWO>
 __device__ __forceinline__ void pre_load(const HOBAK_GLOBAL double* pa, const HOBAK_GLOBAL double* pb, int n,
                                          PreParts& q) {
 #pragma unroll
   for (int k = 0; k < kPre; k++) {
     const int i = threadIdx.x + k * kBlock;
-    q.a[k] = i < n ? pa[i] : 0.0;
-    q.b[k] = (pb && i < n) ? pb[i] : 0.0;
+    const int ic = i < n ? i : 0;
+    const double a = pa[ic];
+    const double bv = TWO ? pb[ic] : 0.0;
+    q.a[k] = i < n ? a : 0.0;
+    q.b[k] = i < n ? bv : 0.0;
   }
 }
+template <bool TWO>
 __device__ __forceinline__ void pre_sum(const HOBAK_GLOBAL double* pa, const HOBAK_GLOBAL double* pb, int n,
                                         const PreParts& q, double& ta, double& tb) {
   double v[2] = {0.0, 0.0};
@@ -105,11 +111,33 @@ __device__ __forceinline__ void pre_sum(const HOBAK_GLOBAL double* pa, const HOB
   }
   for (int i = threadIdx.x + kPre * kBlock; i < n; i += kBlock) {
     v[0] += pa[i];
-    if (pb) v[1] += pb[i];
+    if (TWO) v[1] += pb[i];
   }
   block_sum<2>(v);
   ta = v[0];
   tb = v[1];
+}
+
+// A lane's load at a 32-bit byte offset from a uniform base (plus a uniform offset): the address
+// is a scalar base and one 32-bit lane register, so loads that share the lane offset share it.
+template <typename T>
+__device__ __forceinline__ T ld_at(const HOBAK_GLOBAL T* base, unsigned off) {
+  return *(const HOBAK_GLOBAL T*)((const HOBAK_GLOBAL char*)base + off);
+}
+template <typename T>
+__device__ __forceinline__ T ld_at(const HOBAK_GLOBAL T* base, size_t uniform_off, unsigned off) {
+  return *(const HOBAK_GLOBAL T*)((const HOBAK_GLOBAL char*)base + uniform_off + off);
+}
+// The same as a streaming (non-temporal) load: data read once, which should not displace the
+// gathered vectors from the cache.
+template <typename T>
+__device__ __forceinline__ T ld_stream(const HOBAK_GLOBAL T* base, size_t uniform_off, unsigned off) {
+  return __builtin_nontemporal_load((const HOBAK_GLOBAL T*)((const HOBAK_GLOBAL char*)base + uniform_off + off));
+}
+
+template <typename T>
+__device__ __forceinline__ T d_own(bool rec, T z_own, T p_own) {
+  return rec ? z_own : p_own;
 }
 
 __device__ __forceinline__ bool nonfinite(double v) { return !(v == v) || v > DBL_MAX || v < -DBL_MAX; }
@@ -127,7 +155,8 @@ static inline int grid_of(const Batch& b, int cap) { return cap > 0 && cap < b.t
 // _positionOld = _position; _velocityOld = _velocity; applyKinematicConstraints
 // (BACKWARD_EULER_POSITION.cpp:152-157, TIMESTEPPER.cpp:105-122)
 __global__ __launch_bounds__(kBlock) void begin_step_kernel(Batch b) {
-  for (int vb = blockIdx.x; vb < b.total; vb += gridDim.x) {
+  for (int ws = blockIdx.x; ws < b.total; ws += gridDim.x) {
+    const int vb = xcd_vb(ws, b.total);
     int lb, lg;
     const HOBAK_GLOBAL InstDev* __restrict__ I = seg_find(b, vb, lb, lg);
     const int r = lb * kBlock + threadIdx.x;
@@ -151,7 +180,8 @@ __global__ __launch_bounds__(kBlock) void begin_step_kernel(Batch b) {
 // _position += xDelta (BACKWARD_EULER_POSITION.cpp:262-266)
 template <typename T>
 __global__ __launch_bounds__(kBlock) void apply_dx_kernel(Batch b, int use64) {
-  for (int vb = blockIdx.x; vb < b.total; vb += gridDim.x) {
+  for (int ws = blockIdx.x; ws < b.total; ws += gridDim.x) {
+    const int vb = xcd_vb(ws, b.total);
     int lb, lg;
     const HOBAK_GLOBAL InstDev* __restrict__ I = seg_find(b, vb, lb, lg);
     const int i = lb * kBlock + threadIdx.x;
@@ -161,7 +191,8 @@ __global__ __launch_bounds__(kBlock) void apply_dx_kernel(Batch b, int use64) {
 
 // _velocity = (u - u_old)/dt, then S-filtered (BACKWARD_EULER_POSITION.cpp:300-314)
 __global__ __launch_bounds__(kBlock) void end_step_kernel(Batch b) {
-  for (int vb = blockIdx.x; vb < b.total; vb += gridDim.x) {
+  for (int ws = blockIdx.x; ws < b.total; ws += gridDim.x) {
+    const int vb = xcd_vb(ws, b.total);
     int lb, lg;
     const HOBAK_GLOBAL InstDev* __restrict__ I = seg_find(b, vb, lb, lg);
     const int r = lb * kBlock + threadIdx.x;
@@ -213,7 +244,8 @@ __device__ __forceinline__ void write_precond(const T (&a)[9], HOBAK_GLOBAL T* m
 // the diagonal); the diagonal slot also writes its preconditioner.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void assemble_kernel(Batch b, int target) {
-  for (int vb = blockIdx.x; vb < b.total; vb += gridDim.x) {
+  for (int ws = blockIdx.x; ws < b.total; ws += gridDim.x) {
+    const int vb = xcd_vb(ws, b.total);
     int lb, lg;
     const HOBAK_GLOBAL InstDev* __restrict__ I = seg_find(b, vb, lb, lg);
     const int slot = lb * kBlock + threadIdx.x;
@@ -274,7 +306,8 @@ __global__ __launch_bounds__(kBlock) void assemble_kernel(Batch b, int target) {
 // TIMESTEPPER.cpp:232-233.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void constant_parts_kernel(Batch b) {
-  for (int vb = blockIdx.x; vb < b.total; vb += gridDim.x) {
+  for (int ws = blockIdx.x; ws < b.total; ws += gridDim.x) {
+    const int vb = xcd_vb(ws, b.total);
     int lb, lg;
     const HOBAK_GLOBAL InstDev* __restrict__ I = seg_find(b, vb, lb, lg);
     const int slot = lb * kBlock + threadIdx.x;
@@ -321,7 +354,8 @@ __device__ __forceinline__ void apply_precond(const HOBAK_GLOBAL T* minv, int pr
 // z = M^-1 r; the partial sums of |rhs|^2 and r.z seed Eigen's CG (ConjugateGradient.h:43-66).
 template <typename T>
 __global__ __launch_bounds__(kBlock) void rhs_kernel(Batch b, int init_cg) {
-  for (int vb = blockIdx.x; vb < b.total; vb += gridDim.x) {
+  for (int ws = blockIdx.x; ws < b.total; ws += gridDim.x) {
+    const int vb = xcd_vb(ws, b.total);
     int lb, lg;
     const HOBAK_GLOBAL InstDev* __restrict__ I = seg_find(b, vb, lb, lg);
     const int r = lb * kBlock + threadIdx.x;
@@ -421,7 +455,8 @@ __global__ __launch_bounds__(kBlock) void rhs_kernel(Batch b, int init_cg) {
 // (ConjugateGradient.h:45-61) and the first r.z.  With gate, only instances whose inner solve
 // has just ended and that still need another refinement sweep are (re)started.
 __global__ __launch_bounds__(kBlock) void cg_init_kernel(Batch b, int refine, int gate) {
-  for (int vb = blockIdx.x; vb < b.total; vb += gridDim.x) {
+  for (int ws = blockIdx.x; ws < b.total; ws += gridDim.x) {
+    const int vb = xcd_vb(ws, b.total);
     int lb, lg;
     const HOBAK_GLOBAL InstDev* __restrict__ I = seg_find(b, vb, lb, lg);
     auto c = ctl_of(I);
@@ -476,40 +511,40 @@ __global__ __launch_bounds__(kBlock) void cg_init_kernel(Batch b, int refine, in
 // ---------------------------------------------------------------------------------------------
 // One wavefront per H-row slice, L = 64 / H lanes per row: lane = H * kk + rr handles the row's
 // blocks k = kk, kk + L, ...; a row's L partial sums are combined with cross-lane adds.
-template <typename T, int H, bool PAD>
+template <typename T, int H, bool PAD, bool NT>
 __global__ __launch_bounds__(kBlock) void cg_spmv_kernel(Batch b, int parity) {
-  for (int vb = blockIdx.x; vb < b.total; vb += gridDim.x) {
+  for (int ws = blockIdx.x; ws < b.total; ws += gridDim.x) {
+    const int vb = xcd_vb(ws, b.total);
     int lb, lg;
     const HOBAK_GLOBAL InstDev* __restrict__ I = seg_find(b, vb, lb, lg);
     const int lane = threadIdx.x & 63;
     const int slice = (int)((lb * (unsigned)kBlock + threadIdx.x) >> 6);
     const int rr = lane % H;
     const int r = slice * H + rr;
+    // Round 1: every field of the record and of the scalar set that this kernel uses.
     const int nrows = I->nv;
-    const int nslices = (nrows + H - 1) / H;
-    const bool wave_live = slice < nslices;
-    // loads that depend only on the record go out together: the slice extent, the update
-    // kernel's partials, and the scalar set
+    const int nupd = I->nb[K_UPD];
+    const size_t ns = (size_t)I->nslots;
+    const auto slice_off = gbl(I->slice_off);
+    const auto row_len = gbl(I->row_len);
+    const auto prr = gbl(I->part_rr), prz = gbl(I->part_rz);
+    const auto p0 = gbl((T*)I->p0), p1 = gbl((T*)I->p1);
+    const auto z = gbl((const T*)I->z);
+    const auto A = gbl((const T*)I->A);
+    const auto col = gbl(I->col);
+    const auto t = gbl((T*)I->t);
+    const auto dg = gbl(I->dg);
+    const auto ppt = gbl(I->part_pt);
     auto ctl = ctl_of(I);
     const auto S = &ctl->sc[parity];
     auto N = &ctl->sc[parity ^ 1];
-    const bool lead = lb == 0 && threadIdx.x == 0;
     const int done = S->done;
-    int sbeg = 0, send = 0, len = 0;
-    PreParts q;
-    if (!done) {
-      if (wave_live) {
-        sbeg = gbl(I->slice_off)[slice];
-        send = gbl(I->slice_off)[slice + 1];
-        if (!PAD && r < nrows) len = gbl(I->row_len)[r];
-      }
-      pre_load(gbl(I->part_rr), gbl(I->part_rz), I->nb[K_UPD], q);
-    }
     const int first = S->first;
     const double absOld = S->absNew;
     const int iters = S->iters;
     const double threshold = S->threshold;
     const int maxIters = S->maxIters;
+    const bool lead = lb == 0 && threadIdx.x == 0;
     if (lead) {
       N->rhsNorm2 = S->rhsNorm2;
       N->threshold = threshold;
@@ -525,10 +560,46 @@ __global__ __launch_bounds__(kBlock) void cg_spmv_kernel(Batch b, int parity) {
       N->first = 0;
     }
     if (done) continue;
+    const int nslices = (nrows + H - 1) / H;
+    const bool wave_live = slice < nslices;
+    const bool live = r < nrows;
+    const auto pold = parity ? p1 : p0;
+    const auto pnew = parity ? p0 : p1;
+    // Round 2: the slice extent, the row length, the update kernel's partials and the row's own
+    // z and p_old, all before any of them is used.
+    // Indices are clamped instead of branched on, so these loads are issued together.
+    const int sc = wave_live ? slice : nslices - 1;
+    const int rc = live ? r : nrows - 1;
+    int sbeg = slice_off[sc];
+    int send = slice_off[sc + 1];
+    int len = PAD ? 0 : row_len[rc];
+    PreParts q;
+    pre_load<true>(prr, prz, nupd, q);
+    // The reduced-precision form (T = float) uses the difference-form operator and forms
+    // t = A p_new as A z + beta t_old (kRec): it gathers z only, where the FP64 form gathers z and
+    // p_old and forms p_new at every column, Eigen's arithmetic.
+    constexpr bool kDiff = sizeof(T) < sizeof(double);
+    constexpr bool kRec = kDiff;
+    const size_t io = 3 * (size_t)rc;
+    const auto pq = first ? z : pold;  // p_old is not read for the first iteration (beta = 0)
+    T zo0 = z[io], zo1 = z[io + 1], zo2 = z[io + 2];
+    const T qo0 = pq[io], qo1 = pq[io + 1], qo2 = pq[io + 2];
+    T to0 = T(0), to1 = T(0), to2 = T(0);
+    if (kRec) {  // t_old of the own row; used only after the first iteration
+      to0 = t[io];
+      to1 = t[io + 1];
+      to2 = t[io + 2];
+    }
+    if (!wave_live) sbeg = send = 0;
+    if (!live) {
+      len = 0;
+      zo0 = zo1 = zo2 = T(0);
+    }
+    T po0 = zo0, po1 = zo1, po2 = zo2;
     T beta = T(0);
     if (!first) {
       double rrv, rz;
-      pre_sum(gbl(I->part_rr), gbl(I->part_rz), I->nb[K_UPD], q, rrv, rz);
+      pre_sum<true>(prr, prz, nupd, q, rrv, rz);
       if (nonfinite(rrv) || !(rz == rz)) {
         if (lead) {
           N->resNorm2 = rrv;
@@ -557,72 +628,69 @@ __global__ __launch_bounds__(kBlock) void cg_spmv_kernel(Batch b, int parity) {
       }
       if (iters + 1 >= maxIters) continue;
       beta = (T)bt;
+      po0 += beta * qo0;
+      po1 += beta * qo1;
+      po2 += beta * qo2;
     }
-    auto pold = gbl((const T*)(parity ? I->p1 : I->p0));
-    auto pnew = gbl((T*)(parity ? I->p0 : I->p1));
-    auto z = gbl((const T*)I->z);
-    auto A = gbl((const T*)I->A);
-    auto col = gbl(I->col);
-    const size_t ns = (size_t)I->nslots;
-    const bool live = r < nrows;
-    // p_new = z + beta p_old at every column (p_old is not read for the first iteration).  In
-    // the difference form (T = float) columns are taken relative to the row's own p (po) and dg
-    // adds the diagonal term.  The wave walks its slice's padded width: iteration j reads the 64
-    // consecutive slots slice_off + 64 j + lane (padding slots hold zero blocks and point at the
-    // row itself), so the trip count is uniform and several iterations' loads are in flight.
-    constexpr bool kDiff = sizeof(T) < sizeof(double);
-    T po0 = T(0), po1 = T(0), po2 = T(0);
-    if (live) {
-      const size_t i = 3 * (size_t)r;
-      po0 = z[i];
-      po1 = z[i + 1];
-      po2 = z[i + 2];
-      if (!first) {
-        po0 += beta * pold[i];
-        po1 += beta * pold[i + 1];
-        po2 += beta * pold[i + 2];
-      }
-    }
+    // FP64 form: p_new = z + beta p_old at every column (p_old is not read for the first
+    // iteration).  FP32 form: A z in the difference form (columns relative to the row's own z, dg
+    // adds the diagonal term), then + beta t_old.
     T s0 = T(0), s1 = T(0), s2 = T(0);
     if (wave_live) {
       const int nit = (send - sbeg) >> 6;
       // per-lane step count: the padded width (PAD), or just the lane's own blocks of its row
       const int nj = PAD ? nit : (live ? (len - lane / H + (64 / H) - 1) / (64 / H) : 0);
-      auto cp = col + sbeg + lane;
-      auto Ap = A + sbeg + lane;
-      const T d0 = kDiff ? po0 : T(0), d1 = kDiff ? po1 : T(0), d2 = kDiff ? po2 : T(0);
+      // difference form: columns relative to the row's own entry of the vector multiplied
+      const T d0 = kRec ? zo0 : kDiff ? po0 : T(0);
+      const T d1 = kRec ? zo1 : kDiff ? po1 : T(0);
+      const T d2 = kRec ? zo2 : kDiff ? po2 : T(0);
       const T bt = first ? T(0) : beta;
-      auto pq = first ? z : pold;  // p_old is not read for the first iteration (bt = 0)
-      // two slice steps per pass: all of their loads are issued before any arithmetic
+      // Step j of this lane reads slot sbeg + lane + 64 j.  A step past the lane's own blocks
+      // re-reads step 0's slot (a cache hit, no HBM traffic) and its column values are zeroed, so
+      // every pass issues the same loads without branches.  Each pass also fetches the next
+      // pass's two column indices, so its gathers never wait on a column load.  All addresses
+      // are 32-bit byte offsets from uniform bases (the instance limits keep them in range).
+      const unsigned slot0 = (unsigned)(sbeg + lane);
+      auto slot_of = [&](int j) -> unsigned { return j < nj ? slot0 + 64u * (unsigned)j : slot0; };
+      constexpr unsigned ts = sizeof(T), vs = 3 * sizeof(T);
+      const size_t plane = ns * ts;  // bytes between value planes
+      unsigned ca = ld_at(col, slot_of(0) * 4u), cb = ld_at(col, slot_of(1) * 4u);
       for (int j = 0; j < nit; j += 2) {
+        const unsigned oa = slot_of(j) * ts, ob = slot_of(j + 1) * ts;
+        const unsigned ga = ca * vs, gb = cb * vs;
         T ma[9], mb[9];
-        T pa0 = T(0), pa1 = T(0), pa2 = T(0), pb0 = T(0), pb1 = T(0), pb2 = T(0);
 #pragma unroll
-        for (int e = 0; e < 9; e++) ma[e] = mb[e] = T(0);
-        if (j < nj) {
-          const int o0 = 64 * j;
-          const size_t ca = 3 * (size_t)cp[o0];
-#pragma unroll
-          for (int e = 0; e < 9; e++) ma[e] = Ap[e * ns + o0];
-          pa0 = (z[ca] + bt * pq[ca]) - d0;
-          pa1 = (z[ca + 1] + bt * pq[ca + 1]) - d1;
-          pa2 = (z[ca + 2] + bt * pq[ca + 2]) - d2;
+        for (int e = 0; e < 9; e++) {
+          ma[e] = NT ? ld_stream(A, (size_t)e * plane, oa) : ld_at(A, (size_t)e * plane, oa);
+          mb[e] = NT ? ld_stream(A, (size_t)e * plane, ob) : ld_at(A, (size_t)e * plane, ob);
         }
-        if (j + 1 < nj) {
-          const int o1 = 64 * (j + 1);
-          const size_t cb = 3 * (size_t)cp[o1];
-#pragma unroll
-          for (int e = 0; e < 9; e++) mb[e] = Ap[e * ns + o1];
-          pb0 = (z[cb] + bt * pq[cb]) - d0;
-          pb1 = (z[cb + 1] + bt * pq[cb + 1]) - d1;
-          pb2 = (z[cb + 2] + bt * pq[cb + 2]) - d2;
-        }
+        // (component offsets are added in 64 bits, so the three loads of a vertex are known to
+        // be adjacent and merge into wide loads)
+        const T za0 = ld_at(z, 0, ga), za1 = ld_at(z, ts, ga), za2 = ld_at(z, 2 * ts, ga);
+        const T qa0 = kRec ? T(0) : ld_at(pq, 0, ga), qa1 = kRec ? T(0) : ld_at(pq, ts, ga),
+                qa2 = kRec ? T(0) : ld_at(pq, 2 * ts, ga);
+        const T zb0 = ld_at(z, 0, gb), zb1 = ld_at(z, ts, gb), zb2 = ld_at(z, 2 * ts, gb);
+        const T qb0 = kRec ? T(0) : ld_at(pq, 0, gb), qb1 = kRec ? T(0) : ld_at(pq, ts, gb),
+                qb2 = kRec ? T(0) : ld_at(pq, 2 * ts, gb);
+        ca = ld_at(col, slot_of(j + 2) * 4u);
+        cb = ld_at(col, slot_of(j + 3) * 4u);
+        const bool va = j < nj, vb = j + 1 < nj;
+        const T pa0 = va ? (kRec ? za0 : za0 + bt * qa0) - d0 : T(0);
+        const T pa1 = va ? (kRec ? za1 : za1 + bt * qa1) - d1 : T(0);
+        const T pa2 = va ? (kRec ? za2 : za2 + bt * qa2) - d2 : T(0);
+        const T pb0 = vb ? (kRec ? zb0 : zb0 + bt * qb0) - d0 : T(0);
+        const T pb1 = vb ? (kRec ? zb1 : zb1 + bt * qb1) - d1 : T(0);
+        const T pb2 = vb ? (kRec ? zb2 : zb2 + bt * qb2) - d2 : T(0);
         s0 += ma[0] * pa0 + ma[1] * pa1 + ma[2] * pa2;
         s1 += ma[3] * pa0 + ma[4] * pa1 + ma[5] * pa2;
         s2 += ma[6] * pa0 + ma[7] * pa1 + ma[8] * pa2;
         s0 += mb[0] * pb0 + mb[1] * pb1 + mb[2] * pb2;
         s1 += mb[3] * pb0 + mb[4] * pb1 + mb[5] * pb2;
         s2 += mb[6] * pb0 + mb[7] * pb1 + mb[8] * pb2;
+        // The prefetched indices are consumed here, at the end of the pass: otherwise the
+        // compiler merges this load with the one before the loop and issues it at the top of the
+        // next pass, where the gathers would wait for it.
+        asm volatile("" : "+v"(ca), "+v"(cb));
       }
     }
 #pragma unroll
@@ -635,20 +703,38 @@ __global__ __launch_bounds__(kBlock) void cg_spmv_kernel(Batch b, int parity) {
     if (live && lane < H) {
       const size_t i = 3 * (size_t)r;
       if (kDiff) {
-        s0 += (T)gbl(I->dg)[i] * po0;
-        s1 += (T)gbl(I->dg)[i + 1] * po1;
-        s2 += (T)gbl(I->dg)[i + 2] * po2;
+        s0 += (T)dg[i] * d_own(kRec, zo0, po0);
+        s1 += (T)dg[i + 1] * d_own(kRec, zo1, po1);
+        s2 += (T)dg[i + 2] * d_own(kRec, zo2, po2);
       }
-      auto t = gbl((T*)I->t);
-      t[i] = s0;
-      t[i + 1] = s1;
-      t[i + 2] = s2;
-      pnew[i] = po0;
-      pnew[i + 1] = po1;
-      pnew[i + 2] = po2;
+      if (kRec && !first) {
+        s0 += beta * to0;
+        s1 += beta * to1;
+        s2 += beta * to2;
+      }
       pt = (double)po0 * (double)s0 + (double)po1 * (double)s1 + (double)po2 * (double)s2;
     }
-    publish(pt, 0.0, gbl(I->part_pt), nullptr, lb);
+    // Lane rho < H now holds row slice*H + rho.  The slice's rows own the 3H consecutive entries
+    // of t and p_new from 3*slice*H, and lane l stores entries l, l + 64, ...: every store
+    // instruction writes whole lines instead of three strided partial-line stores per row.
+    if (wave_live) {
+      const size_t base = 3 * (size_t)slice * H;
+      const int nval = 3 * min(H, nrows - slice * H);
+#pragma unroll
+      for (int d0 = 0; d0 < 3 * H; d0 += 64) {
+        const int d = d0 + lane;
+        const int rho = d / 3;
+        const int comp = d - 3 * rho;
+        const int src = rho < H ? rho : 0;
+        const T v0 = __shfl(s0, src, 64), v1 = __shfl(s1, src, 64), v2 = __shfl(s2, src, 64);
+        const T q0 = __shfl(po0, src, 64), q1 = __shfl(po1, src, 64), q2 = __shfl(po2, src, 64);
+        if (d < nval) {
+          t[base + d] = comp == 0 ? v0 : comp == 1 ? v1 : v2;
+          pnew[base + d] = comp == 0 ? q0 : comp == 1 ? q1 : q2;
+        }
+      }
+    }
+    publish(pt, 0.0, ppt, nullptr, lb);
   }
 }
 
@@ -659,7 +745,8 @@ __global__ __launch_bounds__(kBlock) void cg_spmv_kernel(Batch b, int parity) {
 // first wave's vectors are loaded before the p.t partials are reduced, so the two overlap.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void cg_update_kernel(Batch b, int parity) {
-  for (int vb = blockIdx.x; vb < b.total; vb += gridDim.x) {
+  for (int ws = blockIdx.x; ws < b.total; ws += gridDim.x) {
+    const int vb = xcd_vb(ws, b.total);
     int lb, lg;
     const HOBAK_GLOBAL InstDev* __restrict__ I = seg_find(b, vb, lb, lg);
     const auto S = &ctl_of(I)->sc[parity ^ 1];
@@ -691,9 +778,9 @@ __global__ __launch_bounds__(kBlock) void cg_update_kernel(Batch b, int parity) 
       }
     }
     PreParts qp;
-    pre_load(gbl(I->part_pt), nullptr, I->nb[K_SPMV], qp);
+    pre_load<false>(gbl(I->part_pt), nullptr, I->nb[K_SPMV], qp);
     double pt, unused;
-    pre_sum(gbl(I->part_pt), nullptr, I->nb[K_SPMV], qp, pt, unused);
+    pre_sum<false>(gbl(I->part_pt), nullptr, I->nb[K_SPMV], qp, pt, unused);
     const T alpha = (T)(absNew / pt);
     double rr2 = 0.0, rz = 0.0;
     for (int wave = w0; wave < nwaves; wave += stride) {
@@ -742,7 +829,8 @@ __global__ __launch_bounds__(kBlock) void cg_update_kernel(Batch b, int parity) 
 // of PCG iterations of a batch and act only on instances whose inner solve has just ended.
 // ---------------------------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void refine_begin_kernel(Batch b) {
-  for (int vb = blockIdx.x; vb < b.total; vb += gridDim.x) {
+  for (int ws = blockIdx.x; ws < b.total; ws += gridDim.x) {
+    const int vb = xcd_vb(ws, b.total);
     int lb, lg;
     const HOBAK_GLOBAL InstDev* __restrict__ I = seg_find(b, vb, lb, lg);
     const int r = lb * kBlock + threadIdx.x;
@@ -765,6 +853,7 @@ __global__ __launch_bounds__(kBlock) void refine_begin_kernel(Batch b) {
       st->tau = I->tau0;
       st->sweeps = 0;
       st->inner_iters = 0;
+      st->first_iters = 0;
       st->status = bad ? 2 : 0;
       st->done = (b2 == 0.0 || bad) ? 1 : 0;
     }
@@ -773,7 +862,8 @@ __global__ __launch_bounds__(kBlock) void refine_begin_kernel(Batch b) {
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void refine_init_kernel(Batch b, int gate) {
-  for (int vb = blockIdx.x; vb < b.total; vb += gridDim.x) {
+  for (int ws = blockIdx.x; ws < b.total; ws += gridDim.x) {
+    const int vb = xcd_vb(ws, b.total);
     int lb, lg;
     const HOBAK_GLOBAL InstDev* __restrict__ I = seg_find(b, vb, lb, lg);
     if (gate && !solve_ended(ctl_of(I))) continue;
@@ -801,7 +891,8 @@ __global__ __launch_bounds__(kBlock) void refine_init_kernel(Batch b, int gate) 
 // One wavefront per H-row slice as in the SpMV; x64 + x is formed on the fly for every column.
 template <typename T, int H>
 __global__ __launch_bounds__(kBlock) void refine_residual_kernel(Batch b) {
-  for (int vb = blockIdx.x; vb < b.total; vb += gridDim.x) {
+  for (int ws = blockIdx.x; ws < b.total; ws += gridDim.x) {
+    const int vb = xcd_vb(ws, b.total);
     int lb, lg;
     const HOBAK_GLOBAL InstDev* __restrict__ I = seg_find(b, vb, lb, lg);
     const auto ctl = ctl_of(I);
@@ -877,7 +968,8 @@ __global__ __launch_bounds__(kBlock) void refine_residual_kernel(Batch b) {
 // on-the-fly sums above race-free.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void refine_accumulate_kernel(Batch b) {
-  for (int vb = blockIdx.x; vb < b.total; vb += gridDim.x) {
+  for (int ws = blockIdx.x; ws < b.total; ws += gridDim.x) {
+    const int vb = xcd_vb(ws, b.total);
     int lb, lg;
     const HOBAK_GLOBAL InstDev* __restrict__ I = seg_find(b, vb, lb, lg);
     const auto ctl = ctl_of(I);
@@ -892,7 +984,8 @@ __global__ __launch_bounds__(kBlock) void refine_accumulate_kernel(Batch b) {
 // inner tolerance aims at the outer target with a 5x margin (0.2 tol |b| / |res|), floored at
 // 1e-6 (FP32's reach) and capped at the first inner tolerance.
 __global__ __launch_bounds__(kBlock) void finish_check_kernel(Batch b, int refine) {
-  for (int vb = blockIdx.x; vb < b.total; vb += gridDim.x) {
+  for (int ws = blockIdx.x; ws < b.total; ws += gridDim.x) {
+    const int vb = xcd_vb(ws, b.total);
     int lb, lg;
     const HOBAK_GLOBAL InstDev* __restrict__ I = seg_find(b, vb, lb, lg);
     auto c = ctl_of(I);
@@ -915,6 +1008,7 @@ __global__ __launch_bounds__(kBlock) void finish_check_kernel(Batch b, int refin
     st->res2 = res2;
     st->sweeps = sweeps;
     st->inner_iters += c->sc[0].iters;
+    if (sweeps == 1) st->first_iters = c->sc[0].iters;
     int done = 0;
     if (nonfinite(res2) || c->sc[0].status == 2) {
       done = 1;
@@ -936,7 +1030,8 @@ __global__ __launch_bounds__(kBlock) void finish_check_kernel(Batch b, int refin
 
 // One lane per instance of the batch: what the host needs to schedule and report.
 __global__ __launch_bounds__(kBlock) void status_kernel(Batch b, InstStatus* out) {
-  for (int vb = blockIdx.x; vb < b.total; vb += gridDim.x) {
+  for (int ws = blockIdx.x; ws < b.total; ws += gridDim.x) {
+    const int vb = xcd_vb(ws, b.total);
     int lb, lg;
     const HOBAK_GLOBAL InstDev* __restrict__ I = seg_find(b, vb, lb, lg);
     if (threadIdx.x != 0) continue;
@@ -950,7 +1045,7 @@ __global__ __launch_bounds__(kBlock) void status_kernel(Batch b, InstStatus* out
     o.cg_status = s->status;
     o.ref_status = c->rs.status;
     o.ref_done = c->rs.done;
-    o.pad = 0;
+    o.first_iters = I->refine ? c->rs.first_iters : s->iters;
     if (I->refine) {
       o.err = c->rs.b2 > 0 ? sqrt(c->rs.res2 / c->rs.b2) : 0.0;
       o.maxR = c->rs.b2;
@@ -966,7 +1061,8 @@ __global__ __launch_bounds__(kBlock) void status_kernel(Batch b, InstStatus* out
 // (TET_MESH.cpp:756-767); output in FP64 into rhs64.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void force_gather_kernel(Batch b) {
-  for (int vb = blockIdx.x; vb < b.total; vb += gridDim.x) {
+  for (int ws = blockIdx.x; ws < b.total; ws += gridDim.x) {
+    const int vb = xcd_vb(ws, b.total);
     int lb, lg;
     const HOBAK_GLOBAL InstDev* __restrict__ I = seg_find(b, vb, lb, lg);
     const int r = lb * kBlock + threadIdx.x;
@@ -1042,31 +1138,34 @@ hipError_t launch_cg_init(const Batch& b, int refine, int gate, hipStream_t s) {
 // With events, the launch carries them in its own dispatch (kernel start and end), so the
 // measured interval is the kernel's execution, not the queue's.  All instances of a batch share
 // one slice height (1 << shift rows).
-template <typename T, int H, bool PAD>
+template <typename T, int H, bool PAD, bool NT>
 static hipError_t spmv_hp(const Batch& b, int parity, int cap, hipStream_t s, hipEvent_t start, hipEvent_t stop) {
   const dim3 g(grid_of(b, cap)), blk(kBlock);
   if (start)
-    hipExtLaunchKernelGGL((cg_spmv_kernel<T, H, PAD>), g, blk, 0, s, start, stop, 0, b, parity);
+    hipExtLaunchKernelGGL((cg_spmv_kernel<T, H, PAD, NT>), g, blk, 0, s, start, stop, 0, b, parity);
   else
-    hipLaunchKernelGGL((cg_spmv_kernel<T, H, PAD>), g, blk, 0, s, b, parity);
+    hipLaunchKernelGGL((cg_spmv_kernel<T, H, PAD, NT>), g, blk, 0, s, b, parity);
   return hipGetLastError();
 }
 template <typename T, int H>
-static hipError_t spmv_h(const Batch& b, int parity, int cap, int pad, hipStream_t s, hipEvent_t start,
-                         hipEvent_t stop) {
-  return pad ? spmv_hp<T, H, true>(b, parity, cap, s, start, stop)
-             : spmv_hp<T, H, false>(b, parity, cap, s, start, stop);
+static hipError_t spmv_h(const Batch& b, int parity, int cap, int pad, int stream, hipStream_t s,
+                         hipEvent_t start, hipEvent_t stop) {
+  if (stream)
+    return pad ? spmv_hp<T, H, true, true>(b, parity, cap, s, start, stop)
+               : spmv_hp<T, H, false, true>(b, parity, cap, s, start, stop);
+  return pad ? spmv_hp<T, H, true, false>(b, parity, cap, s, start, stop)
+             : spmv_hp<T, H, false, false>(b, parity, cap, s, start, stop);
 }
 
 template <typename T>
-hipError_t launch_cg_spmv(const Batch& b, int shift, int pad, int parity, int cap, hipStream_t s,
+hipError_t launch_cg_spmv(const Batch& b, int shift, int pad, int stream, int parity, int cap, hipStream_t s,
                           hipEvent_t start, hipEvent_t stop) {
   if (b.total <= 0) return hipSuccess;
   switch (shift) {
-    case 3: return spmv_h<T, 8>(b, parity, cap, pad, s, start, stop);
-    case 4: return spmv_h<T, 16>(b, parity, cap, pad, s, start, stop);
-    case 5: return spmv_h<T, 32>(b, parity, cap, pad, s, start, stop);
-    case 6: return spmv_h<T, 64>(b, parity, cap, pad, s, start, stop);
+    case 3: return spmv_h<T, 8>(b, parity, cap, pad, stream, s, start, stop);
+    case 4: return spmv_h<T, 16>(b, parity, cap, pad, stream, s, start, stop);
+    case 5: return spmv_h<T, 32>(b, parity, cap, pad, stream, s, start, stop);
+    case 6: return spmv_h<T, 64>(b, parity, cap, pad, stream, s, start, stop);
     default: return hipErrorInvalidValue;
   }
 }
@@ -1115,7 +1214,7 @@ hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s) 
   template hipError_t launch_assemble<T>(const Batch&, int, int, hipStream_t);               \
   template hipError_t launch_constant_parts<T>(const Batch&, int, hipStream_t);              \
   template hipError_t launch_rhs<T>(const Batch&, int, int, hipStream_t);                    \
-  template hipError_t launch_cg_spmv<T>(const Batch&, int, int, int, int, hipStream_t, hipEvent_t, hipEvent_t); \
+  template hipError_t launch_cg_spmv<T>(const Batch&, int, int, int, int, int, hipStream_t, hipEvent_t, hipEvent_t); \
   template hipError_t launch_cg_update<T>(const Batch&, int, int, hipStream_t);              \
   template hipError_t launch_refine_init<T>(const Batch&, int, int, hipStream_t);            \
   template hipError_t launch_refine_residual<T>(const Batch&, const Batch&, int, int, hipStream_t); \

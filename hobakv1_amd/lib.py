@@ -124,6 +124,7 @@ TUNE_GRAPHS = 4
 TUNE_GRID_CAP = 5
 TUNE_UPDATE_WGS = 6
 TUNE_SPMV_PAD = 7
+TUNE_SPMV_STREAM_MB = 8
 
 _lib = None
 
@@ -172,11 +173,13 @@ class Context:
     """One device + stream + HBM budget (0 = unlimited).
 
     apply_table: apply the autotuned launch shapes of tuning.json (hobakv1_amd/autotune.py) for
-    `precision` to the instances this context creates.
+    `precision` and `workload` to the instances this context creates.  workload "single" is one
+    instance stepped at a time (tuned on the 40^3 scene), "batch" many instances stepped together
+    (tuned on the 256-instance batch); a missing batch entry falls back to the single one.
     """
 
     def __init__(self, device: int = 0, budget_bytes: int = 0, apply_table: bool = True,
-                 precision: str = "fp64"):
+                 precision: str = "fp64", workload: str = "single"):
         self.lib = load()
         h = _vp()
         rc = self.lib.hobak_ctx_create(device, budget_bytes, ctypes.byref(h))
@@ -185,21 +188,23 @@ class Context:
         self.h = h
         self.refine_inner_tol = None
         if apply_table:
-            self.apply_tuning_table(precision)
+            self.apply_tuning_table(precision, workload)
 
-    def apply_tuning_table(self, precision="fp64"):
+    def apply_tuning_table(self, precision="fp64", workload="single"):
         path = os.path.join(HERE, "tuning.json")
         if not os.path.exists(path):
             return
         import json
 
         with open(path) as f:
-            entry = json.load(f).get(precision)
+            table = json.load(f)
+        entry = table.get(f"{precision}_batch") if workload == "batch" else None
+        entry = entry or table.get(precision)
         if entry:
             self.set_tuning(slice_rows=entry.get("slice_rows"), poll_chunk=entry.get("poll_chunk"),
                             max_chunks=entry.get("max_chunks"), graphs=entry.get("graphs"),
                             grid_cap=entry.get("grid_cap"), update_wgs=entry.get("update_wgs"),
-                            spmv_pad=entry.get("spmv_pad"))
+                            spmv_pad=entry.get("spmv_pad"), spmv_stream_mb=entry.get("spmv_stream_mb"))
             self.refine_inner_tol = entry.get("refine_inner_tol")
 
     def error(self) -> str:
@@ -211,12 +216,12 @@ class Context:
         return rc
 
     def set_tuning(self, slice_rows=None, poll_chunk=None, max_chunks=None, graphs=None,
-                   grid_cap=None, update_wgs=None, spmv_pad=None):
+                   grid_cap=None, update_wgs=None, spmv_pad=None, spmv_stream_mb=None):
         """Kernel and engine tuning (see hobak_ctx_set_tuning)."""
         for key, val in ((TUNE_SLICE_ROWS, slice_rows), (TUNE_POLL_CHUNK, poll_chunk),
                          (TUNE_MAX_CHUNKS, max_chunks), (TUNE_GRAPHS, graphs),
                          (TUNE_GRID_CAP, grid_cap), (TUNE_UPDATE_WGS, update_wgs),
-                         (TUNE_SPMV_PAD, spmv_pad)):
+                         (TUNE_SPMV_PAD, spmv_pad), (TUNE_SPMV_STREAM_MB, spmv_stream_mb)):
             if val is not None:
                 self.check(self.lib.hobak_ctx_set_tuning(self.h, key, int(val)), "set_tuning")
 

@@ -150,7 +150,8 @@ int hobak_last_step_timing(hobak_inst* inst, double* out_ms);
 /* Measurement hooks (no counterpart in the reference).
  * hobak_set_kernel_sampling: every `every`-th PCG iteration of the following steps brackets its
  * SpMV launch with HIP events on the instance's stream (0 turns sampling off and clears the
- * accumulators).  hobak_kernel_stats: average duration (ms) and count of the sampled SpMV
+ * accumulators).  A group of PCG iterations that holds a sampled launch runs uncaptured; the
+ * other groups replay their graph.  hobak_kernel_stats: average duration (ms) and count of the sampled SpMV
  * launches that did work, and the SpMV's algorithmic bytes per launch
  * nnzb * (9 s + 4) + 4 (nv + 1) + 2 * 3 s * nv  (s = bytes per real). */
 int hobak_set_kernel_sampling(hobak_inst* inst, int32_t every);
@@ -169,7 +170,12 @@ int hobak_kernel_stats(hobak_inst* inst, double* spmv_avg_ms, int64_t* spmv_coun
  *                          partial sum the next SpMV re-reduces); applies to instances created
  *                          afterwards
  *   HOBAK_TUNE_SPMV_PAD    1: the SpMV walks every slice's padded width with a uniform trip
- *                          count; 0: each lane walks only its own row's blocks */
+ *                          count; 0: each lane walks only its own row's blocks
+ *   HOBAK_TUNE_SPMV_STREAM_MB  a batch holding an instance whose SpMV moves more than this many
+ *                          MiB per launch reads its matrix with non-temporal loads (default 192: a
+ *                          matrix that fits the 256 MB Infinity Cache stays cached across
+ *                          iterations; a larger one would only displace the gathered vectors);
+ *                          0 streams always */
 #define HOBAK_TUNE_SLICE_ROWS 0
 #define HOBAK_TUNE_POLL_CHUNK 2
 #define HOBAK_TUNE_MAX_CHUNKS 3
@@ -177,6 +183,7 @@ int hobak_kernel_stats(hobak_inst* inst, double* spmv_avg_ms, int64_t* spmv_coun
 #define HOBAK_TUNE_GRID_CAP 5
 #define HOBAK_TUNE_UPDATE_WGS 6
 #define HOBAK_TUNE_SPMV_PAD 7
+#define HOBAK_TUNE_SPMV_STREAM_MB 8
 int hobak_ctx_set_tuning(hobak_ctx* ctx, int32_t key, int32_t value);
 
 /* Continuous-batching scheduler (no counterpart in the reference, whose callers step one scene

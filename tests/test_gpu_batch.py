@@ -89,6 +89,23 @@ def test_batch_persistent_grid_matches():
         assert np.array_equal(sa[0], sb[0]) and np.array_equal(sa[1], sb[1])
 
 
+@pytest.mark.parametrize("precision", [lib.PRECISION_FP64, lib.PRECISION_FP32_REFINE])
+def test_streamed_matrix_loads_match(precision):
+    """Non-temporal matrix loads (chosen for batches larger than the Infinity Cache; forced here
+    with a zero threshold) read the same values: bitwise the same trajectory."""
+    scs = _mixed_scenes(k=3, seed=24)
+    res = []
+    for mb in (192, 0):
+        ctx = lib.Context(0, precision="fp64" if precision == lib.PRECISION_FP64 else "fp32")
+        ctx.set_tuning(spmv_stream_mb=mb)
+        _say(f"stream threshold {mb} MiB, precision {precision}")
+        res.append(_run_batched(ctx, scs, 2, precision))
+        ctx.close()
+    for (sa, ia), (sb, ib) in zip(res[0], res[1]):
+        assert ia == ib
+        assert np.array_equal(sa[0], sb[0]) and np.array_equal(sa[1], sb[1])
+
+
 def test_batch_fp64_matches_oracle(gpu_ctx):
     scs = _mixed_scenes(k=4, seed=24)
     insts = [lib.Instance(gpu_ctx, sc) for sc in scs]
