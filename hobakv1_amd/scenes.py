@@ -181,6 +181,7 @@ class Scene:
     newton_iters: int = 1
     ground_y: Optional[float] = None   # penalty ground plane (no reference counterpart)
     ground_k: float = 0.0
+    collision_cubes: tuple = ()        # ((cx, cy, cz, scale), ...) kinematic collision CUBEs
     dims: tuple = ()
 
     @property
@@ -247,6 +248,28 @@ def lattice_scene(n=(10, 10, 10), seed=1, E=1e5, nu=0.45, material=MAT_SNH, rho=
         ids = sc.pinned()
         v[ids] = 0.0
         sc.v0 = v.ravel().copy()
+    return sc
+
+
+def drop_scene(n=(6, 6, 6), seed=7, E=3.0, nu=0.48, material=MAT_SNH, height=0.02,
+               speed=3.0, ground=(0.0, -5.0, 0.0, 10.0), name=None):
+    """A lattice dropped onto a kinematic ground CUBE, as src/Scenes/DROP_TEST.h does with its
+    cylinder: SNH E=3 nu=0.48, density 1, gravity (0, -1, 0), every vertex moving down at 3, the
+    ground CUBE(center (0, -5, 0), scale 10) whose top face is y = 0.  The unit lattice starts
+    `height` above the ground; its surface vertices become plane constraints on contact
+    (TIMESTEPPER.cpp:446-526).  No vertex is pinned."""
+    nx, ny, nz = n
+    h = 1.0 / max(n)
+    verts, tets = kuhn_lattice(nx, ny, nz, h)
+    verts = jitter(verts, nx, ny, nz, h, seed)
+    verts = verts + np.array([-0.5 * nx * h, height, -0.5 * nz * h])[None, :]
+    m, l = lame(E, nu)
+    sc = Scene(name=name or f"drop{nx}x{ny}x{nz}", rest=verts, tets=tets, material=material,
+               mu=m, lam=l, mass_scale=1.0, gravity=(0.0, -1.0, 0.0), pin_cube=None,
+               collision_cubes=(tuple(float(x) for x in ground),), dims=(nx, ny, nz))
+    v = np.zeros((verts.shape[0], 3))
+    v[:, 1] = -speed
+    sc.v0 = v.ravel().copy()
     return sc
 
 

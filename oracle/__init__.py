@@ -67,6 +67,9 @@ def load_ref():
         lib.ref_step.argtypes = [ctypes.c_void_p, _dp, _dp]
         lib.ref_step.restype = ctypes.c_int
         lib.ref_last_rhs_b.argtypes = [ctypes.c_void_p, _dp]
+        lib.ref_add_collision_cube.argtypes = [ctypes.c_void_p, _dp, ctypes.c_double]
+        lib.ref_plane_constraints.argtypes = [ctypes.c_void_p, _ip]
+        lib.ref_plane_constraints.restype = ctypes.c_int
         lib.ref_mass_diag.argtypes = [ctypes.c_void_p, _dp]
         lib.ref_mesh_nnz.argtypes = [ctypes.c_void_p]
         lib.ref_mesh_nnz.restype = ctypes.c_int
@@ -142,6 +145,9 @@ class RefSim:
             self.h = self.lib.ref_create_bdf1(*args, ni, newton_tol)
         else:
             self.h = self.lib.ref_create(*args)
+        for cube in getattr(scene, "collision_cubes", ()):
+            c = _c64(np.asarray(cube[:3]))
+            self.lib.ref_add_collision_cube(self.h, _d(c), float(cube[3]))
         self.dofs = scene.dofs
         self.set_state(scene.initial_u(), scene.initial_v())
 
@@ -181,6 +187,13 @@ class RefSim:
         b = np.zeros(self.dofs)
         self.lib.ref_last_rhs_b(self.h, _d(b))
         return b
+
+    def plane_constraints(self):
+        """Vertex ids of the live plane constraints, in the integrator's order."""
+        n = self.lib.ref_plane_constraints(self.h, None)
+        out = np.zeros(max(n, 1), np.int32)
+        self.lib.ref_plane_constraints(self.h, _i(out))
+        return out[:n]
 
     def mass_diag(self):
         m = np.zeros(self.dofs)
@@ -229,6 +242,11 @@ def load_oracle():
         lib.orc_set_state.argtypes = [ctypes.c_void_p, _dp, _dp]
         lib.orc_get_state.argtypes = [ctypes.c_void_p, _dp, _dp]
         lib.orc_set_ground.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
+        lib.orc_add_collision_cube.argtypes = [ctypes.c_void_p, _dp, ctypes.c_double]
+        lib.orc_plane_constraints.argtypes = [ctypes.c_void_p, _ip]
+        lib.orc_plane_constraints.restype = ctypes.c_int
+        lib.orc_surface_vertices.argtypes = [ctypes.c_void_p, _ip]
+        lib.orc_surface_vertices.restype = ctypes.c_int
         lib.orc_step.argtypes = [ctypes.c_void_p, _dp, _dp, _ip, _ip]
         lib.orc_step.restype = ctypes.c_int
         lib.orc_nnzb.argtypes = [ctypes.c_void_p]
@@ -312,6 +330,9 @@ class Oracle:
                                      precond, ni, newton_tol)
         if scene.ground_y is not None:
             self.lib.orc_set_ground(self.h, scene.ground_y, scene.ground_k)
+        for cube in getattr(scene, "collision_cubes", ()):
+            c = _c64(np.asarray(cube[:3]))
+            self.lib.orc_add_collision_cube(self.h, _d(c), float(cube[3]))
         self.dofs = scene.dofs
         self.set_state(scene.initial_u(), scene.initial_v())
 
@@ -344,6 +365,19 @@ class Oracle:
                                ctypes.byref(nd))
         self.last_iters_per_newton = per[:nd.value].copy()
         return it, err.value, sec.value
+
+    def plane_constraints(self):
+        """Vertex ids of the live plane constraints, in creation order."""
+        n = self.lib.orc_plane_constraints(self.h, None)
+        out = np.zeros(max(n, 1), np.int32)
+        self.lib.orc_plane_constraints(self.h, _i(out))
+        return out[:n]
+
+    def surface_vertices(self):
+        n = self.lib.orc_surface_vertices(self.h, None)
+        out = np.zeros(max(n, 1), np.int32)
+        self.lib.orc_surface_vertices(self.h, _i(out))
+        return out[:n]
 
     def bsr_pattern(self):
         nnzb = self.lib.orc_nnzb(self.h)

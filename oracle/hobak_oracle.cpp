@@ -18,6 +18,8 @@
 #include <cstring>
 #include <limits>
 #include <vector>
+#include <map>
+#include <array>
 
 namespace {
 
@@ -549,6 +551,91 @@ double det_sum(int n, Fn fn) {
   return s;
 }
 
+// A kinematic collision CUBE (src/Geometry/CUBE.cpp:23-193, KINEMATIC_SHAPE.h:55-73): world =
+// R S local + t with S = scale I; the box is [-0.5, 0.5]^3 in local coordinates.  The products
+// follow Eigen's left-to-right evaluation: worldVertexToLocal = (S^-1 R^T)(w - t) with S^-1 from
+// Eigen's 3x3 cofactor inverse, localVertexToWorld = (R S) l + t.
+struct Cube {
+  double R[3][3];
+  double s, sinv;
+  double t[3];
+  Cube(const double* center, double scale) {
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) R[i][j] = i == j ? 1.0 : 0.0;
+    s = scale;
+    sinv = (scale * scale) * (1.0 / ((scale * scale) * scale));
+    for (int i = 0; i < 3; i++) t[i] = center[i];
+  }
+  void toLocal(const double* w, double* l) const {
+    const double d[3] = {w[0] - t[0], w[1] - t[1], w[2] - t[2]};
+    for (int i = 0; i < 3; i++) {
+      const double m0 = sinv * R[0][i], m1 = sinv * R[1][i], m2 = sinv * R[2][i];
+      l[i] = (m0 * d[0] + m1 * d[1]) + m2 * d[2];
+    }
+  }
+  void toWorld(const double* l, double* w) const {
+    for (int i = 0; i < 3; i++)
+      w[i] = ((R[i][0] * s) * l[0] + (R[i][1] * s) * l[1]) + (R[i][2] * s) * l[2] + t[i];
+  }
+  void normalToWorld(const double* n, double* w) const {
+    for (int i = 0; i < 3; i++) w[i] = (R[i][0] * n[0] + R[i][1] * n[1]) + R[i][2] * n[2];
+  }
+  static bool insideLocal(const double* l) {
+    return l[0] <= 0.5 && l[0] >= -0.5 && l[1] <= 0.5 && l[1] >= -0.5 && l[2] <= 0.5 && l[2] >= -0.5;
+  }
+  bool inside(const double* w) const {  // CUBE.cpp:37-48
+    double l[3];
+    toLocal(w, l);
+    return insideLocal(l);
+  }
+  double signedDistance(const double* w) const {  // CUBE.cpp:90-119
+    double l[3];
+    toLocal(w, l);
+    if (insideLocal(l)) {
+      const double xm = std::min(0.5 - l[0], l[0] - (-0.5));
+      const double ym = std::min(0.5 - l[1], l[1] - (-0.5));
+      const double zm = std::min(0.5 - l[2], l[2] - (-0.5));
+      return -std::fabs(std::min(std::min(xm, ym), zm)) * s;
+    }
+    double d[3] = {0, 0, 0};
+    for (int k = 0; k < 3; k++) {
+      if (l[k] > 0.5)
+        d[k] = l[k] - 0.5;
+      else if (l[k] < -0.5)
+        d[k] = -0.5 - l[k];
+    }
+    return std::sqrt((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]) * s;
+  }
+  // CUBE.cpp:124-191: the nearest face (first minimum of the six distances) in local coordinates
+  void closest(const double* w, double* cl, double* nl) const {
+    double l[3];
+    toLocal(w, l);
+    const double diffs[6] = {0.5 + l[0], 0.5 - l[0], 0.5 + l[1], 0.5 - l[1], 0.5 + l[2], 0.5 - l[2]};
+    int mi = 0;
+    double mf = diffs[0];
+    for (int x = 1; x < 6; x++)
+      if (diffs[x] < mf) {
+        mf = diffs[x];
+        mi = x;
+      }
+    const int axis = mi / 2;
+    const double sign = (mi % 2) ? 1.0 : -1.0;
+    for (int k = 0; k < 3; k++) {
+      cl[k] = l[k];
+      nl[k] = 0.0;
+    }
+    cl[axis] = 0.5 * sign;
+    nl[axis] = sign;
+  }
+};
+
+// A Baraff-Witkin plane constraint (src/Geometry/CONSTRAINTS.h PLANE_CONSTRAINT)
+struct Plane {
+  int obj, vertex;
+  double cl[3], nl[3];  // closest point and normal, in the object's local frame
+  bool sep;
+};
+
 struct Sim {
   int nv = 0, nt = 0, dofs = 0;
   std::vector<double> rest;  // nv*3
@@ -581,6 +668,15 @@ struct Sim {
   std::vector<double> Krest;  // nnzb*9
   std::vector<double> u, v;   // state
 
+  // kinematic collision objects and their plane constraints (TIMESTEPPER.h:178-187)
+  std::vector<Cube> cubes;
+  std::vector<int> surface;   // surface vertices, ascending (TET_MESH.cpp:567-593)
+  std::vector<char> inColl;   // per vertex
+  std::vector<Plane> planes;  // in creation order
+  void computeSurface();
+  // per-vertex constraint filter S: 0 free (I), 1 plane (I - n n^T), 2 pinned (0)
+  void buildFilter(std::vector<char>& mode, std::vector<double>& nrm) const;
+
   // last system (for tests)
   std::vector<double> lastRhs, lastLhs;
 
@@ -594,6 +690,43 @@ struct Sim {
           double& err) const;
   int step(double* err, int* itersPerNewton, int* newtonDone);
 };
+
+// Surface vertices: vertices of tet faces that no other tet shares (TET_MESH.cpp:294-354,
+// 567-593; faces (0,1,3), (0,2,1), (0,3,2), (1,2,3) of each tet).
+void Sim::computeSurface() {
+  std::map<std::array<int, 3>, int> count;
+  const int F[4][3] = {{0, 1, 3}, {0, 2, 1}, {0, 3, 2}, {1, 2, 3}};
+  for (int t = 0; t < nt; t++)
+    for (int f = 0; f < 4; f++) {
+      std::array<int, 3> k = {tets[4 * t + F[f][0]], tets[4 * t + F[f][1]], tets[4 * t + F[f][2]]};
+      std::sort(k.begin(), k.end());
+      count[k]++;
+    }
+  std::vector<char> on(nv, 0);
+  for (auto& kv : count)
+    if (kv.second < 2)
+      for (int c = 0; c < 3; c++) on[kv.first[c]] = 1;
+  surface.clear();
+  for (int i = 0; i < nv; i++)
+    if (on[i]) surface.push_back(i);
+  inColl.assign(nv, 0);
+}
+
+// TIMESTEPPER::buildConstraintMatrix (TIMESTEPPER.cpp:128-165): I - n n^T per live plane
+// constraint (n normalized), then the kinematic (pinned) vertices set to zero, last.
+void Sim::buildFilter(std::vector<char>& mode, std::vector<double>& nrm) const {
+  mode.assign(nv, 0);
+  nrm.assign(3 * (size_t)nv, 0.0);
+  for (const Plane& p : planes) {
+    if (p.sep) continue;
+    double n[3];
+    cubes[p.obj].normalToWorld(p.nl, n);
+    const double len = std::sqrt((n[0] * n[0] + n[1] * n[1]) + n[2] * n[2]);
+    for (int c = 0; c < 3; c++) nrm[3 * (size_t)p.vertex + c] = n[c] / len;
+    mode[p.vertex] = 1;
+  }
+  for (int k : pinned) mode[k] = 2;
+}
 
 void Sim::setup() {
   dofs = 3 * nv;
@@ -838,6 +971,13 @@ int Sim::pcg(const std::vector<double>& lhs, const std::vector<double>& b, std::
   return it;
 }
 
+// The 3x3 block of the constraint filter S at one vertex: I, I - n n^T (TIMESTEPPER.cpp:149) or 0.
+static void filterBlock(int mode, const double* n, double S[3][3]) {
+  for (int a = 0; a < 3; a++)
+    for (int c = 0; c < 3; c++)
+      S[a][c] = mode == 2 ? 0.0 : mode == 1 ? (a == c ? 1.0 : 0.0) - n[a] * n[c] : (a == c ? 1.0 : 0.0);
+}
+
 // One stepSimulation (SIMULATION_SCENE.h:120-126): f_ext = one-ring volume * gravity, then
 //   newtonIters == 1: BACKWARD_EULER_POSITION::solveRayleighDamped (BACKWARD_EULER_POSITION.cpp:138-327)
 //   newtonIters  > 1: BDF_1::solveRayleighDamped (BDF_1.cpp:78-239) with residual test maxR > tol
@@ -865,6 +1005,53 @@ int Sim::step(double* errOut, int* itersPerNewton, int* newtonDone) {
   const std::vector<double> uOld(u), vOld(v);
   for (size_t k = 0; k < pinned.size(); k++)  // applyKinematicConstraints (TIMESTEPPER.cpp:105-122)
     for (int c = 0; c < 3; c++) u[3 * pinned[k] + c] = pinTargets[3 * k + c];
+
+  // Plane constraints against kinematic collision objects (BACKWARD_EULER_POSITION.cpp:163-181).
+  // xk: the positions the constraint logic sees during this step (the mesh's vertices).
+  const bool contact = !cubes.empty() && newtonIters == 1;
+  std::vector<char> mode;
+  std::vector<double> nrm, zc, xk;
+  if (contact) {
+    xk.resize(dofs);
+    for (int i = 0; i < dofs; i++) xk[i] = rest[i] + u[i];
+    // findNewSurfaceConstraints (TIMESTEPPER.cpp:446-526)
+    for (int o = 0; o < (int)cubes.size(); o++)
+      for (int sv : surface) {
+        if (inColl[sv]) continue;
+        const double* x = &xk[3 * (size_t)sv];
+        if (!cubes[o].inside(x)) continue;
+        Plane p;
+        p.obj = o;
+        p.vertex = sv;
+        p.sep = false;
+        cubes[o].closest(x, p.cl, p.nl);
+        double n[3];
+        cubes[o].normalToWorld(p.nl, n);
+        const double* vv = &v[3 * (size_t)sv];
+        if ((vv[0] * n[0] + vv[1] * n[1]) + vv[2] * n[2] >= -FLT_EPSILON) continue;
+        planes.push_back(p);
+        inColl[sv] = 1;
+      }
+    buildFilter(mode, nrm);
+    // updateConstraintTargets (BACKWARD_EULER_POSITION.cpp:85-110), then z = (I - S) targets
+    std::vector<double> tg(dofs, 0.0);
+    for (const Plane& p : planes) {
+      if (p.sep) continue;
+      double cw[3];
+      cubes[p.obj].toWorld(p.cl, cw);
+      for (int c = 0; c < 3; c++) tg[3 * (size_t)p.vertex + c] = cw[c] - xk[3 * (size_t)p.vertex + c];
+    }
+    zc.assign(dofs, 0.0);
+    for (int i = 0; i < nv; i++) {
+      double S[3][3], ImS[3][3];
+      filterBlock(mode[i], &nrm[3 * (size_t)i], S);
+      for (int a = 0; a < 3; a++)
+        for (int c = 0; c < 3; c++) ImS[a][c] = (a == c ? 1.0 : 0.0) - S[a][c];
+      for (int a = 0; a < 3; a++)
+        zc[3 * (size_t)i + a] = (ImS[a][0] * tg[3 * (size_t)i] + ImS[a][1] * tg[3 * (size_t)i + 1]) +
+                                ImS[a][2] * tg[3 * (size_t)i + 2];
+    }
+  }
 
   int total = 0, done = 0;
   double err = 0;
@@ -899,6 +1086,40 @@ int Sim::step(double* errOut, int* itersPerNewton, int* newtonDone) {
       for (int i = 0; i < dofs; i++) b[i] = tmp[i] + (((Ma[i] + R[i]) + fext[i]));
     }
     // A = M/dt^2 - C/dt - K; rhs = S b; LHS = S A S + (I - S)   (:208, :242-243)
+    if (contact) {
+      // c = b - A z (:212); rhs = S c; LHS = S A S + (I - S), each product summed in Eigen's
+      // column order
+      std::vector<double> Av(9 * (size_t)nnzb), Az(dofs);
+      for (size_t q = 0; q < Av.size(); q++) Av[q] = D[q] - K[q];
+      spmv(Av, zc.data(), Az.data());
+      std::vector<double> Sb(9 * (size_t)nv);
+      for (int i = 0; i < nv; i++) {
+        double S[3][3];
+        filterBlock(mode[i], &nrm[3 * (size_t)i], S);
+        for (int a = 0; a < 3; a++)
+          for (int c = 0; c < 3; c++) Sb[9 * (size_t)i + 3 * a + c] = S[a][c];
+      }
+      for (int i = 0; i < nv; i++) {
+        const double* Si = &Sb[9 * (size_t)i];
+        double cv[3];
+        for (int a = 0; a < 3; a++) cv[a] = b[3 * i + a] - Az[3 * i + a];
+        for (int a = 0; a < 3; a++) rhs[3 * i + a] = (Si[3 * a] * cv[0] + Si[3 * a + 1] * cv[1]) + Si[3 * a + 2] * cv[2];
+        for (int k = rowPtr[i]; k < rowPtr[i + 1]; k++) {
+          const int j = colIdx[k];
+          const double* Sj = &Sb[9 * (size_t)j];
+          const double* A = &Av[9 * (size_t)k];
+          double SA[3][3];
+          for (int a = 0; a < 3; a++)
+            for (int d = 0; d < 3; d++) SA[a][d] = (Si[3 * a] * A[d] + Si[3 * a + 1] * A[3 + d]) + Si[3 * a + 2] * A[6 + d];
+          for (int a = 0; a < 3; a++)
+            for (int c = 0; c < 3; c++) {
+              double x = (SA[a][0] * Sj[c] + SA[a][1] * Sj[3 + c]) + SA[a][2] * Sj[6 + c];
+              if (j == i) x += (a == c ? 1.0 : 0.0) - Si[3 * a + c];
+              lhs[9 * (size_t)k + 3 * a + c] = x;
+            }
+        }
+      }
+    } else
     for (int i = 0; i < nv; i++) {
       const bool pi = pinnedDof[3 * i];
       for (int k = rowPtr[i]; k < rowPtr[i + 1]; k++) {
@@ -918,13 +1139,59 @@ int Sim::step(double* errOut, int* itersPerNewton, int* newtonDone) {
     const int it = pcg(lhs, rhs, y, err);
     if (itersPerNewton) itersPerNewton[done] = it;
     total += it;
-    for (int i = 0; i < dofs; i++) u[i] += y[i];
+    if (contact)  // xDelta = y + z (:252)
+      for (int i = 0; i < dofs; i++) u[i] += y[i] + zc[i];
+    else
+      for (int i = 0; i < dofs; i++) u[i] += y[i];
     lastRhs = rhs;
     lastLhs = lhs;
     done++;
   }
-  // v = (u - u_old)/dt, then filtered by S (BACKWARD_EULER_POSITION.cpp:300-314)
-  for (int i = 0; i < dofs; i++) v[i] = pinnedDof[i] ? 0.0 : invDt * (u[i] - uOld[i]);
+  if (contact) {
+    // findSeparatingSurfaceConstraints(b) (TIMESTEPPER.cpp:252-329) at this step's positions
+    bool changed = false;
+    for (Plane& p : planes) {
+      const double* x = &xk[3 * (size_t)p.vertex];
+      if (cubes[p.obj].signedDistance(x) > 1e-6) {
+        p.sep = true;
+        changed = true;
+        continue;
+      }
+      double xd[3] = {b[3 * p.vertex], b[3 * p.vertex + 1], b[3 * p.vertex + 2]};
+      const double len = std::sqrt((xd[0] * xd[0] + xd[1] * xd[1]) + xd[2] * xd[2]);
+      if (len > 1.0)
+        for (int c = 0; c < 3; c++) xd[c] /= len;
+      double n[3];
+      cubes[p.obj].normalToWorld(p.nl, n);
+      if ((xd[0] * n[0] + xd[1] * n[1]) + xd[2] * n[2] > 1e-6) {
+        p.sep = true;
+        changed = true;
+      }
+    }
+    if (changed) {  // deleteSurfaceConstraints (TIMESTEPPER.cpp:561-586)
+      std::vector<Plane> keep;
+      for (const Plane& p : planes) {
+        if (p.sep)
+          inColl[p.vertex] = 0;
+        else
+          keep.push_back(p);
+      }
+      planes.swap(keep);
+    }
+    // updateSurfaceConstraints (TIMESTEPPER.cpp:531-553); S is rebuilt only when a constraint
+    // went away, so otherwise the velocity filter keeps this step's normals (:283-296)
+    for (Plane& p : planes) cubes[p.obj].closest(&xk[3 * (size_t)p.vertex], p.cl, p.nl);
+    if (changed) buildFilter(mode, nrm);
+    for (int i = 0; i < nv; i++) {
+      double S[3][3], w[3];
+      filterBlock(mode[i], &nrm[3 * (size_t)i], S);
+      for (int c = 0; c < 3; c++) w[c] = invDt * (u[3 * i + c] - uOld[3 * i + c]);
+      for (int a = 0; a < 3; a++) v[3 * i + a] = (S[a][0] * w[0] + S[a][1] * w[1]) + S[a][2] * w[2];
+    }
+  } else {
+    // v = (u - u_old)/dt, then filtered by S (BACKWARD_EULER_POSITION.cpp:300-314)
+    for (int i = 0; i < dofs; i++) v[i] = pinnedDof[i] ? 0.0 : invDt * (u[i] - uOld[i]);
+  }
   if (errOut) *errOut = err;
   if (newtonDone) *newtonDone = done;
   return total;
@@ -1037,6 +1304,29 @@ void* orc_create(const double* rest_xyz, int nv, const int* tets, int nt, int ma
 }
 
 void orc_destroy(void* h) { delete (Sim*)h; }
+
+// TIMESTEPPER::addKinematicCollisionObject with a CUBE(center, scale) (TIMESTEPPER.cpp:433-441)
+void orc_add_collision_cube(void* h, const double* center, double scale) {
+  Sim* s = (Sim*)h;
+  if (s->surface.empty()) s->computeSurface();
+  s->cubes.emplace_back(center, scale);
+}
+
+// Live plane constraints: count, and their vertex ids in creation order.
+int orc_plane_constraints(void* h, int* vertex_ids) {
+  Sim* s = (Sim*)h;
+  if (vertex_ids)
+    for (size_t i = 0; i < s->planes.size(); i++) vertex_ids[i] = s->planes[i].vertex;
+  return (int)s->planes.size();
+}
+
+// Surface vertices (ascending): count, and the ids.
+int orc_surface_vertices(void* h, int* ids) {
+  Sim* s = (Sim*)h;
+  if (s->surface.empty()) s->computeSurface();
+  if (ids) std::copy(s->surface.begin(), s->surface.end(), ids);
+  return (int)s->surface.size();
+}
 
 void orc_set_ground(void* h, double y, double k) {
   Sim* s = (Sim*)h;

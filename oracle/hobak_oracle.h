@@ -30,6 +30,9 @@ void orc_destroy(void* h);
 void orc_set_state(void* h, const double* u, const double* v);
 void orc_get_state(void* h, double* u, double* v);
 void orc_set_ground(void* h, double y, double k);
+void orc_add_collision_cube(void* h, const double* center, double scale);
+int orc_plane_constraints(void* h, int* vertex_ids);
+int orc_surface_vertices(void* h, int* ids);
 // One SIMULATION_SCENE::stepSimulation.  Returns total PCG iterations; iters_per_newton gets one
 // count per Newton iteration (array of newton_iters, may be NULL).
 int orc_step(void* h, double* pcg_error, double* seconds, int* iters_per_newton,

@@ -10,6 +10,8 @@
 //                    loop src/Timestepper/BDF_1.cpp:78-239 (ref_create_bdf1)
 //   mesh             src/Geometry/TET_MESH_FASTER.cpp:30-133
 //   pinning          src/Timestepper/TIMESTEPPER.cpp:380-403 with src/Geometry/CUBE.cpp:27-55
+//   plane constraints src/Timestepper/TIMESTEPPER.cpp:252-329, 433-584 (kinematic collision
+//                    objects, here CUBEs: src/Geometry/CUBE.cpp:23-193)
 //   materials        src/Hyperelastic/Volume/SNH.cpp, ARAP.cpp, HYPERELASTIC.cpp
 //   SVD              src/util/MATRIX_UTIL.cpp:76-95 (svd_rv)
 //
@@ -73,6 +75,7 @@ struct RefSim {
   HarnessBEP* bep = nullptr;
   HarnessBDF1* bdf = nullptr;
   CUBE* pin = nullptr;
+  std::vector<CUBE*> shapes;  // kinematic collision objects
   VECTOR3 gravity;
   double lastStepSeconds = 0.0;
   TIMESTEPPER::TIMESTEPPER* base() const {
@@ -170,7 +173,26 @@ void ref_destroy(void* h) {
   delete sim->material;
   delete sim->mesh;
   delete sim->pin;
+  for (CUBE* c : sim->shapes) delete c;
   delete sim;
+}
+
+// TIMESTEPPER::addKinematicCollisionObject (TIMESTEPPER.cpp:433-441) with a CUBE(center, scale)
+// (CUBE.cpp:23-30); surface vertices inside it become plane constraints as the scene steps.
+void ref_add_collision_cube(void* h, const double* center, double scale) {
+  RefSim* sim = (RefSim*)h;
+  CUBE* c = new CUBE(VECTOR3(center[0], center[1], center[2]), scale);
+  sim->shapes.push_back(c);
+  sim->base()->addKinematicCollisionObject(c);
+}
+
+// The plane constraints alive after the last step: vertex ids, in the integrator's order.
+int ref_plane_constraints(void* h, int* vertex_ids) {
+  RefSim* sim = (RefSim*)h;
+  const auto& pc = sim->base()->planeConstraints();
+  if (vertex_ids)
+    for (size_t i = 0; i < pc.size(); i++) vertex_ids[i] = pc[i].vertexID;
+  return (int)pc.size();
 }
 
 int ref_num_pinned(void* h) {

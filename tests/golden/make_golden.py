@@ -182,6 +182,37 @@ def make_c1_bdf1(out):
     np.savez_compressed(out, **data)
 
 
+def make_drop(out):
+    """A lattice dropped onto a kinematic ground CUBE (scenes.drop_scene, after
+    src/Scenes/DROP_TEST.h): 40 steps of the reference's plane constraints (contact, sliding and
+    release), with the live constraint count and vertex ids per step."""
+    sc = scenes.drop_scene()
+    r = oracle.RefSim(sc)
+    iters, counts, ids, unorm = [], [], [], []
+    us, vs = {}, {}
+    for k in range(1, 41):
+        it, err, sec = r.step()
+        u, v = r.state()
+        pc = r.plane_constraints()
+        iters.append(it)
+        counts.append(len(pc))
+        ids.append(np.sort(pc))
+        unorm.append(np.linalg.norm(u))
+        if k in (1, 2, 10, 20, 30, 40):
+            us[k] = u.copy()
+            vs[k] = v.copy()
+        print(f"  step {k}: iters {it} constraints {len(pc)}", flush=True)
+    r.close()
+    flat = np.concatenate(ids).astype(np.int32)
+    offs = np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)
+    data = dict(iters=np.array(iters), counts=np.array(counts), plane_ids=flat, plane_offs=offs,
+                unorm=np.array(unorm))
+    for k, u in us.items():
+        data[f"u{k}"] = u
+        data[f"v{k}"] = vs[k]
+    np.savez_compressed(out, **data)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--skip-c2", action="store_true")
@@ -190,7 +221,8 @@ def main():
     if not oracle.ref_available():
         sys.exit("oracle/_ref/libhobak_ref.so is not built (make -C oracle ref)")
     jobs = [("element_fixtures.npz", make_element), ("mesh_small.npz", make_mesh_small),
-            ("c1_traj.npz", make_c1), ("c1_arap.npz", make_c1_arap), ("c1_bdf1.npz", make_c1_bdf1)]
+            ("c1_traj.npz", make_c1), ("c1_arap.npz", make_c1_arap), ("c1_bdf1.npz", make_c1_bdf1),
+            ("drop_traj.npz", make_drop)]
     if not args.skip_c2:
         jobs.append(("c2_traj.npz", make_c2))
     for name, fn in jobs:

@@ -137,3 +137,23 @@ def test_c1_bdf1_trajectory_matches_reference():
             assert rel(X + u, X + g[f"u{k}"]) <= 1e-12, k
             assert rel(u, g[f"u{k}"]) <= 1e-10, k
             assert rel(v, g[f"v{k}"]) <= 1e-10, k
+
+
+def test_drop_plane_constraints_match_reference():
+    """A lattice dropped onto a kinematic ground CUBE: the restatement's Baraff-Witkin plane
+    constraints (TIMESTEPPER.cpp:252-329, 446-586; BACKWARD_EULER_POSITION.cpp:85-110, 163-314)
+    against the reference's own, 40 steps: the same constraint set every step, PCG iterations
+    +-1, positions and velocities within 1e-10."""
+    g = golden("drop_traj.npz")
+    sc = scenes.drop_scene()
+    o = oracle.Oracle(sc, precond=scenes.PRECOND_JACOBI)
+    for k in range(1, 41):
+        it, _, _ = o.step()
+        assert abs(it - int(g["iters"][k - 1])) <= 1, k
+        pc = np.sort(o.plane_constraints())
+        want = g["plane_ids"][g["plane_offs"][k - 1]:g["plane_offs"][k]]
+        assert np.array_equal(pc, want), k
+        if f"u{k}" in g.files:
+            u, v = o.state()
+            assert rel(u, g[f"u{k}"]) <= 1e-10, k
+            assert rel(v, g[f"v{k}"]) <= 1e-10, k
