@@ -25,6 +25,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -335,6 +336,7 @@ struct Plan {
   Batch b[K_COUNT];
   InstStatus* status_dev = nullptr;
   InstStatus* status_host = nullptr;
+  int contact = 0;              // some member has collision objects
   double spmv_bytes = 0.0;      // summed over the members
   double spmv_bytes_max = 0.0;  // of the largest member
   hipGraphExec_t chunk_exec = nullptr;
@@ -362,8 +364,10 @@ int plan_fill(hobak_ctx* ctx, Plan& P) {
   const InstDev* const* dp = (const InstDev* const*)(P.dev + off);
   P.spmv_bytes = 0.0;
   P.spmv_bytes_max = 0.0;
+  P.contact = 0;
   for (int i = 0; i < n; i++) {
     hp[i] = P.members[i]->dev;
+    P.contact |= P.members[i]->h.contact;
     const double sb = spmv_bytes_of(P.members[i]);
     P.spmv_bytes += sb;
     P.spmv_bytes_max = std::max(P.spmv_bytes_max, sb);
@@ -529,6 +533,7 @@ int launch_prologue(hobak_ctx* ctx, const Plan& J, const Plan* S, bool timed) {
   hipStream_t s = ctx->stream;
   const int cap = ctx->grid_cap;
   if (S) HIPCHK(ctx, launch_begin_step(S->b[K_ROW], cap, s));
+  if (S && S->contact) HIPCHK(ctx, launch_contact_begin(S->b[K_ROW], cap, s));
   HIPCHK(ctx, launch_element<T>(J.b[K_ELEM], ELEM_U, 1, cap, s));
   if (timed) HIPCHK(ctx, hipEventRecord(ctx->ev[1], s));
   HIPCHK(ctx, launch_assemble<T>(J.b[K_SLOT], ASM_SYSTEM, cap, s));
@@ -584,6 +589,7 @@ template <typename T>
 int launch_epilogue(hobak_ctx* ctx, const Plan& F, const Plan* C) {
   hipStream_t s = ctx->stream;
   HIPCHK(ctx, launch_apply_dx<T>(F.b[K_DOF], F.refine, ctx->grid_cap, s));
+  if (C && C->contact) HIPCHK(ctx, launch_contact_sep(C->b[K_ROW], ctx->grid_cap, s));
   if (C) HIPCHK(ctx, launch_end_step(C->b[K_ROW], ctx->grid_cap, s));
   return HOBAK_OK;
 }
@@ -1313,6 +1319,88 @@ int hobak_inst_set_pinned(hobak_inst* I, const int32_t* vids, int32_t n, const d
   TRY(upload(I, (unsigned char*)I->h.pinned, flag.data(), flag.size()));
   TRY(upload(I, (double*)I->h.pin_target, tgt.data(), tgt.size()));
   HIPCHK(I->ctx, hipStreamSynchronize(I->ctx->stream));
+  return HOBAK_OK;
+}
+
+// Surface vertices: vertices of tet faces that no other tet shares (TET_MESH.cpp:294-354, 567-593).
+static std::vector<unsigned char> surface_flags(const hobak_inst* I, const std::vector<int32_t>& tets) {
+  std::vector<std::array<int, 3>> faces;
+  faces.reserve(4 * (size_t)I->nt);
+  const int F[4][3] = {{0, 1, 3}, {0, 2, 1}, {0, 3, 2}, {1, 2, 3}};
+  for (int t = 0; t < I->nt; t++)
+    for (int f = 0; f < 4; f++) {
+      std::array<int, 3> k = {tets[4 * (size_t)t + F[f][0]], tets[4 * (size_t)t + F[f][1]],
+                              tets[4 * (size_t)t + F[f][2]]};
+      std::sort(k.begin(), k.end());
+      faces.push_back(k);
+    }
+  std::sort(faces.begin(), faces.end());
+  std::vector<unsigned char> on(I->nv, 0);
+  for (size_t i = 0; i < faces.size();) {
+    size_t j = i + 1;
+    while (j < faces.size() && faces[j] == faces[i]) j++;
+    if (j - i < 2)
+      for (int c = 0; c < 3; c++) on[faces[i][c]] = 1;
+    i = j;
+  }
+  return on;
+}
+
+int hobak_inst_add_collision_cube(hobak_inst* I, const double* center, double scale) {
+  if (!I || !center) return HOBAK_E_INVALID;
+  hobak_ctx* ctx = I->ctx;
+  if (I->requests) return fail(ctx, HOBAK_E_INVALID, "instance has a step in flight");
+  if (I->tsize != 8 || I->h.refine) return fail(ctx, HOBAK_E_INVALID, "collision objects need the FP64 form");
+  if (I->h.bdf) return fail(ctx, HOBAK_E_INVALID, "collision objects need one Newton iteration per step");
+  if (!(scale > 0.0) || !std::isfinite(scale)) return fail(ctx, HOBAK_E_INVALID, "cube scale must be positive");
+  if (I->h.nshapes >= kMaxShapes) return fail(ctx, HOBAK_E_INVALID, "too many collision objects");
+  (void)hipSetDevice(ctx->device);
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  InstDev& h = I->h;
+  if (!h.contact) {
+    int rc;
+    const size_t nv = (size_t)I->nv;
+    unsigned char* surf;
+    if ((rc = dalloc(I, &surf, nv)) || (rc = dalloc(I, &h.pc_obj, nv)) || (rc = dalloc(I, &h.pc_cl, 3 * nv)) ||
+        (rc = dalloc(I, &h.pc_nl, 3 * nv)) || (rc = dalloc(I, &h.pc_sep, nv)) ||
+        (rc = dalloc(I, &h.fmode, nv)) || (rc = dalloc(I, &h.fnrm, 3 * nv)) || (rc = dalloc(I, &h.zc, 3 * nv)) ||
+        (rc = dalloc(I, &h.bunf, 3 * nv)) || (rc = dalloc(I, &h.part_sep, (size_t)h.nb[K_ROW])))
+      return rc;
+    std::vector<int32_t> tets(4 * (size_t)I->nt);
+    HIPCHK(ctx, hipMemcpy(tets.data(), h.tets, sizeof(int32_t) * tets.size(), hipMemcpyDeviceToHost));
+    const std::vector<unsigned char> on = surface_flags(I, tets);
+    TRY(upload(I, surf, on.data(), nv));
+    h.surf = surf;
+    HIPCHK(ctx, hipMemsetAsync(h.pc_obj, 0xff, sizeof(int) * nv, ctx->stream));  // -1: none
+    h.contact = 1;
+  }
+  double* q = h.shape[h.nshapes];
+  for (int k = 0; k < 3; k++) q[k] = center[k];
+  q[3] = scale;
+  q[4] = (scale * scale) * (1.0 / ((scale * scale) * scale));  // Eigen's cofactor inverse (CUBE.cpp:28)
+  h.nshapes++;
+  TRY(upload(I, I->dev, &h, 1));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  plans_forget(ctx, I);  // cached plans were built without the contact kernels
+  return HOBAK_OK;
+}
+
+int hobak_inst_plane_constraints(hobak_inst* I, int32_t* count, int32_t* ids, int32_t cap) {
+  if (!I || !count) return HOBAK_E_INVALID;
+  *count = 0;
+  if (!I->h.contact) return HOBAK_OK;
+  (void)hipSetDevice(I->ctx->device);
+  std::vector<int> obj(I->nv);
+  HIPCHK(I->ctx, hipMemcpyAsync(obj.data(), I->h.pc_obj, sizeof(int) * obj.size(), hipMemcpyDeviceToHost,
+                                I->ctx->stream));
+  HIPCHK(I->ctx, hipStreamSynchronize(I->ctx->stream));
+  int n = 0;
+  for (int i = 0; i < I->nv; i++)
+    if (obj[i] >= 0) {
+      if (ids && n < cap) ids[n] = i;
+      n++;
+    }
+  *count = n;
   return HOBAK_OK;
 }
 

@@ -54,6 +54,7 @@ constexpr int kElementBlock = 128;
 constexpr int kMinvStride = 9;   // block-Jacobi inverse per row; Jacobi uses 1/a_ii per DOF
 constexpr int kUpdateCap = 512;  // default per-instance workgroup cap of the PCG update kernel
 constexpr int kMaxSweeps = 32;   // FP64 refinement sweeps per linear solve
+constexpr int kMaxShapes = 8;    // kinematic collision cubes per instance
 
 // Virtual-workgroup kinds: each instance contributes nb[kind] workgroups to a launch of a kernel
 // of that kind.
@@ -140,6 +141,17 @@ struct InstDev {
   double *part_pt;                  // K_SPMV: p.t
   double *part_rr, *part_rz;        // K_UPD: r.r, r.z
   double *part_res;                 // K_SPMV: refinement residual
+  // Kinematic collision cubes and Baraff-Witkin plane constraints (contact != 0; FP64 form).
+  // shape[o] = {center x, y, z, scale, inverse scale}: world = scale * local + center.
+  int nshapes, contact;
+  double shape[kMaxShapes][5];
+  const unsigned char* surf;  // per vertex: on the surface
+  int* pc_obj;                // per vertex: the constraining shape, or -1
+  double *pc_cl, *pc_nl;      // per vertex: closest point and normal in the shape's frame
+  unsigned char* pc_sep;      // per vertex: separating at the end of this step
+  unsigned char* fmode;       // per vertex: this step's filter block, 0 I, 1 I - n n^T, 2 zero
+  double *fnrm, *zc, *bunf;   // per vertex: the filter's normal, z, the unfiltered b
+  double* part_sep;           // K_ROW: separating-constraint counts
 };
 
 // The control block lives in the same allocation as the record, kCtlOffset bytes after it, so a
@@ -229,6 +241,9 @@ template <typename T>
 hipError_t launch_element(const Batch& b, int src, int with_hessian, int cap, hipStream_t s);
 hipError_t launch_begin_step(const Batch& b, int cap, hipStream_t s);
 hipError_t launch_end_step(const Batch& b, int cap, hipStream_t s);
+// plane constraints: new ones, the filter and z (prologue); the separation test (epilogue)
+hipError_t launch_contact_begin(const Batch& b, int cap, hipStream_t s);
+hipError_t launch_contact_sep(const Batch& b, int cap, hipStream_t s);
 template <typename T>
 hipError_t launch_apply_dx(const Batch& b, int use64, int cap, hipStream_t s);
 template <typename T>

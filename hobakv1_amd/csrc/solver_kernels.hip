@@ -150,6 +150,234 @@ __device__ __forceinline__ bool solve_ended(const HOBAK_GLOBAL Ctl* c) {
 static inline int grid_of(const Batch& b, int cap) { return cap > 0 && cap < b.total ? cap : b.total; }
 
 // ---------------------------------------------------------------------------------------------
+// Kinematic collision cubes and Baraff-Witkin plane constraints (the FP64 form)
+//   contact_begin (after begin_step): surface vertices that are inside a cube and move into it
+//     become constrained (TIMESTEPPER::findNewSurfaceConstraints, TIMESTEPPER.cpp:446-526); the
+//     filter S (I - n n^T, or 0 at pinned vertices, buildConstraintMatrix :128-165) and
+//     z = (I - S) (closest point - x) (updateConstraintTargets, BACKWARD_EULER_POSITION.cpp:85-110)
+//   assembly writes A unprojected; rhs forms c = b - A z, S c and S A S + (I - S) (:212, :242-243)
+//   apply_dx adds z (:252)
+//   contact_sep: the separation test at the step's positions with the unfiltered b
+//     (findSeparatingSurfaceConstraints, TIMESTEPPER.cpp:252-329)
+//   end_step: deletes separating constraints, re-targets the rest (:531-586) and filters the
+//     velocity with S, rebuilt only when a constraint went away (BACKWARD_EULER_POSITION.cpp:283-314)
+// The cube has no rotation: local = s^-1 (w - c), world = s l + c.  Products and sums are
+// rounded one by one (no fused multiply-add), as the reference evaluates them.
+// ---------------------------------------------------------------------------------------------
+struct CubeD {
+  double c[3], s, sinv;
+};
+__device__ __forceinline__ CubeD cube_of(const HOBAK_GLOBAL InstDev* I, int o) {
+  CubeD q;
+  for (int k = 0; k < 3; k++) q.c[k] = I->shape[o][k];
+  q.s = I->shape[o][3];
+  q.sinv = I->shape[o][4];
+  return q;
+}
+__device__ __forceinline__ void cube_local(const CubeD& q, const double* w, double* l) {
+  for (int k = 0; k < 3; k++) l[k] = __dmul_rn(q.sinv, __dsub_rn(w[k], q.c[k]));
+}
+__device__ __forceinline__ bool cube_in(const double* l) {  // CUBE.cpp:44-55
+  return l[0] <= 0.5 && l[0] >= -0.5 && l[1] <= 0.5 && l[1] >= -0.5 && l[2] <= 0.5 && l[2] >= -0.5;
+}
+__device__ __forceinline__ double cube_sd(const CubeD& q, const double* w) {  // CUBE.cpp:90-119
+  double l[3];
+  cube_local(q, w, l);
+  if (cube_in(l)) {
+    const double xm = fmin(__dsub_rn(0.5, l[0]), __dadd_rn(l[0], 0.5));
+    const double ym = fmin(__dsub_rn(0.5, l[1]), __dadd_rn(l[1], 0.5));
+    const double zm = fmin(__dsub_rn(0.5, l[2]), __dadd_rn(l[2], 0.5));
+    return __dmul_rn(-fabs(fmin(fmin(xm, ym), zm)), q.s);
+  }
+  double d[3] = {0.0, 0.0, 0.0};
+  for (int k = 0; k < 3; k++) {
+    if (l[k] > 0.5)
+      d[k] = __dsub_rn(l[k], 0.5);
+    else if (l[k] < -0.5)
+      d[k] = __dsub_rn(-0.5, l[k]);
+  }
+  const double n2 = __dadd_rn(__dadd_rn(__dmul_rn(d[0], d[0]), __dmul_rn(d[1], d[1])), __dmul_rn(d[2], d[2]));
+  return __dmul_rn(sqrt(n2), q.s);
+}
+// CUBE.cpp:124-191: the nearest face (the first of the six smallest distances)
+__device__ __forceinline__ void cube_closest(const CubeD& q, const double* w, double* cl, double* nl) {
+  double l[3];
+  cube_local(q, w, l);
+  const double diffs[6] = {__dadd_rn(0.5, l[0]), __dsub_rn(0.5, l[0]), __dadd_rn(0.5, l[1]),
+                           __dsub_rn(0.5, l[1]), __dadd_rn(0.5, l[2]), __dsub_rn(0.5, l[2])};
+  int mi = 0;
+  double mf = diffs[0];
+  for (int x = 1; x < 6; x++)
+    if (diffs[x] < mf) {
+      mf = diffs[x];
+      mi = x;
+    }
+  const int axis = mi >> 1;
+  const double sign = (mi & 1) ? 1.0 : -1.0;
+  for (int k = 0; k < 3; k++) {
+    cl[k] = k == axis ? 0.5 * sign : l[k];
+    nl[k] = k == axis ? sign : 0.0;
+  }
+}
+__device__ __forceinline__ double dot3_rn(const double* a, const double* b) {
+  return __dadd_rn(__dadd_rn(__dmul_rn(a[0], b[0]), __dmul_rn(a[1], b[1])), __dmul_rn(a[2], b[2]));
+}
+// the filter block: I, I - n n^T or 0
+__device__ __forceinline__ void filter_block(int mode, const double* n, double S[3][3]) {
+  for (int a = 0; a < 3; a++)
+    for (int c = 0; c < 3; c++) {
+      const double id = a == c ? 1.0 : 0.0;
+      S[a][c] = mode == 2 ? 0.0 : mode == 1 ? __dsub_rn(id, __dmul_rn(n[a], n[c])) : id;
+    }
+}
+// the unit normal of the filter (normalized, as buildConstraintMatrix does)
+__device__ __forceinline__ void unit_normal(const double* nl, double* n) {
+  const double len = sqrt(dot3_rn(nl, nl));
+  for (int k = 0; k < 3; k++) n[k] = __ddiv_rn(nl[k], len);
+}
+
+__global__ __launch_bounds__(kBlock) void contact_begin_kernel(Batch b) {
+  for (int ws = blockIdx.x; ws < b.total; ws += gridDim.x) {
+    const int vb = xcd_vb(ws, b.total);
+    int lb, lg;
+    const HOBAK_GLOBAL InstDev* __restrict__ I = seg_find(b, vb, lb, lg);
+    if (!I->contact) continue;
+    const int r = lb * kBlock + threadIdx.x;
+    if (r >= I->nv) continue;
+    double x[3];
+    for (int k = 0; k < 3; k++) x[k] = gbl(I->rest)[3 * (size_t)r + k] + gbl(I->u)[3 * (size_t)r + k];
+    const bool pin = gbl(I->pinned)[r];
+    int o = gbl(I->pc_obj)[r];
+    auto pcl = gbl(I->pc_cl) + 3 * (size_t)r;
+    auto pnl = gbl(I->pc_nl) + 3 * (size_t)r;
+    if (gbl(I->surf)[r] && o < 0) {
+      double v[3];
+      for (int k = 0; k < 3; k++) v[k] = gbl(I->v)[3 * (size_t)r + k];
+      for (int q = 0; q < I->nshapes; q++) {
+        const CubeD cb = cube_of(I, q);
+        double l[3];
+        cube_local(cb, x, l);
+        if (!cube_in(l)) continue;
+        double cl[3], nl[3];
+        cube_closest(cb, x, cl, nl);
+        if (dot3_rn(v, nl) >= -FLT_EPSILON) continue;  // moving away (or along): no constraint
+        o = q;
+        gbl(I->pc_obj)[r] = q;
+        for (int k = 0; k < 3; k++) {
+          pcl[k] = cl[k];
+          pnl[k] = nl[k];
+        }
+        break;
+      }
+    }
+    const int mode = pin ? 2 : (o >= 0 ? 1 : 0);
+    double n[3] = {0.0, 0.0, 0.0}, tg[3] = {0.0, 0.0, 0.0};
+    if (o >= 0) {
+      double nl[3], cl[3], cw[3];
+      for (int k = 0; k < 3; k++) {
+        nl[k] = pnl[k];
+        cl[k] = pcl[k];
+      }
+      unit_normal(nl, n);
+      const CubeD cb = cube_of(I, o);
+      for (int k = 0; k < 3; k++) {
+        cw[k] = __dadd_rn(__dmul_rn(cb.s, cl[k]), cb.c[k]);
+        tg[k] = __dsub_rn(cw[k], x[k]);
+      }
+    }
+    double S[3][3];
+    filter_block(mode, n, S);
+    gbl(I->fmode)[r] = (unsigned char)mode;
+    for (int a = 0; a < 3; a++) {
+      double ims[3];
+      for (int c = 0; c < 3; c++) ims[c] = __dsub_rn(a == c ? 1.0 : 0.0, S[a][c]);
+      gbl(I->fnrm)[3 * (size_t)r + a] = n[a];
+      gbl(I->zc)[3 * (size_t)r + a] = dot3_rn(ims, tg);
+    }
+  }
+}
+
+// The positions the constraint logic saw during this step: rest + u after the pins were applied.
+__device__ __forceinline__ void step_positions(const HOBAK_GLOBAL InstDev* I, int r, double* x) {
+  const bool pin = gbl(I->pinned)[r];
+  for (int k = 0; k < 3; k++) {
+    const size_t i = 3 * (size_t)r + k;
+    x[k] = gbl(I->rest)[i] + (pin ? gbl(I->pin_target)[i] : gbl(I->u_old)[i]);
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void contact_sep_kernel(Batch b) {
+  for (int ws = blockIdx.x; ws < b.total; ws += gridDim.x) {
+    const int vb = xcd_vb(ws, b.total);
+    int lb, lg;
+    const HOBAK_GLOBAL InstDev* __restrict__ I = seg_find(b, vb, lb, lg);
+    if (!I->contact) continue;
+    const int r = lb * kBlock + threadIdx.x;
+    double count = 0.0;
+    if (r < I->nv) {
+      const int o = gbl(I->pc_obj)[r];
+      int sep = 0;
+      if (o >= 0) {
+        double x[3];
+        step_positions(I, r, x);
+        const CubeD cb = cube_of(I, o);
+        if (cube_sd(cb, x) > 1e-6) {
+          sep = 1;
+        } else {
+          double xd[3], nl[3];
+          for (int k = 0; k < 3; k++) {
+            xd[k] = gbl(I->bunf)[3 * (size_t)r + k];
+            nl[k] = gbl(I->pc_nl)[3 * (size_t)r + k];
+          }
+          const double len = sqrt(dot3_rn(xd, xd));
+          if (len > 1.0)
+            for (int k = 0; k < 3; k++) xd[k] = __ddiv_rn(xd[k], len);
+          sep = dot3_rn(xd, nl) > 1e-6;
+        }
+      }
+      gbl(I->pc_sep)[r] = (unsigned char)sep;
+      count = sep;
+    }
+    publish(count, 0.0, gbl(I->part_sep), nullptr, lb);
+  }
+}
+
+// End of a step for an instance with plane constraints: separating constraints go (when any
+// does), the others take the closest point and normal at this step's positions, and
+// v = S (u - u_old) / dt with S rebuilt only if a constraint went away.
+__device__ void contact_end(const HOBAK_GLOBAL InstDev* I, int r, bool changed) {
+  double x[3];
+  step_positions(I, r, x);
+  const bool pin = gbl(I->pinned)[r];
+  int o = gbl(I->pc_obj)[r];
+  int mode = gbl(I->fmode)[r];
+  double n[3];
+  for (int k = 0; k < 3; k++) n[k] = gbl(I->fnrm)[3 * (size_t)r + k];
+  if (o >= 0) {
+    if (changed && gbl(I->pc_sep)[r]) {
+      gbl(I->pc_obj)[r] = -1;
+      o = -1;
+    } else {
+      double cl[3], nl[3];
+      cube_closest(cube_of(I, o), x, cl, nl);
+      for (int k = 0; k < 3; k++) {
+        gbl(I->pc_cl)[3 * (size_t)r + k] = cl[k];
+        gbl(I->pc_nl)[3 * (size_t)r + k] = nl[k];
+      }
+      if (changed) unit_normal(nl, n);
+    }
+  }
+  if (changed) mode = pin ? 2 : (o >= 0 ? 1 : 0);
+  double S[3][3], w[3];
+  filter_block(mode, n, S);
+  for (int k = 0; k < 3; k++) {
+    const size_t i = 3 * (size_t)r + k;
+    w[k] = I->invDt * (gbl(I->u)[i] - gbl(I->u_old)[i]);
+  }
+  for (int a = 0; a < 3; a++) gbl(I->v)[3 * (size_t)r + a] = dot3_rn(S[a], w);
+}
+
+// ---------------------------------------------------------------------------------------------
 // step bookkeeping
 // ---------------------------------------------------------------------------------------------
 // _positionOld = _position; _velocityOld = _velocity; applyKinematicConstraints
@@ -185,7 +413,10 @@ __global__ __launch_bounds__(kBlock) void apply_dx_kernel(Batch b, int use64) {
     int lb, lg;
     const HOBAK_GLOBAL InstDev* __restrict__ I = seg_find(b, vb, lb, lg);
     const int i = lb * kBlock + threadIdx.x;
-    if (i < 3 * I->nv) gbl(I->u)[i] += use64 ? gbl(I->x64)[i] : (double)(gbl((const T*)I->x))[i];
+    if (i < 3 * I->nv) {
+      const double dx = use64 ? gbl(I->x64)[i] : (double)(gbl((const T*)I->x))[i];
+      gbl(I->u)[i] += I->contact ? dx + gbl(I->zc)[i] : dx;  // xDelta = y + z
+    }
   }
 }
 
@@ -196,6 +427,12 @@ __global__ __launch_bounds__(kBlock) void end_step_kernel(Batch b) {
     int lb, lg;
     const HOBAK_GLOBAL InstDev* __restrict__ I = seg_find(b, vb, lb, lg);
     const int r = lb * kBlock + threadIdx.x;
+    if (I->contact) {
+      double nsep, unused;
+      gather(gbl(I->part_sep), nullptr, I->nb[K_ROW], nsep, unused);
+      if (r < I->nv) contact_end(I, r, nsep > 0.0);
+      continue;
+    }
     if (r < I->nv) {
       const double invDt = I->invDt;
       const bool pin = gbl(I->pinned)[r];
@@ -272,8 +509,12 @@ __global__ __launch_bounds__(kBlock) void assemble_kernel(Batch b, int target) {
       continue;
     }
     constexpr bool kDiff = sizeof(T) < sizeof(double);
-    // standard form: pinned rows and columns are zeroed; difference form: only pinned rows
-    const bool pin = kDiff ? (bool)gbl(I->pinned)[r] : (gbl(I->pinned)[r] || gbl(I->pinned)[c]);
+    // standard form: pinned rows and columns are zeroed; difference form: only pinned rows.  With
+    // plane constraints A stays unprojected here: the right-hand side kernel needs A z, then
+    // projects each row in place.
+    const bool contact = I->contact != 0;
+    const bool pin = contact ? false
+                     : kDiff ? (bool)gbl(I->pinned)[r] : (gbl(I->pinned)[r] || gbl(I->pinned)[c]);
     bool ground = false;
     if (I->ground_on && c == r && !pin) {
       const double y = gbl(I->rest)[3 * (size_t)r + 1] + gbl(I->u)[3 * (size_t)r + 1];
@@ -288,7 +529,7 @@ __global__ __launch_bounds__(kBlock) void assemble_kernel(Batch b, int target) {
       val[e] = pin ? ((c == r && (e % 4) == 0) ? T(1) : T(0)) : D[e * ns + slot] + acc[e];
       out[e * ns + slot] = val[e];
     }
-    if (c == r) {
+    if (c == r && !contact) {
       write_precond<T>(val, gbl((T*)I->minv), r, I->precond);
       if (kDiff) {
 #pragma unroll
@@ -352,6 +593,55 @@ __device__ __forceinline__ void apply_precond(const HOBAK_GLOBAL T* minv, int pr
 // Per block row: R = ascending-tet gather of staged forces; b = (E v + R) + f_ext (BEP) or
 // b = C vb + ((M a + R) + f_ext) (BDF_1); rhs = S b.  With init_cg also x = 0, r = rhs,
 // z = M^-1 r; the partial sums of |rhs|^2 and r.z seed Eigen's CG (ConjugateGradient.h:43-66).
+// Row r of an instance with plane constraints (FP64 form): b (unfiltered, kept for the
+// separation test) becomes S (b - A z), and the row's blocks of the unprojected A become
+// S_r A_rj S_j (+ (I - S_r) on the diagonal), in place: the row's thread owns its slots.  The
+// preconditioner is taken from the projected diagonal block.
+template <typename T>
+__device__ void contact_rhs(const HOBAK_GLOBAL InstDev* I, int r, double (&bv)[3]) {
+  const size_t ns = (size_t)I->nslots;
+  const int sh = I->shift;
+  auto col = gbl(I->col);
+  auto A = gbl((T*)I->A);
+  auto zc = gbl(I->zc);
+  for (int k = 0; k < 3; k++) gbl(I->bunf)[3 * (size_t)r + k] = bv[k];
+  double Sr[3][3], nr[3];
+  for (int k = 0; k < 3; k++) nr[k] = gbl(I->fnrm)[3 * (size_t)r + k];
+  filter_block(gbl(I->fmode)[r], nr, Sr);
+  double az[3] = {0.0, 0.0, 0.0};
+  T diag[9];
+  bool has_diag = false;
+  const int len = gbl(I->row_len)[r];
+  const int base = gbl(I->slice_off)[r >> sh] + (r & ((1 << sh) - 1));
+  for (int k = 0; k < len; k++) {
+    const int slot = base + (k << sh);
+    const int c = col[slot];
+    double a[3][3], Sc[3][3], nc[3], zj[3];
+    for (int e = 0; e < 9; e++) a[e / 3][e % 3] = (double)A[e * ns + slot];
+    for (int j = 0; j < 3; j++) {
+      nc[j] = gbl(I->fnrm)[3 * (size_t)c + j];
+      zj[j] = zc[3 * (size_t)c + j];
+    }
+    for (int i = 0; i < 3; i++) az[i] += (a[i][0] * zj[0] + a[i][1] * zj[1]) + a[i][2] * zj[2];
+    filter_block(gbl(I->fmode)[c], nc, Sc);
+    double sa[3][3];
+    for (int i = 0; i < 3; i++)
+      for (int d = 0; d < 3; d++) sa[i][d] = (Sr[i][0] * a[0][d] + Sr[i][1] * a[1][d]) + Sr[i][2] * a[2][d];
+    for (int i = 0; i < 3; i++)
+      for (int d = 0; d < 3; d++) {
+        double v = (sa[i][0] * Sc[0][d] + sa[i][1] * Sc[1][d]) + sa[i][2] * Sc[2][d];
+        if (c == r) v += (i == d ? 1.0 : 0.0) - Sr[i][d];
+        A[(3 * i + d) * ns + slot] = (T)v;
+        if (c == r) diag[3 * i + d] = (T)v;
+      }
+    has_diag |= c == r;
+  }
+  double cv[3];
+  for (int i = 0; i < 3; i++) cv[i] = bv[i] - az[i];
+  for (int i = 0; i < 3; i++) bv[i] = (Sr[i][0] * cv[0] + Sr[i][1] * cv[1]) + Sr[i][2] * cv[2];
+  if (has_diag) write_precond<T>(diag, gbl((T*)I->minv), r, I->precond);
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void rhs_kernel(Batch b, int init_cg) {
   for (int ws = blockIdx.x; ws < b.total; ws += gridDim.x) {
@@ -427,8 +717,12 @@ __global__ __launch_bounds__(kBlock) void rhs_kernel(Batch b, int init_cg) {
         } else {
           bv[c] = (s[c] + R[c]) + fext;
         }
-        if (pin) bv[c] = 0.0;
-        gbl(I->rhs64)[i] = bv[c];
+        if (pin && !I->contact) bv[c] = 0.0;
+      }
+      if (!kDiff && I->contact) contact_rhs<T>(I, r, bv);
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        gbl(I->rhs64)[3 * (size_t)r + c] = bv[c];
         part[0] += bv[c] * bv[c];
       }
       if (init_cg) {
@@ -1112,6 +1406,10 @@ int blocks_of(int kind, int nv, int nt, int nslots, int shift, int update_cap) {
     return hipGetLastError();                                                               \
   } while (0)
 
+hipError_t launch_contact_begin(const Batch& b, int cap, hipStream_t s) {
+  SEG_LAUNCH(contact_begin_kernel, b, cap, s);
+}
+hipError_t launch_contact_sep(const Batch& b, int cap, hipStream_t s) { SEG_LAUNCH(contact_sep_kernel, b, cap, s); }
 hipError_t launch_begin_step(const Batch& b, int cap, hipStream_t s) { SEG_LAUNCH(begin_step_kernel, b, cap, s); }
 hipError_t launch_end_step(const Batch& b, int cap, hipStream_t s) { SEG_LAUNCH(end_step_kernel, b, cap, s); }
 

@@ -2,12 +2,14 @@
 // does (src/Scenes/SIMULATION_SCENE.h:120-126) through TET_MESH / SNH / ARAP / CUBE /
 // BACKWARD_EULER_POSITION, and writes the final state and per-step PCG iterations.
 //
-//   facade_driver <scene.bin> <out.bin>
+//   facade_driver <scene.bin> <out.bin>        (HOBAK_FACADE_PRECOND=0: Jacobi PCG)
 //
-// scene.bin (little-endian): int32 nv, nt, material, steps, has_cube; float64 mu, lambda, dt,
-// rayleigh alpha, beta, mass scale, gravity[3], pcg tol, cube center[3], cube scale; then rest
-// (nv*3 float64), tets (nt*4 int32), u0 and v0 (3*nv float64 each).
-// out.bin: int32 steps, iterations[steps]; float64 u[3*nv], v[3*nv].
+// scene.bin (little-endian): int32 nv, nt, material, steps, flags (1: pin cube, 2: collision
+// cube); float64 mu, lambda, dt, rayleigh alpha, beta, mass scale, gravity[3], pcg tol, pin cube
+// center[3] and scale, collision cube center[3] and scale; then rest (nv*3 float64), tets (nt*4
+// int32), u0 and v0 (3*nv float64 each).
+// out.bin: int32 steps, iterations[steps], plane-constraint counts[steps]; float64 u[3*nv],
+// v[3*nv].
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -29,9 +31,9 @@ int main(int argc, char** argv) {
   FILE* f = std::fopen(argv[1], "rb");
   if (!f) return 2;
   int32_t hdr[5];
-  double prm[15];
-  if (!rd(f, hdr, 5) || !rd(f, prm, 15)) return 2;
-  const int nv = hdr[0], nt = hdr[1], mat = hdr[2], steps = hdr[3], has_cube = hdr[4];
+  double prm[18];
+  if (!rd(f, hdr, 5) || !rd(f, prm, 18)) return 2;
+  const int nv = hdr[0], nt = hdr[1], mat = hdr[2], steps = hdr[3], flags = hdr[4];
   std::vector<hobak::VECTOR3> rest(nv);
   std::vector<hobak::VECTOR4I> tets(nt);
   for (auto& v : rest)
@@ -54,20 +56,26 @@ int main(int argc, char** argv) {
     solver.scaleMass(prm[5]);
     solver.addGravity({prm[6], prm[7], prm[8]});
     solver.setTolerance(prm[9]);
+    // HOBAK_FACADE_PRECOND=0 selects Jacobi, the reference's preconditioner (default: block Jacobi)
+    if (const char* pc = std::getenv("HOBAK_FACADE_PRECOND")) solver.setPreconditioner(std::atoi(pc));
     hobak::CUBE cube({prm[10], prm[11], prm[12]}, prm[13]);
-    if (has_cube) solver.attachKinematicConstraints(&cube);
+    if (flags & 1) solver.attachKinematicConstraints(&cube);
+    hobak::CUBE ground({prm[14], prm[15], prm[16]}, prm[17]);
+    if (flags & 2) solver.addKinematicCollisionObject(&ground);
     solver.position() = u0;
     solver.velocity() = v0;
-    std::vector<int32_t> iters;
+    std::vector<int32_t> iters, planes;
     for (int s = 0; s < steps; s++) {
       solver.solve(false);
       iters.push_back(solver.lastStats().pcg_iters[0]);
+      planes.push_back((int32_t)solver.planeConstraints().size());
     }
     FILE* o = std::fopen(argv[2], "wb");
     if (!o) return 2;
     const int32_t n = steps;
     std::fwrite(&n, sizeof(n), 1, o);
     std::fwrite(iters.data(), sizeof(int32_t), iters.size(), o);
+    std::fwrite(planes.data(), sizeof(int32_t), planes.size(), o);
     std::fwrite(solver.position().data(), sizeof(double), solver.position().size(), o);
     std::fwrite(solver.velocity().data(), sizeof(double), solver.velocity().size(), o);
     std::fclose(o);

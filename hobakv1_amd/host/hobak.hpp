@@ -109,11 +109,14 @@ class ARAP : public HYPERELASTIC {
 
 // Axis-aligned kinematic cube: local = scale^-1 (world - center), world = scale local + center
 // (the transforms of src/Geometry/KINEMATIC_SHAPE.h:55-73, rotation = I, multiplying by the
-// inverse scale as the reference does); inside when every local coordinate is within
-// [-0.5, 0.5] (src/Geometry/CUBE.cpp:44-55).
+// inverse scale as the reference does, which takes it from Eigen's 3x3 cofactor inverse,
+// CUBE.cpp:28); inside when every local coordinate is within [-0.5, 0.5] (CUBE.cpp:44-55).
 class CUBE {
  public:
-  CUBE(const VECTOR3& center, double scale) : c_(center), s_(scale), inv_(1.0 / scale) {}
+  CUBE(const VECTOR3& center, double scale)
+      : c_(center), s_(scale), inv_((scale * scale) * (1.0 / ((scale * scale) * scale))) {}
+  const VECTOR3& center() const { return c_; }
+  double scale() const { return s_; }
   VECTOR3 local(const VECTOR3& w) const {
     return {inv_ * (w[0] - c_[0]), inv_ * (w[1] - c_[1]), inv_ * (w[2] - c_[2])};
   }
@@ -178,6 +181,26 @@ class BACKWARD_EULER_POSITION {
     }
   }
 
+  // TIMESTEPPER::addKinematicCollisionObject (TIMESTEPPER.cpp:433-441): surface vertices that
+  // enter the cube become plane constraints (FP64 form, one Newton iteration).
+  void addKinematicCollisionObject(const CUBE* shape) {
+    for (const CUBE* c : collision_)
+      if (c == shape) return;
+    collision_.push_back(shape);
+    if (inst_) add_cube(shape);
+  }
+  // Vertex ids of the live plane constraints (TIMESTEPPER.h:46), ascending.
+  std::vector<int> planeConstraints() const {
+    std::vector<int> ids;
+    if (!inst_) return ids;
+    int32_t n = 0;
+    ctx_->check(hobak_inst_plane_constraints(inst_, &n, nullptr, 0), "plane_constraints");
+    std::vector<int32_t> buf(n);
+    ctx_->check(hobak_inst_plane_constraints(inst_, &n, buf.data(), n), "plane_constraints");
+    ids.assign(buf.begin(), buf.end());
+    return ids;
+  }
+
   bool solve(bool verbose) {
     ensure();
     ctx_->check(hobak_inst_set_state(inst_, position_.data(), velocity_.data()), "set_state");
@@ -212,6 +235,11 @@ class BACKWARD_EULER_POSITION {
     }
     if (!ids.empty())
       ctx_->check(hobak_inst_set_pinned(inst_, ids.data(), (int32_t)ids.size(), tg.data()), "set_pinned");
+    for (const CUBE* c : collision_) add_cube(c);
+  }
+  void add_cube(const CUBE* c) {
+    const double center[3] = {c->center()[0], c->center()[1], c->center()[2]};
+    ctx_->check(hobak_inst_add_collision_cube(inst_, center, c->scale()), "add_collision_cube");
   }
 
   TET_MESH& mesh_;
@@ -223,6 +251,7 @@ class BACKWARD_EULER_POSITION {
   VECTOR position_, velocity_;
   std::vector<int> pins_;
   std::vector<double> targets_;
+  std::vector<const CUBE*> collision_;
 };
 
 }  // namespace hobak

@@ -94,6 +94,8 @@ SIGNATURES = {
     "hobak_inst_bytes_estimate": (ctypes.c_size_t, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
     "hobak_inst_destroy": (None, [_vp]),
     "hobak_inst_set_pinned": (ctypes.c_int, [_vp, _ip, ctypes.c_int32, _dp]),
+    "hobak_inst_add_collision_cube": (ctypes.c_int, [_vp, _dp, ctypes.c_double]),
+    "hobak_inst_plane_constraints": (ctypes.c_int, [_vp, _ip, _ip, ctypes.c_int32]),
     "hobak_inst_set_state": (ctypes.c_int, [_vp, _dp, _dp]),
     "hobak_inst_get_state": (ctypes.c_int, [_vp, _dp, _dp]),
     "hobak_step": (ctypes.c_int, [_vp, ctypes.POINTER(hobak_step_stats)]),
@@ -297,7 +299,24 @@ class Instance:
             tg = np.ascontiguousarray(scene.pinned_targets(), dtype=np.float64)
             ctx.check(self.lib.hobak_inst_set_pinned(self.h, _i(pins), pins.size, _d(tg)),
                       "set_pinned")
+        for cube in getattr(scene, "collision_cubes", ()):
+            self.add_collision_cube(cube[:3], cube[3])
         self.set_state(scene.initial_u(), scene.initial_v())
+
+    def add_collision_cube(self, center, scale):
+        """addKinematicCollisionObject(CUBE(center, scale)): plane constraints on contact."""
+        c = np.ascontiguousarray(center, dtype=np.float64)
+        self.ctx.check(self.lib.hobak_inst_add_collision_cube(self.h, _d(c), float(scale)),
+                       "add_collision_cube")
+
+    def plane_constraints(self):
+        """Vertex ids under a plane constraint after the last step, ascending."""
+        n = ctypes.c_int32(0)
+        self.ctx.check(self.lib.hobak_inst_plane_constraints(self.h, ctypes.byref(n), None, 0))
+        ids = np.zeros(max(n.value, 1), np.int32)
+        self.ctx.check(self.lib.hobak_inst_plane_constraints(self.h, ctypes.byref(n), _i(ids),
+                                                             ids.size))
+        return ids[:n.value]
 
     def close(self):
         if self.h:
@@ -499,6 +518,7 @@ class BackwardEulerPosition:
         self._mass_scale = 1.0
         self._pins = np.zeros(0, np.int32)
         self._targets = np.zeros((0, 3))
+        self._collision = []
         self._precond = precond
         self._precision = precision
         self._tol = 0.0
@@ -533,6 +553,18 @@ class BackwardEulerPosition:
         self._pins = np.union1d(self._pins, ids).astype(np.int32)
         self._targets = cube.to_world(cube.to_local(X[self._pins])) - X[self._pins]
 
+    def addKinematicCollisionObject(self, cube: _scenes.Cube):
+        """TIMESTEPPER::addKinematicCollisionObject (TIMESTEPPER.cpp:433-441) for a CUBE."""
+        if any(c is cube for c in self._collision):
+            return
+        self._collision.append(cube)
+        if self._inst is not None:
+            self._inst.add_collision_cube(cube.center, cube.scale)
+
+    def planeConstraints(self):
+        """Vertex ids of the live plane constraints (TIMESTEPPER.h:46), ascending."""
+        return np.zeros(0, np.int32) if self._inst is None else self._inst.plane_constraints()
+
     def _ensure(self):
         if self._inst is not None:
             return
@@ -540,7 +572,8 @@ class BackwardEulerPosition:
                            material=self.material.kind, mu=self.material.mu,
                            lam=self.material.lam, dt=self._dt, rayleigh_alpha=self._alpha,
                            rayleigh_beta=self._beta, mass_scale=self._mass_scale,
-                           gravity=tuple(self._gravity), pcg_tol=self._tol)
+                           gravity=tuple(self._gravity), pcg_tol=self._tol,
+                           collision_cubes=tuple((*c.center, c.scale) for c in self._collision))
         self._inst = Instance(self.ctx, sc, precond=self._precond, precision=self._precision)
         if self._pins.size:
             self.ctx.check(self.ctx.lib.hobak_inst_set_pinned(

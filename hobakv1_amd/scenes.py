@@ -142,7 +142,8 @@ class Cube:
     def __init__(self, center, scale):
         self.center = np.asarray(center, dtype=np.float64)
         self.scale = float(scale)
-        self.inv = 1.0 / self.scale
+        s = self.scale
+        self.inv = (s * s) * (1.0 / ((s * s) * s))  # Eigen's cofactor inverse (CUBE.cpp:28)
 
     def to_local(self, w):
         return self.inv * (w - self.center)

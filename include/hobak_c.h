@@ -118,6 +118,17 @@ void hobak_inst_destroy(hobak_inst* inst);
 /* attachKinematicConstraints: pinned vertex ids (ascending) and the displacement each is held at
  * (world(local(X)) - X in the reference; NULL means zero). */
 int hobak_inst_set_pinned(hobak_inst* inst, const int32_t* vids, int32_t n, const double* targets);
+/* addKinematicCollisionObject (src/Timestepper/TIMESTEPPER.cpp:433-441) of a
+ * CUBE(center, scale) (src/Geometry/CUBE.cpp:23-30): from the next step on, surface vertices
+ * that enter the cube moving into it become Baraff-Witkin plane constraints, which slide on its
+ * nearest face and release when the residual force pulls away (TIMESTEPPER.cpp:252-329,
+ * 446-586; BACKWARD_EULER_POSITION.cpp:85-110, 163-314).  FP64 form with one Newton iteration
+ * per step; at most HOBAK_MAX_COLLISION_OBJECTS per instance, tested in the order added. */
+#define HOBAK_MAX_COLLISION_OBJECTS 8
+int hobak_inst_add_collision_cube(hobak_inst* inst, const double* center3, double scale);
+/* The vertices under a plane constraint after the last step (planeConstraints(),
+ * TIMESTEPPER.h:46): *count of them, and the first `cap` ids, ascending, into ids (may be NULL). */
+int hobak_inst_plane_constraints(hobak_inst* inst, int32_t* count, int32_t* ids, int32_t cap);
 /* position() = u and velocity() = v, DOF-length host arrays; NULL leaves that vector unchanged. */
 int hobak_inst_set_state(hobak_inst* inst, const double* u, const double* v);
 int hobak_inst_get_state(hobak_inst* inst, double* u, double* v);

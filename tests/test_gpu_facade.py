@@ -21,24 +21,27 @@ pytestmark = pytest.mark.gpu
 
 
 def _write_scene(path, sc, steps):
-    cube = scenes.PIN_X0_CUBE if sc.pin_cube is not None else (0.0, 0.0, 0.0, 1.0)
+    cube = sc.pin_cube if sc.pin_cube is not None else (0.0, 0.0, 0.0, 1.0)
+    ground = sc.collision_cubes[0] if sc.collision_cubes else (0.0, 0.0, 0.0, 1.0)
+    flags = (1 if sc.pin_cube is not None else 0) | (2 if sc.collision_cubes else 0)
     with open(path, "wb") as f:
-        f.write(struct.pack("<5i", sc.nv, sc.nt, sc.material, steps, 1 if sc.pin_cube is not None else 0))
-        f.write(struct.pack("<15d", sc.mu, sc.lam, sc.dt, sc.rayleigh_alpha, sc.rayleigh_beta,
-                            sc.mass_scale, *sc.gravity, sc.pcg_tol, *cube, 0.0))
+        f.write(struct.pack("<5i", sc.nv, sc.nt, sc.material, steps, flags))
+        f.write(struct.pack("<18d", sc.mu, sc.lam, sc.dt, sc.rayleigh_alpha, sc.rayleigh_beta,
+                            sc.mass_scale, *sc.gravity, sc.pcg_tol, *cube, *ground))
         f.write(np.ascontiguousarray(sc.rest, dtype="<f8").tobytes())
         f.write(np.ascontiguousarray(sc.tets, dtype="<i4").tobytes())
         f.write(np.ascontiguousarray(sc.initial_u(), dtype="<f8").tobytes())
         f.write(np.ascontiguousarray(sc.initial_v(), dtype="<f8").tobytes())
 
 
-def _read_out(path, dofs):
+def _read_out(path, dofs, planes=False):
     with open(path, "rb") as f:
         (n,) = struct.unpack("<i", f.read(4))
         iters = np.frombuffer(f.read(4 * n), dtype="<i4").tolist()
+        counts = np.frombuffer(f.read(4 * n), dtype="<i4").tolist()
         u = np.frombuffer(f.read(8 * dofs), dtype="<f8").copy()
         v = np.frombuffer(f.read(8 * dofs), dtype="<f8").copy()
-    return iters, u, v
+    return (iters, u, v, counts) if planes else (iters, u, v)
 
 
 def test_cpp_facade_matches_c_abi_and_oracle(tmp_path):
@@ -90,3 +93,26 @@ def test_python_facade_matches_c_abi(gpu_ctx):
     ud, vd = inst.state()
     inst.close()
     assert np.array_equal(be.position(), ud) and np.array_equal(be.velocity(), vd)
+
+
+def test_cpp_facade_collision_cube_matches_reference(tmp_path):
+    """addKinematicCollisionObject through the C++ façade: the drop scene's constraint counts
+    and PCG iterations (Jacobi) follow the reference's, and the state after 20 steps is within
+    1e-9 of it."""
+    from conftest import golden
+
+    g = golden("drop_traj.npz")
+    sc = scenes.drop_scene()
+    steps = 20
+    scene_bin = tmp_path / "scene.bin"
+    out_bin = tmp_path / "out.bin"
+    _write_scene(scene_bin, sc, steps)
+    env = dict(os.environ, HOBAK_FACADE_PRECOND="0")
+    print(f"C++ façade: drop scene ({sc.nt} tets), {steps} steps", flush=True)
+    subprocess.run([hb.DRIVER, str(scene_bin), str(out_bin)], check=True, timeout=120, env=env)
+    iters, u, v, counts = _read_out(out_bin, sc.dofs, planes=True)
+    assert counts == g["counts"][:steps].tolist()
+    assert max(abs(a - b) for a, b in zip(iters, g["iters"][:steps])) <= 1
+    X = sc.rest.ravel()
+    assert rel(X + u, X + g["u20"]) <= 1e-9
+    assert rel(v, g["v20"]) <= 1e-9
