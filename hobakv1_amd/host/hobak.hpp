@@ -11,7 +11,8 @@
 //   BACKWARD_EULER_POSITION(mesh, material)       src/Timestepper/BACKWARD_EULER_POSITION.h:36
 //     solve(verbose)                              src/Timestepper/TIMESTEPPER.h:77
 //     position(), velocity()                      src/Timestepper/TIMESTEPPER.h:44-58
-//     addGravity, attachKinematicConstraints, setDt, setRayeligh, scaleMass
+//     addGravity, attachKinematicConstraints, addKinematicCollisionObject, setDt, setRayeligh,
+//     scaleMass; TET_MESH::readTobjFile / writeTobjFile
 //                                                 src/Timestepper/TIMESTEPPER.h:70-100
 //
 // position() is the displacement u = x - X, as in the reference.  The device instance is created
@@ -72,6 +73,44 @@ class TET_MESH {
   const std::vector<VECTOR3>& restVertices() const { return rest_; }
   const std::vector<VECTOR4I>& tets() const { return tets_; }
   int DOFs() const { return 3 * (int)rest_.size(); }
+
+  // TET_MESH::readTobjFile (TET_MESH.cpp:1173-1220): a run of "v x y z" lines, then a run of
+  // "t a b c d" lines; false if the file cannot be opened or ends right after its vertices.
+  static bool readTobjFile(const std::string& filename, std::vector<VECTOR3>& vertices,
+                           std::vector<VECTOR4I>& tets) {
+    vertices.clear();
+    tets.clear();
+    FILE* f = std::fopen(filename.c_str(), "r");
+    if (!f) return false;
+    char line[512];
+    bool in_tets = false, any_after = false;
+    while (std::fgets(line, sizeof(line), f)) {
+      if (!in_tets && line[0] == 'v') {
+        VECTOR3 v;
+        if (std::sscanf(line, "v %lf %lf %lf", &v[0], &v[1], &v[2]) != 3) break;
+        vertices.push_back(v);
+        continue;
+      }
+      any_after = any_after || line[0] != '\n';
+      if (line[0] != 't') break;
+      in_tets = true;
+      VECTOR4I t;
+      if (std::sscanf(line, "t %d %d %d %d", &t[0], &t[1], &t[2], &t[3]) != 4) break;
+      tets.push_back(t);
+    }
+    std::fclose(f);
+    return any_after;
+  }
+  // TET_MESH::writeTobjFile (TET_MESH.cpp:1225-1259) of the rest vertices: %.17g, so they read
+  // back bit for bit.
+  static bool writeTobjFile(const std::string& filename, const TET_MESH& mesh) {
+    FILE* f = std::fopen(filename.c_str(), "w");
+    if (!f) return false;
+    for (const auto& v : mesh.restVertices()) std::fprintf(f, "v %.17g %.17g %.17g\n", v[0], v[1], v[2]);
+    for (const auto& t : mesh.tets()) std::fprintf(f, "t %i %i %i %i\n", t[0], t[1], t[2], t[3]);
+    std::fclose(f);
+    return true;
+  }
 
  private:
   std::vector<VECTOR3> rest_;

@@ -464,6 +464,20 @@ class TetMesh:
     def DOFs(self):
         return 3 * self._rest.shape[0]
 
+    @staticmethod
+    def readTobjFile(path):
+        """TET_MESH::readTobjFile (TET_MESH.cpp:1173-1220): (vertices, tets)."""
+        from .formats import read_tobj
+
+        return read_tobj(path)
+
+    @staticmethod
+    def writeTobjFile(path, mesh):
+        """TET_MESH::writeTobjFile (TET_MESH.cpp:1225-1259) of the rest vertices."""
+        from .formats import write_tobj
+
+        write_tobj(path, mesh.restVertices(), mesh.tets())
+
 
 class HYPERELASTIC:
     kind = -1

@@ -274,6 +274,24 @@ def drop_scene(n=(6, 6, 6), seed=7, E=3.0, nu=0.48, material=MAT_SNH, height=0.0
     return sc
 
 
+def drop_test(tobj_path, pcg_tol=1e-8):
+    """src/Scenes/DROP_TEST.h on a .tobj mesh (the reference uses data/cylinder_10.tobj): SNH
+    E=3 nu=0.48, density 1, gravity (0, -1, 0), every vertex moving down at 3, the ground
+    CUBE(center (0, -5, 0), scale 10), self-collisions off.  The PCG tolerance is the workloads'
+    1e-8 (the scene itself keeps Eigen's default)."""
+    from .formats import read_tobj
+
+    verts, tets = read_tobj(tobj_path)
+    m, l = lame(3.0, 0.48)
+    sc = Scene(name="drop_test", rest=verts, tets=tets, material=MAT_SNH, mu=m, lam=l,
+               mass_scale=1.0, gravity=(0.0, -1.0, 0.0), pin_cube=None, pcg_tol=pcg_tol,
+               collision_cubes=((0.0, -5.0, 0.0, 10.0),))
+    v = np.zeros((verts.shape[0], 3))
+    v[:, 1] = -3.0
+    sc.v0 = v.ravel().copy()
+    return sc
+
+
 def config1(seed=1):
     """10^3 lattice, SNH E=1e5 nu=0.45 rho=1000, x=0 pinned, dt=1/60, 1 Newton iteration."""
     return lattice_scene((10, 10, 10), seed=seed, name="cube10_snh")

@@ -68,6 +68,12 @@ def load_ref():
         lib.ref_step.restype = ctypes.c_int
         lib.ref_last_rhs_b.argtypes = [ctypes.c_void_p, _dp]
         lib.ref_add_collision_cube.argtypes = [ctypes.c_void_p, _dp, ctypes.c_double]
+        lib.ref_read_tobj.argtypes = [ctypes.c_char_p, _ip, _dp, _ip]
+        lib.ref_read_tobj.restype = ctypes.c_int
+        lib.ref_write_tobj.argtypes = [ctypes.c_char_p, _dp, ctypes.c_int, _ip, ctypes.c_int]
+        lib.ref_write_tobj.restype = ctypes.c_int
+        lib.ref_read_scene_json.argtypes = [ctypes.c_char_p, _ip, _dp, _dp, _dp, _ip]
+        lib.ref_read_scene_json.restype = ctypes.c_int
         lib.ref_plane_constraints.argtypes = [ctypes.c_void_p, _ip]
         lib.ref_plane_constraints.restype = ctypes.c_int
         lib.ref_mass_diag.argtypes = [ctypes.c_void_p, _dp]
@@ -121,6 +127,46 @@ class RefElement:
         H = np.zeros(81)
         self.lib.ref_clamped_hessian(mat, mu, lam, _d(_colmajor(F)), _d(H))
         return _from_colmajor(H, 9)
+
+
+def ref_read_tobj(path):
+    """TET_MESH::readTobjFile of the literal reference: (vertices (nv, 3), tets (nt, 4))."""
+    lib = load_ref()
+    nt = ctypes.c_int(0)
+    nv = lib.ref_read_tobj(str(path).encode(), ctypes.byref(nt), None, None)
+    if nv < 0:
+        raise OSError(f"reference reader failed on {path}")
+    v = np.zeros(3 * nv)
+    t = np.zeros(4 * nt.value, np.int32)
+    lib.ref_read_tobj(str(path).encode(), ctypes.byref(nt), _d(v), _i(t))
+    return v.reshape(-1, 3), t.reshape(-1, 4)
+
+
+def ref_write_tobj(path, vertices, tets):
+    """TET_MESH::writeTobjFile of the literal reference (rest vertices)."""
+    lib = load_ref()
+    v = _c64(np.asarray(vertices).ravel())
+    t = _c32(np.asarray(tets).ravel())
+    if lib.ref_write_tobj(str(path).encode(), _d(v), v.size // 3, _i(t), t.size // 4) != 0:
+        raise OSError(f"reference writer failed on {path}")
+
+
+def ref_read_scene_json(path):
+    """readSceneJSON of the literal reference (an uncompressed record): positions and velocities
+    per frame, gravity and the number of kinematic shapes."""
+    lib = load_ref()
+    dofs = ctypes.c_int(0)
+    nf = lib.ref_read_scene_json(str(path).encode(), ctypes.byref(dofs), None, None, None, None)
+    if nf < 0:
+        raise OSError(f"reference reader failed on {path}")
+    pos = np.zeros(nf * dofs.value)
+    vel = np.zeros(nf * dofs.value)
+    g = np.zeros(3)
+    ns = ctypes.c_int(0)
+    lib.ref_read_scene_json(str(path).encode(), ctypes.byref(dofs), _d(pos), _d(vel), _d(g),
+                            ctypes.byref(ns))
+    return {"positions": pos.reshape(nf, -1), "velocities": vel.reshape(nf, -1), "gravity": g,
+            "nshapes": ns.value}
 
 
 class RefSim:

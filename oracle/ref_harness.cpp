@@ -14,6 +14,8 @@
 //                    objects, here CUBEs: src/Geometry/CUBE.cpp:23-193)
 //   materials        src/Hyperelastic/Volume/SNH.cpp, ARAP.cpp, HYPERELASTIC.cpp
 //   SVD              src/util/MATRIX_UTIL.cpp:76-95 (svd_rv)
+//   file formats     .tobj meshes (src/Geometry/TET_MESH.cpp:1174-1259), the JSON scene record
+//                    (src/util/FILE_IO.cpp:393-625)
 //
 // The harness subclasses BACKWARD_EULER_POSITION and BDF_1 only to reach protected members: the
 // Eigen CG object (src/Timestepper/TIMESTEPPER.h:207), so that the PCG tolerance can be set as the
@@ -24,6 +26,7 @@
 #include <vector>
 
 #include "Geometry/CUBE.h"
+#include "util/FILE_IO.h"
 #include "Geometry/TET_MESH_FASTER.h"
 #include "Hyperelastic/Volume/ARAP.h"
 #include "Hyperelastic/Volume/SNH.h"
@@ -323,6 +326,53 @@ void ref_clamped_hessian(int material, double mu, double lambda, const double* F
   for (int j = 0; j < 9; j++)
     for (int i = 0; i < 9; i++) H[9 * j + i] = h(i, j);
   delete m;
+}
+
+// ---- file formats ----
+// TET_MESH::readTobjFile: returns the vertex count and fills *nt; verts/tets (may be NULL) get
+// nv*3 doubles and nt*4 ints.  Returns -1 when the file cannot be read.
+int ref_read_tobj(const char* path, int* nt, double* verts, int* tets) {
+  std::vector<VECTOR3> v;
+  std::vector<VECTOR4I> t;
+  if (!TET_MESH::readTobjFile(path, v, t)) return -1;
+  *nt = (int)t.size();
+  if (verts)
+    for (size_t i = 0; i < v.size(); i++)
+      for (int c = 0; c < 3; c++) verts[3 * i + c] = v[i][c];
+  if (tets)
+    for (size_t i = 0; i < t.size(); i++)
+      for (int c = 0; c < 4; c++) tets[4 * i + c] = t[i][c];
+  return (int)v.size();
+}
+
+// TET_MESH::writeTobjFile of a mesh with these rest vertices.
+int ref_write_tobj(const char* path, const double* verts, int nv, const int* tets, int nt) {
+  std::vector<VECTOR3> v(nv);
+  for (int i = 0; i < nv; i++) v[i] = VECTOR3(verts[3 * i], verts[3 * i + 1], verts[3 * i + 2]);
+  std::vector<VECTOR4I> t(nt);
+  for (int i = 0; i < nt; i++) t[i] = VECTOR4I(tets[4 * i], tets[4 * i + 1], tets[4 * i + 2], tets[4 * i + 3]);
+  TET_MESH mesh(v, t);
+  return TET_MESH::writeTobjFile(path, mesh, true) ? 0 : -1;
+}
+
+// readSceneJSON (FILE_IO.cpp:522-625) of an uncompressed record: the frame count and DOFs, and
+// (when the arrays are given) every frame's position and velocity, the gravity and the number of
+// kinematic shapes.
+int ref_read_scene_json(const char* path, int* dofs, double* positions, double* velocities,
+                        double* gravity, int* nshapes) {
+  JSON_SCENE scene;
+  if (!readSceneJSON(path, scene)) return -1;
+  const int nf = (int)scene.positions().size();
+  *dofs = nf ? (int)scene.positions()[0].size() : 0;
+  for (int f = 0; f < nf; f++)
+    for (int i = 0; i < *dofs; i++) {
+      if (positions) positions[(size_t)f * *dofs + i] = scene.positions()[f][i];
+      if (velocities) velocities[(size_t)f * *dofs + i] = scene.velocities()[f][i];
+    }
+  if (gravity)
+    for (int c = 0; c < 3; c++) gravity[c] = scene.gravity()[c];
+  if (nshapes) *nshapes = (int)scene.kinematicShapes().size();
+  return nf;
 }
 
 }  // extern "C"

@@ -182,6 +182,38 @@ def make_c1_bdf1(out):
     np.savez_compressed(out, **data)
 
 
+def make_drop_test(out):
+    """src/Scenes/DROP_TEST.h itself: the reference's cylinder_10.tobj (copied next to this
+    script) dropped onto the ground CUBE, 60 steps (PCG tolerance 1e-8)."""
+    sc = scenes.drop_test(os.path.join(HERE, "cylinder_10.tobj"))
+    _drop_record(sc, 60, (1, 5, 10, 20, 40, 60), out)
+
+
+def _drop_record(sc, steps, keep, out):
+    r = oracle.RefSim(sc)
+    iters, counts, ids = [], [], []
+    us, vs = {}, {}
+    for k in range(1, steps + 1):
+        it, err, sec = r.step()
+        u, v = r.state()
+        pc = r.plane_constraints()
+        iters.append(it)
+        counts.append(len(pc))
+        ids.append(np.sort(pc))
+        if k in keep:
+            us[k] = u.copy()
+            vs[k] = v.copy()
+        print(f"  step {k}: iters {it} constraints {len(pc)}", flush=True)
+    r.close()
+    data = dict(iters=np.array(iters), counts=np.array(counts),
+                plane_ids=np.concatenate(ids).astype(np.int32),
+                plane_offs=np.concatenate([[0], np.cumsum(counts)]).astype(np.int32))
+    for k, u in us.items():
+        data[f"u{k}"] = u
+        data[f"v{k}"] = vs[k]
+    np.savez_compressed(out, **data)
+
+
 def make_drop(out):
     """A lattice dropped onto a kinematic ground CUBE (scenes.drop_scene, after
     src/Scenes/DROP_TEST.h): 40 steps of the reference's plane constraints (contact, sliding and
@@ -222,7 +254,7 @@ def main():
         sys.exit("oracle/_ref/libhobak_ref.so is not built (make -C oracle ref)")
     jobs = [("element_fixtures.npz", make_element), ("mesh_small.npz", make_mesh_small),
             ("c1_traj.npz", make_c1), ("c1_arap.npz", make_c1_arap), ("c1_bdf1.npz", make_c1_bdf1),
-            ("drop_traj.npz", make_drop)]
+            ("drop_traj.npz", make_drop), ("drop_test_cylinder.npz", make_drop_test)]
     if not args.skip_c2:
         jobs.append(("c2_traj.npz", make_c2))
     for name, fn in jobs:

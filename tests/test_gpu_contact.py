@@ -6,11 +6,13 @@ tests/golden/make_golden.py from the literal reference) and against the oracle's
 set of constrained vertices is discrete and must be identical every step.  A contact instance in
 a batch computes bitwise what it computes alone.
 """
+import os
+
 import numpy as np
 import pytest
 
 import oracle
-from conftest import golden
+from conftest import GOLDEN, golden
 from helpers import rel
 from hobakv1_amd import lib, scenes
 
@@ -100,3 +102,25 @@ def test_collision_cube_needs_fp64_single_newton():
     with pytest.raises(lib.HobakError):
         lib.Instance(ctx, sc, newton_iters=3)
     ctx.close()
+
+
+def test_drop_test_cylinder_matches_reference(gpu_ctx):
+    """src/Scenes/DROP_TEST.h on the reference's cylinder_10.tobj (read by
+    hobakv1_amd.formats.read_tobj), Jacobi PCG, 60 steps against the reference: the same plane
+    constraint set and PCG iterations +-1 every step, positions within 1e-9 at steps 1, 5, 10,
+    20, 40 and 60."""
+    g = golden("drop_test_cylinder.npz")
+    sc = scenes.drop_test(os.path.join(GOLDEN, "cylinder_10.tobj"))
+    _say(f"DROP_TEST cylinder nv {sc.nv} nt {sc.nt}, 60 steps")
+    inst = lib.Instance(gpu_ctx, sc, precond=scenes.PRECOND_JACOBI)
+    X = sc.rest.ravel()
+    for k in range(1, 61):
+        st = inst.step()
+        assert abs(st["pcg_iters"][0] - int(g["iters"][k - 1])) <= 1, (k, st["pcg_iters"][0])
+        want = g["plane_ids"][g["plane_offs"][k - 1]:g["plane_offs"][k]]
+        assert np.array_equal(inst.plane_constraints(), want), k
+        if f"u{k}" in g.files:
+            u, v = inst.state()
+            assert rel(X + u, X + g[f"u{k}"]) <= 1e-9, k
+            assert rel(u, g[f"u{k}"]) <= 1e-9, k
+    inst.close()

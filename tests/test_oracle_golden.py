@@ -4,11 +4,13 @@ The fixtures were written by tests/golden/make_golden.py from oracle/_ref, the r
 from its own sources.  These tests are what makes the oracle trustworthy as the checker of the
 GPU path.
 """
+import os
+
 import numpy as np
 import pytest
 
 import oracle
-from conftest import golden
+from conftest import GOLDEN, golden
 from helpers import bsr_entry_rel, rel, rel_finite
 from hobakv1_amd import scenes
 
@@ -157,3 +159,20 @@ def test_drop_plane_constraints_match_reference():
             u, v = o.state()
             assert rel(u, g[f"u{k}"]) <= 1e-10, k
             assert rel(v, g[f"v{k}"]) <= 1e-10, k
+
+
+def test_drop_test_cylinder_matches_reference():
+    """src/Scenes/DROP_TEST.h on the reference's own cylinder_10.tobj, 60 steps: the same plane
+    constraint set and PCG iterations every step, positions within 1e-9 (contact events amplify
+    round-off slowly: ~1e-10 by step 60)."""
+    g = golden("drop_test_cylinder.npz")
+    sc = scenes.drop_test(os.path.join(GOLDEN, "cylinder_10.tobj"))
+    o = oracle.Oracle(sc, precond=scenes.PRECOND_JACOBI)
+    for k in range(1, 61):
+        it, _, _ = o.step()
+        assert abs(it - int(g["iters"][k - 1])) <= 1, k
+        want = g["plane_ids"][g["plane_offs"][k - 1]:g["plane_offs"][k]]
+        assert np.array_equal(np.sort(o.plane_constraints()), want), k
+        if f"u{k}" in g.files:
+            u, v = o.state()
+            assert rel(u, g[f"u{k}"]) <= 1e-9, k
