@@ -534,7 +534,7 @@ int launch_prologue(hobak_ctx* ctx, const Plan& J, const Plan* S, bool timed) {
   const int cap = ctx->grid_cap;
   if (S) HIPCHK(ctx, launch_begin_step(S->b[K_ROW], cap, s));
   if (S && S->contact) HIPCHK(ctx, launch_contact_begin(S->b[K_ROW], cap, s));
-  HIPCHK(ctx, launch_element<T>(J.b[K_ELEM], ELEM_U, 1, cap, s));
+  HIPCHK(ctx, launch_element<T>(J.b[K_ELEM], ELEM_U, 1, 1, cap, s));
   if (timed) HIPCHK(ctx, hipEventRecord(ctx->ev[1], s));
   HIPCHK(ctx, launch_assemble<T>(J.b[K_SLOT], ASM_SYSTEM, cap, s));
   if (timed) HIPCHK(ctx, hipEventRecord(ctx->ev[2], s));
@@ -943,7 +943,7 @@ int setup_constant_parts(hobak_inst* I) {
   TRY(plan_transient(ctx, {I}, P));
   // rest-state stiffness (u = 0): TIMESTEPPER::buildRayleighDampingMatrix (TIMESTEPPER.cpp:213-236)
   HIPCHK(ctx, hipMemsetAsync(I->h.tmp64, 0, sizeof(double) * I->dofs, s));
-  HIPCHK(ctx, launch_element<T>(P.b[K_ELEM], ELEM_TMP, 1, ctx->grid_cap, s));
+  HIPCHK(ctx, launch_element<T>(P.b[K_ELEM], ELEM_TMP, 1, 0, ctx->grid_cap, s));
   HIPCHK(ctx, launch_assemble<T>(P.b[K_SLOT], ASM_REST, ctx->grid_cap, s));
   HIPCHK(ctx, launch_constant_parts<T>(P.b[K_SLOT], ctx->grid_cap, s));
   HIPCHK(ctx, hipStreamSynchronize(s));
@@ -979,7 +979,7 @@ int eval_forces_hessian(hobak_inst* I, const double* u, double* forces, double* 
   Plan P;
   TRY(plan_transient(ctx, {I}, P));
   TRY(upload(I, I->h.tmp64, u, (size_t)I->dofs));
-  HIPCHK(ctx, launch_element<T>(P.b[K_ELEM], ELEM_TMP, kvals != nullptr, ctx->grid_cap, s));
+  HIPCHK(ctx, launch_element<T>(P.b[K_ELEM], ELEM_TMP, kvals != nullptr, 0, ctx->grid_cap, s));
   if (forces) {
     HIPCHK(ctx, launch_force_gather<T>(P.b[K_ROW], ctx->grid_cap, s));
     HIPCHK(ctx, hipMemcpyAsync(forces, I->h.rhs64, sizeof(double) * I->dofs, hipMemcpyDeviceToHost, s));
@@ -988,6 +988,31 @@ int eval_forces_hessian(hobak_inst* I, const double* u, double* forces, double* 
   if (kvals) {
     HIPCHK(ctx, launch_assemble<T>(P.b[K_SLOT], ASM_PLAIN_A, ctx->grid_cap, s));
     TRY(download_sell_as_bsr<T>(I, I->h.A, kvals, -1.0));
+  }
+  HIPCHK(ctx, hipStreamSynchronize(s));
+  return HOBAK_OK;
+}
+
+// computeInternalForces(hyperelastic, damping) and TET_MESH_FASTER::computeDampingHessian at u
+// with the held velocity gradients.
+template <typename T>
+int eval_damping(hobak_inst* I, const double* u, double* forces, double* cvals) {
+  hobak_ctx* ctx = I->ctx;
+  hipStream_t s = ctx->stream;
+  ctx->arena_off = 0;
+  Plan P;
+  TRY(plan_transient(ctx, {I}, P));
+  TRY(upload(I, I->h.tmp64, u, (size_t)I->dofs));
+  I->has_system = false;  // rhs64 and A are reused
+  if (forces) {
+    HIPCHK(ctx, launch_element<T>(P.b[K_ELEM], ELEM_TMP, 0, 1, ctx->grid_cap, s));
+    HIPCHK(ctx, launch_force_gather<T>(P.b[K_ROW], ctx->grid_cap, s));
+    HIPCHK(ctx, hipMemcpyAsync(forces, I->h.rhs64, sizeof(double) * I->dofs, hipMemcpyDeviceToHost, s));
+  }
+  if (cvals) {
+    HIPCHK(ctx, launch_damping_hessian<T>(P.b[K_ELEM], ELEM_TMP, ctx->grid_cap, s));
+    HIPCHK(ctx, launch_assemble<T>(P.b[K_SLOT], ASM_PLAIN_A, ctx->grid_cap, s));
+    TRY(download_sell_as_bsr<T>(I, I->h.A, cvals, -1.0));
   }
   HIPCHK(ctx, hipStreamSynchronize(s));
   return HOBAK_OK;
@@ -1404,6 +1429,53 @@ int hobak_inst_plane_constraints(hobak_inst* I, int32_t* count, int32_t* ids, in
   return HOBAK_OK;
 }
 
+int hobak_inst_set_damping(hobak_inst* I, int32_t model, double mu) {
+  if (!I) return HOBAK_E_INVALID;
+  hobak_ctx* ctx = I->ctx;
+  if (I->requests) return fail(ctx, HOBAK_E_INVALID, "instance has a step in flight");
+  if (model != HOBAK_DAMPING_RAYLEIGH && model != HOBAK_DAMPING_GREEN)
+    return fail(ctx, HOBAK_E_INVALID, "unknown damping model");
+  if (model == HOBAK_DAMPING_GREEN && I->h.bdf)
+    return fail(ctx, HOBAK_E_INVALID, "energy damping needs one Newton iteration per step (the "
+                                      "reference's BDF_1 has no energy-damped solve)");
+  if (model == HOBAK_DAMPING_GREEN && !std::isfinite(mu)) return fail(ctx, HOBAK_E_INVALID, "damping mu must be finite");
+  (void)hipSetDevice(ctx->device);
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  InstDev& h = I->h;
+  if (model == HOBAK_DAMPING_GREEN && !h.fdot) {
+    double* fd;
+    TRY(dalloc(I, &fd, 9 * (size_t)I->nt));
+    HIPCHK(ctx, hipMemsetAsync(fd, 0, sizeof(double) * 9 * (size_t)I->nt, ctx->stream));  // held Fdot = 0
+    h.fdot = fd;
+  }
+  h.damping = model == HOBAK_DAMPING_GREEN;
+  h.dmu = h.damping ? mu : 0.0;
+  // solveEnergyDamped has no Rayleigh terms: b = (M/dt) v + R + f, A = M/dt^2 - K
+  h.ra = h.damping ? 0.0 : I->prm.rayleigh_alpha;
+  h.rb = h.damping ? 0.0 : I->prm.rayleigh_beta;
+  TRY(upload(I, I->dev, &h, 1));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  TRY(I->tsize == 8 ? setup_constant_parts<double>(I) : setup_constant_parts<float>(I));
+  plans_forget(ctx, I);
+  return HOBAK_OK;
+}
+
+int hobak_inst_compute_fdots(hobak_inst* I, const double* velocity) {
+  if (!I) return HOBAK_E_INVALID;
+  hobak_ctx* ctx = I->ctx;
+  if (I->requests) return fail(ctx, HOBAK_E_INVALID, "instance has a step in flight");
+  if (!I->h.fdot) return fail(ctx, HOBAK_E_INVALID, "velocity gradients need energy damping (hobak_inst_set_damping)");
+  (void)hipSetDevice(ctx->device);
+  const double* vel = I->h.v;
+  if (velocity) {
+    TRY(upload(I, I->h.tmp64, velocity, (size_t)I->dofs));
+    vel = I->h.tmp64;
+  }
+  HIPCHK(ctx, launch_fdots(I->h, vel, (double*)I->h.fdot, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return HOBAK_OK;
+}
+
 int hobak_inst_set_state(hobak_inst* I, const double* u, const double* v) {
   if (!I) return HOBAK_E_INVALID;
   if (I->requests) return fail(I->ctx, HOBAK_E_INVALID, "instance has a step in flight");
@@ -1477,6 +1549,14 @@ int hobak_eval_forces_hessian(hobak_inst* I, const double* u, double* forces, do
   (void)hipSetDevice(I->ctx->device);
   return I->tsize == 8 ? eval_forces_hessian<double>(I, u, forces, kvals)
                        : eval_forces_hessian<float>(I, u, forces, kvals);
+}
+
+int hobak_eval_damping(hobak_inst* I, const double* u, double* forces, double* cvals) {
+  if (!I || !u) return HOBAK_E_INVALID;
+  if (I->requests) return fail(I->ctx, HOBAK_E_INVALID, "instance has a step in flight");
+  if (!I->h.damping) return fail(I->ctx, HOBAK_E_INVALID, "the instance has no energy damping");
+  (void)hipSetDevice(I->ctx->device);
+  return I->tsize == 8 ? eval_damping<double>(I, u, forces, cvals) : eval_damping<float>(I, u, forces, cvals);
 }
 
 int hobak_get_last_system(hobak_inst* I, double* rhs, double* lhs_vals) {

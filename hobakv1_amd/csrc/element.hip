@@ -7,6 +7,11 @@
 //   TET_MESH::computeHyperelasticForces   src/Geometry/TET_MESH.cpp:736-770 (per-tet part)
 //   TET_MESH_FASTER::computeHyperelasticClampedHessian element loop
 //                                         src/Geometry/TET_MESH_FASTER.cpp:93-103
+//   TET_MESH::computeInternalForces       src/Geometry/TET_MESH.cpp:816-858 (per-tet part: the
+//                                         GREEN_DAMPING stress added to the elastic one)
+//   TET_MESH::computeFdots                src/Geometry/TET_MESH.cpp:660-684 (fdot_kernel)
+//   TET_MESH_FASTER::computeDampingHessian element loop, src/Geometry/TET_MESH_FASTER.cpp:144-150
+//                                         (damping_hessian_kernel)
 //
 // The reference forms the 12x12 stiffness as dF/dx^T (-V H9) dF/dx with dense 9x12 products.
 // Here the clamped H9 is kept in the singular-value frame, where it is block diagonal (a 3x3
@@ -34,8 +39,32 @@ struct ElementIn {
   HOBAK_GLOBAL void* stage_f;
   const HOBAK_GLOBAL int* hdest;
   const HOBAK_GLOBAL int* fdest;
-  int with_hessian;
+  const HOBAK_GLOBAL double* fdot;  // held velocity gradients (damp != 0)
+  double dmu;
+  int with_hessian, damp;
 };
+
+// GREEN_DAMPING stress mu F (F^T Fdot + Fdot^T F) (src/Damping/Volume/GREEN_DAMPING.cpp:49-52),
+// with Fdot read from its 9 planes.
+__device__ __forceinline__ void green_pk1(double mu, const double (&F)[3][3], const HOBAK_GLOBAL double* fdot,
+                                          size_t nt, int t, double (&P)[3][3]) {
+  double Fd[3][3];
+#pragma unroll
+  for (int r = 0; r < 3; r++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) Fd[r][c] = fdot[(3 * r + c) * nt + t];
+  double X[3][3];  // F^T Fdot
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) X[i][j] = (F[0][i] * Fd[0][j] + F[1][i] * Fd[1][j]) + F[2][i] * Fd[2][j];
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+      P[i][j] = ((mu * F[i][0]) * (X[0][j] + X[j][0]) + (mu * F[i][1]) * (X[1][j] + X[j][1])) +
+                (mu * F[i][2]) * (X[2][j] + X[j][2]);
+}
 
 template <typename T, int MAT>
 __device__ __forceinline__ void element_tet(const ElementIn& a, int t) {
@@ -103,6 +132,14 @@ __device__ __forceinline__ void element_tet(const ElementIn& a, int t) {
     snh_pk1<T>((T)a.mu, (T)a.lam, (T)a.alpha, Ft, P);
   } else {
     arap_pk1<T>((T)a.mu, U, S, V, P);
+  }
+  if (a.damp) {  // computeInternalForces: flatten(elasticPK1 + dampingPK1)
+    double Pd[3][3];
+    green_pk1(a.dmu, F, a.fdot, nt, t, Pd);
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+      for (int j = 0; j < 3; j++) P[i][j] += (T)Pd[i][j];
   }
   auto sf = (HOBAK_GLOBAL T*)a.stage_f;
 #pragma unroll
@@ -178,9 +215,11 @@ __device__ __forceinline__ void element_tet(const ElementIn& a, int t) {
 }
 
 // Segmented over instances (one lane per tet, kElementBlock tets per virtual workgroup); the
-// material is a per-instance, workgroup-uniform branch.
+// material is a per-instance, workgroup-uniform branch.  damp: add the damping stress of
+// instances that use energy damping (the step and the damped-force hook; the elastic-force hook
+// and the rest-state stiffness pass 0).
 template <typename T>
-__global__ __launch_bounds__(kElementBlock) void element_kernel(Batch b, int src, int with_hessian) {
+__global__ __launch_bounds__(kElementBlock) void element_kernel(Batch b, int src, int with_hessian, int damp) {
   for (int ws = blockIdx.x; ws < b.total; ws += gridDim.x) {
     const int vb = xcd_vb(ws, b.total);
     int lb, lg;
@@ -200,6 +239,9 @@ __global__ __launch_bounds__(kElementBlock) void element_kernel(Batch b, int src
     a.hdest = gbl(I->hdest);
     a.fdest = gbl(I->fdest);
     a.with_hessian = with_hessian;
+    a.damp = damp && I->damping;
+    a.fdot = gbl(I->fdot);
+    a.dmu = I->dmu;
     const int material = I->material;
     const int t = lb * kElementBlock + threadIdx.x;
     if (t < a.nt) {
@@ -212,14 +254,130 @@ __global__ __launch_bounds__(kElementBlock) void element_kernel(Batch b, int src
 }
 
 template <typename T>
-hipError_t launch_element(const Batch& b, int src, int with_hessian, int cap, hipStream_t s) {
+hipError_t launch_element(const Batch& b, int src, int with_hessian, int damp, int cap, hipStream_t s) {
   if (b.total <= 0) return hipSuccess;
   const int grid = cap > 0 && cap < b.total ? cap : b.total;
-  hipLaunchKernelGGL((element_kernel<T>), dim3(grid), dim3(kElementBlock), 0, s, b, src, with_hessian);
+  hipLaunchKernelGGL((element_kernel<T>), dim3(grid), dim3(kElementBlock), 0, s, b, src, with_hessian, damp);
   return hipGetLastError();
 }
 
-template hipError_t launch_element<double>(const Batch&, int, int, int, hipStream_t);
-template hipError_t launch_element<float>(const Batch&, int, int, int, hipStream_t);
+template hipError_t launch_element<double>(const Batch&, int, int, int, int, hipStream_t);
+template hipError_t launch_element<float>(const Batch&, int, int, int, int, hipStream_t);
+
+// TET_MESH::computeFdots: one lane per tet, Fdot = V DmInv with V's columns v_k - v_0
+// (TET_MESH.cpp:666-682), the product in the same summation order as F = Ds DmInv.
+__global__ __launch_bounds__(kBlock) void fdot_kernel(int nt, const int4* __restrict__ tets,
+                                                      const double* __restrict__ dminv,
+                                                      const double* __restrict__ vel,
+                                                      double* __restrict__ fdot) {
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= nt) return;
+  const size_t n = (size_t)nt;
+  const int4 q = tets[t];
+  const int vid[4] = {q.x, q.y, q.z, q.w};
+  double w[4][3];
+#pragma unroll
+  for (int k = 0; k < 4; k++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) w[k][c] = vel[3 * (size_t)vid[k] + c];
+#pragma unroll
+  for (int r = 0; r < 3; r++)
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+      fdot[(3 * r + c) * n + t] = ((w[1][r] - w[0][r]) * dminv[(3 * 0 + c) * n + t] +
+                                   (w[2][r] - w[0][r]) * dminv[(3 * 1 + c) * n + t]) +
+                                  (w[3][r] - w[0][r]) * dminv[(3 * 2 + c) * n + t];
+}
+
+hipError_t launch_fdots(const InstDev& h, const double* vel, double* fdot, hipStream_t s) {
+  if (h.nt <= 0) return hipSuccess;
+  hipLaunchKernelGGL(fdot_kernel, dim3((h.nt + kBlock - 1) / kBlock), dim3(kBlock), 0, s, h.nt,
+                     (const int4*)h.tets, h.dminv, vel, fdot);
+  return hipGetLastError();
+}
+
+// TET_MESH_FASTER::computeDampingHessian, per tet: the stiffness blocks of vol * dP/dFdot with
+// dP/dFdot = mu (F F^T (x) I + F^T-coupling) (GREEN_DAMPING.cpp:58-72).  Block (v, w) of
+// dF/dx^T H dF/dx is
+//   mu vol ( (F F^T) (d_v . d_w) + (F d_w) (F d_v)^T ),
+// d_v the row of dF/dx for corner v.  H is positive semi-definite (it is the Hessian of the
+// convex quadratic mu |sym(F^T Fdot)|^2 in Fdot), so the reference's eigenvalue clamp
+// (GREEN_DAMPING.cpp:79-88) changes it only by round-off and is not repeated here.  Staged with
+// the positive sign, like the element stiffness; the assembled result is -C.
+template <typename T>
+__global__ __launch_bounds__(kElementBlock) void damping_hessian_kernel(Batch b, int src) {
+  for (int ws = blockIdx.x; ws < b.total; ws += gridDim.x) {
+    const int vb = xcd_vb(ws, b.total);
+    int lb, lg;
+    const HOBAK_GLOBAL InstDev* __restrict__ I = seg_find(b, vb, lb, lg);
+    const int t = lb * kElementBlock + threadIdx.x;
+    const int nt = I->nt;
+    if (t >= nt) continue;
+    const size_t n = (size_t)nt;
+    auto tets = gbl((const int*)I->tets);
+    auto rest = gbl(I->rest);
+    auto u = gbl(src == ELEM_TMP ? I->tmp64 : I->u);
+    auto dminv = gbl(I->dminv);
+    const int vid[4] = {tets[4 * (size_t)t], tets[4 * (size_t)t + 1], tets[4 * (size_t)t + 2],
+                        tets[4 * (size_t)t + 3]};
+    double x[4][3], Di[3][3], F[3][3];
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+#pragma unroll
+      for (int c = 0; c < 3; c++) x[k][c] = rest[3 * (size_t)vid[k] + c] + u[3 * (size_t)vid[k] + c];
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+      for (int c = 0; c < 3; c++) Di[r][c] = dminv[(3 * r + c) * n + t];
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+      for (int c = 0; c < 3; c++)
+        F[r][c] = ((x[1][r] - x[0][r]) * Di[0][c] + (x[2][r] - x[0][r]) * Di[1][c]) +
+                  (x[3][r] - x[0][r]) * Di[2][c];
+    double d[4][3];
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      d[0][j] = (-Di[0][j] - Di[1][j]) - Di[2][j];
+      d[1][j] = Di[0][j];
+      d[2][j] = Di[1][j];
+      d[3][j] = Di[2][j];
+    }
+    double G[3][3], g[4][3];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+      for (int j = 0; j < 3; j++) G[i][j] = (F[i][0] * F[j][0] + F[i][1] * F[j][1]) + F[i][2] * F[j][2];
+#pragma unroll
+    for (int v = 0; v < 4; v++)
+#pragma unroll
+      for (int i = 0; i < 3; i++) g[v][i] = (F[i][0] * d[v][0] + F[i][1] * d[v][1]) + F[i][2] * d[v][2];
+    const double s = I->dmu * gbl(I->vol)[t];
+    auto sh = gbl((T*)I->stage_h);
+    auto hd = gbl(I->hdest) + 16 * (size_t)t;
+#pragma unroll
+    for (int v = 0; v < 4; v++)
+#pragma unroll
+      for (int w = 0; w < 4; w++) {
+        const double dd = (d[v][0] * d[w][0] + d[v][1] * d[w][1]) + d[v][2] * d[w][2];
+        auto dst = sh + 9 * (size_t)hd[4 * v + w];
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+#pragma unroll
+          for (int c2 = 0; c2 < 3; c2++) dst[3 * c + c2] = (T)(s * (G[c][c2] * dd + g[w][c] * g[v][c2]));
+      }
+  }
+}
+
+template <typename T>
+hipError_t launch_damping_hessian(const Batch& b, int src, int cap, hipStream_t s) {
+  if (b.total <= 0) return hipSuccess;
+  const int grid = cap > 0 && cap < b.total ? cap : b.total;
+  hipLaunchKernelGGL((damping_hessian_kernel<T>), dim3(grid), dim3(kElementBlock), 0, s, b, src);
+  return hipGetLastError();
+}
+
+template hipError_t launch_damping_hessian<double>(const Batch&, int, int, hipStream_t);
+template hipError_t launch_damping_hessian<float>(const Batch&, int, int, hipStream_t);
 
 }  // namespace hobak

@@ -152,6 +152,12 @@ struct InstDev {
   unsigned char* fmode;       // per vertex: this step's filter block, 0 I, 1 I - n n^T, 2 zero
   double *fnrm, *zc, *bunf;   // per vertex: the filter's normal, z, the unfiltered b
   double* part_sep;           // K_ROW: separating-constraint counts
+  // Energy-based damping (GREEN_DAMPING) in place of Rayleigh damping: the step adds the damping
+  // stress mu_d F (F^T Fdot + Fdot^T F) at the held per-tet velocity gradients (9 planes of nt,
+  // row-major entry (r, c) in plane 3 r + c) to the elastic stress; ra = rb = 0 then.
+  int damping;
+  double dmu;
+  const double* fdot;
 };
 
 // The control block lives in the same allocation as the record, kCtlOffset bytes after it, so a
@@ -238,7 +244,13 @@ enum ElemSrc { ELEM_U = 0, ELEM_TMP = 1 };
 enum AsmTarget { ASM_REST = 0, ASM_PLAIN_A = 1, ASM_SYSTEM = 2 };
 
 template <typename T>
-hipError_t launch_element(const Batch& b, int src, int with_hessian, int cap, hipStream_t s);
+hipError_t launch_element(const Batch& b, int src, int with_hessian, int damp, int cap, hipStream_t s);
+// Velocity gradients Fdot = [v1 - v0, v2 - v0, v3 - v0] DmInv of one instance's tets.
+hipError_t launch_fdots(const InstDev& h, const double* vel, double* fdot, hipStream_t s);
+// The damping stiffness blocks vol * dP/dFdot (positive semi-definite) of every tet at the
+// displacement `src` selects, staged for the assembly like the element stiffness.
+template <typename T>
+hipError_t launch_damping_hessian(const Batch& b, int src, int cap, hipStream_t s);
 hipError_t launch_begin_step(const Batch& b, int cap, hipStream_t s);
 hipError_t launch_end_step(const Batch& b, int cap, hipStream_t s);
 // plane constraints: new ones, the filter and z (prologue); the separation test (epilogue)

@@ -5,9 +5,10 @@
 //   facade_driver <scene.bin> <out.bin>        (HOBAK_FACADE_PRECOND=0: Jacobi PCG)
 //
 // scene.bin (little-endian): int32 nv, nt, material, steps, flags (1: pin cube, 2: collision
-// cube); float64 mu, lambda, dt, rayleigh alpha, beta, mass scale, gravity[3], pcg tol, pin cube
-// center[3] and scale, collision cube center[3] and scale; then rest (nv*3 float64), tets (nt*4
-// int32), u0 and v0 (3*nv float64 each).
+// cube, 4: GREEN_DAMPING); float64 mu, lambda, dt, rayleigh alpha, beta, mass scale, gravity[3],
+// pcg tol, pin cube center[3] and scale, collision cube center[3] and scale; then rest (nv*3
+// float64), tets (nt*4 int32), u0 and v0 (3*nv float64 each); with flag 4 also the damping mu
+// (float64) and the velocity given to TET_MESH::computeFdots before the first step (3*nv float64).
 // out.bin: int32 steps, iterations[steps], plane-constraint counts[steps]; float64 u[3*nv],
 // v[3*nv].
 #include <cstdio>
@@ -40,8 +41,10 @@ int main(int argc, char** argv) {
     if (!rd(f, v.data(), 3)) return 2;
   for (auto& t : tets)
     if (!rd(f, t.data(), 4)) return 2;
-  std::vector<double> u0(3 * (size_t)nv), v0(3 * (size_t)nv);
+  std::vector<double> u0(3 * (size_t)nv), v0(3 * (size_t)nv), vf(3 * (size_t)nv);
   if (!rd(f, u0.data(), u0.size()) || !rd(f, v0.data(), v0.size())) return 2;
+  double dmu = 0.0;
+  if ((flags & 4) && (!rd(f, &dmu, 1) || !rd(f, vf.data(), vf.size()))) return 2;
   std::fclose(f);
 
   try {
@@ -62,6 +65,11 @@ int main(int argc, char** argv) {
     if (flags & 1) solver.attachKinematicConstraints(&cube);
     hobak::CUBE ground({prm[14], prm[15], prm[16]}, prm[17]);
     if (flags & 2) solver.addKinematicCollisionObject(&ground);
+    hobak::VOLUME::GREEN_DAMPING damping(dmu);
+    if (flags & 4) {
+      solver.setDamping(&damping);
+      mesh.computeFdots(vf);
+    }
     solver.position() = u0;
     solver.velocity() = v0;
     std::vector<int32_t> iters, planes;

@@ -73,6 +73,14 @@ class TET_MESH {
   const std::vector<VECTOR3>& restVertices() const { return rest_; }
   const std::vector<VECTOR4I>& tets() const { return tets_; }
   int DOFs() const { return 3 * (int)rest_.size(); }
+  // TET_MESH::computeFdots (TET_MESH.cpp:660-684): the velocity gradients an energy-damped step
+  // reads.  An integrator on this mesh takes them to the device at its next solve().
+  void computeFdots(const VECTOR& velocity) {
+    fdotVelocity_ = velocity;
+    fdotSerial_++;
+  }
+  const VECTOR& fdotVelocity() const { return fdotVelocity_; }
+  long fdotSerial() const { return fdotSerial_; }
 
   // TET_MESH::readTobjFile (TET_MESH.cpp:1173-1220): a run of "v x y z" lines, then a run of
   // "t a b c d" lines; false if the file cannot be opened or ends right after its vertices.
@@ -115,6 +123,8 @@ class TET_MESH {
  private:
   std::vector<VECTOR3> rest_;
   std::vector<VECTOR4I> tets_;
+  VECTOR fdotVelocity_;
+  long fdotSerial_ = 0;
 };
 
 namespace VOLUME {
@@ -143,6 +153,16 @@ class ARAP : public HYPERELASTIC {
   using HYPERELASTIC::HYPERELASTIC;
   std::string name() const override { return "ARAP"; }
   int kind() const override { return HOBAK_MAT_ARAP; }
+};
+// VOLUME::GREEN_DAMPING(mu) (src/Damping/Volume/GREEN_DAMPING.cpp:25-28).
+class GREEN_DAMPING {
+ public:
+  explicit GREEN_DAMPING(double mu) : mu_(mu) {}
+  std::string name() const { return "GREEN_DAMPING"; }
+  double mu() const { return mu_; }
+
+ private:
+  double mu_;
 };
 }  // namespace VOLUME
 
@@ -228,6 +248,13 @@ class BACKWARD_EULER_POSITION {
     collision_.push_back(shape);
     if (inst_) add_cube(shape);
   }
+  // The integrator's damping object (TIMESTEPPER.h:155; the reference sets it in its constructor,
+  // BACKWARD_EULER_POSITION.cpp:71-76): non-null makes solve() take solveEnergyDamped
+  // (BACKWARD_EULER_POSITION.cpp:125-131).
+  void setDamping(const VOLUME::GREEN_DAMPING* damping) {
+    damping_ = damping;
+    if (inst_) apply_damping();
+  }
   // Vertex ids of the live plane constraints (TIMESTEPPER.h:46), ascending.
   std::vector<int> planeConstraints() const {
     std::vector<int> ids;
@@ -243,6 +270,10 @@ class BACKWARD_EULER_POSITION {
   bool solve(bool verbose) {
     ensure();
     ctx_->check(hobak_inst_set_state(inst_, position_.data(), velocity_.data()), "set_state");
+    if (damping_ && mesh_.fdotSerial() != fdotApplied_) {
+      ctx_->check(hobak_inst_compute_fdots(inst_, mesh_.fdotVelocity().data()), "compute_fdots");
+      fdotApplied_ = mesh_.fdotSerial();
+    }
     const int rc = hobak_step(inst_, &stats_);
     if (rc != HOBAK_OK && rc != HOBAK_E_NOT_CONVERGED) ctx_->check(rc, "hobak_step");
     ctx_->check(hobak_inst_get_state(inst_, position_.data(), velocity_.data()), "get_state");
@@ -275,6 +306,12 @@ class BACKWARD_EULER_POSITION {
     if (!ids.empty())
       ctx_->check(hobak_inst_set_pinned(inst_, ids.data(), (int32_t)ids.size(), tg.data()), "set_pinned");
     for (const CUBE* c : collision_) add_cube(c);
+    if (damping_) apply_damping();
+  }
+  void apply_damping() {
+    ctx_->check(hobak_inst_set_damping(inst_, damping_ ? HOBAK_DAMPING_GREEN : HOBAK_DAMPING_RAYLEIGH,
+                                       damping_ ? damping_->mu() : 0.0),
+                "set_damping");
   }
   void add_cube(const CUBE* c) {
     const double center[3] = {c->center()[0], c->center()[1], c->center()[2]};
@@ -291,6 +328,8 @@ class BACKWARD_EULER_POSITION {
   std::vector<int> pins_;
   std::vector<double> targets_;
   std::vector<const CUBE*> collision_;
+  const VOLUME::GREEN_DAMPING* damping_ = nullptr;
+  long fdotApplied_ = 0;
 };
 
 }  // namespace hobak

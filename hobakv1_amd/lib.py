@@ -35,6 +35,9 @@ MAX_NEWTON = 8
 PRECISION_FP64 = 0
 PRECISION_FP32_REFINE = 1
 
+DAMPING_RAYLEIGH = 0
+DAMPING_GREEN = 1
+
 
 class hobak_material(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int32), ("mu", ctypes.c_double), ("lam", ctypes.c_double)]
@@ -96,6 +99,9 @@ SIGNATURES = {
     "hobak_inst_set_pinned": (ctypes.c_int, [_vp, _ip, ctypes.c_int32, _dp]),
     "hobak_inst_add_collision_cube": (ctypes.c_int, [_vp, _dp, ctypes.c_double]),
     "hobak_inst_plane_constraints": (ctypes.c_int, [_vp, _ip, _ip, ctypes.c_int32]),
+    "hobak_inst_set_damping": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_double]),
+    "hobak_inst_compute_fdots": (ctypes.c_int, [_vp, _dp]),
+    "hobak_eval_damping": (ctypes.c_int, [_vp, _dp, _dp, _dp]),
     "hobak_inst_set_state": (ctypes.c_int, [_vp, _dp, _dp]),
     "hobak_inst_get_state": (ctypes.c_int, [_vp, _dp, _dp]),
     "hobak_step": (ctypes.c_int, [_vp, ctypes.POINTER(hobak_step_stats)]),
@@ -301,7 +307,31 @@ class Instance:
                       "set_pinned")
         for cube in getattr(scene, "collision_cubes", ()):
             self.add_collision_cube(cube[:3], cube[3])
+        if getattr(scene, "green_mu", None) is not None:
+            self.set_damping(DAMPING_GREEN, scene.green_mu)
+            if getattr(scene, "fdot_v", None) is not None:
+                self.compute_fdots(scene.fdot_v)
         self.set_state(scene.initial_u(), scene.initial_v())
+
+    def set_damping(self, model, mu=0.0):
+        """DAMPING_RAYLEIGH, or DAMPING_GREEN: GREEN_DAMPING(mu) and the energy-damped step."""
+        self.ctx.check(self.lib.hobak_inst_set_damping(self.h, int(model), float(mu)), "set_damping")
+
+    def compute_fdots(self, velocity=None):
+        """TET_MESH::computeFdots: the held velocity gradients, from `velocity` or the current v."""
+        vv = None if velocity is None else np.ascontiguousarray(velocity, dtype=np.float64)
+        self.ctx.check(self.lib.hobak_inst_compute_fdots(self.h, _d(vv) if vv is not None else None),
+                       "compute_fdots")
+
+    def damping_forces_hessian(self, u, with_c=True):
+        """computeInternalForces(hyperelastic, damping) and computeDampingHessian at u."""
+        rp, ci = self.bsr_pattern()
+        R = np.zeros(self.dofs)
+        C = np.zeros(9 * ci.size) if with_c else None
+        uu = np.ascontiguousarray(u, dtype=np.float64)
+        self.ctx.check(self.lib.hobak_eval_damping(self.h, _d(uu), _d(R), _d(C) if with_c else None),
+                       "eval_damping")
+        return R, (C.reshape(-1, 3, 3) if with_c else None)
 
     def add_collision_cube(self, center, scale):
         """addKinematicCollisionObject(CUBE(center, scale)): plane constraints on contact."""
@@ -454,6 +484,14 @@ class TetMesh:
     def __init__(self, rest_vertices, tets):
         self._rest = np.ascontiguousarray(rest_vertices, dtype=np.float64).reshape(-1, 3)
         self._tets = np.ascontiguousarray(tets, dtype=np.int32).reshape(-1, 4)
+        self._fdot_velocity = None  # the velocity of the last computeFdots call
+        self._fdot_serial = 0
+
+    def computeFdots(self, velocity):
+        """TET_MESH::computeFdots (TET_MESH.cpp:660-684): the velocity gradients an energy-damped
+        step reads.  The device instance picks them up at the next solve()."""
+        self._fdot_velocity = np.array(velocity, dtype=np.float64).reshape(-1)
+        self._fdot_serial += 1
 
     def restVertices(self):
         return self._rest
@@ -477,6 +515,16 @@ class TetMesh:
         from .formats import write_tobj
 
         write_tobj(path, mesh.restVertices(), mesh.tets())
+
+
+class GREEN_DAMPING:
+    """VOLUME::GREEN_DAMPING(mu) (src/Damping/Volume/GREEN_DAMPING.cpp:25-28)."""
+
+    def __init__(self, mu):
+        self.mu = float(mu)
+
+    def name(self):
+        return "GREEN_DAMPING"
 
 
 class HYPERELASTIC:
@@ -533,6 +581,8 @@ class BackwardEulerPosition:
         self._pins = np.zeros(0, np.int32)
         self._targets = np.zeros((0, 3))
         self._collision = []
+        self._damping = None
+        self._fdot_applied = 0
         self._precond = precond
         self._precision = precision
         self._tol = 0.0
@@ -575,6 +625,16 @@ class BackwardEulerPosition:
         if self._inst is not None:
             self._inst.add_collision_cube(cube.center, cube.scale)
 
+    def setDamping(self, damping: Optional[GREEN_DAMPING]):
+        """The integrator's damping object (TIMESTEPPER.h:155): a GREEN_DAMPING makes solve() take
+        solveEnergyDamped (BACKWARD_EULER_POSITION.cpp:125-131); None restores Rayleigh."""
+        self._damping = damping
+        if self._inst is not None:
+            if damping is None:
+                self._inst.set_damping(DAMPING_RAYLEIGH)
+            else:
+                self._inst.set_damping(DAMPING_GREEN, damping.mu)
+
     def planeConstraints(self):
         """Vertex ids of the live plane constraints (TIMESTEPPER.h:46), ascending."""
         return np.zeros(0, np.int32) if self._inst is None else self._inst.plane_constraints()
@@ -587,7 +647,8 @@ class BackwardEulerPosition:
                            lam=self.material.lam, dt=self._dt, rayleigh_alpha=self._alpha,
                            rayleigh_beta=self._beta, mass_scale=self._mass_scale,
                            gravity=tuple(self._gravity), pcg_tol=self._tol,
-                           collision_cubes=tuple((*c.center, c.scale) for c in self._collision))
+                           collision_cubes=tuple((*c.center, c.scale) for c in self._collision),
+                           green_mu=None if self._damping is None else self._damping.mu)
         self._inst = Instance(self.ctx, sc, precond=self._precond, precision=self._precision)
         if self._pins.size:
             self.ctx.check(self.ctx.lib.hobak_inst_set_pinned(
@@ -597,6 +658,9 @@ class BackwardEulerPosition:
     def solve(self, verbose=False):
         self._ensure()
         self._inst.set_state(self._position, self._velocity)
+        if self._damping is not None and self.mesh._fdot_serial != self._fdot_applied:
+            self._inst.compute_fdots(self.mesh._fdot_velocity)
+            self._fdot_applied = self.mesh._fdot_serial
         self.last_stats = self._inst.step(allow_unconverged=True)
         if verbose:
             print(f"  PCG iters: {self.last_stats['pcg_iters'][0]:3d} "

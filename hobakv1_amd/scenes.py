@@ -183,6 +183,8 @@ class Scene:
     ground_y: Optional[float] = None   # penalty ground plane (no reference counterpart)
     ground_k: float = 0.0
     collision_cubes: tuple = ()        # ((cx, cy, cz, scale), ...) kinematic collision CUBEs
+    green_mu: Optional[float] = None   # GREEN_DAMPING(mu) energy damping in place of Rayleigh
+    fdot_v: Optional[np.ndarray] = None  # (3 nv) velocity given to computeFdots before stepping
     dims: tuple = ()
 
     @property
@@ -289,6 +291,26 @@ def drop_test(tobj_path, pcg_tol=1e-8):
     v = np.zeros((verts.shape[0], 3))
     v[:, 1] = -3.0
     sc.v0 = v.ravel().copy()
+    return sc
+
+
+def green_scene(n=(8, 8, 8), seed=11, E=1e5, nu=0.45, material=MAT_SNH, green_frac=0.02,
+                rate=1.0, name=None):
+    """The pinned lattice with GREEN_DAMPING(green_frac * mu) in place of Rayleigh damping
+    (BACKWARD_EULER_POSITION::solveEnergyDamped).  The initial velocity is the affine field
+    v = B (X - c), B = a seeded diagonal stretch rate plus a seeded spin, entries ~ `rate` per
+    second (pinned vertices at rest), and the same field is handed to computeFdots, so the
+    energy-damped steps carry the damping force of that velocity gradient."""
+    sc = lattice_scene(n, seed=seed, E=E, nu=nu, material=material, name=name or f"green{n[0]}x{n[1]}x{n[2]}")
+    sc.green_mu = green_frac * sc.mu
+    d = uniform(seed, 20, 3, -rate, rate)
+    w = uniform(seed, 21, 3, -rate, rate)
+    B = np.diag(d) + np.array([[0.0, -w[2], w[1]], [w[2], 0.0, -w[0]], [-w[1], w[0], 0.0]])
+    c = sc.rest.mean(axis=0)
+    v = (sc.rest - c[None, :]) @ B.T
+    v[sc.pinned()] = 0.0
+    sc.v0 = v.ravel().copy()
+    sc.fdot_v = v.ravel().copy()
     return sc
 
 

@@ -14,6 +14,8 @@
  *   setRayeligh, scaleMass
  *   computeHyperelasticForces                        src/Geometry/TET_MESH.h:72-75
  *   computeHyperelasticClampedHessian                src/Geometry/TET_MESH.h:107-110
+ *   _damping = new GREEN_DAMPING(mu), computeFdots,  src/Timestepper/BACKWARD_EULER_POSITION.cpp:71-76,
+ *   computeInternalForces, computeDampingHessian     125-131; src/Geometry/TET_MESH.h:74, 114, 124
  *
  * Conventions
  *   - Every function returns HOBAK_OK (0) or a negative HOBAK_E_* code; no C++ exception crosses
@@ -129,6 +131,25 @@ int hobak_inst_add_collision_cube(hobak_inst* inst, const double* center3, doubl
 /* The vertices under a plane constraint after the last step (planeConstraints(),
  * TIMESTEPPER.h:46): *count of them, and the first `cap` ids, ascending, into ids (may be NULL). */
 int hobak_inst_plane_constraints(hobak_inst* inst, int32_t* count, int32_t* ids, int32_t cap);
+/* Damping model.  HOBAK_DAMPING_RAYLEIGH (the default) is solveRayleighDamped with the params'
+ * alpha and beta.  HOBAK_DAMPING_GREEN sets the integrator's damping object to
+ * VOLUME::GREEN_DAMPING(mu) (src/Damping/Volume/GREEN_DAMPING.cpp), which makes solve() take
+ * solveEnergyDamped (BACKWARD_EULER_POSITION.cpp:125-131, 332-473):
+ *   b = (M/dt) v + R + f_ext, R the elastic force plus the damping force of the stress
+ *       mu F (F^T Fdot + Fdot^T F) at the instance's held velocity gradients Fdot;
+ *   A = M/dt^2 - K, with no Rayleigh terms.
+ * The reference assembles the damping stiffness C as well, but computeCollisionResponse empties
+ * it before A is formed (TIMESTEPPER.cpp:657), so it does not reach the system; it is available
+ * through hobak_eval_damping.  The step never updates Fdot: it is zero until
+ * hobak_inst_compute_fdots sets it.  One Newton iteration per step only (the reference's BDF_1
+ * has no energy-damped solve, BDF_1.cpp:244-255); plane constraints work with either model. */
+#define HOBAK_DAMPING_RAYLEIGH 0
+#define HOBAK_DAMPING_GREEN 1
+int hobak_inst_set_damping(hobak_inst* inst, int32_t model, double mu);
+/* TET_MESH::computeFdots(velocity) (src/Geometry/TET_MESH.cpp:660-684): the held per-tet
+ * velocity gradients Fdot = [v1 - v0, v2 - v0, v3 - v0] DmInv from a DOF-length host velocity,
+ * or from the instance's current velocity when velocity is NULL.  Needs HOBAK_DAMPING_GREEN. */
+int hobak_inst_compute_fdots(hobak_inst* inst, const double* velocity);
 /* position() = u and velocity() = v, DOF-length host arrays; NULL leaves that vector unchanged. */
 int hobak_inst_set_state(hobak_inst* inst, const double* u, const double* v);
 int hobak_inst_get_state(hobak_inst* inst, double* u, double* v);
@@ -150,6 +171,11 @@ int hobak_get_bsr_pattern(hobak_inst* inst, int32_t* nnzb, int32_t* row_ptr, int
  * forces (DOFs) and K (nnzb*9, row-major blocks, the reference's sign: K = -stiffness).
  * Either output may be NULL. */
 int hobak_eval_forces_hessian(hobak_inst* inst, const double* u, double* forces, double* kvals);
+/* With energy damping: computeInternalForces(hyperelastic, damping) (elastic plus damping force,
+ * TET_MESH.cpp:816-858) and TET_MESH_FASTER::computeDampingHessian (TET_MESH_FASTER.cpp:138-180)
+ * at displacement u and the held Fdot: forces (DOFs) and C (nnzb*9, row-major blocks, the
+ * reference's sign: C = -vol dF/dx^T dP/dFdot dF/dx).  Either output may be NULL. */
+int hobak_eval_damping(hobak_inst* inst, const double* u, double* forces, double* cvals);
 /* The last Newton iteration's projected system: rhs = S b (DOFs) and LHS = S A S + (I - S)
  * (nnzb*9).  Either output may be NULL. */
 int hobak_get_last_system(hobak_inst* inst, double* rhs, double* lhs_vals);

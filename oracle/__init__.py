@@ -85,6 +85,14 @@ def load_ref():
         lib.ref_psi.restype = ctypes.c_double
         for f in (lib.ref_pk1, lib.ref_hessian, lib.ref_clamped_hessian):
             f.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, _dp]
+        lib.ref_set_green_damping.argtypes = [ctypes.c_void_p, ctypes.c_double]
+        lib.ref_set_green_damping.restype = ctypes.c_int
+        lib.ref_compute_fdots.argtypes = [ctypes.c_void_p, _dp]
+        lib.ref_mesh_damped_forces_hessian.argtypes = [ctypes.c_void_p, _dp, _dp, _ip, _ip, _dp]
+        lib.ref_green_psi.argtypes = [ctypes.c_double, _dp, _dp]
+        lib.ref_green_psi.restype = ctypes.c_double
+        lib.ref_green_pk1.argtypes = [ctypes.c_double, _dp, _dp, _dp]
+        lib.ref_green_hessian.argtypes = [ctypes.c_double, _dp, _dp, ctypes.c_int, _dp]
         _ref_lib = lib
     return _ref_lib
 
@@ -126,6 +134,19 @@ class RefElement:
     def clamped_hessian(self, mat, mu, lam, F):
         H = np.zeros(81)
         self.lib.ref_clamped_hessian(mat, mu, lam, _d(_colmajor(F)), _d(H))
+        return _from_colmajor(H, 9)
+
+    def green_psi(self, mu, F, Fdot):
+        return self.lib.ref_green_psi(mu, _d(_colmajor(F)), _d(_colmajor(Fdot)))
+
+    def green_pk1(self, mu, F, Fdot):
+        P = np.zeros(9)
+        self.lib.ref_green_pk1(mu, _d(_colmajor(F)), _d(_colmajor(Fdot)), _d(P))
+        return _from_colmajor(P, 3)
+
+    def green_hessian(self, mu, F, Fdot, clamped=True):
+        H = np.zeros(81)
+        self.lib.ref_green_hessian(mu, _d(_colmajor(F)), _d(_colmajor(Fdot)), int(clamped), _d(H))
         return _from_colmajor(H, 9)
 
 
@@ -194,6 +215,11 @@ class RefSim:
         for cube in getattr(scene, "collision_cubes", ()):
             c = _c64(np.asarray(cube[:3]))
             self.lib.ref_add_collision_cube(self.h, _d(c), float(cube[3]))
+        if getattr(scene, "green_mu", None) is not None:
+            if self.lib.ref_set_green_damping(self.h, float(scene.green_mu)) != 0:
+                raise ValueError("the reference's BDF_1 has no energy-damped solve")
+            if getattr(scene, "fdot_v", None) is not None:
+                self.compute_fdots(scene.fdot_v)
         self.dofs = scene.dofs
         self.set_state(scene.initial_u(), scene.initial_v())
 
@@ -245,6 +271,21 @@ class RefSim:
         m = np.zeros(self.dofs)
         self.lib.ref_mass_diag(self.h, _d(m))
         return m
+
+    def compute_fdots(self, vel):
+        """TET_MESH::computeFdots(vel): the velocity gradients the energy-damped step reads."""
+        self.lib.ref_compute_fdots(self.h, _d(_c64(vel)))
+
+    def damped_forces_hessian(self, u):
+        """computeInternalForces(hyperelastic, damping) and the damping stiffness C (CSC) at u."""
+        nnz = self.lib.ref_mesh_nnz(self.h)
+        R = np.zeros(self.dofs)
+        colptr = np.zeros(self.dofs + 1, np.int32)
+        rowidx = np.zeros(nnz, np.int32)
+        vals = np.zeros(nnz)
+        self.lib.ref_mesh_damped_forces_hessian(self.h, _d(_c64(u)), _d(R), _i(colptr),
+                                                _i(rowidx), _d(vals))
+        return R, (colptr, rowidx, vals)
 
     def forces_hessian(self, u):
         """R and the clamped K (scipy-free CSC arrays) of the mesh at displacement u."""
@@ -302,6 +343,14 @@ def load_oracle():
         lib.orc_last_system.argtypes = [ctypes.c_void_p, _dp, _dp]
         lib.orc_rest_volumes.argtypes = [ctypes.c_void_p, _dp, _dp]
         lib.orc_set_threads.argtypes = [ctypes.c_int]
+        lib.orc_green_psi.argtypes = [ctypes.c_double, _dp, _dp]
+        lib.orc_green_psi.restype = ctypes.c_double
+        lib.orc_green_pk1.argtypes = [ctypes.c_double, _dp, _dp, _dp]
+        lib.orc_green_hessian.argtypes = [ctypes.c_double, _dp, ctypes.c_int, _dp]
+        lib.orc_set_green_damping.argtypes = [ctypes.c_void_p, ctypes.c_double]
+        lib.orc_set_green_damping.restype = ctypes.c_int
+        lib.orc_compute_fdots.argtypes = [ctypes.c_void_p, _dp]
+        lib.orc_damped_forces_hessian.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
         _orc_lib = lib
     return _orc_lib
 
@@ -343,6 +392,19 @@ class OracleElement:
                                          _d(_colmajor(V)), _d(H))
         return _from_colmajor(H, 9)
 
+    def green_psi(self, mu, F, Fdot):
+        return self.lib.orc_green_psi(mu, _d(_colmajor(F)), _d(_colmajor(Fdot)))
+
+    def green_pk1(self, mu, F, Fdot):
+        P = np.zeros(9)
+        self.lib.orc_green_pk1(mu, _d(_colmajor(F)), _d(_colmajor(Fdot)), _d(P))
+        return _from_colmajor(P, 3)
+
+    def green_hessian(self, mu, F, clamped=True):
+        H = np.zeros(81)
+        self.lib.orc_green_hessian(mu, _d(_colmajor(F)), int(clamped), _d(H))
+        return _from_colmajor(H, 9)
+
     def tet_force_hessian(self, mat, mu, lam, x4, DmInv, vol):
         f = np.zeros(12)
         H = np.zeros(144)
@@ -379,6 +441,11 @@ class Oracle:
         for cube in getattr(scene, "collision_cubes", ()):
             c = _c64(np.asarray(cube[:3]))
             self.lib.orc_add_collision_cube(self.h, _d(c), float(cube[3]))
+        if getattr(scene, "green_mu", None) is not None:
+            if self.lib.orc_set_green_damping(self.h, float(scene.green_mu)) != 0:
+                raise ValueError("energy damping needs the single-iteration step")
+            if getattr(scene, "fdot_v", None) is not None:
+                self.compute_fdots(scene.fdot_v)
         self.dofs = scene.dofs
         self.set_state(scene.initial_u(), scene.initial_v())
 
@@ -438,6 +505,17 @@ class Oracle:
         K = np.zeros(9 * nnzb) if with_k else None
         self.lib.orc_forces_hessian(self.h, _d(_c64(u)), _d(R), _d(K) if with_k else None)
         return R, (K.reshape(nnzb, 3, 3) if with_k else None)
+
+    def compute_fdots(self, vel):
+        self.lib.orc_compute_fdots(self.h, _d(_c64(vel)))
+
+    def damped_forces_hessian(self, u):
+        """Elastic + damping forces and the assembled damping stiffness C at u."""
+        nnzb = self.lib.orc_nnzb(self.h)
+        R = np.zeros(self.dofs)
+        C = np.zeros(9 * nnzb)
+        self.lib.orc_damped_forces_hessian(self.h, _d(_c64(u)), _d(R), _d(C))
+        return R, C.reshape(nnzb, 3, 3)
 
     def last_system(self):
         nnzb = self.lib.orc_nnzb(self.h)

@@ -503,10 +503,16 @@ void pFpx(const M3 Dm, double P[9][12]) {
 }
 
 // Element force (TET_MESH.cpp:744-749) and element clamped stiffness (TET_MESH_FASTER.cpp:95-102)
+// Pextra (may be NULL) is added to the elastic PK1 before the force, as computeInternalForces
+// does with the damping PK1 (TET_MESH.cpp:831-834).
 void element(const Material& mat, const M3 F, const M3 U, const double S[3], const M3 V,
-             const double P9x12[9][12], double vol, double f[12], double H[12][12]) {
+             const double P9x12[9][12], double vol, double f[12], double H[12][12],
+             const double (*Pextra)[3] = nullptr) {
   M3 P;
   pk1(mat, F, P);
+  if (Pextra)
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) P[i][j] = P[i][j] + Pextra[i][j];
   double vp[9];
   flatten(P, vp);
   for (int k = 0; k < 12; k++) {
@@ -531,6 +537,146 @@ void element(const Material& mat, const M3 F, const M3 U, const double S[3], con
       double s = 0;
       for (int b = 0; b < 9; b++) s += T[i][b] * P9x12[b][j];
       H[i][j] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Energy-based damping: GREEN_DAMPING (src/Damping/Volume/GREEN_DAMPING.cpp:38-97) on the rate of
+// the Green strain, Edot = (Fdot^T F + F^T Fdot) / 2.
+//   psi = mu |Edot|_F^2
+//   P   = mu F (F^T Fdot + Fdot^T F)
+//   dP/dFdot, column (i, j) (flattened column-major, index 3 j + i) = mu (F F^T E_ij + F E_ij^T F)
+//   clampedHessian: the same matrix with its eigenvalues clamped at zero (MATRIX_UTIL.cpp:100-111)
+// ---------------------------------------------------------------------------------------------
+// Cyclic Jacobi eigensolver for a symmetric 9x9 (eigenvectors in columns).  As for sym_eig3, the
+// clamp only needs the spectral projector, so any accurate symmetric eigensolver reproduces the
+// reference's SelfAdjointEigenSolver result to round-off.
+void sym_eig9(const double Ain[9][9], double w[9], double Q[9][9]) {
+  double A[9][9];
+  std::memcpy(A, Ain, sizeof(A));
+  for (int i = 0; i < 9; i++)
+    for (int j = 0; j < 9; j++) Q[i][j] = i == j ? 1.0 : 0.0;
+  for (int sweep = 0; sweep < 100; sweep++) {
+    double off = 0.0, dia = 0.0;
+    for (int i = 0; i < 9; i++) {
+      dia += A[i][i] * A[i][i];
+      for (int j = i + 1; j < 9; j++) off += A[i][j] * A[i][j];
+    }
+    if (off <= 1e-40 * dia || off == 0.0) break;
+    for (int p = 0; p < 8; p++)
+      for (int q = p + 1; q < 9; q++) {
+        if (A[p][q] == 0.0) continue;
+        const double theta = (A[q][q] - A[p][p]) / (2.0 * A[p][q]);
+        const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+        const double c = 1.0 / std::sqrt(t * t + 1.0);
+        const double s = t * c;
+        for (int k = 0; k < 9; k++) {
+          const double akp = A[k][p], akq = A[k][q];
+          A[k][p] = c * akp - s * akq;
+          A[k][q] = s * akp + c * akq;
+        }
+        for (int k = 0; k < 9; k++) {
+          const double apk = A[p][k], aqk = A[q][k];
+          A[p][k] = c * apk - s * aqk;
+          A[q][k] = s * apk + c * aqk;
+        }
+        for (int k = 0; k < 9; k++) {
+          const double qkp = Q[k][p], qkq = Q[k][q];
+          Q[k][p] = c * qkp - s * qkq;
+          Q[k][q] = s * qkp + c * qkq;
+        }
+      }
+  }
+  for (int i = 0; i < 9; i++) w[i] = A[i][i];
+}
+
+inline void m3_tmul(const M3 A, const M3 B, M3 C) {  // C = A^T B
+  M3 At;
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) At[i][j] = A[j][i];
+  m3_mul(At, B, C);
+}
+
+double green_psi(double mu, const M3 F, const M3 Fd) {
+  M3 FdF, E;
+  m3_tmul(Fd, F, FdF);  // Fdot^T F
+  double s = 0.0;
+  for (int j = 0; j < 3; j++)  // squaredNorm, column by column
+    for (int i = 0; i < 3; i++) {
+      E[i][j] = 0.5 * (FdF[i][j] + FdF[j][i]);
+      s += E[i][j] * E[i][j];
+    }
+  return mu * s;
+}
+
+void green_pk1(double mu, const M3 F, const M3 Fd, M3 P) {
+  M3 A, B, Sm, muF;
+  m3_tmul(F, Fd, A);   // F^T Fdot
+  m3_tmul(Fd, F, B);   // Fdot^T F
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      Sm[i][j] = A[i][j] + B[i][j];
+      muF[i][j] = mu * F[i][j];
+    }
+  m3_mul(muF, Sm, P);  // (mu F) (F^T Fdot + Fdot^T F)
+}
+
+void green_hessian(double mu, const M3 F, double H[9][9]) {
+  M3 FFt;
+  m3_mul_bt(F, F, FFt);
+  for (int j = 0; j < 3; j++)
+    for (int i = 0; i < 3; i++) {
+      // column 3 j + i: F F^T E_ij + F E_ij^T F;  (F F^T E_ij)(a, b) = FFt(a, i) [b == j],
+      // (F E_ij^T F)(a, b) = F(a, j) F(i, b)
+      const int col = 3 * j + i;
+      for (int b = 0; b < 3; b++)
+        for (int a = 0; a < 3; a++) {
+          const double v = (b == j ? FFt[a][i] : 0.0) + F[a][j] * F[i][b];
+          H[3 * b + a][col] = mu * v;
+        }
+    }
+}
+
+void green_clamped_hessian(double mu, const M3 F, double H[9][9]) {
+  double H0[9][9], w[9], Q[9][9];
+  green_hessian(mu, F, H0);
+  sym_eig9(H0, w, Q);
+  for (int k = 0; k < 9; k++)
+    if (!(w[k] > 0.0)) w[k] = 0.0;
+  for (int a = 0; a < 9; a++)
+    for (int b = 0; b < 9; b++) {
+      double v = 0.0;
+      for (int k = 0; k < 9; k++) v += Q[a][k] * w[k] * Q[b][k];
+      H[a][b] = v;
+    }
+}
+
+// TET_MESH::computeFdots (TET_MESH.cpp:660-684): Fdot = [v1 - v0, v2 - v0, v3 - v0] DmInv.
+void tet_fdot(const int* T, const double* vel, const M3 Di, M3 Fd) {
+  M3 Vm;
+  for (int r = 0; r < 3; r++)
+    for (int c = 0; c < 3; c++) Vm[r][c] = vel[3 * T[c + 1] + r] - vel[3 * T[0] + r];
+  m3_mul(Vm, Di, Fd);
+}
+
+// Element damping stiffness -V pFpx^T clampedHessian pFpx (TET_MESH_FASTER.cpp:144-150).
+void element_damping(double mu, const M3 F, const double P9x12[9][12], double vol, double C[12][12]) {
+  double H9[9][9];
+  green_clamped_hessian(mu, F, H9);
+  for (int a = 0; a < 9; a++)
+    for (int b = 0; b < 9; b++) H9[a][b] = -vol * H9[a][b];
+  double T[12][9];
+  for (int i = 0; i < 12; i++)
+    for (int b = 0; b < 9; b++) {
+      double s = 0;
+      for (int a = 0; a < 9; a++) s += P9x12[a][i] * H9[a][b];
+      T[i][b] = s;
+    }
+  for (int i = 0; i < 12; i++)
+    for (int j = 0; j < 12; j++) {
+      double s = 0;
+      for (int b = 0; b < 9; b++) s += T[i][b] * P9x12[b][j];
+      C[i][j] = s;
     }
 }
 
@@ -655,6 +801,11 @@ struct Sim {
   double newtonTol = 1e-7;
   bool ground = false;
   double groundY = 0, groundK = 0;
+  // energy-based damping (GREEN_DAMPING) in place of Rayleigh damping, and the mesh's held
+  // velocity gradients (nt*9 row-major; zero until computeFdots is called)
+  bool green = false;
+  double greenMu = 0.0;
+  std::vector<double> Fdot;
 
   // BSR pattern, block rows by vertex, columns ascending
   std::vector<int> rowPtr, colIdx;
@@ -681,8 +832,12 @@ struct Sim {
   std::vector<double> lastRhs, lastLhs;
 
   void setup();
+  // damped: add the damping PK1 to the forces; CT (may be NULL) gets the element damping
+  // stiffnesses.
   void evalElements(const std::vector<double>& uu, std::vector<double>& fT,
-                    std::vector<double>* HT) const;
+                    std::vector<double>* HT, std::vector<double>* CT = nullptr,
+                    bool damped = false) const;
+  void computeFdots(const double* vel);
   void assembleK(const std::vector<double>& HT, std::vector<double>& K) const;
   void gatherForces(const std::vector<double>& fT, std::vector<double>& R) const;
   void spmv(const std::vector<double>& vals, const double* x, double* y) const;
@@ -804,6 +959,7 @@ void Sim::setup() {
 
   u.assign(dofs, 0.0);
   v.assign(dofs, 0.0);
+  Fdot.assign(9 * (size_t)nt, 0.0);
   // Rayleigh stiffness at the rest state (TIMESTEPPER.cpp:213-236), constant across steps.
   std::vector<double> fT, HT;
   std::vector<double> zero(dofs, 0.0);
@@ -812,9 +968,10 @@ void Sim::setup() {
 }
 
 void Sim::evalElements(const std::vector<double>& uu, std::vector<double>& fT,
-                       std::vector<double>* HT) const {
+                       std::vector<double>* HT, std::vector<double>* CT, bool damped) const {
   fT.assign(12 * (size_t)nt, 0.0);
   if (HT) HT->assign(144 * (size_t)nt, 0.0);
+  if (CT) CT->assign(144 * (size_t)nt, 0.0);
 #pragma omp parallel for schedule(static)
   for (int t = 0; t < nt; t++) {
     const int* T = &tets[4 * t];
@@ -833,10 +990,34 @@ void Sim::evalElements(const std::vector<double>& uu, std::vector<double>& fT,
     double P9x12[9][12];
     pFpx(Di, P9x12);
     double H[12][12];
-    element(mat, F, U, S, V, P9x12, vol[t], &fT[12 * (size_t)t], HT ? H : nullptr);
+    M3 Pd, Fd;
+    if (damped) {  // computeInternalForces(hyperelastic, damping): P = elastic + damping PK1
+      for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) Fd[r][c] = Fdot[9 * (size_t)t + 3 * r + c];
+      green_pk1(greenMu, F, Fd, Pd);
+    }
+    element(mat, F, U, S, V, P9x12, vol[t], &fT[12 * (size_t)t], HT ? H : nullptr,
+            damped ? Pd : nullptr);
     if (HT)
       for (int i = 0; i < 12; i++)
         for (int j = 0; j < 12; j++) (*HT)[144 * (size_t)t + 12 * i + j] = H[i][j];
+    if (CT) {
+      element_damping(greenMu, F, P9x12, vol[t], H);
+      for (int i = 0; i < 12; i++)
+        for (int j = 0; j < 12; j++) (*CT)[144 * (size_t)t + 12 * i + j] = H[i][j];
+    }
+  }
+}
+
+void Sim::computeFdots(const double* vel) {
+  Fdot.assign(9 * (size_t)nt, 0.0);
+  for (int t = 0; t < nt; t++) {
+    M3 Di, Fd;
+    for (int r = 0; r < 3; r++)
+      for (int c = 0; c < 3; c++) Di[r][c] = DmInv[9 * (size_t)t + 3 * r + c];
+    tet_fdot(&tets[4 * t], vel, Di, Fd);
+    for (int r = 0; r < 3; r++)
+      for (int c = 0; c < 3; c++) Fdot[9 * (size_t)t + 3 * r + c] = Fd[r][c];
   }
 }
 
@@ -979,7 +1160,13 @@ static void filterBlock(int mode, const double* n, double S[3][3]) {
 }
 
 // One stepSimulation (SIMULATION_SCENE.h:120-126): f_ext = one-ring volume * gravity, then
-//   newtonIters == 1: BACKWARD_EULER_POSITION::solveRayleighDamped (BACKWARD_EULER_POSITION.cpp:138-327)
+//   newtonIters == 1: BACKWARD_EULER_POSITION::solveRayleighDamped (BACKWARD_EULER_POSITION.cpp:138-327),
+//                     or solveEnergyDamped (:332-473) with GREEN_DAMPING: R gains the damping
+//                     force at the held Fdot (the step never updates it), b = (M/dt) v + R +
+//                     f_ext (:384) and A = M/dt^2 - K.  The step does assemble the damping
+//                     stiffness C (:378), but computeCollisionResponse(R, K, C) (:381) takes C as
+//                     its collisionC argument and empties it (TIMESTEPPER.cpp:657) before
+//                     A = M/dt^2 - C/dt - K (:389), so with self-collisions off C is zero there.
 //   newtonIters  > 1: BDF_1::solveRayleighDamped (BDF_1.cpp:78-239) with residual test maxR > tol
 int Sim::step(double* errOut, int* itersPerNewton, int* newtonDone) {
   const int nnzb = rowPtr[nv];
@@ -988,7 +1175,9 @@ int Sim::step(double* errOut, int* itersPerNewton, int* newtonDone) {
   std::vector<double> fext(dofs);
   for (int i = 0; i < nv; i++)
     for (int c = 0; c < 3; c++) fext[3 * i + c] = oneRing[i] * gravity[c];
-  // C = alpha M + beta K_rest (TIMESTEPPER.cpp:232-233); D = M/dt^2 - C/dt; E = M/dt - C
+  // C = alpha M + beta K_rest (TIMESTEPPER.cpp:232-233); D = M/dt^2 - C/dt; E = M/dt - C.
+  // Energy damping has no Rayleigh part: D = M/dt^2, E = M/dt (BACKWARD_EULER_POSITION.cpp:384, 389).
+  const double ra_ = green ? 0.0 : ra, rb_ = green ? 0.0 : rb;
   std::vector<double> Cv(9 * (size_t)nnzb), D(9 * (size_t)nnzb), E(9 * (size_t)nnzb);
   for (int i = 0; i < nv; i++)
     for (int k = rowPtr[i]; k < rowPtr[i + 1]; k++)
@@ -997,7 +1186,7 @@ int Sim::step(double* errOut, int* itersPerNewton, int* newtonDone) {
           const size_t e = 9 * (size_t)k + 3 * a + b;
           const bool dg = (colIdx[k] == i) && (a == b);
           const double m = dg ? M[3 * i + a] : 0.0;
-          const double c = dg ? ra * m + rb * Krest[e] : rb * Krest[e];
+          const double c = dg ? ra_ * m + rb_ * Krest[e] : rb_ * Krest[e];
           Cv[e] = c;
           D[e] = dg ? m * invDt2 - c * invDt : -(c * invDt);
           E[e] = dg ? invDt * m - c : -c;
@@ -1058,8 +1247,9 @@ int Sim::step(double* errOut, int* itersPerNewton, int* newtonDone) {
   double maxR = 10 * newtonTol;
   std::vector<double> fT, HT, K, R, b(dofs), rhs(dofs), lhs(9 * (size_t)nnzb), y, tmp(dofs);
   while (done < newtonIters && (newtonIters == 1 || maxR > newtonTol)) {
-    evalElements(u, fT, &HT);
+    evalElements(u, fT, &HT, nullptr, green);
     assembleK(HT, K);
+    const std::vector<double>& Dq = D;
     gatherForces(fT, R);
     if (ground) {  // penalty floor: psi = k/2 (y_g - y)^2 below the plane (no reference counterpart)
       for (int i = 0; i < nv; i++) {
@@ -1090,7 +1280,7 @@ int Sim::step(double* errOut, int* itersPerNewton, int* newtonDone) {
       // c = b - A z (:212); rhs = S c; LHS = S A S + (I - S), each product summed in Eigen's
       // column order
       std::vector<double> Av(9 * (size_t)nnzb), Az(dofs);
-      for (size_t q = 0; q < Av.size(); q++) Av[q] = D[q] - K[q];
+      for (size_t q = 0; q < Av.size(); q++) Av[q] = Dq[q] - K[q];
       spmv(Av, zc.data(), Az.data());
       std::vector<double> Sb(9 * (size_t)nv);
       for (int i = 0; i < nv; i++) {
@@ -1127,7 +1317,7 @@ int Sim::step(double* errOut, int* itersPerNewton, int* newtonDone) {
         const bool pj = pinnedDof[3 * j];
         for (int e = 0; e < 9; e++) {
           const size_t q = 9 * (size_t)k + e;
-          double a = D[q] - K[q];
+          double a = Dq[q] - K[q];
           if (pi || pj) a = (j == i && (e % 4 == 0)) ? 1.0 : 0.0;
           lhs[q] = a;
         }
@@ -1255,6 +1445,33 @@ void orc_clamped_hessian(int material, double mu, double lambda, const double* F
     for (int i = 0; i < 9; i++) H9[9 * j + i] = H[i][j];
 }
 
+double orc_green_psi(double mu, const double* F, const double* Fdot) {
+  M3 f, fd;
+  m3_from_colmajor(F, f);
+  m3_from_colmajor(Fdot, fd);
+  return green_psi(mu, f, fd);
+}
+
+void orc_green_pk1(double mu, const double* F, const double* Fdot, double* P) {
+  M3 f, fd, p;
+  m3_from_colmajor(F, f);
+  m3_from_colmajor(Fdot, fd);
+  green_pk1(mu, f, fd, p);
+  m3_to_colmajor(p, P);
+}
+
+void orc_green_hessian(double mu, const double* F, int clamped, double* H9) {
+  M3 f;
+  m3_from_colmajor(F, f);
+  double H[9][9];
+  if (clamped)
+    green_clamped_hessian(mu, f, H);
+  else
+    green_hessian(mu, f, H);
+  for (int j = 0; j < 9; j++)
+    for (int i = 0; i < 9; i++) H9[9 * j + i] = H[i][j];
+}
+
 void orc_tet_force_hessian(int material, double mu, double lambda, const double* x,
                            const double* DmInvC, double restVolume, double* f12, double* H144) {
   Material m(material, mu, lambda);
@@ -1373,6 +1590,33 @@ void orc_forces_hessian(void* h, const double* u, double* R, double* Kvals) {
   if (Kvals) {
     s->assembleK(HT, K);
     std::copy(K.begin(), K.end(), Kvals);
+  }
+}
+
+// Energy damping with GREEN_DAMPING(mu) in place of Rayleigh damping (solveEnergyDamped,
+// BACKWARD_EULER_POSITION.cpp:332-473).  -1 for a multi-iteration handle: the reference's BDF_1
+// has no energy-damped solve (BDF_1.cpp:244-255).
+int orc_set_green_damping(void* h, double mu) {
+  Sim* s = (Sim*)h;
+  if (s->newtonIters > 1) return -1;
+  s->green = true;
+  s->greenMu = mu;
+  return 0;
+}
+
+void orc_compute_fdots(void* h, const double* vel) { ((Sim*)h)->computeFdots(vel); }
+
+// computeInternalForces(hyperelastic, damping) and the assembled damping stiffness C
+// (TET_MESH_FASTER::computeDampingHessian) at displacement u with the held Fdot.
+void orc_damped_forces_hessian(void* h, const double* u, double* R, double* Cvals) {
+  Sim* s = (Sim*)h;
+  std::vector<double> uu(u, u + s->dofs), fT, CT, C, Rv;
+  s->evalElements(uu, fT, nullptr, Cvals ? &CT : nullptr, true);
+  s->gatherForces(fT, Rv);
+  std::copy(Rv.begin(), Rv.end(), R);
+  if (Cvals) {
+    s->assembleK(CT, C);
+    std::copy(C.begin(), C.end(), Cvals);
   }
 }
 

@@ -18,6 +18,11 @@ void orc_clamped_hessian_usv(int material, double mu, double lambda, const doubl
 // row-major), DmInv (column-major) and rest volume.
 void orc_tet_force_hessian(int material, double mu, double lambda, const double* x,
                            const double* DmInv, double restVolume, double* f12, double* H144);
+// GREEN_DAMPING(mu): psi(F, Fdot), PK1(F, Fdot) and dP/dFdot (which does not depend on Fdot),
+// optionally eigen-clamped.
+double orc_green_psi(double mu, const double* F, const double* Fdot);
+void orc_green_pk1(double mu, const double* F, const double* Fdot, double* P);
+void orc_green_hessian(double mu, const double* F, int clamped, double* H9);
 
 // ---- mesh / integrator ----
 // precond: 0 = Jacobi (the reference's DiagonalPreconditioner), 1 = 3x3 block Jacobi.
@@ -43,6 +48,12 @@ void orc_bsr_pattern(void* h, int* row_ptr, int* col_idx);
 void orc_forces_hessian(void* h, const double* u, double* R, double* Kvals);
 // The last step's right-hand side rhs = S b and system matrix (LHS blocks, row-major).
 void orc_last_system(void* h, double* rhs, double* lhs_vals);
+// Energy damping: GREEN_DAMPING(mu) replaces Rayleigh damping (single-iteration handles only;
+// returns -1 otherwise).  compute_fdots sets the held velocity gradients from a velocity.
+int orc_set_green_damping(void* h, double mu);
+void orc_compute_fdots(void* h, const double* vel);
+// Elastic + damping forces and the assembled damping stiffness C at displacement u.
+void orc_damped_forces_hessian(void* h, const double* u, double* R, double* Cvals);
 void orc_rest_volumes(void* h, double* vol, double* one_ring);
 void orc_set_threads(int n);
 }

@@ -16,15 +16,25 @@
 //   SVD              src/util/MATRIX_UTIL.cpp:76-95 (svd_rv)
 //   file formats     .tobj meshes (src/Geometry/TET_MESH.cpp:1174-1259), the JSON scene record
 //                    (src/util/FILE_IO.cpp:393-625)
+//   energy damping   GREEN_DAMPING (src/Damping/Volume/GREEN_DAMPING.cpp) through
+//                    BACKWARD_EULER_POSITION::solveEnergyDamped (BACKWARD_EULER_POSITION.cpp:332-473)
 //
 // The harness subclasses BACKWARD_EULER_POSITION and BDF_1 only to reach protected members: the
 // Eigen CG object (src/Timestepper/TIMESTEPPER.h:207), so that the PCG tolerance can be set as the
-// workload definitions ask (the reference itself never calls setTolerance), and BDF_1's Newton
-// limits (src/Timestepper/NEWMARK.h:61-63).
+// workload definitions ask (the reference itself never calls setTolerance), BDF_1's Newton
+// limits (src/Timestepper/NEWMARK.h:61-63) and the damping object (TIMESTEPPER.h:155), which the
+// reference only sets in commented-out lines of its constructors (BACKWARD_EULER_POSITION.cpp:71-76).
+//
+// Velocity gradients: TET_MESH sizes its Fdot array with vector<MATRIX3>::resize
+// (TET_MESH.cpp:59), which leaves the Eigen matrices uninitialised, and solveEnergyDamped never
+// calls computeFdots, so the step reads whatever the mesh holds.  make_sim defines that value as
+// zero by calling computeFdots once with a zero velocity; ref_compute_fdots sets it from a velocity
+// as a caller of the public TET_MESH::computeFdots would.
 #include <chrono>
 #include <cstring>
 #include <vector>
 
+#include "Damping/Volume/GREEN_DAMPING.h"
 #include "Geometry/CUBE.h"
 #include "util/FILE_IO.h"
 #include "Geometry/TET_MESH_FASTER.h"
@@ -58,6 +68,7 @@ class Harness : public Base {
   }
   const SPARSE_MATRIX& massMatrix() const { return this->_M; }
   const VECTOR& lastB() const { return this->_b; }
+  void setDamping(VOLUME::DAMPING* d) { this->_damping = d; }
 };
 typedef Harness<TIMESTEPPER::BACKWARD_EULER_POSITION> HarnessBEP;
 
@@ -78,6 +89,7 @@ struct RefSim {
   HarnessBEP* bep = nullptr;
   HarnessBDF1* bdf = nullptr;
   CUBE* pin = nullptr;
+  VOLUME::DAMPING* damping = nullptr;
   std::vector<CUBE*> shapes;  // kinematic collision objects
   VECTOR3 gravity;
   double lastStepSeconds = 0.0;
@@ -116,6 +128,9 @@ RefSim* make_sim(const double* rest_xyz, int nv, const int* tets, int nt, int ma
     t[i] = VECTOR4I(tets[4 * i], tets[4 * i + 1], tets[4 * i + 2], tets[4 * i + 3]);
   RefSim* sim = new RefSim;
   sim->mesh = new TET_MESH_FASTER(verts, t);
+  VECTOR zero(3 * nv);
+  zero.setZero();
+  sim->mesh->computeFdots(zero);  // defines the held velocity gradients (see the header)
   sim->material = makeMaterial(material, mu, lambda);
   sim->gravity = VECTOR3(gravity[0], gravity[1], gravity[2]);
   return sim;
@@ -176,6 +191,7 @@ void ref_destroy(void* h) {
   delete sim->material;
   delete sim->mesh;
   delete sim->pin;
+  delete sim->damping;
   for (CUBE* c : sim->shapes) delete c;
   delete sim;
 }
@@ -207,6 +223,27 @@ void ref_pinned_nodes(void* h, int* out) {
   RefSim* sim = (RefSim*)h;
   std::vector<int> n = sim->base()->constrainedNodes();
   for (size_t i = 0; i < n.size(); i++) out[i] = n[i];
+}
+
+// _damping = new VOLUME::GREEN_DAMPING(mu) (BACKWARD_EULER_POSITION.cpp:73-75): solve() then takes
+// solveEnergyDamped.  Returns -1 for a BDF_1 handle, whose energy-damped solve only prints
+// "NOT IMPLEMENTED" and exits (BDF_1.cpp:244-255).
+int ref_set_green_damping(void* h, double mu) {
+  RefSim* sim = (RefSim*)h;
+  if (!sim->bep) return -1;
+  delete sim->damping;
+  sim->damping = new VOLUME::GREEN_DAMPING(mu);
+  sim->bep->setDamping(sim->damping);
+  return 0;
+}
+
+// TET_MESH::computeFdots(velocity) (TET_MESH.cpp:660-684): the mesh's held velocity gradients.
+void ref_compute_fdots(void* h, const double* vel) {
+  RefSim* sim = (RefSim*)h;
+  const int dofs = (int)sim->base()->position().size();
+  VECTOR v(dofs);
+  for (int i = 0; i < dofs; i++) v[i] = vel[i];
+  sim->mesh->computeFdots(v);
 }
 
 void ref_set_state(void* h, const double* u, const double* v) {
@@ -285,6 +322,28 @@ void ref_mesh_forces_hessian(void* h, const double* u, double* R, int* colptr, i
   }
 }
 
+// computeInternalForces(hyperelastic, damping) and TET_MESH_FASTER::computeDampingHessian
+// (TET_MESH.cpp:816-858, TET_MESH_FASTER.cpp:138-180) at displacement u with the held velocity
+// gradients; C in the same compressed-column layout as ref_mesh_forces_hessian's K.
+void ref_mesh_damped_forces_hessian(void* h, const double* u, double* R, int* colptr, int* rowidx,
+                                    double* vals) {
+  RefSim* sim = (RefSim*)h;
+  const int dofs = (int)sim->base()->position().size();
+  VECTOR vu(dofs);
+  for (int i = 0; i < dofs; i++) vu[i] = u[i];
+  sim->mesh->setDisplacement(vu);
+  sim->mesh->computeFs();
+  sim->mesh->computeSVDs();
+  VECTOR f = sim->mesh->computeInternalForces(*sim->material, *sim->damping);
+  for (int i = 0; i < dofs; i++) R[i] = f[i];
+  SPARSE_MATRIX C = sim->mesh->computeDampingHessian(*sim->damping);
+  for (int j = 0; j <= C.outerSize(); j++) colptr[j] = C.outerIndexPtr()[j];
+  for (int k = 0; k < C.nonZeros(); k++) {
+    rowidx[k] = C.innerIndexPtr()[k];
+    vals[k] = C.valuePtr()[k];
+  }
+}
+
 // ---- element level (column-major 3x3, column-major 9x9) ----
 void ref_svd_rv(const double* F, double* U, double* S, double* V) {
   MATRIX3 Um, Vm;
@@ -326,6 +385,25 @@ void ref_clamped_hessian(int material, double mu, double lambda, const double* F
   for (int j = 0; j < 9; j++)
     for (int i = 0; i < 9; i++) H[9 * j + i] = h(i, j);
   delete m;
+}
+
+// GREEN_DAMPING(mu) at (F, Fdot): psi, PK1, hessian and clampedHessian (GREEN_DAMPING.cpp:38-97).
+double ref_green_psi(double mu, const double* F, const double* Fdot) {
+  VOLUME::GREEN_DAMPING d(mu);
+  return d.psi(loadColMajor(F), loadColMajor(Fdot));
+}
+
+void ref_green_pk1(double mu, const double* F, const double* Fdot, double* P) {
+  VOLUME::GREEN_DAMPING d(mu);
+  storeColMajor(d.PK1(loadColMajor(F), loadColMajor(Fdot)), P);
+}
+
+void ref_green_hessian(double mu, const double* F, const double* Fdot, int clamped, double* H) {
+  VOLUME::GREEN_DAMPING d(mu);
+  const MATRIX9 h = clamped ? d.clampedHessian(loadColMajor(F), loadColMajor(Fdot))
+                            : d.hessian(loadColMajor(F), loadColMajor(Fdot));
+  for (int j = 0; j < 9; j++)
+    for (int i = 0; i < 9; i++) H[9 * j + i] = h(i, j);
 }
 
 // ---- file formats ----

@@ -15,6 +15,11 @@ small .npz fixtures (no pickles) committed beside this script:
   c1_bdf1.npz           the C1 scene with the BDF_1 Newton loop (src/Timestepper/BDF_1.cpp:78-239,
                         3 Newton iterations, residual tolerance 1e-7), 10 steps: the last Newton
                         iteration's PCG count per step, states after steps 1, 5 and 10
+  green_damping.npz     energy damping: GREEN_DAMPING psi, PK1, dP/dFdot and its clamped form per
+                        (F, Fdot) (src/Damping/Volume/GREEN_DAMPING.cpp); the elastic + damping
+                        forces and the damping stiffness C of a 3x3x3 lattice at a seeded state
+                        and velocity gradient (TET_MESH.cpp:816-858, TET_MESH_FASTER.cpp:138-180);
+                        20 solveEnergyDamped steps of scenes.green_scene (Jacobi PCG, tol 1e-8)
 
 Usage:  python tests/golden/make_golden.py [--skip-c2]
 """
@@ -245,6 +250,54 @@ def make_drop(out):
     np.savez_compressed(out, **data)
 
 
+def make_green(out):
+    ref = oracle.RefElement()
+    Fs = element_fs()
+    rng = np.random.default_rng(17)
+    Fds = rng.uniform(-1, 1, (len(Fs), 3, 3))
+    Fds[0] = 0.0
+    W = rng.normal(size=(3, 3))
+    Fds[1] = (W - W.T) @ Fs[1]  # a spin rate: Edot = 0
+    mu = 0.37
+    data = {"F": Fs, "Fdot": Fds, "mu": np.array(mu)}
+    data["psi"] = np.array([ref.green_psi(mu, F, Fd) for F, Fd in zip(Fs, Fds)])
+    data["P"] = np.array([ref.green_pk1(mu, F, Fd) for F, Fd in zip(Fs, Fds)])
+    data["H"] = np.array([ref.green_hessian(mu, F, Fd, clamped=False) for F, Fd in zip(Fs, Fds)])
+    data["Hc"] = np.array([ref.green_hessian(mu, F, Fd, clamped=True) for F, Fd in zip(Fs, Fds)])
+    # mesh level
+    sc = scenes.lattice_scene((3, 3, 3), seed=11, E=1e5, nu=0.45, pin=False)
+    sc.green_mu = 0.02 * sc.mu
+    u = 0.05 * rng.uniform(-1, 1, sc.dofs)
+    vf = rng.uniform(-1, 1, sc.dofs)
+    r = oracle.RefSim(sc)
+    r.compute_fdots(vf)
+    R, csc = r.damped_forces_hessian(u)
+    o = oracle.Oracle(sc)
+    rp, ci = o.bsr_pattern()
+    data.update(mesh_rest=sc.rest, mesh_tets=sc.tets, mesh_mat=np.array([sc.material, sc.mu, sc.lam]),
+                mesh_green_mu=np.array(sc.green_mu), mesh_u=u, mesh_fdot_v=vf, mesh_R=R,
+                mesh_row_ptr=rp, mesh_col_idx=ci, mesh_C=oracle.csc_to_bsr_vals(csc, rp, ci))
+    r.close()
+    o.close()
+    # trajectory
+    sc = scenes.green_scene()
+    r = oracle.RefSim(sc)
+    iters, errs = [], []
+    for k in range(1, 21):
+        it, err, sec = r.step()
+        iters.append(it)
+        errs.append(err)
+        if k in (1, 5, 10, 20):
+            uk, vk = r.state()
+            data[f"u{k}"] = uk
+            data[f"v{k}"] = vk
+        print(f"  step {k}: iters {it} err {err:.3e}", flush=True)
+    r.close()
+    data["iters"] = np.array(iters)
+    data["errs"] = np.array(errs)
+    np.savez_compressed(out, **data)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--skip-c2", action="store_true")
@@ -254,7 +307,8 @@ def main():
         sys.exit("oracle/_ref/libhobak_ref.so is not built (make -C oracle ref)")
     jobs = [("element_fixtures.npz", make_element), ("mesh_small.npz", make_mesh_small),
             ("c1_traj.npz", make_c1), ("c1_arap.npz", make_c1_arap), ("c1_bdf1.npz", make_c1_bdf1),
-            ("drop_traj.npz", make_drop), ("drop_test_cylinder.npz", make_drop_test)]
+            ("drop_traj.npz", make_drop), ("drop_test_cylinder.npz", make_drop_test),
+            ("green_damping.npz", make_green)]
     if not args.skip_c2:
         jobs.append(("c2_traj.npz", make_c2))
     for name, fn in jobs:

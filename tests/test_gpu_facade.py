@@ -23,7 +23,8 @@ pytestmark = pytest.mark.gpu
 def _write_scene(path, sc, steps):
     cube = sc.pin_cube if sc.pin_cube is not None else (0.0, 0.0, 0.0, 1.0)
     ground = sc.collision_cubes[0] if sc.collision_cubes else (0.0, 0.0, 0.0, 1.0)
-    flags = (1 if sc.pin_cube is not None else 0) | (2 if sc.collision_cubes else 0)
+    flags = ((1 if sc.pin_cube is not None else 0) | (2 if sc.collision_cubes else 0)
+             | (4 if sc.green_mu is not None else 0))
     with open(path, "wb") as f:
         f.write(struct.pack("<5i", sc.nv, sc.nt, sc.material, steps, flags))
         f.write(struct.pack("<18d", sc.mu, sc.lam, sc.dt, sc.rayleigh_alpha, sc.rayleigh_beta,
@@ -32,6 +33,10 @@ def _write_scene(path, sc, steps):
         f.write(np.ascontiguousarray(sc.tets, dtype="<i4").tobytes())
         f.write(np.ascontiguousarray(sc.initial_u(), dtype="<f8").tobytes())
         f.write(np.ascontiguousarray(sc.initial_v(), dtype="<f8").tobytes())
+        if sc.green_mu is not None:
+            fv = sc.fdot_v if sc.fdot_v is not None else np.zeros(sc.dofs)
+            f.write(struct.pack("<d", sc.green_mu))
+            f.write(np.ascontiguousarray(fv, dtype="<f8").tobytes())
 
 
 def _read_out(path, dofs, planes=False):
@@ -116,3 +121,44 @@ def test_cpp_facade_collision_cube_matches_reference(tmp_path):
     X = sc.rest.ravel()
     assert rel(X + u, X + g["u20"]) <= 1e-9
     assert rel(v, g["v20"]) <= 1e-9
+
+
+def test_cpp_facade_energy_damping_matches_reference(tmp_path):
+    """setDamping(GREEN_DAMPING) and TET_MESH::computeFdots through the C++ façade: the energy-damped
+    steps of scenes.green_scene follow the reference's (Jacobi PCG) to +-1 iteration and 1e-9."""
+    from conftest import golden
+
+    g = golden("green_damping.npz")
+    sc = scenes.green_scene()
+    steps = 20
+    scene_bin = tmp_path / "scene.bin"
+    out_bin = tmp_path / "out.bin"
+    _write_scene(scene_bin, sc, steps)
+    env = dict(os.environ, HOBAK_FACADE_PRECOND="0")
+    print(f"C++ façade: energy-damped scene ({sc.nt} tets), {steps} steps", flush=True)
+    subprocess.run([hb.DRIVER, str(scene_bin), str(out_bin)], check=True, timeout=120, env=env)
+    iters, u, v = _read_out(out_bin, sc.dofs)
+    assert max(abs(a - b) for a, b in zip(iters, g["iters"][:steps])) <= 1
+    assert rel(u, g["u20"]) <= 1e-9
+    assert rel(v, g["v20"]) <= 1e-9
+
+
+def test_python_facade_energy_damping_matches_c_abi(gpu_ctx):
+    sc = scenes.green_scene(n=(5, 5, 5))
+    mesh = lib.TetMesh(sc.rest, sc.tets)
+    be = lib.BackwardEulerPosition(mesh, lib.SNH(sc.mu, sc.lam), ctx=gpu_ctx)
+    be.setDt(sc.dt)
+    be.scaleMass(sc.mass_scale)
+    be.addGravity(sc.gravity)
+    be.setTolerance(sc.pcg_tol)
+    be.attachKinematicConstraints(scenes.Cube(scenes.PIN_X0_CUBE[:3], scenes.PIN_X0_CUBE[3]))
+    be.setDamping(lib.GREEN_DAMPING(sc.green_mu))
+    mesh.computeFdots(sc.fdot_v)
+    be.velocity()[:] = sc.initial_v()
+    inst = lib.Instance(gpu_ctx, sc, precond=scenes.PRECOND_BLOCK_JACOBI)
+    for _ in range(4):
+        be.solve(False)
+        inst.step()
+    ud, vd = inst.state()
+    inst.close()
+    assert np.array_equal(be.position(), ud) and np.array_equal(be.velocity(), vd)
