@@ -42,6 +42,12 @@ sys.path.insert(0, ROOT)
 
 METRIC = "tet-Newton-steps/s + p50/p99 step latency vs FP64 GPU path at 1e-6 error"
 HBM_PEAK_GBS = 8000.0
+FP64_VALU_PEAK_TFLOPS = 78.6
+# FP64 flops the element kernel executes per tet (forces and clamped stiffness), counted with
+# rocprofv3 (64 x SQ_INSTS_VALU_FLOPS_FP64 incl. transcendental ops / tets), and the kernel's
+# VALUBusy from the same runs: profiles/element_kernel_40cube_valu.md
+ELEMENT_FP64_FLOPS_PER_TET = 2094.0
+ELEMENT_VALU_BUSY = 0.072
 GRAPHS = [1]  # --graphs, applied to every context the benchmark creates
 
 
@@ -345,6 +351,18 @@ def main():
     ks = inst.kernel_stats()
     u, _ = inst.state()
     finite = bool(np.all(np.isfinite(u)))
+    # one more step outside the timed region, with direct launches, so that the element kernel
+    # has events of its own (a captured step times its prologue as one piece)
+    ctx.set_tuning(graphs=0)
+    inst.step()
+    el_ms = inst.timing_ms()["element"]
+    ctx.set_tuning(graphs=GRAPHS[0])
+    el_tflops = ELEMENT_FP64_FLOPS_PER_TET * sc.nt / (el_ms * 1e-3) / 1e12 if el_ms > 0 else 0.0
+    element = {"kernel": "element_kernel<double>", "bound": "latency", "ms": el_ms,
+               "achieved": round(el_tflops, 3), "peak": FP64_VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
+               "frac": round(el_tflops / FP64_VALU_PEAK_TFLOPS, 4), "valu_busy": ELEMENT_VALU_BUSY,
+               "flops_per_tet": ELEMENT_FP64_FLOPS_PER_TET,
+               "source": "profiles/element_kernel_40cube_valu.md"}
 
     spmv_gbs = ks["spmv_bytes"] / (ks["spmv_avg_ms"] * 1e-3) / 1e9 if ks["spmv_avg_ms"] > 0 else 0.0
     tr = measured_traffic(args.n)
@@ -390,6 +408,7 @@ def main():
         "setup_s": round(setup_s, 3),
         "finite": finite,
         "roofline": roofline,
+        "element_kernel": element,
     }
     if args.precision_form == "on":
         out["fp32_refine"] = precision_form(lib, sc, precond, args, total, u)

@@ -58,7 +58,7 @@ struct hobak_ctx {
   int grid_cap = 0;                    // max workgroups per launch (0: one per virtual workgroup)
   int update_cap = kUpdateCap;         // per-instance workgroups of the PCG update kernel
   int spmv_pad = 0;                    // SpMV loop over padded slice widths (1) or row lengths (0)
-  int spmv_stream_mb = 192;            // non-temporal matrix loads above this SpMV size (MiB)
+  int spmv_stream_mb = 384;            // non-temporal matrix loads above this SpMV size (MiB)
   hipStream_t stream = nullptr;
   size_t budget = 0;
   size_t in_use = 0;
@@ -337,10 +337,10 @@ struct Plan {
   InstStatus* status_dev = nullptr;
   InstStatus* status_host = nullptr;
   int contact = 0;              // some member has collision objects
+  int damping = 0;              // some member uses energy damping
   double spmv_bytes = 0.0;      // summed over the members
-  double spmv_bytes_max = 0.0;  // of the largest member
-  hipGraphExec_t chunk_exec = nullptr;
-  int chunk_k = 0;
+  hipGraphExec_t chunk_exec[2] = {nullptr, nullptr};  // PCG group with cached / streamed matrix loads
+  int chunk_k[2] = {0, 0};
   hipGraphExec_t pro_exec[2] = {nullptr, nullptr};  // prologue without / with the step start
   hipGraphExec_t epi_exec = nullptr;
   uint64_t stamp = 0;
@@ -363,14 +363,13 @@ int plan_fill(hobak_ctx* ctx, Plan& P) {
   const InstDev** hp = (const InstDev**)(P.host + off);
   const InstDev* const* dp = (const InstDev* const*)(P.dev + off);
   P.spmv_bytes = 0.0;
-  P.spmv_bytes_max = 0.0;
   P.contact = 0;
+  P.damping = 0;
   for (int i = 0; i < n; i++) {
     hp[i] = P.members[i]->dev;
     P.contact |= P.members[i]->h.contact;
-    const double sb = spmv_bytes_of(P.members[i]);
-    P.spmv_bytes += sb;
-    P.spmv_bytes_max = std::max(P.spmv_bytes_max, sb);
+    P.damping |= P.members[i]->h.damping;
+    P.spmv_bytes += spmv_bytes_of(P.members[i]);
   }
   off += al(sizeof(void*) * n);
   for (int k = 0; k < K_COUNT; k++) {
@@ -400,7 +399,8 @@ int plan_check_members(hobak_ctx* ctx, const std::vector<hobak_inst*>& m) {
 
 void plan_free(Plan* P) {
   if (!P) return;
-  if (P->chunk_exec) (void)hipGraphExecDestroy(P->chunk_exec);
+  for (auto& g : P->chunk_exec)
+    if (g) (void)hipGraphExecDestroy(g);
   for (auto& g : P->pro_exec)
     if (g) (void)hipGraphExecDestroy(g);
   if (P->epi_exec) (void)hipGraphExecDestroy(P->epi_exec);
@@ -534,7 +534,8 @@ int launch_prologue(hobak_ctx* ctx, const Plan& J, const Plan* S, bool timed) {
   const int cap = ctx->grid_cap;
   if (S) HIPCHK(ctx, launch_begin_step(S->b[K_ROW], cap, s));
   if (S && S->contact) HIPCHK(ctx, launch_contact_begin(S->b[K_ROW], cap, s));
-  HIPCHK(ctx, launch_element<T>(J.b[K_ELEM], ELEM_U, 1, 1, cap, s));
+  HIPCHK(ctx, launch_element<T>(J.b[K_ELEM], ELEM_U, 1, cap, s));
+  if (J.damping) HIPCHK(ctx, launch_damping_force<T>(J.b[K_ELEM], ELEM_U, cap, s));
   if (timed) HIPCHK(ctx, hipEventRecord(ctx->ev[1], s));
   HIPCHK(ctx, launch_assemble<T>(J.b[K_SLOT], ASM_SYSTEM, cap, s));
   if (timed) HIPCHK(ctx, hipEventRecord(ctx->ev[2], s));
@@ -549,17 +550,22 @@ int launch_prologue(hobak_ctx* ctx, const Plan& J, const Plan* S, bool timed) {
   return HOBAK_OK;
 }
 
+// Whether a group's SpMV launches read the matrix with non-temporal loads: when the matrices of
+// the instances still solving exceed the threshold (default 1.5x the 256 MB Infinity Cache), few
+// of their lines survive to the next iteration, and streaming them keeps the gathered vectors
+// cached instead.  Below it much of the matrix is re-read from the cache, which non-temporal
+// loads would give up.
+bool stream_matrix(const hobak_ctx* ctx, double solving_bytes) {
+  return solving_bytes > (double)ctx->spmv_stream_mb * (1 << 20);
+}
+
 // k PCG iterations over the batch (k even, so the current scalar set is set 0 again at the end),
 // then the gated end-of-solve kernels: the refinement residual, check and re-seed, or the plain
 // form's finish flag.  With a sampling scheduler, some SpMV launches carry timing events.
 template <typename T>
-int launch_chunk(hobak_ctx* ctx, const Plan& P, int k, hobak_sched* S, int every) {
+int launch_chunk(hobak_ctx* ctx, const Plan& P, int k, int stream, hobak_sched* S, int every) {
   hipStream_t s = ctx->stream;
   const int cap = ctx->grid_cap;
-  // streamed matrix loads pay off when one instance's matrix is too large to stay cached (its
-  // gathered vectors are then re-fetched if the matrix displaces them); a batch of small
-  // instances gathers from small windows either way
-  const int stream = P.spmv_bytes_max > (double)ctx->spmv_stream_mb * (1 << 20) ? 1 : 0;
   for (int j = 0; j < k; j++) {
     hipEvent_t a = nullptr, b = nullptr;
     if (S) {
@@ -730,10 +736,12 @@ int tick_impl(hobak_sched* S) {
   bool ran = false;
   int m = ctx->max_chunks;
   int nsolving = 0;
+  double solving_bytes = 0.0;
   const int k = ctx->chunk_iters;
   for (auto& r : S->active) {
     if (r.stage != 1) continue;
     nsolving++;
+    solving_bytes += spmv_bytes_of(r.I);
     const int p = r.I->pred[r.newton_k];
     const int q = p > 0 ? (p - r.done_iters) / k : 0;
     m = std::min(m, std::max(1, q));
@@ -744,11 +752,12 @@ int tick_impl(hobak_sched* S) {
     if (S->sampling)
       for (auto* I : S->members)
         if (I->sample_every > 0) every = every ? std::min(every, I->sample_every) : I->sample_every;
-    if (graphs && (!M.chunk_exec || M.chunk_k != k)) {
-      if (M.chunk_exec) (void)hipGraphExecDestroy(M.chunk_exec);
-      M.chunk_exec = nullptr;
-      TRY(capture(ctx, &M.chunk_exec, [&]() { return launch_chunk<T>(ctx, M, k, nullptr, 0); }));
-      M.chunk_k = k;
+    const int st = stream_matrix(ctx, solving_bytes) ? 1 : 0;
+    if (graphs && (!M.chunk_exec[st] || M.chunk_k[st] != k)) {
+      if (M.chunk_exec[st]) (void)hipGraphExecDestroy(M.chunk_exec[st]);
+      M.chunk_exec[st] = nullptr;
+      TRY(capture(ctx, &M.chunk_exec[st], [&]() { return launch_chunk<T>(ctx, M, k, st, nullptr, 0); }));
+      M.chunk_k[st] = k;
     }
     for (int j = 0; j < m; j++) {
       // a group that holds a sampled SpMV launch (index g with g % every == every - 1) runs
@@ -757,10 +766,10 @@ int tick_impl(hobak_sched* S) {
       const bool sampled = every > 0 && (int)S->samples.size() < kMaxSamples &&
                            (g0 / every) * every + every - 1 < g0 + k;
       if (graphs && !sampled) {
-        HIPCHK(ctx, hipGraphLaunch(M.chunk_exec, s));
+        HIPCHK(ctx, hipGraphLaunch(M.chunk_exec[st], s));
         if (S->sampling) S->sample_it += k;
       } else {
-        TRY(launch_chunk<T>(ctx, M, k, S->sampling ? S : nullptr, every));
+        TRY(launch_chunk<T>(ctx, M, k, st, S->sampling ? S : nullptr, every));
       }
     }
     HIPCHK(ctx, hipEventRecord(ctx->ev[5], s));
@@ -943,7 +952,7 @@ int setup_constant_parts(hobak_inst* I) {
   TRY(plan_transient(ctx, {I}, P));
   // rest-state stiffness (u = 0): TIMESTEPPER::buildRayleighDampingMatrix (TIMESTEPPER.cpp:213-236)
   HIPCHK(ctx, hipMemsetAsync(I->h.tmp64, 0, sizeof(double) * I->dofs, s));
-  HIPCHK(ctx, launch_element<T>(P.b[K_ELEM], ELEM_TMP, 1, 0, ctx->grid_cap, s));
+  HIPCHK(ctx, launch_element<T>(P.b[K_ELEM], ELEM_TMP, 1, ctx->grid_cap, s));
   HIPCHK(ctx, launch_assemble<T>(P.b[K_SLOT], ASM_REST, ctx->grid_cap, s));
   HIPCHK(ctx, launch_constant_parts<T>(P.b[K_SLOT], ctx->grid_cap, s));
   HIPCHK(ctx, hipStreamSynchronize(s));
@@ -979,7 +988,7 @@ int eval_forces_hessian(hobak_inst* I, const double* u, double* forces, double* 
   Plan P;
   TRY(plan_transient(ctx, {I}, P));
   TRY(upload(I, I->h.tmp64, u, (size_t)I->dofs));
-  HIPCHK(ctx, launch_element<T>(P.b[K_ELEM], ELEM_TMP, kvals != nullptr, 0, ctx->grid_cap, s));
+  HIPCHK(ctx, launch_element<T>(P.b[K_ELEM], ELEM_TMP, kvals != nullptr, ctx->grid_cap, s));
   if (forces) {
     HIPCHK(ctx, launch_force_gather<T>(P.b[K_ROW], ctx->grid_cap, s));
     HIPCHK(ctx, hipMemcpyAsync(forces, I->h.rhs64, sizeof(double) * I->dofs, hipMemcpyDeviceToHost, s));
@@ -1005,7 +1014,8 @@ int eval_damping(hobak_inst* I, const double* u, double* forces, double* cvals) 
   TRY(upload(I, I->h.tmp64, u, (size_t)I->dofs));
   I->has_system = false;  // rhs64 and A are reused
   if (forces) {
-    HIPCHK(ctx, launch_element<T>(P.b[K_ELEM], ELEM_TMP, 0, 1, ctx->grid_cap, s));
+    HIPCHK(ctx, launch_element<T>(P.b[K_ELEM], ELEM_TMP, 0, ctx->grid_cap, s));
+    HIPCHK(ctx, launch_damping_force<T>(P.b[K_ELEM], ELEM_TMP, ctx->grid_cap, s));
     HIPCHK(ctx, launch_force_gather<T>(P.b[K_ROW], ctx->grid_cap, s));
     HIPCHK(ctx, hipMemcpyAsync(forces, I->h.rhs64, sizeof(double) * I->dofs, hipMemcpyDeviceToHost, s));
   }

@@ -8,7 +8,7 @@
 //   TET_MESH_FASTER::computeHyperelasticClampedHessian element loop
 //                                         src/Geometry/TET_MESH_FASTER.cpp:93-103
 //   TET_MESH::computeInternalForces       src/Geometry/TET_MESH.cpp:816-858 (per-tet part: the
-//                                         GREEN_DAMPING stress added to the elastic one)
+//                                         GREEN_DAMPING force, damping_force_kernel)
 //   TET_MESH::computeFdots                src/Geometry/TET_MESH.cpp:660-684 (fdot_kernel)
 //   TET_MESH_FASTER::computeDampingHessian element loop, src/Geometry/TET_MESH_FASTER.cpp:144-150
 //                                         (damping_hessian_kernel)
@@ -39,9 +39,7 @@ struct ElementIn {
   HOBAK_GLOBAL void* stage_f;
   const HOBAK_GLOBAL int* hdest;
   const HOBAK_GLOBAL int* fdest;
-  const HOBAK_GLOBAL double* fdot;  // held velocity gradients (damp != 0)
-  double dmu;
-  int with_hessian, damp;
+  int with_hessian;
 };
 
 // GREEN_DAMPING stress mu F (F^T Fdot + Fdot^T F) (src/Damping/Volume/GREEN_DAMPING.cpp:49-52),
@@ -133,14 +131,6 @@ __device__ __forceinline__ void element_tet(const ElementIn& a, int t) {
   } else {
     arap_pk1<T>((T)a.mu, U, S, V, P);
   }
-  if (a.damp) {  // computeInternalForces: flatten(elasticPK1 + dampingPK1)
-    double Pd[3][3];
-    green_pk1(a.dmu, F, a.fdot, nt, t, Pd);
-#pragma unroll
-    for (int i = 0; i < 3; i++)
-#pragma unroll
-      for (int j = 0; j < 3; j++) P[i][j] += (T)Pd[i][j];
-  }
   auto sf = (HOBAK_GLOBAL T*)a.stage_f;
 #pragma unroll
   for (int v = 0; v < 4; v++) {
@@ -215,11 +205,9 @@ __device__ __forceinline__ void element_tet(const ElementIn& a, int t) {
 }
 
 // Segmented over instances (one lane per tet, kElementBlock tets per virtual workgroup); the
-// material is a per-instance, workgroup-uniform branch.  damp: add the damping stress of
-// instances that use energy damping (the step and the damped-force hook; the elastic-force hook
-// and the rest-state stiffness pass 0).
+// material is a per-instance, workgroup-uniform branch.
 template <typename T>
-__global__ __launch_bounds__(kElementBlock) void element_kernel(Batch b, int src, int with_hessian, int damp) {
+__global__ __launch_bounds__(kElementBlock) void element_kernel(Batch b, int src, int with_hessian) {
   for (int ws = blockIdx.x; ws < b.total; ws += gridDim.x) {
     const int vb = xcd_vb(ws, b.total);
     int lb, lg;
@@ -239,9 +227,6 @@ __global__ __launch_bounds__(kElementBlock) void element_kernel(Batch b, int src
     a.hdest = gbl(I->hdest);
     a.fdest = gbl(I->fdest);
     a.with_hessian = with_hessian;
-    a.damp = damp && I->damping;
-    a.fdot = gbl(I->fdot);
-    a.dmu = I->dmu;
     const int material = I->material;
     const int t = lb * kElementBlock + threadIdx.x;
     if (t < a.nt) {
@@ -254,15 +239,88 @@ __global__ __launch_bounds__(kElementBlock) void element_kernel(Batch b, int src
 }
 
 template <typename T>
-hipError_t launch_element(const Batch& b, int src, int with_hessian, int damp, int cap, hipStream_t s) {
+hipError_t launch_element(const Batch& b, int src, int with_hessian, int cap, hipStream_t s) {
   if (b.total <= 0) return hipSuccess;
   const int grid = cap > 0 && cap < b.total ? cap : b.total;
-  hipLaunchKernelGGL((element_kernel<T>), dim3(grid), dim3(kElementBlock), 0, s, b, src, with_hessian, damp);
+  hipLaunchKernelGGL((element_kernel<T>), dim3(grid), dim3(kElementBlock), 0, s, b, src, with_hessian);
   return hipGetLastError();
 }
 
-template hipError_t launch_element<double>(const Batch&, int, int, int, int, hipStream_t);
-template hipError_t launch_element<float>(const Batch&, int, int, int, int, hipStream_t);
+template hipError_t launch_element<double>(const Batch&, int, int, int, hipStream_t);
+template hipError_t launch_element<float>(const Batch&, int, int, int, hipStream_t);
+
+// The GREEN_DAMPING part of computeInternalForces, after the element kernel: for each tet of an
+// instance with energy damping, f_v += -V P_damping d_v^T into the staged forces (one lane per
+// tet; instances without damping leave at once).  The reference sums the two stresses before the
+// dF/dx^T product (TET_MESH.cpp:831-834); adding the two products instead differs by round-off
+// only, and keeps the element kernel, and so every undamped instance, bit for bit unchanged.
+template <typename T>
+__global__ __launch_bounds__(kElementBlock) void damping_force_kernel(Batch b, int src) {
+  for (int ws = blockIdx.x; ws < b.total; ws += gridDim.x) {
+    const int vb = xcd_vb(ws, b.total);
+    int lb, lg;
+    const HOBAK_GLOBAL InstDev* __restrict__ I = seg_find(b, vb, lb, lg);
+    if (!I->damping) continue;
+    const int t = lb * kElementBlock + threadIdx.x;
+    const int nt = I->nt;
+    if (t >= nt) continue;
+    const size_t n = (size_t)nt;
+    auto tets = gbl((const int*)I->tets);
+    auto rest = gbl(I->rest);
+    auto u = gbl(src == ELEM_TMP ? I->tmp64 : I->u);
+    auto dminv = gbl(I->dminv);
+    const int vid[4] = {tets[4 * (size_t)t], tets[4 * (size_t)t + 1], tets[4 * (size_t)t + 2],
+                        tets[4 * (size_t)t + 3]};
+    double x[4][3], Di[3][3], F[3][3];
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+#pragma unroll
+      for (int c = 0; c < 3; c++) x[k][c] = rest[3 * (size_t)vid[k] + c] + u[3 * (size_t)vid[k] + c];
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+      for (int c = 0; c < 3; c++) Di[r][c] = dminv[(3 * r + c) * n + t];
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+      for (int c = 0; c < 3; c++)
+        F[r][c] = ((x[1][r] - x[0][r]) * Di[0][c] + (x[2][r] - x[0][r]) * Di[1][c]) +
+                  (x[3][r] - x[0][r]) * Di[2][c];
+    double P[3][3];
+    green_pk1(I->dmu, F, gbl(I->fdot), n, t, P);
+    double d[4][3];
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      d[0][j] = (-Di[0][j] - Di[1][j]) - Di[2][j];
+      d[1][j] = Di[0][j];
+      d[2][j] = Di[1][j];
+      d[3][j] = Di[2][j];
+    }
+    const double vol = gbl(I->vol)[t];
+    auto sf = gbl((T*)I->stage_f);
+    auto fd = gbl(I->fdest) + 4 * (size_t)t;
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      auto dst = sf + 3 * (size_t)fd[v];
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        const double s = (P[c][0] * d[v][0] + P[c][1] * d[v][1]) + P[c][2] * d[v][2];
+        dst[c] = dst[c] + (T)(-vol * s);
+      }
+    }
+  }
+}
+
+template <typename T>
+hipError_t launch_damping_force(const Batch& b, int src, int cap, hipStream_t s) {
+  if (b.total <= 0) return hipSuccess;
+  const int grid = cap > 0 && cap < b.total ? cap : b.total;
+  hipLaunchKernelGGL((damping_force_kernel<T>), dim3(grid), dim3(kElementBlock), 0, s, b, src);
+  return hipGetLastError();
+}
+
+template hipError_t launch_damping_force<double>(const Batch&, int, int, hipStream_t);
+template hipError_t launch_damping_force<float>(const Batch&, int, int, hipStream_t);
 
 // TET_MESH::computeFdots: one lane per tet, Fdot = V DmInv with V's columns v_k - v_0
 // (TET_MESH.cpp:666-682), the product in the same summation order as F = Ds DmInv.

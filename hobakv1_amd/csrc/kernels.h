@@ -244,7 +244,10 @@ enum ElemSrc { ELEM_U = 0, ELEM_TMP = 1 };
 enum AsmTarget { ASM_REST = 0, ASM_PLAIN_A = 1, ASM_SYSTEM = 2 };
 
 template <typename T>
-hipError_t launch_element(const Batch& b, int src, int with_hessian, int damp, int cap, hipStream_t s);
+hipError_t launch_element(const Batch& b, int src, int with_hessian, int cap, hipStream_t s);
+// Adds the GREEN_DAMPING force of the instances with energy damping to the staged element forces.
+template <typename T>
+hipError_t launch_damping_force(const Batch& b, int src, int cap, hipStream_t s);
 // Velocity gradients Fdot = [v1 - v0, v2 - v0, v3 - v0] DmInv of one instance's tets.
 hipError_t launch_fdots(const InstDev& h, const double* vel, double* fdot, hipStream_t s);
 // The damping stiffness blocks vol * dP/dFdot (positive semi-definite) of every tet at the

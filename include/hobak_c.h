@@ -208,11 +208,13 @@ int hobak_kernel_stats(hobak_inst* inst, double* spmv_avg_ms, int64_t* spmv_coun
  *                          afterwards
  *   HOBAK_TUNE_SPMV_PAD    1: the SpMV walks every slice's padded width with a uniform trip
  *                          count; 0: each lane walks only its own row's blocks
- *   HOBAK_TUNE_SPMV_STREAM_MB  a batch holding an instance whose SpMV moves more than this many
- *                          MiB per launch reads its matrix with non-temporal loads (default 192: a
- *                          matrix that fits the 256 MB Infinity Cache stays cached across
- *                          iterations; a larger one would only displace the gathered vectors);
- *                          0 streams always */
+ *   HOBAK_TUNE_SPMV_STREAM_MB  while the matrices of the instances still solving move more than
+ *                          this many MiB per SpMV launch, the SpMV reads them with non-temporal
+ *                          loads (default 384, 1.5x the 256 MB Infinity Cache: below it much of
+ *                          the matrix is still re-read from the cache; above it streaming keeps
+ *                          the gathered vectors cached instead).  Decided per group of PCG
+ *                          iterations, so a batch switches to cached loads as its instances
+ *                          converge; 0 streams always */
 #define HOBAK_TUNE_SLICE_ROWS 0
 #define HOBAK_TUNE_POLL_CHUNK 2
 #define HOBAK_TUNE_MAX_CHUNKS 3
