@@ -303,7 +303,17 @@ def serving_form(lib, scenes, args):
     cb = lib.Context(0)
     ca.set_tuning(graphs=GRAPHS[0])
     cb.set_tuning(graphs=GRAPHS[0])
-    out = serve.compare(ca, cb, stream, scene_of, int(args.budget_gib * (1 << 30)))
+    out = serve.compare(ca, cb, stream, scene_of, int(args.budget_gib * (1 << 30)),
+                        sample_every=args.sample_every)
+    ks = out.pop("spmv", None)
+    if ks and ks["spmv_avg_ms"] > 0:
+        gbs = ks["spmv_bytes"] / (ks["spmv_avg_ms"] * 1e-3) / 1e9
+        out["roofline"] = {"bound": "hbm", "kernel": "cg_spmv_kernel<float>", "achieved": round(gbs, 1),
+                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+                           "bytes_per_launch": ks["spmv_bytes"], "avg_launch_ms": ks["spmv_avg_ms"],
+                           "launches_sampled": ks["spmv_count"],
+                           "note": "sampled launches of the serving engine; bytes are those of the "
+                                   "instances solving in the launch's tick"}
     ca.close()
     cb.close()
     out["rate_sessions_per_s"] = args.rate

@@ -260,6 +260,11 @@ struct hobak_sched {
   std::vector<std::pair<int, int>> samples;
   int sample_it = 0;
   bool sampling = false;
+  // a long-lived scheduler's own sampling (hobak_sched_set_kernel_sampling): every spmv_every-th
+  // SpMV launch is timed and credited with the matrix bytes of the instances solving in its tick
+  int spmv_every = 0;
+  double spmv_ms = 0.0, spmv_bytes = 0.0;
+  long long spmv_n = 0;
 };
 
 namespace {
@@ -749,9 +754,12 @@ int tick_impl(hobak_sched* S) {
   if (nsolving) {
     HIPCHK(ctx, hipEventRecord(ctx->ev[4], s));
     int every = 0;
-    if (S->sampling)
+    if (S->sampling && S->ephemeral) {
       for (auto* I : S->members)
         if (I->sample_every > 0) every = every ? std::min(every, I->sample_every) : I->sample_every;
+    } else if (S->sampling) {
+      every = S->spmv_every;
+    }
     const int st = stream_matrix(ctx, solving_bytes) ? 1 : 0;
     if (graphs && (!M.chunk_exec[st] || M.chunk_k[st] != k)) {
       if (M.chunk_exec[st]) (void)hipGraphExecDestroy(M.chunk_exec[st]);
@@ -793,6 +801,17 @@ int tick_impl(hobak_sched* S) {
   }
   if (ran) S->ms[3] += elapsed(ctx, 4, 5);
   if (epi) S->ms[4] += elapsed(ctx, 6, 7);
+  if (!S->ephemeral && S->sampling && !S->samples.empty()) {
+    // every launch of this tick's groups ran over the same set of solving instances
+    for (auto& sm : S->samples) {
+      float ms = 0.f;
+      HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->sample_ev[2 * sm.second], ctx->sample_ev[2 * sm.second + 1]));
+      S->spmv_ms += ms;
+      S->spmv_bytes += solving_bytes;
+      S->spmv_n += 1;
+    }
+    S->samples.clear();
+  }
   S->stat[0] += 1;
   if (pro) S->stat[2] += elapsed(ctx, 0, 3);
   if (ran) S->stat[3] += elapsed(ctx, 4, 5);
@@ -1753,6 +1772,32 @@ double hobak_sched_clock(hobak_sched* S) { return S ? clock_s(S) : 0.0; }
 int hobak_sched_stats(hobak_sched* S, double* out) {
   if (!S || !out) return HOBAK_E_INVALID;
   std::memcpy(out, S->stat, sizeof(S->stat));
+  return HOBAK_OK;
+}
+
+int hobak_sched_set_kernel_sampling(hobak_sched* S, int32_t every) {
+  if (!S || every < 0) return HOBAK_E_INVALID;
+  hobak_ctx* ctx = S->ctx;
+  if (every > 0 && ctx->sample_ev.empty()) {
+    (void)hipSetDevice(ctx->device);
+    ctx->sample_ev.assign(2 * kMaxSamples, nullptr);
+    for (auto& e : ctx->sample_ev) HIPCHK(ctx, hipEventCreate(&e));
+  }
+  S->spmv_every = every;
+  S->sampling = every > 0;
+  S->sample_it = 0;
+  S->samples.clear();
+  S->spmv_ms = S->spmv_bytes = 0.0;
+  S->spmv_n = 0;
+  return HOBAK_OK;
+}
+
+int hobak_sched_kernel_stats(hobak_sched* S, double* spmv_avg_ms, int64_t* spmv_count, double* spmv_bytes) {
+  if (!S) return HOBAK_E_INVALID;
+  const double n = S->spmv_n ? (double)S->spmv_n : 1.0;
+  if (spmv_avg_ms) *spmv_avg_ms = S->spmv_n ? S->spmv_ms / n : 0.0;
+  if (spmv_count) *spmv_count = S->spmv_n;
+  if (spmv_bytes) *spmv_bytes = S->spmv_n ? S->spmv_bytes / n : 0.0;
   return HOBAK_OK;
 }
 

@@ -123,6 +123,8 @@ SIGNATURES = {
                                         _ip, _ip]),
     "hobak_sched_clock": (ctypes.c_double, [_vp]),
     "hobak_sched_stats": (ctypes.c_int, [_vp, _dp]),
+    "hobak_sched_set_kernel_sampling": (ctypes.c_int, [_vp, ctypes.c_int32]),
+    "hobak_sched_kernel_stats": (ctypes.c_int, [_vp, _dp, ctypes.POINTER(ctypes.c_int64), _dp]),
 }
 
 TUNE_SLICE_ROWS = 0
@@ -448,6 +450,17 @@ class Scheduler:
 
     def clock(self) -> float:
         return self.lib.hobak_sched_clock(self.h)
+
+    def set_kernel_sampling(self, every):
+        self.ctx.check(self.lib.hobak_sched_set_kernel_sampling(self.h, int(every)), "sched_sampling")
+
+    def kernel_stats(self):
+        ms = ctypes.c_double(0)
+        n = ctypes.c_int64(0)
+        b = ctypes.c_double(0)
+        self.ctx.check(self.lib.hobak_sched_kernel_stats(self.h, ctypes.byref(ms), ctypes.byref(n),
+                                                         ctypes.byref(b)), "sched_kernel_stats")
+        return {"spmv_avg_ms": ms.value, "spmv_count": n.value, "spmv_bytes": b.value}
 
     def stats(self):
         out = np.zeros(7)

@@ -93,9 +93,10 @@ def _admit(ctx, waiting, live, budget, precision, scene_of, clock, make_inst):
 
 
 def run(ctx, stream, scene_of, budget=0, precision=lib.PRECISION_FP32_REFINE,
-        precond=scenes.PRECOND_BLOCK_JACOBI, mode="batched", keep_states=True):
+        precond=scenes.PRECOND_BLOCK_JACOBI, mode="batched", keep_states=True, sample_every=0):
     """Serve `stream`; returns throughput, latency percentiles and (optionally) every session's
-    final displacement."""
+    final displacement.  With sample_every (batched mode), every sample_every-th SpMV launch is
+    timed and the result carries the sampled average (`spmv`)."""
     clock = Clock()
     pending = deque(sorted(stream, key=lambda s: s.arrival))
     waiting = deque()
@@ -112,6 +113,8 @@ def run(ctx, stream, scene_of, budget=0, precision=lib.PRECISION_FP32_REFINE,
         return lib.Instance(ctx, sc, precond=precond, precision=precision)
 
     sch = lib.Scheduler(ctx) if mode == "batched" else None
+    if sch is not None and sample_every > 0:
+        sch.set_kernel_sampling(sample_every)
     rr = deque()  # plain mode: round-robin queue of session ids with a pending request
 
     def finish(k):
@@ -174,8 +177,11 @@ def run(ctx, stream, scene_of, budget=0, precision=lib.PRECISION_FP32_REFINE,
                 finish(k)
     makespan = clock.now()
     engine = None
+    spmv = None
     if sch is not None:
         engine = sch.stats()
+        if sample_every > 0:
+            spmv = sch.kernel_stats()
         sch.close()
     return {
         "mode": mode,
@@ -193,14 +199,16 @@ def run(ctx, stream, scene_of, budget=0, precision=lib.PRECISION_FP32_REFINE,
         "max_concurrent_sessions": max_live,
         "admission_waits": admission_waits,
         "engine": engine,
+        "spmv": spmv,
         "final": final,
     }
 
 
-def compare(ctx_batched, ctx_plain, stream, scene_of, budget):
+def compare(ctx_batched, ctx_plain, stream, scene_of, budget, sample_every=0):
     """Serving (FP32 + refinement, batched) and the plain FP64 path on the same stream; the
     position error of every session after its last step, against the plain FP64 result."""
-    sv = run(ctx_batched, stream, scene_of, budget, lib.PRECISION_FP32_REFINE, mode="batched")
+    sv = run(ctx_batched, stream, scene_of, budget, lib.PRECISION_FP32_REFINE, mode="batched",
+             sample_every=sample_every)
     pl = run(ctx_plain, stream, scene_of, budget, lib.PRECISION_FP64, mode="plain")
     errs = []
     for k, u in sv["final"].items():
@@ -209,6 +217,7 @@ def compare(ctx_batched, ctx_plain, stream, scene_of, budget):
         errs.append(float(np.linalg.norm(u - u64) / np.linalg.norm(X + u64)))
     sv.pop("final")
     pl.pop("final")
+    pl.pop("spmv", None)
     return {
         **sv,
         "plain_fp64": pl,

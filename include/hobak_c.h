@@ -257,6 +257,15 @@ double hobak_sched_clock(hobak_sched* sched);
 /* Counters since creation, 7 doubles: ticks, PCG groups queued, GPU milliseconds of prologues,
  * of PCG groups and of epilogues, host milliseconds inside hobak_sched_tick, solves finished. */
 int hobak_sched_stats(hobak_sched* sched, double* out);
+/* Measurement hook of a scheduler: every `every`-th SpMV launch of its PCG groups is bracketed
+ * with HIP events on the context's stream (its group runs uncaptured); 0 turns it off.  Setting
+ * it clears the counters.  hobak_sched_kernel_stats returns the average sampled launch time, the
+ * number of samples and the average algorithmic bytes per sampled launch, counting the matrices
+ * and vectors of the instances that were solving in the launch's tick (the others' workgroups
+ * return at once). */
+int hobak_sched_set_kernel_sampling(hobak_sched* sched, int32_t every);
+int hobak_sched_kernel_stats(hobak_sched* sched, double* spmv_avg_ms, int64_t* spmv_count,
+                             double* spmv_bytes);
 
 /* Streaming copy of `bytes` device bytes (a calibration kernel for hardware byte counters);
  * returns the kernel time in ms. */

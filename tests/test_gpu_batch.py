@@ -134,14 +134,18 @@ def test_batch_bdf_newton_equals_alone(gpu_ctx):
         assert np.array_equal(ua, ub) and np.array_equal(va, vb)
 
 
-def test_scheduler_continuous_batching_equals_alone(gpu_ctx):
-    """Requests arrive while others run; each result equals the instance stepped alone."""
+@pytest.mark.parametrize("sample_every", [0, 4])
+def test_scheduler_continuous_batching_equals_alone(gpu_ctx, sample_every):
+    """Requests arrive while others run; each result equals the instance stepped alone (also with
+    the scheduler's SpMV sampling on, whose sampled groups run uncaptured)."""
     scs = [scenes.serving_instance(k, seed=7) for k in range(4)]
     scs = [s for s in scs if s.nt <= 60000][:3] or scs[:2]
     steps = 3
     alone = _run_alone(gpu_ctx, scs, steps, lib.PRECISION_FP32_REFINE)
     insts = [lib.Instance(gpu_ctx, sc, precision=lib.PRECISION_FP32_REFINE) for sc in scs]
     sch = lib.Scheduler(gpu_ctx)
+    if sample_every:
+        sch.set_kernel_sampling(sample_every)
     remaining = [steps] * len(insts)
     # staggered arrivals: instance i's first request arrives after i ticks
     submitted_first = [False] * len(insts)
@@ -175,6 +179,15 @@ def test_scheduler_continuous_batching_equals_alone(gpu_ctx):
         u, v = inst.state()
         (ua, va), _ = alone[i]
         assert np.array_equal(u, ua) and np.array_equal(v, va), i
+    ks = sch.kernel_stats()
+    if sample_every:
+        # a sampled launch is credited with the bytes of the instances solving in its tick: at
+        # least the smallest instance's, at most all of them
+        sizes = [3 * sc.nv * 4 * 2 + 4 * (sc.nv + 1) for sc in scs]
+        assert ks["spmv_count"] > 0 and ks["spmv_avg_ms"] > 0
+        assert min(sizes) < ks["spmv_bytes"]
+    else:
+        assert ks["spmv_count"] == 0
     sch.close()
     for inst in insts:
         inst.close()
