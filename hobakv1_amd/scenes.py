@@ -314,6 +314,24 @@ def green_scene(n=(8, 8, 8), seed=11, E=1e5, nu=0.45, material=MAT_SNH, green_fr
     return sc
 
 
+def two_tet_drop(tobj_path, E=3.0, name="two_tet_drop"):
+    """src/Scenes/TWO_TET_DROP_VF.h (E = 3) / TWO_TET_DROP_EE.h (E = 1) without their
+    self-collision response: SNH nu=0.45, density 1, gravity (0, -0.3, 0), dt = 1/30, Rayleigh
+    0.01/0.01, the ground
+    CUBE(center (0, -4.9, 0), scale 10) both pinning the surface vertices inside it
+    (attachKinematicSurfaceConstraints; the two meshes are all surface) and acting as a kinematic
+    collision object, Eigen's default PCG tolerance.  Until the first frame that detects a
+    self-collision (frame 27 of two_tet_drop_vf.json, 36 of two_tet_drop_ee.json) the reference's
+    step is exactly this one."""
+    from .formats import read_tobj
+
+    verts, tets = read_tobj(tobj_path)
+    m, l = lame(E, 0.45)
+    return Scene(name=name, rest=verts, tets=tets, material=MAT_SNH, mu=m, lam=l, dt=1.0 / 30.0,
+                 mass_scale=1.0, gravity=(0.0, -0.3, 0.0), pin_cube=(0.0, -4.9, 0.0, 10.0),
+                 pcg_tol=0.0, collision_cubes=((0.0, -4.9, 0.0, 10.0),))
+
+
 def config1(seed=1):
     """10^3 lattice, SNH E=1e5 nu=0.45 rho=1000, x=0 pinned, dt=1/60, 1 Newton iteration."""
     return lattice_scene((10, 10, 10), seed=seed, name="cube10_snh")

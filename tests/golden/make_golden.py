@@ -20,6 +20,11 @@ small .npz fixtures (no pickles) committed beside this script:
                         forces and the damping stiffness C of a 3x3x3 lattice at a seeded state
                         and velocity gradient (TET_MESH.cpp:816-858, TET_MESH_FASTER.cpp:138-180);
                         20 solveEnergyDamped steps of scenes.green_scene (Jacobi PCG, tol 1e-8)
+  two_tet_drop.npz      the reference's own regression recordings data/regression/two_tet_drop_vf.json
+                        and two_tet_drop_ee.json (src/Scenes/TWO_TET_DROP_VF.h, _EE.h), positions
+                        and velocities of every frame before the first self-collision (27 and 36
+                        frames); the meshes two_tets_vertex_face.tobj / two_tets_edge_edge.tobj are
+                        copied next to this script
 
 Usage:  python tests/golden/make_golden.py [--skip-c2]
 """
@@ -40,6 +45,8 @@ import oracle  # noqa: E402
 from hobakv1_amd import scenes  # noqa: E402
 
 C2_STRIDE = 17
+# the reference tree (its data/ folder holds the regression recordings read here)
+REF_ROOT = os.environ.get("HOBAK_REFERENCE_ROOT", "/root/reference")
 
 
 def element_fs(seed=7):
@@ -298,6 +305,23 @@ def make_green(out):
     np.savez_compressed(out, **data)
 
 
+def make_two_tet_drop(out):
+    from hobakv1_amd import formats
+
+    data = {}
+    for tag in ("vf", "ee"):
+        rec = formats.read_scene_json(os.path.join(REF_ROOT, "data", "regression",
+                                                   f"two_tet_drop_{tag}.json"))
+        hit = [k for k, (a, b) in enumerate(zip(rec["vertex_face_collisions"], rec["edge_edge_collisions"]))
+               if a or b]
+        first = hit[0] if hit else len(rec["positions"])
+        data[f"{tag}_positions"] = rec["positions"][:first]
+        data[f"{tag}_velocities"] = rec["velocities"][:first]
+        data[f"{tag}_first_collision_frame"] = np.array(first)
+        print(f"  {tag}: {first} frames before the first self-collision", flush=True)
+    np.savez_compressed(out, **data)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--skip-c2", action="store_true")
@@ -308,7 +332,7 @@ def main():
     jobs = [("element_fixtures.npz", make_element), ("mesh_small.npz", make_mesh_small),
             ("c1_traj.npz", make_c1), ("c1_arap.npz", make_c1_arap), ("c1_bdf1.npz", make_c1_bdf1),
             ("drop_traj.npz", make_drop), ("drop_test_cylinder.npz", make_drop_test),
-            ("green_damping.npz", make_green)]
+            ("green_damping.npz", make_green), ("two_tet_drop.npz", make_two_tet_drop)]
     if not args.skip_c2:
         jobs.append(("c2_traj.npz", make_c2))
     for name, fn in jobs:
