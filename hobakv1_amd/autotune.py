@@ -101,6 +101,7 @@ def main():
     ap.add_argument("--poll", default="8,32")
     ap.add_argument("--update-wgs", default="512")
     ap.add_argument("--spmv-pad", default="0,1")
+    ap.add_argument("--row-sort", default="0", help="vertex relabelling by row length: 0, 1 or 0,1")
     ap.add_argument("--inner-tol", default="1e-4", help="FP32 inner PCG tolerances to sweep")
     ap.add_argument("--workload", default="single", choices=["single", "batch"])
     ap.add_argument("--instances", type=int, default=256, help="batch workload size")
@@ -114,12 +115,13 @@ def main():
     scs = scenes.batch_instances(args.instances) if args.workload == "batch" else None
     results = []
     tols = [float(x) for x in args.inner_tol.split(",")] if args.precision == "fp32" else [0.0]
-    for H, poll, uw, pad, tol in itertools.product([int(x) for x in args.slice_rows.split(",")],
-                                                   [int(x) for x in args.poll.split(",")],
-                                                   [int(x) for x in args.update_wgs.split(",")],
-                                                   [int(x) for x in args.spmv_pad.split(",")], tols):
+    for H, poll, uw, pad, rs, tol in itertools.product([int(x) for x in args.slice_rows.split(",")],
+                                                       [int(x) for x in args.poll.split(",")],
+                                                       [int(x) for x in args.update_wgs.split(",")],
+                                                       [int(x) for x in args.spmv_pad.split(",")],
+                                                       [int(x) for x in args.row_sort.split(",")], tols):
         ctx = lib.Context(0, apply_table=False)
-        ctx.set_tuning(slice_rows=H, poll_chunk=poll, update_wgs=uw, spmv_pad=pad)
+        ctx.set_tuning(slice_rows=H, poll_chunk=poll, update_wgs=uw, spmv_pad=pad, row_sort=rs)
         if scs is not None:
             ctx.refine_inner_tol = tol or 1e-4
             r = measure_batch(ctx, scs, precision, args.steps, args.warmup)
@@ -127,7 +129,8 @@ def main():
             r = measure(ctx, sc, scenes.PRECOND_BLOCK_JACOBI, precision, args.steps, args.warmup,
                         inner_tol=tol or 1e-4)
         ctx.close()
-        r.update({"slice_rows": H, "poll_chunk": poll, "update_wgs": uw, "spmv_pad": pad})
+        r.update({"slice_rows": H, "poll_chunk": poll, "update_wgs": uw, "spmv_pad": pad,
+                  "row_sort": rs})
         if args.precision == "fp32":
             r["refine_inner_tol"] = tol
         results.append(r)
@@ -140,7 +143,7 @@ def main():
         what = (f"{args.instances}-instance batch" if scs is not None else f"{args.n}^3 lattice")
         table[key] = {"slice_rows": best["slice_rows"], "poll_chunk": best["poll_chunk"],
                       "update_wgs": best["update_wgs"], "spmv_pad": best["spmv_pad"],
-                      "measured": {"workload": what, **best}}
+                      "row_sort": best["row_sort"], "measured": {"workload": what, **best}}
         if "refine_inner_tol" in best:
             table[key]["refine_inner_tol"] = best["refine_inner_tol"]
         with open(TABLE, "w") as f:

@@ -135,6 +135,7 @@ TUNE_GRID_CAP = 5
 TUNE_UPDATE_WGS = 6
 TUNE_SPMV_PAD = 7
 TUNE_SPMV_STREAM_MB = 8
+TUNE_ROW_SORT = 9
 
 _lib = None
 
@@ -214,7 +215,8 @@ class Context:
             self.set_tuning(slice_rows=entry.get("slice_rows"), poll_chunk=entry.get("poll_chunk"),
                             max_chunks=entry.get("max_chunks"), graphs=entry.get("graphs"),
                             grid_cap=entry.get("grid_cap"), update_wgs=entry.get("update_wgs"),
-                            spmv_pad=entry.get("spmv_pad"), spmv_stream_mb=entry.get("spmv_stream_mb"))
+                            spmv_pad=entry.get("spmv_pad"), spmv_stream_mb=entry.get("spmv_stream_mb"),
+                            row_sort=entry.get("row_sort"))
             self.refine_inner_tol = entry.get("refine_inner_tol")
 
     def error(self) -> str:
@@ -226,12 +228,14 @@ class Context:
         return rc
 
     def set_tuning(self, slice_rows=None, poll_chunk=None, max_chunks=None, graphs=None,
-                   grid_cap=None, update_wgs=None, spmv_pad=None, spmv_stream_mb=None):
+                   grid_cap=None, update_wgs=None, spmv_pad=None, spmv_stream_mb=None,
+                   row_sort=None):
         """Kernel and engine tuning (see hobak_ctx_set_tuning)."""
         for key, val in ((TUNE_SLICE_ROWS, slice_rows), (TUNE_POLL_CHUNK, poll_chunk),
                          (TUNE_MAX_CHUNKS, max_chunks), (TUNE_GRAPHS, graphs),
                          (TUNE_GRID_CAP, grid_cap), (TUNE_UPDATE_WGS, update_wgs),
-                         (TUNE_SPMV_PAD, spmv_pad), (TUNE_SPMV_STREAM_MB, spmv_stream_mb)):
+                         (TUNE_SPMV_PAD, spmv_pad), (TUNE_SPMV_STREAM_MB, spmv_stream_mb),
+                         (TUNE_ROW_SORT, row_sort)):
             if val is not None:
                 self.check(self.lib.hobak_ctx_set_tuning(self.h, key, int(val)), "set_tuning")
 

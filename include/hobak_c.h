@@ -214,7 +214,13 @@ int hobak_kernel_stats(hobak_inst* inst, double* spmv_avg_ms, int64_t* spmv_coun
  *                          the matrix is still re-read from the cache; above it streaming keeps
  *                          the gathered vectors cached instead).  Decided per group of PCG
  *                          iterations, so a batch switches to cached loads as its instances
- *                          converge; 0 streams always */
+ *                          converge; 0 streams always
+ *   HOBAK_TUNE_ROW_SORT    1: instances created afterwards hold their mesh with the vertices
+ *                          relabelled by descending block-row length (within windows of 2048
+ *                          vertices), so that the rows of a slice have nearly equal length and
+ *                          the slice padding the SpMV streams shrinks; every entry point keeps
+ *                          the caller's vertex numbering.  Row sums are unchanged; the PCG dot
+ *                          products are summed in another order.  0 (default): as given */
 #define HOBAK_TUNE_SLICE_ROWS 0
 #define HOBAK_TUNE_POLL_CHUNK 2
 #define HOBAK_TUNE_MAX_CHUNKS 3
@@ -223,6 +229,7 @@ int hobak_kernel_stats(hobak_inst* inst, double* spmv_avg_ms, int64_t* spmv_coun
 #define HOBAK_TUNE_UPDATE_WGS 6
 #define HOBAK_TUNE_SPMV_PAD 7
 #define HOBAK_TUNE_SPMV_STREAM_MB 8
+#define HOBAK_TUNE_ROW_SORT 9
 int hobak_ctx_set_tuning(hobak_ctx* ctx, int32_t key, int32_t value);
 
 /* Continuous-batching scheduler (no counterpart in the reference, whose callers step one scene
