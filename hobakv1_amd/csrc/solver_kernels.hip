@@ -891,6 +891,23 @@ __global__ __launch_bounds__(kBlock) void cg_spmv_kernel(Batch b, int parity) {
     }
     T po0 = zo0, po1 = zo1, po2 = zo2;
     T beta = T(0);
+    // The first pass's column indices depend only on the slice bounds: they are issued before the
+    // partial-sum reduction below, so its LDS work overlaps their latency.
+    const int nit = wave_live ? (send - sbeg) >> 6 : 0;
+    // per-lane step count: the padded width (PAD), or just the lane's own blocks of its row
+    const int nj = PAD ? nit : (live ? (len - lane / H + (64 / H) - 1) / (64 / H) : 0);
+    // Step j of this lane reads slot sbeg + lane + 64 j.  A step past the lane's own blocks
+    // re-reads step 0's slot (a cache hit, no HBM traffic) and its column values are zeroed, so
+    // every pass issues the same loads without branches.  Each pass also fetches the next
+    // pass's two column indices, so its gathers never wait on a column load.  All addresses
+    // are 32-bit byte offsets from uniform bases (the instance limits keep them in range).
+    const unsigned slot0 = (unsigned)(sbeg + lane);
+    auto slot_of = [&](int j) -> unsigned { return j < nj ? slot0 + 64u * (unsigned)j : slot0; };
+    unsigned ca = 0u, cb = 0u;
+    if (wave_live) {
+      ca = ld_at(col, slot_of(0) * 4u);
+      cb = ld_at(col, slot_of(1) * 4u);
+    }
     if (!first) {
       double rrv, rz;
       pre_sum<true>(prr, prz, nupd, q, rrv, rz);
@@ -931,24 +948,13 @@ __global__ __launch_bounds__(kBlock) void cg_spmv_kernel(Batch b, int parity) {
     // adds the diagonal term), then + beta t_old.
     T s0 = T(0), s1 = T(0), s2 = T(0);
     if (wave_live) {
-      const int nit = (send - sbeg) >> 6;
-      // per-lane step count: the padded width (PAD), or just the lane's own blocks of its row
-      const int nj = PAD ? nit : (live ? (len - lane / H + (64 / H) - 1) / (64 / H) : 0);
       // difference form: columns relative to the row's own entry of the vector multiplied
       const T d0 = kRec ? zo0 : kDiff ? po0 : T(0);
       const T d1 = kRec ? zo1 : kDiff ? po1 : T(0);
       const T d2 = kRec ? zo2 : kDiff ? po2 : T(0);
       const T bt = first ? T(0) : beta;
-      // Step j of this lane reads slot sbeg + lane + 64 j.  A step past the lane's own blocks
-      // re-reads step 0's slot (a cache hit, no HBM traffic) and its column values are zeroed, so
-      // every pass issues the same loads without branches.  Each pass also fetches the next
-      // pass's two column indices, so its gathers never wait on a column load.  All addresses
-      // are 32-bit byte offsets from uniform bases (the instance limits keep them in range).
-      const unsigned slot0 = (unsigned)(sbeg + lane);
-      auto slot_of = [&](int j) -> unsigned { return j < nj ? slot0 + 64u * (unsigned)j : slot0; };
       constexpr unsigned ts = sizeof(T), vs = 3 * sizeof(T);
       const size_t plane = ns * ts;  // bytes between value planes
-      unsigned ca = ld_at(col, slot_of(0) * 4u), cb = ld_at(col, slot_of(1) * 4u);
       for (int j = 0; j < nit; j += 2) {
         const unsigned oa = slot_of(j) * ts, ob = slot_of(j + 1) * ts;
         const unsigned ga = ca * vs, gb = cb * vs;
