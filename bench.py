@@ -185,6 +185,49 @@ def precision_form(lib, sc, precond, args, fp64_total, u64):
     }
 
 
+def c3_errors(lib, scenes, steps):
+    """C3: the 40^3 stretch scene in FP32 + FP64 refinement against the FP64 path, SNH and ARAP
+    (mu = 3.4e4), after every step: the position error, and at the FP64 state the errors of the
+    FP32 forces and of the FP32 stiffness applied to a fixed random vector (normwise relative)."""
+    out = {}
+    rng = np.random.default_rng(12)
+    for mat, name in ((scenes.MAT_SNH, "snh"), (scenes.MAT_ARAP, "arap")):
+        sc = scenes.config3(mat)
+        c64, c32 = lib.Context(0), lib.Context(0, precision="fp32")
+        a = lib.Instance(c64, sc, precond=scenes.PRECOND_BLOCK_JACOBI)
+        b = lib.Instance(c32, sc, precond=scenes.PRECOND_BLOCK_JACOBI, precision=lib.PRECISION_FP32_REFINE)
+        rp, ci = a.bsr_pattern()
+        w = rng.uniform(-1, 1, sc.dofs)
+        X = sc.rest.ravel()
+
+        rows = np.repeat(np.arange(sc.nv), np.diff(rp))
+
+        def kw(K):  # the block matrix applied to w
+            y = np.zeros((sc.nv, 3))
+            np.add.at(y, rows, np.einsum("kab,kb->ka", K, w.reshape(-1, 3)[ci]))
+            return y.ravel()
+
+        pos, frc, kv = [], [], []
+        for _ in range(steps):
+            a.step()
+            b.step()
+            ua, _ = a.state()
+            ub, _ = b.state()
+            pos.append(float(np.linalg.norm(ub - ua) / np.linalg.norm(X + ua)))
+        Ra, Ka = a.forces_hessian(ua)
+        Rb, Kb = b.forces_hessian(ua)
+        frc = float(np.linalg.norm(Rb - Ra) / np.linalg.norm(Ra))
+        ya, yb = kw(Ka), kw(Kb)
+        kv = float(np.linalg.norm(yb - ya) / np.linalg.norm(ya))
+        out[name] = {"rel_position_error_per_step": pos, "max_rel_position_error": max(pos),
+                     "rel_force_error_at_last_state": frc, "rel_stiffness_vector_error_at_last_state": kv}
+        for x in (a, b):
+            x.close()
+        c64.close()
+        c32.close()
+    return out
+
+
 def batch_form(lib, scenes, args):
     """hobak_step_batch over many independent instances against stepping each one alone."""
     scs = scenes.batch_instances(args.batch_instances)
@@ -192,6 +235,7 @@ def batch_form(lib, scenes, args):
     out = {"instances": len(scs), "tets": nt, "steps": args.batch_steps,
            "workload": f"{len(scs)} instances, dims U{{4..24}}^3 Kuhn lattices, SNH/ARAP 50/50, "
                        f"E log-U[1e4,1e6], nu U[0.3,0.48], seeded stretch velocity, block-Jacobi"}
+    final = {}
     for prec, name in ((lib.PRECISION_FP64, "fp64"), (lib.PRECISION_FP32_REFINE, "fp32_refine")):
         ctx = lib.Context(0, precision="fp64" if prec == lib.PRECISION_FP64 else "fp32",
                           workload="batch")
@@ -212,6 +256,7 @@ def batch_form(lib, scenes, args):
             step_s.append(time.perf_counter() - t1)
         tb = time.perf_counter() - t0
         ub = [inst.state()[0] for inst in insts]
+        final[name] = ub
         reset()
         t0 = time.perf_counter()
         for _ in range(args.batch_steps):
@@ -235,6 +280,10 @@ def batch_form(lib, scenes, args):
         if rf:
             rf["note"] = "launches during which every instance was solving (its first inner solve)"
             res["roofline"] = rf
+        if name == "fp32_refine":
+            res["max_rel_position_error_vs_fp64"] = max(
+                float(np.linalg.norm(u32 - u64) / np.linalg.norm(sc.rest.ravel() + u64))
+                for u32, u64, sc in zip(final["fp32_refine"], final["fp64"], scs))
         out[name] = res
         for inst in insts:
             inst.close()
@@ -281,6 +330,12 @@ def beyond_mall_form(lib, scenes, args):
         rf = spmv_roofline(inst.kernel_stats(), prec == lib.PRECISION_FP64, sc.nv)
         out[name] = {"ms_step_sampled": 1e3 * dt, "pcg_iters": st["total_pcg_iters"],
                      "roofline": rf}
+        u_after = inst.state()[0]
+        if prec == lib.PRECISION_FP64:
+            u64 = u_after
+        else:
+            out[name]["rel_position_error_vs_fp64"] = float(
+                np.linalg.norm(u_after - u64) / np.linalg.norm(sc.rest.ravel() + u64))
         inst.close()
         ctx.close()
     return out
@@ -422,6 +477,7 @@ def main():
     }
     if args.precision_form == "on":
         out["fp32_refine"] = precision_form(lib, sc, precond, args, total, u)
+        out["fp32_refine"]["c3_errors_vs_fp64"] = c3_errors(lib, scenes, args.steps)
     if args.batch == "on":
         out["batch"] = batch_form(lib, scenes, args)
     if args.serving == "on":
