@@ -112,7 +112,7 @@ def test_c1_arap_trajectory_matches_reference():
 
 
 def test_c2_first_steps_match_reference():
-    """40^3 stretch scene: the first 2 steps (the full 20 run on the GPU box)."""
+    """40^3 stretch scene: the first 2 steps (the GPU parity tests check all 20)."""
     g = golden("c2_traj.npz")
     sc = scenes.config2()
     _, out = _run_oracle(sc, 2, scenes.PRECOND_JACOBI)
