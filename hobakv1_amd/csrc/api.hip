@@ -19,7 +19,9 @@
 //   2. newly started and continuing instances run the prologue (element, assembly, right-hand
 //      side, PCG / refinement seed) as one segmented launch per kernel over just those instances;
 //   3. groups of PCG iterations run over the scheduler's whole member set (a captured graph per
-//      member set); kernels of instances that are not solving return at once;
+//      member set) while every member is solving, and over a table of the solving members only
+//      (direct launches) once some have finished (compact_pct); kernels of instances that are
+//      not solving return at once;
 //   4. one status kernel and one small copy, then the host waits and reads the statuses.
 // Requests submitted while others are on the GPU join at the next tick: continuous batching.
 #include <hip/hip_runtime.h>
@@ -60,6 +62,8 @@ struct hobak_ctx {
   int spmv_pad = 0;                    // SpMV loop over padded slice widths (1) or row lengths (0)
   int spmv_stream_mb = 384;            // non-temporal matrix loads above this SpMV size (MiB)
   int row_sort = 0;                    // relabel new instances' vertices by descending row length
+  int compact_pct = 100;               // PCG groups launch over the solving instances only below this
+  int update_groups = 2;               // 21-row groups per PCG update wave
   hipStream_t stream = nullptr;
   size_t budget = 0;
   size_t in_use = 0;
@@ -260,6 +264,7 @@ struct hobak_inst {
   hobak_step_params prm{};
   double lam_eff = 0, alpha = 0;
   int tsize = 8;  // bytes of the element/assembly/PCG real type
+  int groups = 1; // 21-row groups per PCG update wave
   Pattern P;
   std::vector<int> pinned_ids;
   // Vertex relabelling (row_sort): the device holds the mesh with vertex p = the caller's vertex
@@ -427,7 +432,7 @@ size_t al(size_t x) { return (x + kAlign - 1) & ~(kAlign - 1); }
 
 struct Plan {
   std::vector<hobak_inst*> members;
-  int n = 0, tsize = 8, shift = 5, refine = 0;
+  int n = 0, tsize = 8, shift = 5, refine = 0, groups = 1;
   bool owned = false;
   char* dev = nullptr;
   char* host = nullptr;
@@ -491,7 +496,7 @@ int plan_fill(hobak_ctx* ctx, Plan& P) {
 int plan_check_members(hobak_ctx* ctx, const std::vector<hobak_inst*>& m) {
   if (m.empty()) return fail(ctx, HOBAK_E_INVALID, "empty batch");
   for (auto* I : m)
-    if (I->tsize != m[0]->tsize || I->P.shift != m[0]->P.shift)
+    if (I->tsize != m[0]->tsize || I->P.shift != m[0]->P.shift || I->groups != m[0]->groups)
       return fail(ctx, HOBAK_E_INVALID, "instances of one batch must share precision and slice height");
   return HOBAK_OK;
 }
@@ -519,6 +524,7 @@ int plan_transient(hobak_ctx* ctx, const std::vector<hobak_inst*>& m, Plan& P) {
   P.members = m;
   P.n = (int)m.size();
   P.tsize = m[0]->tsize;
+  P.groups = m[0]->groups;
   P.shift = m[0]->P.shift;
   P.refine = m[0]->h.refine;
   P.dev = ctx->arena_dev + ctx->arena_off;
@@ -567,6 +573,7 @@ int plan_get(hobak_ctx* ctx, const std::vector<hobak_inst*>& m, Plan** out) {
   P->members = m;
   P->n = (int)m.size();
   P->tsize = m[0]->tsize;
+  P->groups = m[0]->groups;
   P->shift = m[0]->P.shift;
   P->refine = m[0]->h.refine;
   P->owned = true;
@@ -677,7 +684,7 @@ int launch_chunk(hobak_ctx* ctx, const Plan& P, int k, int stream, hobak_sched* 
       }
     }
     HIPCHK(ctx, launch_cg_spmv<T>(P.b[K_SPMV], P.shift, ctx->spmv_pad, stream, j & 1, cap, s, a, b));
-    HIPCHK(ctx, launch_cg_update<T>(P.b[K_UPD], j & 1, cap, s));
+    HIPCHK(ctx, launch_cg_update<T>(P.b[K_UPD], P.groups, j & 1, cap, s));
   }
   if (P.refine) {
     HIPCHK(ctx, launch_refine_residual<T>(P.b[K_SPMV], P.b[K_DOF], P.shift, cap, s));
@@ -720,7 +727,7 @@ template <typename T>
 int tick_impl(hobak_sched* S) {
   hobak_ctx* ctx = S->ctx;
   hipStream_t s = ctx->stream;
-  TRY(arena_ensure(ctx, 4 * plan_bytes(S->members.size() + 1)));
+  TRY(arena_ensure(ctx, 5 * plan_bytes(S->members.size() + 1)));
   ctx->arena_off = 0;
   TRY(plan_get(ctx, S->members, &S->mplan));
   if (S->dirty) {
@@ -830,17 +837,24 @@ int tick_impl(hobak_sched* S) {
       }
   }
 
-  // 4. groups of PCG iterations over the member set: as many as the solving instance closest to
-  //    finishing is predicted to need (from its last step), at least one
+  // 4. groups of PCG iterations: as many as the solving instance closest to finishing is
+  //    predicted to need (from its last step), at least one.  They run over the member set (one
+  //    captured graph) while the solving instances hold more than compact_pct percent of its SpMV
+  //    workgroups; below that, over a table of the solving instances only, launched directly, so
+  //    that the workgroups of finished instances are not dispatched just to return.
   bool ran = false;
   int m = ctx->max_chunks;
   int nsolving = 0;
   double solving_bytes = 0.0;
+  long solving_wgs = 0;
+  std::vector<hobak_inst*> solving;
   const int k = ctx->chunk_iters;
   for (auto& r : S->active) {
     if (r.stage != 1) continue;
     nsolving++;
     solving_bytes += spmv_bytes_of(r.I);
+    solving_wgs += r.I->h.nb[K_SPMV];
+    solving.push_back(r.I);
     const int p = r.I->pred[r.newton_k];
     const int q = p > 0 ? (p - r.done_iters) / k : 0;
     m = std::min(m, std::max(1, q));
@@ -855,7 +869,10 @@ int tick_impl(hobak_sched* S) {
       every = S->spmv_every;
     }
     const int st = stream_matrix(ctx, solving_bytes) ? 1 : 0;
-    if (graphs && (!M.chunk_exec[st] || M.chunk_k[st] != k)) {
+    const bool compact = nsolving < M.n && solving_wgs * 100 <= (long)ctx->compact_pct * M.b[K_SPMV].total;
+    Plan PG;
+    if (compact) TRY(plan_transient(ctx, solving, PG));
+    if (graphs && !compact && (!M.chunk_exec[st] || M.chunk_k[st] != k)) {
       if (M.chunk_exec[st]) (void)hipGraphExecDestroy(M.chunk_exec[st]);
       M.chunk_exec[st] = nullptr;
       TRY(capture(ctx, &M.chunk_exec[st], [&]() { return launch_chunk<T>(ctx, M, k, st, nullptr, 0); }));
@@ -867,11 +884,11 @@ int tick_impl(hobak_sched* S) {
       const int g0 = S->sample_it;
       const bool sampled = every > 0 && (int)S->samples.size() < kMaxSamples &&
                            (g0 / every) * every + every - 1 < g0 + k;
-      if (graphs && !sampled) {
+      if (graphs && !compact && !sampled) {
         HIPCHK(ctx, hipGraphLaunch(M.chunk_exec[st], s));
         if (S->sampling) S->sample_it += k;
       } else {
-        TRY(launch_chunk<T>(ctx, M, k, st, S->sampling ? S : nullptr, every));
+        TRY(launch_chunk<T>(ctx, compact ? PG : M, k, st, S->sampling ? S : nullptr, every));
       }
     }
     HIPCHK(ctx, hipEventRecord(ctx->ev[5], s));
@@ -1309,6 +1326,7 @@ int hobak_inst_create(hobak_ctx* ctx, const double* rest_xyz, int32_t nv, const 
     I->alpha = 0.0;
   }
   I->tsize = params->precision == HOBAK_PRECISION_FP64 ? 8 : 4;
+  I->groups = ctx->update_groups;
   build_pattern(nv, nt, tets, ctx->slice_rows, I->P, perm.empty() ? nullptr : &perm);
   I->perm = std::move(perm);
   if (!I->perm.empty()) {
@@ -1332,7 +1350,7 @@ int hobak_inst_create(hobak_ctx* ctx, const double* rest_xyz, int32_t nv, const 
   h.nt = nt;
   h.nslots = P.nslots;
   h.shift = P.shift;
-  for (int k = 0; k < K_COUNT; k++) h.nb[k] = blocks_of(k, nv, nt, P.nslots, P.shift, ctx->update_cap);
+  for (int k = 0; k < K_COUNT; k++) h.nb[k] = blocks_of(k, nv, nt, P.nslots, P.shift, ctx->update_cap, I->groups);
   h.material = material->kind;
   h.precond = params->precond;
   h.bdf = I->prm.newton_iters > 1;
@@ -1818,6 +1836,17 @@ int hobak_ctx_set_tuning(hobak_ctx* ctx, int32_t key, int32_t value) {
     case HOBAK_TUNE_ROW_SORT:
       if (value != 0 && value != 1) return fail(ctx, HOBAK_E_INVALID, "row sort is 0 or 1");
       ctx->row_sort = value;
+      return HOBAK_OK;
+    case HOBAK_TUNE_COMPACT_PCT:
+      if (value < 0 || value > 100) return fail(ctx, HOBAK_E_INVALID, "compact percentage must be 0..100");
+      ctx->compact_pct = value;
+      return HOBAK_OK;
+    case HOBAK_TUNE_UPDATE_GROUPS:
+      if (value != 1 && value != 2 && value != 4)
+        return fail(ctx, HOBAK_E_INVALID, "update groups must be 1, 2 or 4");
+      if (!ctx->insts.empty() && value != ctx->update_groups)
+        return fail(ctx, HOBAK_E_INVALID, "the update groups are fixed once instances exist");
+      ctx->update_groups = value;
       return HOBAK_OK;
     case HOBAK_TUNE_UPDATE_WGS:
       if (value < 1 || value > 65536) return fail(ctx, HOBAK_E_INVALID, "update workgroups out of range");

@@ -237,7 +237,7 @@ __device__ __forceinline__ HOBAK_GLOBAL T* gbl(T* p) {
 }
 
 // Workgroups per kind for an instance (host side; the same formulas size the partial arrays).
-int blocks_of(int kind, int nv, int nt, int nslots, int shift, int update_cap);
+int blocks_of(int kind, int nv, int nt, int nslots, int shift, int update_cap, int update_groups);
 
 // Launchers.  `cap` limits the launch grid (0: one workgroup per virtual workgroup).
 enum ElemSrc { ELEM_U = 0, ELEM_TMP = 1 };
@@ -276,7 +276,7 @@ template <typename T>
 hipError_t launch_cg_spmv(const Batch& b, int shift, int pad, int stream, int parity, int cap, hipStream_t s,
                           hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
 template <typename T>
-hipError_t launch_cg_update(const Batch& b, int parity, int cap, hipStream_t s);
+hipError_t launch_cg_update(const Batch& b, int groups, int parity, int cap, hipStream_t s);
 hipError_t launch_refine_begin(const Batch& b, int cap, hipStream_t s);
 template <typename T>
 hipError_t launch_refine_init(const Batch& b, int gate, int cap, hipStream_t s);

@@ -1038,12 +1038,16 @@ __global__ __launch_bounds__(kBlock) void cg_spmv_kernel(Batch b, int parity) {
   }
 }
 
-// One lane per DOF, 21 rows (63 lanes) per wavefront so that a row's three DOFs share a wave and
-// the 3x3 preconditioner apply can read its siblings' residuals with cross-lane moves.  Each
-// instance's workgroups stride over its waves (at most a tuned cap per instance), which
-// keeps the partial-sum count (and the work of every workgroup that re-reduces them) small.  The
-// first wave's vectors are loaded before the p.t partials are reduced, so the two overlap.
-template <typename T>
+// One lane per DOF of each of R groups of 21 rows (63 lanes) per wavefront, so that a row's
+// three DOFs share a wave and the 3x3 preconditioner apply can read its siblings' residuals with
+// cross-lane moves.  A wave covers 21 R consecutive rows and issues the loads of all R groups
+// (the preconditioner's included) before it uses any, so each wave keeps R times the bytes in
+// flight and a launch needs R times fewer waves.  Each instance's workgroups stride over its
+// waves (at most a tuned cap per instance), which keeps the partial-sum count (and the work of
+// every workgroup that re-reduces them) small.  The first wave's loads are issued before the
+// p.t partials are reduced, so the two overlap.  A lane sums r.r and r.z over its groups in
+// order, so the sums do not depend on the batch.
+template <typename T, int R>
 __global__ __launch_bounds__(kBlock) void cg_update_kernel(Batch b, int parity) {
   for (int ws = blockIdx.x; ws < b.total; ws += gridDim.x) {
     const int vb = xcd_vb(ws, b.total);
@@ -1056,7 +1060,7 @@ __global__ __launch_bounds__(kBlock) void cg_update_kernel(Batch b, int parity) 
     const int c = lane - 3 * (lane / 3);
     const int bse = lane - c;
     const int nrows = I->nv;
-    const int nwaves = (nrows + 20) / 21;
+    const int nwaves = (nrows + 21 * R - 1) / (21 * R);
     const int stride = lg * (kBlock / 64);
     auto p = gbl((const T*)(parity ? I->p0 : I->p1));  // the p_new this iteration's SpMV wrote
     auto mi = gbl((const T*)I->minv);
@@ -1066,17 +1070,30 @@ __global__ __launch_bounds__(kBlock) void cg_update_kernel(Batch b, int parity) 
     auto t = gbl((const T*)I->t);
     const int precond = I->precond;
     const int w0 = (int)((lb * (unsigned)kBlock + threadIdx.x) >> 6);
-    T pv = T(0), tv = T(0), rv0 = T(0), xv = T(0);
-    {
-      const int row = w0 * 21 + lane / 3;
-      if (w0 < nwaves && lane < 63 && row < nrows) {
-        const size_t i = 3 * (size_t)row + c;
-        pv = p[i];
-        tv = t[i];
-        rv0 = rv_[i];
-        xv = x[i];
+    T pv[R], tv[R], rv0[R], xv[R], m0[R], m1[R], m2[R];
+    auto load = [&](int wave) {
+#pragma unroll
+      for (int q = 0; q < R; q++) {
+        const int row = (wave * R + q) * 21 + lane / 3;
+        pv[q] = tv[q] = rv0[q] = xv[q] = m0[q] = m1[q] = m2[q] = T(0);
+        if (lane < 63 && row < nrows) {
+          const size_t i = 3 * (size_t)row + c;
+          pv[q] = p[i];
+          tv[q] = t[i];
+          rv0[q] = rv_[i];
+          xv[q] = x[i];
+          if (precond == 0) {
+            m0[q] = mi[i];
+          } else {
+            auto m = mi + (size_t)kMinvStride * row + 3 * c;
+            m0[q] = m[0];
+            m1[q] = m[1];
+            m2[q] = m[2];
+          }
+        }
       }
-    }
+    };
+    if (w0 < nwaves) load(w0);
     PreParts qp;
     pre_load<false>(gbl(I->part_pt), nullptr, I->nb[K_SPMV], qp);
     double pt, unused;
@@ -1084,35 +1101,27 @@ __global__ __launch_bounds__(kBlock) void cg_update_kernel(Batch b, int parity) 
     const T alpha = (T)(absNew / pt);
     double rr2 = 0.0, rz = 0.0;
     for (int wave = w0; wave < nwaves; wave += stride) {
-      const int row = wave * 21 + lane / 3;
-      const bool live = lane < 63 && row < nrows;
-      const size_t i = 3 * (size_t)row + c;
-      if (wave != w0 && live) {
-        pv = p[i];
-        tv = t[i];
-        rv0 = rv_[i];
-        xv = x[i];
-      }
-      T rv = T(0);
-      if (live) {
-        x[i] = xv + alpha * pv;
-        rv = rv0 - alpha * tv;
-        rv_[i] = rv;
-      }
-      const T r0 = __shfl(rv, bse, 64);
-      const T r1 = __shfl(rv, bse + 1, 64);
-      const T r2 = __shfl(rv, bse + 2, 64);
-      if (live) {
-        T zv;
-        if (precond == 0) {
-          zv = mi[i] * rv;
-        } else {
-          auto m = mi + (size_t)kMinvStride * row + 3 * c;
-          zv = m[0] * r0 + m[1] * r1 + m[2] * r2;
+      if (wave != w0) load(wave);
+#pragma unroll
+      for (int q = 0; q < R; q++) {
+        const int row = (wave * R + q) * 21 + lane / 3;
+        const bool live = lane < 63 && row < nrows;
+        const size_t i = 3 * (size_t)row + c;
+        T rv = T(0);
+        if (live) {
+          x[i] = xv[q] + alpha * pv[q];
+          rv = rv0[q] - alpha * tv[q];
+          rv_[i] = rv;
         }
-        z[i] = zv;
-        rr2 += (double)rv * (double)rv;
-        rz += (double)rv * (double)zv;
+        const T r0 = __shfl(rv, bse, 64);
+        const T r1 = __shfl(rv, bse + 1, 64);
+        const T r2 = __shfl(rv, bse + 2, 64);
+        if (live) {
+          const T zv = precond == 0 ? m0[q] * rv : m0[q] * r0 + m1[q] * r1 + m2[q] * r2;
+          z[i] = zv;
+          rr2 += (double)rv * (double)rv;
+          rz += (double)rv * (double)zv;
+        }
       }
     }
     publish(rr2, rz, gbl(I->part_rr), gbl(I->part_rz), lb);
@@ -1389,7 +1398,7 @@ __global__ void copy_kernel(const double4* __restrict__ src, double4* __restrict
 // ---------------------------------------------------------------------------------------------
 static inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 
-int blocks_of(int kind, int nv, int nt, int nslots, int shift, int update_cap) {
+int blocks_of(int kind, int nv, int nt, int nslots, int shift, int update_cap, int update_groups) {
   switch (kind) {
     case K_ELEM: return cdiv(nt, kElementBlock);
     case K_SLOT: return cdiv(nslots, kBlock);
@@ -1397,7 +1406,7 @@ int blocks_of(int kind, int nv, int nt, int nslots, int shift, int update_cap) {
     case K_DOF: return cdiv(3L * nv, kBlock);
     case K_SPMV: return cdiv((long)cdiv(nv, 1 << shift) * 64, kBlock);
     case K_UPD: {
-      const int g = cdiv((long)cdiv(nv, 21) * 64, kBlock);
+      const int g = cdiv((long)cdiv(nv, 21 * update_groups) * 64, kBlock);
       return g < update_cap ? g : update_cap;
     }
     case K_ONE: return 1;
@@ -1475,8 +1484,13 @@ hipError_t launch_cg_spmv(const Batch& b, int shift, int pad, int stream, int pa
 }
 
 template <typename T>
-hipError_t launch_cg_update(const Batch& b, int parity, int cap, hipStream_t s) {
-  SEG_LAUNCH((cg_update_kernel<T>), b, cap, s, parity);
+hipError_t launch_cg_update(const Batch& b, int groups, int parity, int cap, hipStream_t s) {
+  switch (groups) {
+    case 1: SEG_LAUNCH((cg_update_kernel<T, 1>), b, cap, s, parity);
+    case 2: SEG_LAUNCH((cg_update_kernel<T, 2>), b, cap, s, parity);
+    case 4: SEG_LAUNCH((cg_update_kernel<T, 4>), b, cap, s, parity);
+    default: return hipErrorInvalidValue;
+  }
 }
 hipError_t launch_refine_begin(const Batch& b, int cap, hipStream_t s) { SEG_LAUNCH(refine_begin_kernel, b, cap, s); }
 template <typename T>
@@ -1519,7 +1533,7 @@ hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s) 
   template hipError_t launch_constant_parts<T>(const Batch&, int, hipStream_t);              \
   template hipError_t launch_rhs<T>(const Batch&, int, int, hipStream_t);                    \
   template hipError_t launch_cg_spmv<T>(const Batch&, int, int, int, int, int, hipStream_t, hipEvent_t, hipEvent_t); \
-  template hipError_t launch_cg_update<T>(const Batch&, int, int, hipStream_t);              \
+  template hipError_t launch_cg_update<T>(const Batch&, int, int, int, hipStream_t);         \
   template hipError_t launch_refine_init<T>(const Batch&, int, int, hipStream_t);            \
   template hipError_t launch_refine_residual<T>(const Batch&, const Batch&, int, int, hipStream_t); \
   template hipError_t launch_force_gather<T>(const Batch&, int, hipStream_t);

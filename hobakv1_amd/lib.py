@@ -136,6 +136,8 @@ TUNE_UPDATE_WGS = 6
 TUNE_SPMV_PAD = 7
 TUNE_SPMV_STREAM_MB = 8
 TUNE_ROW_SORT = 9
+TUNE_COMPACT_PCT = 10
+TUNE_UPDATE_GROUPS = 11
 
 _lib = None
 
@@ -216,7 +218,8 @@ class Context:
                             max_chunks=entry.get("max_chunks"), graphs=entry.get("graphs"),
                             grid_cap=entry.get("grid_cap"), update_wgs=entry.get("update_wgs"),
                             spmv_pad=entry.get("spmv_pad"), spmv_stream_mb=entry.get("spmv_stream_mb"),
-                            row_sort=entry.get("row_sort"))
+                            row_sort=entry.get("row_sort"), compact_pct=entry.get("compact_pct"),
+                            update_groups=entry.get("update_groups"))
             self.refine_inner_tol = entry.get("refine_inner_tol")
 
     def error(self) -> str:
@@ -229,13 +232,14 @@ class Context:
 
     def set_tuning(self, slice_rows=None, poll_chunk=None, max_chunks=None, graphs=None,
                    grid_cap=None, update_wgs=None, spmv_pad=None, spmv_stream_mb=None,
-                   row_sort=None):
+                   row_sort=None, compact_pct=None, update_groups=None):
         """Kernel and engine tuning (see hobak_ctx_set_tuning)."""
         for key, val in ((TUNE_SLICE_ROWS, slice_rows), (TUNE_POLL_CHUNK, poll_chunk),
                          (TUNE_MAX_CHUNKS, max_chunks), (TUNE_GRAPHS, graphs),
                          (TUNE_GRID_CAP, grid_cap), (TUNE_UPDATE_WGS, update_wgs),
                          (TUNE_SPMV_PAD, spmv_pad), (TUNE_SPMV_STREAM_MB, spmv_stream_mb),
-                         (TUNE_ROW_SORT, row_sort)):
+                         (TUNE_ROW_SORT, row_sort), (TUNE_COMPACT_PCT, compact_pct),
+                         (TUNE_UPDATE_GROUPS, update_groups)):
             if val is not None:
                 self.check(self.lib.hobak_ctx_set_tuning(self.h, key, int(val)), "set_tuning")
 

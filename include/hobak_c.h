@@ -220,7 +220,18 @@ int hobak_kernel_stats(hobak_inst* inst, double* spmv_avg_ms, int64_t* spmv_coun
  *                          vertices), so that the rows of a slice have nearly equal length and
  *                          the slice padding the SpMV streams shrinks; every entry point keeps
  *                          the caller's vertex numbering.  Row sums are unchanged; the PCG dot
- *                          products are summed in another order.  0 (default): as given */
+ *                          products are summed in another order.  0 (default): as given
+ *   HOBAK_TUNE_COMPACT_PCT 0..100: a batch's PCG groups run over all of its instances (one
+ *                          captured graph) while the instances still solving hold more than this
+ *                          percentage of its SpMV workgroups, and over those instances only
+ *                          (launched directly, no graph) at or below it, so that finished
+ *                          instances' workgroups are not dispatched.  Results do not depend on
+ *                          it.  Default 100: compact as soon as one instance has finished;
+ *                          0 never compacts
+ *   HOBAK_TUNE_UPDATE_GROUPS  groups of 21 rows that one wave of the PCG update kernel covers
+ *                          (it issues all their loads before using any): 1, 2 (default) or 4;
+ *                          fixed once the context holds instances.  The r.r and r.z partial
+ *                          sums follow the wave layout, so FP64 results differ in the last bits */
 #define HOBAK_TUNE_SLICE_ROWS 0
 #define HOBAK_TUNE_POLL_CHUNK 2
 #define HOBAK_TUNE_MAX_CHUNKS 3
@@ -230,6 +241,8 @@ int hobak_kernel_stats(hobak_inst* inst, double* spmv_avg_ms, int64_t* spmv_coun
 #define HOBAK_TUNE_SPMV_PAD 7
 #define HOBAK_TUNE_SPMV_STREAM_MB 8
 #define HOBAK_TUNE_ROW_SORT 9
+#define HOBAK_TUNE_COMPACT_PCT 10
+#define HOBAK_TUNE_UPDATE_GROUPS 11
 int hobak_ctx_set_tuning(hobak_ctx* ctx, int32_t key, int32_t value);
 
 /* Continuous-batching scheduler (no counterpart in the reference, whose callers step one scene
@@ -237,9 +250,10 @@ int hobak_ctx_set_tuning(hobak_ctx* ctx, int32_t key, int32_t value);
  * A request advances one instance by one step.  Submitting attaches the instance to the
  * scheduler (its state stays resident in HBM); every tick finishes the solves that ended, starts
  * the requests that have arrived since (their prologue kernels run as one segmented launch), runs
- * a group of PCG iterations over every attached instance with one captured graph (instances not
- * solving return at once) and polls.  Requests that arrive while others are on the GPU thus join
- * the running batch at the next tick.  Requests of one instance run in submission order. */
+ * groups of PCG iterations over the instances still solving (one captured graph over every
+ * attached instance while all of them solve; see HOBAK_TUNE_COMPACT_PCT) and polls.  Requests
+ * that arrive while others are on the GPU thus join the running batch at the next tick.
+ * Requests of one instance run in submission order. */
 typedef struct {
   hobak_inst* inst;
   uint64_t tag;

@@ -106,6 +106,58 @@ def test_streamed_matrix_loads_match(precision):
         assert np.array_equal(sa[0], sb[0]) and np.array_equal(sa[1], sb[1])
 
 
+@pytest.mark.parametrize("precision", [lib.PRECISION_FP64, lib.PRECISION_FP32_REFINE])
+def test_compacted_groups_match(precision):
+    """PCG groups over only the still-solving instances (HOBAK_TUNE_COMPACT_PCT) against groups
+    over the whole batch: bitwise the same trajectory, for never, sometimes and always compacting.
+    The instances differ in size, so they finish their solves at different ticks."""
+    scs = _mixed_scenes(k=6, seed=26)
+    res = []
+    for pct in (0, 40, 100):
+        ctx = lib.Context(0, precision="fp64" if precision == lib.PRECISION_FP64 else "fp32")
+        ctx.set_tuning(compact_pct=pct, poll_chunk=4)
+        _say(f"compact at {pct}%, precision {precision}")
+        res.append(_run_batched(ctx, scs, 3, precision))
+        ctx.close()
+    for r in res[1:]:
+        for (sa, ia), (sb, ib) in zip(res[0], r):
+            assert ia == ib
+            assert np.array_equal(sa[0], sb[0]) and np.array_equal(sa[1], sb[1])
+
+
+@pytest.mark.parametrize("groups", [1, 4])
+def test_update_groups_match_oracle(groups):
+    """The PCG update kernel with 1 and 4 groups of 21 rows per wave (the default is 2): C1 for
+    20 steps against the oracle, iterations +-1 and positions within 1e-9; and a batch of
+    instances in that layout equals each instance alone, bitwise."""
+    ctx = lib.Context(0)
+    ctx.set_tuning(update_groups=groups)
+    sc = scenes.config1()
+    _say(f"update groups {groups}: C1, 20 steps against the oracle")
+    inst = lib.Instance(ctx, sc)
+    o = oracle.Oracle(sc, precond=scenes.PRECOND_BLOCK_JACOBI)
+    for k in range(20):
+        st = inst.step()
+        it, _, _ = o.step()
+        assert abs(st["pcg_iters"][0] - it) <= 1, k
+    u, v = inst.state()
+    uo, vo = o.state()
+    assert rel(u, uo) <= 1e-9 and rel(v, vo) <= 1e-9
+    with pytest.raises(lib.HobakError):  # fixed while the context holds instances
+        ctx.set_tuning(update_groups=2)
+    inst.close()
+    o.close()
+    scs = _mixed_scenes(k=3, seed=27)
+    alone = _run_alone(ctx, scs, 2, lib.PRECISION_FP64)
+    batched = _run_batched(ctx, scs, 2, lib.PRECISION_FP64)
+    for (sa, ia), (sb, ib) in zip(alone, batched):
+        assert ia == ib
+        assert np.array_equal(sa[0], sb[0]) and np.array_equal(sa[1], sb[1])
+    with pytest.raises(lib.HobakError):
+        ctx.set_tuning(update_groups=3)
+    ctx.close()
+
+
 def test_batch_fp64_matches_oracle(gpu_ctx):
     scs = _mixed_scenes(k=4, seed=24)
     insts = [lib.Instance(gpu_ctx, sc) for sc in scs]
