@@ -8,9 +8,9 @@ The reference has no serving layer; its caller steps one scene per SIMULATION_SC
 freeing its memory.
 
     batched  hobak_sched: every tick finishes the solves that ended, starts the requests that
-             arrived since (one segmented launch per prologue kernel over just those), runs a
-             group of PCG iterations over every resident instance from one captured graph
-             (instances not solving return at once), and polls
+             arrived since (one segmented launch per prologue kernel over just those), runs
+             groups of PCG iterations over the instances still solving (one captured graph
+             over every resident instance while all of them solve), and polls
     plain    the same stream served one hobak_step at a time, round-robin over admitted
              sessions (no batching): the plain FP64 GPU path
 
