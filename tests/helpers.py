@@ -52,3 +52,43 @@ def bsr_entry_rel(K, Kref, row_ptr, floor=1e-3):
         d = np.abs(K[row_ptr[i]:row_ptr[i + 1]] - blk) / scale
         worst = max(worst, float(d.max()))
     return worst
+
+
+def penalty_ground_case(seed=3):
+    """A scene whose step has a closed form, for the penalty ground plane psi = k/2 (y_g - y)^2
+    below y_g (which has no reference counterpart).  ARAP with mu = 0 has P = 2 mu (F - R) = 0
+    and a zero clamped Hessian, so the elastic forces, the stiffness and the Rayleigh stiffness
+    term vanish exactly; M is diagonal and C = alpha M.  Each y DOF then solves its own equation
+        (m/dt^2 - alpha m/dt + k [y < y_g]) dy = k (y_g - y) [y < y_g] + m g_y
+    with y = X_y + u_y the displaced position (v = 0), and x, z do not move.  The initial
+    displacement puts some vertices below y_g that are above it at rest.
+    Returns (scene, expected du)."""
+    from hobakv1_amd import scenes
+
+    sc = scenes.lattice_scene((4, 3, 3), seed=seed, material=scenes.MAT_ARAP, E=0.0, nu=0.3,
+                              pin=False)
+    rng = np.random.default_rng(seed)
+    u0 = np.zeros((sc.nv, 3))
+    u0[:, 1] = rng.uniform(-0.08, 0.08, sc.nv)
+    sc.u0 = u0.ravel()
+    sc.ground_y = 0.45
+    sc.ground_k = 1e6
+    sc.pcg_tol = 1e-13
+    X = sc.rest
+    T = sc.tets
+    d = np.einsum("ij,ij->i", X[T[:, 3]] - X[T[:, 0]],
+                  np.cross(X[T[:, 1]] - X[T[:, 0]], X[T[:, 2]] - X[T[:, 0]]))
+    vol = np.abs(d) / 6.0
+    ring = np.zeros(sc.nv)
+    for c in range(4):
+        np.add.at(ring, T[:, c], vol / 4.0)
+    m = sc.mass_scale * ring
+    g = sc.gravity[1] / sc.mass_scale
+    y = X[:, 1] + u0[:, 1]
+    below = y < sc.ground_y
+    dt, a, k = sc.dt, sc.rayleigh_alpha, sc.ground_k
+    dy = (np.where(below, k * (sc.ground_y - y), 0.0) + m * g) / (m / dt ** 2 - a * m / dt + np.where(below, k, 0.0))
+    du = np.zeros((sc.nv, 3))
+    du[:, 1] = dy
+    assert 0 < below.sum() < sc.nv and (below & (X[:, 1] >= sc.ground_y)).any()
+    return sc, du.ravel()

@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 import oracle
-from helpers import rel
+from helpers import penalty_ground_case, rel
 from hobakv1_amd import scenes
 
 SNH, ARAP = scenes.MAT_SNH, scenes.MAT_ARAP
@@ -231,3 +231,16 @@ def test_pcg_iteration_count_semantics():
     assert it == 0 and err == 0.0
     u, v = o.state()
     assert np.all(u == 0) and np.all(v == 0)
+
+
+def test_penalty_ground_closed_form():
+    """The penalty ground plane against its closed form (helpers.penalty_ground_case): a wrong
+    sign of k, the force on the wrong component, or the rest position in place of the displaced
+    one each fail it."""
+    sc, du = penalty_ground_case()
+    o = oracle.Oracle(sc, precond=scenes.PRECOND_BLOCK_JACOBI)
+    o.step()
+    u, v = o.state()
+    assert rel(u - sc.u0, du) < 1e-12
+    assert rel(v, du / sc.dt) < 1e-12
+    o.close()
