@@ -6,7 +6,12 @@
 // operations where that order changes round-off (citations are relative to the reference root).
 //
 // Pinned by: the literal reference's outputs (tests/golden/*, made by tests/golden/make_golden.py
-// from oracle/_ref), and by closed-form / invariant checks in tests/test_oracle_*.py.
+// from oracle/_ref), and by closed-form / invariant checks in tests/test_oracle_*.py.  The two
+// parts the reference cannot express are pinned without it:
+//   - block-Jacobi PCG: an independent textbook PCG in numpy that first reproduces the
+//     reference's Jacobi iteration counts (tests/test_oracle_pcg.py);
+//   - the penalty ground plane: the closed-form step of a mesh with zero elastic stiffness
+//     (tests/test_oracle_math.py::test_penalty_ground_closed_form).
 #include "hobak_oracle.h"
 
 #include <omp.h>
