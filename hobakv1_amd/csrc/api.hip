@@ -88,7 +88,8 @@ struct Pattern {
   int H = 16, shift = 4;  // rows per slice and its log2
   std::vector<int> row_ptr, col_idx;  // BSR
   std::vector<int> slice_off, row_len, col, slot_row, bsr_slot, diag_slot;
-  std::vector<int> cptr, hdest;       // per slot contribution ranges; per (tet, x, y) position
+  std::vector<int> cptr, hdest;       // per slot contribution ranges; per (tet, x <= y) position
+  std::vector<int> mirror;            // per upper off-diagonal slot: its transposed slot
   std::vector<int> vptr, fdest;       // per vertex force ranges; per (tet, x) position
 };
 
@@ -182,24 +183,36 @@ void build_pattern(int nv, int nt, const int32_t* tets, int H, Pattern& P,
       }
     }
   }
-  // contributions: block (x, y) of tet t sums into slot (tets[t][x], tets[t][y]).  Walking row i's
-  // incident tets in ascending order visits every contribution of row i's slots in ascending tet
-  // order, which is the order of the reference's gather lists (TET_MESH_FASTER.cpp:213-236).
+  // contributions: block (x, y) of tet t sums into slot (tets[t][x], tets[t][y]).  Only the upper
+  // slots (row <= column) sum their contributions; a lower slot is the transpose of its mirror,
+  // whose contributions are the same tets' transposed blocks.  Walking row i's incident tets in
+  // ascending order visits every contribution of row i's slots in ascending tet order, which is
+  // the order of the reference's gather lists (TET_MESH_FASTER.cpp:213-236); the mirror's sum
+  // runs over the same tets in the same order, so it is bitwise the transpose of the lower
+  // slot's own sum.
   std::vector<int> pos(nv, 0);
   std::vector<int> cnt(P.nslots + 1, 0);
   auto row_slots = [&](int i) {
     for (int k = P.row_ptr[i]; k < P.row_ptr[i + 1]; k++) pos[P.col_idx[k]] = P.bsr_slot[k];
   };
+  P.mirror.assign(P.nslots, -1);
   for (int i = 0; i < nv; i++) {
     row_slots(i);
+    for (int k = P.row_ptr[i]; k < P.row_ptr[i + 1]; k++) {
+      const int j = P.col_idx[k];
+      if (j > i)  // slot (j, i) of row j
+        for (int m = P.row_ptr[j]; m < P.row_ptr[j + 1]; m++)
+          if (P.col_idx[m] == i) P.mirror[P.bsr_slot[k]] = P.bsr_slot[m];
+    }
     for (int q = P.vptr[i]; q < P.vptr[i + 1]; q++) {
       const int32_t* T = tets + 4 * (size_t)inc[q];
-      for (int b = 0; b < 4; b++) cnt[pos[T[b]] + 1]++;
+      for (int b = 0; b < 4; b++)
+        if (T[b] >= i) cnt[pos[T[b]] + 1]++;
     }
   }
   P.cptr.assign(P.nslots + 1, 0);
   for (int s = 0; s < P.nslots; s++) P.cptr[s + 1] = P.cptr[s] + cnt[s + 1];
-  P.hdest.assign(16 * (size_t)nt, 0);
+  P.hdest.assign((size_t)kStagedBlocks * nt, 0);
   std::vector<int> fill(P.cptr.begin(), P.cptr.end() - 1);
   for (int i = 0; i < nv; i++) {
     row_slots(i);
@@ -208,7 +221,13 @@ void build_pattern(int nv, int nt, const int32_t* tets, int H, Pattern& P,
       const int32_t* T = tets + 4 * (size_t)t;
       const int x = P.fdest[4 * (size_t)t + 0] == q ? 0 : P.fdest[4 * (size_t)t + 1] == q ? 1
                   : P.fdest[4 * (size_t)t + 2] == q ? 2 : 3;
-      for (int y = 0; y < 4; y++) P.hdest[16 * (size_t)t + 4 * x + y] = fill[pos[T[y]]]++;
+      for (int y = 0; y < 4; y++) {
+        if (T[y] < i) continue;
+        // the element computes block (min(x, y), max(x, y)); staged transposed when x > y
+        const int word = fill[pos[T[y]]]++;
+        P.hdest[(size_t)kStagedBlocks * t + staged_pair(std::min(x, y), std::max(x, y))] =
+            x > y ? (word | kHdestTranspose) : word;
+      }
     }
   }
 }
@@ -416,8 +435,8 @@ size_t estimate_bytes(int nv, int nt, int nslots_est, int precision) {
   b += (size_t)nt * (16 + 9 * 8 + 8);           // tets, dminv, vol
   b += (size_t)nslots_est * (4 + 4 + 9 * ts);   // col + slot_row + kneg
   b += (size_t)nslots_est * 9 * ts * 4;         // A, D, E, C
-  b += (size_t)nslots_est * 4 + (size_t)nt * 20 * 4;  // cptr + hdest + fdest
-  b += (size_t)nt * (16 * 9 + 12) * ts;         // staging
+  b += (size_t)nslots_est * 8 + (size_t)nt * (kStagedBlocks + 4) * 4;  // cptr, mirror, hdest, fdest
+  b += (size_t)nt * (kStagedBlocks * 9 + 12) * ts;  // staging
   b += (size_t)nv * (9 + 18) * ts;              // minv + PCG vectors
   b += (size_t)nt * 4 * 4 + (size_t)nv * 16;    // vertex lists, misc
   b += 64 * 1024;                               // records, control block, partial sums
@@ -1254,6 +1273,11 @@ int hobak_inst_create(hobak_ctx* ctx, const double* rest_xyz, int32_t nv, const 
     return fail(ctx, HOBAK_E_INVALID, "unknown precision form");
   for (size_t i = 0; i < 4 * (size_t)nt; i++)
     if (tets[i] < 0 || tets[i] >= nv) return fail(ctx, HOBAK_E_INVALID, "tet index out of range");
+  for (size_t t = 0; t < (size_t)nt; t++) {
+    const int32_t* T = tets + 4 * t;
+    if (T[0] == T[1] || T[0] == T[2] || T[0] == T[3] || T[1] == T[2] || T[1] == T[3] || T[2] == T[3])
+      return fail(ctx, HOBAK_E_INVALID, "a tet repeats a vertex");
+  }
   // row_sort: the device holds the mesh with its vertices relabelled (row_sort_perm); the API
   // keeps the caller's numbering
   std::vector<int> perm;
@@ -1378,7 +1402,7 @@ int hobak_inst_create(hobak_ctx* ctx, const double* rest_xyz, int32_t nv, const 
   if ((rc = dalloc(I, (char**)&ptr, (n), ts)) != HOBAK_OK) return bail(rc)
   double *rest, *mass, *one_ring, *pin_target, *dminv, *vol;
   int4* t4d;
-  int *slice_off, *row_len, *col, *slot_row, *cptr, *hdest, *vptr, *fdest;
+  int *slice_off, *row_len, *col, *slot_row, *cptr, *hdest, *vptr, *fdest, *mirror;
   unsigned char* pinned;
   ALLOC(rest, 3 * (size_t)nv);
   ALLOC(h.u, 3 * (size_t)nv);
@@ -1401,6 +1425,7 @@ int hobak_inst_create(hobak_ctx* ctx, const double* rest_xyz, int32_t nv, const 
   ALLOC(slot_row, (size_t)P.nslots);
   ALLOC(cptr, P.cptr.size());
   ALLOC(hdest, P.hdest.size());
+  ALLOC(mirror, P.mirror.size());
   ALLOC(vptr, P.vptr.size());
   ALLOC(fdest, P.fdest.size());
   ALLOC(pinned, (size_t)nv);
@@ -1446,6 +1471,7 @@ int hobak_inst_create(hobak_ctx* ctx, const double* rest_xyz, int32_t nv, const 
   h.slot_row = slot_row;
   h.cptr = cptr;
   h.hdest = hdest;
+  h.mirror = mirror;
   h.vptr = vptr;
   h.fdest = fdest;
   h.pinned = pinned;
@@ -1472,6 +1498,7 @@ int hobak_inst_create(hobak_ctx* ctx, const double* rest_xyz, int32_t nv, const 
       (rc = upload(I, slot_row, P.slot_row.data(), P.slot_row.size())) ||
       (rc = upload(I, cptr, P.cptr.data(), P.cptr.size())) ||
       (rc = upload(I, hdest, P.hdest.data(), P.hdest.size())) ||
+      (rc = upload(I, mirror, P.mirror.data(), P.mirror.size())) ||
       (rc = upload(I, vptr, P.vptr.data(), P.vptr.size())) ||
       (rc = upload(I, fdest, P.fdest.data(), P.fdest.size())) ||
       (rc = upload(I, I->ctl, &ctl0, 1)) ||

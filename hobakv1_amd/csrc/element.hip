@@ -156,7 +156,7 @@ __device__ __forceinline__ void element_tet(const ElementIn& a, int t) {
     for (int b = 0; b < 3; b++) w[v][b] = Wd[v][0] * V[0][b] + Wd[v][1] * V[1][b] + Wd[v][2] * V[2][b];
 
   auto sh = (HOBAK_GLOBAL T*)a.stage_h;
-  auto hd = a.hdest + 16 * (size_t)t;
+  auto hd = a.hdest + kStagedBlocks * (size_t)t;
 #pragma unroll
   for (int v = 0; v < 4; v++) {
 #pragma unroll
@@ -187,19 +187,14 @@ __device__ __forceinline__ void element_tet(const ElementIn& a, int t) {
 #pragma unroll
         for (int c2 = 0; c2 < 3; c2++)
           B[c][c2] = vol * (UM[c][0] * U[c2][0] + UM[c][1] * U[c2][1] + UM[c][2] * U[c2][2]);
-      // block (v, v2) and, off the diagonal, its transpose as block (v2, v)
-      auto d0 = sh + 9 * (size_t)hd[4 * v + v2];
+      // block (v, v2), oriented as the upper slot it sums into
+      const int word = hd[staged_pair(v, v2)];
+      const bool tr = (word & kHdestTranspose) != 0;
+      auto d0 = sh + 9 * (size_t)(word & kHdestIndex);
 #pragma unroll
       for (int c = 0; c < 3; c++)
 #pragma unroll
-        for (int c2 = 0; c2 < 3; c2++) d0[3 * c + c2] = B[c][c2];
-      if (v2 != v) {
-        auto d1 = sh + 9 * (size_t)hd[4 * v2 + v];
-#pragma unroll
-        for (int c = 0; c < 3; c++)
-#pragma unroll
-          for (int c2 = 0; c2 < 3; c2++) d1[3 * c + c2] = B[c2][c];
-      }
+        for (int c2 = 0; c2 < 3; c2++) d0[3 * c + c2] = tr ? B[c2][c] : B[c][c2];
     }
   }
 }
@@ -412,17 +407,23 @@ __global__ __launch_bounds__(kElementBlock) void damping_hessian_kernel(Batch b,
       for (int i = 0; i < 3; i++) g[v][i] = (F[i][0] * d[v][0] + F[i][1] * d[v][1]) + F[i][2] * d[v][2];
     const double s = I->dmu * gbl(I->vol)[t];
     auto sh = gbl((T*)I->stage_h);
-    auto hd = gbl(I->hdest) + 16 * (size_t)t;
+    auto hd = gbl(I->hdest) + kStagedBlocks * (size_t)t;
+    // block (v, w) is s (G (d_v . d_w) + g_w g_v^T), G symmetric, so block (w, v) is its exact
+    // transpose: the upper blocks are staged, each oriented as its slot
 #pragma unroll
     for (int v = 0; v < 4; v++)
 #pragma unroll
-      for (int w = 0; w < 4; w++) {
+      for (int w = v; w < 4; w++) {
         const double dd = (d[v][0] * d[w][0] + d[v][1] * d[w][1]) + d[v][2] * d[w][2];
-        auto dst = sh + 9 * (size_t)hd[4 * v + w];
+        const int word = hd[staged_pair(v, w)];
+        const bool tr = (word & kHdestTranspose) != 0;
+        auto dst = sh + 9 * (size_t)(word & kHdestIndex);
 #pragma unroll
         for (int c = 0; c < 3; c++)
 #pragma unroll
-          for (int c2 = 0; c2 < 3; c2++) dst[3 * c + c2] = (T)(s * (G[c][c2] * dd + g[w][c] * g[v][c2]));
+          for (int c2 = 0; c2 < 3; c2++)
+            dst[3 * c + c2] = tr ? (T)(s * (G[c2][c] * dd + g[w][c2] * g[v][c]))
+                                 : (T)(s * (G[c][c2] * dd + g[w][c] * g[v][c2]));
       }
   }
 }

@@ -5,11 +5,15 @@
 //   vectors        DOF-ordered AoS: entry 3*v + c (the reference's flat VECTOR order)
 //   tets           int4 per tet
 //   DmInv          9 planes of nt doubles (plane 3*r + c holds DmInv(r, c))       -> coalesced
-//   element stage  written directly in the order the assembly reads it: the 3x3 block of vertex
-//                  pair (x, y) of tet t goes to stage_h[9 * hdest[16 t + 4 x + y] + 3 a + b], where
-//                  hdest is the position of that contribution in its slot's ascending-tet list,
-//                  so every slot sums one contiguous run; likewise force 3-vectors go to
-//                  stage_f[3 * fdest[4 t + x] + c] in each vertex's ascending-tet list
+//   element stage  written directly in the order the assembly reads it.  A tet's stiffness is
+//                  symmetric, so only its 10 blocks (x, y) with x <= y are staged: the block goes
+//                  to stage_h[9 * (hdest[10 t + pair(x, y)] & kHdestIndex) + 3 a + b], where
+//                  hdest is the position of that contribution in the ascending-tet list of the
+//                  upper slot (min(v_x, v_y), max(v_x, v_y)); with kHdestTranspose set the block
+//                  is staged transposed, so that it is oriented as that slot.  Every upper slot
+//                  sums one contiguous run and also writes the transpose into its mirror (lower)
+//                  slot.  Force 3-vectors go to stage_f[3 * fdest[4 t + x] + c] in each vertex's
+//                  ascending-tet list
 //   block matrix   SELL-H, k-interleaved (H = 8, 16, 32 or 64 rows per slice, L = 64 / H lanes
 //                  per row): slice s owns slots [slice_off[s], slice_off[s+1]) (a multiple of 64),
 //                  and the k-th block of row r sits at slot = slice_off[r / H] + H * k + (r % H).
@@ -133,6 +137,7 @@ struct InstDev {
   double *u, *v, *u_old, *v_old, *tmp64, *rhs64, *x64, *res64;
   const unsigned char* pinned;
   const int *slice_off, *row_len, *col, *slot_row, *cptr, *hdest, *vptr, *fdest;
+  const int* mirror;  // per upper off-diagonal slot: the slot of the transposed block (else -1)
   void *stage_h, *stage_f, *A, *D, *E, *C, *kneg, *minv, *r, *x, *z, *t, *p0, *p1;
   Ctl* ctl;    // == ctl_of(this) on the device
   double* dg;  // reduced-precision form: per-DOF diagonal term of the difference-form operator
@@ -242,6 +247,13 @@ int blocks_of(int kind, int nv, int nt, int nslots, int shift, int update_cap, i
 // Launchers.  `cap` limits the launch grid (0: one workgroup per virtual workgroup).
 enum ElemSrc { ELEM_U = 0, ELEM_TMP = 1 };
 enum AsmTarget { ASM_REST = 0, ASM_PLAIN_A = 1, ASM_SYSTEM = 2 };
+
+// Staged stiffness blocks per tet (the upper triangle of its 4x4 block pattern), their order, and
+// the destination word's flag and index bits.
+constexpr int kStagedBlocks = 10;
+constexpr int kHdestTranspose = (int)0x80000000u;
+constexpr int kHdestIndex = 0x7fffffff;
+__host__ __device__ constexpr int staged_pair(int x, int y) { return 4 * x - (x * (x - 1)) / 2 + (y - x); }
 
 template <typename T>
 hipError_t launch_element(const Batch& b, int src, int with_hessian, int cap, hipStream_t s);

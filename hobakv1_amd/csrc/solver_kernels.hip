@@ -476,9 +476,11 @@ __device__ __forceinline__ void write_precond(const T (&a)[9], HOBAK_GLOBAL T* m
     for (int k = 0; k < 3; k++) m[3 * row + k] = cof(k, row) * invdet;
 }
 
-// One lane per block slot.  ASM_REST / ASM_PLAIN_A: out = sum of staged contributions (the
-// stiffness, -K) into kneg / A.  ASM_SYSTEM: A = S (D - K) S + (I - S) (+ the ground penalty on
-// the diagonal); the diagonal slot also writes its preconditioner.
+// One lane per upper block slot (row <= column): it sums the slot's staged contributions and
+// writes the slot and, off the diagonal, the transposed block into its mirror slot.
+// ASM_REST / ASM_PLAIN_A: out = that sum (the stiffness, -K) into kneg / A.  ASM_SYSTEM:
+// A = S (D - K) S + (I - S) (+ the ground penalty on the diagonal); the diagonal slot also
+// writes its preconditioner.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void assemble_kernel(Batch b, int target) {
   for (int ws = blockIdx.x; ws < b.total; ws += gridDim.x) {
@@ -491,6 +493,8 @@ __global__ __launch_bounds__(kBlock) void assemble_kernel(Batch b, int target) {
     const int r = gbl(I->slot_row)[slot];
     if (r < 0) continue;
     const int c = gbl(I->col)[slot];
+    if (c < r) continue;  // a lower slot: written by its mirror
+    const int mslot = c == r ? -1 : gbl(I->mirror)[slot];
     const size_t ns = (size_t)nslots;
     T acc[9];
 #pragma unroll
@@ -502,10 +506,16 @@ __global__ __launch_bounds__(kBlock) void assemble_kernel(Batch b, int target) {
 #pragma unroll
       for (int e = 0; e < 9; e++) acc[e] += src[e];
     }
+    // the mirror slot (c, r) holds the transpose: entry (a, b) there is entry (b, a) here
+    constexpr int kTr[9] = {0, 3, 6, 1, 4, 7, 2, 5, 8};
     if (target != ASM_SYSTEM) {
       auto out = gbl((T*)(target == ASM_REST ? I->kneg : I->A));
 #pragma unroll
       for (int e = 0; e < 9; e++) out[e * ns + slot] = acc[e];
+      if (mslot >= 0) {
+#pragma unroll
+        for (int e = 0; e < 9; e++) out[e * ns + mslot] = acc[kTr[e]];
+      }
       continue;
     }
     constexpr bool kDiff = sizeof(T) < sizeof(double);
@@ -513,8 +523,8 @@ __global__ __launch_bounds__(kBlock) void assemble_kernel(Batch b, int target) {
     // plane constraints A stays unprojected here: the right-hand side kernel needs A z, then
     // projects each row in place.
     const bool contact = I->contact != 0;
-    const bool pin = contact ? false
-                     : kDiff ? (bool)gbl(I->pinned)[r] : (gbl(I->pinned)[r] || gbl(I->pinned)[c]);
+    const bool pin_r = gbl(I->pinned)[r] != 0, pin_c = gbl(I->pinned)[c] != 0;
+    const bool pin = contact ? false : kDiff ? pin_r : (pin_r || pin_c);
     bool ground = false;
     if (I->ground_on && c == r && !pin) {
       const double y = gbl(I->rest)[3 * (size_t)r + 1] + gbl(I->u)[3 * (size_t)r + 1];
@@ -528,6 +538,11 @@ __global__ __launch_bounds__(kBlock) void assemble_kernel(Batch b, int target) {
     for (int e = 0; e < 9; e++) {
       val[e] = pin ? ((c == r && (e % 4) == 0) ? T(1) : T(0)) : D[e * ns + slot] + acc[e];
       out[e * ns + slot] = val[e];
+    }
+    if (mslot >= 0) {
+      const bool pin_m = contact ? false : kDiff ? pin_c : (pin_r || pin_c);
+#pragma unroll
+      for (int e = 0; e < 9; e++) out[e * ns + mslot] = pin_m ? T(0) : D[e * ns + mslot] + acc[kTr[e]];
     }
     if (c == r && !contact) {
       write_precond<T>(val, gbl((T*)I->minv), r, I->precond);
