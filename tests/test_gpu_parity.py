@@ -166,6 +166,42 @@ def test_iteration_cap_reports_not_converged(gpu_ctx):
     inst.close()
 
 
+@pytest.mark.parametrize("damped", [False, True])
+def test_assembled_stiffness_is_exactly_symmetric(gpu_ctx, damped):
+    """Only a tet's 10 upper blocks are staged and every lower slot is written as its mirror's
+    transpose, so the off-diagonal blocks of K (and of the damping stiffness C) are symmetric
+    bitwise: block (j, i) equals block (i, j) transposed, entry for entry.  A diagonal block is
+    its own mirror, formed in one piece, and symmetric to round-off."""
+    sc = scenes.lattice_scene((5, 4, 6), seed=12, material=scenes.MAT_ARAP, stretch_velocity=True)
+    if damped:
+        sc = scenes.green_scene(n=(5, 4, 6))
+    inst = lib.Instance(gpu_ctx, sc)
+    u = 0.03 * np.random.default_rng(8).uniform(-1, 1, sc.dofs)
+    _, K = inst.damping_forces_hessian(u) if damped else inst.forces_hessian(u)
+    rp, ci = inst.bsr_pattern()
+    where = {}
+    for i in range(sc.nv):
+        for k in range(rp[i], rp[i + 1]):
+            where[(i, int(ci[k]))] = k
+    scale = np.abs(K).max()
+    for (i, j), k in where.items():
+        if i != j:
+            assert np.array_equal(K[k], K[where[(j, i)]].T), (i, j)
+        else:
+            assert np.abs(K[k] - K[k].T).max() <= 1e-14 * scale, i
+    inst.close()
+
+
+def test_tet_repeating_a_vertex_is_rejected(gpu_ctx):
+    sc = scenes.lattice_scene((2, 2, 2), seed=1)
+    tets = sc.tets.copy()
+    tets[3, 2] = tets[3, 0]
+    with pytest.raises(lib.HobakError) as e:
+        lib.Instance(gpu_ctx, scenes.Scene(name="bad", rest=sc.rest, tets=tets, material=sc.material,
+                                           mu=sc.mu, lam=sc.lam))
+    assert e.value.code == -1
+
+
 def test_c2_jacobi_matches_reference(gpu_ctx):
     """40^3 stretch scene, 20 steps, Jacobi PCG, against the reference's golden trajectory.
 
