@@ -360,7 +360,7 @@ def serving_form(lib, scenes, args):
 
     for s_ in stream:
         scene_of(s_.k)
-    ca = lib.Context(0, precision="fp32", workload="batch")
+    ca = lib.Context(0, precision="fp32", workload="serving")
     cb = lib.Context(0)
     ca.set_tuning(graphs=GRAPHS[0])
     cb.set_tuning(graphs=GRAPHS[0])

@@ -188,7 +188,9 @@ class Context:
     apply_table: apply the autotuned launch shapes of tuning.json (hobakv1_amd/autotune.py) for
     `precision` and `workload` to the instances this context creates.  workload "single" is one
     instance stepped at a time (tuned on the 40^3 scene), "batch" many instances stepped together
-    (tuned on the 256-instance batch); a missing batch entry falls back to the single one.
+    (tuned on the 256-instance batch), "serving" the continuously batched request stream (tuned on
+    the serving stream); a missing serving entry falls back to the batch one, a missing batch
+    entry to the single one.
     """
 
     def __init__(self, device: int = 0, budget_bytes: int = 0, apply_table: bool = True,
@@ -211,7 +213,11 @@ class Context:
 
         with open(path) as f:
             table = json.load(f)
-        entry = table.get(f"{precision}_batch") if workload == "batch" else None
+        entry = None
+        if workload == "serving":
+            entry = table.get(f"{precision}_serving") or table.get(f"{precision}_batch")
+        elif workload == "batch":
+            entry = table.get(f"{precision}_batch")
         entry = entry or table.get(precision)
         if entry:
             self.set_tuning(slice_rows=entry.get("slice_rows"), poll_chunk=entry.get("poll_chunk"),

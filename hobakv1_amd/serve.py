@@ -247,7 +247,7 @@ def main():
 
     for s in stream:
         scene_of(s.k)
-    ca = lib.Context(0, precision="fp32", workload="batch")
+    ca = lib.Context(0, precision="fp32", workload="serving")
     cb = lib.Context(0)
     out = compare(ca, cb, stream, scene_of, int(args.budget_gib * (1 << 30)))
     ca.close()
