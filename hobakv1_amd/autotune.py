@@ -9,9 +9,12 @@ tuning.json next to this file; lib.Context applies it to every instance it creat
 
     python -m hobakv1_amd.autotune --n 40 --steps 4 [--precision fp64|fp32] [--write]
     python -m hobakv1_amd.autotune --workload batch --instances 256 [--precision ...] [--write]
+    python -m hobakv1_amd.autotune --workload serving --precision fp32 [--write]
 
 The batch workload (the 256-instance C4 set stepped together by hobak_step_batch) is written
-as the "<precision>_batch" entry, which contexts created for batched or served instances apply.
+as the "<precision>_batch" entry, which contexts created for batched instances apply; the serving
+workload (the serving stream of serve.py: 48 sessions, Poisson 400/s, 20 steps each, 8 GiB,
+measured by its throughput) as the "<precision>_serving" entry.
 """
 from __future__ import annotations
 
@@ -91,6 +94,24 @@ def measure_batch(ctx, scs, precision, steps, warmup):
                           if ks["spmv_avg_ms"] > 0 else None)}
 
 
+def measure_serving(ctx, precision):
+    from . import lib, scenes, serve
+
+    stream = serve.request_stream(48, 400.0, 20, seed=5)
+    cache = {}
+
+    def scene_of(k):
+        if k not in cache:
+            cache[k] = scenes.serving_instance(k, seed=5)
+        return cache[k]
+
+    r = serve.run(ctx, stream, scene_of, 8 << 30, precision, mode="batched", keep_states=False)
+    # ranked by ms_per_step like the other workloads: the stream's makespan over its requests
+    return {"ms_per_step": 1e3 * r["makespan_s"] / max(1, r["requests"]), "tet_steps_per_s": r["value"],
+            "p50_step_ms": r["p50_step_ms"], "p99_step_ms": r["p99_step_ms"], "spmv_ms": 0.0,
+            "spmv_GBps": None}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--n", type=int, default=40)
@@ -103,7 +124,7 @@ def main():
     ap.add_argument("--spmv-pad", default="0,1")
     ap.add_argument("--row-sort", default="0", help="vertex relabelling by row length: 0, 1 or 0,1")
     ap.add_argument("--inner-tol", default="1e-4", help="FP32 inner PCG tolerances to sweep")
-    ap.add_argument("--workload", default="single", choices=["single", "batch"])
+    ap.add_argument("--workload", default="single", choices=["single", "batch", "serving"])
     ap.add_argument("--instances", type=int, default=256, help="batch workload size")
     ap.add_argument("--write", action="store_true")
     args = ap.parse_args()
@@ -122,7 +143,10 @@ def main():
                                                        [int(x) for x in args.row_sort.split(",")], tols):
         ctx = lib.Context(0, apply_table=False)
         ctx.set_tuning(slice_rows=H, poll_chunk=poll, update_wgs=uw, spmv_pad=pad, row_sort=rs)
-        if scs is not None:
+        if args.workload == "serving":
+            ctx.refine_inner_tol = tol or 1e-4
+            r = measure_serving(ctx, precision)
+        elif scs is not None:
             ctx.refine_inner_tol = tol or 1e-4
             r = measure_batch(ctx, scs, precision, args.steps, args.warmup)
         else:
@@ -139,8 +163,9 @@ def main():
     print("best", json.dumps(best), flush=True)
     if args.write:
         table = load_table()
-        key = args.precision + ("_batch" if scs is not None else "")
-        what = (f"{args.instances}-instance batch" if scs is not None else f"{args.n}^3 lattice")
+        key = args.precision + {"single": "", "batch": "_batch", "serving": "_serving"}[args.workload]
+        what = {"single": f"{args.n}^3 lattice", "batch": f"{args.instances}-instance batch",
+                "serving": "serving stream: 48 sessions, Poisson 400/s, 20 steps each, 8 GiB"}[args.workload]
         table[key] = {"slice_rows": best["slice_rows"], "poll_chunk": best["poll_chunk"],
                       "update_wgs": best["update_wgs"], "spmv_pad": best["spmv_pad"],
                       "row_sort": best["row_sort"], "measured": {"workload": what, **best}}
