@@ -1,31 +1,32 @@
 #!/usr/bin/env python3
 """Benchmark of the backward-Euler Newton step on one MI355X.
 
-A "step" is one timestep of the workload: one Newton iteration over the whole mesh (element
-kernel, assembly, right-hand side, PCG to tol 1e-8, state update).  The headline workload is the
-40^3 Kuhn-lattice cube (384,000 tets, 68,921 vertices), Stable Neo-Hookean E=1e5 nu=0.45 rho=1000,
-x=0 face pinned, seeded [0.7, 1.3] stretch applied as a velocity field, dt = 1/60, FP64,
-block-Jacobi PCG.  Inputs are resident in HBM before timing starts; the warmup steps are followed
-by a reset to the initial state, so the timed steps are steps 1..K of the workload.
+The headline is the metric's own comparison: the serving request stream (tets log-uniform in
+[1e3, 4e5], aspect [0.5, 2], mixed SNH/ARAP, a penalty ground at y = -0.5 with k = 1e6, instances
+dropped from y in [0, 1]), served by the continuous-batching scheduler in the precision form (FP32
+element/assembly/PCG with FP64 refinement) under an HBM budget, against the plain FP64 GPU path
+serving the same stream one step at a time.  A "step" is one step request: one Newton iteration
+of one instance (element kernel, assembly, right-hand side, PCG to tol 1e-8, state update).  Each
+session issues --steps requests one after another; --warmup sessions run through the scheduler
+before the timed stream.  Instance set-up (host preprocessing and uploads, before a session's
+first request) is outside the clock; the end-to-end rate that charges it is reported beside it.
 
 Output: one JSON line (see the field list in main()).
-  value      tet-Newton-steps/s = nt * Newton iterations * K / timed seconds
-  roofline   the PCG's BSR SpMV (the dominant kernel): algorithmic bytes per launch
-             nnzb*(9*8+4) + 4*(nv+1) + 2*3*8*nv divided by its average launch time, sampled live
-             during the timed steps with HIP start/stop events that the SpMV's own dispatch
-             carries (hipExtLaunchKernelGGL) on the library's stream; `traffic` is the HBM bytes
-             per launch from the committed rocprofv3 --pmc summary of this workload
-  cpu_baseline  the literal reference (oracle/_ref, compiled from its sources) timed on a bounded
-             sample of the same workload on this host's cores
-  fp32_refine   the same K steps in the precision form (FP32 element/assembly/PCG, FP64
-             refinement), its speed-up and its position error against the FP64 run
-  batch      256 independent instances (dims U{4..24}^3, SNH/ARAP, seeded E and nu) advanced
-             together by hobak_step_batch, FP64 and FP32+refinement, against stepping each alone
-  serving    a Poisson stream of sessions (tets log-uniform in [1e3, 4e5], penalty ground,
-             20 step requests each) served by the continuous-batching scheduler in the precision
-             form, against the plain FP64 path serving the same stream one step at a time
-  beyond_mall   the SpMV's roofline on one 88^3 lattice instance (4.09 M tets, 0.83 GB FP64 /
-             0.44 GB FP32 per launch), FP64 and FP32, where no cache holds the matrix
+  value      serving tet-Newton-steps/s = sum over requests of nt / the stream's makespan
+  roofline   the dominant kernel of the serving stream, the PCG's BSR SpMV (FP32): algorithmic
+             bytes nnzb*(9*4+4) + 4*(nv+1) + 2*3*4*nv of the instances solving in a launch's tick,
+             over its duration, sampled live with HIP start/stop events that the SpMV's own
+             dispatch carries (hipExtLaunchKernelGGL) on the library's stream; `traffic` is the
+             HBM bytes per launch from the committed rocprofv3 --pmc summary of this stream
+  cpu_baseline  the oracle's FP64 restatement (OpenMP) timed on a bounded sample of the same
+             stream's sessions on this host's cores (the literal reference has no penalty
+             ground); cpu_baseline_c2 times the literal reference (oracle/_ref) on the 40^3 scene
+  plain_fp64_c2  the 40^3 Kuhn-lattice cube (384,000 tets), SNH E=1e5 nu=0.45 rho=1000, x=0
+             pinned, seeded stretch velocity, FP64 block-Jacobi: K timed steps, its SpMV roofline,
+             the element kernel's FLOP rate and the per-phase times
+  fp32_refine   the same 40^3 steps in the precision form, and C3's per-step errors (SNH, ARAP)
+  batch      256 independent instances (dims U{4..24}^3) advanced together by hobak_step_batch
+  beyond_mall   the SpMV's roofline on one 88^3 lattice instance (4.09 M tets), FP64 and FP32
 """
 from __future__ import annotations
 
@@ -54,19 +55,26 @@ GRAPHS = [1]  # --graphs, applied to every context the benchmark creates
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--n", type=int, default=40, help="lattice cells per axis")
+    ap.add_argument("--steps", type=int, default=20,
+                    help="step requests per serving session (and timed steps of the 40^3 section)")
+    ap.add_argument("--warmup", type=int, default=2,
+                    help="untimed sessions served before the timed stream (and untimed steps of "
+                         "the 40^3 section)")
+    ap.add_argument("--n", type=int, default=40, help="lattice cells per axis of the C2 section")
     ap.add_argument("--precond", default="block", choices=["block", "jacobi"])
     ap.add_argument("--cpu-baseline", default="auto", choices=["auto", "reference", "port", "none"])
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=20, help="steps of the 40^3 reference timing")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0,
+                    help="bound on the CPU time of the serving-stream sample")
     ap.add_argument("--sample-every", type=int, default=256,
-                    help="time every n-th SpMV launch of the timed steps with HIP events; a PCG "
-                         "group holding one runs uncaptured, the others replay their graph")
+                    help="time every n-th SpMV launch with HIP events; a PCG group holding one "
+                         "runs uncaptured, the others replay their graph")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run the calibration copy kernel (for counter scaling)")
+    ap.add_argument("--plain-c2", default="on", choices=["on", "off"],
+                    help="the 40^3 plain FP64 section")
     ap.add_argument("--precision-form", default="on", choices=["on", "off"],
-                    help="also time the FP32 + FP64-refinement path on the same workload")
+                    help="also time the FP32 + FP64-refinement path on the 40^3 workload")
     ap.add_argument("--batch", default="on", choices=["on", "off"],
                     help="also time 256 instances stepped together (hobak_step_batch)")
     ap.add_argument("--batch-instances", type=int, default=256)
@@ -74,11 +82,12 @@ def parse():
     ap.add_argument("--batch-alone", default="on", choices=["on", "off"],
                     help="also step each batch instance alone (speedup and bitwise comparison)")
     ap.add_argument("--serving", default="on", choices=["on", "off"],
-                    help="also run the serving stream against the plain FP64 path")
-    ap.add_argument("--sessions", type=int, default=48)
-    ap.add_argument("--rate", type=float, default=400.0, help="session arrivals per second")
-    ap.add_argument("--session-steps", type=int, default=20)
-    ap.add_argument("--budget-gib", type=float, default=8.0)
+                    help="the serving stream (the headline)")
+    ap.add_argument("--plain-serving", default="on", choices=["on", "off"],
+                    help="the plain FP64 path on the same stream (the comparison)")
+    ap.add_argument("--sessions", type=int, default=192)
+    ap.add_argument("--rate", type=float, default=160.0, help="session arrivals per second")
+    ap.add_argument("--budget-gib", type=float, default=6.0)
     ap.add_argument("--beyond-mall", default="on", choices=["on", "off"],
                     help="the SpMV on one large instance whose matrix exceeds the Infinity Cache")
     ap.add_argument("--large-n", type=int, default=88, help="lattice cells per axis of that instance")
@@ -88,10 +97,11 @@ def parse():
     return ap.parse_args()
 
 
-def measured_traffic(n, precision="fp64"):
-    """HBM bytes per SpMV launch from the committed rocprofv3 --pmc summary, when it was taken on
-    this workload (FETCH_SIZE and WRITE_SIZE scaled by a calibration copy kernel)."""
-    path = os.path.join(ROOT, "profiles", f"cg_spmv_{n}cube_{precision}_hbm_traffic.md")
+def measured_traffic(name, precision):
+    """HBM bytes per SpMV launch from the committed rocprofv3 --pmc summary of a workload
+    (profiles/cg_spmv_<name>_<precision>_hbm_traffic.md: FETCH_SIZE and WRITE_SIZE scaled by a
+    calibration copy kernel), with the algorithmic bytes per launch the same run counted."""
+    path = os.path.join(ROOT, "profiles", f"cg_spmv_{name}_{precision}_hbm_traffic.md")
     if not os.path.exists(path):
         return None
     vals = {}
@@ -100,27 +110,73 @@ def measured_traffic(n, precision="fp64"):
             if ":" in line and not line.startswith((" ", "#", "|")):
                 k, v = line.split(":", 1)
                 vals[k.strip()] = v.strip()
-    if vals.get("workload_n") != str(n) or vals.get("precision") != precision:
+    if vals.get("workload") != name or vals.get("precision") != precision:
         return None
-    return {"bytes": float(vals["traffic_bytes_per_launch"]), "kernel": vals.get("kernel"),
-            "source": os.path.relpath(path, ROOT)}
+    out = {"bytes": float(vals["traffic_bytes_per_launch"]), "kernel": vals.get("kernel"),
+           "source": os.path.relpath(path, ROOT)}
+    if "algorithmic_bytes_per_launch" in vals:
+        out["algorithmic"] = float(vals["algorithmic_bytes_per_launch"])
+    return out
+
+
+def _quiet_stdout(fn, *a):
+    """Run fn with stdout sent to stderr (the reference prints progress; stdout carries only the
+    one JSON line)."""
+    sys.stdout.flush()
+    saved = os.dup(1)
+    os.dup2(2, 1)
+    try:
+        return fn(*a)
+    finally:
+        sys.stdout.flush()
+        os.dup2(saved, 1)
+        os.close(saved)
+
+
+def _cores():
+    return int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+
+
+def cpu_baseline_serving(stream, scene_of, steps, seconds):
+    """The oracle's FP64 restatement (OpenMP, block-Jacobi PCG, the penalty ground) on the serving
+    stream's sessions in arrival order, each for its `steps` step requests, until `seconds` of CPU
+    step time have run: tet-Newton-steps/s and the step latency percentiles of that sample."""
+    import oracle  # test/baseline infrastructure only
+
+    def run():
+        from hobakv1_amd import scenes
+
+        secs, tets, done = [], 0, []
+        for s_ in stream:
+            sc = scene_of(s_.k)
+            o = oracle.Oracle(sc, precond=scenes.PRECOND_BLOCK_JACOBI)
+            for _ in range(steps):
+                _, _, sec = o.step()
+                secs.append(sec)
+                tets += sc.nt
+            o.close()
+            done.append(sc.nt)
+            if sum(secs) >= seconds:
+                break
+        total = float(np.sum(secs))
+        return {
+            "value": tets / total, "unit": "tet-Newton-steps/s", "cores": _cores(), "kind": "port",
+            "sample": f"the first {len(done)} sessions of the serving stream (nt {done}), "
+                      f"{steps} steps each ({len(secs)} steps, {total:.1f} s), the oracle's FP64 "
+                      f"restatement with block-Jacobi PCG tol 1e-8 and the penalty ground (the "
+                      f"literal reference has no ground)",
+            "p50_step_ms": 1e3 * float(np.percentile(secs, 50)),
+            "p99_step_ms": 1e3 * float(np.percentile(secs, 99)),
+        }
+
+    return _quiet_stdout(run)
 
 
 def cpu_baseline(scene, steps, kind):
     """Time the CPU checker on the first `steps` steps of the same scene."""
     import oracle  # test/baseline infrastructure only
 
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    # the reference prints progress to stdout; keep stdout for the one JSON line
-    sys.stdout.flush()
-    saved = os.dup(1)
-    os.dup2(2, 1)
-    try:
-        return _cpu_baseline(oracle, scene, steps, kind, cores)
-    finally:
-        sys.stdout.flush()
-        os.dup2(saved, 1)
-        os.close(saved)
+    return _quiet_stdout(_cpu_baseline, oracle, scene, steps, kind, _cores())
 
 
 def _cpu_baseline(oracle, scene, steps, kind, cores):
@@ -148,6 +204,8 @@ def _cpu_baseline(oracle, scene, steps, kind, cores):
         "sample": f"first {steps} steps of the same {scene.dims[0]}^3 scene, Jacobi PCG tol 1e-8 "
                   f"(the reference's preconditioner); {total:.2f} s; PCG iters {iters}",
         "ms_per_step": 1e3 * total / steps,
+        "p50_step_ms": 1e3 * float(np.percentile(secs, 50)),
+        "p99_step_ms": 1e3 * float(np.percentile(secs, 99)),
     }
 
 
@@ -348,9 +406,10 @@ def beyond_mall_form(lib, scenes, args):
 
 
 def serving_form(lib, scenes, args):
+    """The headline: the serving stream in the precision form against the plain FP64 path."""
     from hobakv1_amd import serve
 
-    stream = serve.request_stream(args.sessions, args.rate, args.session_steps, seed=5)
+    stream = serve.request_stream(args.sessions, args.rate, args.steps, seed=5)
     cache = {}
 
     def scene_of(k):
@@ -358,46 +417,58 @@ def serving_form(lib, scenes, args):
             cache[k] = scenes.serving_instance(k, seed=5)
         return cache[k]
 
-    for s_ in stream:
+    for s_ in stream:  # the scenes' host-side generation runs before any clock
         scene_of(s_.k)
+    budget = int(args.budget_gib * (1 << 30))
     ca = lib.Context(0, precision="fp32", workload="serving")
-    cb = lib.Context(0)
     ca.set_tuning(graphs=GRAPHS[0])
-    cb.set_tuning(graphs=GRAPHS[0])
-    out = serve.compare(ca, cb, stream, scene_of, int(args.budget_gib * (1 << 30)),
-                        sample_every=args.sample_every)
+    cb = None
+    if args.plain_serving == "on":
+        cb = lib.Context(0)
+        cb.set_tuning(graphs=GRAPHS[0])
+    if args.warmup > 0:  # untimed sessions of another seed: module loading, clocks, arenas
+        warm = serve.request_stream(args.warmup, args.rate, args.steps, seed=6)
+        wsc = {w.k: scenes.serving_instance(w.k, seed=6) for w in warm}
+        serve.run(ca, warm, lambda k: wsc[k], budget, lib.PRECISION_FP32_REFINE, keep_states=False)
+        if cb is not None:
+            serve.run(cb, warm, lambda k: wsc[k], budget, lib.PRECISION_FP64, mode="plain",
+                      keep_states=False)
+    out = serve.compare(ca, cb, stream, scene_of, budget, sample_every=args.sample_every)
     ks = out.pop("spmv", None)
+    eng = out.get("engine") or {}
     if ks and ks["spmv_avg_ms"] > 0:
         gbs = ks["spmv_bytes"] / (ks["spmv_avg_ms"] * 1e-3) / 1e9
+        tr = measured_traffic("serving_stream", "fp32")
         out["roofline"] = {"bound": "hbm", "kernel": "cg_spmv_kernel<float>", "achieved": round(gbs, 1),
                            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+                           "traffic": tr["bytes"] if tr else None,
+                           "traffic_source": tr["source"] if tr else None,
+                           "traffic_run_algorithmic_bytes": tr.get("algorithmic") if tr else None,
                            "bytes_per_launch": ks["spmv_bytes"], "avg_launch_ms": ks["spmv_avg_ms"],
                            "launches_sampled": ks["spmv_count"],
+                           "stream_avg_bytes_per_launch": eng.get("spmv_bytes_per_launch"),
                            "note": "sampled launches of the serving engine; bytes are those of the "
                                    "instances solving in the launch's tick"}
     ca.close()
-    cb.close()
+    if cb is not None:
+        cb.close()
     out["rate_sessions_per_s"] = args.rate
     out["mean_tets"] = float(np.mean([scene_of(s_.k).nt for s_ in stream]))
-    out["workload"] = (f"{args.sessions} sessions, Poisson arrivals {args.rate:g}/s, "
-                       f"{args.session_steps} step requests each, tets log-U[1e3,4e5], aspect "
-                       f"[0.5,2], SNH/ARAP, penalty ground y=-0.5 k=1e6, HBM budget "
-                       f"{args.budget_gib:g} GiB; instance set-up excluded from both clocks")
-    return out
+    arr = [s_.arrival for s_ in stream]
+    out["arrival_span_s"] = float(arr[-1] - arr[0]) if arr else 0.0
+    out["workload"] = (f"serving request stream: {args.sessions} sessions, Poisson arrivals "
+                       f"{args.rate:g}/s, {args.steps} step requests each, tets log-U[1e3,4e5], "
+                       f"aspect [0.5,2], SNH/ARAP, penalty ground y=-0.5 k=1e6, dropped from "
+                       f"y in [0,1], HBM budget {args.budget_gib:g} GiB, FP32 + FP64 refinement, "
+                       f"block-Jacobi PCG tol 1e-8; instance set-up outside the clock")
+    return out, stream, scene_of
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              "(the path shards by instance: run one process per GPU, each with its own instances)", file=sys.stderr)
-        return 2
-    from hobakv1_amd import lib, scenes
-
+def plain_c2_form(lib, scenes, args):
+    """The 40^3 scene on the plain FP64 path: K timed steps after W warmup steps and a reset."""
     sc = scenes.lattice_scene((args.n, args.n, args.n), seed=2, stretch_velocity=True,
                               name=f"cube{args.n}_snh_stretch")
     precond = scenes.PRECOND_BLOCK_JACOBI if args.precond == "block" else scenes.PRECOND_JACOBI
-    GRAPHS[0] = 1 if args.graphs == "on" else 0
     ctx = lib.Context(0)
     ctx.set_tuning(graphs=GRAPHS[0])
     t0 = time.time()
@@ -409,24 +480,22 @@ def main():
     inst.set_kernel_sampling(args.sample_every)
     step_s = []
     iters = []
-    phase = np.zeros(5)
     t_start = time.perf_counter()
     for _ in range(args.steps):
         t1 = time.perf_counter()
         st = inst.step()
         step_s.append(time.perf_counter() - t1)
         iters.append(st["pcg_iters"][0])
-        ph = inst.timing_ms()
-        phase += np.array([ph["element"], ph["assemble"], ph["rhs"], ph["pcg"], ph["update"]])
     total = time.perf_counter() - t_start
     ks = inst.kernel_stats()
     u, _ = inst.state()
     finite = bool(np.all(np.isfinite(u)))
-    # one more step outside the timed region, with direct launches, so that the element kernel
-    # has events of its own (a captured step times its prologue as one piece)
+    # one more step outside the timed region, with direct launches, so that every phase has
+    # events of its own (a captured step times its prologue as one piece)
     ctx.set_tuning(graphs=0)
     inst.step()
-    el_ms = inst.timing_ms()["element"]
+    ph = inst.timing_ms()
+    el_ms = ph["element"]
     ctx.set_tuning(graphs=GRAPHS[0])
     el_tflops = ELEMENT_FP64_FLOPS_PER_TET * sc.nt / (el_ms * 1e-3) / 1e12 if el_ms > 0 else 0.0
     element = {"kernel": "element_kernel<double>", "bound": "latency", "ms": el_ms,
@@ -434,72 +503,113 @@ def main():
                "frac": round(el_tflops / FP64_VALU_PEAK_TFLOPS, 4), "valu_busy": ELEMENT_VALU_BUSY,
                "flops_per_tet": ELEMENT_FP64_FLOPS_PER_TET,
                "source": "profiles/element_kernel_40cube_valu.md"}
-
     spmv_gbs = ks["spmv_bytes"] / (ks["spmv_avg_ms"] * 1e-3) / 1e9 if ks["spmv_avg_ms"] > 0 else 0.0
-    tr = measured_traffic(args.n)
+    tr = measured_traffic(f"{args.n}cube", "fp64")
     roofline = {
-        "bound": "hbm",
-        "kernel": "cg_spmv_kernel<double>",
-        "achieved": round(spmv_gbs, 1),
-        "peak": HBM_PEAK_GBS,
-        "unit": "GB/s",
-        "frac": round(spmv_gbs / HBM_PEAK_GBS, 4),
-        "traffic": tr["bytes"] if tr else None,
-        "traffic_source": tr["source"] if tr else None,
-        "bytes_per_launch": ks["spmv_bytes"],
-        "avg_launch_ms": ks["spmv_avg_ms"],
+        "bound": "hbm", "kernel": "cg_spmv_kernel<double>", "achieved": round(spmv_gbs, 1),
+        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(spmv_gbs / HBM_PEAK_GBS, 4),
+        "traffic": tr["bytes"] if tr else None, "traffic_source": tr["source"] if tr else None,
+        "bytes_per_launch": ks["spmv_bytes"], "avg_launch_ms": ks["spmv_avg_ms"],
         "launches_sampled": ks["spmv_count"],
     }
     out = {
-        "metric": METRIC,
-        "value": sc.nt * args.steps / total,
-        "unit": "tet-Newton-steps/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
+        "value": sc.nt * args.steps / total, "unit": "tet-Newton-steps/s", "steps": args.steps,
         "ms_per_step": 1e3 * total / args.steps,
         "p50_step_ms": 1e3 * float(np.percentile(step_s, 50)),
         "p99_step_ms": 1e3 * float(np.percentile(step_s, 99)),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
-        "config": {
-            "workload": f"{args.n}^3 Kuhn lattice cube ({sc.nt} tets, {sc.nv} vertices), SNH E=1e5 "
-                        f"nu=0.45 rho=1000, x=0 pinned, seeded [0.7,1.3] stretch velocity, "
-                        f"dt=1/60, 1 Newton iteration/step, FP64, "
-                        f"{'block-Jacobi' if precond else 'Jacobi'} PCG tol 1e-8",
-            "tets": sc.nt, "vertices": sc.nv, "precision": "fp64",
-            "precond": args.precond, "parallelism": "single GPU",
-        },
+        "workload": f"{args.n}^3 Kuhn lattice cube ({sc.nt} tets, {sc.nv} vertices), SNH E=1e5 "
+                    f"nu=0.45 rho=1000, x=0 pinned, seeded [0.7,1.3] stretch velocity, dt=1/60, "
+                    f"1 Newton iteration/step, FP64, "
+                    f"{'block-Jacobi' if precond else 'Jacobi'} PCG tol 1e-8",
         "pcg_iters_per_step": iters,
-        "phase_ms_per_step": dict(zip(["element", "assemble", "rhs", "pcg", "update"],
-                                      (phase / args.steps).round(4).tolist())),
-        "setup_s": round(setup_s, 3),
-        "finite": finite,
-        "roofline": roofline,
+        "phase_ms_direct_launch_step": {k: round(v, 4) for k, v in ph.items()},
+        "setup_s": round(setup_s, 3), "finite": finite, "roofline": roofline,
         "element_kernel": element,
     }
-    if args.precision_form == "on":
-        out["fp32_refine"] = precision_form(lib, sc, precond, args, total, u)
-        out["fp32_refine"]["c3_errors_vs_fp64"] = c3_errors(lib, scenes, args.steps)
+    inst.close()
+    ctx.close()
+    return out, sc, precond, total, u
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(the path shards by instance: run one process per GPU, each with its own instances)",
+              file=sys.stderr)
+        return 2
+    from hobakv1_amd import lib, scenes
+
+    GRAPHS[0] = 1 if args.graphs == "on" else 0
+    out = {"metric": METRIC}
+    stream = scene_of = None
+    if args.serving == "on":
+        sv, stream, scene_of = serving_form(lib, scenes, args)
+        pl = sv.get("plain_fp64") or {}
+        out.update({
+            "value": sv["value"],
+            "unit": "tet-Newton-steps/s",
+            "n_gpus": 1,
+            "steps": args.steps,
+            "warmup": args.warmup,
+            # the stream's makespan per step request (requests overlap on the device)
+            "ms_per_step": 1e3 * sv["makespan_s"] / max(1, sv["requests"]),
+            "p50_step_ms": sv["p50_step_ms"],
+            "p99_step_ms": sv["p99_step_ms"],
+            "vs_fp64_gpu": sv.get("speedup_vs_plain_fp64"),
+            "fp64_gpu": {"value": pl.get("value"), "p50_step_ms": pl.get("p50_step_ms"),
+                         "p99_step_ms": pl.get("p99_step_ms")},
+            "max_rel_position_error_vs_fp64_gpu": sv.get("max_rel_position_error_vs_plain_fp64"),
+            "value_incl_setup": sv.get("value_incl_setup"),
+            "higher_is_better": True,
+            "scaling": "weak",
+            "vs_baseline": None,
+            "dtype": "f32",
+            "data": "synthetic",
+            "config": {
+                "workload": sv["workload"], "sessions": args.sessions, "requests": sv["requests"],
+                "rate_sessions_per_s": args.rate, "hbm_budget_gib": args.budget_gib,
+                "precision": "fp32 element/assembly/PCG with fp64 refinement (fp64 SVD, residual "
+                             "and Newton test)",
+                "parallelism": "single GPU",
+            },
+        })
+        if "roofline" in sv:
+            out["roofline"] = sv.pop("roofline")
+        out["serving"] = sv
+    if args.plain_c2 == "on":
+        c2, sc, precond, total, u = plain_c2_form(lib, scenes, args)
+        out["plain_fp64_c2"] = c2
+        if args.precision_form == "on":
+            out["fp32_refine"] = precision_form(lib, sc, precond, args, total, u)
+            out["fp32_refine"]["c3_errors_vs_fp64"] = c3_errors(lib, scenes, args.steps)
+        if args.serving != "on":  # the 40^3 scene is the headline when the stream is off
+            out.update({"value": c2["value"], "unit": "tet-Newton-steps/s", "n_gpus": 1,
+                        "steps": args.steps, "warmup": args.warmup, "ms_per_step": c2["ms_per_step"],
+                        "p50_step_ms": c2["p50_step_ms"], "p99_step_ms": c2["p99_step_ms"],
+                        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                        "dtype": "f64", "data": "synthetic",
+                        "config": {"workload": c2["workload"], "parallelism": "single GPU"},
+                        "roofline": c2["roofline"]})
     if args.batch == "on":
         out["batch"] = batch_form(lib, scenes, args)
-    if args.serving == "on":
-        out["serving"] = serving_form(lib, scenes, args)
     if args.beyond_mall == "on":
         out["beyond_mall"] = beyond_mall_form(lib, scenes, args)
     if args.copy_probe:
+        ctx = lib.Context(0)
         nb = 512 << 20
         ms = ctx.copy_probe(nb, 20)
         out["copy_probe"] = {"bytes": nb, "ms": ms, "GBps": 2 * nb / (ms * 1e-3) / 1e9}
-    inst.close()
-    ctx.close()
+        ctx.close()
     if args.cpu_baseline != "none":
-        cb = cpu_baseline(sc, args.cpu_steps, args.cpu_baseline)
-        if cb:
-            out["cpu_baseline"] = cb
+        if stream is not None and args.cpu_baseline in ("auto", "port"):
+            out["cpu_baseline"] = cpu_baseline_serving(stream, scene_of, args.steps, args.cpu_seconds)
+        if args.plain_c2 == "on":
+            cb = cpu_baseline(sc, args.cpu_steps, args.cpu_baseline)
+            if cb:
+                out["cpu_baseline_c2"] = cb
+                if "cpu_baseline" not in out:
+                    out["cpu_baseline"] = cb
     print(json.dumps(out), flush=True)
     return 0
 

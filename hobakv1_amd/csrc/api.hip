@@ -11,6 +11,8 @@
 //   BACKWARD_EULER_POSITION::solveRayleighDamped
 //                                         src/Timestepper/BACKWARD_EULER_POSITION.cpp:138-327
 //   BDF_1::solveRayleighDamped            src/Timestepper/BDF_1.cpp:78-239
+//   BACKWARD_EULER_VELOCITY::solveRayleighDamped
+//                                         src/Timestepper/BACKWARD_EULER_VELOCITY.cpp:107-235
 //   SIMULATION_SCENE::stepSimulation      src/Scenes/SIMULATION_SCENE.h:120-126
 //
 // Engine.  Every step, batched or not, runs through one tick loop:
@@ -330,8 +332,9 @@ struct hobak_sched {
   std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
   double ms[5] = {0, 0, 0, 0, 0};
   // counters since creation: ticks, PCG groups queued, GPU ms of prologues / PCG / epilogues,
-  // host wall ms inside ticks, solves finished
-  double stat[7] = {0, 0, 0, 0, 0, 0, 0};
+  // host wall ms inside ticks, solves finished, SpMV launches, their algorithmic bytes (those of
+  // the instances solving in each launch's tick)
+  double stat[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   // sampled SpMV launches of the current batch: (iteration index since the batch's prologue, slot)
   std::vector<std::pair<int, int>> samples;
   int sample_it = 0;
@@ -915,6 +918,10 @@ int tick_impl(hobak_sched* S) {
   }
 
   S->stat[1] += nsolving ? m : 0;
+  if (nsolving) {
+    S->stat[7] += (double)m * k;
+    S->stat[8] += (double)m * k * solving_bytes;
+  }
   // 5. statuses
   HIPCHK(ctx, launch_status(M.b[K_ONE], M.status_dev, s));
   HIPCHK(ctx, hipMemcpyAsync(M.status_host, M.status_dev, sizeof(InstStatus) * M.n, hipMemcpyDeviceToHost, s));
@@ -1251,6 +1258,15 @@ const char* hobak_last_error(hobak_ctx* ctx) { return ctx ? ctx->err.c_str() : "
 
 size_t hobak_ctx_bytes_in_use(hobak_ctx* ctx) { return ctx ? ctx->in_use : 0; }
 
+int hobak_check_offsets(int64_t nv, int64_t nslots, int32_t precision) {
+  if (nv <= 0 || nslots <= 0) return HOBAK_E_INVALID;
+  const int64_t ts = precision == HOBAK_PRECISION_FP64 ? 8 : 4;
+  const int64_t rows_pad = (nv + 63) / 64 * 64;  // vectors are padded to whole slices (<= 64 rows)
+  const int64_t lim = (int64_t)1 << 32;
+  if (nslots * ts >= lim || nslots * 4 >= lim || 3 * rows_pad * ts >= lim) return HOBAK_E_INVALID;
+  return HOBAK_OK;
+}
+
 size_t hobak_inst_bytes_estimate(int32_t nv, int32_t nt, int32_t precision) {
   // a Kuhn lattice has ~15 blocks per row; padding of the slices adds a few percent
   const size_t nnzb = (size_t)nv * 16;
@@ -1271,6 +1287,12 @@ int hobak_inst_create(hobak_ctx* ctx, const double* rest_xyz, int32_t nv, const 
     return fail(ctx, HOBAK_E_INVALID, "unknown preconditioner");
   if (params->precision != HOBAK_PRECISION_FP64 && params->precision != HOBAK_PRECISION_FP32_REFINE)
     return fail(ctx, HOBAK_E_INVALID, "unknown precision form");
+  if (params->integrator != HOBAK_INTEGRATOR_POSITION && params->integrator != HOBAK_INTEGRATOR_VELOCITY)
+    return fail(ctx, HOBAK_E_INVALID, "unknown integrator");
+  if (params->integrator == HOBAK_INTEGRATOR_VELOCITY &&
+      (params->precision != HOBAK_PRECISION_FP64 || params->newton_iters > 1 || params->ground_on))
+    return fail(ctx, HOBAK_E_INVALID,
+                "the velocity integrator runs in the FP64 form, one Newton iteration, no penalty ground");
   for (size_t i = 0; i < 4 * (size_t)nt; i++)
     if (tets[i] < 0 || tets[i] >= nv) return fail(ctx, HOBAK_E_INVALID, "tet index out of range");
   for (size_t t = 0; t < (size_t)nt; t++) {
@@ -1357,6 +1379,10 @@ int hobak_inst_create(hobak_ctx* ctx, const double* rest_xyz, int32_t nv, const 
     I->iperm.resize(nv);
     for (int p = 0; p < nv; p++) I->iperm[I->perm[p]] = p;
   }
+  if (hobak_check_offsets(nv, I->P.nslots, params->precision) != HOBAK_OK) {
+    delete I;
+    return fail(ctx, HOBAK_E_INVALID, "mesh too large for the kernels' 32-bit byte offsets");
+  }
   caller_pattern(I);
   const Pattern& P = I->P;
   const size_t need = estimate_bytes(nv, nt, P.nslots, params->precision);
@@ -1391,6 +1417,9 @@ int hobak_inst_create(hobak_ctx* ctx, const double* rest_xyz, int32_t nv, const 
   for (int c = 0; c < 3; c++) h.g[c] = params->gravity[c];
   h.ground_y = params->ground_y;
   h.ground_k = params->ground_k;
+  h.velform = params->integrator == HOBAK_INTEGRATOR_VELOCITY;
+  h.dt = params->dt;
+  h.kscale = h.velform ? params->dt * params->dt : 1.0;
   h.tol = params->pcg_tol > 0 ? params->pcg_tol : kEps;
   h.tau0 = params->refine_inner_tol > 0 ? params->refine_inner_tol : 1e-4;
   const size_t ts = I->tsize;
@@ -1634,6 +1663,8 @@ int hobak_inst_set_damping(hobak_inst* I, int32_t model, double mu) {
     return fail(ctx, HOBAK_E_INVALID, "energy damping needs one Newton iteration per step (the "
                                       "reference's BDF_1 has no energy-damped solve)");
   if (model == HOBAK_DAMPING_GREEN && !std::isfinite(mu)) return fail(ctx, HOBAK_E_INVALID, "damping mu must be finite");
+  if (model == HOBAK_DAMPING_GREEN && I->h.velform)
+    return fail(ctx, HOBAK_E_INVALID, "energy damping is built for the position integrator only");
   (void)hipSetDevice(ctx->device);
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   InstDev& h = I->h;

@@ -34,8 +34,15 @@ HD T det3t(const T m[3][3]) {
          m[0][2] * (m[1][0] * m[2][1] - m[1][1] * m[2][0]);
 }
 
+// The SVD below rounds every product and sum on its own (no fused multiply-add), in the order of
+// Eigen's JacobiSVD.  Its result at a matrix with repeated singular values (an improper rotation
+// has sigma = (1, 1, 1) up to sign) is one of infinitely many valid factorizations, chosen by
+// round-off; ARAP's PK1 and stiffness depend on that choice, so only identical rounding reproduces
+// the reference's values there.
+
 // Jacobi rotation that diagonalises the symmetric 2x2 [[x, y], [y, z]] (Jacobi/Jacobi.h:83-113).
 HD void make_jacobi(double x, double y, double z, double& c, double& s) {
+#pragma clang fp contract(off)
   const double deno = 2.0 * fabs(y);
   if (deno < DBL_MIN) {
     c = 1.0;
@@ -55,6 +62,7 @@ HD void make_jacobi(double x, double y, double z, double& c, double& s) {
 // misc/RealSvd2x2.h:19-51).  Returns whether the plane was still off-diagonal.
 template <int P, int Q>
 HD bool jacobi_pair(double W[3][3], double U[3][3], double V[3][3], double& maxDiag) {
+#pragma clang fp contract(off)
   const double thr = hmax(DBL_MIN, 2.0 * DBL_EPSILON * maxDiag);
   if (!(fabs(W[P][Q]) > thr || fabs(W[Q][P]) > thr)) return false;
   double m00 = W[P][P], m01 = W[P][Q], m10 = W[Q][P], m11 = W[Q][Q];
@@ -127,6 +135,7 @@ HD void swap_cols(double S[3], double U[3][3], double V[3][3]) {
 // F = U diag(S) V^T with S descending and U, V orthogonal (Eigen JacobiSVD semantics), then the
 // svd_rv fix-up that folds a reflection into S[2].
 HD void svd_rv(const double F[3][3], double U[3][3], double S[3], double V[3][3]) {
+#pragma clang fp contract(off)
   double scale = 0.0;
 #pragma unroll
   for (int i = 0; i < 3; i++)

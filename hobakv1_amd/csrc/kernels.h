@@ -163,6 +163,12 @@ struct InstDev {
   int damping;
   double dmu;
   const double* fdot;
+  // BACKWARD_EULER_VELOCITY (velform != 0): the solve is for the velocity change.  The system is
+  // A = M - dt^2 K (kscale = dt^2 scales the assembled stiffness; 1 in the position form, where
+  // A = M/dt^2 - C/dt - K), b = dt (R + dt K v + f_ext), with K v taken from the unprojected
+  // stiffness the assembly also writes into E.
+  int velform;
+  double dt, kscale;
 };
 
 // The control block lives in the same allocation as the record, kCtlOffset bytes after it, so a

@@ -283,6 +283,9 @@ __global__ __launch_bounds__(kBlock) void contact_begin_kernel(Batch b) {
       for (int k = 0; k < 3; k++) {
         cw[k] = __dadd_rn(__dmul_rn(cb.s, cl[k]), cb.c[k]);
         tg[k] = __dsub_rn(cw[k], x[k]);
+        // velocity form: the target is the velocity change (cl - x)/dt - v
+        // (BACKWARD_EULER_VELOCITY.cpp:51-78)
+        if (I->velform) tg[k] = __dsub_rn(__ddiv_rn(tg[k], I->dt), gbl(I->v)[3 * (size_t)r + k]);
       }
     }
     double S[3][3];
@@ -344,7 +347,8 @@ __global__ __launch_bounds__(kBlock) void contact_sep_kernel(Batch b) {
 
 // End of a step for an instance with plane constraints: separating constraints go (when any
 // does), the others take the closest point and normal at this step's positions, and
-// v = S (u - u_old) / dt with S rebuilt only if a constraint went away.
+// v = S (u - u_old) / dt with S rebuilt only if a constraint went away; in the velocity form
+// v = S v with this step's S.
 __device__ void contact_end(const HOBAK_GLOBAL InstDev* I, int r, bool changed) {
   double x[3];
   step_positions(I, r, x);
@@ -353,6 +357,8 @@ __device__ void contact_end(const HOBAK_GLOBAL InstDev* I, int r, bool changed) 
   int mode = gbl(I->fmode)[r];
   double n[3];
   for (int k = 0; k < 3; k++) n[k] = gbl(I->fnrm)[3 * (size_t)r + k];
+  const int step_mode = mode;
+  const double step_n[3] = {n[0], n[1], n[2]};
   if (o >= 0) {
     if (changed && gbl(I->pc_sep)[r]) {
       gbl(I->pc_obj)[r] = -1;
@@ -366,6 +372,14 @@ __device__ void contact_end(const HOBAK_GLOBAL InstDev* I, int r, bool changed) 
       }
       if (changed) unit_normal(nl, n);
     }
+  }
+  if (I->velform) {
+    // v = S v with this step's filter, before the constraints change (BACKWARD_EULER_VELOCITY.cpp:200)
+    double S[3][3], w[3];
+    filter_block(step_mode, step_n, S);
+    for (int k = 0; k < 3; k++) w[k] = gbl(I->v)[3 * (size_t)r + k];
+    for (int a = 0; a < 3; a++) gbl(I->v)[3 * (size_t)r + a] = dot3_rn(S[a], w);
+    return;
   }
   if (changed) mode = pin ? 2 : (o >= 0 ? 1 : 0);
   double S[3][3], w[3];
@@ -405,7 +419,9 @@ __global__ __launch_bounds__(kBlock) void begin_step_kernel(Batch b) {
   }
 }
 
-// _position += xDelta (BACKWARD_EULER_POSITION.cpp:262-266)
+// _position += xDelta (BACKWARD_EULER_POSITION.cpp:262-266).  Velocity form
+// (BACKWARD_EULER_VELOCITY.cpp:190-194): vDelta = y + z, v += vDelta, u += dt v; the S filter of
+// v follows in end_step.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void apply_dx_kernel(Batch b, int use64) {
   for (int ws = blockIdx.x; ws < b.total; ws += gridDim.x) {
@@ -415,12 +431,20 @@ __global__ __launch_bounds__(kBlock) void apply_dx_kernel(Batch b, int use64) {
     const int i = lb * kBlock + threadIdx.x;
     if (i < 3 * I->nv) {
       const double dx = use64 ? gbl(I->x64)[i] : (double)(gbl((const T*)I->x))[i];
-      gbl(I->u)[i] += I->contact ? dx + gbl(I->zc)[i] : dx;  // xDelta = y + z
+      const double d = I->contact ? dx + gbl(I->zc)[i] : dx;  // y + z
+      if (I->velform) {
+        const double vn = gbl(I->v)[i] + d;
+        gbl(I->v)[i] = vn;
+        gbl(I->u)[i] = __dadd_rn(gbl(I->u)[i], __dmul_rn(I->dt, vn));
+      } else {
+        gbl(I->u)[i] += d;
+      }
     }
   }
 }
 
-// _velocity = (u - u_old)/dt, then S-filtered (BACKWARD_EULER_POSITION.cpp:300-314)
+// _velocity = (u - u_old)/dt, then S-filtered (BACKWARD_EULER_POSITION.cpp:300-314); the velocity
+// form filters its updated v with this step's S (BACKWARD_EULER_VELOCITY.cpp:200)
 __global__ __launch_bounds__(kBlock) void end_step_kernel(Batch b) {
   for (int ws = blockIdx.x; ws < b.total; ws += gridDim.x) {
     const int vb = xcd_vb(ws, b.total);
@@ -436,10 +460,11 @@ __global__ __launch_bounds__(kBlock) void end_step_kernel(Batch b) {
     if (r < I->nv) {
       const double invDt = I->invDt;
       const bool pin = gbl(I->pinned)[r];
+      const bool vel = I->velform != 0;  // v already holds v + vDelta: only S filters it
 #pragma unroll
       for (int c = 0; c < 3; c++) {
         const size_t i = 3 * (size_t)r + c;
-        gbl(I->v)[i] = pin ? 0.0 : invDt * (gbl(I->u)[i] - gbl(I->u_old)[i]);
+        gbl(I->v)[i] = pin ? 0.0 : vel ? gbl(I->v)[i] : invDt * (gbl(I->u)[i] - gbl(I->u_old)[i]);
       }
     }
   }
@@ -533,16 +558,33 @@ __global__ __launch_bounds__(kBlock) void assemble_kernel(Batch b, int target) {
     }
     auto D = gbl((const T*)I->D);
     auto out = gbl((T*)I->A);
+    // velocity form: A = M - dt^2 K (BACKWARD_EULER_VELOCITY.cpp:161), with K = -acc; the
+    // unprojected stiffness also goes to E for b's K v
+    const bool vel = I->velform != 0;
+    const T ks = (T)I->kscale;
     T val[9];
 #pragma unroll
     for (int e = 0; e < 9; e++) {
-      val[e] = pin ? ((c == r && (e % 4) == 0) ? T(1) : T(0)) : D[e * ns + slot] + acc[e];
+      const T a = vel ? D[e * ns + slot] + ks * acc[e] : D[e * ns + slot] + acc[e];
+      val[e] = pin ? ((c == r && (e % 4) == 0) ? T(1) : T(0)) : a;
       out[e * ns + slot] = val[e];
     }
     if (mslot >= 0) {
       const bool pin_m = contact ? false : kDiff ? pin_c : (pin_r || pin_c);
 #pragma unroll
-      for (int e = 0; e < 9; e++) out[e * ns + mslot] = pin_m ? T(0) : D[e * ns + mslot] + acc[kTr[e]];
+      for (int e = 0; e < 9; e++) {
+        const T a = vel ? D[e * ns + mslot] + ks * acc[kTr[e]] : D[e * ns + mslot] + acc[kTr[e]];
+        out[e * ns + mslot] = pin_m ? T(0) : a;
+      }
+    }
+    if (vel) {
+      auto kst = gbl((T*)I->E);
+#pragma unroll
+      for (int e = 0; e < 9; e++) kst[e * ns + slot] = acc[e];
+      if (mslot >= 0) {
+#pragma unroll
+        for (int e = 0; e < 9; e++) kst[e * ns + mslot] = acc[kTr[e]];
+      }
     }
     if (c == r && !contact) {
       write_precond<T>(val, gbl((T*)I->minv), r, I->precond);
@@ -559,7 +601,8 @@ __global__ __launch_bounds__(kBlock) void assemble_kernel(Batch b, int target) {
 
 // D = M/dt^2 - C/dt, E = M/dt - C, C = alpha M + beta K_rest, from the assembled rest stiffness
 // (-K_rest in kneg).  Order of operations as BACKWARD_EULER_POSITION.cpp:198, 208 and
-// TIMESTEPPER.cpp:232-233.
+// TIMESTEPPER.cpp:232-233.  Velocity form: C is emptied before the system is formed
+// (BACKWARD_EULER_VELOCITY.cpp:153, TIMESTEPPER.cpp:657), so D = M and C = E = 0.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void constant_parts_kernel(Batch b) {
   for (int ws = blockIdx.x; ws < b.total; ws += gridDim.x) {
@@ -580,11 +623,17 @@ __global__ __launch_bounds__(kBlock) void constant_parts_kernel(Batch b) {
       const bool dg = (c == r) && (e % 4 == 0);
       const double m = dg ? gbl(I->mass)[3 * (size_t)r + e / 4] : 0.0;
       const double cc = dg ? ra * m + rb * krest : rb * krest;
-      const double d = dg ? m * invDt2 - cc * invDt : -(cc * invDt);
-      const double ee = dg ? invDt * m - cc : -cc;
+      double d = dg ? m * invDt2 - cc * invDt : -(cc * invDt);
+      double ee = dg ? invDt * m - cc : -cc;
+      double cw = cc;
+      if (I->velform) {  // C is emptied before A = M - dt C - dt^2 K is formed: D = M
+        d = m;
+        ee = 0.0;
+        cw = 0.0;
+      }
       (gbl((T*)I->D))[e * ns + slot] = (T)d;
       (gbl((T*)I->E))[e * ns + slot] = (T)ee;
-      (gbl((T*)I->C))[e * ns + slot] = (T)cc;
+      (gbl((T*)I->C))[e * ns + slot] = (T)cw;
     }
   }
 }
@@ -605,8 +654,9 @@ __device__ __forceinline__ void apply_precond(const HOBAK_GLOBAL T* minv, int pr
   }
 }
 
-// Per block row: R = ascending-tet gather of staged forces; b = (E v + R) + f_ext (BEP) or
-// b = C vb + ((M a + R) + f_ext) (BDF_1); rhs = S b.  With init_cg also x = 0, r = rhs,
+// Per block row: R = ascending-tet gather of staged forces; b = (E v + R) + f_ext (BEP),
+// b = C vb + ((M a + R) + f_ext) (BDF_1) or b = dt (R + dt K v + f_ext) (the velocity form, with
+// E holding -K); rhs = S b.  With init_cg also x = 0, r = rhs,
 // z = M^-1 r; the partial sums of |rhs|^2 and r.z seed Eigen's CG (ConjugateGradient.h:43-66).
 // Row r of an instance with plane constraints (FP64 form): b (unfiltered, kept for the
 // separation test) becomes S (b - A z), and the row's blocks of the unprojected A become
@@ -729,6 +779,10 @@ __global__ __launch_bounds__(kBlock) void rhs_kernel(Batch b, int init_cg) {
         if (bdf) {
           const double acc = -invDt2 * (u[i] - uOld[i]) + invDt * vOld[i];
           bv[c] = s[c] + (((gbl(I->mass)[i] * acc + R[c]) + fext));
+        } else if (I->velform) {
+          // b = dt (R + dt K v + f_ext) (BACKWARD_EULER_VELOCITY.cpp:157); s = -K v here
+          const double dt = I->dt;
+          bv[c] = __dmul_rn(dt, __dadd_rn(__dsub_rn(R[c], __dmul_rn(dt, s[c])), fext));
         } else {
           bv[c] = (s[c] + R[c]) + fext;
         }

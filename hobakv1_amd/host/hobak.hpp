@@ -9,6 +9,7 @@
 //   HYPERELASTIC::computeMu / computeLambda       src/Hyperelastic/Volume/HYPERELASTIC.cpp:111-122
 //   CUBE(center, scale)                           src/Geometry/CUBE.cpp:27-55
 //   BACKWARD_EULER_POSITION(mesh, material)       src/Timestepper/BACKWARD_EULER_POSITION.h:36
+//   BACKWARD_EULER_VELOCITY(mesh, material)       src/Timestepper/BACKWARD_EULER_VELOCITY.h:35
 //     solve(verbose)                              src/Timestepper/TIMESTEPPER.h:77
 //     position(), velocity()                      src/Timestepper/TIMESTEPPER.h:44-58
 //     addGravity, attachKinematicConstraints, addKinematicCollisionObject, setDt, setRayeligh,
@@ -197,12 +198,7 @@ class CUBE {
 class BACKWARD_EULER_POSITION {
  public:
   BACKWARD_EULER_POSITION(TET_MESH& mesh, VOLUME::HYPERELASTIC& material, Context* ctx = nullptr)
-      : mesh_(mesh), material_(material), ctx_(ctx ? ctx : &Context::global()) {
-    hobak_default_params(&prm_);
-    prm_.precond = HOBAK_PRECOND_BLOCK_JACOBI;
-    position_.assign(mesh.DOFs(), 0.0);
-    velocity_.assign(mesh.DOFs(), 0.0);
-  }
+      : BACKWARD_EULER_POSITION(mesh, material, ctx, HOBAK_INTEGRATOR_POSITION) {}
   ~BACKWARD_EULER_POSITION() { hobak_inst_destroy(inst_); }
   BACKWARD_EULER_POSITION(const BACKWARD_EULER_POSITION&) = delete;
   BACKWARD_EULER_POSITION& operator=(const BACKWARD_EULER_POSITION&) = delete;
@@ -282,6 +278,16 @@ class BACKWARD_EULER_POSITION {
   }
   const hobak_step_stats& lastStats() const { return stats_; }
 
+ protected:
+  BACKWARD_EULER_POSITION(TET_MESH& mesh, VOLUME::HYPERELASTIC& material, Context* ctx, int integrator)
+      : mesh_(mesh), material_(material), ctx_(ctx ? ctx : &Context::global()) {
+    hobak_default_params(&prm_);
+    prm_.precond = HOBAK_PRECOND_BLOCK_JACOBI;
+    prm_.integrator = integrator;
+    position_.assign(mesh.DOFs(), 0.0);
+    velocity_.assign(mesh.DOFs(), 0.0);
+  }
+
  private:
   void ensure() {
     if (inst_) return;
@@ -330,6 +336,14 @@ class BACKWARD_EULER_POSITION {
   std::vector<const CUBE*> collision_;
   const VOLUME::GREEN_DAMPING* damping_ = nullptr;
   long fdotApplied_ = 0;
+};
+
+// BACKWARD_EULER_VELOCITY(mesh, material) (src/Timestepper/BACKWARD_EULER_VELOCITY.h:35, .cpp:31-46):
+// the same surface, solving for the velocity change (BACKWARD_EULER_VELOCITY.cpp:107-235).
+class BACKWARD_EULER_VELOCITY : public BACKWARD_EULER_POSITION {
+ public:
+  BACKWARD_EULER_VELOCITY(TET_MESH& mesh, VOLUME::HYPERELASTIC& material, Context* ctx = nullptr)
+      : BACKWARD_EULER_POSITION(mesh, material, ctx, HOBAK_INTEGRATOR_VELOCITY) {}
 };
 
 }  // namespace hobak

@@ -6,6 +6,8 @@ surface for this path:
     VOLUME::SNH / VOLUME::ARAP    src/Hyperelastic/Volume/    SNH(mu, lam) / ARAP(mu, lam)
     BACKWARD_EULER_POSITION       src/Timestepper/BACKWARD_EULER_POSITION.h:36
                                                                BackwardEulerPosition(mesh, material)
+    BACKWARD_EULER_VELOCITY       src/Timestepper/BACKWARD_EULER_VELOCITY.h:35
+                                                               BackwardEulerVelocity(mesh, material)
       .solve(verbose) / position() / velocity() / addGravity / attachKinematicConstraints /
       setDt / setRayleigh / scaleMass                          same names
 
@@ -52,6 +54,7 @@ class hobak_step_params(ctypes.Structure):
         ("newton_iters", ctypes.c_int32), ("newton_tol", ctypes.c_double),
         ("precision", ctypes.c_int32), ("refine_inner_tol", ctypes.c_double),
         ("ground_on", ctypes.c_int32), ("ground_y", ctypes.c_double), ("ground_k", ctypes.c_double),
+        ("integrator", ctypes.c_int32),
     ]
 
 
@@ -95,6 +98,7 @@ SIGNATURES = {
                                          ctypes.POINTER(hobak_material),
                                          ctypes.POINTER(hobak_step_params), ctypes.POINTER(_vp)]),
     "hobak_inst_bytes_estimate": (ctypes.c_size_t, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
+    "hobak_check_offsets": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int64, ctypes.c_int32]),
     "hobak_inst_destroy": (None, [_vp]),
     "hobak_inst_set_pinned": (ctypes.c_int, [_vp, _ip, ctypes.c_int32, _dp]),
     "hobak_inst_add_collision_cube": (ctypes.c_int, [_vp, _dp, ctypes.c_double]),
@@ -290,6 +294,7 @@ def make_params(scene, precond=_scenes.PRECOND_BLOCK_JACOBI, precision=PRECISION
         p.ground_on = 1
         p.ground_y = scene.ground_y
         p.ground_k = scene.ground_k
+    p.integrator = getattr(scene, "integrator", _scenes.INTEGRATOR_POSITION)
     return p
 
 
@@ -477,11 +482,13 @@ class Scheduler:
         return {"spmv_avg_ms": ms.value, "spmv_count": n.value, "spmv_bytes": b.value}
 
     def stats(self):
-        out = np.zeros(7)
+        out = np.zeros(9)
         self.ctx.check(self.lib.hobak_sched_stats(self.h, _d(out)), "sched_stats")
         keys = ["ticks", "pcg_groups", "prologue_gpu_ms", "pcg_gpu_ms", "epilogue_gpu_ms",
-                "tick_host_ms", "solves"]
-        return dict(zip(keys, out.tolist()))
+                "tick_host_ms", "solves", "spmv_launches", "spmv_bytes"]
+        d = dict(zip(keys, out.tolist()))
+        d["spmv_bytes_per_launch"] = d["spmv_bytes"] / d["spmv_launches"] if d["spmv_launches"] else 0.0
+        return d
 
     def tick(self):
         """Runs one tick; returns (completions, requests still in flight)."""
@@ -666,10 +673,13 @@ class BackwardEulerPosition:
         """Vertex ids of the live plane constraints (TIMESTEPPER.h:46), ascending."""
         return np.zeros(0, np.int32) if self._inst is None else self._inst.plane_constraints()
 
+    _integrator = _scenes.INTEGRATOR_POSITION
+
     def _ensure(self):
         if self._inst is not None:
             return
         sc = _scenes.Scene(name="facade", rest=self.mesh.restVertices(), tets=self.mesh.tets(),
+                           integrator=self._integrator,
                            material=self.material.kind, mu=self.material.mu,
                            lam=self.material.lam, dt=self._dt, rayleigh_alpha=self._alpha,
                            rayleigh_beta=self._beta, mass_scale=self._mass_scale,
@@ -696,3 +706,11 @@ class BackwardEulerPosition:
         self._position[:] = u
         self._velocity[:] = v
         return True
+
+
+class BackwardEulerVelocity(BackwardEulerPosition):
+    """BACKWARD_EULER_VELOCITY(tetMesh, hyperelastic) (src/Timestepper/BACKWARD_EULER_VELOCITY.h:35,
+    .cpp:31-46): the same setters and solve(), solving for the velocity change
+    (BACKWARD_EULER_VELOCITY.cpp:107-235)."""
+
+    _integrator = _scenes.INTEGRATOR_VELOCITY

@@ -34,6 +34,11 @@ MAT_ARAP = 1
 PRECOND_JACOBI = 0
 PRECOND_BLOCK_JACOBI = 1
 
+# TIMESTEPPER::BACKWARD_EULER_POSITION (src/Timestepper/BACKWARD_EULER_POSITION.cpp) or
+# TIMESTEPPER::BACKWARD_EULER_VELOCITY (src/Timestepper/BACKWARD_EULER_VELOCITY.cpp)
+INTEGRATOR_POSITION = 0
+INTEGRATOR_VELOCITY = 1
+
 _GOLDEN = np.uint64(0x9E3779B97F4A7C15)
 _M1 = np.uint64(0xBF58476D1CE4E5B9)
 _M2 = np.uint64(0x94D049BB133111EB)
@@ -185,6 +190,7 @@ class Scene:
     collision_cubes: tuple = ()        # ((cx, cy, cz, scale), ...) kinematic collision CUBEs
     green_mu: Optional[float] = None   # GREEN_DAMPING(mu) energy damping in place of Rayleigh
     fdot_v: Optional[np.ndarray] = None  # (3 nv) velocity given to computeFdots before stepping
+    integrator: int = 0                # INTEGRATOR_POSITION or INTEGRATOR_VELOCITY
     dims: tuple = ()
 
     @property
@@ -294,6 +300,25 @@ def drop_test(tobj_path, pcg_tol=1e-8):
     return sc
 
 
+def separation_test(tobj_path, pcg_tol=1e-8):
+    """The reference's collision separation test (projects/unitTests/SCENE_TESTS.h:367-453) on
+    the velocity integrator: data/cube_6.tobj, SNH(mu = 1, lambda = 10), BACKWARD_EULER_VELOCITY
+    with self-collisions off, the floor CUBE(center (0, -5, 0), scale 10) as a kinematic collision
+    object, every vertex moving down at 3, gravity (0, -1, 0), 42 steps; afterwards every surface
+    vertex must be above y = 1e-4 (the cube bounced off the floor).  The test keeps Eigen's
+    default PCG tolerance (machine epsilon); the workloads' 1e-8 is the default here."""
+    from .formats import read_tobj
+
+    verts, tets = read_tobj(tobj_path)
+    sc = Scene(name="separation_test", rest=verts, tets=tets, material=MAT_SNH, mu=1.0, lam=10.0,
+               mass_scale=1.0, gravity=(0.0, -1.0, 0.0), pin_cube=None, pcg_tol=pcg_tol,
+               collision_cubes=((0.0, -5.0, 0.0, 10.0),), integrator=INTEGRATOR_VELOCITY)
+    v = np.zeros((verts.shape[0], 3))
+    v[:, 1] = -3.0
+    sc.v0 = v.ravel().copy()
+    return sc
+
+
 def green_scene(n=(8, 8, 8), seed=11, E=1e5, nu=0.45, material=MAT_SNH, green_frac=0.02,
                 rate=1.0, name=None):
     """The pinned lattice with GREEN_DAMPING(green_frac * mu) in place of Rayleigh damping
@@ -330,6 +355,14 @@ def two_tet_drop(tobj_path, E=3.0, name="two_tet_drop"):
     return Scene(name=name, rest=verts, tets=tets, material=MAT_SNH, mu=m, lam=l, dt=1.0 / 30.0,
                  mass_scale=1.0, gravity=(0.0, -0.3, 0.0), pin_cube=(0.0, -4.9, 0.0, 10.0),
                  pcg_tol=0.0, collision_cubes=((0.0, -4.9, 0.0, 10.0),))
+
+
+def config1_velocity(seed=1):
+    """The config-1 scene (10^3 pinned SNH cube) on BACKWARD_EULER_VELOCITY."""
+    sc = config1(seed)
+    sc.integrator = INTEGRATOR_VELOCITY
+    sc.name = "config1_velocity"
+    return sc
 
 
 def config1(seed=1):

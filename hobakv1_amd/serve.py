@@ -16,7 +16,15 @@ freeing its memory.
 
 Clock: instance set-up (host mesh preprocessing, uploads, the rest-state stiffness) and tear-down
 run while the GPU is idle between ticks; both loops exclude them from their clocks and report
-their total separately, so the throughput figures compare the stepping paths.
+their total separately, so the throughput figures compare the stepping paths.  The end-to-end
+rate, which charges that set-up time as well, is reported beside it (`value_incl_setup`).
+
+Failure isolation: a step whose solve produced a NaN or an infinity (HOBAK_E_NONFINITE) ends its
+session at once.  The session is quarantined: its remaining requests are dropped, its instance
+is destroyed (freeing its budget), and it is listed under `failed`.  Every other session runs
+on unaffected, because no kernel lets one instance's values reach another's.  A step that hits
+the PCG iteration limit (HOBAK_E_NOT_CONVERGED) has still advanced the state, as the reference's
+does, so its session continues and the step is counted in `nonzero_status`.
 
     python -m hobakv1_amd.serve --sessions 32 --rate 100 --steps 20
 """
@@ -33,6 +41,8 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import lib, scenes
+
+NONFINITE = -5  # HOBAK_E_NONFINITE
 
 
 @dataclass
@@ -88,7 +98,8 @@ def _admit(ctx, waiting, live, budget, precision, scene_of, clock, make_inst):
         waiting.popleft()
         with clock.paused():
             inst = make_inst(sc)
-        live[s.k] = {"inst": inst, "left": s.steps, "scene": sc, "submit": clock.now()}
+        live[s.k] = {"inst": inst, "left": s.steps, "steps": s.steps, "scene": sc,
+                     "submit": clock.now()}
     return waits
 
 
@@ -108,6 +119,7 @@ def run(ctx, stream, scene_of, budget=0, precision=lib.PRECISION_FP32_REFINE,
     final = {}
     iters = []
     statuses = []
+    failed = []
 
     def make_inst(sc):
         return lib.Instance(ctx, sc, precond=precond, precision=precision)
@@ -117,14 +129,16 @@ def run(ctx, stream, scene_of, budget=0, precision=lib.PRECISION_FP32_REFINE,
         sch.set_kernel_sampling(sample_every)
     rr = deque()  # plain mode: round-robin queue of session ids with a pending request
 
-    def finish(k):
+    def finish(k, quarantine=False):
         ent = live.pop(k)
         with clock.paused():
-            if keep_states:
-                final[k] = ent["inst"].state()[0]
+            if keep_states and not quarantine:
+                final[k] = ent["inst"].state()
             if sch is not None:
                 sch.detach(ent["inst"])
             ent["inst"].close()
+        if quarantine:
+            failed.append({"session": k, "steps_done": ent["steps"] - ent["left"]})
 
     while pending or waiting or live:
         now = clock.now()
@@ -150,11 +164,14 @@ def run(ctx, stream, scene_of, budget=0, precision=lib.PRECISION_FP32_REFINE,
             for c in comps:
                 k = int(c["tag"])
                 ent = live[k]
+                statuses.append(c["stats"]["status"])
+                ent["left"] -= 1
+                if c["stats"]["status"] == NONFINITE:
+                    finish(k, quarantine=True)
+                    continue
                 lat.append(now - ent["submit"])
                 tet_steps += ent["scene"].nt * c["stats"]["newton_iters"]
                 iters.append(c["stats"]["total_pcg_iters"])
-                statuses.append(c["stats"]["status"])
-                ent["left"] -= 1
                 if ent["left"] > 0:
                     ent["submit"] = now
                     sch.submit(ent["inst"], k)
@@ -163,7 +180,15 @@ def run(ctx, stream, scene_of, budget=0, precision=lib.PRECISION_FP32_REFINE,
         else:
             k = rr.popleft()
             ent = live[k]
-            st = ent["inst"].step(allow_unconverged=True)
+            try:
+                st = ent["inst"].step(allow_unconverged=True)
+            except lib.HobakError as e:
+                if e.code != NONFINITE:
+                    raise
+                statuses.append(NONFINITE)
+                ent["left"] -= 1
+                finish(k, quarantine=True)
+                continue
             now = clock.now()
             lat.append(now - ent["submit"])
             tet_steps += ent["scene"].nt * st["newton_iters"]
@@ -188,6 +213,8 @@ def run(ctx, stream, scene_of, budget=0, precision=lib.PRECISION_FP32_REFINE,
         "precision": "fp32_refine" if precision == lib.PRECISION_FP32_REFINE else "fp64",
         "value": tet_steps / makespan if makespan > 0 else 0.0,
         "unit": "tet-Newton-steps/s",
+        "value_incl_setup": (tet_steps / (makespan + clock.excluded)
+                             if makespan + clock.excluded > 0 else 0.0),
         "makespan_s": makespan,
         "setup_s": clock.excluded,
         "requests": len(lat),
@@ -198,6 +225,7 @@ def run(ctx, stream, scene_of, budget=0, precision=lib.PRECISION_FP32_REFINE,
         "nonzero_status": int(sum(1 for s in statuses if s != 0)),
         "max_concurrent_sessions": max_live,
         "admission_waits": admission_waits,
+        "failed": failed,
         "engine": engine,
         "spmv": spmv,
         "final": final,
@@ -206,14 +234,21 @@ def run(ctx, stream, scene_of, budget=0, precision=lib.PRECISION_FP32_REFINE,
 
 def compare(ctx_batched, ctx_plain, stream, scene_of, budget, sample_every=0):
     """Serving (FP32 + refinement, batched) and the plain FP64 path on the same stream; the
-    position error of every session after its last step, against the plain FP64 result."""
+    position error of every session after its last step, against the plain FP64 result.  With
+    ctx_plain None only the serving side runs."""
     sv = run(ctx_batched, stream, scene_of, budget, lib.PRECISION_FP32_REFINE, mode="batched",
              sample_every=sample_every)
+    if ctx_plain is None:
+        sv.pop("final")
+        return {**sv, "sessions": len(stream), "steps_per_session": stream[0].steps if stream else 0,
+                "hbm_budget_bytes": budget}
     pl = run(ctx_plain, stream, scene_of, budget, lib.PRECISION_FP64, mode="plain")
     errs = []
-    for k, u in sv["final"].items():
+    for k, (u, _) in sv["final"].items():
+        if k not in pl["final"]:
+            continue
         X = scene_of(k).rest.ravel()
-        u64 = pl["final"][k]
+        u64 = pl["final"][k][0]
         errs.append(float(np.linalg.norm(u - u64) / np.linalg.norm(X + u64)))
     sv.pop("final")
     pl.pop("final")

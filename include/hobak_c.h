@@ -8,6 +8,7 @@
  *   TET_MESH_FASTER(restVertices, tets)              src/Geometry/TET_MESH_FASTER.h:28-32
  *   VOLUME::SNH(mu, lambda), VOLUME::ARAP(mu, lambda) src/Hyperelastic/Volume/SNH.h:28, ARAP.h:28
  *   BACKWARD_EULER_POSITION(mesh, material)          src/Timestepper/BACKWARD_EULER_POSITION.h:36
+ *   BACKWARD_EULER_VELOCITY(mesh, material)          src/Timestepper/BACKWARD_EULER_VELOCITY.h:35
  *   TIMESTEPPER::solve(verbose)                      src/Timestepper/TIMESTEPPER.h:77
  *   position() / velocity() / externalForces()       src/Timestepper/TIMESTEPPER.h:44-58
  *   addGravity, attachKinematicConstraints, setDt,   src/Timestepper/TIMESTEPPER.h:70-100
@@ -57,6 +58,19 @@ extern "C" {
 
 #define HOBAK_MAX_NEWTON 8
 
+/* The time integrator (hobak_step_params.integrator).
+ * POSITION: BACKWARD_EULER_POSITION::solveRayleighDamped (BACKWARD_EULER_POSITION.cpp:138-327),
+ *   solving for the position change; BDF_1 with newton_iters > 1.
+ * VELOCITY: BACKWARD_EULER_VELOCITY::solveRayleighDamped (BACKWARD_EULER_VELOCITY.cpp:107-235),
+ *   solving (M - dt^2 K) dv = dt (R + dt K v + f_ext) for the velocity change, then
+ *   v += dv, u += dt v, v = S v.  The reference empties its Rayleigh matrix C before forming
+ *   the system (:153 -> TIMESTEPPER.cpp:657), so the step carries no Rayleigh damping; this
+ *   library computes the same.  Plane constraints take velocity targets (cl - x)/dt - v
+ *   (:51-78).  FP64 form, one Newton iteration, Rayleigh damping model, no penalty ground;
+ *   other combinations are refused with HOBAK_E_INVALID. */
+#define HOBAK_INTEGRATOR_POSITION 0
+#define HOBAK_INTEGRATOR_VELOCITY 1
+
 typedef struct hobak_ctx hobak_ctx;
 typedef struct hobak_inst hobak_inst;
 typedef struct hobak_sched hobak_sched;
@@ -86,6 +100,7 @@ typedef struct {
   int32_t ground_on;      /* penalty ground plane y = ground_y, stiffness ground_k (no reference counterpart) */
   double ground_y;
   double ground_k;
+  int32_t integrator;     /* HOBAK_INTEGRATOR_* (default POSITION) */
 } hobak_step_params;
 
 typedef struct {
@@ -115,6 +130,12 @@ int hobak_inst_create(hobak_ctx* ctx, const double* rest_xyz, int32_t nv, const 
                       hobak_inst** out);
 /* Device bytes hobak_inst_create would allocate for this mesh (admission control). */
 size_t hobak_inst_bytes_estimate(int32_t nv, int32_t nt, int32_t precision);
+/* HOBAK_OK when a mesh of nv vertices whose padded block matrix holds nslots slots stays within
+ * the 32-bit byte offsets the PCG kernels address the matrix planes and the vectors with (every
+ * slot of one value plane, and every vertex's three entries of a padded vector); else
+ * HOBAK_E_INVALID, which hobak_inst_create then returns.  Host-only: needs no device.  No
+ * counterpart in the reference, whose Eigen types index with 64-bit integers. */
+int hobak_check_offsets(int64_t nv, int64_t nslots, int32_t precision);
 void hobak_inst_destroy(hobak_inst* inst);
 
 /* attachKinematicConstraints: pinned vertex ids (ascending) and the displacement each is held at
@@ -275,8 +296,10 @@ int hobak_sched_tick(hobak_sched* sched, hobak_completion* out, int32_t max_out,
                      int32_t* in_flight);
 /* The scheduler clock (seconds since hobak_sched_create). */
 double hobak_sched_clock(hobak_sched* sched);
-/* Counters since creation, 7 doubles: ticks, PCG groups queued, GPU milliseconds of prologues,
- * of PCG groups and of epilogues, host milliseconds inside hobak_sched_tick, solves finished. */
+/* Counters since creation, 9 doubles: ticks, PCG groups queued, GPU milliseconds of prologues,
+ * of PCG groups and of epilogues, host milliseconds inside hobak_sched_tick, solves finished,
+ * SpMV launches queued, and their algorithmic bytes summed (per launch: the matrices and vectors
+ * of the instances solving in its tick, nnzb (9 s + 4) + 4 (nv + 1) + 2 * 3 s nv each). */
 int hobak_sched_stats(hobak_sched* sched, double* out);
 /* Measurement hook of a scheduler: every `every`-th SpMV launch of its PCG groups is bracketed
  * with HIP events on the context's stream (its group runs uncaptured); 0 turns it off.  Setting

@@ -59,6 +59,8 @@ def load_ref():
                                    ctypes.c_double, ctypes.c_int]
         lib.ref_create_bdf1.restype = ctypes.c_void_p
         lib.ref_create_bdf1.argtypes = lib.ref_create.argtypes + [ctypes.c_int, ctypes.c_double]
+        lib.ref_create_velocity.restype = ctypes.c_void_p
+        lib.ref_create_velocity.argtypes = lib.ref_create.argtypes
         lib.ref_destroy.argtypes = [ctypes.c_void_p]
         lib.ref_num_pinned.argtypes = [ctypes.c_void_p]
         lib.ref_pinned_nodes.argtypes = [ctypes.c_void_p, _ip]
@@ -192,7 +194,9 @@ def ref_read_scene_json(path):
 
 class RefSim:
     """One reference scene: TET_MESH_FASTER + SNH/ARAP + BACKWARD_EULER_POSITION, or BDF_1
-    (src/Timestepper/BDF_1.cpp:78-239) when newton_iters > 1."""
+    (src/Timestepper/BDF_1.cpp:78-239) when newton_iters > 1, or BACKWARD_EULER_VELOCITY
+    (src/Timestepper/BACKWARD_EULER_VELOCITY.cpp:107-235) for a scene with the velocity
+    integrator."""
 
     def __init__(self, scene, pcg_tol=None, pcg_max_iters=None, newton_iters=None,
                  newton_tol=1e-7):
@@ -208,7 +212,9 @@ class RefSim:
         args = (_d(self._rest), scene.nv, _i(self._tets), scene.nt, scene.material, scene.mu,
                 scene.lam, scene.dt, scene.rayleigh_alpha, scene.rayleigh_beta, scene.mass_scale,
                 _d(g), None if pin is None else _d(pin), tol, mi)
-        if ni > 1:
+        if getattr(scene, "integrator", 0) == 1:
+            self.h = self.lib.ref_create_velocity(*args)
+        elif ni > 1:
             self.h = self.lib.ref_create_bdf1(*args, ni, newton_tol)
         else:
             self.h = self.lib.ref_create(*args)
@@ -351,6 +357,8 @@ def load_oracle():
         lib.orc_set_green_damping.restype = ctypes.c_int
         lib.orc_compute_fdots.argtypes = [ctypes.c_void_p, _dp]
         lib.orc_damped_forces_hessian.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
+        lib.orc_set_velocity_form.argtypes = [ctypes.c_void_p]
+        lib.orc_set_velocity_form.restype = ctypes.c_int
         _orc_lib = lib
     return _orc_lib
 
@@ -438,6 +446,9 @@ class Oracle:
                                      precond, ni, newton_tol)
         if scene.ground_y is not None:
             self.lib.orc_set_ground(self.h, scene.ground_y, scene.ground_k)
+        if getattr(scene, "integrator", 0) == 1:
+            if self.lib.orc_set_velocity_form(self.h) != 0:
+                raise ValueError("the velocity form needs the single-iteration step, no ground")
         for cube in getattr(scene, "collision_cubes", ()):
             c = _c64(np.asarray(cube[:3]))
             self.lib.orc_add_collision_cube(self.h, _d(c), float(cube[3]))

@@ -811,6 +811,8 @@ struct Sim {
   bool green = false;
   double greenMu = 0.0;
   std::vector<double> Fdot;
+  // BACKWARD_EULER_VELOCITY in place of BACKWARD_EULER_POSITION (single Newton iteration)
+  bool velocity = false;
 
   // BSR pattern, block rows by vertex, columns ascending
   std::vector<int> rowPtr, colIdx;
@@ -1173,6 +1175,14 @@ static void filterBlock(int mode, const double* n, double S[3][3]) {
 //                     its collisionC argument and empties it (TIMESTEPPER.cpp:657) before
 //                     A = M/dt^2 - C/dt - K (:389), so with self-collisions off C is zero there.
 //   newtonIters  > 1: BDF_1::solveRayleighDamped (BDF_1.cpp:78-239) with residual test maxR > tol
+//   velocity:         BACKWARD_EULER_VELOCITY::solveRayleighDamped (BACKWARD_EULER_VELOCITY.cpp:
+//                     107-235), which solves for the velocity change dv:
+//                       b = dt (R + dt K v + f_ext) (:157), A = M - dt C - dt^2 K (:161), where
+//                       C = alpha M + beta K_rest (:123) is emptied by computeCollisionResponse
+//                       (:153; TIMESTEPPER.cpp:657), so A = M - dt^2 K;
+//                       plane-constraint targets are velocity changes (cl - x)/dt - v (:51-78);
+//                       dv = y + z, v += dv, u += dt v, v = S v (:190-201), then the same
+//                       separation test and constraint bookkeeping as the position form.
 int Sim::step(double* errOut, int* itersPerNewton, int* newtonDone) {
   const int nnzb = rowPtr[nv];
   const double invDt = 1.0 / dt;
@@ -1182,7 +1192,8 @@ int Sim::step(double* errOut, int* itersPerNewton, int* newtonDone) {
     for (int c = 0; c < 3; c++) fext[3 * i + c] = oneRing[i] * gravity[c];
   // C = alpha M + beta K_rest (TIMESTEPPER.cpp:232-233); D = M/dt^2 - C/dt; E = M/dt - C.
   // Energy damping has no Rayleigh part: D = M/dt^2, E = M/dt (BACKWARD_EULER_POSITION.cpp:384, 389).
-  const double ra_ = green ? 0.0 : ra, rb_ = green ? 0.0 : rb;
+  // The velocity form's C is emptied before A is formed: A = M - dt^2 K (D = M here).
+  const double ra_ = (green || velocity) ? 0.0 : ra, rb_ = (green || velocity) ? 0.0 : rb;
   std::vector<double> Cv(9 * (size_t)nnzb), D(9 * (size_t)nnzb), E(9 * (size_t)nnzb);
   for (int i = 0; i < nv; i++)
     for (int k = rowPtr[i]; k < rowPtr[i + 1]; k++)
@@ -1193,7 +1204,7 @@ int Sim::step(double* errOut, int* itersPerNewton, int* newtonDone) {
           const double m = dg ? M[3 * i + a] : 0.0;
           const double c = dg ? ra_ * m + rb_ * Krest[e] : rb_ * Krest[e];
           Cv[e] = c;
-          D[e] = dg ? m * invDt2 - c * invDt : -(c * invDt);
+          D[e] = velocity ? m : dg ? m * invDt2 - c * invDt : -(c * invDt);
           E[e] = dg ? invDt * m - c : -c;
         }
   const std::vector<double> uOld(u), vOld(v);
@@ -1227,13 +1238,17 @@ int Sim::step(double* errOut, int* itersPerNewton, int* newtonDone) {
         inColl[sv] = 1;
       }
     buildFilter(mode, nrm);
-    // updateConstraintTargets (BACKWARD_EULER_POSITION.cpp:85-110), then z = (I - S) targets
+    // updateConstraintTargets (BACKWARD_EULER_POSITION.cpp:85-110; the velocity form's
+    // BACKWARD_EULER_VELOCITY.cpp:51-78 targets (cl - x)/dt - v), then z = (I - S) targets
     std::vector<double> tg(dofs, 0.0);
     for (const Plane& p : planes) {
       if (p.sep) continue;
       double cw[3];
       cubes[p.obj].toWorld(p.cl, cw);
-      for (int c = 0; c < 3; c++) tg[3 * (size_t)p.vertex + c] = cw[c] - xk[3 * (size_t)p.vertex + c];
+      for (int c = 0; c < 3; c++) {
+        const size_t i = 3 * (size_t)p.vertex + c;
+        tg[i] = velocity ? (cw[c] - xk[i]) / dt - v[i] : cw[c] - xk[i];
+      }
     }
     zc.assign(dofs, 0.0);
     for (int i = 0; i < nv; i++) {
@@ -1265,7 +1280,11 @@ int Sim::step(double* errOut, int* itersPerNewton, int* newtonDone) {
         }
       }
     }
-    if (newtonIters == 1) {
+    if (velocity) {
+      // b = dt (R + dt K v + f_ext)   (BACKWARD_EULER_VELOCITY.cpp:157)
+      spmv(K, v.data(), tmp.data());
+      for (int i = 0; i < dofs; i++) b[i] = dt * ((R[i] + dt * tmp[i]) + fext[i]);
+    } else if (newtonIters == 1) {
       // b = (M/dt - C) v + R + f_ext   (BACKWARD_EULER_POSITION.cpp:198)
       spmv(E, v.data(), tmp.data());
       for (int i = 0; i < dofs; i++) b[i] = (tmp[i] + R[i]) + fext[i];
@@ -1280,12 +1299,14 @@ int Sim::step(double* errOut, int* itersPerNewton, int* newtonDone) {
       spmv(Cv, vb.data(), tmp.data());
       for (int i = 0; i < dofs; i++) b[i] = tmp[i] + (((Ma[i] + R[i]) + fext[i]));
     }
-    // A = M/dt^2 - C/dt - K; rhs = S b; LHS = S A S + (I - S)   (:208, :242-243)
+    // A = M/dt^2 - C/dt - K (velocity form: M - dt^2 K); rhs = S b; LHS = S A S + (I - S)
+    // (:208, :242-243)
+    const double kdt = velocity ? dt * dt : 1.0;
     if (contact) {
       // c = b - A z (:212); rhs = S c; LHS = S A S + (I - S), each product summed in Eigen's
       // column order
       std::vector<double> Av(9 * (size_t)nnzb), Az(dofs);
-      for (size_t q = 0; q < Av.size(); q++) Av[q] = Dq[q] - K[q];
+      for (size_t q = 0; q < Av.size(); q++) Av[q] = velocity ? Dq[q] - kdt * K[q] : Dq[q] - K[q];
       spmv(Av, zc.data(), Az.data());
       std::vector<double> Sb(9 * (size_t)nv);
       for (int i = 0; i < nv; i++) {
@@ -1322,7 +1343,7 @@ int Sim::step(double* errOut, int* itersPerNewton, int* newtonDone) {
         const bool pj = pinnedDof[3 * j];
         for (int e = 0; e < 9; e++) {
           const size_t q = 9 * (size_t)k + e;
-          double a = Dq[q] - K[q];
+          double a = velocity ? Dq[q] - kdt * K[q] : Dq[q] - K[q];
           if (pi || pj) a = (j == i && (e % 4 == 0)) ? 1.0 : 0.0;
           lhs[q] = a;
         }
@@ -1334,10 +1355,28 @@ int Sim::step(double* errOut, int* itersPerNewton, int* newtonDone) {
     const int it = pcg(lhs, rhs, y, err);
     if (itersPerNewton) itersPerNewton[done] = it;
     total += it;
-    if (contact)  // xDelta = y + z (:252)
+    if (velocity) {
+      // vDelta = y + z; v += vDelta; u += dt v; v = S v (BACKWARD_EULER_VELOCITY.cpp:190-201)
+      for (int i = 0; i < dofs; i++) {
+        v[i] = v[i] + (contact ? y[i] + zc[i] : y[i]);
+        u[i] = u[i] + dt * v[i];
+      }
+      if (contact) {
+        for (int i = 0; i < nv; i++) {
+          double S[3][3], w[3];
+          filterBlock(mode[i], &nrm[3 * (size_t)i], S);
+          for (int c = 0; c < 3; c++) w[c] = v[3 * i + c];
+          for (int a = 0; a < 3; a++) v[3 * i + a] = (S[a][0] * w[0] + S[a][1] * w[1]) + S[a][2] * w[2];
+        }
+      } else {
+        for (int i = 0; i < dofs; i++)
+          if (pinnedDof[i]) v[i] = 0.0;
+      }
+    } else if (contact) {  // xDelta = y + z (:252)
       for (int i = 0; i < dofs; i++) u[i] += y[i] + zc[i];
-    else
+    } else {
       for (int i = 0; i < dofs; i++) u[i] += y[i];
+    }
     lastRhs = rhs;
     lastLhs = lhs;
     done++;
@@ -1377,13 +1416,14 @@ int Sim::step(double* errOut, int* itersPerNewton, int* newtonDone) {
     // went away, so otherwise the velocity filter keeps this step's normals (:283-296)
     for (Plane& p : planes) cubes[p.obj].closest(&xk[3 * (size_t)p.vertex], p.cl, p.nl);
     if (changed) buildFilter(mode, nrm);
-    for (int i = 0; i < nv; i++) {
+    // the velocity form filtered v with this step's S before the test (above)
+    for (int i = 0; i < nv && !velocity; i++) {
       double S[3][3], w[3];
       filterBlock(mode[i], &nrm[3 * (size_t)i], S);
       for (int c = 0; c < 3; c++) w[c] = invDt * (u[3 * i + c] - uOld[3 * i + c]);
       for (int a = 0; a < 3; a++) v[3 * i + a] = (S[a][0] * w[0] + S[a][1] * w[1]) + S[a][2] * w[2];
     }
-  } else {
+  } else if (!velocity) {
     // v = (u - u_old)/dt, then filtered by S (BACKWARD_EULER_POSITION.cpp:300-314)
     for (int i = 0; i < dofs; i++) v[i] = pinnedDof[i] ? 0.0 : invDt * (u[i] - uOld[i]);
   }
@@ -1610,6 +1650,15 @@ int orc_set_green_damping(void* h, double mu) {
 }
 
 void orc_compute_fdots(void* h, const double* vel) { ((Sim*)h)->computeFdots(vel); }
+
+// BACKWARD_EULER_VELOCITY in place of BACKWARD_EULER_POSITION.  Single-iteration Rayleigh handles
+// without the penalty ground only (returns -1 otherwise).
+int orc_set_velocity_form(void* h) {
+  Sim* s = (Sim*)h;
+  if (s->newtonIters > 1 || s->green || s->ground) return -1;
+  s->velocity = true;
+  return 0;
+}
 
 // computeInternalForces(hyperelastic, damping) and the assembled damping stiffness C
 // (TET_MESH_FASTER::computeDampingHessian) at displacement u with the held Fdot.

@@ -55,5 +55,8 @@ void orc_compute_fdots(void* h, const double* vel);
 // Elastic + damping forces and the assembled damping stiffness C at displacement u.
 void orc_damped_forces_hessian(void* h, const double* u, double* R, double* Cvals);
 void orc_rest_volumes(void* h, double* vol, double* one_ring);
+// BACKWARD_EULER_VELOCITY (BACKWARD_EULER_VELOCITY.cpp:107-235) in place of the position form:
+// single-iteration Rayleigh handles without the penalty ground (returns -1 otherwise).
+int orc_set_velocity_form(void* h);
 void orc_set_threads(int n);
 }

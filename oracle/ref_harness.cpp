@@ -6,8 +6,10 @@
 //
 // What it drives (reference files, relative to the reference root):
 //   scene step       src/Scenes/SIMULATION_SCENE.h:120-126 (zero f_ext, addGravity, solve)
-//   integrator       src/Timestepper/BACKWARD_EULER_POSITION.cpp:138-327, or the BDF_1 Newton
-//                    loop src/Timestepper/BDF_1.cpp:78-239 (ref_create_bdf1)
+//   integrator       src/Timestepper/BACKWARD_EULER_POSITION.cpp:138-327, the BDF_1 Newton
+//                    loop src/Timestepper/BDF_1.cpp:78-239 (ref_create_bdf1), or
+//                    BACKWARD_EULER_VELOCITY src/Timestepper/BACKWARD_EULER_VELOCITY.cpp:107-235
+//                    (ref_create_velocity)
 //   mesh             src/Geometry/TET_MESH_FASTER.cpp:30-133
 //   pinning          src/Timestepper/TIMESTEPPER.cpp:380-403 with src/Geometry/CUBE.cpp:27-55
 //   plane constraints src/Timestepper/TIMESTEPPER.cpp:252-329, 433-584 (kinematic collision
@@ -41,6 +43,7 @@
 #include "Hyperelastic/Volume/ARAP.h"
 #include "Hyperelastic/Volume/SNH.h"
 #include "Timestepper/BACKWARD_EULER_POSITION.h"
+#include "Timestepper/BACKWARD_EULER_VELOCITY.h"
 #include "Timestepper/BDF_1.h"
 #include "util/MATRIX_UTIL.h"
 
@@ -71,6 +74,7 @@ class Harness : public Base {
   void setDamping(VOLUME::DAMPING* d) { this->_damping = d; }
 };
 typedef Harness<TIMESTEPPER::BACKWARD_EULER_POSITION> HarnessBEP;
+typedef Harness<TIMESTEPPER::BACKWARD_EULER_VELOCITY> HarnessBEV;
 
 typedef Harness<HOBAK::TIMESTEPPER::BDF_1> HarnessBDF1Base;
 class HarnessBDF1 : public HarnessBDF1Base {
@@ -82,24 +86,28 @@ class HarnessBDF1 : public HarnessBDF1Base {
   }
 };
 
-// The integrator behind one handle: exactly one of bep / bdf is set.
+// The integrator behind one handle: exactly one of bep / bdf / bev is set.
 struct RefSim {
   TET_MESH_FASTER* mesh = nullptr;
   VOLUME::HYPERELASTIC* material = nullptr;
   HarnessBEP* bep = nullptr;
   HarnessBDF1* bdf = nullptr;
+  HarnessBEV* bev = nullptr;
   CUBE* pin = nullptr;
   VOLUME::DAMPING* damping = nullptr;
   std::vector<CUBE*> shapes;  // kinematic collision objects
   VECTOR3 gravity;
   double lastStepSeconds = 0.0;
   TIMESTEPPER::TIMESTEPPER* base() const {
-    return bep ? (TIMESTEPPER::TIMESTEPPER*)bep : (TIMESTEPPER::TIMESTEPPER*)bdf;
+    return bep ? (TIMESTEPPER::TIMESTEPPER*)bep
+               : bdf ? (TIMESTEPPER::TIMESTEPPER*)bdf : (TIMESTEPPER::TIMESTEPPER*)bev;
   }
-  int iterations() const { return bep ? bep->iterations() : bdf->iterations(); }
-  double error() const { return bep ? bep->error() : bdf->error(); }
-  const SPARSE_MATRIX& massMatrix() const { return bep ? bep->massMatrix() : bdf->massMatrix(); }
-  const VECTOR& lastB() const { return bep ? bep->lastB() : bdf->lastB(); }
+  int iterations() const { return bep ? bep->iterations() : bdf ? bdf->iterations() : bev->iterations(); }
+  double error() const { return bep ? bep->error() : bdf ? bdf->error() : bev->error(); }
+  const SPARSE_MATRIX& massMatrix() const {
+    return bep ? bep->massMatrix() : bdf ? bdf->massMatrix() : bev->massMatrix();
+  }
+  const VECTOR& lastB() const { return bep ? bep->lastB() : bdf ? bdf->lastB() : bev->lastB(); }
 };
 
 template <typename H>
@@ -184,10 +192,24 @@ void* ref_create_bdf1(const double* rest_xyz, int nv, const int* tets, int nt, i
   return sim;
 }
 
+// The same scene with BACKWARD_EULER_VELOCITY (its solve() takes solveRayleighDamped, which
+// solves for the velocity change: BACKWARD_EULER_VELOCITY.cpp:107-235).
+void* ref_create_velocity(const double* rest_xyz, int nv, const int* tets, int nt, int material,
+                          double mu, double lambda, double dt, double rayleigh_alpha,
+                          double rayleigh_beta, double mass_scale, const double* gravity,
+                          const double* pin_cube, double pcg_tol, int pcg_max_iters) {
+  RefSim* sim = make_sim(rest_xyz, nv, tets, nt, material, mu, lambda, gravity);
+  sim->bev = new HarnessBEV(*sim->mesh, *sim->material);
+  configure(sim, sim->bev, dt, rayleigh_alpha, rayleigh_beta, mass_scale, pin_cube, pcg_tol,
+            pcg_max_iters);
+  return sim;
+}
+
 void ref_destroy(void* h) {
   RefSim* sim = (RefSim*)h;
   delete sim->bep;
   delete sim->bdf;
+  delete sim->bev;
   delete sim->material;
   delete sim->mesh;
   delete sim->pin;

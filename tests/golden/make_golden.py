@@ -20,6 +20,17 @@ small .npz fixtures (no pickles) committed beside this script:
                         forces and the damping stiffness C of a 3x3x3 lattice at a seeded state
                         and velocity gradient (TET_MESH.cpp:816-858, TET_MESH_FASTER.cpp:138-180);
                         20 solveEnergyDamped steps of scenes.green_scene (Jacobi PCG, tol 1e-8)
+  c1_velocity.npz       the C1 scene on BACKWARD_EULER_VELOCITY (src/Timestepper/
+                        BACKWARD_EULER_VELOCITY.cpp:107-235), Jacobi PCG at tol 1e-8, 20 steps:
+                        iterations, errors, states after steps 1, 5, 10 and 20
+  separation_test.npz   the reference's collision separation test (projects/unitTests/
+                        SCENE_TESTS.h:367-453): cube_6.tobj (copied next to this script) on
+                        BACKWARD_EULER_VELOCITY dropped onto the floor CUBE, 42 steps at PCG
+                        tolerance 1e-8: iterations, live plane constraints per step, states
+  unit_tets.npz         the deformation gradients of element_fixtures.npz as a mesh of disjoint unit
+                        tets (Dm = I, tests/helpers.py:unit_tet_mesh): the reference's mesh forces
+                        and clamped stiffness at that displacement, SNH, stiff SNH and ARAP
+                        (TET_MESH.cpp:736-770, TET_MESH_FASTER.cpp:89-133)
   two_tet_drop.npz      the reference's own regression recordings data/regression/two_tet_drop_vf.json
                         and two_tet_drop_ee.json (src/Scenes/TWO_TET_DROP_VF.h, _EE.h), positions
                         and velocities of every frame before the first self-collision (27 and 36
@@ -126,6 +137,35 @@ def make_mesh_small(out):
     np.savez_compressed(out, **data)
 
 
+def make_unit_tets(out):
+    """element_fs() as one mesh of disjoint unit tets, each deformed so that its edge matrix is
+    the F (up to the rounding of x = X + u on its diagonal entries, which the reference and every
+    checker see alike): the mesh path's forces and stiffness at exactly the inputs a GPU run gets."""
+    sys.path.insert(0, os.path.dirname(HERE))
+    from helpers import unit_tet_mesh
+
+    Fs = element_fs()
+    rest, tets, u = unit_tet_mesh(Fs, spacing=0.0)
+    data = {"F": Fs, "rest": rest, "tets": tets, "u": u}
+    params = {"snh": (scenes.MAT_SNH, 1.0, 10.0), "snh_stiff": (scenes.MAT_SNH, 34482.75862068966,
+                                                                310344.8275862069),
+              "arap": (scenes.MAT_ARAP, 1.0, 10.0)}
+    for key, (mat, mu, lam) in params.items():
+        sc = scenes.Scene(name=key, rest=rest, tets=tets, material=mat, mu=mu, lam=lam)
+        r = oracle.RefSim(sc)
+        R, csc = r.forces_hessian(u)
+        o = oracle.Oracle(sc)
+        rp, ci = o.bsr_pattern()
+        data[f"{key}_params"] = np.array([mat, mu, lam])
+        data[f"{key}_R"] = R
+        data[f"{key}_K"] = oracle.csc_to_bsr_vals(csc, rp, ci)
+        data["row_ptr"] = rp
+        data["col_idx"] = ci
+        r.close()
+        o.close()
+    np.savez_compressed(out, **data)
+
+
 def run_traj(sc, steps, keep_steps, stride=1, newton_iters=None):
     r = oracle.RefSim(sc, newton_iters=newton_iters)
     iters, errs, secs, unorm, vnorm = [], [], [], [], []
@@ -192,6 +232,23 @@ def make_c1_bdf1(out):
         data[f"u{k}"] = u
         data[f"v{k}"] = t["vs"][k]
     np.savez_compressed(out, **data)
+
+
+def make_c1_velocity(out):
+    sc = scenes.config1_velocity()
+    t = run_traj(sc, 20, keep_steps=(1, 5, 10, 20))
+    data = dict(iters=t["iters"], errs=t["errs"], unorm=t["unorm"], vnorm=t["vnorm"],
+                keep=np.array(sorted(t["us"])))
+    for k, u in t["us"].items():
+        data[f"u{k}"] = u
+        data[f"v{k}"] = t["vs"][k]
+    np.savez_compressed(out, **data)
+
+
+def make_separation(out):
+    """SCENE_TESTS.h:367-453 with the workloads' PCG tolerance (1e-8): 42 velocity-form steps."""
+    sc = scenes.separation_test(os.path.join(HERE, "cube_6.tobj"))
+    _drop_record(sc, 42, (1, 5, 10, 20, 30, 42), out)
 
 
 def make_drop_test(out):
@@ -332,7 +389,9 @@ def main():
     jobs = [("element_fixtures.npz", make_element), ("mesh_small.npz", make_mesh_small),
             ("c1_traj.npz", make_c1), ("c1_arap.npz", make_c1_arap), ("c1_bdf1.npz", make_c1_bdf1),
             ("drop_traj.npz", make_drop), ("drop_test_cylinder.npz", make_drop_test),
-            ("green_damping.npz", make_green), ("two_tet_drop.npz", make_two_tet_drop)]
+            ("green_damping.npz", make_green), ("two_tet_drop.npz", make_two_tet_drop),
+            ("unit_tets.npz", make_unit_tets), ("c1_velocity.npz", make_c1_velocity),
+            ("separation_test.npz", make_separation)]
     if not args.skip_c2:
         jobs.append(("c2_traj.npz", make_c2))
     for name, fn in jobs:

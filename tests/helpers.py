@@ -92,3 +92,41 @@ def penalty_ground_case(seed=3):
     du[:, 1] = dy
     assert 0 < below.sum() < sc.nv and (below & (X[:, 1] >= sc.ground_y)).any()
     return sc, du.ravel()
+
+
+def unit_tet_mesh(Fs, spacing=4.0):
+    """One disjoint tet per deformation gradient: rest corners X0 = 0, X1 = e_x, X2 = e_y,
+    X3 = e_z (so Dm = I and the rest volume is 1/6), translated apart along x, with the
+    displacement that makes the deformed edge matrix [x1 - x0, x2 - x0, x3 - x0] equal F.
+    Returns (rest (4n, 3), tets (n, 4), u (12n,))."""
+    Fs = np.asarray(Fs, dtype=np.float64)
+    n = len(Fs)
+    corners = np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0]])
+    rest = np.zeros((4 * n, 3))
+    x = np.zeros((4 * n, 3))
+    for t, F in enumerate(Fs):
+        off = np.array([spacing * t, 0.0, 0.0])
+        rest[4 * t:4 * t + 4] = corners + off
+        x[4 * t] = off
+        for a in range(3):
+            x[4 * t + 1 + a] = off + F[:, a]
+    tets = np.arange(4 * n, dtype=np.int32).reshape(n, 4)
+    return rest, tets, (x - rest).ravel()
+
+
+def unit_tet_expected(P, H, vol=1.0 / 6.0):
+    """Element forces and stiffness blocks of a unit tet (Dm = I) from the material's PK1 P (3x3)
+    and clamped dP/dF H (9x9, column-major flattening: index i + 3 j is F_ij), as
+    computeHyperelasticForces and computeHyperelasticClampedHessian form them
+    (src/Geometry/TET_MESH.cpp:736-770, TET_MESH_FASTER.cpp:89-103):
+      f = -vol dF/dx^T vec(P),  K = -vol dF/dx^T H dF/dx,
+    where with Dm = I, F_ij = x_(j+1),i - x_0,i, so dF_ij/dx_a,k = delta_ik c_aj with
+    c_aj = [a == j + 1] - [a == 0].  Returns (forces (4, 3), blocks (4, 4, 3, 3))."""
+    c = np.zeros((4, 3))
+    c[0, :] = -1.0
+    for j in range(3):
+        c[j + 1, j] = 1.0
+    Hm = np.asarray(H, dtype=np.float64).reshape(3, 3, 3, 3, order="F")  # Hm[k, j, l, n] = H[k+3j, l+3n]
+    f = -vol * np.einsum("aj,kj->ak", c, np.asarray(P, dtype=np.float64))
+    K = -vol * np.einsum("aj,bn,kjln->abkl", c, c, Hm)
+    return f, K

@@ -63,3 +63,25 @@ def test_cpp_facade_compiles_against_the_header(tmp_path):
     src = os.path.join(ROOT, "hobakv1_amd", "host", "facade_driver.cpp")
     subprocess.run([cxx, "-std=c++17", "-Wall", "-Werror", "-fsyntax-only", "-I",
                     os.path.join(ROOT, "include"), src], check=True)
+
+
+def test_offset_range_check():
+    """hobak_check_offsets (host-only) accepts the largest workloads and refuses a mesh whose
+    matrix planes or padded vectors would overflow the PCG kernels' 32-bit byte offsets; the
+    boundaries follow from 2^32 bytes per plane (FP64: 536,870,912 slots) and per vector
+    (FP64: 178,956,970 vertices, rounded down to whole 64-row slices)."""
+    so = lib.load()
+    ok, bad = 0, -1
+    # C2 and a 300-instance-sized mesh
+    assert so.hobak_check_offsets(68921, 1179648, 0) == ok
+    assert so.hobak_check_offsets(4_000_000, 70_000_000, 1) == ok
+    # value planes: nslots * 8 < 2^32 (FP64), nslots * 4 < 2^32 (FP32)
+    assert so.hobak_check_offsets(1000, (1 << 29) - 1, 0) == ok
+    assert so.hobak_check_offsets(1000, 1 << 29, 0) == bad
+    assert so.hobak_check_offsets(1000, (1 << 30) - 1, 1) == ok
+    assert so.hobak_check_offsets(1000, 1 << 30, 1) == bad
+    # padded vectors: 3 * ceil64(nv) * s < 2^32
+    nv64 = ((1 << 32) // 24) // 64 * 64
+    assert so.hobak_check_offsets(nv64 - 64, 1000, 0) == ok
+    assert so.hobak_check_offsets(nv64 + 1, 1000, 0) == bad
+    assert so.hobak_check_offsets(0, 1000, 0) == bad

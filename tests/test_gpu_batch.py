@@ -243,3 +243,46 @@ def test_scheduler_continuous_batching_equals_alone(gpu_ctx, sample_every):
     sch.close()
     for inst in insts:
         inst.close()
+
+
+@pytest.mark.parametrize("precision", [lib.PRECISION_FP64, lib.PRECISION_FP32_REFINE])
+def test_full_c4_batch_step(precision):
+    """The C4 batch generator (dims U{4..24}^3, SNH/ARAP, seeded materials) at 300 instances,
+    about 4.7 M tets, with the batch tuning entry, one step: every instance bitwise equal to
+    stepping it alone, and, in FP64, six instances of assorted size and material against the
+    oracle (PCG counts +-1, positions 1e-9).  Beyond 256 instances the segmented instance lookup
+    (kernels.h seg_find) takes a second round of prefix loads, which this batch runs."""
+    scs = scenes.batch_instances(count=300)
+    name = "fp64" if precision == lib.PRECISION_FP64 else "fp32"
+    ctx = lib.Context(0, precision=name, workload="batch")
+    _say(f"full C4 batch {name}: {len(scs)} instances, {sum(s.nt for s in scs)} tets")
+    insts = [lib.Instance(ctx, sc, precision=precision) for sc in scs]
+    st = lib.step_batch(ctx, insts)
+    batched = [inst.state() for inst in insts]
+    for inst, sc in zip(insts, scs):  # back to the initial state, then one step each alone
+        inst.set_state(sc.initial_u(), sc.initial_v())
+    for i, inst in enumerate(insts):
+        sa = inst.step()
+        ua, va = inst.state()
+        assert sa["pcg_iters"] == st[i]["pcg_iters"], i
+        assert np.array_equal(ua, batched[i][0]) and np.array_equal(va, batched[i][1]), i
+    _say(f"  bitwise alone == batched for all {len(scs)}")
+    if precision == lib.PRECISION_FP64:
+        order = np.argsort([sc.nt for sc in scs])
+        pick = [int(order[0]), int(order[len(order) // 3]), int(order[len(order) // 2]),
+                int(order[2 * len(order) // 3]), int(order[-1])]
+        pick += [i for i in range(len(scs)) if scs[i].material != scs[pick[0]].material][:1]
+        for i in pick:
+            o = oracle.Oracle(scs[i], precond=scenes.PRECOND_BLOCK_JACOBI)
+            it, _, _ = o.step()
+            uo, _ = o.state()
+            X = scs[i].rest.ravel()
+            e = rel(X + batched[i][0], X + uo)
+            _say(f"  instance {i} nt {scs[i].nt} mat {scs[i].material}: iters {st[i]['pcg_iters'][0]} "
+                 f"oracle {it} rel x {e:.2e}")
+            assert abs(st[i]["pcg_iters"][0] - it) <= 1
+            assert e <= 1e-9
+            o.close()
+    for inst in insts:
+        inst.close()
+    ctx.close()

@@ -3,10 +3,12 @@
 Tolerances (FP64 path):
   forces            normwise relative <= 1e-12
   stiffness K       normwise relative <= 1e-12; entrywise <= 1e-12 relative to the largest |entry|
-                    of the entry's scalar row; and entrywise <= 1e-11 relative to the entry itself
+                    of the entry's scalar row; and entrywise <= 1e-12 relative to the entry itself
                     for every entry above 1e-3 x its row's largest (tests/helpers.py:bsr_entry_rel).
                     Entries are sums of ~16 element terms of row size, so round-off of a few ulps
-                    of the row scale is what two correct FP64 orders of summation differ by.
+                    of the row scale is what two correct FP64 orders of summation differ by; an
+                    entry a thousand times smaller than its row carries that round-off at ~1e-13
+                    of itself.
   PCG iterations    within +-1 of the checker, every step
   positions         |x - x_ref| / |x_ref| <= 1e-9 after 20 steps, x = X + u
 """
@@ -38,8 +40,12 @@ def test_forces_and_stiffness_match_reference(gpu_ctx, tag):
     R, K = inst.forces_hessian(g[f"{tag}_u"])
     assert rel(R, g[f"{tag}_R"]) <= 1e-12
     assert rel(K, g[f"{tag}_K"]) <= 1e-12
-    assert bsr_entry_rel(K, g[f"{tag}_K"], rp, floor=1.0) <= 1e-12
-    assert bsr_entry_rel(K, g[f"{tag}_K"], rp) <= 1e-11
+    e_row = bsr_entry_rel(K, g[f"{tag}_K"], rp, floor=1.0)
+    e_own = bsr_entry_rel(K, g[f"{tag}_K"], rp)
+    _say(f"  {tag}: forces {rel(R, g[f'{tag}_R']):.2e} K {rel(K, g[f'{tag}_K']):.2e} "
+         f"entrywise (row scale) {e_row:.2e} (own scale) {e_own:.2e}")
+    assert e_row <= 1e-12
+    assert e_own <= 1e-12
     inst.close()
 
 
@@ -57,8 +63,11 @@ def test_forces_and_stiffness_match_oracle_large_deformation(gpu_ctx, mat):
     rp, _ = inst.bsr_pattern()
     assert rel(R, Ro) <= 1e-12
     assert rel(K, Ko) <= 1e-12
+    e_own = bsr_entry_rel(K, Ko, rp)
+    _say(f"  forces {rel(R, Ro):.2e} K {rel(K, Ko):.2e} entrywise (row scale) "
+         f"{bsr_entry_rel(K, Ko, rp, floor=1.0):.2e} (own scale) {e_own:.2e}")
     assert bsr_entry_rel(K, Ko, rp, floor=1.0) <= 1e-12
-    assert bsr_entry_rel(K, Ko, rp) <= 1e-11
+    assert e_own <= 1e-12
     inst.close()
 
 
@@ -127,7 +136,7 @@ def test_last_system_matches_oracle(gpu_ctx):
     assert rel(rhs, rhs_o) <= 1e-12
     assert rel(lhs, lhs_o) <= 1e-12
     assert bsr_entry_rel(lhs, lhs_o, rp, floor=1.0) <= 1e-12
-    assert bsr_entry_rel(lhs, lhs_o, rp) <= 1e-11
+    assert bsr_entry_rel(lhs, lhs_o, rp) <= 1e-12
     inst.close()
 
 
@@ -203,13 +212,10 @@ def test_tet_repeating_a_vertex_is_rejected(gpu_ctx):
 
 
 def test_c2_jacobi_matches_reference(gpu_ctx):
-    """40^3 stretch scene, 20 steps, Jacobi PCG, against the reference's golden trajectory.
-
-    Positions must agree to 1e-9 at steps 1, 10 and 20.  Iteration counts must agree to +-1 while
-    the two trajectories still agree to round-off (steps 1-10); after that the states themselves
-    differ at ~1e-10 and the counts are compared on identical inputs instead
-    (test_c2_iterations_match_oracle_on_same_inputs), here only bounded by +-2.
-    """
+    """40^3 stretch scene, 20 steps, Jacobi PCG, against the reference's golden trajectory:
+    positions within 1e-9 at steps 1, 10 and 20, iteration counts within +-1 every step (the
+    differences are printed; test_c2_iterations_match_oracle_on_same_inputs also compares the
+    counts on identical inputs)."""
     g = golden("c2_traj.npz")
     sc = scenes.config2()
     s = int(g["stride"])
@@ -227,8 +233,7 @@ def test_c2_jacobi_matches_reference(gpu_ctx):
                  f"rel u {rel(u[::s], g[f'u{k}']):.3e}")
             assert e <= 1e-9
     _say(f"  iteration differences vs reference: {diffs}")
-    assert max(abs(d) for d in diffs[:10]) <= 1
-    assert max(abs(d) for d in diffs) <= 2
+    assert max(abs(d) for d in diffs) <= 1
     inst.close()
 
 
@@ -259,19 +264,29 @@ def test_c2_iterations_match_oracle_on_same_inputs(gpu_ctx, precond):
 
 
 def test_c2_block_jacobi_matches_oracle(gpu_ctx):
+    """The benchmarked configuration itself: the 40^3 stretch scene with block-Jacobi PCG (tol
+    1e-8), 20 free-running steps of the GPU FP64 path and of the oracle side by side (neither is
+    restarted from the other): PCG counts within +-1 every step, positions within 1e-9 every step."""
     sc = scenes.config2()
-    _say("c2 block-jacobi 3 steps vs oracle")
+    _say("c2 block-jacobi 20 free-running steps vs oracle")
     inst = lib.Instance(gpu_ctx, sc, precond=scenes.PRECOND_BLOCK_JACOBI)
     o = oracle.Oracle(sc, precond=scenes.PRECOND_BLOCK_JACOBI)
     X = sc.rest.ravel()
-    for k in range(3):
+    diffs, errs = [], []
+    for k in range(20):
         st = inst.step()
         it, _, _ = o.step()
         u, _ = inst.state()
         uo, _ = o.state()
-        _say(f"  step {k}: iters {st['pcg_iters'][0]} oracle {it} rel x {rel(X + u, X + uo):.3e}")
-        assert abs(st["pcg_iters"][0] - it) <= 1
-        assert rel(X + u, X + uo) <= 1e-9
+        diffs.append(int(st["pcg_iters"][0] - it))
+        errs.append(rel(X + u, X + uo))
+        if k in (0, 9, 19):
+            _say(f"  step {k + 1}: iters {st['pcg_iters'][0]} oracle {it} rel x {errs[-1]:.3e} "
+                 f"rel u {rel(u, uo):.3e}")
+    _say(f"  iteration differences vs oracle: {diffs}")
+    _say(f"  max rel x over the steps: {max(errs):.3e}")
+    assert max(abs(d) for d in diffs) <= 1
+    assert max(errs) <= 1e-9
     inst.close()
 
 
