@@ -85,16 +85,22 @@ def parse():
                     help="the serving stream (the headline)")
     ap.add_argument("--plain-serving", default="on", choices=["on", "off"],
                     help="the plain FP64 path on the same stream (the comparison)")
-    ap.add_argument("--sessions", type=int, default=192)
-    ap.add_argument("--rate", type=float, default=160.0, help="session arrivals per second")
-    ap.add_argument("--budget-gib", type=float, default=6.0)
+    ap.add_argument("--sessions", type=int, default=None)
+    ap.add_argument("--rate", type=float, default=None, help="session arrivals per second")
+    ap.add_argument("--budget-gib", type=float, default=None)
     ap.add_argument("--beyond-mall", default="on", choices=["on", "off"],
                     help="the SpMV on one large instance whose matrix exceeds the Infinity Cache")
     ap.add_argument("--large-n", type=int, default=88, help="lattice cells per axis of that instance")
     ap.add_argument("--graphs", default="on", choices=["on", "off"],
                     help="off: launch every kernel directly (for profilers that do not trace "
                          "graph launches)")
-    return ap.parse_args()
+    args = ap.parse_args()
+    from hobakv1_amd import serve
+
+    for key, attr in (("sessions", "sessions"), ("rate", "rate"), ("budget_gib", "budget_gib")):
+        if getattr(args, attr) is None:
+            setattr(args, attr, serve.BENCH_STREAM[key])
+    return args
 
 
 def measured_traffic(name, precision):

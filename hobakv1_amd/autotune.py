@@ -13,8 +13,8 @@ tuning.json next to this file; lib.Context applies it to every instance it creat
 
 The batch workload (the 256-instance C4 set stepped together by hobak_step_batch) is written
 as the "<precision>_batch" entry, which contexts created for batched instances apply; the serving
-workload (the serving stream of serve.py: 48 sessions, Poisson 400/s, 20 steps each, 8 GiB,
-measured by its throughput) as the "<precision>_serving" entry.
+workload (serve.BENCH_STREAM, the serving benchmark's stream, measured by its throughput, three
+runs per shape ranked by the median) as the "<precision>_serving" entry.
 """
 from __future__ import annotations
 
@@ -97,15 +97,17 @@ def measure_batch(ctx, scs, precision, steps, warmup):
 def measure_serving(ctx, precision):
     from . import lib, scenes, serve
 
-    stream = serve.request_stream(48, 400.0, 20, seed=5)
+    b = serve.BENCH_STREAM
+    stream = serve.request_stream(b["sessions"], b["rate"], b["steps"], seed=b["seed"])
     cache = {}
 
     def scene_of(k):
         if k not in cache:
-            cache[k] = scenes.serving_instance(k, seed=5)
+            cache[k] = scenes.serving_instance(k, seed=b["seed"])
         return cache[k]
 
-    r = serve.run(ctx, stream, scene_of, 8 << 30, precision, mode="batched", keep_states=False)
+    r = serve.run(ctx, stream, scene_of, int(b["budget_gib"] * (1 << 30)), precision, mode="batched",
+                  keep_states=False)
     # ranked by ms_per_step like the other workloads: the stream's makespan over its requests
     return {"ms_per_step": 1e3 * r["makespan_s"] / max(1, r["requests"]), "tet_steps_per_s": r["value"],
             "p50_step_ms": r["p50_step_ms"], "p99_step_ms": r["p99_step_ms"], "spmv_ms": 0.0,
@@ -126,10 +128,14 @@ def main():
     ap.add_argument("--inner-tol", default="1e-4", help="FP32 inner PCG tolerances to sweep")
     ap.add_argument("--workload", default="single", choices=["single", "batch", "serving"])
     ap.add_argument("--instances", type=int, default=256, help="batch workload size")
+    ap.add_argument("--repeats", type=int, default=None,
+                    help="runs per shape, ranked by the median (default: 3 for the serving "
+                         "workload, whose run-to-run spread is as large as the differences between "
+                         "shapes; 1 otherwise)")
     ap.add_argument("--write", action="store_true")
     args = ap.parse_args()
     sys.path.insert(0, os.path.dirname(HERE))
-    from hobakv1_amd import lib, scenes
+    from hobakv1_amd import lib, scenes, serve
 
     precision = lib.PRECISION_FP64 if args.precision == "fp64" else lib.PRECISION_FP32_REFINE
     sc = scenes.lattice_scene((args.n,) * 3, seed=2, stretch_velocity=True)
@@ -141,18 +147,26 @@ def main():
                                                        [int(x) for x in args.update_wgs.split(",")],
                                                        [int(x) for x in args.spmv_pad.split(",")],
                                                        [int(x) for x in args.row_sort.split(",")], tols):
-        ctx = lib.Context(0, apply_table=False)
-        ctx.set_tuning(slice_rows=H, poll_chunk=poll, update_wgs=uw, spmv_pad=pad, row_sort=rs)
-        if args.workload == "serving":
-            ctx.refine_inner_tol = tol or 1e-4
-            r = measure_serving(ctx, precision)
-        elif scs is not None:
-            ctx.refine_inner_tol = tol or 1e-4
-            r = measure_batch(ctx, scs, precision, args.steps, args.warmup)
-        else:
-            r = measure(ctx, sc, scenes.PRECOND_BLOCK_JACOBI, precision, args.steps, args.warmup,
-                        inner_tol=tol or 1e-4)
-        ctx.close()
+        repeats = args.repeats or (3 if args.workload == "serving" else 1)
+        runs = []
+        for _ in range(repeats):
+            ctx = lib.Context(0, apply_table=False)
+            ctx.set_tuning(slice_rows=H, poll_chunk=poll, update_wgs=uw, spmv_pad=pad, row_sort=rs)
+            if args.workload == "serving":
+                ctx.refine_inner_tol = tol or 1e-4
+                runs.append(measure_serving(ctx, precision))
+            elif scs is not None:
+                ctx.refine_inner_tol = tol or 1e-4
+                runs.append(measure_batch(ctx, scs, precision, args.steps, args.warmup))
+            else:
+                runs.append(measure(ctx, sc, scenes.PRECOND_BLOCK_JACOBI, precision, args.steps,
+                                    args.warmup, inner_tol=tol or 1e-4))
+            ctx.close()
+        # the run of median ms_per_step stands for the shape; every run's value is kept
+        runs.sort(key=lambda q: q["ms_per_step"])
+        r = dict(runs[len(runs) // 2])
+        if repeats > 1:
+            r["ms_per_step_runs"] = [q["ms_per_step"] for q in runs]
         r.update({"slice_rows": H, "poll_chunk": poll, "update_wgs": uw, "spmv_pad": pad,
                   "row_sort": rs})
         if args.precision == "fp32":
@@ -164,8 +178,10 @@ def main():
     if args.write:
         table = load_table()
         key = args.precision + {"single": "", "batch": "_batch", "serving": "_serving"}[args.workload]
+        bs = serve.BENCH_STREAM
         what = {"single": f"{args.n}^3 lattice", "batch": f"{args.instances}-instance batch",
-                "serving": "serving stream: 48 sessions, Poisson 400/s, 20 steps each, 8 GiB"}[args.workload]
+                "serving": f"serving stream: {bs['sessions']} sessions, Poisson {bs['rate']:g}/s, "
+                           f"{bs['steps']} steps each, {bs['budget_gib']:g} GiB"}[args.workload]
         table[key] = {"slice_rows": best["slice_rows"], "poll_chunk": best["poll_chunk"],
                       "update_wgs": best["update_wgs"], "spmv_pad": best["spmv_pad"],
                       "row_sort": best["row_sort"], "measured": {"workload": what, **best}}

@@ -44,6 +44,10 @@ from . import lib, scenes
 
 NONFINITE = -5  # HOBAK_E_NONFINITE
 
+# The stream the serving benchmark and the serving tuning workload measure: enough sessions that
+# the budget binds and the run is mostly steady state, arriving faster than the GPU serves them.
+BENCH_STREAM = {"sessions": 192, "rate": 160.0, "steps": 20, "budget_gib": 6.0, "seed": 5}
+
 
 @dataclass
 class Session:
