@@ -35,6 +35,7 @@
 #include <cstdio>
 #include <cstring>
 #include <deque>
+#include <mutex>
 #include <new>
 #include <set>
 #include <string>
@@ -81,7 +82,35 @@ struct hobak_ctx {
   size_t arena_off = 0;
   hipEvent_t ev[8] = {};
   std::vector<hipEvent_t> sample_ev;  // pairs
+  void* scratch = nullptr;  // this device's copy of `scratch` (hobak_eval_forces_noalloc)
 };
+
+// The device memory of hobak_eval_forces_noalloc, the entry point that allocates nothing: the
+// per-tet force staging (12 reals per tet) of one call.  Every GPU the library runs on holds its
+// own copy until the process ends; calls on one GPU share it, and the last call's contents stay
+// (nothing in it needs to start at zero: a call writes every entry it reads).  Host code never
+// uses this name as an address: hobak_ctx_create looks up each device's copy with
+// hipGetSymbolAddress (scratch_of_device) and kernels receive that pointer.
+static __device__ __align__(256) unsigned char scratch[HOBAK_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The address of `scratch` on `device` (which must be current), looked up once per device.
+int scratch_of_device(int device, void** out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (device < 0 || device >= kMaxDevices) return HOBAK_E_NODEVICE;
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return HOBAK_E_HIP;
+    g_scratch_ptr[device] = p;
+  }
+  *out = g_scratch_ptr[device];
+  return HOBAK_OK;
+}
+}  // namespace
 
 namespace {
 
@@ -1226,7 +1255,7 @@ int hobak_ctx_create(int device, size_t hbm_budget_bytes, hobak_ctx** out) {
   }
   bool ok = true;
   for (auto& e : ctx->ev) ok = ok && hipEventCreate(&e) == hipSuccess;
-  if (!ok || arena_ensure(ctx, 1 << 20) != HOBAK_OK) {
+  if (!ok || arena_ensure(ctx, 1 << 20) != HOBAK_OK || scratch_of_device(device, &ctx->scratch) != HOBAK_OK) {
     hobak_ctx_destroy(ctx);
     return HOBAK_E_HIP;
   }
@@ -1779,6 +1808,37 @@ int hobak_eval_forces_hessian(hobak_inst* I, const double* u, double* forces, do
   (void)hipSetDevice(I->ctx->device);
   return I->tsize == 8 ? eval_forces_hessian<double>(I, u, forces, kvals)
                        : eval_forces_hessian<float>(I, u, forces, kvals);
+}
+
+int hobak_eval_forces_noalloc(hobak_inst* I, const double* u_dev, double* forces_dev) {
+  if (!I || !u_dev || !forces_dev) return HOBAK_E_INVALID;
+  hobak_ctx* ctx = I->ctx;
+  if (!I->perm.empty())
+    return fail(ctx, HOBAK_E_INVALID, "hobak_eval_forces_noalloc: the instance is held relabelled (row sort); "
+                                      "device vectors would need the caller's numbering");
+  const size_t need = (size_t)12 * I->nt * I->tsize;
+  if (need > (size_t)HOBAK_NOALLOC_SCRATCH_BYTES)
+    return fail(ctx, HOBAK_E_INVALID,
+                "hobak_eval_forces_noalloc: " + std::to_string(I->nt) + " tets need " + std::to_string(need) +
+                    " bytes of force staging; the static scratch holds " +
+                    std::to_string((size_t)HOBAK_NOALLOC_SCRATCH_BYTES) + " (at most " +
+                    std::to_string((size_t)HOBAK_NOALLOC_SCRATCH_BYTES / (12 * (size_t)I->tsize)) + " tets)");
+  if (I->requests) return fail(ctx, HOBAK_E_INVALID, "instance has a step in flight");
+  if (!ctx->scratch) return fail(ctx, HOBAK_E_HIP, "no scratch address for this device");
+  (void)hipSetDevice(ctx->device);
+  hipStream_t s = ctx->stream;
+  // a batch of one names its record directly: no device table is needed
+  const Batch be{nullptr, nullptr, I->dev, 1, I->h.nb[K_ELEM]};
+  const Batch br{nullptr, nullptr, I->dev, 1, I->h.nb[K_ROW]};
+  if (I->tsize == 8) {
+    HIPCHK(ctx, launch_element<double>(be, ELEM_EXT, 0, ctx->grid_cap, s, u_dev, ctx->scratch));
+    HIPCHK(ctx, launch_force_gather<double>(br, ctx->grid_cap, s, ctx->scratch, forces_dev));
+  } else {
+    HIPCHK(ctx, launch_element<float>(be, ELEM_EXT, 0, ctx->grid_cap, s, u_dev, ctx->scratch));
+    HIPCHK(ctx, launch_force_gather<float>(br, ctx->grid_cap, s, ctx->scratch, forces_dev));
+  }
+  HIPCHK(ctx, hipStreamSynchronize(s));
+  return HOBAK_OK;
 }
 
 int hobak_eval_damping(hobak_inst* I, const double* u, double* forces, double* cvals) {

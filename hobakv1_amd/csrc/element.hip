@@ -202,7 +202,8 @@ __device__ __forceinline__ void element_tet(const ElementIn& a, int t) {
 // Segmented over instances (one lane per tet, kElementBlock tets per virtual workgroup); the
 // material is a per-instance, workgroup-uniform branch.
 template <typename T>
-__global__ __launch_bounds__(kElementBlock) void element_kernel(Batch b, int src, int with_hessian) {
+__global__ __launch_bounds__(kElementBlock) void element_kernel(Batch b, int src, int with_hessian,
+                                                                const double* u_ext, void* sf_ext) {
   for (int ws = blockIdx.x; ws < b.total; ws += gridDim.x) {
     const int vb = xcd_vb(ws, b.total);
     int lb, lg;
@@ -214,11 +215,11 @@ __global__ __launch_bounds__(kElementBlock) void element_kernel(Batch b, int src
     a.alpha = I->alpha;
     a.tets = gbl((const int*)I->tets);
     a.rest = gbl(I->rest);
-    a.u = gbl(src == ELEM_TMP ? I->tmp64 : I->u);
+    a.u = gbl(src == ELEM_EXT ? u_ext : src == ELEM_TMP ? I->tmp64 : I->u);
     a.dminv = gbl(I->dminv);
     a.vol = gbl(I->vol);
     a.stage_h = gbl(I->stage_h);
-    a.stage_f = gbl(I->stage_f);
+    a.stage_f = gbl(sf_ext ? sf_ext : I->stage_f);
     a.hdest = gbl(I->hdest);
     a.fdest = gbl(I->fdest);
     a.with_hessian = with_hessian;
@@ -234,15 +235,17 @@ __global__ __launch_bounds__(kElementBlock) void element_kernel(Batch b, int src
 }
 
 template <typename T>
-hipError_t launch_element(const Batch& b, int src, int with_hessian, int cap, hipStream_t s) {
+hipError_t launch_element(const Batch& b, int src, int with_hessian, int cap, hipStream_t s,
+                          const double* u_ext, void* stage_f_ext) {
   if (b.total <= 0) return hipSuccess;
   const int grid = cap > 0 && cap < b.total ? cap : b.total;
-  hipLaunchKernelGGL((element_kernel<T>), dim3(grid), dim3(kElementBlock), 0, s, b, src, with_hessian);
+  hipLaunchKernelGGL((element_kernel<T>), dim3(grid), dim3(kElementBlock), 0, s, b, src, with_hessian,
+                     u_ext, stage_f_ext);
   return hipGetLastError();
 }
 
-template hipError_t launch_element<double>(const Batch&, int, int, int, hipStream_t);
-template hipError_t launch_element<float>(const Batch&, int, int, int, hipStream_t);
+template hipError_t launch_element<double>(const Batch&, int, int, int, hipStream_t, const double*, void*);
+template hipError_t launch_element<float>(const Batch&, int, int, int, hipStream_t, const double*, void*);
 
 // The GREEN_DAMPING part of computeInternalForces, after the element kernel: for each tet of an
 // instance with energy damping, f_v += -V P_damping d_v^T into the staged forces (one lane per

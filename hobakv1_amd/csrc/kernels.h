@@ -251,7 +251,8 @@ __device__ __forceinline__ HOBAK_GLOBAL T* gbl(T* p) {
 int blocks_of(int kind, int nv, int nt, int nslots, int shift, int update_cap, int update_groups);
 
 // Launchers.  `cap` limits the launch grid (0: one workgroup per virtual workgroup).
-enum ElemSrc { ELEM_U = 0, ELEM_TMP = 1 };
+// ELEM_EXT: a displacement the caller holds in device memory (the no-allocation entry point)
+enum ElemSrc { ELEM_U = 0, ELEM_TMP = 1, ELEM_EXT = 2 };
 enum AsmTarget { ASM_REST = 0, ASM_PLAIN_A = 1, ASM_SYSTEM = 2 };
 
 // Staged stiffness blocks per tet (the upper triangle of its 4x4 block pattern), their order, and
@@ -261,8 +262,11 @@ constexpr int kHdestTranspose = (int)0x80000000u;
 constexpr int kHdestIndex = 0x7fffffff;
 __host__ __device__ constexpr int staged_pair(int x, int y) { return 4 * x - (x * (x - 1)) / 2 + (y - x); }
 
+// u_ext (with ELEM_EXT) and stage_f_ext (non-null: the force staging) replace the instance's own
+// buffers for a batch of one.
 template <typename T>
-hipError_t launch_element(const Batch& b, int src, int with_hessian, int cap, hipStream_t s);
+hipError_t launch_element(const Batch& b, int src, int with_hessian, int cap, hipStream_t s,
+                          const double* u_ext = nullptr, void* stage_f_ext = nullptr);
 // Adds the GREEN_DAMPING force of the instances with energy damping to the staged element forces.
 template <typename T>
 hipError_t launch_damping_force(const Batch& b, int src, int cap, hipStream_t s);
@@ -302,8 +306,10 @@ template <typename T>
 hipError_t launch_refine_residual(const Batch& spmv, const Batch& dof, int shift, int cap, hipStream_t s);
 hipError_t launch_finish_check(const Batch& b, int refine, hipStream_t s);
 hipError_t launch_status(const Batch& b, InstStatus* out, hipStream_t s);
+// The per-vertex force gather into rhs64, or from stage_f_ext into out_ext (both non-null).
 template <typename T>
-hipError_t launch_force_gather(const Batch& b, int cap, hipStream_t s);
+hipError_t launch_force_gather(const Batch& b, int cap, hipStream_t s, const void* stage_f_ext = nullptr,
+                               double* out_ext = nullptr);
 hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
 
 }  // namespace hobak

@@ -1436,23 +1436,24 @@ __global__ __launch_bounds__(kBlock) void status_kernel(Batch b, InstStatus* out
 }
 
 // computeHyperelasticForces' scatter as a per-vertex gather in ascending tet order
-// (TET_MESH.cpp:756-767); output in FP64 into rhs64.
+// (TET_MESH.cpp:756-767); output in FP64 into rhs64, or from sf_ext into out_ext.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void force_gather_kernel(Batch b) {
+__global__ __launch_bounds__(kBlock) void force_gather_kernel(Batch b, const void* sf_ext, double* out_ext) {
   for (int ws = blockIdx.x; ws < b.total; ws += gridDim.x) {
     const int vb = xcd_vb(ws, b.total);
     int lb, lg;
     const HOBAK_GLOBAL InstDev* __restrict__ I = seg_find(b, vb, lb, lg);
     const int r = lb * kBlock + threadIdx.x;
     if (r >= I->nv) continue;
-    auto sf = gbl((const T*)I->stage_f);
+    auto sf = gbl((const T*)(sf_ext ? sf_ext : I->stage_f));
+    auto out = gbl(out_ext ? out_ext : I->rhs64);
     double s[3] = {0.0, 0.0, 0.0};
     for (int q = gbl(I->vptr)[r]; q < gbl(I->vptr)[r + 1]; q++) {
 #pragma unroll
       for (int c = 0; c < 3; c++) s[c] += (double)sf[3 * (size_t)q + c];
     }
 #pragma unroll
-    for (int c = 0; c < 3; c++) gbl(I->rhs64)[3 * (size_t)r + c] = s[c];
+    for (int c = 0; c < 3; c++) out[3 * (size_t)r + c] = s[c];
   }
 }
 
@@ -1586,8 +1587,9 @@ hipError_t launch_finish_check(const Batch& b, int refine, hipStream_t s) {
 }
 hipError_t launch_status(const Batch& b, InstStatus* out, hipStream_t s) { SEG_LAUNCH(status_kernel, b, 0, s, out); }
 template <typename T>
-hipError_t launch_force_gather(const Batch& b, int cap, hipStream_t s) {
-  SEG_LAUNCH((force_gather_kernel<T>), b, cap, s);
+hipError_t launch_force_gather(const Batch& b, int cap, hipStream_t s, const void* stage_f_ext,
+                               double* out_ext) {
+  SEG_LAUNCH((force_gather_kernel<T>), b, cap, s, stage_f_ext, out_ext);
 }
 
 hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s) {
@@ -1605,7 +1607,7 @@ hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s) 
   template hipError_t launch_cg_update<T>(const Batch&, int, int, int, hipStream_t);         \
   template hipError_t launch_refine_init<T>(const Batch&, int, int, hipStream_t);            \
   template hipError_t launch_refine_residual<T>(const Batch&, const Batch&, int, int, hipStream_t); \
-  template hipError_t launch_force_gather<T>(const Batch&, int, hipStream_t);
+  template hipError_t launch_force_gather<T>(const Batch&, int, hipStream_t, const void*, double*);
 INST(double)
 INST(float)
 #undef INST

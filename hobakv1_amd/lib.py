@@ -113,6 +113,7 @@ SIGNATURES = {
                                         ctypes.POINTER(hobak_step_stats)]),
     "hobak_get_bsr_pattern": (ctypes.c_int, [_vp, _ip, _ip, _ip]),
     "hobak_eval_forces_hessian": (ctypes.c_int, [_vp, _dp, _dp, _dp]),
+    "hobak_eval_forces_noalloc": (ctypes.c_int, [_vp, ctypes.c_void_p, ctypes.c_void_p]),
     "hobak_get_last_system": (ctypes.c_int, [_vp, _dp, _dp]),
     "hobak_last_step_timing": (ctypes.c_int, [_vp, _dp]),
     "hobak_set_kernel_sampling": (ctypes.c_int, [_vp, ctypes.c_int32]),

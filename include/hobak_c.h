@@ -192,6 +192,19 @@ int hobak_get_bsr_pattern(hobak_inst* inst, int32_t* nnzb, int32_t* row_ptr, int
  * forces (DOFs) and K (nnzb*9, row-major blocks, the reference's sign: K = -stiffness).
  * Either output may be NULL. */
 int hobak_eval_forces_hessian(hobak_inst* inst, const double* u, double* forces, double* kvals);
+/* No counterpart in the reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  The element forces
+ * of hobak_eval_forces_hessian (computeHyperelasticForces, TET_MESH.cpp:736-770), from a
+ * displacement held in device memory into device memory, allocating nothing: u_dev and
+ * forces_dev are 3 nv doubles on the instance's device, in the caller's vertex numbering.  The
+ * per-tet force staging lives in one static device array of HOBAK_NOALLOC_SCRATCH_BYTES per GPU,
+ * so an instance may have at most HOBAK_NOALLOC_SCRATCH_BYTES / (12 * real size) tets (FP64:
+ * 524,288; FP32: 1,048,576), and must not be relabelled (HOBAK_TUNE_ROW_SORT); otherwise
+ * HOBAK_E_INVALID with the limit in hobak_last_error.  Every call on a GPU uses that GPU's one
+ * array: calls that overlap on one GPU (from several host threads or contexts) race.  Returns
+ * once the forces are written. */
+#define HOBAK_NOALLOC_SCRATCH_BYTES (48u << 20)
+int hobak_eval_forces_noalloc(hobak_inst* inst, const double* u_dev, double* forces_dev);
 /* With energy damping: computeInternalForces(hyperelastic, damping) (elastic plus damping force,
  * TET_MESH.cpp:816-858) and TET_MESH_FASTER::computeDampingHessian (TET_MESH_FASTER.cpp:138-180)
  * at displacement u and the held Fdot: forces (DOFs) and C (nnzb*9, row-major blocks, the
