@@ -67,6 +67,7 @@ struct hobak_ctx {
   int row_sort = 0;                    // relabel new instances' vertices by descending row length
   int compact_pct = 100;               // PCG groups launch over the solving instances only below this
   int update_groups = 2;               // 21-row groups per PCG update wave
+  int p_form = 1;                      // FP64 p-form iterations: 0 never, 1 with streamed matrices, 2 always
   hipStream_t stream = nullptr;
   size_t budget = 0;
   size_t in_use = 0;
@@ -723,6 +724,10 @@ template <typename T>
 int launch_chunk(hobak_ctx* ctx, const Plan& P, int k, int stream, hobak_sched* S, int every) {
   hipStream_t s = ctx->stream;
   const int cap = ctx->grid_cap;
+  // FP64 groups whose matrices stream from HBM form p_new once per entry and gather it alone
+  // (three kernels per iteration, bitwise the same results); the extra launch does not pay for
+  // a working set that stays in the caches
+  const bool pform = sizeof(T) == 8 && (ctx->p_form == 2 || (ctx->p_form == 1 && stream));
   for (int j = 0; j < k; j++) {
     hipEvent_t a = nullptr, b = nullptr;
     if (S) {
@@ -734,7 +739,12 @@ int launch_chunk(hobak_ctx* ctx, const Plan& P, int k, int stream, hobak_sched* 
         S->samples.emplace_back(g, slot);
       }
     }
-    HIPCHK(ctx, launch_cg_spmv<T>(P.b[K_SPMV], P.shift, ctx->spmv_pad, stream, j & 1, cap, s, a, b));
+    if (pform) {
+      HIPCHK(ctx, launch_cg_p(P.b[K_UPD], j & 1, cap, s));
+      HIPCHK(ctx, launch_cg_spmv_p(P.b[K_SPMV], P.shift, ctx->spmv_pad, stream, j & 1, cap, s, a, b));
+    } else {
+      HIPCHK(ctx, launch_cg_spmv<T>(P.b[K_SPMV], P.shift, ctx->spmv_pad, stream, j & 1, cap, s, a, b));
+    }
     HIPCHK(ctx, launch_cg_update<T>(P.b[K_UPD], P.groups, j & 1, cap, s));
   }
   if (P.refine) {
@@ -1965,6 +1975,16 @@ int hobak_ctx_set_tuning(hobak_ctx* ctx, int32_t key, int32_t value) {
       if (!ctx->insts.empty() && value != ctx->update_groups)
         return fail(ctx, HOBAK_E_INVALID, "the update groups are fixed once instances exist");
       ctx->update_groups = value;
+      return HOBAK_OK;
+    case HOBAK_TUNE_P_FORM:
+      if (value < 0 || value > 2) return fail(ctx, HOBAK_E_INVALID, "p-form is 0, 1 or 2");
+      if (value != ctx->p_form) {  // cached graphs hold the old choice
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        for (auto* P : ctx->plans) plan_free(P);
+        ctx->plans.clear();
+        for (auto* S : ctx->scheds) S->mplan = nullptr;
+      }
+      ctx->p_form = value;
       return HOBAK_OK;
     case HOBAK_TUNE_UPDATE_WGS:
       if (value < 1 || value > 65536) return fail(ctx, HOBAK_E_INVALID, "update workgroups out of range");

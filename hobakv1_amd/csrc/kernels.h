@@ -299,6 +299,12 @@ hipError_t launch_cg_spmv(const Batch& b, int shift, int pad, int stream, int pa
                           hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
 template <typename T>
 hipError_t launch_cg_update(const Batch& b, int groups, int parity, int cap, hipStream_t s);
+// The FP64 p-form of an iteration (solver_kernels.hip): p finishes the previous iteration and
+// forms p_new = z + beta p_old over the update kernel's workgroups (Batch of kind K_UPD);
+// spmv_p forms t = A p_new gathering p_new only.  Bitwise the same results as the two-kernel form.
+hipError_t launch_cg_p(const Batch& upd, int parity, int cap, hipStream_t s);
+hipError_t launch_cg_spmv_p(const Batch& b, int shift, int pad, int stream, int parity, int cap, hipStream_t s,
+                            hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
 hipError_t launch_refine_begin(const Batch& b, int cap, hipStream_t s);
 template <typename T>
 hipError_t launch_refine_init(const Batch& b, int gate, int cap, hipStream_t s);

@@ -1107,6 +1107,196 @@ __global__ __launch_bounds__(kBlock) void cg_spmv_kernel(Batch b, int parity) {
   }
 }
 
+// ---------------------------------------------------------------------------------------------
+// The FP64 iteration in three kernels (the p-form), for solving sets whose matrices stream from
+// HBM: the SpMV above gathers z and p_old at every column to form p_new = z + beta p_old on the
+// fly, two gathered vectors per block; here
+//   p(p):       the previous iteration is finished exactly as the SpMV's first part does it (the
+//               same partial sums, tests and scalar set), and p_new = z + beta p_old is formed once
+//               per entry, with the same fused multiply-add;
+//   spmv_p(p):  t = A p_new gathering p_new only, the partials of p.t;
+//   update(p):  unchanged.
+// Every value is computed by the same operations in the same order as in the two-kernel form, so
+// the results are bitwise those of the two-kernel form.
+// ---------------------------------------------------------------------------------------------
+// One thread per DOF entry of p, over the instance's update-kernel workgroups (capped per
+// instance, so few workgroups re-reduce the update kernel's partials).
+__global__ __launch_bounds__(kBlock) void cg_p_kernel(Batch b, int parity) {
+  for (int ws = blockIdx.x; ws < b.total; ws += gridDim.x) {
+    const int vb = xcd_vb(ws, b.total);
+    int lb, lg;
+    const HOBAK_GLOBAL InstDev* __restrict__ I = seg_find(b, vb, lb, lg);
+    const int nupd = I->nb[K_UPD];
+    const auto prr = gbl(I->part_rr), prz = gbl(I->part_rz);
+    const auto p0 = gbl((double*)I->p0), p1 = gbl((double*)I->p1);
+    const auto z = gbl((const double*)I->z);
+    auto ctl = ctl_of(I);
+    const auto S = &ctl->sc[parity];
+    auto N = &ctl->sc[parity ^ 1];
+    const int done = S->done;
+    const int first = S->first;
+    const double absOld = S->absNew;
+    const int iters = S->iters;
+    const double threshold = S->threshold;
+    const int maxIters = S->maxIters;
+    const bool lead = lb == 0 && threadIdx.x == 0;
+    if (lead) {
+      N->rhsNorm2 = S->rhsNorm2;
+      N->threshold = threshold;
+      N->absNew = absOld;
+      N->alpha = S->alpha;
+      N->beta = S->beta;
+      N->resNorm2 = S->resNorm2;
+      N->maxR = S->maxR;
+      N->iters = iters;
+      N->maxIters = maxIters;
+      N->done = done;
+      N->status = S->status;
+      N->first = 0;
+    }
+    if (done) continue;
+    double beta = 0.0;
+    if (!first) {
+      PreParts q;
+      pre_load<true>(prr, prz, nupd, q);
+      double rrv, rz;
+      pre_sum<true>(prr, prz, nupd, q, rrv, rz);
+      if (nonfinite(rrv) || !(rz == rz)) {
+        if (lead) {
+          N->resNorm2 = rrv;
+          N->done = 1;
+          N->status = 2;
+        }
+        continue;
+      }
+      if (rrv < threshold) {
+        if (lead) {
+          N->resNorm2 = rrv;
+          N->done = 1;
+        }
+        continue;
+      }
+      const double bt = rz / absOld;
+      if (lead) {
+        N->resNorm2 = rrv;
+        N->beta = bt;
+        N->absNew = rz;
+        N->iters = iters + 1;
+        if (iters + 1 >= maxIters) {
+          N->done = 1;
+          N->status = 1;
+        }
+      }
+      if (iters + 1 >= maxIters) continue;
+      beta = bt;
+    }
+    const auto pold = parity ? p1 : p0;
+    const auto pnew = parity ? p0 : p1;
+    const int n = 3 * I->nv;
+    for (int i = lb * kBlock + threadIdx.x; i < n; i += lg * kBlock) pnew[i] = first ? z[i] : z[i] + beta * pold[i];
+  }
+}
+
+// t = A p_new gathering p_new only (FP64 standard form); the layout and loop are the SpMV's.
+template <int H, bool PAD, bool NT>
+__global__ __launch_bounds__(kBlock) void cg_spmv_p_kernel(Batch b, int parity) {
+  typedef double T;
+  for (int ws = blockIdx.x; ws < b.total; ws += gridDim.x) {
+    const int vb = xcd_vb(ws, b.total);
+    int lb, lg;
+    const HOBAK_GLOBAL InstDev* __restrict__ I = seg_find(b, vb, lb, lg);
+    const int lane = threadIdx.x & 63;
+    const int slice = (int)((lb * (unsigned)kBlock + threadIdx.x) >> 6);
+    const int rr = lane % H;
+    const int r = slice * H + rr;
+    const int nrows = I->nv;
+    const size_t ns = (size_t)I->nslots;
+    const auto slice_off = gbl(I->slice_off);
+    const auto row_len = gbl(I->row_len);
+    const auto pnew = gbl((const T*)(parity ? I->p0 : I->p1));
+    const auto A = gbl((const T*)I->A);
+    const auto col = gbl(I->col);
+    const auto t = gbl((T*)I->t);
+    const auto ppt = gbl(I->part_pt);
+    if (ctl_of(I)->sc[parity ^ 1].done) continue;  // the p kernel ended the solve
+    const int nslices = (nrows + H - 1) / H;
+    const bool wave_live = slice < nslices;
+    const bool live = r < nrows;
+    const int sc = wave_live ? slice : nslices - 1;
+    const int rc = live ? r : nrows - 1;
+    int sbeg = slice_off[sc];
+    int send = slice_off[sc + 1];
+    int len = PAD ? 0 : row_len[rc];
+    const size_t io = 3 * (size_t)rc;
+    T po0 = pnew[io], po1 = pnew[io + 1], po2 = pnew[io + 2];
+    if (!wave_live) sbeg = send = 0;
+    if (!live) {
+      len = 0;
+      po0 = po1 = po2 = T(0);
+    }
+    const int nit = wave_live ? (send - sbeg) >> 6 : 0;
+    const int nj = PAD ? nit : (live ? (len - lane / H + (64 / H) - 1) / (64 / H) : 0);
+    const unsigned slot0 = (unsigned)(sbeg + lane);
+    auto slot_of = [&](int j) -> unsigned { return j < nj ? slot0 + 64u * (unsigned)j : slot0; };
+    unsigned ca = 0u, cb = 0u;
+    if (wave_live) {
+      ca = ld_at(col, slot_of(0) * 4u);
+      cb = ld_at(col, slot_of(1) * 4u);
+    }
+    T s0 = T(0), s1 = T(0), s2 = T(0);
+    if (wave_live) {
+      constexpr unsigned ts = sizeof(T), vs = 3 * sizeof(T);
+      const size_t plane = ns * ts;
+      for (int j = 0; j < nit; j += 2) {
+        const unsigned oa = slot_of(j) * ts, ob = slot_of(j + 1) * ts;
+        const unsigned ga = ca * vs, gb = cb * vs;
+        T ma[9], mb[9];
+#pragma unroll
+        for (int e = 0; e < 9; e++) {
+          ma[e] = NT ? ld_stream(A, (size_t)e * plane, oa) : ld_at(A, (size_t)e * plane, oa);
+          mb[e] = NT ? ld_stream(A, (size_t)e * plane, ob) : ld_at(A, (size_t)e * plane, ob);
+        }
+        const T qa0 = ld_at(pnew, 0, ga), qa1 = ld_at(pnew, ts, ga), qa2 = ld_at(pnew, 2 * ts, ga);
+        const T qb0 = ld_at(pnew, 0, gb), qb1 = ld_at(pnew, ts, gb), qb2 = ld_at(pnew, 2 * ts, gb);
+        ca = ld_at(col, slot_of(j + 2) * 4u);
+        cb = ld_at(col, slot_of(j + 3) * 4u);
+        const bool va = j < nj, vb2 = j + 1 < nj;
+        const T pa0 = va ? qa0 : T(0), pa1 = va ? qa1 : T(0), pa2 = va ? qa2 : T(0);
+        const T pb0 = vb2 ? qb0 : T(0), pb1 = vb2 ? qb1 : T(0), pb2 = vb2 ? qb2 : T(0);
+        s0 += ma[0] * pa0 + ma[1] * pa1 + ma[2] * pa2;
+        s1 += ma[3] * pa0 + ma[4] * pa1 + ma[5] * pa2;
+        s2 += ma[6] * pa0 + ma[7] * pa1 + ma[8] * pa2;
+        s0 += mb[0] * pb0 + mb[1] * pb1 + mb[2] * pb2;
+        s1 += mb[3] * pb0 + mb[4] * pb1 + mb[5] * pb2;
+        s2 += mb[6] * pb0 + mb[7] * pb1 + mb[8] * pb2;
+        asm volatile("" : "+v"(ca), "+v"(cb));
+      }
+    }
+#pragma unroll
+    for (int off = H; off < 64; off <<= 1) {
+      s0 += __shfl_xor(s0, off, 64);
+      s1 += __shfl_xor(s1, off, 64);
+      s2 += __shfl_xor(s2, off, 64);
+    }
+    double pt = 0.0;
+    if (live && lane < H) pt = (double)po0 * (double)s0 + (double)po1 * (double)s1 + (double)po2 * (double)s2;
+    if (wave_live) {
+      const size_t base = 3 * (size_t)slice * H;
+      const int nval = 3 * min(H, nrows - slice * H);
+#pragma unroll
+      for (int d0 = 0; d0 < 3 * H; d0 += 64) {
+        const int d = d0 + lane;
+        const int rho = d / 3;
+        const int comp = d - 3 * rho;
+        const int src = rho < H ? rho : 0;
+        const T v0 = __shfl(s0, src, 64), v1 = __shfl(s1, src, 64), v2 = __shfl(s2, src, 64);
+        if (d < nval) t[base + d] = comp == 0 ? v0 : comp == 1 ? v1 : v2;
+      }
+    }
+    publish(pt, 0.0, ppt, nullptr, lb);
+  }
+}
+
 // One lane per DOF of each of R groups of 21 rows (63 lanes) per wavefront, so that a row's
 // three DOFs share a wave and the 3x3 preconditioner apply can read its siblings' residuals with
 // cross-lane moves.  A wave covers 21 R consecutive rows and issues the loads of all R groups
@@ -1538,6 +1728,41 @@ static hipError_t spmv_h(const Batch& b, int parity, int cap, int pad, int strea
                : spmv_hp<T, H, false, true>(b, parity, cap, s, start, stop);
   return pad ? spmv_hp<T, H, true, false>(b, parity, cap, s, start, stop)
              : spmv_hp<T, H, false, false>(b, parity, cap, s, start, stop);
+}
+
+template <int H, bool PAD, bool NT>
+static hipError_t spmv_p_hp(const Batch& b, int parity, int cap, hipStream_t s, hipEvent_t start, hipEvent_t stop) {
+  const dim3 g(grid_of(b, cap)), blk(kBlock);
+  if (start)
+    hipExtLaunchKernelGGL((cg_spmv_p_kernel<H, PAD, NT>), g, blk, 0, s, start, stop, 0, b, parity);
+  else
+    hipLaunchKernelGGL((cg_spmv_p_kernel<H, PAD, NT>), g, blk, 0, s, b, parity);
+  return hipGetLastError();
+}
+template <int H>
+static hipError_t spmv_p_h(const Batch& b, int parity, int cap, int pad, int stream, hipStream_t s,
+                           hipEvent_t start, hipEvent_t stop) {
+  if (stream)
+    return pad ? spmv_p_hp<H, true, true>(b, parity, cap, s, start, stop)
+               : spmv_p_hp<H, false, true>(b, parity, cap, s, start, stop);
+  return pad ? spmv_p_hp<H, true, false>(b, parity, cap, s, start, stop)
+             : spmv_p_hp<H, false, false>(b, parity, cap, s, start, stop);
+}
+
+hipError_t launch_cg_p(const Batch& upd, int parity, int cap, hipStream_t s) {
+  SEG_LAUNCH(cg_p_kernel, upd, cap, s, parity);
+}
+
+hipError_t launch_cg_spmv_p(const Batch& b, int shift, int pad, int stream, int parity, int cap, hipStream_t s,
+                            hipEvent_t start, hipEvent_t stop) {
+  if (b.total <= 0) return hipSuccess;
+  switch (shift) {
+    case 3: return spmv_p_h<8>(b, parity, cap, pad, stream, s, start, stop);
+    case 4: return spmv_p_h<16>(b, parity, cap, pad, stream, s, start, stop);
+    case 5: return spmv_p_h<32>(b, parity, cap, pad, stream, s, start, stop);
+    case 6: return spmv_p_h<64>(b, parity, cap, pad, stream, s, start, stop);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 template <typename T>

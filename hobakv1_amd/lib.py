@@ -143,6 +143,7 @@ TUNE_SPMV_STREAM_MB = 8
 TUNE_ROW_SORT = 9
 TUNE_COMPACT_PCT = 10
 TUNE_UPDATE_GROUPS = 11
+TUNE_P_FORM = 12
 
 _lib = None
 
@@ -243,14 +244,14 @@ class Context:
 
     def set_tuning(self, slice_rows=None, poll_chunk=None, max_chunks=None, graphs=None,
                    grid_cap=None, update_wgs=None, spmv_pad=None, spmv_stream_mb=None,
-                   row_sort=None, compact_pct=None, update_groups=None):
+                   row_sort=None, compact_pct=None, update_groups=None, p_form=None):
         """Kernel and engine tuning (see hobak_ctx_set_tuning)."""
         for key, val in ((TUNE_SLICE_ROWS, slice_rows), (TUNE_POLL_CHUNK, poll_chunk),
                          (TUNE_MAX_CHUNKS, max_chunks), (TUNE_GRAPHS, graphs),
                          (TUNE_GRID_CAP, grid_cap), (TUNE_UPDATE_WGS, update_wgs),
                          (TUNE_SPMV_PAD, spmv_pad), (TUNE_SPMV_STREAM_MB, spmv_stream_mb),
                          (TUNE_ROW_SORT, row_sort), (TUNE_COMPACT_PCT, compact_pct),
-                         (TUNE_UPDATE_GROUPS, update_groups)):
+                         (TUNE_UPDATE_GROUPS, update_groups), (TUNE_P_FORM, p_form)):
             if val is not None:
                 self.check(self.lib.hobak_ctx_set_tuning(self.h, key, int(val)), "set_tuning")
 

@@ -265,7 +265,13 @@ int hobak_kernel_stats(hobak_inst* inst, double* spmv_avg_ms, int64_t* spmv_coun
  *   HOBAK_TUNE_UPDATE_GROUPS  groups of 21 rows that one wave of the PCG update kernel covers
  *                          (it issues all their loads before using any): 1, 2 (default) or 4;
  *                          fixed once the context holds instances.  The r.r and r.z partial
- *                          sums follow the wave layout, so FP64 results differ in the last bits */
+ *                          sums follow the wave layout, so FP64 results differ in the last bits
+ *   HOBAK_TUNE_P_FORM      FP64 PCG iterations in three kernels: the previous iteration is
+ *                          finished and p_new = z + beta p_old formed once per entry, then the
+ *                          SpMV gathers p_new alone (the two-kernel form gathers z and p_old at
+ *                          every block).  0: never; 1 (default): for groups whose matrices stream
+ *                          (see HOBAK_TUNE_SPMV_STREAM_MB); 2: always.  Results do not depend on
+ *                          it (bitwise) */
 #define HOBAK_TUNE_SLICE_ROWS 0
 #define HOBAK_TUNE_POLL_CHUNK 2
 #define HOBAK_TUNE_MAX_CHUNKS 3
@@ -277,6 +283,7 @@ int hobak_kernel_stats(hobak_inst* inst, double* spmv_avg_ms, int64_t* spmv_coun
 #define HOBAK_TUNE_ROW_SORT 9
 #define HOBAK_TUNE_COMPACT_PCT 10
 #define HOBAK_TUNE_UPDATE_GROUPS 11
+#define HOBAK_TUNE_P_FORM 12
 int hobak_ctx_set_tuning(hobak_ctx* ctx, int32_t key, int32_t value);
 
 /* Continuous-batching scheduler (no counterpart in the reference, whose callers step one scene
