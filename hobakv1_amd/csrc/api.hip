@@ -27,6 +27,7 @@
 //   4. one status kernel and one small copy, then the host waits and reads the statuses.
 // Requests submitted while others are on the GPU join at the next tick: continuous batching.
 #include <hip/hip_runtime.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include <algorithm>
 #include <array>
@@ -363,8 +364,9 @@ struct hobak_sched {
   double ms[5] = {0, 0, 0, 0, 0};
   // counters since creation: ticks, PCG groups queued, GPU ms of prologues / PCG / epilogues,
   // host wall ms inside ticks, solves finished, SpMV launches, their algorithmic bytes (those of
-  // the instances solving in each launch's tick)
-  double stat[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  // the instances solving in each launch's tick), PCG groups replayed from a captured graph and
+  // PCG groups launched directly
+  double stat[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   // sampled SpMV launches of the current batch: (iteration index since the batch's prologue, slot)
   std::vector<std::pair<int, int>> samples;
   int sample_it = 0;
@@ -377,6 +379,15 @@ struct hobak_sched {
 };
 
 namespace {
+
+// A profiler range (roctx) over a scope of the step engine: rocprofv3 --marker-trace shows the
+// prologue, PCG groups, epilogue and poll of every tick, and every hobak_step / batch / tick call.
+struct Range {
+  explicit Range(const char* what) { roctxRangePushA(what); }
+  ~Range() { roctxRangePop(); }
+  Range(const Range&) = delete;
+  Range& operator=(const Range&) = delete;
+};
 
 int fail(hobak_ctx* ctx, int code, const std::string& msg) {
   if (ctx) ctx->err = msg;
@@ -820,6 +831,7 @@ int tick_impl(hobak_sched* S) {
   }
   const bool epi = !fin.empty();
   if (epi) {
+    const Range r_epi("hobak epilogue");
     HIPCHK(ctx, hipEventRecord(ctx->ev[6], s));
     if (graphs && (int)fin.size() == M.n && comp.size() == fin.size()) {
       if (!M.epi_exec) TRY(capture(ctx, &M.epi_exec, [&]() { return launch_epilogue<T>(ctx, M, &M); }));
@@ -864,6 +876,7 @@ int tick_impl(hobak_sched* S) {
   const bool pro = !join.empty();
   bool pro_split = false;
   if (pro) {
+    const Range r_pro("hobak prologue");
     HIPCHK(ctx, hipEventRecord(ctx->ev[0], s));
     const bool all = (int)join.size() == M.n && (starters.empty() || starters.size() == join.size());
     if (graphs && all) {
@@ -921,6 +934,7 @@ int tick_impl(hobak_sched* S) {
     m = std::min(m, std::max(1, q));
   }
   if (nsolving) {
+    const Range r_pcg("hobak pcg groups");
     HIPCHK(ctx, hipEventRecord(ctx->ev[4], s));
     int every = 0;
     if (S->sampling && S->ephemeral) {
@@ -948,8 +962,10 @@ int tick_impl(hobak_sched* S) {
       if (graphs && !compact && !sampled) {
         HIPCHK(ctx, hipGraphLaunch(M.chunk_exec[st], s));
         if (S->sampling) S->sample_it += k;
+        S->stat[9] += 1;
       } else {
         TRY(launch_chunk<T>(ctx, compact ? PG : M, k, st, S->sampling ? S : nullptr, every));
+        S->stat[10] += 1;
       }
     }
     HIPCHK(ctx, hipEventRecord(ctx->ev[5], s));
@@ -962,9 +978,12 @@ int tick_impl(hobak_sched* S) {
     S->stat[8] += (double)m * k * solving_bytes;
   }
   // 5. statuses
-  HIPCHK(ctx, launch_status(M.b[K_ONE], M.status_dev, s));
-  HIPCHK(ctx, hipMemcpyAsync(M.status_host, M.status_dev, sizeof(InstStatus) * M.n, hipMemcpyDeviceToHost, s));
-  HIPCHK(ctx, hipStreamSynchronize(s));
+  {
+    const Range r_poll("hobak poll");
+    HIPCHK(ctx, launch_status(M.b[K_ONE], M.status_dev, s));
+    HIPCHK(ctx, hipMemcpyAsync(M.status_host, M.status_dev, sizeof(InstStatus) * M.n, hipMemcpyDeviceToHost, s));
+    HIPCHK(ctx, hipStreamSynchronize(s));
+  }
   const double now = clock_s(S);
   if (pro) {
     if (pro_split) {
@@ -1768,6 +1787,7 @@ int hobak_step_batch(hobak_ctx* ctx, hobak_inst* const* insts, int32_t n, hobak_
   if (!ctx || n < 0 || (n > 0 && !insts)) return HOBAK_E_INVALID;
   if (n == 0) return HOBAK_OK;
   (void)hipSetDevice(ctx->device);
+  const Range r_call(n == 1 ? "hobak_step" : "hobak_step_batch");
   std::vector<hobak_inst*> g64, g32;
   std::vector<int> i64, i32;
   std::set<hobak_inst*> seen;
@@ -2051,6 +2071,7 @@ int hobak_sched_tick(hobak_sched* S, hobak_completion* out, int32_t max_out, int
   if (!S || max_out < 0 || (max_out > 0 && !out)) return HOBAK_E_INVALID;
   (void)hipSetDevice(S->ctx->device);
   if (!S->queue.empty() || !S->active.empty()) {
+    const Range r_call("hobak_sched_tick");
     const auto t0 = std::chrono::steady_clock::now();
     const int rc = tick(S);
     S->stat[5] += 1e3 * std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

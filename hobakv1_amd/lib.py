@@ -484,10 +484,11 @@ class Scheduler:
         return {"spmv_avg_ms": ms.value, "spmv_count": n.value, "spmv_bytes": b.value}
 
     def stats(self):
-        out = np.zeros(9)
+        out = np.zeros(11)
         self.ctx.check(self.lib.hobak_sched_stats(self.h, _d(out)), "sched_stats")
         keys = ["ticks", "pcg_groups", "prologue_gpu_ms", "pcg_gpu_ms", "epilogue_gpu_ms",
-                "tick_host_ms", "solves", "spmv_launches", "spmv_bytes"]
+                "tick_host_ms", "solves", "spmv_launches", "spmv_bytes", "pcg_groups_graph",
+                "pcg_groups_direct"]
         d = dict(zip(keys, out.tolist()))
         d["spmv_bytes_per_launch"] = d["spmv_bytes"] / d["spmv_launches"] if d["spmv_launches"] else 0.0
         return d

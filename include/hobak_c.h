@@ -316,10 +316,15 @@ int hobak_sched_tick(hobak_sched* sched, hobak_completion* out, int32_t max_out,
                      int32_t* in_flight);
 /* The scheduler clock (seconds since hobak_sched_create). */
 double hobak_sched_clock(hobak_sched* sched);
-/* Counters since creation, 9 doubles: ticks, PCG groups queued, GPU milliseconds of prologues,
+/* Counters since creation, 11 doubles: ticks, PCG groups queued, GPU milliseconds of prologues,
  * of PCG groups and of epilogues, host milliseconds inside hobak_sched_tick, solves finished,
- * SpMV launches queued, and their algorithmic bytes summed (per launch: the matrices and vectors
- * of the instances solving in its tick, nnzb (9 s + 4) + 4 (nv + 1) + 2 * 3 s nv each). */
+ * SpMV launches queued, their algorithmic bytes summed (per launch: the matrices and vectors of
+ * the instances solving in its tick, nnzb (9 s + 4) + 4 (nv + 1) + 2 * 3 s nv each), PCG groups
+ * replayed from a captured graph, and PCG groups launched directly (over the still-solving
+ * instances only, or holding a sampled launch).
+ * The engine also marks its work with roctx ranges ("hobak_step", "hobak_step_batch",
+ * "hobak_sched_tick" and, inside, "hobak prologue", "hobak pcg groups", "hobak epilogue",
+ * "hobak poll"), which rocprofv3 --marker-trace records. */
 int hobak_sched_stats(hobak_sched* sched, double* out);
 /* Measurement hook of a scheduler: every `every`-th SpMV launch of its PCG groups is bracketed
  * with HIP events on the context's stream (its group runs uncaptured); 0 turns it off.  Setting
