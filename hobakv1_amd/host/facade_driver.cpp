@@ -5,7 +5,7 @@
 //   facade_driver <scene.bin> <out.bin>        (HOBAK_FACADE_PRECOND=0: Jacobi PCG)
 //
 // scene.bin (little-endian): int32 nv, nt, material, steps, flags (1: pin cube, 2: collision
-// cube, 4: GREEN_DAMPING); float64 mu, lambda, dt, rayleigh alpha, beta, mass scale, gravity[3],
+// cube, 4: GREEN_DAMPING, 8: BACKWARD_EULER_VELOCITY in place of BACKWARD_EULER_POSITION); float64 mu, lambda, dt, rayleigh alpha, beta, mass scale, gravity[3],
 // pcg tol, pin cube center[3] and scale, collision cube center[3] and scale; then rest (nv*3
 // float64), tets (nt*4 int32), u0 and v0 (3*nv float64 each); with flag 4 also the damping mu
 // (float64) and the velocity given to TET_MESH::computeFdots before the first step (3*nv float64).
@@ -13,6 +13,7 @@
 // v[3*nv].
 #include <cstdio>
 #include <cstdlib>
+#include <memory>
 #include <vector>
 
 #include "hobak.hpp"
@@ -53,7 +54,12 @@ int main(int argc, char** argv) {
     hobak::VOLUME::ARAP arap(prm[0], prm[1]);
     hobak::VOLUME::HYPERELASTIC& material = mat == HOBAK_MAT_SNH ? (hobak::VOLUME::HYPERELASTIC&)snh
                                                                   : (hobak::VOLUME::HYPERELASTIC&)arap;
-    hobak::BACKWARD_EULER_POSITION solver(mesh, material);
+    std::unique_ptr<hobak::BACKWARD_EULER_POSITION> integrator;
+    if (flags & 8)
+      integrator.reset(new hobak::BACKWARD_EULER_VELOCITY(mesh, material));
+    else
+      integrator.reset(new hobak::BACKWARD_EULER_POSITION(mesh, material));
+    hobak::BACKWARD_EULER_POSITION& solver = *integrator;
     solver.setDt(prm[2]);
     solver.setRayeligh(prm[3], prm[4]);
     solver.scaleMass(prm[5]);

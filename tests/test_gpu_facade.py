@@ -24,7 +24,8 @@ def _write_scene(path, sc, steps):
     cube = sc.pin_cube if sc.pin_cube is not None else (0.0, 0.0, 0.0, 1.0)
     ground = sc.collision_cubes[0] if sc.collision_cubes else (0.0, 0.0, 0.0, 1.0)
     flags = ((1 if sc.pin_cube is not None else 0) | (2 if sc.collision_cubes else 0)
-             | (4 if sc.green_mu is not None else 0))
+             | (4 if sc.green_mu is not None else 0)
+             | (8 if sc.integrator == scenes.INTEGRATOR_VELOCITY else 0))
     with open(path, "wb") as f:
         f.write(struct.pack("<5i", sc.nv, sc.nt, sc.material, steps, flags))
         f.write(struct.pack("<18d", sc.mu, sc.lam, sc.dt, sc.rayleigh_alpha, sc.rayleigh_beta,
@@ -162,3 +163,34 @@ def test_python_facade_energy_damping_matches_c_abi(gpu_ctx):
     ud, vd = inst.state()
     inst.close()
     assert np.array_equal(be.position(), ud) and np.array_equal(be.velocity(), vd)
+
+
+def test_cpp_facade_velocity_integrator_matches_reference(tmp_path):
+    """BACKWARD_EULER_VELOCITY through the C++ façade on the reference's collision separation
+    test (SCENE_TESTS.h:367-453, Jacobi PCG tol 1e-8): the same plane-constraint counts every
+    step and PCG iterations +-1 as the reference's own run, the state within 1e-9 of it after 20
+    steps, and bitwise the state of the C ABI."""
+    from conftest import GOLDEN, golden
+
+    g = golden("separation_test.npz")
+    sc = scenes.separation_test(os.path.join(GOLDEN, "cube_6.tobj"))
+    steps = 20
+    scene_bin = tmp_path / "scene.bin"
+    out_bin = tmp_path / "out.bin"
+    _write_scene(scene_bin, sc, steps)
+    env = dict(os.environ, HOBAK_FACADE_PRECOND="0")
+    print(f"C++ façade: velocity integrator, separation test ({sc.nt} tets), {steps} steps", flush=True)
+    subprocess.run([hb.DRIVER, str(scene_bin), str(out_bin)], check=True, timeout=120, env=env)
+    iters, u, v, counts = _read_out(out_bin, sc.dofs, planes=True)
+    assert counts == g["counts"][:steps].tolist()
+    assert max(abs(a - b) for a, b in zip(iters, g["iters"][:steps])) <= 1
+    X = sc.rest.ravel()
+    assert rel(X + u, X + g["u20"]) <= 1e-9
+    ctx = lib.Context(0, apply_table=False)
+    inst = lib.Instance(ctx, sc, precond=scenes.PRECOND_JACOBI)
+    for _ in range(steps):
+        inst.step()
+    ud, vd = inst.state()
+    inst.close()
+    ctx.close()
+    assert np.array_equal(u, ud) and np.array_equal(v, vd)
