@@ -122,6 +122,8 @@ def measured_traffic(name, precision):
            "source": os.path.relpath(path, ROOT)}
     if "algorithmic_bytes_per_launch" in vals:
         out["algorithmic"] = float(vals["algorithmic_bytes_per_launch"])
+    if "traffic_to_algorithmic" in vals:
+        out["ratio"] = float(vals["traffic_to_algorithmic"])
     return out
 
 
@@ -451,9 +453,12 @@ def serving_form(lib, scenes, args):
         tr = measured_traffic("serving_stream", "fp32")
         out["roofline"] = {"bound": "hbm", "kernel": "cg_spmv_kernel<float>", "achieved": round(gbs, 1),
                            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
-                           "traffic": tr["bytes"] if tr else None,
+                           # launches differ in size: the profiled run's traffic/algorithmic
+                           # ratio, applied to this run's bytes per launch
+                           "traffic": round(tr["ratio"] * ks["spmv_bytes"]) if tr and "ratio" in tr
+                           else None,
+                           "traffic_to_algorithmic": tr.get("ratio") if tr else None,
                            "traffic_source": tr["source"] if tr else None,
-                           "traffic_run_algorithmic_bytes": tr.get("algorithmic") if tr else None,
                            "bytes_per_launch": ks["spmv_bytes"], "avg_launch_ms": ks["spmv_avg_ms"],
                            "launches_sampled": ks["spmv_count"],
                            "stream_avg_bytes_per_launch": eng.get("spmv_bytes_per_launch"),
