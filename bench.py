@@ -48,7 +48,7 @@ FP64_VALU_PEAK_TFLOPS = 78.6
 # rocprofv3 (64 x SQ_INSTS_VALU_FLOPS_FP64 incl. transcendental ops / tets), and the kernel's
 # VALUBusy from the same runs: profiles/element_kernel_40cube_valu.md
 ELEMENT_FP64_FLOPS_PER_TET = 2158.0
-ELEMENT_VALU_BUSY = 0.121
+ELEMENT_VALU_BUSY = 0.264
 GRAPHS = [1]  # --graphs, applied to every context the benchmark creates
 
 

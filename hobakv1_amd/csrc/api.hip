@@ -121,9 +121,9 @@ struct Pattern {
   int H = 16, shift = 4;  // rows per slice and its log2
   std::vector<int> row_ptr, col_idx;  // BSR
   std::vector<int> slice_off, row_len, col, slot_row, bsr_slot, diag_slot;
-  std::vector<int> cptr, hdest;       // per slot contribution ranges; per (tet, x <= y) position
+  std::vector<int> cptr, cidx;        // per upper slot: its contributions (pair * nt + t | flag)
   std::vector<int> mirror;            // per upper off-diagonal slot: its transposed slot
-  std::vector<int> vptr, fdest;       // per vertex force ranges; per (tet, x) position
+  std::vector<int> vptr, fidx;        // per vertex: its incident corners (x * nt + t)
 };
 
 // Block sparsity (BSR rows by vertex, columns ascending), its SELL-H slot layout, and the
@@ -139,12 +139,13 @@ void build_pattern(int nv, int nt, const int32_t* tets, int H, Pattern& P,
   P.shift = 0;
   while ((1 << P.shift) < H) P.shift++;
   const int L = 64 / H;
-  // vertex -> incident tets, ascending (also the force-gather lists: fdest = position in them)
+  // vertex -> incident tets, ascending, and the corner of each (the force-gather lists)
   P.vptr.assign(nv + 1, 0);
   for (size_t i = 0; i < 4 * (size_t)nt; i++) P.vptr[tets[i] + 1]++;
   for (int i = 0; i < nv; i++) P.vptr[i + 1] += P.vptr[i];
   std::vector<int> inc(P.vptr[nv]);
-  P.fdest.assign(4 * (size_t)nt, 0);
+  std::vector<unsigned char> corner(P.vptr[nv]);
+  P.fidx.assign(4 * (size_t)nt, 0);
   {
     std::vector<int> fill(P.vptr.begin(), P.vptr.end() - 1);
     for (int t = 0; t < nt; t++)
@@ -152,7 +153,8 @@ void build_pattern(int nv, int nt, const int32_t* tets, int H, Pattern& P,
         const int v = tets[4 * (size_t)t + a];
         const int q = fill[v]++;
         inc[q] = t;
-        P.fdest[4 * (size_t)t + a] = q;
+        corner[q] = (unsigned char)a;
+        P.fidx[q] = a * nt + t;
       }
   }
   // rows: distinct vertices of each vertex's incident tets, ascending
@@ -245,21 +247,20 @@ void build_pattern(int nv, int nt, const int32_t* tets, int H, Pattern& P,
   }
   P.cptr.assign(P.nslots + 1, 0);
   for (int s = 0; s < P.nslots; s++) P.cptr[s + 1] = P.cptr[s] + cnt[s + 1];
-  P.hdest.assign((size_t)kStagedBlocks * nt, 0);
+  P.cidx.assign(P.cptr[P.nslots], 0);  // 10 nt: the 10 pairs of 4 distinct corners per tet
   std::vector<int> fill(P.cptr.begin(), P.cptr.end() - 1);
   for (int i = 0; i < nv; i++) {
     row_slots(i);
     for (int q = P.vptr[i]; q < P.vptr[i + 1]; q++) {
       const int t = inc[q];
       const int32_t* T = tets + 4 * (size_t)t;
-      const int x = P.fdest[4 * (size_t)t + 0] == q ? 0 : P.fdest[4 * (size_t)t + 1] == q ? 1
-                  : P.fdest[4 * (size_t)t + 2] == q ? 2 : 3;
+      const int x = corner[q];
       for (int y = 0; y < 4; y++) {
         if (T[y] < i) continue;
-        // the element computes block (min(x, y), max(x, y)); staged transposed when x > y
-        const int word = fill[pos[T[y]]]++;
-        P.hdest[(size_t)kStagedBlocks * t + staged_pair(std::min(x, y), std::max(x, y))] =
-            x > y ? (word | kHdestTranspose) : word;
+        // the element stages block (min(x, y), max(x, y)), rows of corner min(x, y): the slot
+        // (row T[x]) sums it transposed when x > y
+        const int w = staged_pair(std::min(x, y), std::max(x, y)) * nt + t;
+        P.cidx[fill[pos[T[y]]]++] = x > y ? (w | kCidxTranspose) : w;
       }
     }
   }
@@ -479,7 +480,7 @@ size_t estimate_bytes(int nv, int nt, int nslots_est, int precision) {
   b += (size_t)nt * (16 + 9 * 8 + 8);           // tets, dminv, vol
   b += (size_t)nslots_est * (4 + 4 + 9 * ts);   // col + slot_row + kneg
   b += (size_t)nslots_est * 9 * ts * 4;         // A, D, E, C
-  b += (size_t)nslots_est * 8 + (size_t)nt * (kStagedBlocks + 4) * 4;  // cptr, mirror, hdest, fdest
+  b += (size_t)nslots_est * 8 + (size_t)nt * (kStagedBlocks + 4) * 4;  // cptr, mirror, cidx, fidx
   b += (size_t)nt * (kStagedBlocks * 9 + 12) * ts;  // staging
   b += (size_t)nv * (9 + 18) * ts;              // minv + PCG vectors
   b += (size_t)nt * 4 * 4 + (size_t)nv * 16;    // vertex lists, misc
@@ -706,7 +707,7 @@ int launch_prologue(hobak_ctx* ctx, const Plan& J, const Plan* S, bool timed) {
   HIPCHK(ctx, launch_element<T>(J.b[K_ELEM], ELEM_U, 1, cap, s));
   if (J.damping) HIPCHK(ctx, launch_damping_force<T>(J.b[K_ELEM], ELEM_U, cap, s));
   if (timed) HIPCHK(ctx, hipEventRecord(ctx->ev[1], s));
-  HIPCHK(ctx, launch_assemble<T>(J.b[K_SLOT], ASM_SYSTEM, cap, s));
+  HIPCHK(ctx, launch_assemble<T>(J.b[K_ASM], ASM_SYSTEM, cap, s));
   if (timed) HIPCHK(ctx, hipEventRecord(ctx->ev[2], s));
   HIPCHK(ctx, launch_rhs<T>(J.b[K_ROW], J.refine ? 0 : 1, cap, s));
   if (J.refine) {
@@ -1167,7 +1168,7 @@ int setup_constant_parts(hobak_inst* I) {
   // rest-state stiffness (u = 0): TIMESTEPPER::buildRayleighDampingMatrix (TIMESTEPPER.cpp:213-236)
   HIPCHK(ctx, hipMemsetAsync(I->h.tmp64, 0, sizeof(double) * I->dofs, s));
   HIPCHK(ctx, launch_element<T>(P.b[K_ELEM], ELEM_TMP, 1, ctx->grid_cap, s));
-  HIPCHK(ctx, launch_assemble<T>(P.b[K_SLOT], ASM_REST, ctx->grid_cap, s));
+  HIPCHK(ctx, launch_assemble<T>(P.b[K_ASM], ASM_REST, ctx->grid_cap, s));
   HIPCHK(ctx, launch_constant_parts<T>(P.b[K_SLOT], ctx->grid_cap, s));
   HIPCHK(ctx, hipStreamSynchronize(s));
   return HOBAK_OK;
@@ -1210,7 +1211,7 @@ int eval_forces_hessian(hobak_inst* I, const double* u, double* forces, double* 
   }
   I->has_system = false;  // rhs64 and A are reused
   if (kvals) {
-    HIPCHK(ctx, launch_assemble<T>(P.b[K_SLOT], ASM_PLAIN_A, ctx->grid_cap, s));
+    HIPCHK(ctx, launch_assemble<T>(P.b[K_ASM], ASM_PLAIN_A, ctx->grid_cap, s));
     TRY(download_sell_as_bsr<T>(I, I->h.A, kvals, -1.0));
   }
   HIPCHK(ctx, hipStreamSynchronize(s));
@@ -1238,7 +1239,7 @@ int eval_damping(hobak_inst* I, const double* u, double* forces, double* cvals) 
   }
   if (cvals) {
     HIPCHK(ctx, launch_damping_hessian<T>(P.b[K_ELEM], ELEM_TMP, ctx->grid_cap, s));
-    HIPCHK(ctx, launch_assemble<T>(P.b[K_SLOT], ASM_PLAIN_A, ctx->grid_cap, s));
+    HIPCHK(ctx, launch_assemble<T>(P.b[K_ASM], ASM_PLAIN_A, ctx->grid_cap, s));
     TRY(download_sell_as_bsr<T>(I, I->h.A, cvals, -1.0));
   }
   HIPCHK(ctx, hipStreamSynchronize(s));
@@ -1489,7 +1490,7 @@ int hobak_inst_create(hobak_ctx* ctx, const double* rest_xyz, int32_t nv, const 
   if ((rc = dalloc(I, (char**)&ptr, (n), ts)) != HOBAK_OK) return bail(rc)
   double *rest, *mass, *one_ring, *pin_target, *dminv, *vol;
   int4* t4d;
-  int *slice_off, *row_len, *col, *slot_row, *cptr, *hdest, *vptr, *fdest, *mirror;
+  int *slice_off, *row_len, *col, *slot_row, *cptr, *cidx, *vptr, *fidx, *mirror;
   unsigned char* pinned;
   ALLOC(rest, 3 * (size_t)nv);
   ALLOC(h.u, 3 * (size_t)nv);
@@ -1511,14 +1512,14 @@ int hobak_inst_create(hobak_ctx* ctx, const double* rest_xyz, int32_t nv, const 
   ALLOC(col, (size_t)P.nslots);
   ALLOC(slot_row, (size_t)P.nslots);
   ALLOC(cptr, P.cptr.size());
-  ALLOC(hdest, P.hdest.size());
+  ALLOC(cidx, P.cidx.size());
   ALLOC(mirror, P.mirror.size());
   ALLOC(vptr, P.vptr.size());
-  ALLOC(fdest, P.fdest.size());
+  ALLOC(fidx, P.fidx.size());
   ALLOC(pinned, (size_t)nv);
   ALLOCT(h.kneg, 9 * (size_t)P.nslots);
-  ALLOCT(h.stage_h, (size_t)9 * P.cptr[P.nslots]);
-  ALLOCT(h.stage_f, (size_t)3 * P.vptr[nv]);
+  ALLOCT(h.stage_h, (size_t)9 * kStagedBlocks * nt);  // tet-major: 9 (pair nt + t) + entry
+  ALLOCT(h.stage_f, (size_t)3 * 4 * nt);              // 3 (corner nt + t) + component
   ALLOCT(h.A, 9 * (size_t)P.nslots);
   ALLOCT(h.D, 9 * (size_t)P.nslots);
   ALLOCT(h.E, 9 * (size_t)P.nslots);
@@ -1557,10 +1558,10 @@ int hobak_inst_create(hobak_ctx* ctx, const double* rest_xyz, int32_t nv, const 
   h.col = col;
   h.slot_row = slot_row;
   h.cptr = cptr;
-  h.hdest = hdest;
+  h.cidx = cidx;
   h.mirror = mirror;
   h.vptr = vptr;
-  h.fdest = fdest;
+  h.fidx = fidx;
   h.pinned = pinned;
   I->ctl = h.ctl;
   // no solve active: both scalar sets done, so every PCG kernel passes this instance over
@@ -1584,10 +1585,10 @@ int hobak_inst_create(hobak_ctx* ctx, const double* rest_xyz, int32_t nv, const 
       (rc = upload(I, col, P.col.data(), P.col.size())) ||
       (rc = upload(I, slot_row, P.slot_row.data(), P.slot_row.size())) ||
       (rc = upload(I, cptr, P.cptr.data(), P.cptr.size())) ||
-      (rc = upload(I, hdest, P.hdest.data(), P.hdest.size())) ||
+      (rc = upload(I, cidx, P.cidx.data(), P.cidx.size())) ||
       (rc = upload(I, mirror, P.mirror.data(), P.mirror.size())) ||
       (rc = upload(I, vptr, P.vptr.data(), P.vptr.size())) ||
-      (rc = upload(I, fdest, P.fdest.data(), P.fdest.size())) ||
+      (rc = upload(I, fidx, P.fidx.data(), P.fidx.size())) ||
       (rc = upload(I, I->ctl, &ctl0, 1)) ||
       (rc = upload(I, I->dev, &h, 1)))
     return bail(rc);

@@ -19,8 +19,11 @@
 // is formed directly as  V_e * U * M_vw * U^T  with
 //   M_vw[a][a']  = sum_{b,b'} w_v[b] Hhat[(a,b),(a',b')] w_w[b'],   w = (G DmInv) V,
 // G the 4x3 map from corner positions to edge vectors.  That is the same matrix up to round-off,
-// at about a third of the flops, and it needs only ~40 live values per lane.  Each block is
-// written straight to its place in the assembly's per-slot contribution list (hdest).
+// at about a third of the flops, and it needs only ~40 live values per lane.  Blocks and forces
+// are staged tet-major (block pair(x, y) of tet t at 9 * (pair * nt + t), the force of corner x at
+// 3 * (x * nt + t)); a wave's 64 consecutive tets pass each block through an LDS tile, so every
+// store instruction writes 64 consecutive values.  The assembly and the force gather read them
+// through per-slot and per-vertex lists in ascending tet order.
 #include "device_math.h"
 #include "kernels.h"
 
@@ -37,10 +40,34 @@ struct ElementIn {
   const HOBAK_GLOBAL double* vol;
   HOBAK_GLOBAL void* stage_h;
   HOBAK_GLOBAL void* stage_f;
-  const HOBAK_GLOBAL int* hdest;
-  const HOBAK_GLOBAL int* fdest;
   int with_hessian;
 };
+
+// Orders a wave's LDS accesses: the tile written by every lane is then read by every lane.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// The store of N values per lane for a wave's 64 consecutive tets (lane l: the wave's l-th tet) to
+// dst[N * l + e], of which the first nvalid tets are written.  Through the wave's LDS tile, so
+// that each store instruction writes 64 consecutive values (whole lines), where a direct store
+// would write 64 pieces N values apart.
+template <typename T, int N>
+__device__ __forceinline__ void wave_store(HOBAK_GLOBAL T* dst, const T (&v)[N], int nvalid, T* tile) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int e = 0; e < N; e++) tile[N * lane + e] = v[e];
+  wave_sync();
+  const int n = N * nvalid;
+#pragma unroll
+  for (int k = 0; k < N; k++) {
+    const int i = 64 * k + lane;
+    if (i < n) dst[i] = tile[i];
+  }
+  wave_sync();  // the tile is written again by the next store
+}
 
 // GREEN_DAMPING stress mu F (F^T Fdot + Fdot^T F) (src/Damping/Volume/GREEN_DAMPING.cpp:49-52),
 // with Fdot read from its 9 planes.
@@ -64,8 +91,10 @@ __device__ __forceinline__ void green_pk1(double mu, const double (&F)[3][3], co
                 (mu * F[i][2]) * (X[2][j] + X[j][2]);
 }
 
+// Tet t of a wave whose tets start at t0, the first nvalid of them real (the other lanes repeat
+// the last tet, and their results are not stored).
 template <typename T, int MAT>
-__device__ __forceinline__ void element_tet(const ElementIn& a, int t) {
+__device__ __forceinline__ void element_tet(const ElementIn& a, int t, int t0, int nvalid, T* tile) {
   const size_t nt = (size_t)a.nt;
   const int vid[4] = {a.tets[4 * (size_t)t], a.tets[4 * (size_t)t + 1], a.tets[4 * (size_t)t + 2],
                       a.tets[4 * (size_t)t + 3]};
@@ -134,12 +163,13 @@ __device__ __forceinline__ void element_tet(const ElementIn& a, int t) {
   auto sf = (HOBAK_GLOBAL T*)a.stage_f;
 #pragma unroll
   for (int v = 0; v < 4; v++) {
-    auto dst = sf + 3 * (size_t)a.fdest[4 * (size_t)t + v];
+    T f[3];
 #pragma unroll
     for (int c = 0; c < 3; c++) {
       const T s = (P[c][0] * Wd[v][0] + P[c][1] * Wd[v][1]) + P[c][2] * Wd[v][2];
-      dst[c] = -vol * s;
+      f[c] = -vol * s;
     }
+    wave_store<T, 3>(sf + 3 * ((size_t)v * nt + t0), f, nvalid, tile);
   }
   if (!a.with_hessian) return;
 
@@ -156,7 +186,6 @@ __device__ __forceinline__ void element_tet(const ElementIn& a, int t) {
     for (int b = 0; b < 3; b++) w[v][b] = Wd[v][0] * V[0][b] + Wd[v][1] * V[1][b] + Wd[v][2] * V[2][b];
 
   auto sh = (HOBAK_GLOBAL T*)a.stage_h;
-  auto hd = a.hdest + kStagedBlocks * (size_t)t;
 #pragma unroll
   for (int v = 0; v < 4; v++) {
 #pragma unroll
@@ -181,20 +210,13 @@ __device__ __forceinline__ void element_tet(const ElementIn& a, int t) {
       for (int c = 0; c < 3; c++)
 #pragma unroll
         for (int q = 0; q < 3; q++) UM[c][q] = U[c][0] * M[0][q] + U[c][1] * M[1][q] + U[c][2] * M[2][q];
-      T B[3][3];
+      T B[9];  // block (v, v2): rows of corner v
 #pragma unroll
       for (int c = 0; c < 3; c++)
 #pragma unroll
         for (int c2 = 0; c2 < 3; c2++)
-          B[c][c2] = vol * (UM[c][0] * U[c2][0] + UM[c][1] * U[c2][1] + UM[c][2] * U[c2][2]);
-      // block (v, v2), oriented as the upper slot it sums into
-      const int word = hd[staged_pair(v, v2)];
-      const bool tr = (word & kHdestTranspose) != 0;
-      auto d0 = sh + 9 * (size_t)(word & kHdestIndex);
-#pragma unroll
-      for (int c = 0; c < 3; c++)
-#pragma unroll
-        for (int c2 = 0; c2 < 3; c2++) d0[3 * c + c2] = tr ? B[c2][c] : B[c][c2];
+          B[3 * c + c2] = vol * (UM[c][0] * U[c2][0] + UM[c][1] * U[c2][1] + UM[c][2] * U[c2][2]);
+      wave_store<T, 9>(sh + 9 * ((size_t)staged_pair(v, v2) * nt + t0), B, nvalid, tile);
     }
   }
 }
@@ -204,6 +226,7 @@ __device__ __forceinline__ void element_tet(const ElementIn& a, int t) {
 template <typename T>
 __global__ __launch_bounds__(kElementBlock) void element_kernel(Batch b, int src, int with_hessian,
                                                                 const double* u_ext, void* sf_ext) {
+  __shared__ T tiles[kElementBlock / 64][9 * 64];  // one store tile per wave
   for (int ws = blockIdx.x; ws < b.total; ws += gridDim.x) {
     const int vb = xcd_vb(ws, b.total);
     int lb, lg;
@@ -220,17 +243,18 @@ __global__ __launch_bounds__(kElementBlock) void element_kernel(Batch b, int src
     a.vol = gbl(I->vol);
     a.stage_h = gbl(I->stage_h);
     a.stage_f = gbl(sf_ext ? sf_ext : I->stage_f);
-    a.hdest = gbl(I->hdest);
-    a.fdest = gbl(I->fdest);
     a.with_hessian = with_hessian;
     const int material = I->material;
     const int t = lb * kElementBlock + threadIdx.x;
-    if (t < a.nt) {
-      if (material == 0)
-        element_tet<T, 0>(a, t);
-      else
-        element_tet<T, 1>(a, t);
-    }
+    const int t0 = t - (threadIdx.x & 63);  // the wave's first tet
+    if (t0 >= a.nt) continue;               // (uniform over the wave)
+    const int nvalid = min(64, a.nt - t0);
+    const int tc = min(t, a.nt - 1);
+    T* tile = tiles[threadIdx.x >> 6];
+    if (material == 0)
+      element_tet<T, 0>(a, tc, t0, nvalid, tile);
+    else
+      element_tet<T, 1>(a, tc, t0, nvalid, tile);
   }
 }
 
@@ -296,10 +320,9 @@ __global__ __launch_bounds__(kElementBlock) void damping_force_kernel(Batch b, i
     }
     const double vol = gbl(I->vol)[t];
     auto sf = gbl((T*)I->stage_f);
-    auto fd = gbl(I->fdest) + 4 * (size_t)t;
 #pragma unroll
     for (int v = 0; v < 4; v++) {
-      auto dst = sf + 3 * (size_t)fd[v];
+      auto dst = sf + 3 * ((size_t)v * n + t);
 #pragma unroll
       for (int c = 0; c < 3; c++) {
         const double s = (P[c][0] * d[v][0] + P[c][1] * d[v][1]) + P[c][2] * d[v][2];
@@ -410,23 +433,18 @@ __global__ __launch_bounds__(kElementBlock) void damping_hessian_kernel(Batch b,
       for (int i = 0; i < 3; i++) g[v][i] = (F[i][0] * d[v][0] + F[i][1] * d[v][1]) + F[i][2] * d[v][2];
     const double s = I->dmu * gbl(I->vol)[t];
     auto sh = gbl((T*)I->stage_h);
-    auto hd = gbl(I->hdest) + kStagedBlocks * (size_t)t;
     // block (v, w) is s (G (d_v . d_w) + g_w g_v^T), G symmetric, so block (w, v) is its exact
-    // transpose: the upper blocks are staged, each oriented as its slot
+    // transpose: the upper blocks are staged, rows of corner v
 #pragma unroll
     for (int v = 0; v < 4; v++)
 #pragma unroll
       for (int w = v; w < 4; w++) {
         const double dd = (d[v][0] * d[w][0] + d[v][1] * d[w][1]) + d[v][2] * d[w][2];
-        const int word = hd[staged_pair(v, w)];
-        const bool tr = (word & kHdestTranspose) != 0;
-        auto dst = sh + 9 * (size_t)(word & kHdestIndex);
+        auto dst = sh + 9 * ((size_t)staged_pair(v, w) * n + t);
 #pragma unroll
         for (int c = 0; c < 3; c++)
 #pragma unroll
-          for (int c2 = 0; c2 < 3; c2++)
-            dst[3 * c + c2] = tr ? (T)(s * (G[c2][c] * dd + g[w][c2] * g[v][c]))
-                                 : (T)(s * (G[c][c2] * dd + g[w][c] * g[v][c2]));
+          for (int c2 = 0; c2 < 3; c2++) dst[3 * c + c2] = (T)(s * (G[c][c2] * dd + g[w][c] * g[v][c2]));
       }
   }
 }

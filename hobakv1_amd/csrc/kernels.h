@@ -5,15 +5,17 @@
 //   vectors        DOF-ordered AoS: entry 3*v + c (the reference's flat VECTOR order)
 //   tets           int4 per tet
 //   DmInv          9 planes of nt doubles (plane 3*r + c holds DmInv(r, c))       -> coalesced
-//   element stage  written directly in the order the assembly reads it.  A tet's stiffness is
-//                  symmetric, so only its 10 blocks (x, y) with x <= y are staged: the block goes
-//                  to stage_h[9 * (hdest[10 t + pair(x, y)] & kHdestIndex) + 3 a + b], where
-//                  hdest is the position of that contribution in the ascending-tet list of the
-//                  upper slot (min(v_x, v_y), max(v_x, v_y)); with kHdestTranspose set the block
-//                  is staged transposed, so that it is oriented as that slot.  Every upper slot
-//                  sums one contiguous run and also writes the transpose into its mirror (lower)
-//                  slot.  Force 3-vectors go to stage_f[3 * fdest[4 t + x] + c] in each vertex's
-//                  ascending-tet list
+//   element stage  tet-major, so that the element kernel writes whole lines.  A tet's stiffness
+//                  is symmetric, so only its 10 blocks (x, y) with x <= y are staged: entry (a, b)
+//                  of block pair(x, y) of tet t is stage_h[9 * (pair(x, y) * nt + t) + 3 a + b]
+//                  (rows of corner x).  The upper slot (min(v_x, v_y), max(v_x, v_y)) lists its
+//                  contributions in ascending tet order in cidx[cptr[slot] .. cptr[slot + 1]):
+//                  each entry is pair * nt + t, with kCidxTranspose set where corner x is the
+//                  slot's column (the slot sums the block transposed).  Every upper slot also
+//                  writes the transpose of its sum into its mirror (lower) slot.  The force
+//                  3-vector of corner x of tet t is stage_f[3 * (x * nt + t) + c]; vertex v's
+//                  incident corners are listed in ascending tet order in
+//                  fidx[vptr[v] .. vptr[v + 1]) (entries x * nt + t)
 //   block matrix   SELL-H, k-interleaved (H = 8, 16, 32 or 64 rows per slice, L = 64 / H lanes
 //                  per row): slice s owns slots [slice_off[s], slice_off[s+1]) (a multiple of 64),
 //                  and the k-th block of row r sits at slot = slice_off[r / H] + H * k + (r % H).
@@ -62,7 +64,7 @@ constexpr int kMaxShapes = 8;    // kinematic collision cubes per instance
 
 // Virtual-workgroup kinds: each instance contributes nb[kind] workgroups to a launch of a kernel
 // of that kind.
-enum Kind { K_ELEM = 0, K_SLOT, K_ROW, K_DOF, K_SPMV, K_UPD, K_ONE, K_COUNT };
+enum Kind { K_ELEM = 0, K_SLOT, K_ROW, K_DOF, K_SPMV, K_UPD, K_ONE, K_ASM, K_COUNT };
 
 // Scalars of one PCG solve, resident on the device so the loop runs without host round trips.
 // Two copies ping-pong: an iteration of parity p reads set p and its SpMV kernel writes set
@@ -136,7 +138,7 @@ struct InstDev {
   const double *rest, *dminv, *vol, *one_ring, *mass, *pin_target;
   double *u, *v, *u_old, *v_old, *tmp64, *rhs64, *x64, *res64;
   const unsigned char* pinned;
-  const int *slice_off, *row_len, *col, *slot_row, *cptr, *hdest, *vptr, *fdest;
+  const int *slice_off, *row_len, *col, *slot_row, *cptr, *cidx, *vptr, *fidx;
   const int* mirror;  // per upper off-diagonal slot: the slot of the transposed block (else -1)
   void *stage_h, *stage_f, *A, *D, *E, *C, *kneg, *minv, *r, *x, *z, *t, *p0, *p1;
   Ctl* ctl;    // == ctl_of(this) on the device
@@ -256,10 +258,10 @@ enum ElemSrc { ELEM_U = 0, ELEM_TMP = 1, ELEM_EXT = 2 };
 enum AsmTarget { ASM_REST = 0, ASM_PLAIN_A = 1, ASM_SYSTEM = 2 };
 
 // Staged stiffness blocks per tet (the upper triangle of its 4x4 block pattern), their order, and
-// the destination word's flag and index bits.
+// the flag and index bits of a contribution-list entry.
 constexpr int kStagedBlocks = 10;
-constexpr int kHdestTranspose = (int)0x80000000u;
-constexpr int kHdestIndex = 0x7fffffff;
+constexpr int kCidxTranspose = (int)0x80000000u;
+constexpr int kCidxIndex = 0x7fffffff;
 __host__ __device__ constexpr int staged_pair(int x, int y) { return 4 * x - (x * (x - 1)) / 2 + (y - x); }
 
 // u_ext (with ELEM_EXT) and stage_f_ext (non-null: the force staging) replace the instance's own

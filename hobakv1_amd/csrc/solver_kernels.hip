@@ -501,46 +501,80 @@ __device__ __forceinline__ void write_precond(const T (&a)[9], HOBAK_GLOBAL T* m
     for (int k = 0; k < 3; k++) m[3 * row + k] = cof(k, row) * invdet;
 }
 
-// One lane per upper block slot (row <= column): it sums the slot's staged contributions and
-// writes the slot and, off the diagonal, the transposed block into its mirror slot.
+// Entry e of a block's inverse by cofactors (write_precond's arithmetic, one entry per lane):
+// m = cof(k, row) / det with row = e / 3, k = e % 3.
+template <typename T>
+__device__ __forceinline__ T precond_entry(const T (&a)[9], int e, int precond) {
+  if (precond == 0) {  // DiagonalPreconditioner: 1/a_ii, 1 where a_ii == 0 (entries 0..2)
+    const T d = a[4 * e];
+    return (d != T(0)) ? T(1) / d : T(1);
+  }
+  auto cof = [&](int i, int j) {
+    const int i1 = (i + 1) % 3, i2 = (i + 2) % 3, j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+    return a[3 * i1 + j1] * a[3 * i2 + j2] - a[3 * i1 + j2] * a[3 * i2 + j1];
+  };
+  const T c0 = cof(0, 0), c1 = cof(1, 0), c2 = cof(2, 0);
+  const T det = (c0 * a[0] + c1 * a[3]) + c2 * a[6];
+  const T invdet = T(1) / det;
+  const int row = e / 3, k = e - 3 * row;
+  return (row == 0 ? (k == 0 ? c0 : k == 1 ? c1 : c2) : cof(k, row)) * invdet;
+}
+
+// Nine lanes per block slot, 7 slots per wavefront: lane 9 g + e holds entry e of slot 7 w + g.
+// An upper slot (row <= column) sums its staged contributions entry by entry in list order (the
+// nine lanes read one staged block, 72 or 36 contiguous bytes, with one load instruction) and
+// writes the slot and, off the diagonal, the transposed block into its mirror (lower) slot.
 // ASM_REST / ASM_PLAIN_A: out = that sum (the stiffness, -K) into kneg / A.  ASM_SYSTEM:
 // A = S (D - K) S + (I - S) (+ the ground penalty on the diagonal); the diagonal slot also
 // writes its preconditioner.
+constexpr int kAsmSlotsPerWave = 7;
 template <typename T>
 __global__ __launch_bounds__(kBlock) void assemble_kernel(Batch b, int target) {
   for (int ws = blockIdx.x; ws < b.total; ws += gridDim.x) {
     const int vb = xcd_vb(ws, b.total);
     int lb, lg;
     const HOBAK_GLOBAL InstDev* __restrict__ I = seg_find(b, vb, lb, lg);
-    const int slot = lb * kBlock + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int g = lane / 9, e = lane - 9 * g;
+    const int slot = (lb * (kBlock / 64) + (int)(threadIdx.x >> 6)) * kAsmSlotsPerWave + g;
     const int nslots = I->nslots;
-    if (slot >= nslots) continue;
+    if (g >= kAsmSlotsPerWave || slot >= nslots) continue;
+    // the slot's record in one load round (the mirror entry exists for every slot)
     const int r = gbl(I->slot_row)[slot];
-    if (r < 0) continue;
     const int c = gbl(I->col)[slot];
-    if (c < r) continue;  // a lower slot: written by its mirror
-    const int mslot = c == r ? -1 : gbl(I->mirror)[slot];
+    const int kbeg = gbl(I->cptr)[slot], kend = gbl(I->cptr)[slot + 1];
+    const int mirror = gbl(I->mirror)[slot];
+    if (r < 0 || c < r) continue;  // padding, or a lower slot (written by its mirror)
+    // from here on the nine lanes of the group are active together (the tests above are per slot)
+    const int mslot = c == r ? -1 : mirror;
     const size_t ns = (size_t)nslots;
-    T acc[9];
+    const int te = 3 * (e % 3) + e / 3;  // entry (a, b) of a transposed block is entry (b, a)
+    const int gl = lane - e;             // the group's first lane
+    // the slot's contributions in ascending tet order, a flagged block summed transposed; eight
+    // blocks are in flight per pass and the next pass's list entries are fetched with them
+    auto stage = gbl((const T*)I->stage_h);
+    auto ci = gbl(I->cidx);
+    constexpr int kW = 8;
+    int w[kW];
 #pragma unroll
-    for (int e = 0; e < 9; e++) acc[e] = T(0);
-    // the slot's contributions are contiguous and in ascending tet order
-    auto src = gbl((const T*)I->stage_h) + 9 * (size_t)gbl(I->cptr)[slot];
-    auto end = gbl((const T*)I->stage_h) + 9 * (size_t)gbl(I->cptr)[slot + 1];
-    for (; src < end; src += 9) {
+    for (int j = 0; j < kW; j++) w[j] = kbeg + j < kend ? ci[kbeg + j] : 0;
+    T acc = T(0);
+    for (int k = kbeg; k < kend; k += kW) {
+      T v[kW];
 #pragma unroll
-      for (int e = 0; e < 9; e++) acc[e] += src[e];
+      for (int j = 0; j < kW; j++)
+        v[j] = stage[9 * (size_t)(w[j] & kCidxIndex) + ((w[j] & kCidxTranspose) ? te : e)];
+#pragma unroll
+      for (int j = 0; j < kW; j++) w[j] = k + kW + j < kend ? ci[k + kW + j] : 0;
+#pragma unroll
+      for (int j = 0; j < kW; j++)
+        if (k + j < kend) acc += v[j];
     }
-    // the mirror slot (c, r) holds the transpose: entry (a, b) there is entry (b, a) here
-    constexpr int kTr[9] = {0, 3, 6, 1, 4, 7, 2, 5, 8};
     if (target != ASM_SYSTEM) {
       auto out = gbl((T*)(target == ASM_REST ? I->kneg : I->A));
-#pragma unroll
-      for (int e = 0; e < 9; e++) out[e * ns + slot] = acc[e];
-      if (mslot >= 0) {
-#pragma unroll
-        for (int e = 0; e < 9; e++) out[e * ns + mslot] = acc[kTr[e]];
-      }
+      const T accT = __shfl(acc, gl + te, 64);
+      out[e * ns + slot] = acc;
+      if (mslot >= 0) out[e * ns + mslot] = accT;
       continue;
     }
     constexpr bool kDiff = sizeof(T) < sizeof(double);
@@ -554,46 +588,41 @@ __global__ __launch_bounds__(kBlock) void assemble_kernel(Batch b, int target) {
     if (I->ground_on && c == r && !pin) {
       const double y = gbl(I->rest)[3 * (size_t)r + 1] + gbl(I->u)[3 * (size_t)r + 1];
       ground = y < I->ground_y;
-      if (ground) acc[4] += (T)I->ground_k;
+      if (ground && e == 4) acc += (T)I->ground_k;
     }
+    const T accT = __shfl(acc, gl + te, 64);
     auto D = gbl((const T*)I->D);
     auto out = gbl((T*)I->A);
     // velocity form: A = M - dt^2 K (BACKWARD_EULER_VELOCITY.cpp:161), with K = -acc; the
     // unprojected stiffness also goes to E for b's K v
     const bool vel = I->velform != 0;
     const T ks = (T)I->kscale;
-    T val[9];
-#pragma unroll
-    for (int e = 0; e < 9; e++) {
-      const T a = vel ? D[e * ns + slot] + ks * acc[e] : D[e * ns + slot] + acc[e];
-      val[e] = pin ? ((c == r && (e % 4) == 0) ? T(1) : T(0)) : a;
-      out[e * ns + slot] = val[e];
-    }
+    const T a = vel ? D[e * ns + slot] + ks * acc : D[e * ns + slot] + acc;
+    const T val = pin ? ((c == r && (e % 4) == 0) ? T(1) : T(0)) : a;
+    out[e * ns + slot] = val;
     if (mslot >= 0) {
       const bool pin_m = contact ? false : kDiff ? pin_c : (pin_r || pin_c);
-#pragma unroll
-      for (int e = 0; e < 9; e++) {
-        const T a = vel ? D[e * ns + mslot] + ks * acc[kTr[e]] : D[e * ns + mslot] + acc[kTr[e]];
-        out[e * ns + mslot] = pin_m ? T(0) : a;
-      }
+      const T am = vel ? D[e * ns + mslot] + ks * accT : D[e * ns + mslot] + accT;
+      out[e * ns + mslot] = pin_m ? T(0) : am;
     }
     if (vel) {
       auto kst = gbl((T*)I->E);
-#pragma unroll
-      for (int e = 0; e < 9; e++) kst[e * ns + slot] = acc[e];
-      if (mslot >= 0) {
-#pragma unroll
-        for (int e = 0; e < 9; e++) kst[e * ns + mslot] = acc[kTr[e]];
-      }
+      kst[e * ns + slot] = acc;
+      if (mslot >= 0) kst[e * ns + mslot] = accT;
     }
     if (c == r && !contact) {
-      write_precond<T>(val, gbl((T*)I->minv), r, I->precond);
-      if (kDiff) {
+      T blk[9];
 #pragma unroll
-        for (int k = 0; k < 3; k++) {
-          const double mk = gbl(I->mass)[3 * (size_t)r + k] * (I->invDt2 - I->ra * I->invDt);
-          gbl(I->dg)[3 * (size_t)r + k] = pin ? 1.0 : mk + ((k == 1 && ground) ? I->ground_k : 0.0);
-        }
+      for (int j = 0; j < 9; j++) blk[j] = __shfl(val, gl + j, 64);
+      const int precond = I->precond;
+      if (precond == 0) {
+        if (e < 3) gbl((T*)I->minv)[3 * (size_t)r + e] = precond_entry<T>(blk, e, 0);
+      } else {
+        gbl((T*)I->minv)[(size_t)kMinvStride * r + e] = precond_entry<T>(blk, e, precond);
+      }
+      if (kDiff && e < 3) {
+        const double mk = gbl(I->mass)[3 * (size_t)r + e] * (I->invDt2 - I->ra * I->invDt);
+        gbl(I->dg)[3 * (size_t)r + e] = pin ? 1.0 : mk + ((e == 1 && ground) ? I->ground_k : 0.0);
       }
     }
   }
@@ -707,6 +736,34 @@ __device__ void contact_rhs(const HOBAK_GLOBAL InstDev* I, int r, double (&bv)[3
   if (has_diag) write_precond<T>(diag, gbl((T*)I->minv), r, I->precond);
 }
 
+// A vertex's force: the staged corner forces of its list fidx[q0 .. q1) (ascending tet), summed
+// in FP64 in list order.  Eight corners are in flight per pass and the next pass's list entries
+// are fetched with them.
+template <typename T>
+__device__ __forceinline__ void gather_forces(const HOBAK_GLOBAL T* sf, const HOBAK_GLOBAL int* fi, int q0,
+                                              int q1, double (&f)[3]) {
+  f[0] = f[1] = f[2] = 0.0;
+  constexpr int kW = 8;
+  int w[kW];
+#pragma unroll
+  for (int j = 0; j < kW; j++) w[j] = q0 + j < q1 ? fi[q0 + j] : 0;
+  for (int q = q0; q < q1; q += kW) {
+    T v[kW][3];
+#pragma unroll
+    for (int j = 0; j < kW; j++)
+#pragma unroll
+      for (int c = 0; c < 3; c++) v[j][c] = sf[3 * (size_t)w[j] + c];
+#pragma unroll
+    for (int j = 0; j < kW; j++) w[j] = q + kW + j < q1 ? fi[q + kW + j] : 0;
+#pragma unroll
+    for (int j = 0; j < kW; j++)
+      if (q + j < q1) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) f[c] += (double)v[j][c];
+      }
+  }
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void rhs_kernel(Batch b, int init_cg) {
   for (int ws = blockIdx.x; ws < b.total; ws += gridDim.x) {
@@ -718,11 +775,8 @@ __global__ __launch_bounds__(kBlock) void rhs_kernel(Batch b, int init_cg) {
     if (r < I->nv) {
       auto sf = gbl((const T*)I->stage_f);
       auto vptr = gbl(I->vptr);
-      double R[3] = {0.0, 0.0, 0.0};
-      for (int q = vptr[r]; q < vptr[r + 1]; q++) {
-#pragma unroll
-        for (int c = 0; c < 3; c++) R[c] += (double)sf[3 * (size_t)q + c];
-      }
+      double R[3];
+      gather_forces(sf, gbl(I->fidx), vptr[r], vptr[r + 1], R);
       auto u = gbl(I->u);
       if (I->ground_on) {
         const double y = gbl(I->rest)[3 * (size_t)r + 1] + u[3 * (size_t)r + 1];
@@ -1637,11 +1691,8 @@ __global__ __launch_bounds__(kBlock) void force_gather_kernel(Batch b, const voi
     if (r >= I->nv) continue;
     auto sf = gbl((const T*)(sf_ext ? sf_ext : I->stage_f));
     auto out = gbl(out_ext ? out_ext : I->rhs64);
-    double s[3] = {0.0, 0.0, 0.0};
-    for (int q = gbl(I->vptr)[r]; q < gbl(I->vptr)[r + 1]; q++) {
-#pragma unroll
-      for (int c = 0; c < 3; c++) s[c] += (double)sf[3 * (size_t)q + c];
-    }
+    double s[3];
+    gather_forces(sf, gbl(I->fidx), gbl(I->vptr)[r], gbl(I->vptr)[r + 1], s);
 #pragma unroll
     for (int c = 0; c < 3; c++) out[3 * (size_t)r + c] = s[c];
   }
@@ -1670,6 +1721,7 @@ int blocks_of(int kind, int nv, int nt, int nslots, int shift, int update_cap, i
       return g < update_cap ? g : update_cap;
     }
     case K_ONE: return 1;
+    case K_ASM: return cdiv(nslots, kAsmSlotsPerWave * (kBlock / 64));
     default: return 0;
   }
 }
