@@ -546,6 +546,11 @@ def plain_c2_form(lib, scenes, args):
     return out, sc, precond, total, u
 
 
+def _progress(msg):
+    """A progress line on stderr (stdout carries only the JSON line)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -559,7 +564,9 @@ def main():
     out = {"metric": METRIC}
     stream = scene_of = None
     if args.serving == "on":
+        _progress(f"serving stream: {args.sessions} sessions")
         sv, stream, scene_of = serving_form(lib, scenes, args)
+        _progress(f"serving stream done: {sv['value'] / 1e6:.1f} M tet-steps/s")
         pl = sv.get("plain_fp64") or {}
         out.update({
             "value": sv["value"],
@@ -593,10 +600,13 @@ def main():
             out["roofline"] = sv.pop("roofline")
         out["serving"] = sv
     if args.plain_c2 == "on":
+        _progress("40^3 plain FP64")
         c2, sc, precond, total, u = plain_c2_form(lib, scenes, args)
         out["plain_fp64_c2"] = c2
         if args.precision_form == "on":
+            _progress("40^3 precision form")
             out["fp32_refine"] = precision_form(lib, sc, precond, args, total, u)
+            _progress("40^3 precision-form errors")
             out["fp32_refine"]["c3_errors_vs_fp64"] = c3_errors(lib, scenes, args.steps)
         if args.serving != "on":  # the 40^3 scene is the headline when the stream is off
             out.update({"value": c2["value"], "unit": "tet-Newton-steps/s", "n_gpus": 1,
@@ -607,8 +617,10 @@ def main():
                         "config": {"workload": c2["workload"], "parallelism": "single GPU"},
                         "roofline": c2["roofline"]})
     if args.batch == "on":
+        _progress("batch")
         out["batch"] = batch_form(lib, scenes, args)
     if args.beyond_mall == "on":
+        _progress("88^3 instance")
         out["beyond_mall"] = beyond_mall_form(lib, scenes, args)
     if args.copy_probe:
         ctx = lib.Context(0)
@@ -617,6 +629,7 @@ def main():
         out["copy_probe"] = {"bytes": nb, "ms": ms, "GBps": 2 * nb / (ms * 1e-3) / 1e9}
         ctx.close()
     if args.cpu_baseline != "none":
+        _progress("CPU baselines")
         if stream is not None and args.cpu_baseline in ("auto", "port"):
             out["cpu_baseline"] = cpu_baseline_serving(stream, scene_of, args.steps, args.cpu_seconds)
         if args.plain_c2 == "on":

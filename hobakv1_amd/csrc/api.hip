@@ -123,6 +123,7 @@ struct Pattern {
   std::vector<int> slice_off, row_len, col, slot_row, bsr_slot, diag_slot;
   std::vector<int> cptr, cidx;        // per upper slot: its contributions (pair * nt + t | flag)
   std::vector<int> mirror;            // per upper off-diagonal slot: its transposed slot
+  std::vector<int> aslot;             // the upper slots in slot order (the assembly's list)
   std::vector<int> vptr, fidx;        // per vertex: its incident corners (x * nt + t)
 };
 
@@ -247,6 +248,9 @@ void build_pattern(int nv, int nt, const int32_t* tets, int H, Pattern& P,
   }
   P.cptr.assign(P.nslots + 1, 0);
   for (int s = 0; s < P.nslots; s++) P.cptr[s + 1] = P.cptr[s] + cnt[s + 1];
+  P.aslot.clear();
+  for (int s = 0; s < P.nslots; s++)
+    if (P.slot_row[s] >= 0 && P.col[s] >= P.slot_row[s]) P.aslot.push_back(s);
   P.cidx.assign(P.cptr[P.nslots], 0);  // 10 nt: the 10 pairs of 4 distinct corners per tet
   std::vector<int> fill(P.cptr.begin(), P.cptr.end() - 1);
   for (int i = 0; i < nv; i++) {
@@ -480,7 +484,7 @@ size_t estimate_bytes(int nv, int nt, int nslots_est, int precision) {
   b += (size_t)nt * (16 + 9 * 8 + 8);           // tets, dminv, vol
   b += (size_t)nslots_est * (4 + 4 + 9 * ts);   // col + slot_row + kneg
   b += (size_t)nslots_est * 9 * ts * 4;         // A, D, E, C
-  b += (size_t)nslots_est * 8 + (size_t)nt * (kStagedBlocks + 4) * 4;  // cptr, mirror, cidx, fidx
+  b += (size_t)nslots_est * 12 + (size_t)nt * (kStagedBlocks + 4) * 4;  // cptr, mirror, aslot, cidx, fidx
   b += (size_t)nt * (kStagedBlocks * 9 + 12) * ts;  // staging
   b += (size_t)nv * (9 + 18) * ts;              // minv + PCG vectors
   b += (size_t)nt * 4 * 4 + (size_t)nv * 16;    // vertex lists, misc
@@ -1460,6 +1464,8 @@ int hobak_inst_create(hobak_ctx* ctx, const double* rest_xyz, int32_t nv, const 
   h.nslots = P.nslots;
   h.shift = P.shift;
   for (int k = 0; k < K_COUNT; k++) h.nb[k] = blocks_of(k, nv, nt, P.nslots, P.shift, ctx->update_cap, I->groups);
+  h.nb[K_ASM] = blocks_of(K_ASM, nv, nt, (int)P.aslot.size(), P.shift, ctx->update_cap, I->groups);
+  h.nasm = (int)P.aslot.size();
   h.material = material->kind;
   h.precond = params->precond;
   h.bdf = I->prm.newton_iters > 1;
@@ -1490,7 +1496,7 @@ int hobak_inst_create(hobak_ctx* ctx, const double* rest_xyz, int32_t nv, const 
   if ((rc = dalloc(I, (char**)&ptr, (n), ts)) != HOBAK_OK) return bail(rc)
   double *rest, *mass, *one_ring, *pin_target, *dminv, *vol;
   int4* t4d;
-  int *slice_off, *row_len, *col, *slot_row, *cptr, *cidx, *vptr, *fidx, *mirror;
+  int *slice_off, *row_len, *col, *slot_row, *cptr, *cidx, *vptr, *fidx, *mirror, *aslot;
   unsigned char* pinned;
   ALLOC(rest, 3 * (size_t)nv);
   ALLOC(h.u, 3 * (size_t)nv);
@@ -1514,6 +1520,7 @@ int hobak_inst_create(hobak_ctx* ctx, const double* rest_xyz, int32_t nv, const 
   ALLOC(cptr, P.cptr.size());
   ALLOC(cidx, P.cidx.size());
   ALLOC(mirror, P.mirror.size());
+  ALLOC(aslot, P.aslot.size());
   ALLOC(vptr, P.vptr.size());
   ALLOC(fidx, P.fidx.size());
   ALLOC(pinned, (size_t)nv);
@@ -1560,6 +1567,7 @@ int hobak_inst_create(hobak_ctx* ctx, const double* rest_xyz, int32_t nv, const 
   h.cptr = cptr;
   h.cidx = cidx;
   h.mirror = mirror;
+  h.aslot = aslot;
   h.vptr = vptr;
   h.fidx = fidx;
   h.pinned = pinned;
@@ -1587,6 +1595,7 @@ int hobak_inst_create(hobak_ctx* ctx, const double* rest_xyz, int32_t nv, const 
       (rc = upload(I, cptr, P.cptr.data(), P.cptr.size())) ||
       (rc = upload(I, cidx, P.cidx.data(), P.cidx.size())) ||
       (rc = upload(I, mirror, P.mirror.data(), P.mirror.size())) ||
+      (rc = upload(I, aslot, P.aslot.data(), P.aslot.size())) ||
       (rc = upload(I, vptr, P.vptr.data(), P.vptr.size())) ||
       (rc = upload(I, fidx, P.fidx.data(), P.fidx.size())) ||
       (rc = upload(I, I->ctl, &ctl0, 1)) ||

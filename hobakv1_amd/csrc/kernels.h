@@ -61,6 +61,7 @@ constexpr int kMinvStride = 9;   // block-Jacobi inverse per row; Jacobi uses 1/
 constexpr int kUpdateCap = 512;  // default per-instance workgroup cap of the PCG update kernel
 constexpr int kMaxSweeps = 32;   // FP64 refinement sweeps per linear solve
 constexpr int kMaxShapes = 8;    // kinematic collision cubes per instance
+constexpr int kAsmSlotsPerWave = 7;  // the assembly's block slots per wavefront (9 lanes each)
 
 // Virtual-workgroup kinds: each instance contributes nb[kind] workgroups to a launch of a kernel
 // of that kind.
@@ -127,6 +128,7 @@ struct InstStatus {
 // Read-only per-instance record.  Pointers typed void* hold the solve's real type T.
 struct InstDev {
   int nv, nt, nslots, shift;
+  int nasm;           // upper slots (the assembly's work list, aslot)
   int nb[K_COUNT];
   int material, precond, bdf, ground_on, refine, max_iters;
   double mu, lam, alpha;           // SNH: lam = lambda + mu, alpha = 1 + mu / lam
@@ -140,6 +142,7 @@ struct InstDev {
   const unsigned char* pinned;
   const int *slice_off, *row_len, *col, *slot_row, *cptr, *cidx, *vptr, *fidx;
   const int* mirror;  // per upper off-diagonal slot: the slot of the transposed block (else -1)
+  const int* aslot;   // the upper slots in slot order
   void *stage_h, *stage_f, *A, *D, *E, *C, *kneg, *minv, *r, *x, *z, *t, *p0, *p1;
   Ctl* ctl;    // == ctl_of(this) on the device
   double* dg;  // reduced-precision form: per-DOF diagonal term of the difference-form operator

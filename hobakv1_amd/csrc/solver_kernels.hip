@@ -520,14 +520,15 @@ __device__ __forceinline__ T precond_entry(const T (&a)[9], int e, int precond) 
   return (row == 0 ? (k == 0 ? c0 : k == 1 ? c1 : c2) : cof(k, row)) * invdet;
 }
 
-// Nine lanes per block slot, 7 slots per wavefront: lane 9 g + e holds entry e of slot 7 w + g.
-// An upper slot (row <= column) sums its staged contributions entry by entry in list order (the
-// nine lanes read one staged block, 72 or 36 contiguous bytes, with one load instruction) and
-// writes the slot and, off the diagonal, the transposed block into its mirror (lower) slot.
+// Nine lanes per upper block slot (row <= column), 7 slots per wavefront: lane 9 g + e holds
+// entry e of the slot aslot[7 w + g] (the upper slots in slot order, so no lane waits on a lower
+// slot, which its mirror writes).  A slot sums its
+// staged contributions entry by entry in list order (the nine lanes read one staged block, 72 or
+// 36 contiguous bytes, with one load instruction) and writes the slot and, off the diagonal, the
+// transposed block into its mirror (lower) slot.
 // ASM_REST / ASM_PLAIN_A: out = that sum (the stiffness, -K) into kneg / A.  ASM_SYSTEM:
 // A = S (D - K) S + (I - S) (+ the ground penalty on the diagonal); the diagonal slot also
 // writes its preconditioner.
-constexpr int kAsmSlotsPerWave = 7;
 template <typename T>
 __global__ __launch_bounds__(kBlock) void assemble_kernel(Batch b, int target) {
   for (int ws = blockIdx.x; ws < b.total; ws += gridDim.x) {
@@ -536,16 +537,16 @@ __global__ __launch_bounds__(kBlock) void assemble_kernel(Batch b, int target) {
     const HOBAK_GLOBAL InstDev* __restrict__ I = seg_find(b, vb, lb, lg);
     const int lane = threadIdx.x & 63;
     const int g = lane / 9, e = lane - 9 * g;
-    const int slot = (lb * (kBlock / 64) + (int)(threadIdx.x >> 6)) * kAsmSlotsPerWave + g;
+    const int k = (lb * (kBlock / 64) + (int)(threadIdx.x >> 6)) * kAsmSlotsPerWave + g;
+    if (g >= kAsmSlotsPerWave || k >= I->nasm) continue;
+    // from here on the nine lanes of the group are active together (the tests above are per slot)
     const int nslots = I->nslots;
-    if (g >= kAsmSlotsPerWave || slot >= nslots) continue;
-    // the slot's record in one load round (the mirror entry exists for every slot)
+    const int slot = gbl(I->aslot)[k];
+    // the slot's record in one load round
     const int r = gbl(I->slot_row)[slot];
     const int c = gbl(I->col)[slot];
     const int kbeg = gbl(I->cptr)[slot], kend = gbl(I->cptr)[slot + 1];
     const int mirror = gbl(I->mirror)[slot];
-    if (r < 0 || c < r) continue;  // padding, or a lower slot (written by its mirror)
-    // from here on the nine lanes of the group are active together (the tests above are per slot)
     const int mslot = c == r ? -1 : mirror;
     const size_t ns = (size_t)nslots;
     const int te = 3 * (e % 3) + e / 3;  // entry (a, b) of a transposed block is entry (b, a)
@@ -1721,7 +1722,7 @@ int blocks_of(int kind, int nv, int nt, int nslots, int shift, int update_cap, i
       return g < update_cap ? g : update_cap;
     }
     case K_ONE: return 1;
-    case K_ASM: return cdiv(nslots, kAsmSlotsPerWave * (kBlock / 64));
+    case K_ASM: return cdiv(nslots, kAsmSlotsPerWave * (kBlock / 64));  // nslots: here the upper slots
     default: return 0;
   }
 }
