@@ -124,6 +124,8 @@ struct Pattern {
   std::vector<int> cptr, cidx;        // per upper slot: its contributions (pair * nt + t | flag)
   std::vector<int> mirror;            // per upper off-diagonal slot: its transposed slot
   std::vector<int> aslot;             // the upper slots in slot order (the assembly's list)
+  std::vector<int> evptr, evert;      // per element workgroup: its tets' distinct vertices
+  std::vector<unsigned short> eloc;   // per tet corner: its position in that list
   std::vector<int> vptr, fidx;        // per vertex: its incident corners (x * nt + t)
 };
 
@@ -251,6 +253,36 @@ void build_pattern(int nv, int nt, const int32_t* tets, int H, Pattern& P,
   P.aslot.clear();
   for (int s = 0; s < P.nslots; s++)
     if (P.slot_row[s] >= 0 && P.col[s] >= P.slot_row[s]) P.aslot.push_back(s);
+  // the element kernel's vertex lists: each workgroup of kElementBlock consecutive tets stages its
+  // distinct vertices (ascending) in LDS; a corner names its vertex by its place in the list
+  {
+    const int nblk = (nt + kElementBlock - 1) / kElementBlock;
+    P.evptr.assign(nblk + 1, 0);
+    P.evert.clear();
+    P.evert.reserve((size_t)nt);
+    P.eloc.assign(4 * (size_t)nt, 0);
+    std::vector<int> seen(nv, -1), place(nv, 0), list;
+    list.reserve(kElementVerts);
+    for (int w = 0; w < nblk; w++) {
+      const int t1 = std::min(nt, (w + 1) * kElementBlock);
+      list.clear();
+      for (int t = w * kElementBlock; t < t1; t++)
+        for (int a = 0; a < 4; a++) {
+          const int v = tets[4 * (size_t)t + a];
+          if (seen[v] != w) {
+            seen[v] = w;
+            list.push_back(v);
+          }
+        }
+      std::sort(list.begin(), list.end());
+      for (size_t i = 0; i < list.size(); i++) place[list[i]] = (int)i;
+      for (int t = w * kElementBlock; t < t1; t++)
+        for (int a = 0; a < 4; a++)
+          P.eloc[4 * (size_t)t + a] = (unsigned short)place[tets[4 * (size_t)t + a]];
+      P.evert.insert(P.evert.end(), list.begin(), list.end());
+      P.evptr[w + 1] = (int)P.evert.size();
+    }
+  }
   P.cidx.assign(P.cptr[P.nslots], 0);  // 10 nt: the 10 pairs of 4 distinct corners per tet
   std::vector<int> fill(P.cptr.begin(), P.cptr.end() - 1);
   for (int i = 0; i < nv; i++) {
@@ -487,7 +519,7 @@ size_t estimate_bytes(int nv, int nt, int nslots_est, int precision) {
   b += (size_t)nslots_est * 12 + (size_t)nt * (kStagedBlocks + 4) * 4;  // cptr, mirror, aslot, cidx, fidx
   b += (size_t)nt * (kStagedBlocks * 9 + 12) * ts;  // staging
   b += (size_t)nv * (9 + 18) * ts;              // minv + PCG vectors
-  b += (size_t)nt * 4 * 4 + (size_t)nv * 16;    // vertex lists, misc
+  b += (size_t)nt * (4 * 4 + 8 + 4) + (size_t)nv * 16;  // vertex lists, element vertex lists, misc
   b += 64 * 1024;                               // records, control block, partial sums
   return b;
 }
@@ -1496,7 +1528,8 @@ int hobak_inst_create(hobak_ctx* ctx, const double* rest_xyz, int32_t nv, const 
   if ((rc = dalloc(I, (char**)&ptr, (n), ts)) != HOBAK_OK) return bail(rc)
   double *rest, *mass, *one_ring, *pin_target, *dminv, *vol;
   int4* t4d;
-  int *slice_off, *row_len, *col, *slot_row, *cptr, *cidx, *vptr, *fidx, *mirror, *aslot;
+  int *slice_off, *row_len, *col, *slot_row, *cptr, *cidx, *vptr, *fidx, *mirror, *aslot, *evptr, *evert;
+  unsigned short* eloc;
   unsigned char* pinned;
   ALLOC(rest, 3 * (size_t)nv);
   ALLOC(h.u, 3 * (size_t)nv);
@@ -1521,6 +1554,9 @@ int hobak_inst_create(hobak_ctx* ctx, const double* rest_xyz, int32_t nv, const 
   ALLOC(cidx, P.cidx.size());
   ALLOC(mirror, P.mirror.size());
   ALLOC(aslot, P.aslot.size());
+  ALLOC(evptr, P.evptr.size());
+  ALLOC(evert, P.evert.size());
+  ALLOC(eloc, P.eloc.size());
   ALLOC(vptr, P.vptr.size());
   ALLOC(fidx, P.fidx.size());
   ALLOC(pinned, (size_t)nv);
@@ -1568,6 +1604,9 @@ int hobak_inst_create(hobak_ctx* ctx, const double* rest_xyz, int32_t nv, const 
   h.cidx = cidx;
   h.mirror = mirror;
   h.aslot = aslot;
+  h.evptr = evptr;
+  h.evert = evert;
+  h.eloc = eloc;
   h.vptr = vptr;
   h.fidx = fidx;
   h.pinned = pinned;
@@ -1596,6 +1635,9 @@ int hobak_inst_create(hobak_ctx* ctx, const double* rest_xyz, int32_t nv, const 
       (rc = upload(I, cidx, P.cidx.data(), P.cidx.size())) ||
       (rc = upload(I, mirror, P.mirror.data(), P.mirror.size())) ||
       (rc = upload(I, aslot, P.aslot.data(), P.aslot.size())) ||
+      (rc = upload(I, evptr, P.evptr.data(), P.evptr.size())) ||
+      (rc = upload(I, evert, P.evert.data(), P.evert.size())) ||
+      (rc = upload(I, eloc, P.eloc.data(), P.eloc.size())) ||
       (rc = upload(I, vptr, P.vptr.data(), P.vptr.size())) ||
       (rc = upload(I, fidx, P.fidx.data(), P.fidx.size())) ||
       (rc = upload(I, I->ctl, &ctl0, 1)) ||

@@ -23,7 +23,11 @@
 // are staged tet-major (block pair(x, y) of tet t at 9 * (pair * nt + t), the force of corner x at
 // 3 * (x * nt + t)); a wave's 64 consecutive tets pass each block through an LDS tile, so every
 // store instruction writes 64 consecutive values.  The assembly and the force gather read them
-// through per-slot and per-vertex lists in ascending tet order.
+// through per-slot and per-vertex lists in ascending tet order.  The corner positions are staged
+// in LDS: a workgroup's 128 consecutive tets touch a few dozen to a few hundred distinct vertices
+// (a list built with the mesh, ascending), whose x = X + u the workgroup loads once, coalesced,
+// into three coordinate planes; each tet then reads its corners there through 16-bit local
+// indices (eloc) instead of gathering 4 x 2 x 24 bytes from HBM.
 #include "device_math.h"
 #include "kernels.h"
 
@@ -33,9 +37,8 @@ namespace hobak {
 struct ElementIn {
   int nt;
   double mu, lam, alpha;
-  const HOBAK_GLOBAL int* tets;  // 4 per tet
-  const HOBAK_GLOBAL double* rest;
-  const HOBAK_GLOBAL double* u;
+  const double (*xs)[kElementVerts];           // LDS: the workgroup's vertex positions, by plane
+  const HOBAK_GLOBAL unsigned short* eloc;     // 4 per tet: its corners' indices into xs
   const HOBAK_GLOBAL double* dminv;
   const HOBAK_GLOBAL double* vol;
   HOBAK_GLOBAL void* stage_h;
@@ -96,15 +99,15 @@ __device__ __forceinline__ void green_pk1(double mu, const double (&F)[3][3], co
 template <typename T, int MAT>
 __device__ __forceinline__ void element_tet(const ElementIn& a, int t, int t0, int nvalid, T* tile) {
   const size_t nt = (size_t)a.nt;
-  const int vid[4] = {a.tets[4 * (size_t)t], a.tets[4 * (size_t)t + 1], a.tets[4 * (size_t)t + 2],
-                      a.tets[4 * (size_t)t + 3]};
-
-  // corner positions x = X + u (TET_MESH::setDisplacement) and edge matrix Ds
+  // corner positions x = X + u (TET_MESH::setDisplacement), staged in LDS, and edge matrix Ds
+  const unsigned long long lc = ((const HOBAK_GLOBAL unsigned long long*)a.eloc)[t];  // 4 x 16 bits
+  const int loc[4] = {(int)(lc & 0xffff), (int)((lc >> 16) & 0xffff), (int)((lc >> 32) & 0xffff),
+                      (int)(lc >> 48)};
   double x[4][3];
 #pragma unroll
   for (int k = 0; k < 4; k++)
 #pragma unroll
-    for (int c = 0; c < 3; c++) x[k][c] = a.rest[3 * (size_t)vid[k] + c] + a.u[3 * (size_t)vid[k] + c];
+    for (int c = 0; c < 3; c++) x[k][c] = a.xs[c][loc[k]];
   double Di[3][3];
 #pragma unroll
   for (int r = 0; r < 3; r++)
@@ -227,6 +230,7 @@ template <typename T>
 __global__ __launch_bounds__(kElementBlock) void element_kernel(Batch b, int src, int with_hessian,
                                                                 const double* u_ext, void* sf_ext) {
   __shared__ T tiles[kElementBlock / 64][9 * 64];  // one store tile per wave
+  __shared__ double xs[3][kElementVerts];          // the workgroup's vertices x = X + u
   for (int ws = blockIdx.x; ws < b.total; ws += gridDim.x) {
     const int vb = xcd_vb(ws, b.total);
     int lb, lg;
@@ -236,25 +240,39 @@ __global__ __launch_bounds__(kElementBlock) void element_kernel(Batch b, int src
     a.mu = I->mu;
     a.lam = I->lam;
     a.alpha = I->alpha;
-    a.tets = gbl((const int*)I->tets);
-    a.rest = gbl(I->rest);
-    a.u = gbl(src == ELEM_EXT ? u_ext : src == ELEM_TMP ? I->tmp64 : I->u);
+    a.xs = xs;
+    a.eloc = gbl(I->eloc);
     a.dminv = gbl(I->dminv);
     a.vol = gbl(I->vol);
     a.stage_h = gbl(I->stage_h);
     a.stage_f = gbl(sf_ext ? sf_ext : I->stage_f);
     a.with_hessian = with_hessian;
     const int material = I->material;
+    // stage the workgroup's vertex positions (the same sum X + u each corner gather made)
+    {
+      const auto rest = gbl(I->rest);
+      const auto u = gbl(src == ELEM_EXT ? u_ext : src == ELEM_TMP ? I->tmp64 : I->u);
+      const auto evert = gbl(I->evert);
+      const int v0 = gbl(I->evptr)[lb], nvx = gbl(I->evptr)[lb + 1] - v0;
+      for (int i = threadIdx.x; i < nvx; i += kElementBlock) {
+        const size_t v = 3 * (size_t)evert[v0 + i];
+#pragma unroll
+        for (int c = 0; c < 3; c++) xs[c][i] = rest[v + c] + u[v + c];
+      }
+    }
+    __syncthreads();
     const int t = lb * kElementBlock + threadIdx.x;
     const int t0 = t - (threadIdx.x & 63);  // the wave's first tet
-    if (t0 >= a.nt) continue;               // (uniform over the wave)
-    const int nvalid = min(64, a.nt - t0);
-    const int tc = min(t, a.nt - 1);
-    T* tile = tiles[threadIdx.x >> 6];
-    if (material == 0)
-      element_tet<T, 0>(a, tc, t0, nvalid, tile);
-    else
-      element_tet<T, 1>(a, tc, t0, nvalid, tile);
+    if (t0 < a.nt) {                        // (uniform over the wave)
+      const int nvalid = min(64, a.nt - t0);
+      const int tc = min(t, a.nt - 1);
+      T* tile = tiles[threadIdx.x >> 6];
+      if (material == 0)
+        element_tet<T, 0>(a, tc, t0, nvalid, tile);
+      else
+        element_tet<T, 1>(a, tc, t0, nvalid, tile);
+    }
+    __syncthreads();  // xs is staged again for the next workgroup of this launch slot
   }
 }
 

@@ -57,6 +57,7 @@ namespace hobak {
 constexpr int kDefaultSliceRows = 32;
 constexpr int kBlock = 256;
 constexpr int kElementBlock = 128;
+constexpr int kElementVerts = 4 * kElementBlock;  // distinct vertices of a workgroup's tets, at most
 constexpr int kMinvStride = 9;   // block-Jacobi inverse per row; Jacobi uses 1/a_ii per DOF
 constexpr int kUpdateCap = 512;  // default per-instance workgroup cap of the PCG update kernel
 constexpr int kMaxSweeps = 32;   // FP64 refinement sweeps per linear solve
@@ -143,6 +144,10 @@ struct InstDev {
   const int *slice_off, *row_len, *col, *slot_row, *cptr, *cidx, *vptr, *fidx;
   const int* mirror;  // per upper off-diagonal slot: the slot of the transposed block (else -1)
   const int* aslot;   // the upper slots in slot order
+  // element kernel: per workgroup of kElementBlock tets, its distinct vertices (ascending) in
+  // evert[evptr[w] .. evptr[w + 1]); per tet, its 4 corners' positions in that list
+  const int *evptr, *evert;
+  const unsigned short* eloc;
   void *stage_h, *stage_f, *A, *D, *E, *C, *kneg, *minv, *r, *x, *z, *t, *p0, *p1;
   Ctl* ctl;    // == ctl_of(this) on the device
   double* dg;  // reduced-precision form: per-DOF diagonal term of the difference-form operator
