@@ -69,6 +69,7 @@ struct hobak_ctx {
   int compact_pct = 100;               // PCG groups launch over the solving instances only below this
   int update_groups = 2;               // 21-row groups per PCG update wave
   int p_form = 1;                      // FP64 p-form iterations: 0 never, 1 with streamed matrices, 2 always
+  int coarse = 1;                      // reduced-precision form: rigid-translation coarse correction
   hipStream_t stream = nullptr;
   size_t budget = 0;
   size_t in_use = 0;
@@ -521,7 +522,17 @@ size_t estimate_bytes(int nv, int nt, int nslots_est, int precision) {
   b += (size_t)nv * (9 + 18) * ts;              // minv + PCG vectors
   b += (size_t)nt * (4 * 4 + 8 + 4) + (size_t)nv * 16;  // vertex lists, element vertex lists, misc
   b += 64 * 1024;                               // records, control block, partial sums
+  if (precision != HOBAK_PRECISION_FP64)        // coarse-correction partial sums
+    b += ((size_t)6 * (nv / kBlock + 1) + (size_t)3 * kUpdateCap) * 8 + 2 * 256;
   return b;
+}
+
+// The reduced-precision form's coarse correction over the rigid translations holds when the
+// operator annihilates a translation off its diagonal and nothing pins one: no pinned vertex, the
+// position form (its difference-form diagonal is dg) and no energy damping.
+int coarse_on(const hobak_ctx* ctx, const hobak_inst* I) {
+  const InstDev& h = I->h;
+  return ctx->coarse && h.refine && !h.velform && !h.damping && I->pinned_ids.empty() ? 1 : 0;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -542,6 +553,7 @@ struct Plan {
   InstStatus* status_host = nullptr;
   int contact = 0;              // some member has collision objects
   int damping = 0;              // some member uses energy damping
+  int coarse = 0;               // some member has the coarse correction on
   double spmv_bytes = 0.0;      // summed over the members
   hipGraphExec_t chunk_exec[2] = {nullptr, nullptr};  // PCG group with cached / streamed matrix loads
   int chunk_k[2] = {0, 0};
@@ -569,10 +581,12 @@ int plan_fill(hobak_ctx* ctx, Plan& P) {
   P.spmv_bytes = 0.0;
   P.contact = 0;
   P.damping = 0;
+  P.coarse = 0;
   for (int i = 0; i < n; i++) {
     hp[i] = P.members[i]->dev;
     P.contact |= P.members[i]->h.contact;
     P.damping |= P.members[i]->h.damping;
+    P.coarse |= P.members[i]->h.coarse;
     P.spmv_bytes += spmv_bytes_of(P.members[i]);
   }
   off += al(sizeof(void*) * n);
@@ -791,9 +805,9 @@ int launch_chunk(hobak_ctx* ctx, const Plan& P, int k, int stream, hobak_sched* 
       HIPCHK(ctx, launch_cg_p(P.b[K_UPD], j & 1, cap, s));
       HIPCHK(ctx, launch_cg_spmv_p(P.b[K_SPMV], P.shift, ctx->spmv_pad, stream, j & 1, cap, s, a, b));
     } else {
-      HIPCHK(ctx, launch_cg_spmv<T>(P.b[K_SPMV], P.shift, ctx->spmv_pad, stream, j & 1, cap, s, a, b));
+      HIPCHK(ctx, launch_cg_spmv<T>(P.b[K_SPMV], P.shift, ctx->spmv_pad, stream, P.coarse, j & 1, cap, s, a, b));
     }
-    HIPCHK(ctx, launch_cg_update<T>(P.b[K_UPD], P.groups, j & 1, cap, s));
+    HIPCHK(ctx, launch_cg_update<T>(P.b[K_UPD], P.groups, P.coarse, j & 1, cap, s));
   }
   if (P.refine) {
     HIPCHK(ctx, launch_refine_residual<T>(P.b[K_SPMV], P.b[K_DOF], P.shift, cap, s));
@@ -1581,6 +1595,10 @@ int hobak_inst_create(hobak_ctx* ctx, const double* rest_xyz, int32_t nv, const 
   ALLOC(h.part_res, (size_t)h.nb[K_SPMV]);
   ALLOC(h.part_rr, (size_t)h.nb[K_UPD]);
   ALLOC(h.part_rz, (size_t)h.nb[K_UPD]);
+  if (h.refine) {
+    ALLOC(h.part_row_c, 6 * (size_t)h.nb[K_ROW]);
+    ALLOC(h.part_upd_g, 3 * (size_t)h.nb[K_UPD]);
+  }
   {
     char* rec = nullptr;  // the record and, kCtlOffset bytes after it, the control block
     if ((rc = dalloc(I, &rec, kCtlOffset + sizeof(Ctl), 1)) != HOBAK_OK) return bail(rc);
@@ -1610,6 +1628,7 @@ int hobak_inst_create(hobak_ctx* ctx, const double* rest_xyz, int32_t nv, const 
   h.vptr = vptr;
   h.fidx = fidx;
   h.pinned = pinned;
+  h.coarse = coarse_on(ctx, I);
   I->ctl = h.ctl;
   // no solve active: both scalar sets done, so every PCG kernel passes this instance over
   Ctl ctl0;
@@ -1676,6 +1695,12 @@ int hobak_inst_set_pinned(hobak_inst* I, const int32_t* vids, int32_t n, const d
     for (int c = 0; c < 3; c++) tgt[3 * (size_t)v + c] = targets ? targets[3 * (size_t)k + c] : 0.0;
   }
   I->pinned_ids.assign(vids, vids + n);
+  const int coarse = coarse_on(I->ctx, I);
+  if (coarse != I->h.coarse) {  // cached plans launch the SpMV variant of their members' setting
+    I->h.coarse = coarse;
+    TRY(upload(I, I->dev, &I->h, 1));
+    plans_forget(I->ctx, I);
+  }
   TRY(upload(I, (unsigned char*)I->h.pinned, flag.data(), flag.size()));
   TRY(upload(I, (double*)I->h.pin_target, tgt.data(), tgt.size()));
   HIPCHK(I->ctx, hipStreamSynchronize(I->ctx->stream));
@@ -1789,6 +1814,7 @@ int hobak_inst_set_damping(hobak_inst* I, int32_t model, double mu) {
   // solveEnergyDamped has no Rayleigh terms: b = (M/dt) v + R + f, A = M/dt^2 - K
   h.ra = h.damping ? 0.0 : I->prm.rayleigh_alpha;
   h.rb = h.damping ? 0.0 : I->prm.rayleigh_beta;
+  h.coarse = coarse_on(ctx, I);
   TRY(upload(I, I->dev, &h, 1));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   TRY(I->tsize == 8 ? setup_constant_parts<double>(I) : setup_constant_parts<float>(I));
@@ -2057,6 +2083,12 @@ int hobak_ctx_set_tuning(hobak_ctx* ctx, int32_t key, int32_t value) {
         for (auto* S : ctx->scheds) S->mplan = nullptr;
       }
       ctx->p_form = value;
+      return HOBAK_OK;
+    case HOBAK_TUNE_COARSE:
+      if (value != 0 && value != 1) return fail(ctx, HOBAK_E_INVALID, "coarse is 0 or 1");
+      if (!ctx->insts.empty() && value != ctx->coarse)
+        return fail(ctx, HOBAK_E_INVALID, "the coarse correction is fixed once instances exist");
+      ctx->coarse = value;
       return HOBAK_OK;
     case HOBAK_TUNE_UPDATE_WGS:
       if (value < 1 || value > 65536) return fail(ctx, HOBAK_E_INVALID, "update workgroups out of range");

@@ -87,6 +87,8 @@ struct CgScalars {
   int status;         // 0 ok, 1 not converged, 2 non-finite
   int first;          // 1 until the first SpMV of the solve has run
   int pad[3];
+  double e[3];        // coarse correction of this set's z (written for the first set only)
+  double pad2;
 };
 
 // FP64 iterative refinement around the reduced-precision PCG (the precision form).
@@ -110,6 +112,8 @@ struct Ctl {
   RefineState rs;
   int phase;
   int pad[3];
+  double ac_inv[3];  // coarse correction: 1 / (P^T A P), diagonal (see InstDev::coarse)
+  double pad2;
 };
 
 // What the host reads back after a group of PCG iterations, one per instance of a batch.
@@ -156,6 +160,15 @@ struct InstDev {
   double *part_pt;                  // K_SPMV: p.t
   double *part_rr, *part_rz;        // K_UPD: r.r, r.z
   double *part_res;                 // K_SPMV: refinement residual
+  // Reduced-precision form of an instance with no pinned vertex: the inner PCG's preconditioner
+  // adds a coarse correction over the rigid translations to block Jacobi,
+  //   z = M^-1 r + P (P^T A P)^-1 P^T r,   P = [I I ... I]^T  (one 3x3 identity per vertex).
+  // The difference form annihilates a translation off the diagonal, so P^T A P = sum_i diag(dg_i)
+  // and A (z + P e) = A z + dg (.) e.  z holds M^-1 r; the SpMV adds e = (P^T A P)^-1 P^T r
+  // where it needs the whole z (its own rows and r.z).
+  int coarse;
+  double* part_row_c;  // K_ROW: 6 planes of nb[K_ROW]: sum of the seed r (3), sum of dg (3)
+  double* part_upd_g;  // K_UPD: 3 planes of nb[K_UPD]: sum of r
   // Kinematic collision cubes and Baraff-Witkin plane constraints (contact != 0; FP64 form).
   // shape[o] = {center x, y, z, scale, inverse scale}: world = scale * local + center.
   int nshapes, contact;
@@ -304,11 +317,13 @@ hipError_t launch_cg_init(const Batch& b, int refine, int gate, hipStream_t s);
 // stream: 1 reads the matrix with non-temporal loads (for matrices larger than the Infinity
 // Cache, whose lines would only displace the gathered vectors), 0 with ordinary loads (a matrix
 // that fits stays cached from one iteration to the next).
+// coarse: some instance of the batch has the reduced-precision form's coarse correction on
+// (InstDev::coarse); the kernel then adds it for those instances.
 template <typename T>
-hipError_t launch_cg_spmv(const Batch& b, int shift, int pad, int stream, int parity, int cap, hipStream_t s,
-                          hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
+hipError_t launch_cg_spmv(const Batch& b, int shift, int pad, int stream, int coarse, int parity, int cap,
+                          hipStream_t s, hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
 template <typename T>
-hipError_t launch_cg_update(const Batch& b, int groups, int parity, int cap, hipStream_t s);
+hipError_t launch_cg_update(const Batch& b, int groups, int coarse, int parity, int cap, hipStream_t s);
 // The FP64 p-form of an iteration (solver_kernels.hip): p finishes the previous iteration and
 // forms p_new = z + beta p_old over the update kernel's workgroups (Batch of kind K_UPD);
 // spmv_p forms t = A p_new gathering p_new only.  Bitwise the same results as the two-kernel form.

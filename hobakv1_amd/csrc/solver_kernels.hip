@@ -84,14 +84,15 @@ __device__ __forceinline__ void gather(const HOBAK_GLOBAL double* pa, const HOBA
 // kPre * kBlock) and reduces over the workgroup.  Summation order is fixed, as in gather().  TWO
 // selects one array (pa) or two (pa, pb); n >= 1, so the loads need no branch.
 constexpr int kPre = 4;
+template <int KP = kPre>
 struct PreParts {
-  double a[kPre], b[kPre];
+  double a[KP], b[KP];
 };
-template <bool TWO>
+template <bool TWO, int KP>
 __device__ __forceinline__ void pre_load(const HOBAK_GLOBAL double* pa, const HOBAK_GLOBAL double* pb, int n,
-                                         PreParts& q) {
+                                         PreParts<KP>& q) {
 #pragma unroll
-  for (int k = 0; k < kPre; k++) {
+  for (int k = 0; k < KP; k++) {
     const int i = threadIdx.x + k * kBlock;
     const int ic = i < n ? i : 0;
     const double a = pa[ic];
@@ -100,16 +101,16 @@ __device__ __forceinline__ void pre_load(const HOBAK_GLOBAL double* pa, const HO
     q.b[k] = i < n ? bv : 0.0;
   }
 }
-template <bool TWO>
+template <bool TWO, int KP>
 __device__ __forceinline__ void pre_sum(const HOBAK_GLOBAL double* pa, const HOBAK_GLOBAL double* pb, int n,
-                                        const PreParts& q, double& ta, double& tb) {
+                                        const PreParts<KP>& q, double& ta, double& tb) {
   double v[2] = {0.0, 0.0};
 #pragma unroll
-  for (int k = 0; k < kPre; k++) {
+  for (int k = 0; k < KP; k++) {
     v[0] += q.a[k];
     v[1] += q.b[k];
   }
-  for (int i = threadIdx.x + kPre * kBlock; i < n; i += kBlock) {
+  for (int i = threadIdx.x + KP * kBlock; i < n; i += kBlock) {
     v[0] += pa[i];
     if (TWO) v[1] += pb[i];
   }
@@ -133,6 +134,17 @@ __device__ __forceinline__ T ld_at(const HOBAK_GLOBAL T* base, size_t uniform_of
 template <typename T>
 __device__ __forceinline__ T ld_stream(const HOBAK_GLOBAL T* base, size_t uniform_off, unsigned off) {
   return __builtin_nontemporal_load((const HOBAK_GLOBAL T*)((const HOBAK_GLOBAL char*)base + uniform_off + off));
+}
+
+// A value that is the same in every lane, moved to a scalar register.
+__device__ __forceinline__ float uniform_f(float v) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+__device__ __forceinline__ double uniform_f(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffffll));
+  const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
 template <typename T>
@@ -881,7 +893,27 @@ __global__ __launch_bounds__(kBlock) void cg_init_kernel(Batch b, int refine, in
     if (gate && !solve_ended(c)) continue;
     double bb, rz;
     gather(gbl(I->part_row_a), gbl(I->part_row_b), I->nb[K_ROW], bb, rz);
+    // coarse correction of the seed (refinement sweeps of an instance with I->coarse):
+    // e = (sum dg)^-1 (sum r) per component, and r.z gains (P^T r).e
+    double ce[3] = {0.0, 0.0, 0.0}, ai[3] = {0.0, 0.0, 0.0};
+    const bool coarse = refine && I->coarse != 0;
+    if (coarse) {
+      const int nb = I->nb[K_ROW];
+      auto pc = gbl(I->part_row_c);
+      double g[3], d[3], unused;
+      gather(pc, pc + nb, nb, g[0], g[1]);
+      gather(pc + 2 * (size_t)nb, pc + 3 * (size_t)nb, nb, g[2], d[0]);
+      gather(pc + 4 * (size_t)nb, pc + 5 * (size_t)nb, nb, d[1], d[2]);
+      (void)unused;
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        ai[k] = d[k] > 0.0 ? 1.0 / d[k] : 0.0;
+        ce[k] = ai[k] * g[k];
+      }
+      rz += g[0] * ce[0] + g[1] * ce[1] + g[2] * ce[2];
+    }
     if (threadIdx.x != 0) continue;
+    for (int k = 0; k < 3; k++) c->ac_inv[k] = ai[k];
     const double tol = refine ? c->rs.tau : I->tol;
     const int rdone = refine ? c->rs.done : 0;
     const double thr0 = tol * tol * bb;
@@ -899,6 +931,9 @@ __global__ __launch_bounds__(kBlock) void cg_init_kernel(Batch b, int refine, in
     s->resNorm2 = bb;
     s->threshold = thr;
     s->absNew = rz;
+    s->e[0] = ce[0];
+    s->e[1] = ce[1];
+    s->e[2] = ce[2];
     s->alpha = 0.0;
     s->beta = 0.0;
     s->iters = 0;
@@ -929,7 +964,7 @@ __global__ __launch_bounds__(kBlock) void cg_init_kernel(Batch b, int refine, in
 // ---------------------------------------------------------------------------------------------
 // One wavefront per H-row slice, L = 64 / H lanes per row: lane = H * kk + rr handles the row's
 // blocks k = kk, kk + L, ...; a row's L partial sums are combined with cross-lane adds.
-template <typename T, int H, bool PAD, bool NT>
+template <typename T, int H, bool PAD, bool NT, bool CO>
 __global__ __launch_bounds__(kBlock) void cg_spmv_kernel(Batch b, int parity) {
   for (int ws = blockIdx.x; ws < b.total; ws += gridDim.x) {
     const int vb = xcd_vb(ws, b.total);
@@ -962,6 +997,19 @@ __global__ __launch_bounds__(kBlock) void cg_spmv_kernel(Batch b, int parity) {
     const int iters = S->iters;
     const double threshold = S->threshold;
     const int maxIters = S->maxIters;
+    // coarse correction (reduced-precision form only): e of the first set, or 1 / P^T A P and the
+    // update kernel's sums of r, from which e = (P^T A P)^-1 P^T r
+    const bool coarse = CO && I->coarse != 0;
+    const auto pg = gbl(I->part_upd_g);
+    double ce0 = 0.0, ce1 = 0.0, ce2 = 0.0, ai0 = 0.0, ai1 = 0.0, ai2 = 0.0;
+    if (coarse) {
+      ce0 = S->e[0];
+      ce1 = S->e[1];
+      ce2 = S->e[2];
+      ai0 = ctl->ac_inv[0];
+      ai1 = ctl->ac_inv[1];
+      ai2 = ctl->ac_inv[2];
+    }
     const bool lead = lb == 0 && threadIdx.x == 0;
     if (lead) {
       N->rhsNorm2 = S->rhsNorm2;
@@ -991,7 +1039,7 @@ __global__ __launch_bounds__(kBlock) void cg_spmv_kernel(Batch b, int parity) {
     int sbeg = slice_off[sc];
     int send = slice_off[sc + 1];
     int len = PAD ? 0 : row_len[rc];
-    PreParts q;
+    PreParts<> q;
     pre_load<true>(prr, prz, nupd, q);
     // The reduced-precision form (T = float) uses the difference-form operator and forms
     // t = A p_new as A z + beta t_old (kRec): it gathers z only, where the FP64 form gathers z and
@@ -1035,6 +1083,15 @@ __global__ __launch_bounds__(kBlock) void cg_spmv_kernel(Batch b, int parity) {
     if (!first) {
       double rrv, rz;
       pre_sum<true>(prr, prz, nupd, q, rrv, rz);
+      if (coarse) {
+        double g0, g1, g2, unused;
+        gather(pg, pg + nupd, nupd, g0, g1);
+        gather(pg + 2 * (size_t)nupd, nullptr, nupd, g2, unused);
+        ce0 = ai0 * g0;
+        ce1 = ai1 * g1;
+        ce2 = ai2 * g2;
+        rz += g0 * ce0 + g1 * ce1 + g2 * ce2;  // r.(M^-1 r) + (P^T r).e
+      }
       if (nonfinite(rrv) || !(rz == rz)) {
         if (lead) {
           N->resNorm2 = rrv;
@@ -1063,10 +1120,22 @@ __global__ __launch_bounds__(kBlock) void cg_spmv_kernel(Batch b, int parity) {
       }
       if (iters + 1 >= maxIters) continue;
       beta = (T)bt;
+      if (coarse && live) {  // the whole z of the own row: M^-1 r + e
+        po0 += (T)ce0;
+        po1 += (T)ce1;
+        po2 += (T)ce2;
+      }
       po0 += beta * qo0;
       po1 += beta * qo1;
       po2 += beta * qo2;
+    } else if (coarse && live) {
+      po0 += (T)ce0;
+      po1 += (T)ce1;
+      po2 += (T)ce2;
     }
+    // e as uniform values for the diagonal term below (the columns' differences need only
+    // M^-1 r: e cancels in them)
+    const T cf0 = uniform_f((T)ce0), cf1 = uniform_f((T)ce1), cf2 = uniform_f((T)ce2);
     // FP64 form: p_new = z + beta p_old at every column (p_old is not read for the first
     // iteration).  FP32 form: A z in the difference form (columns relative to the row's own z, dg
     // adds the diagonal term), then + beta t_old.
@@ -1127,9 +1196,9 @@ __global__ __launch_bounds__(kBlock) void cg_spmv_kernel(Batch b, int parity) {
     if (live && lane < H) {
       const size_t i = 3 * (size_t)r;
       if (kDiff) {
-        s0 += (T)dg[i] * d_own(kRec, zo0, po0);
-        s1 += (T)dg[i + 1] * d_own(kRec, zo1, po1);
-        s2 += (T)dg[i + 2] * d_own(kRec, zo2, po2);
+        s0 += (T)dg[i] * d_own(kRec, coarse ? zo0 + cf0 : zo0, po0);
+        s1 += (T)dg[i + 1] * d_own(kRec, coarse ? zo1 + cf1 : zo1, po1);
+        s2 += (T)dg[i + 2] * d_own(kRec, coarse ? zo2 + cf2 : zo2, po2);
       }
       if (kRec && !first) {
         s0 += beta * to0;
@@ -1212,7 +1281,7 @@ __global__ __launch_bounds__(kBlock) void cg_p_kernel(Batch b, int parity) {
     if (done) continue;
     double beta = 0.0;
     if (!first) {
-      PreParts q;
+      PreParts<> q;
       pre_load<true>(prr, prz, nupd, q);
       double rrv, rz;
       pre_sum<true>(prr, prz, nupd, q, rrv, rz);
@@ -1361,7 +1430,7 @@ __global__ __launch_bounds__(kBlock) void cg_spmv_p_kernel(Batch b, int parity) 
 // every workgroup that re-reduces them) small.  The first wave's loads are issued before the
 // p.t partials are reduced, so the two overlap.  A lane sums r.r and r.z over its groups in
 // order, so the sums do not depend on the batch.
-template <typename T, int R>
+template <typename T, int R, bool CO>
 __global__ __launch_bounds__(kBlock) void cg_update_kernel(Batch b, int parity) {
   for (int ws = blockIdx.x; ws < b.total; ws += gridDim.x) {
     const int vb = xcd_vb(ws, b.total);
@@ -1408,12 +1477,14 @@ __global__ __launch_bounds__(kBlock) void cg_update_kernel(Batch b, int parity) 
       }
     };
     if (w0 < nwaves) load(w0);
-    PreParts qp;
+    PreParts<> qp;
     pre_load<false>(gbl(I->part_pt), nullptr, I->nb[K_SPMV], qp);
     double pt, unused;
     pre_sum<false>(gbl(I->part_pt), nullptr, I->nb[K_SPMV], qp, pt, unused);
     const T alpha = (T)(absNew / pt);
     double rr2 = 0.0, rz = 0.0;
+    double gc = 0.0;  // coarse correction: this lane's sum of r (component c)
+    const bool coarse = CO && I->coarse != 0;
     for (int wave = w0; wave < nwaves; wave += stride) {
       if (wave != w0) load(wave);
 #pragma unroll
@@ -1435,10 +1506,26 @@ __global__ __launch_bounds__(kBlock) void cg_update_kernel(Batch b, int parity) 
           z[i] = zv;
           rr2 += (double)rv * (double)rv;
           rz += (double)rv * (double)zv;
+          gc += (double)rv;
         }
       }
     }
-    publish(rr2, rz, gbl(I->part_rr), gbl(I->part_rz), lb);
+    if (coarse) {
+      // r.r, r.z and the three components' sums of r (P^T r) in one reduction
+      double v[5] = {rr2, rz, c == 0 ? gc : 0.0, c == 1 ? gc : 0.0, c == 2 ? gc : 0.0};
+      block_sum<5>(v);
+      if (threadIdx.x == 0) {
+        const int nupd = I->nb[K_UPD];
+        gbl(I->part_rr)[lb] = v[0];
+        gbl(I->part_rz)[lb] = v[1];
+        auto pg = gbl(I->part_upd_g);
+        pg[lb] = v[2];
+        pg[nupd + lb] = v[3];
+        pg[2 * (size_t)nupd + lb] = v[4];
+      }
+    } else {
+      publish(rr2, rz, gbl(I->part_rr), gbl(I->part_rz), lb);
+    }
   }
 }
 
@@ -1492,6 +1579,8 @@ __global__ __launch_bounds__(kBlock) void refine_init_kernel(Batch b, int gate) 
     if (gate && !solve_ended(ctl_of(I))) continue;
     const int r = lb * kBlock + threadIdx.x;
     double rr = 0.0, rz = 0.0;
+    double gs[3] = {0.0, 0.0, 0.0}, ds[3] = {0.0, 0.0, 0.0};  // coarse correction: sums of r, dg
+    const bool coarse = sizeof(T) < sizeof(double) && I->coarse != 0;
     if (r < I->nv) {
       T rv[3], zv[3];
 #pragma unroll
@@ -1505,9 +1594,26 @@ __global__ __launch_bounds__(kBlock) void refine_init_kernel(Batch b, int gate) 
         (gbl((T*)I->z))[i] = zv[c];
         rr += (double)rv[c] * (double)rv[c];
         rz += (double)rv[c] * (double)zv[c];
+        if (coarse) {
+          gs[c] = (double)rv[c];
+          ds[c] = gbl(I->dg)[i];
+        }
       }
     }
-    publish(rr, rz, gbl(I->part_row_a), gbl(I->part_row_b), lb);
+    if (coarse) {
+      double v[8] = {rr, rz, gs[0], gs[1], gs[2], ds[0], ds[1], ds[2]};
+      block_sum<8>(v);
+      if (threadIdx.x == 0) {
+        const size_t nb = (size_t)I->nb[K_ROW];
+        gbl(I->part_row_a)[lb] = v[0];
+        gbl(I->part_row_b)[lb] = v[1];
+        auto pc = gbl(I->part_row_c);
+#pragma unroll
+        for (int k = 0; k < 6; k++) pc[k * nb + lb] = v[2 + k];
+      }
+    } else {
+      publish(rr, rz, gbl(I->part_row_a), gbl(I->part_row_b), lb);
+    }
   }
 }
 
@@ -1764,23 +1870,32 @@ hipError_t launch_cg_init(const Batch& b, int refine, int gate, hipStream_t s) {
 // With events, the launch carries them in its own dispatch (kernel start and end), so the
 // measured interval is the kernel's execution, not the queue's.  All instances of a batch share
 // one slice height (1 << shift rows).
-template <typename T, int H, bool PAD, bool NT>
+template <typename T, int H, bool PAD, bool NT, bool CO>
 static hipError_t spmv_hp(const Batch& b, int parity, int cap, hipStream_t s, hipEvent_t start, hipEvent_t stop) {
   const dim3 g(grid_of(b, cap)), blk(kBlock);
   if (start)
-    hipExtLaunchKernelGGL((cg_spmv_kernel<T, H, PAD, NT>), g, blk, 0, s, start, stop, 0, b, parity);
+    hipExtLaunchKernelGGL((cg_spmv_kernel<T, H, PAD, NT, CO>), g, blk, 0, s, start, stop, 0, b, parity);
   else
-    hipLaunchKernelGGL((cg_spmv_kernel<T, H, PAD, NT>), g, blk, 0, s, b, parity);
+    hipLaunchKernelGGL((cg_spmv_kernel<T, H, PAD, NT, CO>), g, blk, 0, s, b, parity);
   return hipGetLastError();
 }
-template <typename T, int H>
-static hipError_t spmv_h(const Batch& b, int parity, int cap, int pad, int stream, hipStream_t s,
-                         hipEvent_t start, hipEvent_t stop) {
+template <typename T, int H, bool CO>
+static hipError_t spmv_hc(const Batch& b, int parity, int cap, int pad, int stream, hipStream_t s,
+                          hipEvent_t start, hipEvent_t stop) {
   if (stream)
-    return pad ? spmv_hp<T, H, true, true>(b, parity, cap, s, start, stop)
-               : spmv_hp<T, H, false, true>(b, parity, cap, s, start, stop);
-  return pad ? spmv_hp<T, H, true, false>(b, parity, cap, s, start, stop)
-             : spmv_hp<T, H, false, false>(b, parity, cap, s, start, stop);
+    return pad ? spmv_hp<T, H, true, true, CO>(b, parity, cap, s, start, stop)
+               : spmv_hp<T, H, false, true, CO>(b, parity, cap, s, start, stop);
+  return pad ? spmv_hp<T, H, true, false, CO>(b, parity, cap, s, start, stop)
+             : spmv_hp<T, H, false, false, CO>(b, parity, cap, s, start, stop);
+}
+// The coarse-correction variant exists for the reduced-precision form only.
+template <typename T, int H>
+static hipError_t spmv_h(const Batch& b, int parity, int cap, int pad, int stream, int coarse, hipStream_t s,
+                         hipEvent_t start, hipEvent_t stop) {
+  if constexpr (sizeof(T) < sizeof(double)) {
+    if (coarse) return spmv_hc<T, H, true>(b, parity, cap, pad, stream, s, start, stop);
+  }
+  return spmv_hc<T, H, false>(b, parity, cap, pad, stream, s, start, stop);
 }
 
 template <int H, bool PAD, bool NT>
@@ -1819,26 +1934,33 @@ hipError_t launch_cg_spmv_p(const Batch& b, int shift, int pad, int stream, int 
 }
 
 template <typename T>
-hipError_t launch_cg_spmv(const Batch& b, int shift, int pad, int stream, int parity, int cap, hipStream_t s,
-                          hipEvent_t start, hipEvent_t stop) {
+hipError_t launch_cg_spmv(const Batch& b, int shift, int pad, int stream, int coarse, int parity, int cap,
+                          hipStream_t s, hipEvent_t start, hipEvent_t stop) {
   if (b.total <= 0) return hipSuccess;
   switch (shift) {
-    case 3: return spmv_h<T, 8>(b, parity, cap, pad, stream, s, start, stop);
-    case 4: return spmv_h<T, 16>(b, parity, cap, pad, stream, s, start, stop);
-    case 5: return spmv_h<T, 32>(b, parity, cap, pad, stream, s, start, stop);
-    case 6: return spmv_h<T, 64>(b, parity, cap, pad, stream, s, start, stop);
+    case 3: return spmv_h<T, 8>(b, parity, cap, pad, stream, coarse, s, start, stop);
+    case 4: return spmv_h<T, 16>(b, parity, cap, pad, stream, coarse, s, start, stop);
+    case 5: return spmv_h<T, 32>(b, parity, cap, pad, stream, coarse, s, start, stop);
+    case 6: return spmv_h<T, 64>(b, parity, cap, pad, stream, coarse, s, start, stop);
     default: return hipErrorInvalidValue;
   }
 }
 
-template <typename T>
-hipError_t launch_cg_update(const Batch& b, int groups, int parity, int cap, hipStream_t s) {
+template <typename T, bool CO>
+static hipError_t update_c(const Batch& b, int groups, int parity, int cap, hipStream_t s) {
   switch (groups) {
-    case 1: SEG_LAUNCH((cg_update_kernel<T, 1>), b, cap, s, parity);
-    case 2: SEG_LAUNCH((cg_update_kernel<T, 2>), b, cap, s, parity);
-    case 4: SEG_LAUNCH((cg_update_kernel<T, 4>), b, cap, s, parity);
+    case 1: SEG_LAUNCH((cg_update_kernel<T, 1, CO>), b, cap, s, parity);
+    case 2: SEG_LAUNCH((cg_update_kernel<T, 2, CO>), b, cap, s, parity);
+    case 4: SEG_LAUNCH((cg_update_kernel<T, 4, CO>), b, cap, s, parity);
     default: return hipErrorInvalidValue;
   }
+}
+template <typename T>
+hipError_t launch_cg_update(const Batch& b, int groups, int coarse, int parity, int cap, hipStream_t s) {
+  if constexpr (sizeof(T) < sizeof(double)) {
+    if (coarse) return update_c<T, true>(b, groups, parity, cap, s);
+  }
+  return update_c<T, false>(b, groups, parity, cap, s);
 }
 hipError_t launch_refine_begin(const Batch& b, int cap, hipStream_t s) { SEG_LAUNCH(refine_begin_kernel, b, cap, s); }
 template <typename T>
@@ -1881,8 +2003,9 @@ hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s) 
   template hipError_t launch_assemble<T>(const Batch&, int, int, hipStream_t);               \
   template hipError_t launch_constant_parts<T>(const Batch&, int, hipStream_t);              \
   template hipError_t launch_rhs<T>(const Batch&, int, int, hipStream_t);                    \
-  template hipError_t launch_cg_spmv<T>(const Batch&, int, int, int, int, int, hipStream_t, hipEvent_t, hipEvent_t); \
-  template hipError_t launch_cg_update<T>(const Batch&, int, int, int, hipStream_t);         \
+  template hipError_t launch_cg_spmv<T>(const Batch&, int, int, int, int, int, int, hipStream_t, hipEvent_t, \
+                                         hipEvent_t); \
+  template hipError_t launch_cg_update<T>(const Batch&, int, int, int, int, hipStream_t);    \
   template hipError_t launch_refine_init<T>(const Batch&, int, int, hipStream_t);            \
   template hipError_t launch_refine_residual<T>(const Batch&, const Batch&, int, int, hipStream_t); \
   template hipError_t launch_force_gather<T>(const Batch&, int, hipStream_t, const void*, double*);

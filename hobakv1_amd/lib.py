@@ -144,6 +144,7 @@ TUNE_ROW_SORT = 9
 TUNE_COMPACT_PCT = 10
 TUNE_UPDATE_GROUPS = 11
 TUNE_P_FORM = 12
+TUNE_COARSE = 13
 
 _lib = None
 
@@ -231,7 +232,7 @@ class Context:
                             grid_cap=entry.get("grid_cap"), update_wgs=entry.get("update_wgs"),
                             spmv_pad=entry.get("spmv_pad"), spmv_stream_mb=entry.get("spmv_stream_mb"),
                             row_sort=entry.get("row_sort"), compact_pct=entry.get("compact_pct"),
-                            update_groups=entry.get("update_groups"))
+                            update_groups=entry.get("update_groups"), coarse=entry.get("coarse"))
             self.refine_inner_tol = entry.get("refine_inner_tol")
 
     def error(self) -> str:
@@ -244,14 +245,15 @@ class Context:
 
     def set_tuning(self, slice_rows=None, poll_chunk=None, max_chunks=None, graphs=None,
                    grid_cap=None, update_wgs=None, spmv_pad=None, spmv_stream_mb=None,
-                   row_sort=None, compact_pct=None, update_groups=None, p_form=None):
+                   row_sort=None, compact_pct=None, update_groups=None, p_form=None, coarse=None):
         """Kernel and engine tuning (see hobak_ctx_set_tuning)."""
         for key, val in ((TUNE_SLICE_ROWS, slice_rows), (TUNE_POLL_CHUNK, poll_chunk),
                          (TUNE_MAX_CHUNKS, max_chunks), (TUNE_GRAPHS, graphs),
                          (TUNE_GRID_CAP, grid_cap), (TUNE_UPDATE_WGS, update_wgs),
                          (TUNE_SPMV_PAD, spmv_pad), (TUNE_SPMV_STREAM_MB, spmv_stream_mb),
                          (TUNE_ROW_SORT, row_sort), (TUNE_COMPACT_PCT, compact_pct),
-                         (TUNE_UPDATE_GROUPS, update_groups), (TUNE_P_FORM, p_form)):
+                         (TUNE_UPDATE_GROUPS, update_groups), (TUNE_P_FORM, p_form),
+                         (TUNE_COARSE, coarse)):
             if val is not None:
                 self.check(self.lib.hobak_ctx_set_tuning(self.h, key, int(val)), "set_tuning")
 
@@ -335,6 +337,14 @@ class Instance:
             if getattr(scene, "fdot_v", None) is not None:
                 self.compute_fdots(scene.fdot_v)
         self.set_state(scene.initial_u(), scene.initial_v())
+
+    def set_pinned(self, vids, targets=None):
+        """Pin vertices `vids` (caller numbering) at displacements `targets` (n x 3, default 0)."""
+        vids = np.ascontiguousarray(vids, dtype=np.int32)
+        tg = None if targets is None else np.ascontiguousarray(targets, dtype=np.float64)
+        self.ctx.check(self.lib.hobak_inst_set_pinned(self.h, _i(vids), vids.size,
+                                                      _d(tg) if tg is not None else None),
+                       "set_pinned")
 
     def set_damping(self, model, mu=0.0):
         """DAMPING_RAYLEIGH, or DAMPING_GREEN: GREEN_DAMPING(mu) and the energy-damped step."""

@@ -271,7 +271,16 @@ int hobak_kernel_stats(hobak_inst* inst, double* spmv_avg_ms, int64_t* spmv_coun
  *                          SpMV gathers p_new alone (the two-kernel form gathers z and p_old at
  *                          every block).  0: never; 1 (default): for groups whose matrices stream
  *                          (see HOBAK_TUNE_SPMV_STREAM_MB); 2: always.  Results do not depend on
- *                          it (bitwise) */
+ *                          it (bitwise)
+ *   HOBAK_TUNE_COARSE      reduced-precision form: the inner PCG's preconditioner adds to block
+ *                          (or point) Jacobi a coarse correction over the rigid translations,
+ *                          z = M^-1 r + P (P^T A P)^-1 P^T r with P one 3x3 identity per vertex,
+ *                          for instances with no pinned vertex, in the position form and without
+ *                          energy damping (others are unaffected).  The refined solve still stops
+ *                          at the same FP64 residual; fewer inner iterations get there.  No
+ *                          counterpart in the reference, whose PCG preconditions with the
+ *                          diagonal only.  1 (default) on, 0 off; fixed once the context holds
+ *                          instances.  The FP64 form never uses it */
 #define HOBAK_TUNE_SLICE_ROWS 0
 #define HOBAK_TUNE_POLL_CHUNK 2
 #define HOBAK_TUNE_MAX_CHUNKS 3
@@ -284,6 +293,7 @@ int hobak_kernel_stats(hobak_inst* inst, double* spmv_avg_ms, int64_t* spmv_coun
 #define HOBAK_TUNE_COMPACT_PCT 10
 #define HOBAK_TUNE_UPDATE_GROUPS 11
 #define HOBAK_TUNE_P_FORM 12
+#define HOBAK_TUNE_COARSE 13
 int hobak_ctx_set_tuning(hobak_ctx* ctx, int32_t key, int32_t value);
 
 /* Continuous-batching scheduler (no counterpart in the reference, whose callers step one scene
