@@ -462,8 +462,10 @@ def serving_form(lib, scenes, args):
                            "bytes_per_launch": ks["spmv_bytes"], "avg_launch_ms": ks["spmv_avg_ms"],
                            "launches_sampled": ks["spmv_count"],
                            "stream_avg_bytes_per_launch": eng.get("spmv_bytes_per_launch"),
-                           "note": "sampled launches of the serving engine; bytes are those of the "
-                                   "instances solving in the launch's tick"}
+                           "note": "average time of every n-th SpMV launch of the serving engine; "
+                                   "algorithmic bytes per launch averaged over all its launches, "
+                                   "each instance's matrix and vectors counted in the launches "
+                                   "that streamed them (counted on the device)"}
     ca.close()
     if cb is not None:
         cb.close()

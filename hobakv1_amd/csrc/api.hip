@@ -376,6 +376,7 @@ struct hobak_inst {
   int first_iters = 0;              // iterations of the last step's first (inner) solve
   double spmv_ms_sum = 0.0, spmv_bytes_sum = 0.0;
   long long spmv_count = 0;
+  unsigned work_seen = 0;           // Ctl::spmv_work at the last status poll
 };
 
 struct hobak_sched {
@@ -412,8 +413,8 @@ struct hobak_sched {
   // a long-lived scheduler's own sampling (hobak_sched_set_kernel_sampling): every spmv_every-th
   // SpMV launch is timed and credited with the matrix bytes of the instances solving in its tick
   int spmv_every = 0;
-  double spmv_ms = 0.0, spmv_bytes = 0.0;
-  long long spmv_n = 0;
+  double spmv_ms = 0.0, spmv_bytes = 0.0;  // sampled launch time; algorithmic bytes of the window
+  long long spmv_n = 0, spmv_launches = 0;  // sampled launches; all SpMV launches of the window
 };
 
 namespace {
@@ -1024,10 +1025,7 @@ int tick_impl(hobak_sched* S) {
   }
 
   S->stat[1] += nsolving ? m : 0;
-  if (nsolving) {
-    S->stat[7] += (double)m * k;
-    S->stat[8] += (double)m * k * solving_bytes;
-  }
+  if (nsolving) S->stat[7] += (double)m * k;
   // 5. statuses
   {
     const Range r_poll("hobak poll");
@@ -1035,6 +1033,16 @@ int tick_impl(hobak_sched* S) {
     HIPCHK(ctx, hipMemcpyAsync(M.status_host, M.status_dev, sizeof(InstStatus) * M.n, hipMemcpyDeviceToHost, s));
     HIPCHK(ctx, hipStreamSynchronize(s));
   }
+  // algorithmic SpMV bytes of this tick: each member's matrix and vectors once per launch that
+  // streamed them (counted on the device; launches after a member's solve ended read nothing)
+  double work_bytes = 0.0;
+  for (int i = 0; i < M.n; i++) {
+    hobak_inst* I = M.members[i];
+    const unsigned w = M.status_host[i].spmv_work;
+    work_bytes += (double)(unsigned)(w - I->work_seen) * spmv_bytes_of(I);
+    I->work_seen = w;
+  }
+  S->stat[8] += work_bytes;
   const double now = clock_s(S);
   if (pro) {
     if (pro_split) {
@@ -1047,13 +1055,17 @@ int tick_impl(hobak_sched* S) {
   }
   if (ran) S->ms[3] += elapsed(ctx, 4, 5);
   if (epi) S->ms[4] += elapsed(ctx, 6, 7);
+  if (!S->ephemeral && S->sampling) {
+    // the window's algorithmic bytes and launches; the sampled launches (every `every`-th one)
+    // give the average launch time, so bytes / (launches * average) is the window's rate
+    S->spmv_bytes += work_bytes;
+    if (ran) S->spmv_launches += (long long)m * k;
+  }
   if (!S->ephemeral && S->sampling && !S->samples.empty()) {
-    // every launch of this tick's groups ran over the same set of solving instances
     for (auto& sm : S->samples) {
       float ms = 0.f;
       HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->sample_ev[2 * sm.second], ctx->sample_ev[2 * sm.second + 1]));
       S->spmv_ms += ms;
-      S->spmv_bytes += solving_bytes;
       S->spmv_n += 1;
     }
     S->samples.clear();
@@ -2190,7 +2202,7 @@ int hobak_sched_set_kernel_sampling(hobak_sched* S, int32_t every) {
   S->sample_it = 0;
   S->samples.clear();
   S->spmv_ms = S->spmv_bytes = 0.0;
-  S->spmv_n = 0;
+  S->spmv_n = S->spmv_launches = 0;
   return HOBAK_OK;
 }
 
@@ -2199,7 +2211,7 @@ int hobak_sched_kernel_stats(hobak_sched* S, double* spmv_avg_ms, int64_t* spmv_
   const double n = S->spmv_n ? (double)S->spmv_n : 1.0;
   if (spmv_avg_ms) *spmv_avg_ms = S->spmv_n ? S->spmv_ms / n : 0.0;
   if (spmv_count) *spmv_count = S->spmv_n;
-  if (spmv_bytes) *spmv_bytes = S->spmv_n ? S->spmv_bytes / n : 0.0;
+  if (spmv_bytes) *spmv_bytes = S->spmv_launches ? S->spmv_bytes / (double)S->spmv_launches : 0.0;
   return HOBAK_OK;
 }
 

@@ -113,7 +113,8 @@ struct Ctl {
   int phase;
   int pad[3];
   double ac_inv[3];  // coarse correction: 1 / (P^T A P), diagonal (see InstDev::coarse)
-  double pad2;
+  unsigned spmv_work;  // SpMV launches that streamed this instance's matrix (wraps; a count)
+  int pad4;
 };
 
 // What the host reads back after a group of PCG iterations, one per instance of a batch.
@@ -128,6 +129,8 @@ struct InstStatus {
   int first_iters;  // iterations of the first (refinement: first sweep's) inner solve
   double err;       // sqrt(res^2 / b^2) of the finished solve
   double maxR;      // |S b|^2
+  unsigned spmv_work;  // Ctl::spmv_work
+  int pad;
 };
 
 // Read-only per-instance record.  Pointers typed void* hold the solve's real type T.

@@ -1136,6 +1136,7 @@ __global__ __launch_bounds__(kBlock) void cg_spmv_kernel(Batch b, int parity) {
     // e as uniform values for the diagonal term below (the columns' differences need only
     // M^-1 r: e cancels in them)
     const T cf0 = uniform_f((T)ce0), cf1 = uniform_f((T)ce1), cf2 = uniform_f((T)ce2);
+    if (lead) ctl->spmv_work += 1u;  // this launch streams A (a measurement count)
     // FP64 form: p_new = z + beta p_old at every column (p_old is not read for the first
     // iteration).  FP32 form: A z in the difference form (columns relative to the row's own z, dg
     // adds the diagonal term), then + beta t_old.
@@ -1343,6 +1344,7 @@ __global__ __launch_bounds__(kBlock) void cg_spmv_p_kernel(Batch b, int parity) 
     const auto t = gbl((T*)I->t);
     const auto ppt = gbl(I->part_pt);
     if (ctl_of(I)->sc[parity ^ 1].done) continue;  // the p kernel ended the solve
+    if (lb == 0 && threadIdx.x == 0) ctl_of(I)->spmv_work += 1u;  // this launch streams A
     const int nslices = (nrows + H - 1) / H;
     const bool wave_live = slice < nslices;
     const bool live = r < nrows;
@@ -1775,6 +1777,7 @@ __global__ __launch_bounds__(kBlock) void status_kernel(Batch b, InstStatus* out
     o.ref_status = c->rs.status;
     o.ref_done = c->rs.done;
     o.first_iters = I->refine ? c->rs.first_iters : s->iters;
+    o.spmv_work = c->spmv_work;
     if (I->refine) {
       o.err = c->rs.b2 > 0 ? sqrt(c->rs.res2 / c->rs.b2) : 0.0;
       o.maxR = c->rs.b2;

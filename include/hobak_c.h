@@ -329,7 +329,8 @@ double hobak_sched_clock(hobak_sched* sched);
 /* Counters since creation, 11 doubles: ticks, PCG groups queued, GPU milliseconds of prologues,
  * of PCG groups and of epilogues, host milliseconds inside hobak_sched_tick, solves finished,
  * SpMV launches queued, their algorithmic bytes summed (per launch: the matrices and vectors of
- * the instances solving in its tick, nnzb (9 s + 4) + 4 (nv + 1) + 2 * 3 s nv each), PCG groups
+ * the instances whose solve was still running in it, nnzb (9 s + 4) + 4 (nv + 1) + 2 * 3 s nv
+ * each; the SpMV counts those launches per instance on the device), PCG groups
  * replayed from a captured graph, and PCG groups launched directly (over the still-solving
  * instances only, or holding a sampled launch).
  * The engine also marks its work with roctx ranges ("hobak_step", "hobak_step_batch",
@@ -339,9 +340,11 @@ int hobak_sched_stats(hobak_sched* sched, double* out);
 /* Measurement hook of a scheduler: every `every`-th SpMV launch of its PCG groups is bracketed
  * with HIP events on the context's stream (its group runs uncaptured); 0 turns it off.  Setting
  * it clears the counters.  hobak_sched_kernel_stats returns the average sampled launch time, the
- * number of samples and the average algorithmic bytes per sampled launch, counting the matrices
- * and vectors of the instances that were solving in the launch's tick (the others' workgroups
- * return at once). */
+ * number of samples and the algorithmic bytes per SpMV launch averaged over every launch since
+ * sampling was set (an instance's matrix and vectors count in the launches that streamed them,
+ * as counted on the device; the workgroups of an instance whose solve has ended return at once).
+ * Bytes over the average time is then the rate over all launches of the window, the samples
+ * being spread evenly over them. */
 int hobak_sched_set_kernel_sampling(hobak_sched* sched, int32_t every);
 int hobak_sched_kernel_stats(hobak_sched* sched, double* spmv_avg_ms, int64_t* spmv_count,
                              double* spmv_bytes);

@@ -233,16 +233,65 @@ def test_scheduler_continuous_batching_equals_alone(gpu_ctx, sample_every):
         assert np.array_equal(u, ua) and np.array_equal(v, va), i
     ks = sch.kernel_stats()
     if sample_every:
-        # a sampled launch is credited with the bytes of the instances solving in its tick: at
-        # least the smallest instance's, at most all of them
-        sizes = [3 * sc.nv * 4 * 2 + 4 * (sc.nv + 1) for sc in scs]
+        # bytes per launch over the window: each instance's matrix and vectors count in the
+        # launches that streamed them, so the average is positive and at most all of them
+        total = sum(_spmv_bytes(inst, 4) for inst in insts)
         assert ks["spmv_count"] > 0 and ks["spmv_avg_ms"] > 0
-        assert min(sizes) < ks["spmv_bytes"]
+        assert 0 < ks["spmv_bytes"] <= total
     else:
         assert ks["spmv_count"] == 0
     sch.close()
     for inst in insts:
         inst.close()
+
+
+def _spmv_bytes(inst, s):
+    """Algorithmic bytes of one SpMV over the instance: nnzb (9 s + 4) + 4 (nv + 1) + 2 * 3 s nv."""
+    row_ptr, _ = inst.bsr_pattern()
+    nnzb = int(row_ptr[-1])
+    return nnzb * (9 * s + 4) + 4 * (inst.nv + 1) + 2 * 3 * s * inst.nv
+
+
+@pytest.mark.parametrize("precision", [lib.PRECISION_FP64, lib.PRECISION_FP32_REFINE])
+def test_scheduler_counts_spmv_bytes_of_working_launches(gpu_ctx, precision):
+    """The engine's algorithmic SpMV bytes count an instance once per launch that streamed its
+    matrix: per (inner) solve, its iterations plus the first launch (p = z), not the launches
+    queued after it converged.  Two instances of different sizes finish at different launches of
+    the same queued groups."""
+    scs = [scenes.lattice_scene((6, 6, 6), seed=3, stretch_velocity=True),
+           scenes.lattice_scene((9, 8, 7), seed=4, stretch_velocity=True)]
+    ctx = gpu_ctx if precision == lib.PRECISION_FP64 else lib.Context(0, precision="fp32")
+    insts = [lib.Instance(ctx, sc, precision=precision) for sc in scs]
+    s = 8 if precision == lib.PRECISION_FP64 else 4
+    sch = lib.Scheduler(ctx)
+    expect = 0
+    for _ in range(2):
+        for i, inst in enumerate(insts):
+            sch.submit(inst, i)
+        got = 0
+        while got < len(insts):
+            comps, _ = sch.tick()
+            for c in comps:
+                st = c["stats"]
+                assert st["status"] == 0
+                inst = insts[int(c["tag"])]
+                # FP64: pcg_iters per Newton iteration; refinement: inner iterations summed over
+                # the sweeps, one first launch per sweep
+                n = st["newton_iters"]
+                launches = (sum(st["pcg_iters"][:n]) + n if precision == lib.PRECISION_FP64 else
+                            st["total_pcg_iters"] + sum(st["refine_iters"][:n]))
+                expect += launches * _spmv_bytes(inst, s)
+                got += 1
+    eng = sch.stats()
+    _say(f"  engine SpMV bytes {eng['spmv_bytes']:.0f}, expected {expect}, launches "
+         f"{eng['spmv_launches']:.0f}")
+    assert eng["spmv_bytes"] == expect
+    assert eng["spmv_bytes"] < eng["spmv_launches"] * sum(_spmv_bytes(i, s) for i in insts)
+    sch.close()
+    for inst in insts:
+        inst.close()
+    if ctx is not gpu_ctx:
+        ctx.close()
 
 
 @pytest.mark.parametrize("precision", [lib.PRECISION_FP64, lib.PRECISION_FP32_REFINE])
