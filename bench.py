@@ -477,7 +477,9 @@ def serving_form(lib, scenes, args):
                        f"{args.rate:g}/s, {args.steps} step requests each, tets log-U[1e3,4e5], "
                        f"aspect [0.5,2], SNH/ARAP, penalty ground y=-0.5 k=1e6, dropped from "
                        f"y in [0,1], HBM budget {args.budget_gib:g} GiB, FP32 + FP64 refinement, "
-                       f"block-Jacobi PCG tol 1e-8; instance set-up outside the clock")
+                       f"block-Jacobi PCG tol 1e-8 (the FP32 inner PCG adds a rigid-translation "
+                       f"coarse correction; the plain FP64 path does not); instance set-up outside "
+                       f"the clock")
     return out, stream, scene_of
 
 
