@@ -124,3 +124,30 @@ def test_pinning_turns_the_correction_off(ctx_on, ctx_off):
     assert a < b
     inst_on.close()
     inst_off.close()
+
+
+def test_coarse_bdf_newton_matches_oracle(ctx_on, ctx_off):
+    """Three Newton iterations per step (BDF_1): every Newton solve of a free instance takes the
+    correction; the state after the steps agrees with the FP64 oracle's BDF_1 within 1e-6."""
+    sc = _free(6)
+    out = []
+    for ctx in (ctx_on, ctx_off):
+        inst = lib.Instance(ctx, sc, precision=lib.PRECISION_FP32_REFINE, newton_iters=3)
+        iters = 0
+        for _ in range(3):
+            st = inst.step()
+            assert st["status"] == 0 and st["newton_iters"] >= 1
+            iters += st["total_pcg_iters"]
+        out.append((inst.state()[0], iters))
+        inst.close()
+    o = oracle.Oracle(sc, precond=scenes.PRECOND_BLOCK_JACOBI, newton_iters=3)
+    for _ in range(3):
+        o.step()
+    uo, _ = o.state()
+    o.close()
+    X = sc.rest.ravel()
+    (u1, i1), (u0, i0) = out
+    e1, e0 = rel(X + u1, X + uo), rel(X + u0, X + uo)
+    _say(f"  BDF_1: inner iterations on {i1}, off {i0}; rel x vs FP64 oracle on {e1:.2e}, off {e0:.2e}")
+    assert e1 <= 1e-6 and e0 <= 1e-6
+    assert i1 < i0
