@@ -147,6 +147,18 @@ __device__ __forceinline__ double uniform_f(double v) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
+// s + (m0 p0 + m1 p1 + m2 p2) with its two fused multiply-adds spelled out.  Kernels that must
+// agree bitwise (the SpMV's variants, the FP64 two-kernel and p-form iterations) then contract
+// the products the same way, whatever the compiler would pick for each.
+template <typename T>
+__device__ __forceinline__ T row3(T s, T m0, T m1, T m2, T p0, T p1, T p2) {
+  return s + fma(m2, p2, fma(m0, p0, m1 * p1));
+}
+// a0 b0 + a1 b1 + a2 b2 in FP64 with its fused multiply-adds spelled out (for the same reason).
+__device__ __forceinline__ double dot3(double a0, double a1, double a2, double b0, double b1, double b2) {
+  return fma(a2, b2, fma(a1, b1, a0 * b0));
+}
+
 template <typename T>
 __device__ __forceinline__ T d_own(bool rec, T z_own, T p_own) {
   return rec ? z_own : p_own;
@@ -1125,9 +1137,9 @@ __global__ __launch_bounds__(kBlock) void cg_spmv_kernel(Batch b, int parity) {
         po1 += (T)ce1;
         po2 += (T)ce2;
       }
-      po0 += beta * qo0;
-      po1 += beta * qo1;
-      po2 += beta * qo2;
+      po0 = fma(beta, qo0, po0);
+      po1 = fma(beta, qo1, po1);
+      po2 = fma(beta, qo2, po2);
     } else if (coarse && live) {
       po0 += (T)ce0;
       po1 += (T)ce1;
@@ -1169,18 +1181,18 @@ __global__ __launch_bounds__(kBlock) void cg_spmv_kernel(Batch b, int parity) {
         ca = ld_at(col, slot_of(j + 2) * 4u);
         cb = ld_at(col, slot_of(j + 3) * 4u);
         const bool va = j < nj, vb = j + 1 < nj;
-        const T pa0 = va ? (kRec ? za0 : za0 + bt * qa0) - d0 : T(0);
-        const T pa1 = va ? (kRec ? za1 : za1 + bt * qa1) - d1 : T(0);
-        const T pa2 = va ? (kRec ? za2 : za2 + bt * qa2) - d2 : T(0);
-        const T pb0 = vb ? (kRec ? zb0 : zb0 + bt * qb0) - d0 : T(0);
-        const T pb1 = vb ? (kRec ? zb1 : zb1 + bt * qb1) - d1 : T(0);
-        const T pb2 = vb ? (kRec ? zb2 : zb2 + bt * qb2) - d2 : T(0);
-        s0 += ma[0] * pa0 + ma[1] * pa1 + ma[2] * pa2;
-        s1 += ma[3] * pa0 + ma[4] * pa1 + ma[5] * pa2;
-        s2 += ma[6] * pa0 + ma[7] * pa1 + ma[8] * pa2;
-        s0 += mb[0] * pb0 + mb[1] * pb1 + mb[2] * pb2;
-        s1 += mb[3] * pb0 + mb[4] * pb1 + mb[5] * pb2;
-        s2 += mb[6] * pb0 + mb[7] * pb1 + mb[8] * pb2;
+        const T pa0 = va ? (kRec ? za0 : fma(bt, qa0, za0)) - d0 : T(0);
+        const T pa1 = va ? (kRec ? za1 : fma(bt, qa1, za1)) - d1 : T(0);
+        const T pa2 = va ? (kRec ? za2 : fma(bt, qa2, za2)) - d2 : T(0);
+        const T pb0 = vb ? (kRec ? zb0 : fma(bt, qb0, zb0)) - d0 : T(0);
+        const T pb1 = vb ? (kRec ? zb1 : fma(bt, qb1, zb1)) - d1 : T(0);
+        const T pb2 = vb ? (kRec ? zb2 : fma(bt, qb2, zb2)) - d2 : T(0);
+        s0 = row3(s0, ma[0], ma[1], ma[2], pa0, pa1, pa2);
+        s1 = row3(s1, ma[3], ma[4], ma[5], pa0, pa1, pa2);
+        s2 = row3(s2, ma[6], ma[7], ma[8], pa0, pa1, pa2);
+        s0 = row3(s0, mb[0], mb[1], mb[2], pb0, pb1, pb2);
+        s1 = row3(s1, mb[3], mb[4], mb[5], pb0, pb1, pb2);
+        s2 = row3(s2, mb[6], mb[7], mb[8], pb0, pb1, pb2);
         // The prefetched indices are consumed here, at the end of the pass: otherwise the
         // compiler merges this load with the one before the loop and issues it at the top of the
         // next pass, where the gathers would wait for it.
@@ -1197,16 +1209,16 @@ __global__ __launch_bounds__(kBlock) void cg_spmv_kernel(Batch b, int parity) {
     if (live && lane < H) {
       const size_t i = 3 * (size_t)r;
       if (kDiff) {
-        s0 += (T)dg[i] * d_own(kRec, coarse ? zo0 + cf0 : zo0, po0);
-        s1 += (T)dg[i + 1] * d_own(kRec, coarse ? zo1 + cf1 : zo1, po1);
-        s2 += (T)dg[i + 2] * d_own(kRec, coarse ? zo2 + cf2 : zo2, po2);
+        s0 = fma((T)dg[i], d_own(kRec, coarse ? zo0 + cf0 : zo0, po0), s0);
+        s1 = fma((T)dg[i + 1], d_own(kRec, coarse ? zo1 + cf1 : zo1, po1), s1);
+        s2 = fma((T)dg[i + 2], d_own(kRec, coarse ? zo2 + cf2 : zo2, po2), s2);
       }
       if (kRec && !first) {
-        s0 += beta * to0;
-        s1 += beta * to1;
-        s2 += beta * to2;
+        s0 = fma(beta, to0, s0);
+        s1 = fma(beta, to1, s1);
+        s2 = fma(beta, to2, s2);
       }
-      pt = (double)po0 * (double)s0 + (double)po1 * (double)s1 + (double)po2 * (double)s2;
+      pt = dot3((double)po0, (double)po1, (double)po2, (double)s0, (double)s1, (double)s2);
     }
     // Lane rho < H now holds row slice*H + rho.  The slice's rows own the 3H consecutive entries
     // of t and p_new from 3*slice*H, and lane l stores entries l, l + 64, ...: every store
@@ -1318,7 +1330,7 @@ __global__ __launch_bounds__(kBlock) void cg_p_kernel(Batch b, int parity) {
     const auto pold = parity ? p1 : p0;
     const auto pnew = parity ? p0 : p1;
     const int n = 3 * I->nv;
-    for (int i = lb * kBlock + threadIdx.x; i < n; i += lg * kBlock) pnew[i] = first ? z[i] : z[i] + beta * pold[i];
+    for (int i = lb * kBlock + threadIdx.x; i < n; i += lg * kBlock) pnew[i] = first ? z[i] : fma(beta, pold[i], z[i]);
   }
 }
 
@@ -1389,12 +1401,12 @@ __global__ __launch_bounds__(kBlock) void cg_spmv_p_kernel(Batch b, int parity) 
         const bool va = j < nj, vb2 = j + 1 < nj;
         const T pa0 = va ? qa0 : T(0), pa1 = va ? qa1 : T(0), pa2 = va ? qa2 : T(0);
         const T pb0 = vb2 ? qb0 : T(0), pb1 = vb2 ? qb1 : T(0), pb2 = vb2 ? qb2 : T(0);
-        s0 += ma[0] * pa0 + ma[1] * pa1 + ma[2] * pa2;
-        s1 += ma[3] * pa0 + ma[4] * pa1 + ma[5] * pa2;
-        s2 += ma[6] * pa0 + ma[7] * pa1 + ma[8] * pa2;
-        s0 += mb[0] * pb0 + mb[1] * pb1 + mb[2] * pb2;
-        s1 += mb[3] * pb0 + mb[4] * pb1 + mb[5] * pb2;
-        s2 += mb[6] * pb0 + mb[7] * pb1 + mb[8] * pb2;
+        s0 = row3(s0, ma[0], ma[1], ma[2], pa0, pa1, pa2);
+        s1 = row3(s1, ma[3], ma[4], ma[5], pa0, pa1, pa2);
+        s2 = row3(s2, ma[6], ma[7], ma[8], pa0, pa1, pa2);
+        s0 = row3(s0, mb[0], mb[1], mb[2], pb0, pb1, pb2);
+        s1 = row3(s1, mb[3], mb[4], mb[5], pb0, pb1, pb2);
+        s2 = row3(s2, mb[6], mb[7], mb[8], pb0, pb1, pb2);
         asm volatile("" : "+v"(ca), "+v"(cb));
       }
     }
@@ -1405,7 +1417,7 @@ __global__ __launch_bounds__(kBlock) void cg_spmv_p_kernel(Batch b, int parity) 
       s2 += __shfl_xor(s2, off, 64);
     }
     double pt = 0.0;
-    if (live && lane < H) pt = (double)po0 * (double)s0 + (double)po1 * (double)s1 + (double)po2 * (double)s2;
+    if (live && lane < H) pt = dot3((double)po0, (double)po1, (double)po2, (double)s0, (double)s1, (double)s2);
     if (wave_live) {
       const size_t base = 3 * (size_t)slice * H;
       const int nval = 3 * min(H, nrows - slice * H);
@@ -1496,15 +1508,15 @@ __global__ __launch_bounds__(kBlock) void cg_update_kernel(Batch b, int parity) 
         const size_t i = 3 * (size_t)row + c;
         T rv = T(0);
         if (live) {
-          x[i] = xv[q] + alpha * pv[q];
-          rv = rv0[q] - alpha * tv[q];
+          x[i] = fma(alpha, pv[q], xv[q]);
+          rv = fma(-alpha, tv[q], rv0[q]);
           rv_[i] = rv;
         }
         const T r0 = __shfl(rv, bse, 64);
         const T r1 = __shfl(rv, bse + 1, 64);
         const T r2 = __shfl(rv, bse + 2, 64);
         if (live) {
-          const T zv = precond == 0 ? m0[q] * rv : m0[q] * r0 + m1[q] * r1 + m2[q] * r2;
+          const T zv = precond == 0 ? m0[q] * rv : fma(m2[q], r2, fma(m0[q], r0, m1[q] * r1));
           z[i] = zv;
           rr2 += (double)rv * (double)rv;
           rz += (double)rv * (double)zv;
@@ -1663,9 +1675,9 @@ __global__ __launch_bounds__(kBlock) void refine_residual_kernel(Batch b) {
         const double x0 = (x64[c] + (double)xi[c]) - o0;
         const double x1 = (x64[c + 1] + (double)xi[c + 1]) - o1;
         const double x2 = (x64[c + 2] + (double)xi[c + 2]) - o2;
-        s0 += (double)Ap[0 * ns + o] * x0 + (double)Ap[1 * ns + o] * x1 + (double)Ap[2 * ns + o] * x2;
-        s1 += (double)Ap[3 * ns + o] * x0 + (double)Ap[4 * ns + o] * x1 + (double)Ap[5 * ns + o] * x2;
-        s2 += (double)Ap[6 * ns + o] * x0 + (double)Ap[7 * ns + o] * x1 + (double)Ap[8 * ns + o] * x2;
+        s0 = row3(s0, (double)Ap[0 * ns + o], (double)Ap[1 * ns + o], (double)Ap[2 * ns + o], x0, x1, x2);
+        s1 = row3(s1, (double)Ap[3 * ns + o], (double)Ap[4 * ns + o], (double)Ap[5 * ns + o], x0, x1, x2);
+        s2 = row3(s2, (double)Ap[6 * ns + o], (double)Ap[7 * ns + o], (double)Ap[8 * ns + o], x0, x1, x2);
       }
     }
 #pragma unroll
@@ -1678,9 +1690,9 @@ __global__ __launch_bounds__(kBlock) void refine_residual_kernel(Batch b) {
     if (live && kk == 0) {
       if (kDiff) {
         const size_t i = 3 * (size_t)r;
-        s0 += gbl(I->dg)[i] * o0;
-        s1 += gbl(I->dg)[i + 1] * o1;
-        s2 += gbl(I->dg)[i + 2] * o2;
+        s0 = fma(gbl(I->dg)[i], o0, s0);
+        s1 = fma(gbl(I->dg)[i + 1], o1, s1);
+        s2 = fma(gbl(I->dg)[i + 2], o2, s2);
       }
       const double s[3] = {s0, s1, s2};
 #pragma unroll
