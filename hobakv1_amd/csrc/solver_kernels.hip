@@ -912,11 +912,10 @@ __global__ __launch_bounds__(kBlock) void cg_init_kernel(Batch b, int refine, in
     if (coarse) {
       const int nb = I->nb[K_ROW];
       auto pc = gbl(I->part_row_c);
-      double g[3], d[3], unused;
+      double g[3], d[3];
       gather(pc, pc + nb, nb, g[0], g[1]);
       gather(pc + 2 * (size_t)nb, pc + 3 * (size_t)nb, nb, g[2], d[0]);
       gather(pc + 4 * (size_t)nb, pc + 5 * (size_t)nb, nb, d[1], d[2]);
-      (void)unused;
 #pragma unroll
       for (int k = 0; k < 3; k++) {
         ai[k] = d[k] > 0.0 ? 1.0 / d[k] : 0.0;
