@@ -151,3 +151,26 @@ def test_coarse_bdf_newton_matches_oracle(ctx_on, ctx_off):
     _say(f"  BDF_1: inner iterations on {i1}, off {i0}; rel x vs FP64 oracle on {e1:.2e}, off {e0:.2e}")
     assert e1 <= 1e-6 and e0 <= 1e-6
     assert i1 < i0
+
+
+def test_coarse_with_point_jacobi(ctx_on, ctx_off):
+    """The correction adds to point Jacobi as to block Jacobi: a free instance with the diagonal
+    preconditioner, against the FP64 oracle's Jacobi trajectory."""
+    sc = _free(6)
+    out = []
+    for ctx in (ctx_on, ctx_off):
+        inst = lib.Instance(ctx, sc, precond=scenes.PRECOND_JACOBI, precision=lib.PRECISION_FP32_REFINE)
+        iters = sum(inst.step()["total_pcg_iters"] for _ in range(STEPS))
+        out.append((inst.state()[0], iters))
+        inst.close()
+    o = oracle.Oracle(sc, precond=scenes.PRECOND_JACOBI)
+    for _ in range(STEPS):
+        o.step()
+    uo, _ = o.state()
+    o.close()
+    X = sc.rest.ravel()
+    (u1, i1), (u0, i0) = out
+    e1, e0 = rel(X + u1, X + uo), rel(X + u0, X + uo)
+    _say(f"  Jacobi: inner iterations on {i1}, off {i0}; rel x vs FP64 oracle on {e1:.2e}, off {e0:.2e}")
+    assert e1 <= 1e-6 and e0 <= 1e-6
+    assert i1 < i0
