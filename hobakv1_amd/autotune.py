@@ -3,7 +3,7 @@
 Sweeps the block-matrix slice height (rows per SELL slice, which sets lanes per row in the SpMV),
 the PCG update kernel's workgroup cap (the partial sums the next SpMV re-reduces) and row groups
 per wave, how many PCG iterations are queued per group, and (reduced-precision form) the first
-inner tolerance, on a
+inner tolerance and the local aggregates (HOBAK_TUNE_AGG), on a
 given workload, and records per configuration the average step time and the sampled SpMV launch
 time.  The best configuration per precision is written to
 tuning.json next to this file; lib.Context applies it to every instance it creates.
@@ -128,6 +128,7 @@ def main():
     ap.add_argument("--spmv-pad", default="0,1")
     ap.add_argument("--row-sort", default="0", help="vertex relabelling by row length: 0, 1 or 0,1")
     ap.add_argument("--inner-tol", default="1e-4", help="FP32 inner PCG tolerances to sweep")
+    ap.add_argument("--agg", default="0", help="local aggregates (reduced-precision form): 0, 1 or 0,1")
     ap.add_argument("--workload", default="single", choices=["single", "batch", "serving"])
     ap.add_argument("--instances", type=int, default=256, help="batch workload size")
     ap.add_argument("--repeats", type=int, default=None,
@@ -144,19 +145,19 @@ def main():
     scs = scenes.batch_instances(args.instances) if args.workload == "batch" else None
     results = []
     tols = [float(x) for x in args.inner_tol.split(",")] if args.precision == "fp32" else [0.0]
-    for H, poll, uw, pad, rs, ug, tol in itertools.product([int(x) for x in args.slice_rows.split(",")],
+    for H, poll, uw, pad, rs, ug, tol, ag in itertools.product([int(x) for x in args.slice_rows.split(",")],
                                                            [int(x) for x in args.poll.split(",")],
                                                            [int(x) for x in args.update_wgs.split(",")],
                                                            [int(x) for x in args.spmv_pad.split(",")],
                                                            [int(x) for x in args.row_sort.split(",")],
                                                            [int(x) for x in args.update_groups.split(",")],
-                                                           tols):
+                                                           tols, [int(x) for x in args.agg.split(",")]):
         repeats = args.repeats or (3 if args.workload == "serving" else 1)
         runs = []
         for _ in range(repeats):
             ctx = lib.Context(0, apply_table=False)
             ctx.set_tuning(slice_rows=H, poll_chunk=poll, update_wgs=uw, spmv_pad=pad, row_sort=rs,
-                           update_groups=ug)
+                           update_groups=ug, agg=ag)
             if args.workload == "serving":
                 ctx.refine_inner_tol = tol or 1e-4
                 runs.append(measure_serving(ctx, precision))
@@ -173,7 +174,7 @@ def main():
         if repeats > 1:
             r["ms_per_step_runs"] = [q["ms_per_step"] for q in runs]
         r.update({"slice_rows": H, "poll_chunk": poll, "update_wgs": uw, "spmv_pad": pad,
-                  "row_sort": rs, "update_groups": ug})
+                  "row_sort": rs, "update_groups": ug, "agg": ag})
         if args.precision == "fp32":
             r["refine_inner_tol"] = tol
         results.append(r)
@@ -190,7 +191,7 @@ def main():
         table[key] = {"slice_rows": best["slice_rows"], "poll_chunk": best["poll_chunk"],
                       "update_wgs": best["update_wgs"], "spmv_pad": best["spmv_pad"],
                       "row_sort": best["row_sort"], "update_groups": best["update_groups"],
-                      "measured": {"workload": what, **best}}
+                      "agg": best["agg"], "measured": {"workload": what, **best}}
         if "refine_inner_tol" in best:
             table[key]["refine_inner_tol"] = best["refine_inner_tol"]
         with open(TABLE, "w") as f:

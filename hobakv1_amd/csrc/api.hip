@@ -70,6 +70,7 @@ struct hobak_ctx {
   int update_groups = 2;               // 21-row groups per PCG update wave
   int p_form = 1;                      // FP64 p-form iterations: 0 never, 1 with streamed matrices, 2 always
   int coarse = 1;                      // reduced-precision form: rigid-translation coarse correction
+  int agg = 0;                         // reduced-precision form: spatial order and local aggregates
   hipStream_t stream = nullptr;
   size_t budget = 0;
   size_t in_use = 0;
@@ -347,6 +348,48 @@ static std::vector<int> row_sort_perm(int nv, int nt, const int32_t* tets) {
   return perm;
 }
 
+// The local aggregates' relabelling: the vertices in Morton (Z-curve) order of their rest
+// positions (21 bits per axis over the bounding box; ties keep their order), so that every chunk
+// of `chunk` consecutive vertices, an aggregate, is spatially compact.  With by_len, also by
+// descending block-row length within each chunk (the slices' padding shrinks as with row_sort,
+// and every aggregate keeps its vertices).  perm[p] = the caller's vertex that becomes vertex p.
+static std::vector<int> spatial_perm(int nv, int nt, const double* xyz, const int32_t* tets, int chunk,
+                                     bool by_len) {
+  double lo[3], hi[3];
+  for (int c = 0; c < 3; c++) lo[c] = hi[c] = xyz[c];
+  for (int v = 1; v < nv; v++)
+    for (int c = 0; c < 3; c++) {
+      lo[c] = std::min(lo[c], xyz[3 * (size_t)v + c]);
+      hi[c] = std::max(hi[c], xyz[3 * (size_t)v + c]);
+    }
+  constexpr int kBits = 21;
+  constexpr double kCells = (double)(1 << kBits);
+  std::vector<uint64_t> key(nv, 0);
+  for (int v = 0; v < nv; v++) {
+    uint64_t q[3];
+    for (int c = 0; c < 3; c++) {
+      const double ext = hi[c] - lo[c];
+      const double f = ext > 0 ? (xyz[3 * (size_t)v + c] - lo[c]) / ext * kCells : 0.0;
+      q[c] = (uint64_t)std::min(kCells - 1.0, std::max(0.0, std::floor(f)));
+    }
+    uint64_t k = 0;
+    for (int b = 0; b < kBits; b++)
+      for (int c = 0; c < 3; c++) k |= ((q[c] >> b) & 1u) << (3 * b + c);
+    key[v] = k;
+  }
+  std::vector<int> perm(nv);
+  for (int i = 0; i < nv; i++) perm[i] = i;
+  std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return key[a] < key[b]; });
+  if (by_len) {
+    const std::vector<int> len = row_lengths(nv, nt, tets);
+    for (int w0 = 0; w0 < nv; w0 += chunk) {
+      const int w1 = std::min(nv, w0 + chunk);
+      std::stable_sort(perm.begin() + w0, perm.begin() + w1, [&](int a, int b) { return len[a] > len[b]; });
+    }
+  }
+  return perm;
+}
+
 struct hobak_inst {
   hobak_ctx* ctx = nullptr;
   int nv = 0, nt = 0, dofs = 0;
@@ -355,6 +398,7 @@ struct hobak_inst {
   double lam_eff = 0, alpha = 0;
   int tsize = 8;  // bytes of the element/assembly/PCG real type
   int groups = 1; // 21-row groups per PCG update wave
+  bool spatial = false;  // held in the spatial order of the local aggregates (spatial_perm)
   Pattern P;
   std::vector<int> pinned_ids;
   // Vertex relabelling (row_sort): the device holds the mesh with vertex p = the caller's vertex
@@ -523,8 +567,8 @@ size_t estimate_bytes(int nv, int nt, int nslots_est, int precision) {
   b += (size_t)nv * (9 + 18) * ts;              // minv + PCG vectors
   b += (size_t)nt * (4 * 4 + 8 + 4) + (size_t)nv * 16;  // vertex lists, element vertex lists, misc
   b += 64 * 1024;                               // records, control block, partial sums
-  if (precision != HOBAK_PRECISION_FP64)        // coarse-correction partial sums
-    b += ((size_t)6 * (nv / kBlock + 1) + (size_t)3 * kUpdateCap) * 8 + 2 * 256;
+  if (precision != HOBAK_PRECISION_FP64)        // coarse-correction partial sums, aggregate inverses
+    b += ((size_t)6 * (nv / kBlock + 1) + (size_t)3 * kUpdateCap) * 8 + 2 * 256 + (size_t)9 * (nv / 21 + 1) * ts;
   return b;
 }
 
@@ -534,6 +578,13 @@ size_t estimate_bytes(int nv, int nt, int nslots_est, int precision) {
 int coarse_on(const hobak_ctx* ctx, const hobak_inst* I) {
   const InstDev& h = I->h;
   return ctx->coarse && h.refine && !h.velform && !h.damping && I->pinned_ids.empty() ? 1 : 0;
+}
+
+// Local aggregates hold for any operator of the difference form: an instance of the
+// reduced-precision form held in the spatial order, wherever the coarse correction is off.
+int agg_on(const hobak_ctx* ctx, const hobak_inst* I) {
+  const InstDev& h = I->h;
+  return I->spatial && h.refine && !h.contact && !coarse_on(ctx, I) ? 1 : 0;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -555,6 +606,7 @@ struct Plan {
   int contact = 0;              // some member has collision objects
   int damping = 0;              // some member uses energy damping
   int coarse = 0;               // some member has the coarse correction on
+  int agg = 0;                  // some member has local aggregates
   double spmv_bytes = 0.0;      // summed over the members
   hipGraphExec_t chunk_exec[2] = {nullptr, nullptr};  // PCG group with cached / streamed matrix loads
   int chunk_k[2] = {0, 0};
@@ -583,11 +635,13 @@ int plan_fill(hobak_ctx* ctx, Plan& P) {
   P.contact = 0;
   P.damping = 0;
   P.coarse = 0;
+  P.agg = 0;
   for (int i = 0; i < n; i++) {
     hp[i] = P.members[i]->dev;
     P.contact |= P.members[i]->h.contact;
     P.damping |= P.members[i]->h.damping;
     P.coarse |= P.members[i]->h.coarse;
+    P.agg |= P.members[i]->h.agg;
     P.spmv_bytes += spmv_bytes_of(P.members[i]);
   }
   off += al(sizeof(void*) * n);
@@ -763,7 +817,8 @@ int launch_prologue(hobak_ctx* ctx, const Plan& J, const Plan* S, bool timed) {
   HIPCHK(ctx, launch_rhs<T>(J.b[K_ROW], J.refine ? 0 : 1, cap, s));
   if (J.refine) {
     HIPCHK(ctx, launch_refine_begin(J.b[K_ROW], cap, s));
-    HIPCHK(ctx, launch_refine_init<T>(J.b[K_ROW], 0, cap, s));
+    if (J.agg) HIPCHK(ctx, launch_agg_build<T>(J.b[K_ROW], J.groups, cap, s));
+    HIPCHK(ctx, launch_refine_init<T>(J.b[K_ROW], 0, J.groups, cap, s));
     HIPCHK(ctx, launch_cg_init(J.b[K_ONE], 1, 0, s));
   } else {
     HIPCHK(ctx, launch_cg_init(J.b[K_ONE], 0, 0, s));
@@ -808,12 +863,12 @@ int launch_chunk(hobak_ctx* ctx, const Plan& P, int k, int stream, hobak_sched* 
     } else {
       HIPCHK(ctx, launch_cg_spmv<T>(P.b[K_SPMV], P.shift, ctx->spmv_pad, stream, P.coarse, j & 1, cap, s, a, b));
     }
-    HIPCHK(ctx, launch_cg_update<T>(P.b[K_UPD], P.groups, P.coarse, j & 1, cap, s));
+    HIPCHK(ctx, launch_cg_update<T>(P.b[K_UPD], P.groups, P.coarse | (P.agg << 1), j & 1, cap, s));
   }
   if (P.refine) {
     HIPCHK(ctx, launch_refine_residual<T>(P.b[K_SPMV], P.b[K_DOF], P.shift, cap, s));
     HIPCHK(ctx, launch_finish_check(P.b[K_ONE], 1, s));
-    HIPCHK(ctx, launch_refine_init<T>(P.b[K_ROW], 1, cap, s));
+    HIPCHK(ctx, launch_refine_init<T>(P.b[K_ROW], 1, P.groups, cap, s));
     HIPCHK(ctx, launch_cg_init(P.b[K_ONE], 1, 1, s));
   } else {
     HIPCHK(ctx, launch_finish_check(P.b[K_ONE], 0, s));
@@ -1426,8 +1481,10 @@ int hobak_inst_create(hobak_ctx* ctx, const double* rest_xyz, int32_t nv, const 
   std::vector<int> perm;
   std::vector<double> rest_r;
   std::vector<int32_t> tets_r;
-  if (ctx->row_sort) {
-    perm = row_sort_perm(nv, nt, tets);
+  const bool spatial = ctx->agg && params->precision == HOBAK_PRECISION_FP32_REFINE;
+  if (spatial || ctx->row_sort) {
+    perm = spatial ? spatial_perm(nv, nt, rest_xyz, tets, 21 * ctx->update_groups, ctx->row_sort != 0)
+                   : row_sort_perm(nv, nt, tets);
     std::vector<int> ip(nv);
     for (int p = 0; p < nv; p++) ip[perm[p]] = p;
     rest_r.resize(3 * (size_t)nv);
@@ -1494,6 +1551,7 @@ int hobak_inst_create(hobak_ctx* ctx, const double* rest_xyz, int32_t nv, const 
   }
   I->tsize = params->precision == HOBAK_PRECISION_FP64 ? 8 : 4;
   I->groups = ctx->update_groups;
+  I->spatial = spatial;
   build_pattern(nv, nt, tets, ctx->slice_rows, I->P, perm.empty() ? nullptr : &perm);
   I->perm = std::move(perm);
   if (!I->perm.empty()) {
@@ -1611,6 +1669,7 @@ int hobak_inst_create(hobak_ctx* ctx, const double* rest_xyz, int32_t nv, const 
     ALLOC(h.part_row_c, 6 * (size_t)h.nb[K_ROW]);
     ALLOC(h.part_upd_g, 3 * (size_t)h.nb[K_UPD]);
   }
+  if (I->spatial) ALLOCT(h.agg_inv, 9 * (size_t)((nv + 21 * I->groups - 1) / (21 * I->groups)));
   {
     char* rec = nullptr;  // the record and, kCtlOffset bytes after it, the control block
     if ((rc = dalloc(I, &rec, kCtlOffset + sizeof(Ctl), 1)) != HOBAK_OK) return bail(rc);
@@ -1641,6 +1700,7 @@ int hobak_inst_create(hobak_ctx* ctx, const double* rest_xyz, int32_t nv, const 
   h.fidx = fidx;
   h.pinned = pinned;
   h.coarse = coarse_on(ctx, I);
+  h.agg = agg_on(ctx, I);
   I->ctl = h.ctl;
   // no solve active: both scalar sets done, so every PCG kernel passes this instance over
   Ctl ctl0;
@@ -1708,8 +1768,10 @@ int hobak_inst_set_pinned(hobak_inst* I, const int32_t* vids, int32_t n, const d
   }
   I->pinned_ids.assign(vids, vids + n);
   const int coarse = coarse_on(I->ctx, I);
-  if (coarse != I->h.coarse) {  // cached plans launch the SpMV variant of their members' setting
+  const int agg = agg_on(I->ctx, I);
+  if (coarse != I->h.coarse || agg != I->h.agg) {  // cached plans launch their members' variants
     I->h.coarse = coarse;
+    I->h.agg = agg;
     TRY(upload(I, I->dev, &I->h, 1));
     plans_forget(I->ctx, I);
   }
@@ -1827,6 +1889,7 @@ int hobak_inst_set_damping(hobak_inst* I, int32_t model, double mu) {
   h.ra = h.damping ? 0.0 : I->prm.rayleigh_alpha;
   h.rb = h.damping ? 0.0 : I->prm.rayleigh_beta;
   h.coarse = coarse_on(ctx, I);
+  h.agg = agg_on(ctx, I);
   TRY(upload(I, I->dev, &h, 1));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   TRY(I->tsize == 8 ? setup_constant_parts<double>(I) : setup_constant_parts<float>(I));
@@ -2101,6 +2164,12 @@ int hobak_ctx_set_tuning(hobak_ctx* ctx, int32_t key, int32_t value) {
       if (!ctx->insts.empty() && value != ctx->coarse)
         return fail(ctx, HOBAK_E_INVALID, "the coarse correction is fixed once instances exist");
       ctx->coarse = value;
+      return HOBAK_OK;
+    case HOBAK_TUNE_AGG:
+      if (value != 0 && value != 1) return fail(ctx, HOBAK_E_INVALID, "aggregates is 0 or 1");
+      if (!ctx->insts.empty() && value != ctx->agg)
+        return fail(ctx, HOBAK_E_INVALID, "the local aggregates are fixed once instances exist");
+      ctx->agg = value;
       return HOBAK_OK;
     case HOBAK_TUNE_UPDATE_WGS:
       if (value < 1 || value > 65536) return fail(ctx, HOBAK_E_INVALID, "update workgroups out of range");

@@ -172,6 +172,16 @@ struct InstDev {
   int coarse;
   double* part_row_c;  // K_ROW: 6 planes of nb[K_ROW]: sum of the seed r (3), sum of dg (3)
   double* part_upd_g;  // K_UPD: 3 planes of nb[K_UPD]: sum of r
+  // Reduced-precision form of an instance with pinned vertices, held in a spatial (Morton) vertex
+  // order: local aggregates in place of the correction above.  Every update chunk (the 21 *
+  // groups consecutive rows one wave of the update kernel handles together) is an aggregate I,
+  // and the preconditioner is block Jacobi plus a block-diagonal coarse space,
+  //   z = M^-1 r + sum_I P_I (P_I^T A P_I)^-1 P_I^T r,   P_I = translations of I's free vertices,
+  // where P_I^T A P_I = sum over I's free rows i of (dg_i - sum of A_ij over the row's columns j
+  // outside I's free vertices).  agg_inv (T, row-major, 9 per chunk) holds the inverses, built
+  // once per solve from A; z then holds the whole correction, so the SpMV is unchanged.
+  int agg;
+  void* agg_inv;
   // Kinematic collision cubes and Baraff-Witkin plane constraints (contact != 0; FP64 form).
   // shape[o] = {center x, y, z, scale, inverse scale}: world = scale * local + center.
   int nshapes, contact;
@@ -325,8 +335,15 @@ hipError_t launch_cg_init(const Batch& b, int refine, int gate, hipStream_t s);
 template <typename T>
 hipError_t launch_cg_spmv(const Batch& b, int shift, int pad, int stream, int coarse, int parity, int cap,
                           hipStream_t s, hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
+// coarse: bit 0, some instance has the coarse correction (InstDev::coarse); bit 1, some has
+// local aggregates (InstDev::agg).
 template <typename T>
 hipError_t launch_cg_update(const Batch& b, int groups, int coarse, int parity, int cap, hipStream_t s);
+// The local aggregates' inverses (InstDev::agg_inv) from the assembled A, once per solve, for the
+// instances with agg set (a Batch of kind K_ROW; groups: the update kernel's, 21 * groups rows per
+// chunk).
+template <typename T>
+hipError_t launch_agg_build(const Batch& b, int groups, int cap, hipStream_t s);
 // The FP64 p-form of an iteration (solver_kernels.hip): p finishes the previous iteration and
 // forms p_new = z + beta p_old over the update kernel's workgroups (Batch of kind K_UPD);
 // spmv_p forms t = A p_new gathering p_new only.  Bitwise the same results as the two-kernel form.
@@ -335,7 +352,7 @@ hipError_t launch_cg_spmv_p(const Batch& b, int shift, int pad, int stream, int 
                             hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
 hipError_t launch_refine_begin(const Batch& b, int cap, hipStream_t s);
 template <typename T>
-hipError_t launch_refine_init(const Batch& b, int gate, int cap, hipStream_t s);
+hipError_t launch_refine_init(const Batch& b, int gate, int groups, int cap, hipStream_t s);
 template <typename T>
 hipError_t launch_refine_residual(const Batch& spmv, const Batch& dof, int shift, int cap, hipStream_t s);
 hipError_t launch_finish_check(const Batch& b, int refine, hipStream_t s);

@@ -31,6 +31,13 @@ __device__ __forceinline__ double wave_sum(double v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   return v;
 }
+// (butterfly: each step adds the same two values in both lanes, so every lane ends with the same
+// bits)
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
 
 // Sum of each value over the workgroup, returned in every lane.
 template <int NV>
@@ -1443,8 +1450,33 @@ __global__ __launch_bounds__(kBlock) void cg_spmv_p_kernel(Batch b, int parity) 
 // every workgroup that re-reduces them) small.  The first wave's loads are issued before the
 // p.t partials are reduced, so the two overlap.  A lane sums r.r and r.z over its groups in
 // order, so the sums do not depend on the batch.
-template <typename T, int R, bool CO>
+// Local aggregate correction of update chunk ch (rows (ch * R + q) * 21 + lane / 3, component
+// c = lane % 3 of each, as in the update kernel): g = P_I^T r over the chunk's free rows, summed
+// in a fixed lane order to the same value in every lane, then z += (P_I^T A P_I)^-1 g at the free
+// rows.  pn[q] != 0: row q is pinned or past the end.  Every lane of the wave takes part.
+template <typename T, int R>
+__device__ __forceinline__ void agg_correct(const HOBAK_GLOBAL T* ainv, int ch, int c, const T (&rq)[R],
+                                            const unsigned char (&pn)[R], T (&zq)[R]) {
+  T s = T(0);
+#pragma unroll
+  for (int q = 0; q < R; q++) s += pn[q] ? T(0) : rq[q];
+  const T g0 = wave_sum(c == 0 ? s : T(0));
+  const T g1 = wave_sum(c == 1 ? s : T(0));
+  const T g2 = wave_sum(c == 2 ? s : T(0));
+  const HOBAK_GLOBAL T* m = ainv + 9 * (size_t)ch + 3 * c;
+  const T e = fma(m[2], g2, fma(m[1], g1, m[0] * g0));
+#pragma unroll
+  for (int q = 0; q < R; q++)
+    if (!pn[q]) zq[q] += e;
+}
+
+// CM bit 0: the coarse correction (InstDev::coarse) may be on for an instance of the batch; bit 1:
+// local aggregates (InstDev::agg).  The variants without bit 1 keep the single pass over a wave's
+// rows; with it, z is written after the chunk's correction.
+template <typename T, int R, int CM>
 __global__ __launch_bounds__(kBlock) void cg_update_kernel(Batch b, int parity) {
+  constexpr bool CO = (CM & 1) != 0;
+  constexpr bool LO = (CM & 2) != 0;
   for (int ws = blockIdx.x; ws < b.total; ws += gridDim.x) {
     const int vb = xcd_vb(ws, b.total);
     int lb, lg;
@@ -1466,14 +1498,21 @@ __global__ __launch_bounds__(kBlock) void cg_update_kernel(Batch b, int parity) 
     auto t = gbl((const T*)I->t);
     const int precond = I->precond;
     const int w0 = (int)((lb * (unsigned)kBlock + threadIdx.x) >> 6);
+    const bool local = LO && I->agg != 0;
+    const auto pin = gbl(I->pinned);
     T pv[R], tv[R], rv0[R], xv[R], m0[R], m1[R], m2[R];
+    unsigned char pn[R];
     auto load = [&](int wave) {
 #pragma unroll
       for (int q = 0; q < R; q++) {
         const int row = (wave * R + q) * 21 + lane / 3;
         pv[q] = tv[q] = rv0[q] = xv[q] = m0[q] = m1[q] = m2[q] = T(0);
+        if constexpr (LO) pn[q] = 1;
         if (lane < 63 && row < nrows) {
           const size_t i = 3 * (size_t)row + c;
+          if constexpr (LO) {
+            if (local) pn[q] = pin[row];
+          }
           pv[q] = p[i];
           tv[q] = t[i];
           rv0[q] = rv_[i];
@@ -1500,26 +1539,58 @@ __global__ __launch_bounds__(kBlock) void cg_update_kernel(Batch b, int parity) 
     const bool coarse = CO && I->coarse != 0;
     for (int wave = w0; wave < nwaves; wave += stride) {
       if (wave != w0) load(wave);
+      if constexpr (LO) {
+        T rq[R], zq[R];
 #pragma unroll
-      for (int q = 0; q < R; q++) {
-        const int row = (wave * R + q) * 21 + lane / 3;
-        const bool live = lane < 63 && row < nrows;
-        const size_t i = 3 * (size_t)row + c;
-        T rv = T(0);
-        if (live) {
-          x[i] = fma(alpha, pv[q], xv[q]);
-          rv = fma(-alpha, tv[q], rv0[q]);
-          rv_[i] = rv;
+        for (int q = 0; q < R; q++) {
+          const int row = (wave * R + q) * 21 + lane / 3;
+          const bool live = lane < 63 && row < nrows;
+          const size_t i = 3 * (size_t)row + c;
+          T rv = T(0);
+          if (live) {
+            x[i] = fma(alpha, pv[q], xv[q]);
+            rv = fma(-alpha, tv[q], rv0[q]);
+            rv_[i] = rv;
+          }
+          const T r0 = __shfl(rv, bse, 64);
+          const T r1 = __shfl(rv, bse + 1, 64);
+          const T r2 = __shfl(rv, bse + 2, 64);
+          rq[q] = rv;
+          zq[q] = precond == 0 ? m0[q] * rv : fma(m2[q], r2, fma(m0[q], r0, m1[q] * r1));
         }
-        const T r0 = __shfl(rv, bse, 64);
-        const T r1 = __shfl(rv, bse + 1, 64);
-        const T r2 = __shfl(rv, bse + 2, 64);
-        if (live) {
-          const T zv = precond == 0 ? m0[q] * rv : fma(m2[q], r2, fma(m0[q], r0, m1[q] * r1));
-          z[i] = zv;
-          rr2 += (double)rv * (double)rv;
-          rz += (double)rv * (double)zv;
-          gc += (double)rv;
+        if (local) agg_correct<T, R>(gbl((const T*)I->agg_inv), wave, c, rq, pn, zq);
+#pragma unroll
+        for (int q = 0; q < R; q++) {
+          const int row = (wave * R + q) * 21 + lane / 3;
+          if (lane < 63 && row < nrows) {
+            z[3 * (size_t)row + c] = zq[q];
+            rr2 += (double)rq[q] * (double)rq[q];
+            rz += (double)rq[q] * (double)zq[q];
+            gc += (double)rq[q];
+          }
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < R; q++) {
+          const int row = (wave * R + q) * 21 + lane / 3;
+          const bool live = lane < 63 && row < nrows;
+          const size_t i = 3 * (size_t)row + c;
+          T rv = T(0);
+          if (live) {
+            x[i] = fma(alpha, pv[q], xv[q]);
+            rv = fma(-alpha, tv[q], rv0[q]);
+            rv_[i] = rv;
+          }
+          const T r0 = __shfl(rv, bse, 64);
+          const T r1 = __shfl(rv, bse + 1, 64);
+          const T r2 = __shfl(rv, bse + 2, 64);
+          if (live) {
+            const T zv = precond == 0 ? m0[q] * rv : fma(m2[q], r2, fma(m0[q], r0, m1[q] * r1));
+            z[i] = zv;
+            rr2 += (double)rv * (double)rv;
+            rz += (double)rv * (double)zv;
+            gc += (double)rv;
+          }
         }
       }
     }
@@ -1539,6 +1610,86 @@ __global__ __launch_bounds__(kBlock) void cg_update_kernel(Batch b, int parity) 
     } else {
       publish(rr2, rz, gbl(I->part_rr), gbl(I->part_rz), lb);
     }
+  }
+}
+
+// The local aggregates' matrices P_I^T A P_I = sum over the chunk's free rows i of (dg_i - sum of
+// A_ij over the row's columns j outside the chunk's free vertices), the operator the difference
+// form applies to a translation of the chunk's free vertices, summed in FP64 with one wave per
+// chunk, symmetrised and inverted (cofactors; zero for a chunk without free vertices).
+template <typename T, int R>
+__device__ void agg_build_chunks(const HOBAK_GLOBAL InstDev* I, int w0, int stride) {
+  constexpr int G = 21 * R;
+  const int lane = threadIdx.x & 63;
+  const int nrows = I->nv;
+  const int nch = (nrows + G - 1) / G;
+  const int sh = I->shift;
+  const int hm = (1 << sh) - 1;
+  const size_t ns = (size_t)I->nslots;
+  const auto A = gbl((const T*)I->A);
+  const auto col = gbl(I->col);
+  const auto pin = gbl(I->pinned);
+  const auto dg = gbl(I->dg);
+  const auto so = gbl(I->slice_off);
+  const auto rl = gbl(I->row_len);
+  auto out = gbl((T*)I->agg_inv);
+  for (int ch = w0; ch < nch; ch += stride) {
+    double a[9];
+#pragma unroll
+    for (int e = 0; e < 9; e++) a[e] = 0.0;
+    for (int k0 = lane; k0 < G; k0 += 64) {
+      const int row = ch * G + k0;
+      if (row >= nrows || pin[row]) continue;
+      a[0] += dg[3 * (size_t)row];
+      a[4] += dg[3 * (size_t)row + 1];
+      a[8] += dg[3 * (size_t)row + 2];
+      const int sb = so[row >> sh] + (row & hm);
+      const int len = rl[row];
+      for (int k = 0; k < len; k++) {
+        const int slot = sb + (k << sh);
+        const int j = col[slot];
+        if (j / G == ch && !pin[j]) continue;  // inside the aggregate (the own block included)
+#pragma unroll
+        for (int e = 0; e < 9; e++) a[e] -= (double)A[e * ns + slot];
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < 9; e++) a[e] = wave_sum(a[e]);
+    if (lane == 0) {
+      const double s01 = 0.5 * (a[1] + a[3]), s02 = 0.5 * (a[2] + a[6]), s12 = 0.5 * (a[5] + a[7]);
+      const double c00 = a[4] * a[8] - s12 * s12, c01 = s02 * s12 - s01 * a[8], c02 = s01 * s12 - s02 * a[4];
+      const double c11 = a[0] * a[8] - s02 * s02, c12 = s01 * s02 - a[0] * s12, c22 = a[0] * a[4] - s01 * s01;
+      const double det = a[0] * c00 + s01 * c01 + s02 * c02;
+      const double id = (det > 0.0 && det < HUGE_VAL) ? 1.0 / det : 0.0;
+      auto o = out + 9 * (size_t)ch;
+      o[0] = (T)(c00 * id);
+      o[1] = (T)(c01 * id);
+      o[2] = (T)(c02 * id);
+      o[3] = (T)(c01 * id);
+      o[4] = (T)(c11 * id);
+      o[5] = (T)(c12 * id);
+      o[6] = (T)(c02 * id);
+      o[7] = (T)(c12 * id);
+      o[8] = (T)(c22 * id);
+    }
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void agg_build_kernel(Batch b, int groups) {
+  for (int ws = blockIdx.x; ws < b.total; ws += gridDim.x) {
+    const int vb = xcd_vb(ws, b.total);
+    int lb, lg;
+    const HOBAK_GLOBAL InstDev* __restrict__ I = seg_find(b, vb, lb, lg);
+    if (I->agg == 0) continue;
+    const int w0 = (int)((lb * (unsigned)kBlock + threadIdx.x) >> 6);
+    const int stride = lg * (kBlock / 64);
+    if (groups == 1)
+      agg_build_chunks<T, 1>(I, w0, stride);
+    else if (groups == 2)
+      agg_build_chunks<T, 2>(I, w0, stride);
+    else
+      agg_build_chunks<T, 4>(I, w0, stride);
   }
 }
 
@@ -1583,13 +1734,83 @@ __global__ __launch_bounds__(kBlock) void refine_begin_kernel(Batch b) {
   }
 }
 
+// The inner-solve seed of an instance with local aggregates: the update kernel's row mapping (one
+// wave per chunk at a time), so that z = M^-1 r gains the chunk's correction before it is written.
+template <typename T, int R>
+__device__ void agg_seed_chunks(const HOBAK_GLOBAL InstDev* I, int w0, int stride, double& rr, double& rz) {
+  const int lane = threadIdx.x & 63;
+  const int c = lane - 3 * (lane / 3);
+  const int bse = lane - c;
+  const int nrows = I->nv;
+  const int nch = (nrows + 21 * R - 1) / (21 * R);
+  const auto res = gbl(I->res64);
+  const auto mi = gbl((const T*)I->minv);
+  const auto pin = gbl(I->pinned);
+  auto rvec = gbl((T*)I->r);
+  auto x = gbl((T*)I->x);
+  auto z = gbl((T*)I->z);
+  const int precond = I->precond;
+  for (int ch = w0; ch < nch; ch += stride) {
+    T rq[R], zq[R];
+    unsigned char pn[R];
+#pragma unroll
+    for (int q = 0; q < R; q++) {
+      const int row = (ch * R + q) * 21 + lane / 3;
+      const bool live = lane < 63 && row < nrows;
+      const size_t i = 3 * (size_t)row + c;
+      const T rv = live ? (T)res[i] : T(0);
+      pn[q] = live ? pin[row] : 1;
+      const T r0 = __shfl(rv, bse, 64);
+      const T r1 = __shfl(rv, bse + 1, 64);
+      const T r2 = __shfl(rv, bse + 2, 64);
+      T zv = T(0);
+      if (live) {
+        if (precond == 0) {
+          zv = mi[i] * rv;
+        } else {
+          auto m = mi + (size_t)kMinvStride * row + 3 * c;
+          zv = fma(m[2], r2, fma(m[0], r0, m[1] * r1));
+        }
+      }
+      rq[q] = rv;
+      zq[q] = zv;
+    }
+    agg_correct<T, R>(gbl((const T*)I->agg_inv), ch, c, rq, pn, zq);
+#pragma unroll
+    for (int q = 0; q < R; q++) {
+      const int row = (ch * R + q) * 21 + lane / 3;
+      if (lane < 63 && row < nrows) {
+        const size_t i = 3 * (size_t)row + c;
+        rvec[i] = rq[q];
+        x[i] = T(0);
+        z[i] = zq[q];
+        rr += (double)rq[q] * (double)rq[q];
+        rz += (double)rq[q] * (double)zq[q];
+      }
+    }
+  }
+}
+
 template <typename T>
-__global__ __launch_bounds__(kBlock) void refine_init_kernel(Batch b, int gate) {
+__global__ __launch_bounds__(kBlock) void refine_init_kernel(Batch b, int gate, int groups) {
   for (int ws = blockIdx.x; ws < b.total; ws += gridDim.x) {
     const int vb = xcd_vb(ws, b.total);
     int lb, lg;
     const HOBAK_GLOBAL InstDev* __restrict__ I = seg_find(b, vb, lb, lg);
     if (gate && !solve_ended(ctl_of(I))) continue;
+    if (sizeof(T) < sizeof(double) && I->agg != 0) {
+      double rr = 0.0, rz = 0.0;
+      const int w0 = (int)((lb * (unsigned)kBlock + threadIdx.x) >> 6);
+      const int stride = lg * (kBlock / 64);
+      if (groups == 1)
+        agg_seed_chunks<T, 1>(I, w0, stride, rr, rz);
+      else if (groups == 2)
+        agg_seed_chunks<T, 2>(I, w0, stride, rr, rz);
+      else
+        agg_seed_chunks<T, 4>(I, w0, stride, rr, rz);
+      publish(rr, rz, gbl(I->part_row_a), gbl(I->part_row_b), lb);
+      continue;
+    }
     const int r = lb * kBlock + threadIdx.x;
     double rr = 0.0, rz = 0.0;
     double gs[3] = {0.0, 0.0, 0.0}, ds[3] = {0.0, 0.0, 0.0};  // coarse correction: sums of r, dg
@@ -1960,26 +2181,36 @@ hipError_t launch_cg_spmv(const Batch& b, int shift, int pad, int stream, int co
   }
 }
 
-template <typename T, bool CO>
+template <typename T, int CM>
 static hipError_t update_c(const Batch& b, int groups, int parity, int cap, hipStream_t s) {
   switch (groups) {
-    case 1: SEG_LAUNCH((cg_update_kernel<T, 1, CO>), b, cap, s, parity);
-    case 2: SEG_LAUNCH((cg_update_kernel<T, 2, CO>), b, cap, s, parity);
-    case 4: SEG_LAUNCH((cg_update_kernel<T, 4, CO>), b, cap, s, parity);
+    case 1: SEG_LAUNCH((cg_update_kernel<T, 1, CM>), b, cap, s, parity);
+    case 2: SEG_LAUNCH((cg_update_kernel<T, 2, CM>), b, cap, s, parity);
+    case 4: SEG_LAUNCH((cg_update_kernel<T, 4, CM>), b, cap, s, parity);
     default: return hipErrorInvalidValue;
   }
 }
 template <typename T>
 hipError_t launch_cg_update(const Batch& b, int groups, int coarse, int parity, int cap, hipStream_t s) {
   if constexpr (sizeof(T) < sizeof(double)) {
-    if (coarse) return update_c<T, true>(b, groups, parity, cap, s);
+    switch (coarse & 3) {
+      case 1: return update_c<T, 1>(b, groups, parity, cap, s);
+      case 2: return update_c<T, 2>(b, groups, parity, cap, s);
+      case 3: return update_c<T, 3>(b, groups, parity, cap, s);
+      default: break;
+    }
   }
-  return update_c<T, false>(b, groups, parity, cap, s);
+  return update_c<T, 0>(b, groups, parity, cap, s);
+}
+template <typename T>
+hipError_t launch_agg_build(const Batch& b, int groups, int cap, hipStream_t s) {
+  if (groups != 1 && groups != 2 && groups != 4) return hipErrorInvalidValue;
+  SEG_LAUNCH((agg_build_kernel<T>), b, cap, s, groups);
 }
 hipError_t launch_refine_begin(const Batch& b, int cap, hipStream_t s) { SEG_LAUNCH(refine_begin_kernel, b, cap, s); }
 template <typename T>
-hipError_t launch_refine_init(const Batch& b, int gate, int cap, hipStream_t s) {
-  SEG_LAUNCH((refine_init_kernel<T>), b, cap, s, gate);
+hipError_t launch_refine_init(const Batch& b, int gate, int groups, int cap, hipStream_t s) {
+  SEG_LAUNCH((refine_init_kernel<T>), b, cap, s, gate, groups);
 }
 template <typename T>
 hipError_t launch_refine_residual(const Batch& spmv, const Batch& dof, int shift, int cap, hipStream_t s) {
@@ -2020,7 +2251,8 @@ hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s) 
   template hipError_t launch_cg_spmv<T>(const Batch&, int, int, int, int, int, int, hipStream_t, hipEvent_t, \
                                          hipEvent_t); \
   template hipError_t launch_cg_update<T>(const Batch&, int, int, int, int, hipStream_t);    \
-  template hipError_t launch_refine_init<T>(const Batch&, int, int, hipStream_t);            \
+  template hipError_t launch_refine_init<T>(const Batch&, int, int, int, hipStream_t);       \
+  template hipError_t launch_agg_build<T>(const Batch&, int, int, hipStream_t);             \
   template hipError_t launch_refine_residual<T>(const Batch&, const Batch&, int, int, hipStream_t); \
   template hipError_t launch_force_gather<T>(const Batch&, int, hipStream_t, const void*, double*);
 INST(double)

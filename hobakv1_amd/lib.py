@@ -145,6 +145,7 @@ TUNE_COMPACT_PCT = 10
 TUNE_UPDATE_GROUPS = 11
 TUNE_P_FORM = 12
 TUNE_COARSE = 13
+TUNE_AGG = 14
 
 _lib = None
 
@@ -232,7 +233,8 @@ class Context:
                             grid_cap=entry.get("grid_cap"), update_wgs=entry.get("update_wgs"),
                             spmv_pad=entry.get("spmv_pad"), spmv_stream_mb=entry.get("spmv_stream_mb"),
                             row_sort=entry.get("row_sort"), compact_pct=entry.get("compact_pct"),
-                            update_groups=entry.get("update_groups"), coarse=entry.get("coarse"))
+                            update_groups=entry.get("update_groups"), coarse=entry.get("coarse"),
+                            agg=entry.get("agg"))
             self.refine_inner_tol = entry.get("refine_inner_tol")
 
     def error(self) -> str:
@@ -245,7 +247,8 @@ class Context:
 
     def set_tuning(self, slice_rows=None, poll_chunk=None, max_chunks=None, graphs=None,
                    grid_cap=None, update_wgs=None, spmv_pad=None, spmv_stream_mb=None,
-                   row_sort=None, compact_pct=None, update_groups=None, p_form=None, coarse=None):
+                   row_sort=None, compact_pct=None, update_groups=None, p_form=None, coarse=None,
+                   agg=None):
         """Kernel and engine tuning (see hobak_ctx_set_tuning)."""
         for key, val in ((TUNE_SLICE_ROWS, slice_rows), (TUNE_POLL_CHUNK, poll_chunk),
                          (TUNE_MAX_CHUNKS, max_chunks), (TUNE_GRAPHS, graphs),
@@ -253,7 +256,7 @@ class Context:
                          (TUNE_SPMV_PAD, spmv_pad), (TUNE_SPMV_STREAM_MB, spmv_stream_mb),
                          (TUNE_ROW_SORT, row_sort), (TUNE_COMPACT_PCT, compact_pct),
                          (TUNE_UPDATE_GROUPS, update_groups), (TUNE_P_FORM, p_form),
-                         (TUNE_COARSE, coarse)):
+                         (TUNE_COARSE, coarse), (TUNE_AGG, agg)):
             if val is not None:
                 self.check(self.lib.hobak_ctx_set_tuning(self.h, key, int(val)), "set_tuning")
 

@@ -280,7 +280,19 @@ int hobak_kernel_stats(hobak_inst* inst, double* spmv_avg_ms, int64_t* spmv_coun
  *                          at the same FP64 residual; fewer inner iterations get there.  No
  *                          counterpart in the reference, whose PCG preconditions with the
  *                          diagonal only.  1 (default) on, 0 off; fixed once the context holds
- *                          instances.  The FP64 form never uses it */
+ *                          instances.  The FP64 form never uses it
+ *   HOBAK_TUNE_AGG         reduced-precision form: 1 holds instances created afterwards in a
+ *                          spatial (Morton) vertex order, and wherever HOBAK_TUNE_COARSE does
+ *                          not apply (an instance with pinned vertices, or COARSE 0) adds local
+ *                          aggregates to block (or point) Jacobi: every chunk of 21 *
+ *                          HOBAK_TUNE_UPDATE_GROUPS consecutive vertices of that order is an
+ *                          aggregate I, and z = M^-1 r + sum_I P_I (P_I^T A P_I)^-1 P_I^T r with
+ *                          P_I the translations of I's free vertices.  With HOBAK_TUNE_ROW_SORT
+ *                          also, each chunk's vertices are ordered by descending row length.
+ *                          Every entry point keeps the caller's numbering (as for ROW_SORT,
+ *                          hobak_eval_forces_noalloc refuses a relabelled instance).  The refined
+ *                          solve stops at the same FP64 residual.  No counterpart in the
+ *                          reference.  0 (default) off; fixed once the context holds instances */
 #define HOBAK_TUNE_SLICE_ROWS 0
 #define HOBAK_TUNE_POLL_CHUNK 2
 #define HOBAK_TUNE_MAX_CHUNKS 3
@@ -294,6 +306,7 @@ int hobak_kernel_stats(hobak_inst* inst, double* spmv_avg_ms, int64_t* spmv_coun
 #define HOBAK_TUNE_UPDATE_GROUPS 11
 #define HOBAK_TUNE_P_FORM 12
 #define HOBAK_TUNE_COARSE 13
+#define HOBAK_TUNE_AGG 14
 int hobak_ctx_set_tuning(hobak_ctx* ctx, int32_t key, int32_t value);
 
 /* Continuous-batching scheduler (no counterpart in the reference, whose callers step one scene
