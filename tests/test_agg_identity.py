@@ -1,0 +1,144 @@
+"""The reduced-precision form's local aggregates (HOBAK_TUNE_AGG), checked on the oracle's own
+matrices:
+
+1. The device builds each aggregate's 3x3 matrix from one row pass,
+       P_I^T A P_I = sum_{i in I} (dg_i - sum_{j not in I} A_ij),
+   which holds because every row of A sums to dg_i = m_i (1/dt^2 - alpha/dt) (+ k_ground on y
+   below ground): element stiffnesses annihilate a translation (BACKWARD_EULER_POSITION.cpp:198-208
+   forms A = M/dt^2 - C/dt - K with C = alpha M + beta K_rest).  Checked against the direct double
+   sum sum_{i,j in I} A_ij of the assembled A, on Morton-ordered chunks of a free instance.
+2. Block Jacobi plus the aggregates (each chunk's translation solved alone, pinned vertices left
+   out) solves a pinned lattice's system to the same solution in fewer textbook-PCG iterations,
+   with chunks of a Morton order; chunks of the lattice's natural order do not help, which is why
+   the instances are relabelled.
+
+Tolerances: the identity to 1e-9 relative to the largest chunk diagonal (cancelling FP64 sums);
+the PCG solutions to the solve's own tolerance (1e-8 relative residual each, so 1e-6 apart at most).
+"""
+import numpy as np
+import scipy.sparse as sp
+
+import oracle
+from hobakv1_amd import scenes
+
+
+def _morton_rank(X, bits=21):
+    """Rank of every vertex in the Z-curve order of its rest position (21 bits per axis over the
+    bounding box, ties by index), as the library relabels."""
+    lo, hi = X.min(0), X.max(0)
+    ext = hi - lo
+    cells = float(1 << bits)
+    q = np.where(ext > 0, np.floor((X - lo) / np.where(ext > 0, ext, 1.0) * cells), 0.0)
+    q = np.clip(q, 0, cells - 1).astype(np.uint64)
+    key = np.zeros(len(X), dtype=np.uint64)
+    for b in range(bits):
+        for c in range(3):
+            key |= ((q[:, c] >> np.uint64(b)) & np.uint64(1)) << np.uint64(3 * b + c)
+    order = np.argsort(key, kind="stable")
+    rank = np.empty(len(X), dtype=np.int64)
+    rank[order] = np.arange(len(X))
+    return rank
+
+
+def _system(sc):
+    o = oracle.Oracle(sc, precond=scenes.PRECOND_BLOCK_JACOBI)
+    u0, _ = o.state()
+    o.step()
+    rhs, lhs = o.last_system()
+    row_ptr, col_idx = o.bsr_pattern()
+    _, one = o.rest_volumes()
+    o.close()
+    return u0, rhs, lhs, np.asarray(row_ptr), np.asarray(col_idx), one
+
+
+def test_chunk_matrix_from_one_row_pass():
+    sc = scenes.serving_instance(6, seed=5)
+    sc.ground_y = float(sc.rest[:, 1].min()) + 0.02  # the bottom layer on the penalty ground
+    u0, rhs, lhs, row_ptr, col_idx, one = _system(sc)
+    nv = sc.nv
+    r_of = np.repeat(np.arange(nv), np.diff(row_ptr))
+    m = one * sc.mass_scale
+    base = m * (1.0 / sc.dt ** 2 - sc.rayleigh_alpha / sc.dt)
+    dg = np.repeat(base[:, None], 3, axis=1)
+    y = sc.rest[:, 1] + np.asarray(u0).reshape(-1, 3)[:, 1]
+    dg[y < sc.ground_y, 1] += sc.ground_k
+    rank = _morton_rank(np.asarray(sc.rest, np.float64))
+    for G in (21, 42, 84):
+        agg = rank // G
+        na = int(agg.max()) + 1
+        inside = agg[r_of] == agg[col_idx]
+        direct = np.zeros((na, 3, 3))
+        np.add.at(direct, agg[r_of[inside]], lhs[inside])
+        one_pass = np.zeros((na, 3, 3))
+        np.add.at(one_pass, agg, np.einsum("vi,ij->vij", dg, np.eye(3)))
+        np.add.at(one_pass, agg[r_of[~inside]], -lhs[~inside])
+        err = np.abs(direct - one_pass).max() / np.abs(direct).max()
+        print(f"chunks of {G}: {na} aggregates, |direct - one pass| / max = {err:.2e}")
+        assert err < 1e-9, (G, err)
+        # and the chunk matrices are symmetric positive definite (the device inverts them)
+        assert np.all(np.linalg.eigvalsh(0.5 * (direct + direct.transpose(0, 2, 1))) > 0)
+
+
+def _pcg(A, prec, b, tol=1e-8, maxit=20000):
+    x = np.zeros_like(b)
+    r = b.copy()
+    thr = tol * tol * float(b @ b)
+    z = prec(r)
+    p = z.copy()
+    rz = r @ z
+    for i in range(maxit):
+        t = A @ p
+        a = rz / (p @ t)
+        x += a * p
+        r -= a * t
+        if r @ r < thr:
+            return x, i + 1
+        z = prec(r)
+        rz2 = r @ z
+        p = z + (rz2 / rz) * p
+        rz = rz2
+    return x, maxit
+
+
+def test_aggregates_cut_pcg_iterations_on_a_pinned_lattice():
+    sc = scenes.lattice_scene((12, 12, 12), seed=4, stretch_velocity=True)
+    _, rhs, lhs, row_ptr, col_idx, _ = _system(sc)
+    nv = sc.nv
+    n = 3 * nv
+    free = np.ones(nv, bool)
+    free[np.asarray(sc.pinned(), dtype=np.int64)] = False
+    A = sp.bsr_matrix((lhs, col_idx, row_ptr), shape=(n, n)).tocsr()
+    r_of = np.repeat(np.arange(nv), np.diff(row_ptr))
+    Minv = np.linalg.inv(lhs[np.nonzero(r_of == col_idx)[0]])
+
+    def bj(r):
+        return np.einsum("vij,vj->vi", Minv, r.reshape(-1, 3)).reshape(-1)
+
+    def with_aggregates(agg):
+        na = int(agg.max()) + 1
+        keep = free[r_of] & free[col_idx] & (agg[r_of] == agg[col_idx])
+        Ac = np.zeros((na, 3, 3))
+        np.add.at(Ac, agg[r_of[keep]], lhs[keep])
+        ok = np.abs(np.linalg.det(Ac)) > 0
+        Ai = np.zeros_like(Ac)
+        Ai[ok] = np.linalg.inv(Ac[ok])
+        F = free[:, None].astype(float)
+
+        def prec(r):
+            g = np.zeros((na, 3))
+            np.add.at(g, agg, r.reshape(-1, 3) * F)
+            e = np.einsum("aij,aj->ai", Ai, g)
+            return bj(r) + (e[agg] * F).reshape(-1)
+
+        return prec
+
+    x0, i0 = _pcg(A, bj, rhs)
+    rank = _morton_rank(np.asarray(sc.rest, np.float64))
+    x1, i1 = _pcg(A, with_aggregates(rank // 42), rhs)
+    _, i2 = _pcg(A, with_aggregates(np.arange(nv) // 42), rhs)
+    d = np.linalg.norm(x1 - x0) / np.linalg.norm(x0)
+    print(f"block Jacobi {i0} iterations; with Morton-chunk aggregates {i1}; with natural-order "
+          f"chunks {i2}; solutions {d:.2e} apart")
+    assert i1 < 0.85 * i0
+    assert i2 > i1
+    assert d < 1e-6
