@@ -348,11 +348,40 @@ static std::vector<int> row_sort_perm(int nv, int nt, const int32_t* tets) {
   return perm;
 }
 
-// The local aggregates' relabelling: the vertices in Morton (Z-curve) order of their rest
-// positions (21 bits per axis over the bounding box; ties keep their order), so that every chunk
-// of `chunk` consecutive vertices, an aggregate, is spatially compact.  With by_len, also by
-// descending block-row length within each chunk (the slices' padding shrinks as with row_sort,
-// and every aggregate keeps its vertices).  perm[p] = the caller's vertex that becomes vertex p.
+// Hilbert index of a point with `bits`-bit coordinates: Skilling's transform of the axes to the
+// transposed index, then its bits interleaved, most significant first (J. Skilling, "Programming
+// the Hilbert curve", AIP Conf. Proc. 707, 381 (2004)).
+static uint64_t hilbert_key(uint32_t x[3], int bits) {
+  const uint32_t top = 1u << (bits - 1);
+  for (uint32_t q = top; q > 1; q >>= 1) {  // undo excess work
+    const uint32_t p = q - 1;
+    for (int i = 0; i < 3; i++) {
+      if (x[i] & q) {
+        x[0] ^= p;
+      } else {
+        const uint32_t t = (x[0] ^ x[i]) & p;
+        x[0] ^= t;
+        x[i] ^= t;
+      }
+    }
+  }
+  for (int i = 1; i < 3; i++) x[i] ^= x[i - 1];  // Gray encode
+  uint32_t t = 0;
+  for (uint32_t q = top; q > 1; q >>= 1)
+    if (x[2] & q) t ^= q - 1;
+  for (int i = 0; i < 3; i++) x[i] ^= t;
+  uint64_t key = 0;
+  for (int j = bits - 1; j >= 0; j--)
+    for (int i = 0; i < 3; i++) key = (key << 1) | ((x[i] >> j) & 1u);
+  return key;
+}
+
+// The local aggregates' relabelling: the vertices in Hilbert-curve order of their rest positions
+// (21 bits per axis over the bounding box; ties keep their order), so that every chunk of
+// `chunk` consecutive vertices, an aggregate, is spatially compact (more so than Z-curve chunks,
+// which jump at octree boundaries).  With by_len, also by descending block-row length within each
+// chunk (the slices' padding shrinks as with row_sort, and every aggregate keeps its vertices).
+// perm[p] = the caller's vertex that becomes vertex p.
 static std::vector<int> spatial_perm(int nv, int nt, const double* xyz, const int32_t* tets, int chunk,
                                      bool by_len) {
   double lo[3], hi[3];
@@ -366,16 +395,13 @@ static std::vector<int> spatial_perm(int nv, int nt, const double* xyz, const in
   constexpr double kCells = (double)(1 << kBits);
   std::vector<uint64_t> key(nv, 0);
   for (int v = 0; v < nv; v++) {
-    uint64_t q[3];
+    uint32_t q[3];
     for (int c = 0; c < 3; c++) {
       const double ext = hi[c] - lo[c];
       const double f = ext > 0 ? (xyz[3 * (size_t)v + c] - lo[c]) / ext * kCells : 0.0;
-      q[c] = (uint64_t)std::min(kCells - 1.0, std::max(0.0, std::floor(f)));
+      q[c] = (uint32_t)std::min(kCells - 1.0, std::max(0.0, std::floor(f)));
     }
-    uint64_t k = 0;
-    for (int b = 0; b < kBits; b++)
-      for (int c = 0; c < 3; c++) k |= ((q[c] >> b) & 1u) << (3 * b + c);
-    key[v] = k;
+    key[v] = hilbert_key(q, kBits);
   }
   std::vector<int> perm(nv);
   for (int i = 0; i < nv; i++) perm[i] = i;

@@ -172,7 +172,7 @@ struct InstDev {
   int coarse;
   double* part_row_c;  // K_ROW: 6 planes of nb[K_ROW]: sum of the seed r (3), sum of dg (3)
   double* part_upd_g;  // K_UPD: 3 planes of nb[K_UPD]: sum of r
-  // Reduced-precision form of an instance with pinned vertices, held in a spatial (Morton) vertex
+  // Reduced-precision form of an instance with pinned vertices, held in a spatial (Hilbert-curve) vertex
   // order: local aggregates in place of the correction above.  Every update chunk (the 21 *
   // groups consecutive rows one wave of the update kernel handles together) is an aggregate I,
   // and the preconditioner is block Jacobi plus a block-diagonal coarse space,

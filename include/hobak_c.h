@@ -282,7 +282,7 @@ int hobak_kernel_stats(hobak_inst* inst, double* spmv_avg_ms, int64_t* spmv_coun
  *                          diagonal only.  1 (default) on, 0 off; fixed once the context holds
  *                          instances.  The FP64 form never uses it
  *   HOBAK_TUNE_AGG         reduced-precision form: 1 holds instances created afterwards in a
- *                          spatial (Morton) vertex order, and wherever HOBAK_TUNE_COARSE does
+ *                          spatial (Hilbert-curve) vertex order, and wherever HOBAK_TUNE_COARSE does
  *                          not apply (an instance with pinned vertices, or COARSE 0) adds local
  *                          aggregates to block (or point) Jacobi: every chunk of 21 *
  *                          HOBAK_TUNE_UPDATE_GROUPS consecutive vertices of that order is an

@@ -6,11 +6,14 @@ matrices:
    which holds because every row of A sums to dg_i = m_i (1/dt^2 - alpha/dt) (+ k_ground on y
    below ground): element stiffnesses annihilate a translation (BACKWARD_EULER_POSITION.cpp:198-208
    forms A = M/dt^2 - C/dt - K with C = alpha M + beta K_rest).  Checked against the direct double
-   sum sum_{i,j in I} A_ij of the assembled A, on Morton-ordered chunks of a free instance.
+   sum sum_{i,j in I} A_ij of the assembled A, on Hilbert-ordered chunks of a free instance.
 2. Block Jacobi plus the aggregates (each chunk's translation solved alone, pinned vertices left
    out) solves a pinned lattice's system to the same solution in fewer textbook-PCG iterations,
-   with chunks of a Morton order; chunks of the lattice's natural order do not help, which is why
-   the instances are relabelled.
+   with chunks of a Hilbert-curve order; chunks of the lattice's natural order do not help, which
+   is why the instances are relabelled.  The Hilbert index is a numpy restatement of Skilling's
+   algorithm (J. Skilling, "Programming the Hilbert curve", AIP Conf. Proc. 707, 381 (2004)),
+   checked here for the curve's defining property: consecutive cells of a grid are face
+   neighbours.
 
 Tolerances: the identity to 1e-9 relative to the largest chunk diagonal (cancelling FP64 sums);
 the PCG solutions to the solve's own tolerance (1e-8 relative residual each, so 1e-6 apart at most).
@@ -22,22 +25,47 @@ import oracle
 from hobakv1_amd import scenes
 
 
-def _morton_rank(X, bits=21):
-    """Rank of every vertex in the Z-curve order of its rest position (21 bits per axis over the
-    bounding box, ties by index), as the library relabels."""
+def _hilbert_rank(X, bits=21):
+    """Rank of every vertex in the Hilbert-curve order of its rest position (21 bits per axis over
+    the bounding box, ties by index), as the library relabels."""
     lo, hi = X.min(0), X.max(0)
     ext = hi - lo
     cells = float(1 << bits)
     q = np.where(ext > 0, np.floor((X - lo) / np.where(ext > 0, ext, 1.0) * cells), 0.0)
-    q = np.clip(q, 0, cells - 1).astype(np.uint64)
+    x = [c.astype(np.int64) for c in np.clip(q, 0, cells - 1).T]
+    top = 1 << (bits - 1)
+    b = top
+    while b > 1:  # undo excess work
+        p = b - 1
+        for i in range(3):
+            m = (x[i] & b) != 0
+            x[0] = np.where(m, x[0] ^ p, x[0])
+            t = np.where(m, 0, (x[0] ^ x[i]) & p)
+            x[0] ^= t
+            x[i] ^= t
+        b >>= 1
+    for i in range(1, 3):  # Gray encode
+        x[i] ^= x[i - 1]
+    t = np.zeros_like(x[0])
+    b = top
+    while b > 1:
+        t = np.where((x[2] & b) != 0, t ^ (b - 1), t)
+        b >>= 1
     key = np.zeros(len(X), dtype=np.uint64)
-    for b in range(bits):
-        for c in range(3):
-            key |= ((q[:, c] >> np.uint64(b)) & np.uint64(1)) << np.uint64(3 * b + c)
+    for j in range(bits - 1, -1, -1):
+        for i in range(3):
+            key = (key << np.uint64(1)) | (((x[i] ^ t) >> j) & 1).astype(np.uint64)
     order = np.argsort(key, kind="stable")
     rank = np.empty(len(X), dtype=np.int64)
     rank[order] = np.arange(len(X))
     return rank
+
+
+def test_hilbert_order_steps_to_face_neighbours():
+    g = np.array([[i, j, k] for i in range(8) for j in range(8) for k in range(8)], dtype=np.float64)
+    order = np.argsort(_hilbert_rank(g, bits=3))
+    steps = np.abs(np.diff(g[order], axis=0)).sum(axis=1)
+    assert np.all(steps == 1)
 
 
 def _system(sc):
@@ -62,7 +90,7 @@ def test_chunk_matrix_from_one_row_pass():
     dg = np.repeat(base[:, None], 3, axis=1)
     y = sc.rest[:, 1] + np.asarray(u0).reshape(-1, 3)[:, 1]
     dg[y < sc.ground_y, 1] += sc.ground_k
-    rank = _morton_rank(np.asarray(sc.rest, np.float64))
+    rank = _hilbert_rank(np.asarray(sc.rest, np.float64))
     for G in (21, 42, 84):
         agg = rank // G
         na = int(agg.max()) + 1
@@ -133,11 +161,11 @@ def test_aggregates_cut_pcg_iterations_on_a_pinned_lattice():
         return prec
 
     x0, i0 = _pcg(A, bj, rhs)
-    rank = _morton_rank(np.asarray(sc.rest, np.float64))
+    rank = _hilbert_rank(np.asarray(sc.rest, np.float64))
     x1, i1 = _pcg(A, with_aggregates(rank // 42), rhs)
     _, i2 = _pcg(A, with_aggregates(np.arange(nv) // 42), rhs)
     d = np.linalg.norm(x1 - x0) / np.linalg.norm(x0)
-    print(f"block Jacobi {i0} iterations; with Morton-chunk aggregates {i1}; with natural-order "
+    print(f"block Jacobi {i0} iterations; with Hilbert-chunk aggregates {i1}; with natural-order "
           f"chunks {i2}; solutions {d:.2e} apart")
     assert i1 < 0.85 * i0
     assert i2 > i1

@@ -1,6 +1,6 @@
 """The reduced-precision form's local aggregates (HOBAK_TUNE_AGG): instances held in a spatial
-(Morton) vertex order, and for those where the rigid-translation correction does not apply (pinned
-vertices) the inner FP32 PCG's preconditioner
+(Hilbert-curve) vertex order, and for those where the rigid-translation correction does not
+apply (pinned vertices) the inner FP32 PCG's preconditioner
   z = M^-1 r + sum_I P_I (P_I^T A P_I)^-1 P_I^T r,
 I = every chunk of 21 * update-groups consecutive vertices, P_I the translations of its free
 vertices.
