@@ -168,3 +168,30 @@ def test_pinning_switches_to_aggregates(ctx_agg, ctx_plain):
     assert ia < ib
     a.close()
     b.close()
+
+
+def test_agg_smaller_than_one_chunk(ctx_agg):
+    """A 2x2x2-cell lattice (27 vertices, one partial 42-vertex chunk) with its x = 0 face pinned."""
+    sc = _lattice(2, seed=7)
+    u1, _, _ = _run(ctx_agg, sc, steps=3)
+    uo = _oracle(sc, steps=3, precond=scenes.PRECOND_BLOCK_JACOBI)
+    X = sc.rest.ravel()
+    e1 = rel(X + u1, X + uo)
+    _say(f"  27 vertices: rel x vs oracle {e1:.2e}")
+    assert e1 <= 1e-6
+
+
+def test_agg_every_vertex_pinned(ctx_agg):
+    """Every vertex pinned where it rests (zero displacement targets): every chunk's matrix is empty
+    (its inverse is taken as zero), and the displacements stay where they are."""
+    sc = _lattice(4, seed=8)
+    inst = lib.Instance(ctx_agg, sc, precision=lib.PRECISION_FP32_REFINE)
+    inst.set_pinned(np.arange(sc.nv, dtype=np.int32))
+    u0, _ = inst.state()
+    for _ in range(2):
+        assert inst.step()["status"] == 0
+    u, v = inst.state()
+    inst.close()
+    _say(f"  all pinned: |u - u0| {np.abs(u - u0).max():.2e}, |v| {np.abs(v).max():.2e}")
+    assert np.all(np.isfinite(u)) and np.all(np.isfinite(v))
+    assert np.abs(u - u0).max() <= 1e-12
