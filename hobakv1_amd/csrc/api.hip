@@ -70,7 +70,7 @@ struct hobak_ctx {
   int update_groups = 2;               // 21-row groups per PCG update wave
   int p_form = 1;                      // FP64 p-form iterations: 0 never, 1 with streamed matrices, 2 always
   int coarse = 1;                      // reduced-precision form: rigid-translation coarse correction
-  int agg = 0;                         // reduced-precision form: spatial order and local aggregates
+  int agg = 0;                         // reduced-precision form: spatial order and local aggregates (2: also with the coarse correction)
   hipStream_t stream = nullptr;
   size_t budget = 0;
   size_t in_use = 0;
@@ -607,10 +607,12 @@ int coarse_on(const hobak_ctx* ctx, const hobak_inst* I) {
 }
 
 // Local aggregates hold for any operator of the difference form: an instance of the
-// reduced-precision form held in the spatial order, wherever the coarse correction is off.
+// reduced-precision form held in the spatial order, wherever the coarse correction is off, and
+// with ctx->agg == 2 also added to it (both corrections, for bodies resting on the ground).
 int agg_on(const hobak_ctx* ctx, const hobak_inst* I) {
   const InstDev& h = I->h;
-  return I->spatial && h.refine && !h.contact && !coarse_on(ctx, I) ? 1 : 0;
+  if (!I->spatial || !h.refine || h.contact) return 0;
+  return (ctx->agg == 2 || !coarse_on(ctx, I)) ? 1 : 0;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -2192,7 +2194,7 @@ int hobak_ctx_set_tuning(hobak_ctx* ctx, int32_t key, int32_t value) {
       ctx->coarse = value;
       return HOBAK_OK;
     case HOBAK_TUNE_AGG:
-      if (value != 0 && value != 1) return fail(ctx, HOBAK_E_INVALID, "aggregates is 0 or 1");
+      if (value < 0 || value > 2) return fail(ctx, HOBAK_E_INVALID, "aggregates is 0, 1 or 2");
       if (!ctx->insts.empty() && value != ctx->agg)
         return fail(ctx, HOBAK_E_INVALID, "the local aggregates are fixed once instances exist");
       ctx->agg = value;

@@ -1736,8 +1736,11 @@ __global__ __launch_bounds__(kBlock) void refine_begin_kernel(Batch b) {
 
 // The inner-solve seed of an instance with local aggregates: the update kernel's row mapping (one
 // wave per chunk at a time), so that z = M^-1 r gains the chunk's correction before it is written.
+// gs and ds: this lane's sums of r and dg (its component), for an instance that also has the
+// coarse correction.
 template <typename T, int R>
-__device__ void agg_seed_chunks(const HOBAK_GLOBAL InstDev* I, int w0, int stride, double& rr, double& rz) {
+__device__ void agg_seed_chunks(const HOBAK_GLOBAL InstDev* I, int w0, int stride, double& rr, double& rz,
+                                double& gs, double& ds) {
   const int lane = threadIdx.x & 63;
   const int c = lane - 3 * (lane / 3);
   const int bse = lane - c;
@@ -1750,6 +1753,8 @@ __device__ void agg_seed_chunks(const HOBAK_GLOBAL InstDev* I, int w0, int strid
   auto x = gbl((T*)I->x);
   auto z = gbl((T*)I->z);
   const int precond = I->precond;
+  const bool coarse = I->coarse != 0;
+  const auto dg = gbl(I->dg);
   for (int ch = w0; ch < nch; ch += stride) {
     T rq[R], zq[R];
     unsigned char pn[R];
@@ -1786,6 +1791,10 @@ __device__ void agg_seed_chunks(const HOBAK_GLOBAL InstDev* I, int w0, int strid
         z[i] = zq[q];
         rr += (double)rq[q] * (double)rq[q];
         rz += (double)rq[q] * (double)zq[q];
+        if (coarse) {
+          gs += (double)rq[q];
+          ds += dg[i];
+        }
       }
     }
   }
@@ -1799,16 +1808,31 @@ __global__ __launch_bounds__(kBlock) void refine_init_kernel(Batch b, int gate, 
     const HOBAK_GLOBAL InstDev* __restrict__ I = seg_find(b, vb, lb, lg);
     if (gate && !solve_ended(ctl_of(I))) continue;
     if (sizeof(T) < sizeof(double) && I->agg != 0) {
-      double rr = 0.0, rz = 0.0;
+      double rr = 0.0, rz = 0.0, gs = 0.0, ds = 0.0;
       const int w0 = (int)((lb * (unsigned)kBlock + threadIdx.x) >> 6);
       const int stride = lg * (kBlock / 64);
       if (groups == 1)
-        agg_seed_chunks<T, 1>(I, w0, stride, rr, rz);
+        agg_seed_chunks<T, 1>(I, w0, stride, rr, rz, gs, ds);
       else if (groups == 2)
-        agg_seed_chunks<T, 2>(I, w0, stride, rr, rz);
+        agg_seed_chunks<T, 2>(I, w0, stride, rr, rz, gs, ds);
       else
-        agg_seed_chunks<T, 4>(I, w0, stride, rr, rz);
-      publish(rr, rz, gbl(I->part_row_a), gbl(I->part_row_b), lb);
+        agg_seed_chunks<T, 4>(I, w0, stride, rr, rz, gs, ds);
+      if (I->coarse != 0) {  // both corrections: the coarse one's seed sums too (component c)
+        const int c = (threadIdx.x & 63) % 3;
+        double v[8] = {rr, rz, c == 0 ? gs : 0.0, c == 1 ? gs : 0.0, c == 2 ? gs : 0.0,
+                       c == 0 ? ds : 0.0, c == 1 ? ds : 0.0, c == 2 ? ds : 0.0};
+        block_sum<8>(v);
+        if (threadIdx.x == 0) {
+          const size_t nb = (size_t)I->nb[K_ROW];
+          gbl(I->part_row_a)[lb] = v[0];
+          gbl(I->part_row_b)[lb] = v[1];
+          auto pc = gbl(I->part_row_c);
+#pragma unroll
+          for (int k = 0; k < 6; k++) pc[k * nb + lb] = v[2 + k];
+        }
+      } else {
+        publish(rr, rz, gbl(I->part_row_a), gbl(I->part_row_b), lb);
+      }
       continue;
     }
     const int r = lb * kBlock + threadIdx.x;

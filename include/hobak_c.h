@@ -292,7 +292,11 @@ int hobak_kernel_stats(hobak_inst* inst, double* spmv_avg_ms, int64_t* spmv_coun
  *                          Every entry point keeps the caller's numbering (as for ROW_SORT,
  *                          hobak_eval_forces_noalloc refuses a relabelled instance).  The refined
  *                          solve stops at the same FP64 residual.  No counterpart in the
- *                          reference.  0 (default) off; fixed once the context holds instances */
+ *                          reference.  0 (default) off; 2 as 1, and for instances where
+ *                          HOBAK_TUNE_COARSE applies also adds the aggregates to its correction
+ *                          (both at once: for bodies resting on the ground, where the rigid
+ *                          translation alone no longer helps).  Fixed once the context holds
+ *                          instances */
 #define HOBAK_TUNE_SLICE_ROWS 0
 #define HOBAK_TUNE_POLL_CHUNK 2
 #define HOBAK_TUNE_MAX_CHUNKS 3

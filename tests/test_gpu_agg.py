@@ -195,3 +195,45 @@ def test_agg_every_vertex_pinned(ctx_agg):
     _say(f"  all pinned: |u - u0| {np.abs(u - u0).max():.2e}, |v| {np.abs(v).max():.2e}")
     assert np.all(np.isfinite(u)) and np.all(np.isfinite(v))
     assert np.abs(u - u0).max() <= 1e-12
+
+
+@pytest.fixture(scope="module")
+def ctx_both():
+    ctx = _ctx(2)
+    yield ctx
+    ctx.close()
+
+
+@pytest.mark.parametrize("lift", [0.02, -0.5])
+def test_agg_with_the_coarse_correction(ctx_agg, ctx_both, lift):
+    """HOBAK_TUNE_AGG 2: a free instance adds the aggregates to the rigid-translation correction.
+    Resting on the penalty ground (lift 0.02) that takes fewer inner iterations than the
+    translation alone; in free fall (lift -0.5) only the oracle bound is asserted."""
+    sc = scenes.serving_instance(6, seed=5)
+    sc.ground_y = float(sc.rest[:, 1].min()) + lift
+    u2, _, i2 = _run(ctx_both, sc)
+    u1, _, i1 = _run(ctx_agg, sc)
+    uo = _oracle(sc, precond=scenes.PRECOND_BLOCK_JACOBI)
+    X = sc.rest.ravel()
+    e2, e1 = rel(X + u2, X + uo), rel(X + u1, X + uo)
+    _say(f"  lift {lift}: iterations both {i2}, translation alone {i1}; rel x vs oracle {e2:.2e}, {e1:.2e}")
+    assert e2 <= 1e-6 and e1 <= 1e-6
+    if lift > 0:
+        assert i2 < i1
+
+
+def test_agg_with_the_coarse_correction_batch_equals_alone(ctx_both):
+    free = scenes.serving_instance(6, seed=5)
+    free.ground_y = float(free.rest[:, 1].min()) + 0.02
+    scs = [free, _lattice(10, seed=3)]
+    alone = [_run(ctx_both, sc, steps=3)[:2] for sc in scs]
+    insts = [lib.Instance(ctx_both, sc, precision=lib.PRECISION_FP32_REFINE) for sc in scs]
+    for _ in range(3):
+        for st in lib.step_batch(ctx_both, insts):
+            assert st["status"] == 0
+    for i, inst in enumerate(insts):
+        u, v = inst.state()
+        same = np.array_equal(u, alone[i][0]) and np.array_equal(v, alone[i][1])
+        _say(f"  instance {i}: bitwise {same}")
+        assert same, i
+        inst.close()
