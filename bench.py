@@ -217,6 +217,12 @@ def _cpu_baseline(oracle, scene, steps, kind, cores):
     }
 
 
+# The FP32 inner PCG's preconditioner for pinned instances under the fp32 and fp32_batch tuning
+# entries (HOBAK_TUNE_AGG 1): the plain FP64 path keeps the reference's block Jacobi.
+INNER_PRECOND_PINNED = ("block Jacobi + local aggregates: each chunk of 42 consecutive vertices of a "
+                        "Hilbert-curve vertex order solves its own translation (HOBAK_TUNE_AGG)")
+
+
 def precision_form(lib, sc, precond, args, fp64_total, u64):
     """The same K steps with FP32 element/assembly/PCG and FP64 refinement; compared with the
     FP64 GPU path's positions after the same steps."""
@@ -250,6 +256,7 @@ def precision_form(lib, sc, precond, args, fp64_total, u64):
         "rel_position_error_vs_fp64_after_steps": err,
         "inner_pcg_iters_per_step": iters,
         "refine_sweeps_per_step": sweeps,
+        "inner_preconditioner": INNER_PRECOND_PINNED,
     }
 
 
@@ -330,6 +337,7 @@ def batch_form(lib, scenes, args):
         ub = [inst.state()[0] for inst in insts]
         final[name] = ub
         res = {"value": nt * args.batch_steps / tb, "unit": "tet-Newton-steps/s",
+               **({"inner_preconditioner": INNER_PRECOND_PINNED} if name == "fp32_refine" else {}),
                "ms_per_batch_step": 1e3 * tb / args.batch_steps,
                "p50_batch_step_ms": 1e3 * float(np.percentile(step_s, 50)),
                "p99_batch_step_ms": 1e3 * float(np.percentile(step_s, 99))}
