@@ -2025,7 +2025,7 @@ int hobak_eval_forces_noalloc(hobak_inst* I, const double* u_dev, double* forces
   if (!I || !u_dev || !forces_dev) return HOBAK_E_INVALID;
   hobak_ctx* ctx = I->ctx;
   if (!I->perm.empty())
-    return fail(ctx, HOBAK_E_INVALID, "hobak_eval_forces_noalloc: the instance is held relabelled (row sort); "
+    return fail(ctx, HOBAK_E_INVALID, "hobak_eval_forces_noalloc: the instance is held relabelled (row sort or aggregates); "
                                       "device vectors would need the caller's numbering");
   const size_t need = (size_t)12 * I->nt * I->tsize;
   if (need > (size_t)HOBAK_NOALLOC_SCRATCH_BYTES)

@@ -199,7 +199,8 @@ int hobak_eval_forces_hessian(hobak_inst* inst, const double* u, double* forces,
  * forces_dev are 3 nv doubles on the instance's device, in the caller's vertex numbering.  The
  * per-tet force staging lives in one static device array of HOBAK_NOALLOC_SCRATCH_BYTES per GPU,
  * so an instance may have at most HOBAK_NOALLOC_SCRATCH_BYTES / (12 * real size) tets (FP64:
- * 524,288; FP32: 1,048,576), and must not be relabelled (HOBAK_TUNE_ROW_SORT); otherwise
+ * 524,288; FP32: 1,048,576), and must not be relabelled (HOBAK_TUNE_ROW_SORT, HOBAK_TUNE_AGG,
+ * which the fp32 and fp32_batch tuning entries of the Python layer set); otherwise
  * HOBAK_E_INVALID with the limit in hobak_last_error.  Every call on a GPU uses that GPU's one
  * array: calls that overlap on one GPU (from several host threads or contexts) race.  Returns
  * once the forces are written. */
