@@ -235,15 +235,25 @@ void build_pattern(int nv, int nt, const int32_t* tets, int H, Pattern& P,
   auto row_slots = [&](int i) {
     for (int k = P.row_ptr[i]; k < P.row_ptr[i + 1]; k++) pos[P.col_idx[k]] = P.bsr_slot[k];
   };
+  // mirrors in one pass (the pattern is symmetric): visiting the rows in the order their columns
+  // are sorted by, entry (r, c) of row r finds its transpose (c, r) at row c's next unvisited
+  // entry, since row c lists its columns in that same order
   P.mirror.assign(P.nslots, -1);
+  {
+    std::vector<int> next(P.row_ptr.begin(), P.row_ptr.end() - 1);
+    std::vector<int> order(nv);
+    for (int i = 0; i < nv; i++) order[i] = i;
+    if (order_key)
+      std::sort(order.begin(), order.end(), [&](int a, int b) { return (*order_key)[a] < (*order_key)[b]; });
+    for (int r : order)
+      for (int k = P.row_ptr[r]; k < P.row_ptr[r + 1]; k++) {
+        const int c = P.col_idx[k];
+        const int m = next[c]++;
+        if (c > r) P.mirror[P.bsr_slot[k]] = P.bsr_slot[m];  // upper slot (r, c): its mirror (c, r)
+      }
+  }
   for (int i = 0; i < nv; i++) {
     row_slots(i);
-    for (int k = P.row_ptr[i]; k < P.row_ptr[i + 1]; k++) {
-      const int j = P.col_idx[k];
-      if (j > i)  // slot (j, i) of row j
-        for (int m = P.row_ptr[j]; m < P.row_ptr[j + 1]; m++)
-          if (P.col_idx[m] == i) P.mirror[P.bsr_slot[k]] = P.bsr_slot[m];
-    }
     for (int q = P.vptr[i]; q < P.vptr[i + 1]; q++) {
       const int32_t* T = tets + 4 * (size_t)inc[q];
       for (int b = 0; b < 4; b++)
