@@ -1441,15 +1441,6 @@ __global__ __launch_bounds__(kBlock) void cg_spmv_p_kernel(Batch b, int parity) 
   }
 }
 
-// One lane per DOF of each of R groups of 21 rows (63 lanes) per wavefront, so that a row's
-// three DOFs share a wave and the 3x3 preconditioner apply can read its siblings' residuals with
-// cross-lane moves.  A wave covers 21 R consecutive rows and issues the loads of all R groups
-// (the preconditioner's included) before it uses any, so each wave keeps R times the bytes in
-// flight and a launch needs R times fewer waves.  Each instance's workgroups stride over its
-// waves (at most a tuned cap per instance), which keeps the partial-sum count (and the work of
-// every workgroup that re-reduces them) small.  The first wave's loads are issued before the
-// p.t partials are reduced, so the two overlap.  A lane sums r.r and r.z over its groups in
-// order, so the sums do not depend on the batch.
 // Local aggregate correction of update chunk ch (rows (ch * R + q) * 21 + lane / 3, component
 // c = lane % 3 of each, as in the update kernel): g = P_I^T r over the chunk's free rows, summed
 // in a fixed lane order to the same value in every lane, then z += (P_I^T A P_I)^-1 g at the free
@@ -1470,6 +1461,15 @@ __device__ __forceinline__ void agg_correct(const HOBAK_GLOBAL T* ainv, int ch, 
     if (!pn[q]) zq[q] += e;
 }
 
+// One lane per DOF of each of R groups of 21 rows (63 lanes) per wavefront, so that a row's
+// three DOFs share a wave and the 3x3 preconditioner apply can read its siblings' residuals with
+// cross-lane moves.  A wave covers 21 R consecutive rows and issues the loads of all R groups
+// (the preconditioner's included) before it uses any, so each wave keeps R times the bytes in
+// flight and a launch needs R times fewer waves.  Each instance's workgroups stride over its
+// waves (at most a tuned cap per instance), which keeps the partial-sum count (and the work of
+// every workgroup that re-reduces them) small.  The first wave's loads are issued before the
+// p.t partials are reduced, so the two overlap.  A lane sums r.r and r.z over its groups in
+// order, so the sums do not depend on the batch.
 // CM bit 0: the coarse correction (InstDev::coarse) may be on for an instance of the batch; bit 1:
 // local aggregates (InstDev::agg).  The variants without bit 1 keep the single pass over a wave's
 // rows; with it, z is written after the chunk's correction.
