@@ -992,7 +992,7 @@ __global__ __launch_bounds__(kBlock) void cg_spmv_kernel(Batch b, int parity) {
     const int slice = (int)((lb * (unsigned)kBlock + threadIdx.x) >> 6);
     const int rr = lane % H;
     const int r = slice * H + rr;
-    // Round 1: every field of the record and of the scalar set that this kernel uses.
+    // First load group: every field of the record and of the scalar set that this kernel uses.
     const int nrows = I->nv;
     const int nupd = I->nb[K_UPD];
     const size_t ns = (size_t)I->nslots;
@@ -1049,7 +1049,7 @@ __global__ __launch_bounds__(kBlock) void cg_spmv_kernel(Batch b, int parity) {
     const bool live = r < nrows;
     const auto pold = parity ? p1 : p0;
     const auto pnew = parity ? p0 : p1;
-    // Round 2: the slice extent, the row length, the update kernel's partials and the row's own
+    // Second load group: the slice extent, the row length, the update kernel's partials and the row's own
     // z and p_old, all before any of them is used.
     // Indices are clamped instead of branched on, so these loads are issued together.
     const int sc = wave_live ? slice : nslices - 1;
